@@ -1,0 +1,96 @@
+"""ctypes binding of ``libffsreml.so`` (the C ABI declared in include/ffs_reml.h).
+
+The library is required: every compute entry point raises when it is missing
+or when no HIP device is present.  There is no CPU fallback.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+from pathlib import Path
+
+import torch
+
+LIB_PATH = Path(__file__).resolve().parent / "libffsreml.so"
+
+_c_int = ctypes.c_int32
+_i64 = ctypes.c_int64
+_vp = ctypes.c_void_p
+_pp = ctypes.POINTER(ctypes.c_void_p)
+
+# name -> argtypes (all return int32 status unless listed in _RESTYPES)
+SIGNATURES: dict[str, list] = {
+    "ffs_reml_version": [],
+    "ffs_reml_last_error": [],
+    "ffs_reml_plan_create": [_pp, _c_int, _c_int, _vp, _vp, _vp, _c_int, _vp, _vp, _c_int, _c_int, _vp, _c_int,
+                             _c_int, _c_int, _c_int, _vp, _vp],
+    "ffs_reml_plan_destroy": [_vp],
+    "ffs_reml_plan_logdets": [_vp, _vp, _vp],
+    "ffs_reml_plan_dims": [_vp, _vp, _vp, _vp],
+    "ffs_reml_prep": [_vp, _vp, _i64, _i64, _vp, _i64, _vp, _c_int, _vp, _vp],
+    "ffs_reml_energy": [_vp, _vp, _i64, _i64, _vp, _i64, _vp],
+    "ffs_reml_project": [_vp, _vp, _i64, _i64, _vp, _i64, _vp],
+    "ffs_reml_select": [_vp, _vp, _i64, _i64, _c_int, _vp, _vp, _vp],
+    "ffs_reml_gls": [_vp, _vp, _i64, _vp, _vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _vp, _vp, _vp, _vp],
+    "ffs_reml_ljung_box": [_vp, _i64, _i64, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp],
+}
+_RESTYPES = {"ffs_reml_version": ctypes.c_char_p, "ffs_reml_last_error": ctypes.c_char_p}
+
+_lib = None
+
+
+class RemlLibraryError(RuntimeError):
+    """A call into libffsreml.so failed."""
+
+
+def load(required_symbols: bool = True):
+    """Load the shared library (once) and bind the declared signatures."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise RemlLibraryError(
+            f"{LIB_PATH} is missing: build it with `python -m fastfuncstuff_amd.build` (hipcc, gfx950)")
+    lib = ctypes.CDLL(str(LIB_PATH), mode=os.RTLD_NOW | ctypes.RTLD_GLOBAL)
+    for name, argtypes in SIGNATURES.items():
+        fn = getattr(lib, name, None)
+        if fn is None:
+            if required_symbols:
+                raise RemlLibraryError(f"{LIB_PATH} does not export {name}")
+            continue
+        fn.argtypes = argtypes
+        fn.restype = _RESTYPES.get(name, ctypes.c_int32)
+    _lib = lib
+    return lib
+
+
+def last_error() -> str:
+    return load().ffs_reml_last_error().decode()
+
+
+def check(rc: int, what: str) -> None:
+    if rc != 0:
+        msg = last_error()
+        if rc == -2:
+            raise torch.linalg.LinAlgError(f"{what}: {msg}")
+        if rc == -3:
+            raise ValueError(f"{what}: {msg}")
+        raise RemlLibraryError(f"{what} failed ({rc}): {msg}")
+
+
+def require_device(device: torch.device | None = None) -> torch.device:
+    """The HIP device to run on; raises when there is none (no CPU fallback)."""
+    if not torch.cuda.is_available():
+        raise RemlLibraryError("fastfuncstuff_amd needs a HIP device (MI355X); none is visible")
+    if device is None or device.type != "cuda":
+        return torch.device("cuda", torch.cuda.current_device())
+    return device
+
+
+def ptr(t: torch.Tensor | None):
+    return None if t is None else ctypes.c_void_p(t.data_ptr())
+
+
+def stream_handle(device: torch.device):
+    return ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)

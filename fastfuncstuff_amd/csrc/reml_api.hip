@@ -1,0 +1,361 @@
+// C ABI of the REML engine: plan ownership, launch geometry, error strings.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/ffs_reml.h"
+#include "reml_kernels.h"
+
+// The kernels are compiled in this translation unit so that the templated
+// launches below instantiate them.
+#include "reml_kernels.hip"
+
+using namespace remlk;
+
+static thread_local std::string g_err;
+
+static int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                          \
+    do {                                                                                       \
+        hipError_t _e = (expr);                                                                \
+        if (_e != hipSuccess) return fail(-1, std::string(#expr ": ") + hipGetErrorString(_e)); \
+    } while (0)
+
+struct ffs_reml_plan {
+    PlanDev d{};
+    int device = 0;
+    std::vector<void*> allocs;
+    const double* pinvT = nullptr;
+    const double* ldR = nullptr;
+    const double* ldX = nullptr;
+    template <class Tp>
+    int alloc(Tp** out, size_t n) {
+        void* p = nullptr;
+        if (n == 0) n = 1;
+        hipError_t e = hipMalloc(&p, n * sizeof(Tp));
+        if (e != hipSuccess) return fail(-1, std::string("hipMalloc: ") + hipGetErrorString(e));
+        allocs.push_back(p);
+        *out = static_cast<Tp*>(p);
+        return 0;
+    }
+    ~ffs_reml_plan() {
+        for (void* p : allocs) (void)hipFree(p);
+    }
+};
+
+extern "C" const char* ffs_reml_version(void) { return "ffs_reml 0.1 gfx950"; }
+extern "C" const char* ffs_reml_last_error(void) { return g_err.c_str(); }
+
+template <class Tp>
+static int upload(ffs_reml_plan* P, Tp** dst, const Tp* src, size_t n, hipStream_t s) {
+    if (int rc = P->alloc(dst, n)) return rc;
+    if (n) HIP_TRY(hipMemcpyAsync(*dst, src, n * sizeof(Tp), hipMemcpyHostToDevice, s));
+    return 0;
+}
+
+static int pick_bn(int p, int* npt) {
+    // Widest supported tile that holds at least one pair; prefer the one with
+    // the least padding per pair.
+    const int cands[2] = {64, 96};
+    int best = -1;
+    double waste = 1e9;
+    for (int bn : cands) {
+        if (p > bn) continue;
+        const int n = bn / p;
+        const double w = (double)(bn - n * p) / (n * p);
+        if (w < waste - 1e-12) {
+            waste = w;
+            best = bn;
+        }
+    }
+    if (best < 0) return -1;
+    *npt = best / p;
+    return best;
+}
+
+extern "C" int ffs_reml_plan_create(ffs_reml_plan** out, int32_t T, int32_t p, const double* X, const double* pinvT,
+                                    const int32_t* gap, int32_t K, const double* ab, const int32_t* white,
+                                    int32_t n_a, int32_t n_b, const int32_t* lut, int32_t zero_a, int32_t zero_b,
+                                    int32_t ltop, int32_t n_task, const int32_t* task_idx, void* stream) {
+    if (!out) return fail(-3, "plan_create: null output handle");
+    *out = nullptr;
+    if (T < 2 || p < 1 || K < 1) return fail(-3, "plan_create: need T >= 2, p >= 1, K >= 1");
+    if (p > 96) return fail(-3, "plan_create: p > 96 is not supported");
+    if (gap[T - 1] != 0) return fail(-3, "plan_create: gap[T-1] must be 0 (end of the last run)");
+    for (int t = 0; t < T; ++t)
+        if (gap[t] < 0) return fail(-3, "plan_create: negative time gap");
+    hipStream_t s = (hipStream_t)stream;
+    auto* P = new ffs_reml_plan();
+    std::unique_ptr<ffs_reml_plan> guard(P);
+    HIP_TRY(hipGetDevice(&P->device));
+    PlanDev& d = P->d;
+    d.T = T;
+    d.Tpad = (T + kProjBK - 1) / kProjBK * kProjBK;
+    d.p = p;
+    d.K = K;
+    d.n_a = n_a;
+    d.n_b = n_b;
+    d.ltop = ltop;
+    d.zero_a = zero_a;
+    d.zero_b = zero_b;
+    int npt = 0;
+    const int BN = pick_bn(p, &npt);
+    if (BN < 0) return fail(-3, "plan_create: no projection tile fits p");
+    d.BN = BN;
+    d.npt = npt;
+    d.n_tiles = (K + npt - 1) / npt;
+    const int full_f = (n_task <= 0);
+    std::vector<int32_t> tidx;
+    if (full_f) {
+        for (int j = 0; j < p; ++j) tidx.push_back(j);
+    } else {
+        for (int j = 0; j < n_task; ++j) {
+            if (task_idx[j] < 0 || task_idx[j] >= p) return fail(-3, "plan_create: task index out of range");
+            tidx.push_back(task_idx[j]);
+        }
+    }
+    d.n_task = (int)tidx.size();
+
+    // Energy chunks: lambda == 0 pairs form chunk 0; the rest are grouped by
+    // identical a (consecutive in the a-major pair order), kEnergyNB per chunk.
+    std::vector<int32_t> whites, chunks;
+    for (int k = 0; k < K; ++k)
+        if (white[k]) whites.push_back(k);
+    if (!whites.empty()) chunks.insert(chunks.end(), kEnergyNB, -1);
+    int k = 0;
+    while (k < K) {
+        int e = k;
+        while (e < K && ab[2 * e] == ab[2 * k]) ++e;
+        std::vector<int32_t> grp;
+        for (int q = k; q < e; ++q)
+            if (!white[q]) grp.push_back(q);
+        for (size_t q = 0; q < grp.size(); q += kEnergyNB)
+            for (int j = 0; j < kEnergyNB; ++j) chunks.push_back(q + j < grp.size() ? grp[q + j] : -1);
+        k = e;
+    }
+    d.n_chunks = (int)chunks.size() / kEnergyNB;
+    d.n_white = (int)whites.size();
+
+    int rc = 0;
+    int32_t *dgap, *dwhiteflag, *dtask, *dwhite, *dchunk, *dlut, *dstatus;
+    double *dab, *dX, *dpinv;
+    if ((rc = upload(P, &dgap, gap, T, s)) || (rc = upload(P, &dwhiteflag, white, K, s)) ||
+        (rc = upload(P, &dtask, tidx.data(), tidx.size(), s)) ||
+        (rc = upload(P, &dwhite, whites.data(), whites.size(), s)) ||
+        (rc = upload(P, &dchunk, chunks.data(), chunks.size(), s)) ||
+        (rc = upload(P, &dlut, lut, (size_t)n_a * n_b, s)) || (rc = upload(P, &dab, ab, (size_t)2 * K, s)) ||
+        (rc = upload(P, &dX, X, (size_t)T * p, s)) || (rc = upload(P, &dpinv, pinvT, (size_t)T * p, s)))
+        return rc;
+    if ((rc = P->alloc(&dstatus, 1))) return rc;
+    HIP_TRY(hipMemsetAsync(dstatus, 0, sizeof(int32_t), s));
+    double *soa, *coef64, *Xw64, *G64, *Lx, *Ainv, *Fm, *work, *ld, *ldR, *ldX;
+    float *G32, *ecoef, *ec, *es0;
+    const size_t nt = (size_t)d.n_task;
+    if ((rc = P->alloc(&soa, (size_t)K * 3 * T)) || (rc = P->alloc(&coef64, (size_t)K * 3 * T)) ||
+        (rc = P->alloc(&Xw64, (size_t)K * T * p)) || (rc = P->alloc(&G64, (size_t)K * T * p)) ||
+        (rc = P->alloc(&Lx, (size_t)K * p * p)) || (rc = P->alloc(&Ainv, (size_t)K * p * p)) ||
+        (rc = P->alloc(&Fm, (size_t)K * nt * nt)) || (rc = P->alloc(&work, (size_t)K * 2 * p * p)) ||
+        (rc = P->alloc(&ld, K)) || (rc = P->alloc(&ldR, K)) || (rc = P->alloc(&ldX, K)) ||
+        (rc = P->alloc(&G32, (size_t)d.n_tiles * d.Tpad * BN)) ||
+        (rc = P->alloc(&ecoef, (size_t)d.n_chunks * T * 2 * kEnergyNB)) ||
+        (rc = P->alloc(&ec, (size_t)d.n_chunks * T)) || (rc = P->alloc(&es0, (size_t)d.n_chunks * kEnergyNB)))
+        return rc;
+    HIP_TRY(hipMemsetAsync(G32, 0, sizeof(float) * d.n_tiles * d.Tpad * BN, s));
+    hipLaunchKernelGGL(k_tables, dim3(K), dim3(64), 0, s, T, d.Tpad, p, K, dab, dwhiteflag, dgap, dX, d.n_task, dtask,
+                       full_f, BN, npt, soa, coef64, Xw64, G64, Lx, Ainv, Fm, work, ld, ldR, ldX, G32, dstatus);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(k_energy_tables, dim3(d.n_chunks), dim3(128), 0, s, T, d.n_chunks, dchunk, soa, ecoef, ec,
+                       es0);
+    HIP_TRY(hipGetLastError());
+    int32_t status = 0;
+    HIP_TRY(hipMemcpyAsync(&status, dstatus, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    if (status == 1) return fail(-2, "plan_create: X'R^-1X is not positive definite (rank-deficient design)");
+    if (status == 2) return fail(-2, "plan_create: task block of (X'R^-1X)^-1 is singular");
+    d.gap = dgap;
+    d.ab = dab;
+    d.ld = ld;
+    d.coef64 = coef64;
+    d.X = dX;
+    d.Xw64 = Xw64;
+    d.G64 = G64;
+    d.Lx = Lx;
+    d.Ainv = Ainv;
+    d.Fm = Fm;
+    d.task_idx = dtask;
+    d.G32 = G32;
+    d.ecoef = ecoef;
+    d.ec = ec;
+    d.es0 = es0;
+    d.echunk = dchunk;
+    d.white = dwhite;
+    d.lut = dlut;
+    P->pinvT = dpinv;
+    P->ldR = ldR;
+    P->ldX = ldX;
+    *out = guard.release();
+    return 0;
+}
+
+extern "C" int ffs_reml_plan_destroy(ffs_reml_plan* plan) {
+    delete plan;
+    return 0;
+}
+
+extern "C" int ffs_reml_plan_logdets(const ffs_reml_plan* P, double* logdet_R, double* logdet_XRX) {
+    if (!P) return fail(-3, "plan_logdets: null plan");
+    if (logdet_R) HIP_TRY(hipMemcpy(logdet_R, P->ldR, sizeof(double) * P->d.K, hipMemcpyDeviceToHost));
+    if (logdet_XRX) HIP_TRY(hipMemcpy(logdet_XRX, P->ldX, sizeof(double) * P->d.K, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+extern "C" int ffs_reml_plan_dims(const ffs_reml_plan* P, int32_t* Tpad, int32_t* vox_pad, int32_t* K) {
+    if (!P) return fail(-3, "plan_dims: null plan");
+    if (Tpad) *Tpad = P->d.Tpad;
+    if (vox_pad) *vox_pad = kVoxPad;
+    if (K) *K = P->d.K;
+    return 0;
+}
+
+static int64_t vpad_of(int64_t V) { return (V + kVoxPad - 1) / kVoxPad * kVoxPad; }
+
+template <int PM>
+static void launch_prep(const ffs_reml_plan* P, const float* Y, int64_t ldY, int V, int Vpad, float* Yt,
+                        int64_t ldYt, float* scale, int normalize, float* feats, hipStream_t s) {
+    const dim3 grid((Vpad + 255) / 256), block(256);
+    if (feats)
+        hipLaunchKernelGGL((k_prep<PM, 10>), grid, block, 0, s, P->d.T, P->d.p, Y, ldY, V, Vpad, P->pinvT, P->d.X,
+                           normalize, Yt, ldYt, scale, feats);
+    else
+        hipLaunchKernelGGL((k_prep<PM, 0>), grid, block, 0, s, P->d.T, P->d.p, Y, ldY, V, Vpad, P->pinvT, P->d.X,
+                           normalize, Yt, ldYt, scale, feats);
+}
+
+extern "C" int ffs_reml_prep(const ffs_reml_plan* P, const float* Y, int64_t ldY, int64_t V, float* Yt,
+                             int64_t ldYt, float* scale, int32_t normalize, float* feats, void* stream) {
+    if (!P) return fail(-3, "prep: null plan");
+    if (V <= 0) return 0;
+    const int64_t Vpad = vpad_of(V);
+    if (ldYt < Vpad) return fail(-3, "prep: ldYt must be >= the padded voxel count");
+    if (ldY < P->d.T) return fail(-3, "prep: ldY must be >= T");
+    hipStream_t s = (hipStream_t)stream;
+    const int p = P->d.p;
+    if (p <= 8)
+        launch_prep<8>(P, Y, ldY, (int)V, (int)Vpad, Yt, ldYt, scale, normalize, feats, s);
+    else if (p <= 16)
+        launch_prep<16>(P, Y, ldY, (int)V, (int)Vpad, Yt, ldYt, scale, normalize, feats, s);
+    else if (p <= 32)
+        launch_prep<32>(P, Y, ldY, (int)V, (int)Vpad, Yt, ldYt, scale, normalize, feats, s);
+    else
+        launch_prep<96>(P, Y, ldY, (int)V, (int)Vpad, Yt, ldYt, scale, normalize, feats, s);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+extern "C" int ffs_reml_energy(const ffs_reml_plan* P, const float* Yt, int64_t ldYt, int64_t V, float* E,
+                               int64_t ldE, void* stream) {
+    if (!P) return fail(-3, "energy: null plan");
+    if (V <= 0) return 0;
+    const int64_t Vpad = vpad_of(V);
+    if (ldYt < Vpad) return fail(-3, "energy: ldYt must be >= the padded voxel count");
+    if (ldE < P->d.K) return fail(-3, "energy: ldE must be >= K");
+    hipStream_t s = (hipStream_t)stream;
+    const dim3 grid((unsigned)(Vpad / kEnergyVox), (P->d.n_chunks + kEnergyWaves - 1) / kEnergyWaves);
+    hipLaunchKernelGGL(k_energy, grid, dim3(kEnergyVox * kEnergyWaves), 0, s, P->d, Yt, ldYt, (int)V, E, ldE);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+extern "C" int ffs_reml_project(const ffs_reml_plan* P, const float* Yt, int64_t ldYt, int64_t V, float* S,
+                                int64_t ldS, void* stream) {
+    if (!P) return fail(-3, "project: null plan");
+    if (V <= 0) return 0;
+    const int64_t Vpad = vpad_of(V);
+    if (ldYt < Vpad) return fail(-3, "project: ldYt must be >= the padded voxel count");
+    if (ldS < P->d.K) return fail(-3, "project: ldS must be >= K");
+    hipStream_t s = (hipStream_t)stream;
+    const int64_t nvt = (V + kProjBM - 1) / kProjBM;
+    const int64_t nb = nvt * P->d.n_tiles;
+    if (nb > 0x7fffffff) return fail(-3, "project: grid too large; split the voxel batch");
+    if (P->d.BN == 64)
+        hipLaunchKernelGGL(k_project<64>, dim3((unsigned)nb), dim3(256), 0, s, P->d, Yt, ldYt, (int)V, S, ldS);
+    else
+        hipLaunchKernelGGL(k_project<96>, dim3((unsigned)nb), dim3(256), 0, s, P->d, Yt, ldYt, (int)V, S, ldS);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+extern "C" int ffs_reml_select(const ffs_reml_plan* P, const float* S, int64_t ldS, int64_t V, int32_t mode,
+                               int32_t* best, float* best_score, void* stream) {
+    if (!P) return fail(-3, "select: null plan");
+    if (mode != 0 && mode != 1) return fail(-3, "select: mode must be 0 (exhaustive) or 1 (hierarchical)");
+    if (V <= 0) return 0;
+    hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_select, dim3((unsigned)((V + 255) / 256)), dim3(256), 0, s, P->d, S, ldS, (int)V, mode,
+                       best, best_score);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+template <int PM>
+static void launch_gls(const ffs_reml_plan* P, const float* Y, int64_t ldY, const float* scale, const int32_t* perm,
+                       const int32_t* best, int V, float* betas, float* tstats, float* fstats, float* sigma2,
+                       float* r2, double* resid_w, int64_t ldR, float* resid, float* pred, hipStream_t s) {
+    hipLaunchKernelGGL(k_gls<PM>, dim3((V + kGlsBlock - 1) / kGlsBlock), dim3(kGlsBlock), 0, s, P->d, Y, ldY, scale,
+                       perm, best, V, betas, tstats, fstats, sigma2, r2, resid_w, ldR, resid, pred);
+}
+
+extern "C" int ffs_reml_gls(const ffs_reml_plan* P, const float* Y, int64_t ldY, const float* scale,
+                            const int32_t* best, int64_t V, float* betas, float* tstats, float* fstats,
+                            float* sigma2, float* r2, double* resid_w, int64_t ldR, float* resid, float* pred,
+                            int32_t* work, void* stream) {
+    if (!P) return fail(-3, "gls: null plan");
+    if (V <= 0) return 0;
+    if (ldY < P->d.T) return fail(-3, "gls: ldY must be >= T");
+    if ((resid_w || resid || pred) && ldR < P->d.T) return fail(-3, "gls: ldR must be >= T");
+    hipStream_t s = (hipStream_t)stream;
+    const int K = P->d.K;
+    int32_t* perm = nullptr;
+    if (work) {
+        int32_t* cnt = work + V;
+        perm = work;
+        HIP_TRY(hipMemsetAsync(cnt, 0, sizeof(int32_t) * (K + 1), s));
+        const unsigned g = (unsigned)((V + 255) / 256);
+        hipLaunchKernelGGL(k_hist, dim3(g), dim3(256), 0, s, best, (int)V, K, cnt);
+        hipLaunchKernelGGL(k_scan, dim3(1), dim3(1), 0, s, cnt, K + 1);
+        hipLaunchKernelGGL(k_scatter, dim3(g), dim3(256), 0, s, best, (int)V, K, cnt, perm);
+        HIP_TRY(hipGetLastError());
+    }
+    const int p = P->d.p;
+    if (p <= 8)
+        launch_gls<8>(P, Y, ldY, scale, perm, best, (int)V, betas, tstats, fstats, sigma2, r2, resid_w, ldR, resid, pred, s);
+    else if (p <= 16)
+        launch_gls<16>(P, Y, ldY, scale, perm, best, (int)V, betas, tstats, fstats, sigma2, r2, resid_w, ldR, resid, pred, s);
+    else if (p <= 32)
+        launch_gls<32>(P, Y, ldY, scale, perm, best, (int)V, betas, tstats, fstats, sigma2, r2, resid_w, ldR, resid, pred, s);
+    else
+        launch_gls<96>(P, Y, ldY, scale, perm, best, (int)V, betas, tstats, fstats, sigma2, r2, resid_w, ldR, resid, pred, s);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+extern "C" int ffs_reml_ljung_box(const double* resid_w, int64_t ldR, int64_t V, int32_t n, int32_t nbins,
+                                  const int32_t* bin_off, const int32_t* pairs, const double* wbin, float* out,
+                                  void* stream) {
+    if (V <= 0) return 0;
+    if (ldR < n) return fail(-3, "ljung_box: ldR must be >= n");
+    hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_ljung_box, dim3((unsigned)((V + 3) / 4)), dim3(256), 0, s, resid_w, ldR, (int)V, n, nbins,
+                       bin_off, pairs, wbin, out);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}

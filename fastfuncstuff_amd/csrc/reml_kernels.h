@@ -1,0 +1,61 @@
+// Shared definitions between the HIP kernels and the C-ABI layer.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace remlk {
+
+// Pairs handled by one wave of the energy kernel: all share the AR coefficient
+// a (the pair list is a-major), so the AR-filtered sample z_t = y_{t+1} - c_t y_t
+// is computed once per time step for all of them.
+constexpr int kEnergyNB = 8;
+// Voxels per wave in the energy kernel (one per lane).
+constexpr int kEnergyVox = 64;
+// Waves per energy-kernel block: they share the block's 64-voxel slab via L1.
+constexpr int kEnergyWaves = 8;
+
+// Projection GEMM tiling: BM voxels x BN columns, BK time samples per stage.
+constexpr int kProjBM = 128;
+constexpr int kProjBK = 32;
+// Voxel padding unit of every voxel-major / voxel-minor buffer.
+constexpr int kVoxPad = 256;
+// Threads per block of the GLS kernel.
+constexpr int kGlsBlock = 64;
+
+// Device-side description of the pair tables of one plan.
+struct PlanDev {
+    int T;        // retained time points
+    int Tpad;     // T rounded up to kProjBK
+    int p;        // design columns
+    int K;        // grid pairs
+    int n_chunks; // energy work chunks (chunk 0 is the lambda==0 chunk when n_white > 0)
+    int n_white;  // lambda == 0 pairs
+    int BN;       // projection tile width (columns)
+    int npt;      // pairs per projection tile
+    int n_tiles;  // projection tiles
+    int n_a, n_b; // lattice of the hierarchical selector
+    int ltop;     // top level of the hierarchical descent
+    int zero_a, zero_b;
+    int n_task;   // columns in the F statistic (p when no task subset)
+
+    const int32_t* gap;        // [T]
+    const double* ab;          // [K][2]
+    const double* ld;          // [K] logdet R + logdet X'R^-1X
+    const double* coef64;      // [K][T][3]  (sqrt(1/v_t), c_t, alpha_t)
+    const double* X;           // [T][p] design
+    const double* Xw64;        // [K][T][p]  L^-1 X
+    const double* G64;         // [K][T][p]  R^-1 X Lx^-T
+    const double* Lx;          // [K][p][p]  chol(X'R^-1X)
+    const double* Ainv;        // [K][p][p]  (X'R^-1X)^-1
+    const double* Fm;          // [K][n_task][n_task] F-statistic quadratic form
+    const int32_t* task_idx;   // [n_task]
+    const float* G32;          // [n_tiles][Tpad][BN] projection operand
+    const float* ecoef;        // [n_chunks][T][2*NB] (alpha_t, 1/v_{t+1}) per chunk slot
+    const float* ec;           // [n_chunks][T] c_t of the chunk's AR coefficient
+    const float* es0;          // [n_chunks][NB] 1/v_0
+    const int32_t* echunk;     // [n_chunks][NB] pair index (-1 = empty slot)
+    const int32_t* white;      // [n_white] lambda==0 pairs
+    const int32_t* lut;        // [n_a][n_b] pair index or -1
+};
+
+}  // namespace remlk

@@ -1,0 +1,716 @@
+// gfx950 kernels of the ARMA(1,1) REML search and GLS refit.
+//
+//   k_tables      FP64 per-pair tables: innovation coefficients, L^-1 X,
+//                 chol(X'R^-1X), its inverse, G = R^-1 X Lx^-T      (one block per pair)
+//   k_prep        per-voxel std normalisation + residualisation (+ ACF features)
+//   k_energy      (i)  ||L^-1 y||^2 for every (voxel, pair) by the innovations recurrence
+//   k_project     (ii) MFMA  Y (V x T) . G (T x K p), fused REML score epilogue
+//   k_select      (iii) exhaustive / hierarchical argmin over the surface
+//   k_gls         (iv) FP64 GLS refit + t / F / R^2 / sigma^2
+//   k_ljung_box   whitened-residual Ljung-Box statistic
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+
+#define FFS_HD __host__ __device__
+#include "reml_kernels.h"
+#include "reml_math.h"
+
+namespace remlk {
+
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// ---------------------------------------------------------------------------
+// FP64 per-pair tables.  One 64-thread block per pair.
+//   scratch: [K][3][T]  s2 / c / alpha (SoA, kept for the energy tables)
+//   work:    [K][2*p*p] Gauss-Jordan / Cholesky-inverse workspace
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(64) k_tables(int T, int Tpad, int p, int K, const double* __restrict__ ab,
+                                               const int32_t* __restrict__ white_flag,
+                                               const int32_t* __restrict__ gap, const double* __restrict__ X,
+                                               int n_task, const int32_t* __restrict__ task_idx, int full_f,
+                                               int BN, int npt, double* __restrict__ soa, double* __restrict__ coef64,
+                                               double* __restrict__ Xw64, double* __restrict__ G64,
+                                               double* __restrict__ Lx, double* __restrict__ Ainv,
+                                               double* __restrict__ Fm, double* __restrict__ work,
+                                               double* __restrict__ ld, double* __restrict__ ldR_out,
+                                               double* __restrict__ ldX_out, float* __restrict__ G32,
+                                               int* __restrict__ status) {
+    const int k = blockIdx.x;
+    if (k >= K) return;
+    const int tid = threadIdx.x;
+    const double a = ab[2 * k], b = ab[2 * k + 1];
+    double* s2 = soa + (size_t)k * 3 * T;
+    double* cc = s2 + T;
+    double* al = cc + T;
+    double* xw = Xw64 + (size_t)k * T * p;
+    double* g = G64 + (size_t)k * T * p;
+    double* L = Lx + (size_t)k * p * p;
+    double* Ai = Ainv + (size_t)k * p * p;
+    double* wk = work + (size_t)k * 2 * p * p;
+    __shared__ double sh_ldR, sh_ldX;
+    __shared__ int sh_fail;
+
+    if (tid == 0) {
+        sh_ldR = pair_coefficients(a, b, T, gap, white_flag[k], s2, cc, al);
+        sh_fail = 0;
+    }
+    __syncthreads();
+    double* cf = coef64 + (size_t)k * T * 3;
+    for (int t = tid; t < T; t += 64) {
+        cf[3 * t + 0] = sqrt(s2[t]);
+        cf[3 * t + 1] = cc[t];
+        cf[3 * t + 2] = al[t];
+    }
+    // X_w = L^-1 X, one column per thread.
+    for (int j = tid; j < p; j += 64) whiten_forward(T, s2, cc, al, gap, X + j, p, xw + j, p);
+    __syncthreads();
+    // A = X_w' X_w  (into L, full symmetric)
+    for (int idx = tid; idx < p * p; idx += 64) {
+        const int i = idx / p, j = idx % p;
+        if (j > i) continue;
+        double s = 0.0;
+        for (int t = 0; t < T; ++t) s += xw[(size_t)t * p + i] * xw[(size_t)t * p + j];
+        L[i * p + j] = s;
+        L[j * p + i] = s;
+    }
+    __syncthreads();
+    double* F = Fm + (size_t)k * n_task * n_task;
+    if (tid == 0) {
+        if (full_f)
+            for (int i = 0; i < p * p; ++i) F[i] = L[i];
+        if (cholesky_inplace(p, L) != 0) {
+            sh_fail = 1;
+        } else {
+            double s = 0.0;
+            for (int j = 0; j < p; ++j) s += 2.0 * log(fabs(L[j * p + j]) + 1e-10);
+            sh_ldX = s;
+            cholesky_inverse(p, L, Ai, wk);
+            if (!full_f) {
+                // inv(A^-1[task, task] + 1e-8 I)   (glm/arma.py:4888-4896)
+                double* At = wk;
+                double* W2 = wk + p * p;
+                for (int i = 0; i < n_task; ++i)
+                    for (int j = 0; j < n_task; ++j)
+                        At[i * n_task + j] = Ai[task_idx[i] * p + task_idx[j]] + (i == j ? 1e-8 : 0.0);
+                if (gauss_jordan_inverse(n_task, At, F, W2) != 0) sh_fail = 2;
+            }
+        }
+    }
+    __syncthreads();
+    if (sh_fail) {
+        if (tid == 0) atomicMax(status, sh_fail);
+        return;
+    }
+    // Q rows: q_t = Lx^-1 xw_t  (forward substitution), stored in g.
+    for (int t = tid; t < T; t += 64) {
+        const double* x = xw + (size_t)t * p;
+        double* q = g + (size_t)t * p;
+        for (int i = 0; i < p; ++i) {
+            double s = x[i];
+            for (int m = 0; m < i; ++m) s -= L[i * p + m] * q[m];
+            q[i] = s / L[i * p + i];
+        }
+    }
+    __syncthreads();
+    // G = L^-T Q, in place, one column per thread.
+    for (int j = tid; j < p; j += 64) whiten_adjoint(T, s2, cc, al, gap, g + j, p, g + j, p);
+    __syncthreads();
+    // fp32 projection operand, tile-major: G32[tile][t][slot*p + j]
+    const int tile = k / npt, slot = k % npt;
+    float* Gt = G32 + (size_t)tile * Tpad * BN + slot * p;
+    for (int idx = tid; idx < T * p; idx += 64) {
+        const int t = idx / p, j = idx % p;
+        Gt[(size_t)t * BN + j] = (float)g[idx];
+    }
+    if (tid == 0) {
+        ld[k] = sh_ldR + sh_ldX;
+        ldR_out[k] = sh_ldR;
+        ldX_out[k] = sh_ldX;
+    }
+}
+
+// Energy-kernel tables from the FP64 SoA coefficients.
+__global__ void k_energy_tables(int T, int n_chunks, const int32_t* __restrict__ echunk,
+                                const double* __restrict__ soa, float* __restrict__ ecoef,
+                                float* __restrict__ ec, float* __restrict__ es0) {
+    const int chunk = blockIdx.x;
+    if (chunk >= n_chunks) return;
+    const int32_t* slots = echunk + chunk * kEnergyNB;
+    int first = -1;
+    for (int j = 0; j < kEnergyNB; ++j)
+        if (slots[j] >= 0) {
+            first = slots[j];
+            break;
+        }
+    for (int t = threadIdx.x; t < T; t += blockDim.x) {
+        ec[(size_t)chunk * T + t] = first >= 0 ? (float)soa[(size_t)first * 3 * T + T + t] : 0.f;
+        for (int j = 0; j < kEnergyNB; ++j) {
+            const int k = slots[j];
+            float alpha = 0.f, s2n = 0.f;
+            if (k >= 0) {
+                const double* s2 = soa + (size_t)k * 3 * T;
+                alpha = (float)s2[2 * T + t];
+                s2n = (t + 1 < T) ? (float)s2[t + 1] : 0.f;
+            }
+            ecoef[((size_t)chunk * T + t) * 2 * kEnergyNB + 2 * j] = alpha;
+            ecoef[((size_t)chunk * T + t) * 2 * kEnergyNB + 2 * j + 1] = s2n;
+        }
+    }
+    if (threadIdx.x < kEnergyNB) {
+        const int k = slots[threadIdx.x];
+        es0[chunk * kEnergyNB + threadIdx.x] = k >= 0 ? (float)soa[(size_t)k * 3 * T] : 0.f;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Prep: normalise by the unbiased std (glm/arma.py:3947-3954), residualise
+// against the design with its pseudo-inverse (glm/arma.py:1554-1571) and
+// write the residual time-major (Yt[t][v]) for the search kernels.
+// Optional: lag-1..nfeat autocorrelation of the residual (proposal features).
+// ---------------------------------------------------------------------------
+template <int PM, int NF>
+__global__ void __launch_bounds__(256) k_prep(int T, int p, const float* __restrict__ Y, int64_t ldY, int V,
+                                              int Vpad, const double* __restrict__ pinvT,
+                                              const double* __restrict__ X, int normalize, float* __restrict__ Yt,
+                                              int64_t ldYt, float* __restrict__ scale, float* __restrict__ feats) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= Vpad) return;
+    if (v >= V) {
+        for (int t = 0; t < T; ++t) Yt[(size_t)t * ldYt + v] = 0.f;
+        return;
+    }
+    const float* y = Y + (size_t)v * ldY;
+    double s1 = 0.0, s2 = 0.0, c[PM];
+#pragma unroll
+    for (int j = 0; j < PM; ++j) c[j] = 0.0;
+    for (int t = 0; t < T; ++t) {
+        const double yt = (double)y[t];
+        s1 += yt;
+        s2 += yt * yt;
+        const double* pr = pinvT + (size_t)t * p;
+#pragma unroll
+        for (int j = 0; j < PM; ++j)
+            if (j < p) c[j] += pr[j] * yt;
+    }
+    double s = 1.0;
+    if (normalize) {
+        const double mean = s1 / T;
+        double var = T > 1 ? (s2 - T * mean * mean) / (T - 1) : 0.0;
+        double sd = var > 0.0 ? sqrt(var) : 0.0;
+        s = (double)fmaxf((float)sd, 1e-8f);
+    }
+    const double inv = 1.0 / s;
+#pragma unroll
+    for (int j = 0; j < PM; ++j) c[j] *= inv;
+    float hist[NF > 0 ? NF : 1];
+    double acf[NF > 0 ? NF : 1];
+    double acf0 = 0.0;
+#pragma unroll
+    for (int k = 0; k < (NF > 0 ? NF : 1); ++k) {
+        hist[k] = 0.f;
+        acf[k] = 0.0;
+    }
+    for (int t = 0; t < T; ++t) {
+        double r = (double)y[t] * inv;
+        const double* xr = X + (size_t)t * p;
+#pragma unroll
+        for (int j = 0; j < PM; ++j)
+            if (j < p) r -= xr[j] * c[j];
+        const float rf = (float)r;
+        Yt[(size_t)t * ldYt + v] = rf;
+        if (NF > 0) {
+            acf0 += (double)rf * rf;
+#pragma unroll
+            for (int k = 0; k < NF; ++k) acf[k] += (double)rf * hist[k];
+#pragma unroll
+            for (int k = NF - 1; k > 0; --k) hist[k] = hist[k - 1];
+            hist[0] = rf;
+        }
+    }
+    scale[v] = (float)s;
+    if (NF > 0 && feats) {
+#pragma unroll
+        for (int k = 0; k < NF; ++k) feats[(size_t)v * NF + k] = acf0 > 0.0 ? (float)(acf[k] / acf0) : 0.f;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// (i) Energy ||L^-1 y||^2 for every (voxel, pair).
+// Block: kEnergyWaves waves over one 64-voxel slab; wave w takes chunk
+// blockIdx.y * kEnergyWaves + w (kEnergyNB pairs sharing the AR coefficient).
+// Per step and pair: e <- alpha e + z ; E += e^2 / v   (3 VALU ops), fp32 partial
+// sums flushed into fp64 every 16 steps.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kEnergyVox* kEnergyWaves) k_energy(PlanDev P, const float* __restrict__ Yt,
+                                                                     int64_t ldY, int V, float* __restrict__ E,
+                                                                     int64_t ldE) {
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int chunk = blockIdx.y * kEnergyWaves + w;
+    if (chunk >= P.n_chunks) return;
+    const int v = blockIdx.x * kEnergyVox + lane;
+    const int T = P.T;
+    const float* __restrict__ y = Yt + v;
+
+    if (P.n_white > 0 && chunk == 0) {
+        // lambda == 0: R = I, so the energy is sum y^2, shared by every such pair.
+        double acc = 0.0;
+        float part = 0.f;
+        for (int t = 0; t < T; ++t) {
+            const float yt = y[(size_t)t * ldY];
+            part = fmaf(yt, yt, part);
+            if ((t & 15) == 15) {
+                acc += part;
+                part = 0.f;
+            }
+        }
+        acc += part;
+        if (v < V)
+            for (int i = 0; i < P.n_white; ++i) E[(size_t)v * ldE + P.white[i]] = (float)acc;
+        return;
+    }
+    const float* __restrict__ cz = P.ec + (size_t)chunk * T;
+    const float* __restrict__ co = P.ecoef + (size_t)chunk * T * 2 * kEnergyNB;
+    const float* __restrict__ s0 = P.es0 + chunk * kEnergyNB;
+    float e[kEnergyNB], part[kEnergyNB];
+    double acc[kEnergyNB];
+    const float y0 = y[0];
+#pragma unroll
+    for (int j = 0; j < kEnergyNB; ++j) {
+        e[j] = y0;
+        part[j] = y0 * y0 * s0[j];
+        acc[j] = 0.0;
+    }
+    float yc = y0;
+    constexpr int U = 8;
+    float ybuf[U];
+#pragma unroll
+    for (int i = 0; i < U; ++i) ybuf[i] = (1 + i < T) ? y[(size_t)(1 + i) * ldY] : 0.f;
+    for (int t0 = 0; t0 < T - 1; t0 += U) {
+        float ycur[U];
+#pragma unroll
+        for (int i = 0; i < U; ++i) ycur[i] = ybuf[i];
+#pragma unroll
+        for (int i = 0; i < U; ++i) {
+            const int tn = t0 + U + 1 + i;
+            ybuf[i] = (tn < T) ? y[(size_t)tn * ldY] : 0.f;
+        }
+#pragma unroll
+        for (int i = 0; i < U; ++i) {
+            const int t = t0 + i;
+            if (t < T - 1) {
+                const float yn = ycur[i];
+                const float z = fmaf(-cz[t], yc, yn);
+                const float* c = co + (size_t)t * 2 * kEnergyNB;
+#pragma unroll
+                for (int j = 0; j < kEnergyNB; ++j) {
+                    e[j] = fmaf(c[2 * j], e[j], z);
+                    part[j] = fmaf(e[j] * c[2 * j + 1], e[j], part[j]);
+                }
+                yc = yn;
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < kEnergyNB; ++j) {
+            acc[j] += part[j];
+            part[j] = 0.f;
+        }
+    }
+    if (v < V) {
+        const int32_t* slots = P.echunk + chunk * kEnergyNB;
+#pragma unroll
+        for (int j = 0; j < kEnergyNB; ++j) {
+            const int k = slots[j];
+            if (k >= 0) E[(size_t)v * ldE + k] = (float)(acc[j] + part[j]);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// (ii) Projection GEMM on MFMA (fp32 32x32x2) with the REML score epilogue.
+// Block tile: kProjBM voxels x BN columns (npt pairs of p columns), 4 waves,
+// each wave 32 voxel rows x BN columns.  LDS double-buffered BK-deep stages.
+// Epilogue: P = sum_j (G_k' y)_j^2, rss = E - P, score = ld_k + (T-p) log(rss+1e-10)
+// (glm/arma.py:1988-2004), written over E in the surface.
+// ---------------------------------------------------------------------------
+template <int BN>
+__global__ void __launch_bounds__(256) k_project(PlanDev P, const float* __restrict__ Yt, int64_t ldY, int V,
+                                                 float* __restrict__ S, int64_t ldS) {
+    constexpr int BM = kProjBM, BK = kProjBK, NBLK = BN / 32;
+    constexpr int A_ELEMS = BK * BM, B_ELEMS = BK * BN;
+    constexpr int STAGE = A_ELEMS + B_ELEMS;
+    constexpr int C_LD = BN + 1;
+    constexpr int LDS_FLOATS = (2 * STAGE > BM * C_LD) ? 2 * STAGE : BM * C_LD;
+    __shared__ float lds[LDS_FLOATS];
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int w = tid >> 6;
+    // XCD-aware order: consecutive block ids land on different XCDs; give each
+    // XCD a contiguous run of voxel tiles so one tile's column tiles share L2.
+    const int nvt = gridDim.x / P.n_tiles;
+    const int nb = gridDim.x;
+    const int bid = blockIdx.x;
+    const int xcd = bid & 7, local = bid >> 3;
+    const int per_xcd = (nb + 7) >> 3;
+    int lin = xcd * per_xcd + local;
+    if ((nb & 7) != 0) lin = bid;  // fall back to the plain order on ragged grids
+    const int vt = lin / P.n_tiles;
+    const int tile = lin % P.n_tiles;
+    if (vt >= nvt) return;
+    const int v0 = vt * BM;
+    const float* __restrict__ Ag = Yt + v0;
+    const float* __restrict__ Bg = P.G32 + (size_t)tile * P.Tpad * BN;
+
+    f32x16 acc[NBLK];
+#pragma unroll
+    for (int i = 0; i < NBLK; ++i)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) acc[i][r] = 0.f;
+
+    constexpr int A_V4 = A_ELEMS / 4 / 256;  // float4 per thread
+    constexpr int B_V4 = (B_ELEMS / 4 + 255) / 256;
+    float4 ra[A_V4], rb[B_V4];
+    auto gload = [&](int kt) {
+        const int t0 = kt * BK;
+#pragma unroll
+        for (int i = 0; i < A_V4; ++i) {
+            const int idx = tid + i * 256;
+            const int r = idx / (BM / 4), c4 = idx % (BM / 4);
+            ra[i] = *reinterpret_cast<const float4*>(Ag + (size_t)(t0 + r) * ldY + c4 * 4);
+        }
+#pragma unroll
+        for (int i = 0; i < B_V4; ++i) {
+            const int idx = tid + i * 256;
+            if (idx < B_ELEMS / 4) {
+                const int r = idx / (BN / 4), c4 = idx % (BN / 4);
+                rb[i] = *reinterpret_cast<const float4*>(Bg + (size_t)(t0 + r) * BN + c4 * 4);
+            }
+        }
+    };
+    auto sstore = [&](int buf) {
+        float* As = lds + buf * STAGE;
+        float* Bs = As + A_ELEMS;
+#pragma unroll
+        for (int i = 0; i < A_V4; ++i) {
+            const int idx = tid + i * 256;
+            *reinterpret_cast<float4*>(As + idx * 4) = ra[i];
+        }
+#pragma unroll
+        for (int i = 0; i < B_V4; ++i) {
+            const int idx = tid + i * 256;
+            if (idx < B_ELEMS / 4) *reinterpret_cast<float4*>(Bs + idx * 4) = rb[i];
+        }
+    };
+
+    const int nk = P.Tpad / BK;
+    gload(0);
+    sstore(0);
+    __syncthreads();
+    for (int kt = 0; kt < nk; ++kt) {
+        const int cur = kt & 1;
+        if (kt + 1 < nk) gload(kt + 1);
+        const float* As = lds + cur * STAGE;
+        const float* Bs = As + A_ELEMS;
+        const int ar = lane >> 5, ac = lane & 31;
+#pragma unroll
+        for (int kk = 0; kk < BK / 2; ++kk) {
+            const float av = As[(2 * kk + ar) * BM + w * 32 + ac];
+#pragma unroll
+            for (int n = 0; n < NBLK; ++n) {
+                const float bv = Bs[(2 * kk + ar) * BN + n * 32 + ac];
+                acc[n] = __builtin_amdgcn_mfma_f32_32x32x2f32(av, bv, acc[n], 0, 0, 0);
+            }
+        }
+        if (kt + 1 < nk) sstore(cur ^ 1);
+        __syncthreads();
+    }
+    // Scatter the accumulators (32x32 layout: col = lane%32,
+    // row = 8*(r/4) + 4*(lane/32) + r%4) to LDS.
+    float* Cs = lds;
+#pragma unroll
+    for (int n = 0; n < NBLK; ++n)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const int row = w * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+            const int col = n * 32 + (lane & 31);
+            Cs[row * C_LD + col] = acc[n][r];
+        }
+    __syncthreads();
+    const int p = P.p, npt = P.npt, K = P.K;
+    const double tp = (double)(P.T - p);
+    for (int cidx = tid; cidx < BM * npt; cidx += 256) {
+        const int row = cidx / npt, q = cidx % npt;
+        const int k = tile * npt + q;
+        const int v = v0 + row;
+        if (k >= K || v >= V) continue;
+        const float* cr = Cs + row * C_LD + q * p;
+        double pp = 0.0;
+        for (int j = 0; j < p; ++j) pp += (double)cr[j] * (double)cr[j];
+        float* sp = S + (size_t)v * ldS + k;
+        const double rss = (double)(*sp) - pp;
+        *sp = (float)(P.ld[k] + tp * log(rss + 1e-10));
+    }
+}
+
+// ---------------------------------------------------------------------------
+// (iii) Selection over the (V, K) surface.
+//   mode 0: global argmin, strict '<' in pair order (glm/arma.py:2006-2010)
+//   mode 1: hierarchical descent (glm/arma.py:2224-2322): start (0,0); top
+//           level = coarse lattice; lower levels = clamped 3x3 at +/-step;
+//           candidates b-outer / a-inner, first minimum; a NaN candidate
+//           makes the level's minimum NaN (torch.min), which never improves.
+// ---------------------------------------------------------------------------
+__device__ inline float lat_val(const PlanDev& P, const float* row, int ai, int bi) {
+    const int k = P.lut[ai * P.n_b + bi];
+    return k >= 0 ? row[k] : INFINITY;
+}
+
+__global__ void __launch_bounds__(256) k_select(PlanDev P, const float* __restrict__ S, int64_t ldS, int V, int mode,
+                                                int32_t* __restrict__ best, float* __restrict__ best_score) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= V) return;
+    const float* row = S + (size_t)v * ldS;
+    const int k00 = P.lut[P.zero_a * P.n_b + P.zero_b];
+    if (mode == 0) {
+        float bs = INFINITY;
+        int bk = -1;
+        for (int k = 0; k < P.K; ++k) {
+            const float s = row[k];
+            if (s < bs) {
+                bs = s;
+                bk = k;
+            }
+        }
+        best[v] = bk >= 0 ? bk : k00;
+        best_score[v] = bs;
+        return;
+    }
+    int ba = P.zero_a, bb = P.zero_b;
+    float cur = lat_val(P, row, ba, bb);
+    for (int lev = P.ltop; lev >= 0; --lev) {
+        const int step = 1 << lev;
+        float mn = 0.f;
+        int sa = 0, sb = 0;
+        bool first = true, nan = false;
+        if (lev == P.ltop) {
+            for (int y = 0; y < P.n_b; y += step)
+                for (int x = 0; x < P.n_a; x += step) {
+                    const float s = lat_val(P, row, x, y);
+                    if (s != s) nan = true;
+                    if (first || s < mn) {
+                        mn = s;
+                        sa = x;
+                        sb = y;
+                        first = false;
+                    }
+                }
+        } else {
+            for (int oy = -1; oy <= 1; ++oy) {
+                const int y = min(max(bb + oy * step, 0), P.n_b - 1);
+                for (int ox = -1; ox <= 1; ++ox) {
+                    const int x = min(max(ba + ox * step, 0), P.n_a - 1);
+                    const float s = lat_val(P, row, x, y);
+                    if (s != s) nan = true;
+                    if (first || s < mn) {
+                        mn = s;
+                        sa = x;
+                        sb = y;
+                        first = false;
+                    }
+                }
+            }
+        }
+        if (!nan && mn < cur) {
+            cur = mn;
+            ba = sa;
+            bb = sb;
+        }
+    }
+    best[v] = P.lut[ba * P.n_b + bb];
+    best_score[v] = cur;
+}
+
+// Counting sort of voxels by selected pair (the reference groups voxels by
+// unique (a,b), glm/arma.py:4784-4811), so GLS waves read one pair's tables.
+// Out-of-range pair indices go to the overflow bucket K.
+__device__ inline int bucket_of(int k, int K) { return (k >= 0 && k < K) ? k : K; }
+__global__ void k_hist(const int32_t* __restrict__ best, int V, int K, int32_t* __restrict__ cnt) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v < V) atomicAdd(&cnt[bucket_of(best[v], K)], 1);
+}
+__global__ void k_scan(int32_t* __restrict__ cnt, int n) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        int s = 0;
+        for (int i = 0; i < n; ++i) {
+            const int c = cnt[i];
+            cnt[i] = s;
+            s += c;
+        }
+    }
+}
+__global__ void k_scatter(const int32_t* __restrict__ best, int V, int K, int32_t* __restrict__ off,
+                          int32_t* __restrict__ perm) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v < V) perm[atomicAdd(&off[bucket_of(best[v], K)], 1)] = v;
+}
+
+// ---------------------------------------------------------------------------
+// (iv) FP64 GLS refit (glm/arma.py:4784-5601, X'X branch):
+//   beta = (X'R^-1X)^-1 X'R^-1 y = Lx^-T (G' y);  whitened residual
+//   r = L^-1 y - X_w beta;  sigma2 = sum r^2 / dof;  t = beta / (sqrt(sigma2 diag A^-1) + 1e-10);
+//   F = beta_t' Fm beta_t / (sigma2 n_task + 1e-10);  R^2 = CoD(y, X beta).
+// y is the std-normalised series (scale = 1 for the FP64 variant); betas and
+// sigma2 are unscaled on output (glm/arma.py:5827-5862).
+// ---------------------------------------------------------------------------
+template <int PM>
+__global__ void __launch_bounds__(kGlsBlock) k_gls(PlanDev P, const float* __restrict__ Y, int64_t ldY,
+                                             const float* __restrict__ scale, const int32_t* __restrict__ perm,
+                                             const int32_t* __restrict__ best, int V, float* __restrict__ betas,
+                                             float* __restrict__ tstats, float* __restrict__ fstats,
+                                             float* __restrict__ sigma2, float* __restrict__ r2,
+                                             double* __restrict__ resid_w, int64_t ldR, float* __restrict__ resid,
+                                             float* __restrict__ pred_out) {
+    __shared__ double gls_beta[kGlsBlock * PM];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= V) return;
+    const int v = perm ? perm[i] : i;
+    const int k = best[v];
+    const int T = P.T, p = P.p;
+    if (k < 0 || k >= P.K) {  // no valid pair: NaN statistics
+        for (int j = 0; j < p; ++j) {
+            betas[(size_t)v * p + j] = NAN;
+            tstats[(size_t)v * p + j] = NAN;
+        }
+        fstats[v] = sigma2[v] = r2[v] = NAN;
+        return;
+    }
+    const double s = scale ? (double)scale[v] : 1.0;
+    const double inv = 1.0 / s;
+    const float* y = Y + (size_t)v * ldY;
+    const double* G = P.G64 + (size_t)k * T * p;
+    const double* Xw = P.Xw64 + (size_t)k * T * p;
+    const double* cf = P.coef64 + (size_t)k * T * 3;
+    const double* L = P.Lx + (size_t)k * p * p;
+    double h[PM], be[PM];
+#pragma unroll
+    for (int j = 0; j < PM; ++j) h[j] = 0.0;
+    double sy = 0.0;
+    for (int t = 0; t < T; ++t) {
+        const double yt = (double)y[t] * inv;
+        sy += yt;
+        const double* gr = G + (size_t)t * p;
+#pragma unroll
+        for (int j = 0; j < PM; ++j)
+            if (j < p) h[j] += gr[j] * yt;
+    }
+    // beta = Lx^-T h  (back substitution on the upper factor Lx')
+#pragma unroll
+    for (int jj = PM - 1; jj >= 0; --jj) {
+        if (jj >= p) {
+            be[jj] = 0.0;
+            continue;
+        }
+        double acc = h[jj];
+#pragma unroll
+        for (int m = jj + 1; m < PM; ++m)
+            if (m < p) acc -= L[m * p + jj] * be[m];
+        be[jj] = acc / L[jj * p + jj];
+    }
+    const double mean = sy / T;
+    double ssw = 0.0, ssres = 0.0, sstot = 0.0;
+    double yp = 0.0, e = 0.0;
+    for (int t = 0; t < T; ++t) {
+        const double yt = (double)y[t] * inv;
+        if (t == 0 || P.gap[t - 1] <= 0)
+            e = yt;
+        else
+            e = (yt - cf[3 * (t - 1) + 1] * yp) + cf[3 * (t - 1) + 2] * e;
+        yp = yt;
+        double r = e * cf[3 * t];
+        const double* xw = Xw + (size_t)t * p;
+        const double* xr = P.X + (size_t)t * p;
+        double pred = 0.0;
+#pragma unroll
+        for (int j = 0; j < PM; ++j)
+            if (j < p) {
+                r -= xw[j] * be[j];
+                pred += xr[j] * be[j];
+            }
+        ssw += r * r;
+        const double dr = yt - pred, dm = yt - mean;
+        ssres += dr * dr;
+        sstot += dm * dm;
+        if (resid_w) resid_w[(size_t)v * ldR + t] = r * s;
+        if (resid) resid[(size_t)v * ldR + t] = (float)(dr * s);
+        if (pred_out) pred_out[(size_t)v * ldR + t] = (float)(pred * s);
+    }
+    const int dof = max(1, T - p);
+    const double sig = ssw / dof;
+    const double* Ai = P.Ainv + (size_t)k * p * p;
+#pragma unroll
+    for (int j = 0; j < PM; ++j)
+        if (j < p) {
+            const double se = sqrt(sig * Ai[j * p + j]);
+            tstats[(size_t)v * p + j] = (float)(be[j] / (se + 1e-10));
+            betas[(size_t)v * p + j] = (float)(be[j] * s);
+        }
+    const int nt = P.n_task;
+    const double* F = P.Fm + (size_t)k * nt * nt;
+    double* sb = gls_beta + threadIdx.x * PM;
+#pragma unroll
+    for (int j = 0; j < PM; ++j) sb[j] = be[j];
+    double quad = 0.0;
+    for (int a = 0; a < nt; ++a) {
+        double ra = 0.0;
+        for (int b2 = 0; b2 < nt; ++b2) ra += F[a * nt + b2] * sb[P.task_idx[b2]];
+        quad += sb[P.task_idx[a]] * ra;
+    }
+    fstats[v] = (float)(quad / (sig * nt + 1e-10));
+    sigma2[v] = (float)(sig * s * s);
+    double rr;
+    if (sstot <= 1e-10)
+        rr = 0.0;
+    else
+        rr = fmin(1.0 - ssres / sstot, 1.0);
+    r2[v] = (float)rr;
+}
+
+// ---------------------------------------------------------------------------
+// Ljung-Box over whitened residuals (glm/arma.py:6059-6141), one wave per
+// voxel.  Lag pairs are pre-binned by time lag on the host: pairs[bin_off[d] ..
+// bin_off[d+1]) hold index pairs (i, j) whose tau lag is d; wbin[d] = 1/n_d for
+// bins with n_d > 1, else 0.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_ljung_box(const double* __restrict__ rw, int64_t ldR, int V, int n,
+                                                   int nbins, const int32_t* __restrict__ bin_off,
+                                                   const int32_t* __restrict__ pairs,
+                                                   const double* __restrict__ wbin, float* __restrict__ out) {
+    const int lane = threadIdx.x & 63;
+    const int v = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (v >= V) return;
+    const double* r = rw + (size_t)v * ldR;
+    double s0 = 0.0;
+    for (int t = lane; t < n; t += 64) s0 += r[t] * r[t];
+    for (int o = 32; o > 0; o >>= 1) s0 += __shfl_xor(s0, o);
+    const bool ok = s0 >= 1e-10;
+    const double sum0 = ok ? s0 : 1.0;
+    double g = 0.0;
+    for (int d = 1; d <= nbins; ++d) {
+        const double wd = wbin[d];
+        if (wd == 0.0) continue;
+        double sk = 0.0;
+        for (int q = bin_off[d] + lane; q < bin_off[d + 1]; q += 64) sk += r[pairs[2 * q]] * r[pairs[2 * q + 1]];
+        for (int o = 32; o > 0; o >>= 1) sk += __shfl_xor(sk, o);
+        const double ck = sk / sum0;
+        g += ck * ck * wd;
+    }
+    if (lane == 0) {
+        g *= (double)n * (n + 2.0);
+        out[v] = ok ? (float)fmin(g, 1.0e10) : 0.f;
+    }
+}
+
+}  // namespace remlk

@@ -1,0 +1,656 @@
+"""ARMA(1,1) REML GLM on MI355X: the ``fastfuncstuff.glm.arma`` fit API.
+
+Host-side mirror of the reference interface (fastfuncstuff/glm/arma.py):
+``fit_glm_arma11`` -> ``ARMA11Results``, ``precompute_reml_grid``,
+``search_voxels_precomputed_grid``, ``select_arma_params_hierarchical_from_surface``
+and the grid / lambda / censoring helpers, with the same argument meaning and
+error behaviour.  All per-voxel arithmetic runs in ``libffsreml.so``
+(hand-written gfx950 kernels); this module only shapes inputs, owns the device
+buffers and collects results on the host, as the reference does.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import math
+from collections.abc import Mapping
+from typing import Any
+
+import numpy as np
+import torch
+
+from .. import _lib
+
+# glm/arma.py:234-243 (AFNI 3dREMLfit -MAXa 0.8 -MAXb 0.8 -Grid 3)
+DEFAULT_ARMA_A_GRID = (0.0, 0.8, 9)
+DEFAULT_ARMA_B_GRID = (-0.8, 0.8, 17)
+
+# glm/arma.py:173-200
+VOXEL_SCATTER_ATTRS = [
+    "betas", "tstats", "r2", "r2_partial", "r2_partial_nuisance", "r2_semipartial",
+    "r2_semipartial_nuisance", "arma_params", "arma_lambda", "reml_likelihood", "sigma2", "fstats",
+    "residuals", "residuals_whitened", "predicted", "contrast_betas", "contrast_tstats",
+    "contrast_fstats", "contrast_r2_partial", "contrast_r2_semipartial", "dsort_betas", "dsort_tstats",
+    "ljung_box", "set_fstats", "set_sd", "set_coef",
+]
+
+# Runs are offset by this much in the Ljung-Box pseudo-time (glm/arma.py:5969).
+_LJ_RUN_SEPARATION = 66666
+
+
+# ---------------------------------------------------------------------------
+# Grid helpers (host logic, float32 semantics of the reference)
+# ---------------------------------------------------------------------------
+
+
+def get_default_arma_grids(device: torch.device | None = None) -> tuple[torch.Tensor, torch.Tensor]:
+    """glm/arma.py:484-513: a = linspace(0, 0.8, 9), b = linspace(-0.8, 0.8, 17)."""
+    return (torch.linspace(*DEFAULT_ARMA_A_GRID, device=device),
+            torch.linspace(*DEFAULT_ARMA_B_GRID, device=device))
+
+
+def ensure_zero_in_grid(a_grid: torch.Tensor, b_grid: torch.Tensor, tolerance: float = 1e-6):
+    """glm/arma.py:516-582: snap |g| < tolerance to exactly 0, or insert 0 and sort."""
+
+    def fix(g: torch.Tensor) -> torch.Tensor:
+        near = torch.abs(g) < tolerance
+        if torch.any(near):
+            g = g.clone()
+            g[near] = 0.0
+            return g
+        return torch.sort(torch.cat([torch.zeros(1, dtype=g.dtype, device=g.device), g]))[0]
+
+    return fix(a_grid), fix(b_grid)
+
+
+def _compute_arma11_lambda(a, b):
+    """glm/arma.py:695-729: lambda = (a+b)(1+ab)/(1+b^2+2ab)."""
+    return (a + b) * (1 + a * b) / (1 + b**2 + 2 * a * b)
+
+
+def compute_arma_lambda(a: float, b: float) -> float:
+    """glm/arma.py:434-462."""
+    return float(_compute_arma11_lambda(a, b))
+
+
+def _grid_tensor(g, name: str) -> torch.Tensor:
+    t = torch.as_tensor(g)
+    if t.ndim != 1 or t.numel() == 0:
+        raise ValueError(f"{name} must be a non-empty 1-D grid")
+    return t.detach().cpu().to(torch.float32) if not t.dtype.is_floating_point else t.detach().cpu()
+
+
+def valid_grid_pairs(a_grid: torch.Tensor, b_grid: torch.Tensor) -> torch.Tensor:
+    """The reference's validity filter (glm/arma.py:999-1048), in the grid dtype:
+    meshgrid 'ij' (a-major, b-minor), |a|,|b| < 1, |1-a^2| >= 1e-6, gamma0 > 0,
+    lambda >= 0.  Returns the (K, 2) valid pairs in search order."""
+    am, bm = torch.meshgrid(a_grid, b_grid, indexing="ij")
+    a, b = am.reshape(-1), bm.reshape(-1)
+    ok = (torch.abs(a) < 1.0) & (torch.abs(b) < 1.0)
+    denom = 1 - a**2
+    ok &= torch.abs(denom) >= 1e-6
+    ok &= (1 + b**2 + 2 * a * b) / (denom + 1e-10) > 0
+    a, b = a[ok], b[ok]
+    keep = _compute_arma11_lambda(a, b) >= 0
+    return torch.stack([a[keep], b[keep]], dim=1)
+
+
+def estimate_valid_grid_pairs(a_grid: torch.Tensor, b_grid: torch.Tensor) -> int:
+    """glm/arma.py:291-338."""
+    return int(valid_grid_pairs(_grid_tensor(a_grid, "a_grid"), _grid_tensor(b_grid, "b_grid")).shape[0])
+
+
+def build_censor_run_info(run_starts, n_total: int, good_list=None):
+    """glm/arma.py:752-820: run starts in retained-index space + within-run tau."""
+    starts = run_starts.tolist() if isinstance(run_starts, torch.Tensor) else list(run_starts)
+    if good_list is None:
+        return [int(s) for s in starts], None
+    retained = torch.as_tensor(good_list.tolist() if isinstance(good_list, torch.Tensor) else list(good_list),
+                               dtype=torch.long).flatten()
+    st = torch.tensor(starts, dtype=torch.long)
+    run_id = torch.bucketize(retained, st, right=True) - 1
+    tau = retained - st[run_id]
+    out: list[int] = []
+    for r in range(len(starts)):
+        pos = (run_id == r).nonzero(as_tuple=False)
+        if pos.numel() > 0:
+            out.append(int(pos[0].item()))
+    return (out or [0]), tau
+
+
+def run_bounds(n: int, run_starts=None) -> list[tuple[int, int]]:
+    """glm/whiten.py:27-38."""
+    if run_starts is None:
+        return [(0, n)]
+    s = sorted({int(x) for x in (run_starts.tolist() if isinstance(run_starts, torch.Tensor) else run_starts)})
+    if not s or s[0] != 0:
+        s = [0, *s]
+    e = [*s[1:], n]
+    return [(x, y) for x, y in zip(s, e) if y > x]
+
+
+def timeline_gaps(n: int, run_starts=None, tau=None) -> np.ndarray:
+    """Per retained index: tau[t+1]-tau[t] inside a run, 0 at each run's last point."""
+    gap = np.zeros(n, dtype=np.int32)
+    t_arr = None if tau is None else np.asarray(torch.as_tensor(tau).flatten().tolist(), dtype=np.int64)
+    if t_arr is not None and t_arr.shape[0] != n:
+        raise ValueError(f"tau must have one entry per retained timepoint ({n}), got {t_arr.shape[0]}")
+    for s, e in run_bounds(n, run_starts):
+        if t_arr is None:
+            gap[s:e - 1] = 1
+        else:
+            d = np.diff(t_arr[s:e])
+            if np.any(d <= 0):
+                raise ValueError("tau must be strictly increasing within each run")
+            gap[s:e - 1] = d
+        gap[e - 1] = 0
+    return gap
+
+
+def hierarchical_ltop(n_a: int, n_b: int) -> int:
+    """glm/arma.py:2290-2296 (default, non-AFNI mode)."""
+    return max(0, min(n_a, n_b).bit_length() - 2)
+
+
+def design_pinv(X: np.ndarray, tol_dtype=np.float32) -> np.ndarray:
+    """glm/core.py:248-272: rank-truncated SVD pseudo-inverse in float64 with the
+    design dtype's tolerance.  Returns pinv (p x T)."""
+    Xd = np.asarray(X, dtype=np.float64)
+    U, S, Vh = np.linalg.svd(Xd, full_matrices=False)
+    tol = max(Xd.shape) * np.finfo(tol_dtype).eps * (S[0] if S.size else 0.0)
+    r = int((S > tol).sum())
+    return (Vh[:r].T / S[:r]) @ U[:, :r].T
+
+
+# ---------------------------------------------------------------------------
+# Precomputed grid = a device plan
+# ---------------------------------------------------------------------------
+
+
+class RemlGrid(Mapping):
+    """Per-pair tables for one design (replaces the dict of
+    ``precompute_reml_grid``, glm/arma.py:1371-1531).
+
+    Maps ``(a, b)`` to ``{'a', 'b', 'logdet_Rcorr', 'logdet_XwTXw'}`` like the
+    reference's dict; the whiteners, whitened designs and projection operands
+    live on the device inside the plan."""
+
+    def __init__(self, X, a_grid, b_grid, device, run_starts=None, tau=None, task_indices=None, pairs=None):
+        self.device = _lib.require_device(device)
+        lib = _lib.load()
+        X = torch.as_tensor(X)
+        if X.ndim != 2:
+            raise ValueError("design must be 2-D (n_timepoints, n_regressors)")
+        self.X64 = np.ascontiguousarray(X.detach().cpu().to(torch.float64).numpy())
+        self.T, self.p = self.X64.shape
+        if pairs is None:
+            a_grid, b_grid = ensure_zero_in_grid(_grid_tensor(a_grid, "a_grid"), _grid_tensor(b_grid, "b_grid"))
+            pairs = valid_grid_pairs(a_grid, b_grid)
+        self.params = pairs.to(torch.float32) if pairs.dtype != torch.float64 else pairs
+        if self.params.shape[0] == 0:
+            raise ValueError("No valid (a,b) parameters in grid")
+        self.K = int(self.params.shape[0])
+        a_np = self.params[:, 0].double().numpy()
+        b_np = self.params[:, 1].double().numpy()
+        lam32 = _compute_arma11_lambda(self.params[:, 0], self.params[:, 1])
+        white = (lam32 == 0).to(torch.int32).numpy()
+        self.a_axis = sorted(set(a_np.tolist()))
+        self.b_axis = sorted(set(b_np.tolist()))
+        n_a, n_b = len(self.a_axis), len(self.b_axis)
+        ai = {v: i for i, v in enumerate(self.a_axis)}
+        bi = {v: i for i, v in enumerate(self.b_axis)}
+        lut = -np.ones((n_a, n_b), dtype=np.int32)
+        for k in range(self.K):
+            lut[ai[a_np[k]], bi[b_np[k]]] = k
+        self.lut = lut
+        za = next((i for i, v in enumerate(self.a_axis) if abs(v) < 1e-6), None)
+        zb = next((i for i, v in enumerate(self.b_axis) if abs(v) < 1e-6), None)
+        self.has_zero = za is not None and zb is not None and lut[za, zb] >= 0
+        za = 0 if za is None else za
+        zb = 0 if zb is None else zb
+        self.zero = (za, zb)
+        self.ltop = hierarchical_ltop(n_a, n_b)
+        self.gap = timeline_gaps(self.T, run_starts, tau)
+        self.run_starts = run_starts
+        self.tau = tau
+        pinvT = np.ascontiguousarray(design_pinv(self.X64).T)
+        ab = np.ascontiguousarray(np.stack([a_np, b_np], 1))
+        task = np.asarray(task_indices if task_indices is not None else [], dtype=np.int32)
+        self.task_indices = None if task_indices is None else list(task_indices)
+        handle = ctypes.c_void_p()
+        with torch.cuda.device(self.device):
+            rc = lib.ffs_reml_plan_create(
+                ctypes.byref(handle), self.T, self.p, self.X64.ctypes.data, pinvT.ctypes.data,
+                self.gap.ctypes.data, self.K, ab.ctypes.data, np.ascontiguousarray(white).ctypes.data,
+                n_a, n_b, lut.ctypes.data, za, zb, self.ltop, int(task.size),
+                task.ctypes.data if task.size else None, _lib.stream_handle(self.device))
+        _lib.check(rc, "precompute_reml_grid")
+        self._handle = handle
+        tp, vp, _ = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+        _lib.check(lib.ffs_reml_plan_dims(handle, ctypes.byref(tp), ctypes.byref(vp), None), "plan_dims")
+        self.Tpad, self.vox_pad = tp.value, vp.value
+        ldR = np.zeros(self.K)
+        ldX = np.zeros(self.K)
+        _lib.check(lib.ffs_reml_plan_logdets(handle, ldR.ctypes.data, ldX.ctypes.data), "plan_logdets")
+        self.logdet_R, self.logdet_XRX = ldR, ldX
+        self.param_list = [(float(a), float(b)) for a, b in self.params.tolist()]
+
+    # Mapping protocol (the reference's dict keyed by (a, b))
+    def __getitem__(self, key):
+        k = self.param_list.index((float(key[0]), float(key[1])))
+        return {"a": self.param_list[k][0], "b": self.param_list[k][1],
+                "logdet_Rcorr": torch.tensor(self.logdet_R[k]), "logdet_XwTXw": torch.tensor(self.logdet_XRX[k])}
+
+    def __iter__(self):
+        return iter(self.param_list)
+
+    def __len__(self):
+        return self.K
+
+    @property
+    def handle(self):
+        return self._handle
+
+    def close(self):
+        h = getattr(self, "_handle", None)
+        if h is not None and h.value:
+            _lib.load().ffs_reml_plan_destroy(h)
+            self._handle = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- device helpers -------------------------------------------------
+    def vpad(self, V: int) -> int:
+        return (V + self.vox_pad - 1) // self.vox_pad * self.vox_pad
+
+    def prep(self, Y: torch.Tensor, normalize: bool = True, want_features: bool = False):
+        """Normalise + residualise Y (V, T) float32 on the device.  Returns
+        (Yt (Tpad, Vpad) time-major residual, scale (V,), features (V, 10) | None)."""
+        V = Y.shape[0]
+        Vp = self.vpad(V)
+        Yt = torch.zeros(self.Tpad, Vp, dtype=torch.float32, device=self.device)
+        scale = torch.empty(V, dtype=torch.float32, device=self.device)
+        feats = torch.empty(V, 10, dtype=torch.float32, device=self.device) if want_features else None
+        lib = _lib.load()
+        _lib.check(lib.ffs_reml_prep(self._handle, _lib.ptr(Y), Y.stride(0), V, _lib.ptr(Yt), Vp, _lib.ptr(scale),
+                                     int(normalize), _lib.ptr(feats), _lib.stream_handle(self.device)), "prep")
+        return Yt, scale, feats
+
+    def surface(self, Yt: torch.Tensor, V: int, out: torch.Tensor | None = None) -> torch.Tensor:
+        """(V, K) REML score surface: energy kernel then MFMA projection."""
+        S = out if out is not None else torch.empty(V, self.K, dtype=torch.float32, device=self.device)
+        lib = _lib.load()
+        st = _lib.stream_handle(self.device)
+        _lib.check(lib.ffs_reml_energy(self._handle, _lib.ptr(Yt), Yt.stride(0), V, _lib.ptr(S), S.stride(0), st),
+                   "energy")
+        _lib.check(lib.ffs_reml_project(self._handle, _lib.ptr(Yt), Yt.stride(0), V, _lib.ptr(S), S.stride(0), st),
+                   "project")
+        return S
+
+    def select(self, S: torch.Tensor, hierarchical: bool):
+        V = S.shape[0]
+        best = torch.empty(V, dtype=torch.int32, device=self.device)
+        score = torch.empty(V, dtype=torch.float32, device=self.device)
+        _lib.check(_lib.load().ffs_reml_select(self._handle, _lib.ptr(S), S.stride(0), V, int(hierarchical),
+                                               _lib.ptr(best), _lib.ptr(score), _lib.stream_handle(self.device)),
+                   "select")
+        return best, score
+
+    def gls(self, Y: torch.Tensor, scale: torch.Tensor | None, best: torch.Tensor, want_resid_w: bool = False,
+            want_residuals: bool = False, want_predicted: bool = False):
+        V = Y.shape[0]
+        dev = self.device
+        f32 = dict(dtype=torch.float32, device=dev)
+        out = {"betas": torch.empty(V, self.p, **f32), "tstats": torch.empty(V, self.p, **f32),
+               "fstats": torch.empty(V, **f32), "sigma2": torch.empty(V, **f32), "r2": torch.empty(V, **f32)}
+        rw = torch.empty(V, self.T, dtype=torch.float64, device=dev) if want_resid_w else None
+        res = torch.empty(V, self.T, **f32) if want_residuals else None
+        pred = torch.empty(V, self.T, **f32) if want_predicted else None
+        work = torch.empty(V + self.K + 1, dtype=torch.int32, device=dev)
+        _lib.check(_lib.load().ffs_reml_gls(
+            self._handle, _lib.ptr(Y), Y.stride(0), _lib.ptr(scale), _lib.ptr(best), V, _lib.ptr(out["betas"]),
+            _lib.ptr(out["tstats"]), _lib.ptr(out["fstats"]), _lib.ptr(out["sigma2"]), _lib.ptr(out["r2"]),
+            _lib.ptr(rw), self.T, _lib.ptr(res), _lib.ptr(pred), _lib.ptr(work), _lib.stream_handle(dev)), "gls")
+        out["resid_w"], out["residuals"], out["predicted"] = rw, res, pred
+        return out
+
+
+def precompute_reml_grid(X, n_timepoints, a_grid, b_grid, device=None, verbose=False, cholesky_on_cpu=True,
+                         dtype=torch.float32, debug_memory=False, use_qr=False, run_starts=None,
+                         autocorr_cache=None, tau=None, task_indices=None) -> RemlGrid:
+    """glm/arma.py:1371-1531: build the per-pair tables for design X."""
+    X = torch.as_tensor(X)
+    if X.shape[0] != n_timepoints:
+        raise ValueError(f"Timepoints mismatch: Data has {n_timepoints}, Design has {X.shape[0]}")
+    return RemlGrid(X, a_grid, b_grid, device, run_starts=run_starts, tau=tau, task_indices=task_indices)
+
+
+def _to_device_f32(Y, device) -> torch.Tensor:
+    Y = torch.as_tensor(Y)
+    return Y.to(device=device, dtype=torch.float32).contiguous()
+
+
+def search_voxels_precomputed_grid(X, Y_batch, precomputed_grid: RemlGrid, device=None, verbose=False,
+                                   enable_timing=False, return_profile=False):
+    """glm/arma.py:1886-2018: exhaustive search (strict '<', first pair wins ties).
+
+    Returns (best_params (V,2), best_lik (V,)) and, with return_profile, the
+    (V, K) surface and the param list.  Outputs stay on the device."""
+    grid = precomputed_grid
+    Y = _to_device_f32(Y_batch, grid.device)
+    if Y.ndim != 2 or Y.shape[1] != grid.T:
+        raise ValueError(f"Timepoints mismatch: Data has {Y.shape[-1]}, Design has {grid.T}")
+    V = Y.shape[0]
+    Yt, _, _ = grid.prep(Y, normalize=False)
+    S = grid.surface(Yt, V)
+    best, score = grid.select(S, hierarchical=False)
+    params = grid.params.to(grid.device)[best.long()]
+    if return_profile:
+        return params, score, S, list(grid.param_list)
+    return params, score
+
+
+def select_arma_params_hierarchical_from_surface(surface: torch.Tensor, param_list, device=None,
+                                                 grid: RemlGrid | None = None):
+    """glm/arma.py:2224-2322 on a (V, K) surface.  ``grid`` (optional) supplies
+    the lattice; otherwise it is rebuilt from ``param_list``."""
+    if grid is None or list(grid.param_list) != list(param_list):
+        raise ValueError("select_arma_params_hierarchical_from_surface needs the RemlGrid the surface came from")
+    S = surface.to(device=grid.device, dtype=torch.float32).contiguous()
+    best, score = grid.select(S, hierarchical=True)
+    return grid.params.to(grid.device)[best.long()], score
+
+
+# ---------------------------------------------------------------------------
+# Ljung-Box helpers (glm/arma.py:5972-6049) and the device call
+# ---------------------------------------------------------------------------
+
+
+def ljung_box_max_lag(n_time: int, n_regressors: int = 0, min_run: int | None = None) -> int:
+    """glm/arma.py:5972-6000."""
+    if min_run is None:
+        min_run = n_time
+    if min_run < 2:
+        return 0
+    hh = n_regressors + 2 + min(min_run // 8, int(round(3.0 * math.log(min_run))))
+    return min(hh, min_run // 2)
+
+
+def _resolve_ljung_box_lag(n_time: int, max_lag) -> int:
+    """glm/arma.py:6003-6014."""
+    hh = int(max_lag) if max_lag else 0
+    if hh < 2 or hh > n_time // 2:
+        hh = 2 + min(n_time // 8, int(round(3.0 * math.log(n_time))))
+        hh = min(hh, n_time // 2)
+    return hh
+
+
+def build_ljung_box_tau(n_time: int, run_starts=None, tau=None, device=None):
+    """glm/arma.py:6017-6049."""
+    starts = (run_starts.tolist() if isinstance(run_starts, torch.Tensor)
+              else (list(run_starts) if run_starts is not None else []))
+    starts = sorted({int(s) for s in starts if 0 <= int(s) < n_time}) or [0]
+    if tau is None and len(starts) == 1 and starts[0] == 0:
+        return None
+    st = torch.tensor(starts, dtype=torch.long)
+    idx = torch.arange(n_time, dtype=torch.long)
+    rid = torch.bucketize(idx, st, right=True) - 1
+    within = torch.as_tensor(tau).to(torch.long).flatten().cpu() if tau is not None else idx - st[rid]
+    out = within + _LJ_RUN_SEPARATION * rid
+    return out.to(device) if device is not None else out
+
+
+def ljung_box_bins(n: int, max_lag, tau=None):
+    """Host binning of lag pairs by time lag (the reference bins on the host too,
+    glm/arma.py:6103-6123).  Returns (hh, bin_off, pairs, wbin)."""
+    hh = _resolve_ljung_box_lag(n, max_lag)
+    t = np.arange(n) if tau is None else np.asarray(torch.as_tensor(tau).tolist(), dtype=np.int64)
+    by_bin: list[list[tuple[int, int]]] = [[] for _ in range(hh + 1)]
+    for kk in range(1, hh + 1):
+        dj = t[kk:] - t[: n - kk]
+        for j in np.nonzero((dj > 0) & (dj <= hh))[0]:
+            by_bin[int(dj[j])].append((int(j), int(j + kk)))
+    off = np.zeros(hh + 2, dtype=np.int32)
+    flat: list[int] = []
+    for d in range(1, hh + 1):
+        off[d] = len(flat) // 2
+        for i, j in by_bin[d]:
+            flat += [i, j]
+        off[d + 1] = len(flat) // 2
+    nj = np.array([len(b) for b in by_bin], dtype=np.float64)
+    w = np.where(nj > 1, 1.0 / np.maximum(nj, 1), 0.0)
+    w[0] = 0.0
+    return hh, off, np.asarray(flat, dtype=np.int32), w
+
+
+def ljung_box_device(resid_w: torch.Tensor, max_lag=None, tau=None) -> torch.Tensor:
+    """Ljung-Box of (V, n) float64 whitened residuals on the device (float32 out)."""
+    V, n = resid_w.shape
+    dev = resid_w.device
+    if n < 10:
+        return torch.zeros(V, dtype=torch.float32, device=dev)
+    hh, off, pairs, w = ljung_box_bins(n, max_lag, tau)
+    if hh < 1:
+        return torch.zeros(V, dtype=torch.float32, device=dev)
+    d_off = torch.from_numpy(off).to(dev)
+    d_pairs = torch.from_numpy(pairs if pairs.size else np.zeros(2, np.int32)).to(dev)
+    d_w = torch.from_numpy(w).to(dev)
+    out = torch.empty(V, dtype=torch.float32, device=dev)
+    _lib.check(_lib.load().ffs_reml_ljung_box(_lib.ptr(resid_w), resid_w.stride(0), V, n, hh, _lib.ptr(d_off),
+                                              _lib.ptr(d_pairs), _lib.ptr(d_w), _lib.ptr(out),
+                                              _lib.stream_handle(dev)), "ljung_box")
+    return out
+
+
+# ---------------------------------------------------------------------------
+# Results container + fit
+# ---------------------------------------------------------------------------
+
+
+class ARMA11Results:
+    """Container for ARMA(1,1) GLM results (glm/arma.py:585-692): same fields."""
+
+    def __init__(self):
+        self.betas = None
+        self.tstats = None
+        self.r2 = None
+        self.r2_partial = None
+        self.r2_partial_nuisance = None
+        self.r2_semipartial = None
+        self.r2_semipartial_nuisance = None
+        self.arma_params = None
+        self.arma_lambda = None
+        self.reml_likelihood = None
+        self.residuals = None
+        self.predicted = None
+        self.residuals_whitened = None
+        self.ljung_box = None
+        self.ljung_box_dof = None
+        self.sigma2 = None
+        self.var_betas = None
+        self.original_shape = None
+        self.fstats = None
+        self.set_fstats = None
+        self.set_sd = None
+        self.set_coef = None
+        self.nuisance_sets = None
+        self.voxel_mean = None
+        self.dof = None
+        self.tr = None
+        self.voxel_mask = None
+        self.full_shape = None
+        self.fitted_column_indices = None
+        self.n_regressors_full = None
+        self.contrast_labels = None
+        self.contrast_betas = None
+        self.contrast_tstats = None
+        self.contrast_fstats = None
+        self.contrast_r2_partial = None
+        self.contrast_r2_semipartial = None
+        self.affine = None
+        self.ols_results = None
+        self.dsort_betas = None
+        self.dsort_tstats = None
+        self.dsort_labels = None
+        self.nods_results = None
+        self.reml_lklhd_surface = None
+        self.reml_surface_params = None
+        # selected pair index per voxel (search order), kept for diagnostics
+        self.pair_index = None
+
+
+_UNSUPPORTED = {
+    "dsort": "voxel-wise -dsort regressors",
+    "column_sets": "-nuisance_summary column sets",
+    "glt_matrices": "in-loop GLT contrasts",
+}
+
+
+def fit_glm_arma11(data, design, tr, a_grid=None, b_grid=None, estimate_per_voxel=True, batch_size=None,
+                   want_residuals=False, want_predicted=False, want_ljung_box=False, want_r2_partial=False,
+                   r2_partial_mode="full", want_r2_semipartial=False, r2_semipartial_mode="full",
+                   precomputed_arma_params=None, device=None, verbose=False, use_double=False,
+                   force_exhaustive_search=False, task_indices=None, save_profile_likelihoods=False,
+                   run_starts=None, tau=None, grid: RemlGrid | None = None, **kwargs) -> ARMA11Results:
+    """glm/arma.py:3650-5878 ``fit_glm_arma11`` on the MI355X engine.
+
+    Per-voxel REML search over the (a, b) grid (hierarchical descent by default,
+    global argmin with ``force_exhaustive_search``), then an FP64 GLS refit at
+    the selected pair.  float32 mode divides each voxel by its std before the
+    fit and unscales betas / sigma2 / residuals afterwards (glm/arma.py:3947-3954,
+    5827-5862); ``use_double`` skips the normalisation.  Outputs are CPU tensors.
+    """
+    for key, what in _UNSUPPORTED.items():
+        if kwargs.get(key) is not None:
+            raise NotImplementedError(f"{what} ({key}=...) is not supported by the MI355X engine")
+    dev = _lib.require_device(device)
+    out_dtype = torch.float64 if use_double else torch.float32
+    data = torch.as_tensor(data)
+    design = torch.as_tensor(design)
+    original_shape = None
+    if data.ndim == 4:
+        original_shape = tuple(data.shape[:3])
+        data = data.reshape(-1, data.shape[-1])
+    elif data.ndim != 2:
+        raise ValueError(f"Data must be 2D or 4D, got {data.ndim}D")
+    V, T = data.shape
+    if design.shape[0] != T:
+        raise ValueError(f"Timepoints mismatch: Data has {T}, Design has {design.shape[0]}")
+    p = design.shape[1]
+    if not estimate_per_voxel:
+        raise NotImplementedError("global (mean time series) ARMA estimation is not supported")
+
+    res = ARMA11Results()
+    res.original_shape = original_shape
+    res.fitted_column_indices = task_indices
+    res.n_regressors_full = p
+    res.dof = max(1, T - p)
+    res.tr = tr
+
+    if precomputed_arma_params is not None:
+        pp = torch.as_tensor(precomputed_arma_params).detach().cpu()
+        if pp.shape != (V, 2):
+            raise ValueError(f"precomputed_arma_params must have shape ({V}, 2), got {tuple(pp.shape)}")
+        uniq, inv = torch.unique(pp.to(torch.float64), dim=0, return_inverse=True)
+        bad = (uniq.abs() >= 1).any(dim=1)
+        if bool(bad.any()):
+            a, b = uniq[bad][0].tolist()
+            raise ValueError(f"Invalid ARMA(1,1) parameters: a={a}, b={b}")
+        grid = RemlGrid(design, None, None, dev, run_starts=run_starts, tau=tau, task_indices=task_indices,
+                        pairs=uniq.to(pp.dtype if pp.dtype.is_floating_point else torch.float64))
+        fixed_best = inv.to(torch.int32)
+    else:
+        if a_grid is None or b_grid is None:
+            da, db = get_default_arma_grids()
+            a_grid = da if a_grid is None else a_grid
+            b_grid = db if b_grid is None else b_grid
+        if grid is None:
+            grid = RemlGrid(design, a_grid, b_grid, dev, run_starts=run_starts, tau=tau, task_indices=task_indices)
+        fixed_best = None
+
+    K = grid.K
+    nb = batch_size or min(V, 1 << 20)
+    f = torch.float32
+    res.betas = torch.zeros(V, len(task_indices) if task_indices else p, dtype=out_dtype)
+    res.tstats = torch.zeros_like(res.betas)
+    res.r2 = torch.zeros(V, dtype=torch.float32)
+    res.arma_params = torch.zeros(V, 2, dtype=out_dtype)
+    res.arma_lambda = torch.zeros(V, dtype=out_dtype)
+    res.reml_likelihood = torch.zeros(V, dtype=out_dtype)
+    res.sigma2 = torch.zeros(V, dtype=out_dtype)
+    res.fstats = torch.zeros(V, dtype=out_dtype)
+    res.pair_index = torch.zeros(V, dtype=torch.int64)
+    if want_residuals:
+        res.residuals = torch.zeros(V, T, dtype=out_dtype)
+        res.residuals_whitened = torch.zeros(V, T, dtype=out_dtype)
+    if want_predicted:
+        res.predicted = torch.zeros(V, T, dtype=out_dtype)
+    surface_acc = torch.empty(V, K, dtype=f) if save_profile_likelihoods and fixed_best is None else None
+    lj_lag = lj_tau = None
+    if want_ljung_box:
+        starts = sorted({int(s) for s in (run_starts.tolist() if isinstance(run_starts, torch.Tensor)
+                                          else run_starts)} if run_starts is not None else {0})
+        min_run = (min(b - a for a, b in zip(starts, starts[1:] + [T])) if len(starts) > 1 else T)
+        lj_lag = ljung_box_max_lag(T, p, min_run)
+        lj_tau = build_ljung_box_tau(T, run_starts, tau)
+        res.ljung_box = torch.zeros(V, dtype=torch.float32)
+        res.ljung_box_dof = max(1, _resolve_ljung_box_lag(T, lj_lag) - 2)
+    params_dev = grid.params.to(dev)
+    with torch.cuda.device(dev):
+        for b0 in range(0, V, nb):
+            b1 = min(V, b0 + nb)
+            Yb = _to_device_f32(data[b0:b1], dev)
+            Vb = b1 - b0
+            if fixed_best is None:
+                Yt, scale, _ = grid.prep(Yb, normalize=not use_double)
+                S = grid.surface(Yt, Vb)
+                del Yt
+                best, score = grid.select(S, hierarchical=not force_exhaustive_search)
+                if surface_acc is not None:
+                    surface_acc[b0:b1] = S.cpu()
+                del S
+            else:
+                best = fixed_best[b0:b1].to(dev)
+                score = torch.zeros(Vb, dtype=f, device=dev)
+                scale = None
+                if not use_double:
+                    _, scale, _ = grid.prep(Yb, normalize=True)
+            g = grid.gls(Yb, None if use_double else scale, best, want_resid_w=want_ljung_box or want_residuals,
+                         want_residuals=want_residuals, want_predicted=want_predicted)
+            ti = task_indices
+            res.betas[b0:b1] = (g["betas"][:, ti] if ti else g["betas"]).cpu().to(out_dtype)
+            res.tstats[b0:b1] = (g["tstats"][:, ti] if ti else g["tstats"]).cpu().to(out_dtype)
+            res.fstats[b0:b1] = g["fstats"].cpu().to(out_dtype)
+            res.sigma2[b0:b1] = g["sigma2"].cpu().to(out_dtype)
+            res.r2[b0:b1] = g["r2"].cpu()
+            sel = params_dev[best.long()]
+            res.arma_params[b0:b1] = sel.cpu().to(out_dtype)
+            res.reml_likelihood[b0:b1] = score.cpu().to(out_dtype)
+            res.pair_index[b0:b1] = best.cpu().long()
+            if want_residuals:
+                res.residuals[b0:b1] = g["residuals"].cpu().to(out_dtype)
+                res.residuals_whitened[b0:b1] = g["resid_w"].cpu().to(out_dtype)
+            if want_predicted:
+                res.predicted[b0:b1] = g["predicted"].cpu().to(out_dtype)
+            if want_ljung_box:
+                res.ljung_box[b0:b1] = ljung_box_device(g["resid_w"], lj_lag, lj_tau).cpu()
+    a, b = res.arma_params[:, 0], res.arma_params[:, 1]
+    res.arma_lambda = (b + a) * (1 + a * b) / (1 + 2 * a * b + b**2 + 1e-10)  # glm/arma.py:4769-4777
+    if fixed_best is not None:
+        res.reml_likelihood = None
+    if surface_acc is not None:
+        res.reml_lklhd_surface = surface_acc
+        res.reml_surface_params = list(grid.param_list)
+    if want_r2_partial or want_r2_semipartial:
+        _partial_r2(res, p, task_indices, want_r2_partial, r2_partial_mode, want_r2_semipartial,
+                    r2_semipartial_mode)
+    return res
+
+
+def _partial_r2(res, p, task_indices, want_partial, partial_mode, want_semi, semi_mode):
+    """glm/arma.py:5054-5175: r2_partial = t^2/(t^2+dof); semi = partial*(1-R^2)."""
+    raise NotImplementedError("partial / semi-partial R^2 are not provided yet")

@@ -1,0 +1,157 @@
+"""Seeded synthetic fMRI workloads for the ARMA(1,1) REML path.
+
+Designs follow the reference's inputs: a double-gamma HRF (gamma pdf shape 6
+minus 1/6 of shape 16, unit scale, fastfuncstuff/design/hrf.py:354-372)
+convolved with boxcars, Legendre polynomial baselines on [-1, 1]
+(design/builder.py:105-140), AR(1) random-walk motion-like columns; noise is
+stationary ARMA(1,1) with unit marginal variance.
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+from scipy import stats
+
+
+def double_gamma_hrf(tr: float, peak: float = 6.0, under: float = 16.0, ratio: float = 1.0 / 6.0,
+                     duration: float = 32.0, dt: float = 0.1) -> tuple[np.ndarray, float]:
+    """High-resolution double-gamma kernel (sampled every ``dt`` s) and its dt."""
+    t = np.arange(0.0, duration, dt)
+    h = stats.gamma.pdf(t, peak) - ratio * stats.gamma.pdf(t, under)
+    return h / np.abs(h).max(), dt
+
+
+def boxcar_regressor(onsets: np.ndarray, dur: float, n: int, tr: float, hrf=None) -> np.ndarray:
+    """Boxcars at ``onsets`` (s) of length ``dur`` (s), convolved with the HRF at
+    high resolution and sampled every TR, scaled to unit peak."""
+    h, dt = hrf if hrf is not None else double_gamma_hrf(tr)
+    nh = int(round(n * tr / dt))
+    s = np.zeros(nh)
+    for o in onsets:
+        i0 = int(round(o / dt))
+        s[i0:min(nh, i0 + max(1, int(round(dur / dt))))] = 1.0
+    x = np.convolve(s, h)[:nh][:: int(round(tr / dt))][:n]
+    m = np.abs(x).max()
+    return x / m if m > 0 else x
+
+
+def legendre_baseline(n: int, order: int) -> np.ndarray:
+    """Legendre polynomials P_0..P_order on [-1, 1], unnormalised (AFNI -polort)."""
+    x = np.linspace(-1.0, 1.0, n)
+    return np.stack([np.polynomial.legendre.Legendre.basis(k)(x) for k in range(order + 1)], axis=1)
+
+
+def arma11_noise(rng: np.random.Generator, a: np.ndarray, b: np.ndarray, T: int, df: np.ndarray | None = None,
+                 dtype=np.float32) -> np.ndarray:
+    """Stationary ARMA(1,1) x_t = a x_{t-1} + e_t + b e_{t-1}, unit marginal variance.
+
+    ``df`` (optional, per voxel, inf = Gaussian) draws Student-t innovations
+    scaled to unit variance.  Returns (V, T)."""
+    V = a.shape[0]
+    sig2 = (1 - a**2) / (1 + 2 * a * b + b**2)
+    sd = np.sqrt(sig2)
+
+    def innov(size):
+        z = rng.standard_normal(size)
+        if df is not None:
+            heavy = np.isfinite(df)
+            if heavy.any():
+                d = df[heavy][:, None] if z.ndim == 2 else df[heavy]
+                tz = rng.standard_t(np.broadcast_to(d, z[heavy].shape))
+                z[heavy] = tz / np.sqrt(d / (d - 2.0))
+        return z
+
+    out = np.empty((V, T), dtype=np.float64)
+    e_prev = innov(V) * sd
+    # stationary start: Var(x)=1, Cov(x_0, e_0) = sig2
+    x = e_prev + np.sqrt(np.maximum(1.0 - sig2, 0.0)) * rng.standard_normal(V)
+    out[:, 0] = x
+    for t in range(1, T):
+        e = innov(V) * sd
+        x = a * x + e + b * e_prev
+        out[:, t] = x
+        e_prev = e
+    return out.astype(dtype)
+
+
+@dataclass
+class Workload:
+    name: str
+    Y: np.ndarray  # (V, T_retained) float32
+    X: np.ndarray  # (T_retained, p) float64
+    tr: float
+    run_starts: list | None
+    tau: np.ndarray | None
+    a_true: np.ndarray
+    b_true: np.ndarray
+    betas: np.ndarray
+    a_grid: np.ndarray
+    b_grid: np.ndarray
+    task_indices: list | None = None
+
+
+def config_grid() -> tuple[np.ndarray, np.ndarray]:
+    """a, b in [-0.8, 0.8] step 0.05 (33 x 33), float32 linspace."""
+    import torch
+
+    g = torch.linspace(-0.8, 0.8, 33).numpy()
+    return g.copy(), g.copy()
+
+
+def plain_config(V: int = 20_000, T: int = 200, tr: float = 2.0, seed: int = 1, snr_range=None) -> Workload:
+    """Two task boxcar regressors (20 random-onset 10 s blocks each) + Legendre
+    0-3 baseline (p=6), ARMA a~U(0.1,0.8), b~U(-0.5,0.3), betas N(0,1).
+    ``snr_range=(lo, hi)``: per-voxel task amplitude log-uniform in that range."""
+    rng = np.random.default_rng(seed)
+    hrf = double_gamma_hrf(tr)
+    dur = T * tr
+    cols = []
+    for _ in range(2):
+        on = np.sort(rng.uniform(0, dur - 10.0, 20))
+        cols.append(boxcar_regressor(on, 10.0, T, tr, hrf))
+    X = np.concatenate([np.stack(cols, 1), legendre_baseline(T, 3)], 1)
+    a = rng.uniform(0.1, 0.8, V)
+    b = rng.uniform(-0.5, 0.3, V)
+    beta = rng.standard_normal((V, X.shape[1]))
+    if snr_range is not None:
+        amp = np.exp(rng.uniform(np.log(snr_range[0]), np.log(snr_range[1]), V))
+        beta[:, :2] *= amp[:, None] / np.maximum(np.abs(beta[:, :2]).max(1, keepdims=True), 1e-6)
+    noise = arma11_noise(rng, a, b, T)
+    Y = (beta @ X.T + noise).astype(np.float32)
+    ag, bg = config_grid()
+    return Workload("plain_T200_p6", Y, X, tr, None, None, a, b, beta, ag, bg)
+
+
+def gpu_config(V: int = 200_000, T: int = 400, tr: float = 2.0, seed: int = 2, censor_frac: float = 0.05) -> Workload:
+    """p=12: 4 HRF-convolved boxcar task regressors (20 random 10 s blocks each),
+    6 AR(1) random-walk motion-like columns, constant + linear; 5% of the TRs
+    censored uniformly at random (the retained series keeps its true time
+    index); ARMA(1,1) noise as in the plain config."""
+    rng = np.random.default_rng(seed)
+    hrf = double_gamma_hrf(tr)
+    cols = []
+    for _ in range(4):
+        on = np.sort(rng.uniform(0, T * tr - 10.0, 20))
+        cols.append(boxcar_regressor(on, 10.0, T, tr, hrf))
+    motion = np.zeros((T, 6))
+    for j in range(6):
+        m = 0.0
+        for t in range(T):
+            m = 0.9 * m + rng.standard_normal() * 0.1
+            motion[t, j] = m
+    motion -= motion.mean(0)
+    lin = np.linspace(-1.0, 1.0, T)
+    Xf = np.concatenate([np.stack(cols, 1), motion, np.ones((T, 1)), lin[:, None]], 1)
+    n_cens = int(round(censor_frac * T))
+    censored = np.sort(rng.choice(T, n_cens, replace=False)) if n_cens else np.zeros(0, int)
+    keep = np.setdiff1d(np.arange(T), censored)
+    a = rng.uniform(0.1, 0.8, V)
+    b = rng.uniform(-0.5, 0.3, V)
+    beta = rng.standard_normal((V, Xf.shape[1]))
+    noise = arma11_noise(rng, a, b, T)
+    Yf = (beta @ Xf.T + noise).astype(np.float32)
+    ag, bg = config_grid()
+    return Workload(f"censored_T{T}_p12", np.ascontiguousarray(Yf[:, keep]), Xf[keep], tr, [0],
+                    keep.astype(np.int64), a, b, beta, ag, bg)

@@ -1,0 +1,121 @@
+/*
+ * ffs_reml -- C ABI of the MI355X ARMA(1,1) REML engine.
+ *
+ * Every entry point returns 0 on success and a negative status on failure;
+ * ffs_reml_last_error() returns the calling thread's last error message.
+ * Device pointers are caller-owned (e.g. torch tensors on the current HIP
+ * device); the library allocates only the workspace that a plan owns and
+ * frees it in ffs_reml_plan_destroy.  All work is enqueued on the given
+ * hipStream_t (passed as void*); nothing synchronises except
+ * ffs_reml_plan_create (which returns only once the tables are built) and
+ * ffs_reml_plan_logdets (a host copy).
+ *
+ * Reference interfaces replaced (paths relative to fastfuncstuff/):
+ *   ffs_reml_plan_create   precompute_reml_grid           glm/arma.py:1371-1531
+ *                          (+ precompute_autocorr_grid    glm/arma.py:1306-1368,
+ *                             arma11_whiteners            glm/whiten.py:197-271)
+ *   ffs_reml_prep          std normalisation              glm/arma.py:3947-3954
+ *                          _residualize_reml_search_data  glm/arma.py:1554-1571
+ *   ffs_reml_energy        ||Y_w||^2 of search_voxels_precomputed_grid  glm/arma.py:1983-1995
+ *   ffs_reml_project       ||Q'Y_w||^2 + score of the same loop         glm/arma.py:1986-2004
+ *   ffs_reml_select        strict-< update (glm/arma.py:2006-2010) and
+ *                          select_arma_params_hierarchical_from_surface glm/arma.py:2224-2322
+ *   ffs_reml_gls           per-(a,b)-group GLS refit      glm/arma.py:4784-5601
+ *   ffs_reml_ljung_box     _ljung_box_batched             glm/arma.py:6059-6141
+ */
+#ifndef FFS_REML_H
+#define FFS_REML_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ffs_reml_plan ffs_reml_plan;
+
+/* Library version string. */
+const char* ffs_reml_version(void);
+
+/* Last error message of the calling thread ("" when none). */
+const char* ffs_reml_last_error(void);
+
+/*
+ * Build the per-pair FP64 tables on the current device.
+ *   T, p        retained time points and design columns (p <= 96)
+ *   X           host, T x p row-major float64 design
+ *   pinvT       host, T x p row-major float64 transpose of the design pseudo-inverse
+ *   gap         host, T int32: tau[t+1]-tau[t] inside a run, 0 at the last point of a run
+ *   K, ab       host, K x 2 float64 (a, b) pairs in search order
+ *   white       host, K int32: 1 where lambda == 0 (R is exactly the identity)
+ *   n_a, n_b, lut   host lattice of the hierarchical selector: lut[ai*n_b+bi] = pair or -1
+ *   zero_a, zero_b  lattice coordinates of (0, 0)
+ *   ltop        top level of the hierarchical descent
+ *   n_task, task_idx  host; F statistic over these columns (n_task = 0: all columns, F = b'Ab)
+ *   stream      hipStream_t
+ * Returns -2 on a rank-deficient whitened design (the reference's QR / inv
+ * fallbacks are not provided).
+ */
+int ffs_reml_plan_create(ffs_reml_plan** out, int32_t T, int32_t p, const double* X, const double* pinvT,
+                         const int32_t* gap, int32_t K, const double* ab, const int32_t* white, int32_t n_a,
+                         int32_t n_b, const int32_t* lut, int32_t zero_a, int32_t zero_b, int32_t ltop,
+                         int32_t n_task, const int32_t* task_idx, void* stream);
+
+int ffs_reml_plan_destroy(ffs_reml_plan* plan);
+
+/* Host copies of logdet R and logdet X'R^-1X per pair (synchronous). */
+int ffs_reml_plan_logdets(const ffs_reml_plan* plan, double* logdet_R, double* logdet_XRX);
+
+/* Padded sizes the search buffers must have: Tpad rows of Vpad floats for the
+ * time-major residual buffer (rows >= T must be zero), and the voxel padding unit. */
+int ffs_reml_plan_dims(const ffs_reml_plan* plan, int32_t* Tpad, int32_t* vox_pad, int32_t* K);
+
+/*
+ * Normalise + residualise Y (V x T float32, row stride ldY) into the
+ * time-major buffer Yt (Tpad x ldYt float32, ldYt >= Vpad, rows >= T left
+ * untouched), scale[V] = unbiased std clamped at 1e-8 (1 when normalize == 0).
+ * feats (optional, V x 10) receives lag-1..10 autocorrelations of the residual.
+ */
+int ffs_reml_prep(const ffs_reml_plan* plan, const float* Y, int64_t ldY, int64_t V, float* Yt, int64_t ldYt,
+                  float* scale, int32_t normalize, float* feats, void* stream);
+
+/* (i) E[v*ldE + k] = ||L_k^-1 y_v||^2 for every voxel and pair. */
+int ffs_reml_energy(const ffs_reml_plan* plan, const float* Yt, int64_t ldYt, int64_t V, float* E, int64_t ldE,
+                    void* stream);
+
+/* (ii) S[v*ldS + k]: in = energy from ffs_reml_energy, out = REML score
+ * logdet R + logdet X'R^-1X + (T-p) log(E - ||G_k' y||^2 + 1e-10). */
+int ffs_reml_project(const ffs_reml_plan* plan, const float* Yt, int64_t ldYt, int64_t V, float* S, int64_t ldS,
+                     void* stream);
+
+/* (iii) mode 0: global argmin (first minimum); mode 1: hierarchical descent. */
+int ffs_reml_select(const ffs_reml_plan* plan, const float* S, int64_t ldS, int64_t V, int32_t mode, int32_t* best,
+                    float* best_score, void* stream);
+
+/*
+ * (iv) FP64 GLS refit of Y (V x T float32, original units) at pair best[v].
+ * scale (nullable) = per-voxel std used to normalise before the fit.
+ * Outputs: betas, tstats (V x p), fstats, sigma2, r2 (V).  Optional (nullable,
+ * row stride ldR): resid_w (float64 whitened residuals), resid (float32
+ * Y - X beta) and pred (float32 X beta), all in original units.
+ * work: caller-provided int32 workspace of V + K + 1 elements (voxel sort;
+ * nullable = no sort).
+ */
+int ffs_reml_gls(const ffs_reml_plan* plan, const float* Y, int64_t ldY, const float* scale, const int32_t* best,
+                 int64_t V, float* betas, float* tstats, float* fstats, float* sigma2, float* r2, double* resid_w,
+                 int64_t ldR, float* resid, float* pred, int32_t* work, void* stream);
+
+/*
+ * Ljung-Box of whitened residuals (V x ldR float64).  Lag pairs are pre-binned
+ * by time lag: pairs[2q], pairs[2q+1] for q in [bin_off[d], bin_off[d+1]) have
+ * lag d (1 <= d <= nbins); wbin[d] = 1/n_d for usable bins, else 0.  All three
+ * arrays are device pointers.
+ */
+int ffs_reml_ljung_box(const double* resid_w, int64_t ldR, int64_t V, int32_t n, int32_t nbins,
+                       const int32_t* bin_off, const int32_t* pairs, const double* wbin, float* out, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FFS_REML_H */

@@ -1,0 +1,300 @@
+"""Pin the CPU oracle (oracle/reml_oracle.py) before it is used as the checker.
+
+Every check below is independent of the oracle's own formulas: known answers
+from the reference's tests, closed forms (AR(1) inverse / determinant, MA(1)
+eigenvalues), exact rational enumeration, textbook GLS via an explicit
+R^-1 solve, and brute force.
+"""
+
+from fractions import Fraction
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import reml_oracle as O
+
+F64 = torch.float64
+
+
+# --- lambda / grids (tests/test_arma_extended.py:42-57, test_arma_glm_comprehensive.py:29-45)
+
+def test_lambda_known_answers():
+    assert abs(O.arma_lambda(0.5, 0.0) - 0.5) < 1e-12
+    assert O.arma_lambda(0.0, 0.0) == 0.0
+    assert abs(O.arma_lambda(0.5, 0.2) - 0.77 / 1.24) < 1e-12
+    assert abs(O.arma_lambda(0.6, 0.2) - 0.8 * 1.12 / (1 + 0.04 + 0.24)) < 1e-12
+
+
+def _exact_counts(a_vals, b_vals):
+    n_valid = n_white = 0
+    for a in a_vals:
+        for b in b_vals:
+            if abs(a) >= 1 or abs(b) >= 1:
+                continue
+            num = (a + b) * (1 + a * b)
+            if num >= 0:
+                n_valid += 1
+                n_white += num == 0
+    return n_valid, n_white
+
+
+def test_pair_counts_match_exact_enumeration():
+    # 33x33 config grid: exact rationals -0.8 + k/20
+    vals = [Fraction(-4, 5) + Fraction(k, 20) for k in range(33)]
+    nv, nw = _exact_counts(vals, vals)
+    assert (nv, nw) == (561, 33)
+    g = torch.linspace(-0.8, 0.8, 33)
+    a, b = O.ensure_zero_in_grid(g, g)
+    pairs = O.valid_pairs(a, b)
+    assert pairs.shape[0] == nv
+    lam = O.arma_lambda(pairs[:, 0], pairs[:, 1])
+    assert int((lam == 0).sum()) == nw
+    # default AFNI grid 9 x 17
+    av = [Fraction(k, 10) for k in range(9)]
+    bv = [Fraction(k - 8, 10) for k in range(17)]
+    nv2, _ = _exact_counts(av, bv)
+    da, db = O.ensure_zero_in_grid(*O.default_grids())
+    assert O.valid_pairs(da, db).shape[0] == nv2
+
+
+def test_pair_order_is_a_major():
+    g = torch.linspace(-0.8, 0.8, 33)
+    pairs = O.valid_pairs(*O.ensure_zero_in_grid(g, g))
+    a = pairs[:, 0].numpy()
+    assert np.all(np.diff(a) >= 0)
+    for av in np.unique(a):
+        assert np.all(np.diff(pairs[pairs[:, 0] == av, 1].numpy()) > 0)
+
+
+def test_ensure_zero_in_grid_cases():
+    a, b = O.ensure_zero_in_grid(torch.tensor([0.1, 0.2]), torch.tensor([0.1, 0.2]))
+    assert 0.0 in a and 0.0 in b and len(a) == 3 and bool(torch.all(a[1:] >= a[:-1]))
+    a0, b0 = torch.tensor([0.0, 0.1]), torch.tensor([-0.1, 0.0, 0.1])
+    a, b = O.ensure_zero_in_grid(a0, b0)
+    assert torch.equal(a, a0) and torch.equal(b, b0)
+    g = torch.linspace(-0.8, 0.8, 33)
+    a, _ = O.ensure_zero_in_grid(g, g)
+    assert (a == 0).sum() == 1 and len(a) == 33
+
+
+# --- censoring / covariance (tests/test_arma_censoring.py, test_arma_core.py:144-222)
+
+def test_build_censor_run_info_example():
+    starts, tau = O.build_censor_run_info([0, 5], 10, [0, 1, 3, 4, 5, 6, 8, 9])
+    assert starts == [0, 4]
+    assert tau.tolist() == [0, 1, 3, 4, 0, 1, 3, 4]
+    s2, t2 = O.build_censor_run_info([0, 100], 200, None)
+    assert s2 == [0, 100] and t2 is None
+
+
+def test_ar1_covariance_closed_form():
+    a, n = 0.6, 12
+    R = O.covariance(a, 0.0, n, dtype=F64)
+    i = torch.arange(n)
+    ref = torch.tensor(a, dtype=F64) ** (i[:, None] - i[None, :]).abs()
+    torch.testing.assert_close(R, ref, rtol=0, atol=1e-15)
+
+
+def test_ma1_covariance_tridiagonal():
+    b, n = 0.4, 9
+    R = O.covariance(0.0, b, n, dtype=F64)
+    rho = b / (1 + b * b)
+    ref = torch.from_numpy(np.eye(n) + rho * (np.eye(n, k=1) + np.eye(n, k=-1)))
+    torch.testing.assert_close(R, ref, rtol=0, atol=1e-15)
+
+
+def test_censor_gap_and_run_blocks():
+    a, b = 0.7, 0.1
+    lam = (a + b) * (1 + a * b) / (1 + 2 * a * b + b * b)
+    starts, tau = O.build_censor_run_info([0, 5], 10, [0, 1, 3, 4, 5, 6, 8, 9])
+    R = O.covariance(a, b, 8, starts, tau, F64)
+    assert torch.count_nonzero(R[:4, 4:]) == 0
+    assert abs(float(R[1, 2]) - lam * a) < 1e-12
+    assert abs(float(R[0, 1]) - lam) < 1e-12
+    assert torch.allclose(R, R.T)
+    assert float(torch.linalg.eigvalsh(R).min()) > 0
+
+
+# --- whitener and REML score vs closed forms
+
+def test_ar1_whitener_closed_form():
+    a, n = 0.55, 30
+    Linv, logdet = O.whitener(a, 0.0, n, dtype=F64)
+    ref = torch.zeros(n, n, dtype=F64)
+    ref[0, 0] = 1.0
+    s = 1.0 / np.sqrt(1 - a * a)
+    for t in range(1, n):
+        ref[t, t] = s
+        ref[t, t - 1] = -a * s
+    torch.testing.assert_close(Linv, ref, rtol=0, atol=1e-12)
+    assert abs(float(logdet) - (n - 1) * np.log(1 - a * a)) < 1e-8
+
+
+def test_ma1_logdet_eigen_closed_form():
+    b, n = -0.45, 40
+    _, logdet = O.whitener(0.0, b, n, dtype=F64)
+    rho = b / (1 + b * b)
+    k = np.arange(1, n + 1)
+    ref = np.sum(np.log(1 + 2 * rho * np.cos(k * np.pi / (n + 1))))
+    assert abs(float(logdet) - ref) < 1e-8
+
+
+def _ar1_precision(a, n):
+    """AR(1) correlation inverse: tridiagonal closed form."""
+    Q = np.zeros((n, n))
+    for t in range(n):
+        Q[t, t] = 1 + a * a if 0 < t < n - 1 else 1.0
+        if t + 1 < n:
+            Q[t, t + 1] = Q[t + 1, t] = -a
+    return Q / (1 - a * a)
+
+
+def test_reml_score_ar1_closed_form():
+    rng = np.random.default_rng(3)
+    n, p = 50, 3
+    X = np.column_stack([np.ones(n), np.linspace(-1, 1, n), rng.standard_normal(n)])
+    Y = rng.standard_normal((6, n))
+    a_grid = torch.tensor([0.0, 0.55], dtype=torch.float64)
+    b_grid = torch.tensor([0.0], dtype=torch.float64)
+    grid = O.precompute_grid(torch.from_numpy(X), a_grid, b_grid, dtype=F64)
+    k = [i for i, (a, b) in enumerate(grid.param_list) if a == 0.55][0]
+    S = O.score_surface(torch.from_numpy(X), torch.from_numpy(Y), grid)
+    a = 0.55
+    Qi = _ar1_precision(a, n)
+    ldR = (n - 1) * np.log(1 - a * a)
+    A = X.T @ Qi @ X
+    for v in range(Y.shape[0]):
+        y = Y[v]
+        beta = np.linalg.solve(A, X.T @ Qi @ y)
+        r = y - X @ beta
+        rss = r @ Qi @ r
+        ref = ldR + np.linalg.slogdet(A)[1] + (n - p) * np.log(rss + 1e-10)
+        assert abs(float(S[v, k]) - ref) < 1e-7
+
+
+def test_gls_matches_explicit_solve_and_ols_at_zero():
+    rng = np.random.default_rng(7)
+    n, p = 40, 4
+    X = np.column_stack([np.ones(n), rng.standard_normal((n, p - 1))])
+    Y = rng.standard_normal((5, n))
+    g = torch.tensor([0.0, 0.3, 0.6], dtype=F64)
+    gb = torch.tensor([-0.2, 0.0, 0.4], dtype=F64)
+    grid = O.precompute_grid(torch.from_numpy(X), g, gb, dtype=F64)
+    for k, (a, b) in enumerate(grid.param_list):
+        idx = torch.full((Y.shape[0],), k, dtype=torch.long)
+        betas, *_ = O.gls_refit(torch.from_numpy(X), torch.from_numpy(Y), grid, idx, dtype=F64)
+        R = O.covariance(a, b, n, dtype=F64).numpy()
+        Ri = np.linalg.inv(R)
+        ref = np.linalg.solve(X.T @ Ri @ X, X.T @ Ri @ Y.T).T
+        np.testing.assert_allclose(betas.numpy(), ref, rtol=1e-10, atol=1e-12)
+        if a == 0.0 and b == 0.0:
+            ols = np.linalg.lstsq(X, Y.T, rcond=None)[0].T
+            np.testing.assert_allclose(betas.numpy(), ols, rtol=1e-10, atol=1e-12)
+
+
+def test_better_model_scores_lower():
+    rng = np.random.default_rng(11)
+    n, V = 120, 40
+    X = np.ones((n, 1))
+    e = np.zeros((V, n))
+    for t in range(n):
+        e[:, t] = (0.6 * e[:, t - 1] if t else 0) + rng.standard_normal(V)
+    grid = O.precompute_grid(torch.from_numpy(X), torch.tensor([0.0, 0.6], dtype=F64),
+                             torch.tensor([0.0], dtype=F64), dtype=F64)
+    S = O.score_surface(torch.from_numpy(X), torch.from_numpy(e), grid)
+    k0 = grid.param_list.index((0.0, 0.0))
+    k6 = [i for i, (a, _) in enumerate(grid.param_list) if abs(a - 0.6) < 1e-12][0]
+    assert float((S[:, k6] < S[:, k0]).double().mean()) > 0.95
+
+
+# --- selectors
+
+def _bowl_surface(params, centers):
+    a = params[:, 0].double()
+    b = params[:, 1].double()
+    out = []
+    for ca, cb in centers:
+        out.append(((a - ca) ** 2 + 0.5 * (b - cb) ** 2) * 100.0)
+    return torch.stack(out).to(torch.float32)
+
+
+def test_hierarchical_finds_global_min_of_convex_bowl():
+    g = torch.linspace(-0.8, 0.8, 33)
+    params = O.valid_pairs(*O.ensure_zero_in_grid(g, g))
+    rng = np.random.default_rng(0)
+    centers = [(float(x), float(y)) for x, y in zip(rng.uniform(-0.2, 0.7, 50), rng.uniform(0.0, 0.7, 50))]
+    S = _bowl_surface(params, centers)
+    k_ex, _ = O.select_exhaustive(S)
+    k_h, _ = O.select_hierarchical(S, params)
+    assert float((k_ex == k_h).double().mean()) >= 0.9
+
+
+def test_exhaustive_first_minimum_wins_ties():
+    S = torch.tensor([[3.0, 1.0, 1.0, 2.0], [float("nan"), 5.0, 5.0, float("nan")]])
+    k, s = O.select_exhaustive(S)
+    assert k.tolist() == [1, 1] and s.tolist() == [1.0, 5.0]
+
+
+def test_hierarchical_vs_subset_descent_bound():
+    """tests/test_arma_extended.py:350-374: the surface descent and the
+    subset-GEMM descent disagree on <= 0.2% of voxels."""
+    torch.manual_seed(0)
+    T, Kc, V = 160, 10, 1000
+    X = torch.randn(T, Kc, dtype=F64)
+    X[:, 0] = 1.0
+    e = torch.randn(V, T, dtype=F64)
+    rho = torch.rand(V, 1, dtype=F64) * 0.8
+    for t in range(1, T):
+        e[:, t:t + 1] = rho * e[:, t - 1:t] + e[:, t:t + 1]
+    Y = (torch.randn(V, Kc, dtype=F64) @ X.T + e).float()
+    grid = O.precompute_grid(X.float(), torch.linspace(0, 0.8, 9), torch.linspace(-0.8, 0.8, 17),
+                             dtype=torch.float32)
+    S = O.score_surface(X.float(), Y, grid)
+    k_surf, s_surf = O.select_hierarchical(S, grid.params)
+    k_sub, s_sub = O.search_hierarchical_subset(S, grid.params)
+    assert int((k_surf != k_sub).sum()) <= V // 500
+    assert torch.all(s_surf <= s_sub + 1e-2)
+
+
+# --- Ljung-Box (tests/test_arma_public_api.py:54-95)
+
+def test_ljung_box_zero_and_ar():
+    n = 200
+    assert float(O.ljung_box(torch.zeros(3, n), 10).abs().max()) == 0.0
+    rng = np.random.default_rng(5)
+    e = np.zeros((4, n))
+    for t in range(n):
+        e[:, t] = (0.7 * e[:, t - 1] if t else 0) + rng.standard_normal(4)
+    lb_ar = O.ljung_box(torch.from_numpy(e), 10)
+    lb_w = O.ljung_box(torch.from_numpy(rng.standard_normal((4, n))), 10)
+    assert float(lb_ar.min()) > 100 and float(lb_w.max()) < 40
+
+
+def test_ljung_box_matches_textbook_uncensored():
+    rng = np.random.default_rng(9)
+    n, h = 80, 12
+    r = rng.standard_normal((3, n))
+    got = O.ljung_box(torch.from_numpy(r), h).numpy()
+    for v in range(3):
+        s0 = np.sum(r[v] ** 2)
+        q = sum((np.sum(r[v, :-k] * r[v, k:]) / s0) ** 2 / (n - k) for k in range(1, h + 1))
+        assert abs(got[v] - n * (n + 2) * q) < 1e-9 * max(1.0, got[v])
+
+
+def test_float32_oracle_selects_float64_parameters():
+    """tests/test_reml_conditioning.py:23-38 in spirit: the float32 restatement
+    (normalised + residualised) selects the FP64 oracle's (a, b)."""
+    rng = np.random.default_rng(21)
+    T, V = 90, 60
+    X = np.column_stack([np.ones(T), np.linspace(-1, 1, T), rng.standard_normal(T)])
+    e = np.zeros((V, T))
+    for t in range(T):
+        e[:, t] = (0.5 * e[:, t - 1] if t else 0) + rng.standard_normal(V)
+    Y = 1000.0 + rng.standard_normal((V, 3)) @ X.T + e
+    a_g, b_g = torch.linspace(0, 0.8, 9), torch.linspace(-0.8, 0.8, 17)
+    f64 = O.fit_arma11(torch.from_numpy(Y), torch.from_numpy(X), a_g, b_g, dtype=F64, selector="exhaustive")
+    f32 = O.fit_arma11(torch.from_numpy(Y).float(), torch.from_numpy(X).float(), a_g, b_g, dtype=torch.float32,
+                       selector="exhaustive")
+    assert float((f64.pair_index == f32.pair_index).double().mean()) >= 0.95
+    np.testing.assert_allclose(f32.sigma2.double().numpy(), f64.sigma2.numpy(), rtol=2e-3)
