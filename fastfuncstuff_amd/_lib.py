@@ -28,14 +28,26 @@ SIGNATURES: dict[str, list] = {
     "ffs_reml_plan_destroy": [_vp],
     "ffs_reml_plan_logdets": [_vp, _vp, _vp],
     "ffs_reml_plan_dims": [_vp, _vp, _vp, _vp],
-    "ffs_reml_prep": [_vp, _vp, _i64, _i64, _vp, _i64, _vp, _c_int, _vp, _vp],
+    "ffs_reml_prep": [_vp, _vp, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _c_int, _vp, _vp],
     "ffs_reml_energy": [_vp, _vp, _i64, _i64, _vp, _i64, _vp],
     "ffs_reml_project": [_vp, _vp, _i64, _i64, _vp, _i64, _vp],
     "ffs_reml_select": [_vp, _vp, _i64, _i64, _c_int, _vp, _vp, _vp],
     "ffs_reml_gls": [_vp, _vp, _i64, _vp, _vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _vp, _vp, _vp, _vp],
     "ffs_reml_ljung_box": [_vp, _i64, _i64, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp],
+    "ffs_reml_local_workspace": [_vp, _i64, _vp],
+    "ffs_reml_local_search": [_vp, _vp, _i64, _i64, _vp, ctypes.c_float, _vp, _vp, _vp, _vp, _vp, _i64, _vp],
+    "ffs_reml_compact": [_vp, _i64, _vp, _vp, _vp],
+    "ffs_reml_scatter_best": [_vp, _i64, _vp, _vp, _vp, _vp, _vp],
+    "ffs_mlp_param_count": [],
+    "ffs_mlp_workspace_floats": [_i64],
+    "ffs_mlp_last_error": [],
+    "ffs_mlp_forward": [_vp, _vp, _vp, _i64, _vp, _vp],
+    "ffs_mlp_grad": [_vp, _vp, _vp, _vp, _i64, _vp, _vp, _vp, _vp],
+    "ffs_mlp_adam": [_vp, _vp, _vp, _vp, _i64, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                     _c_int, _vp],
 }
-_RESTYPES = {"ffs_reml_version": ctypes.c_char_p, "ffs_reml_last_error": ctypes.c_char_p}
+_RESTYPES = {"ffs_reml_version": ctypes.c_char_p, "ffs_reml_last_error": ctypes.c_char_p,
+             "ffs_mlp_last_error": ctypes.c_char_p}
 
 _lib = None
 

@@ -37,6 +37,7 @@ struct ffs_reml_plan {
     const double* pinvT = nullptr;
     const double* ldR = nullptr;
     const double* ldX = nullptr;
+    bool local_ok = false;
     template <class Tp>
     int alloc(Tp** out, size_t n) {
         void* p = nullptr;
@@ -170,8 +171,34 @@ extern "C" int ffs_reml_plan_create(ffs_reml_plan** out, int32_t T, int32_t p, c
         (rc = P->alloc(&ec, (size_t)d.n_chunks * T)) || (rc = P->alloc(&es0, (size_t)d.n_chunks * kEnergyNB)))
         return rc;
     HIP_TRY(hipMemsetAsync(G32, 0, sizeof(float) * d.n_tiles * d.Tpad * BN, s));
+    // Local-search tables: lattice axes, per-pair fp32 coefficients, pair-padded operand.
+    std::vector<float> a_axis(n_a, 0.f), b_axis(n_b, 0.f);
+    for (int ai = 0; ai < n_a; ++ai)
+        for (int bi = 0; bi < n_b; ++bi) {
+            const int q = lut[ai * n_b + bi];
+            if (q >= 0) {
+                a_axis[ai] = (float)ab[2 * q];
+                b_axis[bi] = (float)ab[2 * q + 1];
+            }
+        }
+    float *da_axis, *db_axis, *lc, *ls0, *Gp = nullptr, *zero1;
+    float2 *lp, *zero2;
+    if ((rc = upload(P, &da_axis, a_axis.data(), a_axis.size(), s)) ||
+        (rc = upload(P, &db_axis, b_axis.data(), b_axis.size(), s)) || (rc = P->alloc(&lp, (size_t)K * T)) ||
+        (rc = P->alloc(&lc, (size_t)K * T)) || (rc = P->alloc(&ls0, K)) || (rc = P->alloc(&zero1, T)) ||
+        (rc = P->alloc(&zero2, T)))
+        return rc;
+    HIP_TRY(hipMemsetAsync(zero1, 0, sizeof(float) * T, s));
+    HIP_TRY(hipMemsetAsync(zero2, 0, sizeof(float2) * T, s));
+    P->local_ok = (p <= kLocalPP);
+    if (P->local_ok) {
+        if ((rc = P->alloc(&Gp, (size_t)d.Tpad * K * kLocalPP))) return rc;
+        HIP_TRY(hipMemsetAsync(Gp, 0, sizeof(float) * d.Tpad * K * kLocalPP, s));
+    }
     hipLaunchKernelGGL(k_tables, dim3(K), dim3(64), 0, s, T, d.Tpad, p, K, dab, dwhiteflag, dgap, dX, d.n_task, dtask,
-                       full_f, BN, npt, soa, coef64, Xw64, G64, Lx, Ainv, Fm, work, ld, ldR, ldX, G32, dstatus);
+                       full_f, BN, npt, soa, coef64, Xw64, G64, Lx, Ainv, Fm, work, ld, ldR, ldX, G32, Gp, dstatus);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(k_local_tables, dim3(K), dim3(128), 0, s, T, K, soa, lp, lc, ls0);
     HIP_TRY(hipGetLastError());
     hipLaunchKernelGGL(k_energy_tables, dim3(d.n_chunks), dim3(128), 0, s, T, d.n_chunks, dchunk, soa, ecoef, ec,
                        es0);
@@ -199,6 +226,15 @@ extern "C" int ffs_reml_plan_create(ffs_reml_plan** out, int32_t T, int32_t p, c
     d.echunk = dchunk;
     d.white = dwhite;
     d.lut = dlut;
+    d.a_axis = da_axis;
+    d.b_axis = db_axis;
+    d.white_flag = dwhiteflag;
+    d.lp = lp;
+    d.lc = lc;
+    d.ls0 = ls0;
+    d.Gp = Gp;
+    d.zero1 = zero1;
+    d.zero2 = zero2;
     P->pinvT = dpinv;
     P->ldR = ldR;
     P->ldX = ldX;
@@ -229,36 +265,53 @@ extern "C" int ffs_reml_plan_dims(const ffs_reml_plan* P, int32_t* Tpad, int32_t
 static int64_t vpad_of(int64_t V) { return (V + kVoxPad - 1) / kVoxPad * kVoxPad; }
 
 template <int PM>
-static void launch_prep(const ffs_reml_plan* P, const float* Y, int64_t ldY, int V, int Vpad, float* Yt,
-                        int64_t ldYt, float* scale, int normalize, float* feats, hipStream_t s) {
-    const dim3 grid((Vpad + 255) / 256), block(256);
+static void launch_prep(const ffs_reml_plan* P, const float* Y, int64_t ldY, int V, const int32_t* rows, int ncols,
+                        int ncols_pad, float* Yt, int64_t ldYt, float* scale, int normalize, float* feats,
+                        hipStream_t s) {
+    const dim3 grid((ncols_pad + 255) / 256), block(256);
     if (feats)
-        hipLaunchKernelGGL((k_prep<PM, 10>), grid, block, 0, s, P->d.T, P->d.p, Y, ldY, V, Vpad, P->pinvT, P->d.X,
-                           normalize, Yt, ldYt, scale, feats);
+        hipLaunchKernelGGL((k_prep<PM, 10>), grid, block, 0, s, P->d.T, P->d.p, Y, ldY, V, rows, ncols, ncols_pad,
+                           P->pinvT, P->d.X, normalize, Yt, ldYt, scale, feats);
     else
-        hipLaunchKernelGGL((k_prep<PM, 0>), grid, block, 0, s, P->d.T, P->d.p, Y, ldY, V, Vpad, P->pinvT, P->d.X,
-                           normalize, Yt, ldYt, scale, feats);
+        hipLaunchKernelGGL((k_prep<PM, 0>), grid, block, 0, s, P->d.T, P->d.p, Y, ldY, V, rows, ncols, ncols_pad,
+                           P->pinvT, P->d.X, normalize, Yt, ldYt, scale, feats);
 }
 
-extern "C" int ffs_reml_prep(const ffs_reml_plan* P, const float* Y, int64_t ldY, int64_t V, float* Yt,
-                             int64_t ldYt, float* scale, int32_t normalize, float* feats, void* stream) {
-    if (!P) return fail(-3, "prep: null plan");
-    if (V <= 0) return 0;
-    const int64_t Vpad = vpad_of(V);
-    if (ldYt < Vpad) return fail(-3, "prep: ldYt must be >= the padded voxel count");
+static int prep_impl(const ffs_reml_plan* P, const float* Y, int64_t ldY, int64_t V, const int32_t* rows,
+                     int64_t ncols, float* Yt, int64_t ldYt, float* scale, int normalize, float* feats,
+                     hipStream_t s) {
+    const int64_t ncp = vpad_of(ncols);
+    if (ldYt < ncp) return fail(-3, "prep: ldYt must be >= the padded column count");
     if (ldY < P->d.T) return fail(-3, "prep: ldY must be >= T");
-    hipStream_t s = (hipStream_t)stream;
+    if (ncp > 0x7fffffff || V > 0x7fffffff) return fail(-3, "prep: batch too large; split it");
     const int p = P->d.p;
+    const int v = (int)V, nc = (int)ncols, ncpi = (int)ncp;
     if (p <= 8)
-        launch_prep<8>(P, Y, ldY, (int)V, (int)Vpad, Yt, ldYt, scale, normalize, feats, s);
+        launch_prep<8>(P, Y, ldY, v, rows, nc, ncpi, Yt, ldYt, scale, normalize, feats, s);
     else if (p <= 16)
-        launch_prep<16>(P, Y, ldY, (int)V, (int)Vpad, Yt, ldYt, scale, normalize, feats, s);
+        launch_prep<16>(P, Y, ldY, v, rows, nc, ncpi, Yt, ldYt, scale, normalize, feats, s);
     else if (p <= 32)
-        launch_prep<32>(P, Y, ldY, (int)V, (int)Vpad, Yt, ldYt, scale, normalize, feats, s);
+        launch_prep<32>(P, Y, ldY, v, rows, nc, ncpi, Yt, ldYt, scale, normalize, feats, s);
     else
-        launch_prep<96>(P, Y, ldY, (int)V, (int)Vpad, Yt, ldYt, scale, normalize, feats, s);
+        launch_prep<96>(P, Y, ldY, v, rows, nc, ncpi, Yt, ldYt, scale, normalize, feats, s);
     HIP_TRY(hipGetLastError());
+    // rows T..Tpad-1 of the time-major buffer are zero (GEMM k padding)
+    if (P->d.Tpad > P->d.T) {
+        const int64_t n = (int64_t)(P->d.Tpad - P->d.T) * ncp;
+        hipLaunchKernelGGL(k_zero_rows, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, Yt, ldYt, P->d.T,
+                           P->d.Tpad, ncp);
+        HIP_TRY(hipGetLastError());
+    }
     return 0;
+}
+
+extern "C" int ffs_reml_prep(const ffs_reml_plan* P, const float* Y, int64_t ldY, int64_t V, const int32_t* rows,
+                             int64_t ncols, float* Yt, int64_t ldYt, float* scale, int32_t normalize, float* feats,
+                             void* stream) {
+    if (!P) return fail(-3, "prep: null plan");
+    if (!rows) ncols = V;
+    if (ncols <= 0) return 0;
+    return prep_impl(P, Y, ldY, V, rows, ncols, Yt, ldYt, scale, normalize, feats, (hipStream_t)stream);
 }
 
 extern "C" int ffs_reml_energy(const ffs_reml_plan* P, const float* Yt, int64_t ldYt, int64_t V, float* E,
@@ -328,7 +381,7 @@ extern "C" int ffs_reml_gls(const ffs_reml_plan* P, const float* Y, int64_t ldY,
     if (work) {
         int32_t* cnt = work + V;
         perm = work;
-        HIP_TRY(hipMemsetAsync(cnt, 0, sizeof(int32_t) * (K + 1), s));
+        hipLaunchKernelGGL(k_fill_i32, dim3((K + 1 + 255) / 256), dim3(256), 0, s, cnt, (int64_t)(K + 1), 0);
         const unsigned g = (unsigned)((V + 255) / 256);
         hipLaunchKernelGGL(k_hist, dim3(g), dim3(256), 0, s, best, (int)V, K, cnt);
         hipLaunchKernelGGL(k_scan, dim3(1), dim3(1), 0, s, cnt, K + 1);
@@ -356,6 +409,111 @@ extern "C" int ffs_reml_ljung_box(const double* resid_w, int64_t ldR, int64_t V,
     hipStream_t s = (hipStream_t)stream;
     hipLaunchKernelGGL(k_ljung_box, dim3((unsigned)((V + 3) / 4)), dim3(256), 0, s, resid_w, ldR, (int)V, n, nbins,
                        bin_off, pairs, wbin, out);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Learned-proposal local search
+// ---------------------------------------------------------------------------
+namespace {
+struct LocalLayout {
+    int64_t ng, max_cols, max_tiles;
+    size_t off_center, off_cnt, off_fill, off_tg, off_tc, off_nt, off_rows, off_yt, off_e, total;
+};
+
+size_t align256(size_t x) { return (x + 255) / 256 * 256; }
+
+LocalLayout local_layout(const ffs_reml_plan* P, int64_t V) {
+    LocalLayout L{};
+    L.ng = (int64_t)P->d.n_a * P->d.n_b;
+    const int64_t groups = std::min<int64_t>(V, L.ng);
+    L.max_cols = vpad_of(V + (kLocalTile - 1) * groups);
+    L.max_tiles = L.max_cols / kLocalTile;
+    size_t o = 0;
+    L.off_center = o;
+    o = align256(o + 4 * (size_t)V);
+    L.off_cnt = o;
+    o = align256(o + 4 * (size_t)(L.ng + 1));
+    L.off_fill = o;
+    o = align256(o + 4 * (size_t)L.ng);
+    L.off_tg = o;
+    o = align256(o + 4 * (size_t)L.max_tiles);
+    L.off_tc = o;
+    o = align256(o + 4 * (size_t)L.max_tiles);
+    L.off_nt = o;
+    o = align256(o + 4);
+    L.off_rows = o;
+    o = align256(o + 4 * (size_t)L.max_cols);
+    L.off_yt = o;
+    o = align256(o + 4 * (size_t)P->d.Tpad * L.max_cols);
+    L.off_e = o;
+    o = align256(o + 4 * (size_t)L.max_cols * kSlots);
+    L.total = o;
+    return L;
+}
+}  // namespace
+
+extern "C" int ffs_reml_local_workspace(const ffs_reml_plan* P, int64_t V, int64_t* bytes) {
+    if (!P || !bytes) return fail(-3, "local_workspace: null argument");
+    *bytes = (int64_t)local_layout(P, V).total;
+    return 0;
+}
+
+extern "C" int ffs_reml_local_search(const ffs_reml_plan* P, const float* Y, int64_t ldY, int64_t V,
+                                     const float* props, float conf_min, int32_t* best, float* best_score,
+                                     int32_t* need_full, float* scale, void* workspace, int64_t ws_bytes,
+                                     void* stream) {
+    if (!P) return fail(-3, "local_search: null plan");
+    if (!P->local_ok) return fail(-4, "local_search: needs p <= 16 (use the full grid)");
+    if (V <= 0) return 0;
+    const LocalLayout L = local_layout(P, V);
+    if (!workspace || ws_bytes < (int64_t)L.total) return fail(-3, "local_search: workspace too small");
+    if (L.max_cols > 0x7fffffff) return fail(-3, "local_search: batch too large; split it");
+    hipStream_t s = (hipStream_t)stream;
+    char* ws = static_cast<char*>(workspace);
+    int32_t* center = reinterpret_cast<int32_t*>(ws + L.off_center);
+    int32_t* cnt = reinterpret_cast<int32_t*>(ws + L.off_cnt);
+    int32_t* fill = reinterpret_cast<int32_t*>(ws + L.off_fill);
+    int32_t* tg = reinterpret_cast<int32_t*>(ws + L.off_tg);
+    int32_t* tc = reinterpret_cast<int32_t*>(ws + L.off_tc);
+    int32_t* nt = reinterpret_cast<int32_t*>(ws + L.off_nt);
+    int32_t* rows = reinterpret_cast<int32_t*>(ws + L.off_rows);
+    float* Ytg = reinterpret_cast<float*>(ws + L.off_yt);
+    float* Eloc = reinterpret_cast<float*>(ws + L.off_e);
+    const unsigned gv = (unsigned)((V + 255) / 256);
+    hipLaunchKernelGGL(k_local_center, dim3(gv), dim3(256), 0, s, P->d, props, (int)V, conf_min, center, need_full);
+    hipLaunchKernelGGL(k_fill_i32, dim3((unsigned)((L.ng + 1 + 255) / 256)), dim3(256), 0, s, cnt, L.ng + 1, 0);
+    hipLaunchKernelGGL(k_hist, dim3(gv), dim3(256), 0, s, center, (int)V, (int)L.ng, cnt);
+    hipLaunchKernelGGL(k_local_layout, dim3(1), dim3(1), 0, s, (int)L.ng, cnt, fill, tg, tc, nt);
+    hipLaunchKernelGGL(k_fill_i32, dim3((unsigned)((L.max_cols + 255) / 256)), dim3(256), 0, s, rows, L.max_cols,
+                       -1);
+    hipLaunchKernelGGL(k_local_scatter, dim3(gv), dim3(256), 0, s, center, (int)V, (int)L.ng, fill, rows);
+    HIP_TRY(hipGetLastError());
+    if (int rc = prep_impl(P, Y, ldY, V, rows, L.max_cols, Ytg, L.max_cols, scale, 1, nullptr, s)) return rc;
+    hipLaunchKernelGGL(k_local_energy, dim3((unsigned)L.max_tiles, 2), dim3(kWin * 64), 0, s, P->d, Ytg,
+                       (int64_t)L.max_cols, tg, tc, nt, Eloc);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(k_local_project, dim3((unsigned)L.max_tiles), dim3(256), 0, s, P->d, Ytg, (int64_t)L.max_cols,
+                       tg, tc, nt, rows, Eloc, best, best_score, need_full);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+extern "C" int ffs_reml_compact(const int32_t* flags, int64_t V, int32_t* idx, int32_t* count, void* stream) {
+    if (V <= 0) return 0;
+    hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, count, (int64_t)1, 0);
+    hipLaunchKernelGGL(k_compact, dim3((unsigned)((V + 255) / 256)), dim3(256), 0, s, flags, (int)V, idx, count);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+extern "C" int ffs_reml_scatter_best(const int32_t* idx, int64_t n, const int32_t* best_sub, const float* score_sub,
+                                     int32_t* best, float* score, void* stream) {
+    if (n <= 0) return 0;
+    hipLaunchKernelGGL(k_scatter_best, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, idx,
+                       (int)n, best_sub, score_sub, best, score);
     HIP_TRY(hipGetLastError());
     return 0;
 }

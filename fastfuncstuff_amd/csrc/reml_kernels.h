@@ -56,6 +56,23 @@ struct PlanDev {
     const int32_t* echunk;     // [n_chunks][NB] pair index (-1 = empty slot)
     const int32_t* white;      // [n_white] lambda==0 pairs
     const int32_t* lut;        // [n_a][n_b] pair index or -1
+    // Local (5 x 5 window) search tables.
+    const float* a_axis;       // [n_a] lattice a values
+    const float* b_axis;       // [n_b] lattice b values
+    const int32_t* white_flag; // [K]
+    const float2* lp;          // [K][T] (alpha_t, 1/v_{t+1})
+    const float* lc;           // [K][T] c_t
+    const float* ls0;          // [K] 1/v_0
+    const float* Gp;           // [Tpad][K * kLocalPP] pair-padded projection operand (p <= kLocalPP)
+    const float2* zero2;       // [T] zeros
+    const float* zero1;        // [T] zeros
 };
+
+// Local search: window half-width, slots, pair column padding, voxel tile.
+constexpr int kWin = 5;
+constexpr int kWinHalf = 2;
+constexpr int kSlots = kWin * kWin;
+constexpr int kLocalPP = 16;
+constexpr int kLocalTile = 128;
 
 }  // namespace remlk

@@ -36,7 +36,7 @@ __global__ void __launch_bounds__(64) k_tables(int T, int Tpad, int p, int K, co
                                                double* __restrict__ Fm, double* __restrict__ work,
                                                double* __restrict__ ld, double* __restrict__ ldR_out,
                                                double* __restrict__ ldX_out, float* __restrict__ G32,
-                                               int* __restrict__ status) {
+                                               float* __restrict__ Gp, int* __restrict__ status) {
     const int k = blockIdx.x;
     if (k >= K) return;
     const int tid = threadIdx.x;
@@ -124,6 +124,12 @@ __global__ void __launch_bounds__(64) k_tables(int T, int Tpad, int p, int K, co
         const int t = idx / p, j = idx % p;
         Gt[(size_t)t * BN + j] = (float)g[idx];
     }
+    if (Gp && p <= kLocalPP) {
+        for (int idx = tid; idx < T * p; idx += 64) {
+            const int t = idx / p, j = idx % p;
+            Gp[(size_t)t * K * kLocalPP + (size_t)k * kLocalPP + j] = (float)g[idx];
+        }
+    }
     if (tid == 0) {
         ld[k] = sh_ldR + sh_ldX;
         ldR_out[k] = sh_ldR;
@@ -164,6 +170,19 @@ __global__ void k_energy_tables(int T, int n_chunks, const int32_t* __restrict__
     }
 }
 
+// Per-pair fp32 coefficients of the local search (same roundings as the chunk tables).
+__global__ void k_local_tables(int T, int K, const double* __restrict__ soa, float2* __restrict__ lp,
+                               float* __restrict__ lc, float* __restrict__ ls0) {
+    const int k = blockIdx.x;
+    if (k >= K) return;
+    const double* s2 = soa + (size_t)k * 3 * T;
+    for (int t = threadIdx.x; t < T; t += blockDim.x) {
+        lp[(size_t)k * T + t] = make_float2((float)s2[2 * T + t], (t + 1 < T) ? (float)s2[t + 1] : 0.f);
+        lc[(size_t)k * T + t] = (float)s2[T + t];
+    }
+    if (threadIdx.x == 0) ls0[k] = (float)s2[0];
+}
+
 // ---------------------------------------------------------------------------
 // Prep: normalise by the unbiased std (glm/arma.py:3947-3954), residualise
 // against the design with its pseudo-inverse (glm/arma.py:1554-1571) and
@@ -172,13 +191,15 @@ __global__ void k_energy_tables(int T, int n_chunks, const int32_t* __restrict__
 // ---------------------------------------------------------------------------
 template <int PM, int NF>
 __global__ void __launch_bounds__(256) k_prep(int T, int p, const float* __restrict__ Y, int64_t ldY, int V,
-                                              int Vpad, const double* __restrict__ pinvT,
-                                              const double* __restrict__ X, int normalize, float* __restrict__ Yt,
-                                              int64_t ldYt, float* __restrict__ scale, float* __restrict__ feats) {
-    const int v = blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= Vpad) return;
-    if (v >= V) {
-        for (int t = 0; t < T; ++t) Yt[(size_t)t * ldYt + v] = 0.f;
+                                              const int32_t* __restrict__ rows, int ncols, int ncols_pad,
+                                              const double* __restrict__ pinvT, const double* __restrict__ X,
+                                              int normalize, float* __restrict__ Yt, int64_t ldYt,
+                                              float* __restrict__ scale, float* __restrict__ feats) {
+    const int col = blockIdx.x * blockDim.x + threadIdx.x;
+    if (col >= ncols_pad) return;
+    const int v = (col < ncols) ? (rows ? rows[col] : col) : -1;
+    if (v < 0 || v >= V) {
+        for (int t = 0; t < T; ++t) Yt[(size_t)t * ldYt + col] = 0.f;
         return;
     }
     const float* y = Y + (size_t)v * ldY;
@@ -219,7 +240,7 @@ __global__ void __launch_bounds__(256) k_prep(int T, int p, const float* __restr
         for (int j = 0; j < PM; ++j)
             if (j < p) r -= xr[j] * c[j];
         const float rf = (float)r;
-        Yt[(size_t)t * ldYt + v] = rf;
+        Yt[(size_t)t * ldYt + col] = rf;
         if (NF > 0) {
             acf0 += (double)rf * rf;
 #pragma unroll
@@ -229,11 +250,31 @@ __global__ void __launch_bounds__(256) k_prep(int T, int p, const float* __restr
             hist[0] = rf;
         }
     }
-    scale[v] = (float)s;
+    if (scale) scale[v] = (float)s;
     if (NF > 0 && feats) {
 #pragma unroll
         for (int k = 0; k < NF; ++k) feats[(size_t)v * NF + k] = acf0 > 0.0 ? (float)(acf[k] / acf0) : 0.f;
     }
+}
+
+// Energy of the lambda == 0 pairs: sum y_t^2 with the accumulation schedule of
+// k_energy (first sample, then groups of 8 steps flushed into fp64).
+__device__ inline float white_energy(const float* __restrict__ y, int64_t ldY, int T) {
+    const float y0 = y[0];
+    float part = y0 * y0;
+    double acc = 0.0;
+    for (int t0 = 0; t0 < T - 1; t0 += 8) {
+        for (int i = 0; i < 8; ++i) {
+            const int t = t0 + i;
+            if (t < T - 1) {
+                const float e = y[(size_t)(t + 1) * ldY];
+                part = fmaf(e * 1.0f, e, part);
+            }
+        }
+        acc += part;
+        part = 0.f;
+    }
+    return (float)(acc + part);
 }
 
 // ---------------------------------------------------------------------------
@@ -256,19 +297,11 @@ __global__ void __launch_bounds__(kEnergyVox* kEnergyWaves) k_energy(PlanDev P, 
 
     if (P.n_white > 0 && chunk == 0) {
         // lambda == 0: R = I, so the energy is sum y^2, shared by every such pair.
-        double acc = 0.0;
-        float part = 0.f;
-        for (int t = 0; t < T; ++t) {
-            const float yt = y[(size_t)t * ldY];
-            part = fmaf(yt, yt, part);
-            if ((t & 15) == 15) {
-                acc += part;
-                part = 0.f;
-            }
-        }
-        acc += part;
+        // Same accumulation schedule as energy_accumulate (bitwise equal to it
+        // with identity coefficients).
+        const float acc = white_energy(y, ldY, T);
         if (v < V)
-            for (int i = 0; i < P.n_white; ++i) E[(size_t)v * ldE + P.white[i]] = (float)acc;
+            for (int i = 0; i < P.n_white; ++i) E[(size_t)v * ldE + P.white[i]] = acc;
         return;
     }
     const float* __restrict__ cz = P.ec + (size_t)chunk * T;
@@ -328,6 +361,42 @@ __global__ void __launch_bounds__(kEnergyVox* kEnergyWaves) k_energy(PlanDev P, 
     }
 }
 
+// Global -> register stage of the projection tiles (A: BK x BM from the
+// time-major residual, B: BK x BN from the tile-major operand).
+template <int BM, int BN, int BK, int A_V4, int B_V4>
+__device__ __forceinline__ void proj_gload(float4 (&ra)[A_V4], float4 (&rb)[B_V4], const float* __restrict__ Ag,
+                                           const float* __restrict__ Bg, int64_t ldY, int kt, int tid) {
+    const int t0 = kt * BK;
+#pragma unroll
+    for (int i = 0; i < A_V4; ++i) {
+        const int idx = tid + i * 256;
+        const int r = idx / (BM / 4), c4 = idx % (BM / 4);
+        ra[i] = *reinterpret_cast<const float4*>(Ag + (size_t)(t0 + r) * ldY + c4 * 4);
+    }
+#pragma unroll
+    for (int i = 0; i < B_V4; ++i) {
+        const int idx = tid + i * 256;
+        if (idx < BK * BN / 4) {
+            const int r = idx / (BN / 4), c4 = idx % (BN / 4);
+            rb[i] = *reinterpret_cast<const float4*>(Bg + (size_t)(t0 + r) * BN + c4 * 4);
+        }
+    }
+}
+
+template <int BM, int BN, int BK, int A_V4, int B_V4>
+__device__ __forceinline__ void proj_sstore(const float4 (&ra)[A_V4], const float4 (&rb)[B_V4], float* stage,
+                                            int tid) {
+    float* As = stage;
+    float* Bs = stage + BK * BM;
+#pragma unroll
+    for (int i = 0; i < A_V4; ++i) *reinterpret_cast<float4*>(As + (tid + i * 256) * 4) = ra[i];
+#pragma unroll
+    for (int i = 0; i < B_V4; ++i) {
+        const int idx = tid + i * 256;
+        if (idx < BK * BN / 4) *reinterpret_cast<float4*>(Bs + idx * 4) = rb[i];
+    }
+}
+
 // ---------------------------------------------------------------------------
 // (ii) Projection GEMM on MFMA (fp32 32x32x2) with the REML score epilogue.
 // Block tile: kProjBM voxels x BN columns (npt pairs of p columns), 4 waves,
@@ -373,45 +442,13 @@ __global__ void __launch_bounds__(256) k_project(PlanDev P, const float* __restr
     constexpr int A_V4 = A_ELEMS / 4 / 256;  // float4 per thread
     constexpr int B_V4 = (B_ELEMS / 4 + 255) / 256;
     float4 ra[A_V4], rb[B_V4];
-    auto gload = [&](int kt) {
-        const int t0 = kt * BK;
-#pragma unroll
-        for (int i = 0; i < A_V4; ++i) {
-            const int idx = tid + i * 256;
-            const int r = idx / (BM / 4), c4 = idx % (BM / 4);
-            ra[i] = *reinterpret_cast<const float4*>(Ag + (size_t)(t0 + r) * ldY + c4 * 4);
-        }
-#pragma unroll
-        for (int i = 0; i < B_V4; ++i) {
-            const int idx = tid + i * 256;
-            if (idx < B_ELEMS / 4) {
-                const int r = idx / (BN / 4), c4 = idx % (BN / 4);
-                rb[i] = *reinterpret_cast<const float4*>(Bg + (size_t)(t0 + r) * BN + c4 * 4);
-            }
-        }
-    };
-    auto sstore = [&](int buf) {
-        float* As = lds + buf * STAGE;
-        float* Bs = As + A_ELEMS;
-#pragma unroll
-        for (int i = 0; i < A_V4; ++i) {
-            const int idx = tid + i * 256;
-            *reinterpret_cast<float4*>(As + idx * 4) = ra[i];
-        }
-#pragma unroll
-        for (int i = 0; i < B_V4; ++i) {
-            const int idx = tid + i * 256;
-            if (idx < B_ELEMS / 4) *reinterpret_cast<float4*>(Bs + idx * 4) = rb[i];
-        }
-    };
-
     const int nk = P.Tpad / BK;
-    gload(0);
-    sstore(0);
+    proj_gload<BM, BN, BK, A_V4, B_V4>(ra, rb, Ag, Bg, ldY, 0, tid);
+    proj_sstore<BM, BN, BK, A_V4, B_V4>(ra, rb, lds, tid);
     __syncthreads();
     for (int kt = 0; kt < nk; ++kt) {
         const int cur = kt & 1;
-        if (kt + 1 < nk) gload(kt + 1);
+        if (kt + 1 < nk) proj_gload<BM, BN, BK, A_V4, B_V4>(ra, rb, Ag, Bg, ldY, kt + 1, tid);
         const float* As = lds + cur * STAGE;
         const float* Bs = As + A_ELEMS;
         const int ar = lane >> 5, ac = lane & 31;
@@ -424,7 +461,7 @@ __global__ void __launch_bounds__(256) k_project(PlanDev P, const float* __restr
                 acc[n] = __builtin_amdgcn_mfma_f32_32x32x2f32(av, bv, acc[n], 0, 0, 0);
             }
         }
-        if (kt + 1 < nk) sstore(cur ^ 1);
+        if (kt + 1 < nk) proj_sstore<BM, BN, BK, A_V4, B_V4>(ra, rb, lds + (cur ^ 1) * STAGE, tid);
         __syncthreads();
     }
     // Scatter the accumulators (32x32 layout: col = lane%32,
@@ -448,10 +485,13 @@ __global__ void __launch_bounds__(256) k_project(PlanDev P, const float* __restr
         if (k >= K || v >= V) continue;
         const float* cr = Cs + row * C_LD + q * p;
         double pp = 0.0;
-        for (int j = 0; j < p; ++j) pp += (double)cr[j] * (double)cr[j];
+        for (int j = 0; j < p; ++j) {
+            const double cv = (double)cr[j];
+            pp = fma(cv, cv, pp);
+        }
         float* sp = S + (size_t)v * ldS + k;
         const double rss = (double)(*sp) - pp;
-        *sp = (float)(P.ld[k] + tp * log(rss + 1e-10));
+        *sp = (float)fma(tp, log(rss + 1e-10), P.ld[k]);
     }
 }
 
@@ -711,6 +751,327 @@ __global__ void __launch_bounds__(256) k_ljung_box(const double* __restrict__ rw
         g *= (double)n * (n + 2.0);
         out[v] = ok ? (float)fmin(g, 1.0e10) : 0.f;
     }
+}
+
+// ---------------------------------------------------------------------------
+// Learned-proposal local search over a 5 x 5 lattice window.
+// ---------------------------------------------------------------------------
+__device__ inline int nearest_index(const float* axis, int n, float x) {
+    int best = 0;
+    float bd = fabsf(axis[0] - x);
+    for (int i = 1; i < n; ++i) {
+        const float d = fabsf(axis[i] - x);
+        if (d < bd) {
+            bd = d;
+            best = i;
+        }
+    }
+    return best;
+}
+
+// Window centre per voxel from the proposal (a_hat, b_hat, c); the window is
+// kept inside the lattice.  flags[v] = 1 when the confidence c < conf_min or
+// the proposal is not finite (such voxels take the full grid).
+__global__ void k_local_center(PlanDev P, const float* __restrict__ props, int V, float conf_min,
+                               int32_t* __restrict__ center, int32_t* __restrict__ flags) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= V) return;
+    const float ah = props[3 * v], bh = props[3 * v + 1], c = props[3 * v + 2];
+    const bool finite = isfinite(ah) && isfinite(bh) && isfinite(c);
+    int ai = finite ? nearest_index(P.a_axis, P.n_a, ah) : P.zero_a;
+    int bi = finite ? nearest_index(P.b_axis, P.n_b, bh) : P.zero_b;
+    if (P.n_a >= kWin) ai = min(max(ai, kWinHalf), P.n_a - 1 - kWinHalf);
+    if (P.n_b >= kWin) bi = min(max(bi, kWinHalf), P.n_b - 1 - kWinHalf);
+    center[v] = ai * P.n_b + bi;
+    flags[v] = (!finite || !(c >= conf_min)) ? 1 : 0;
+}
+
+// Group layout: each centre's voxels get a run of columns padded to
+// kLocalTile; tiles list (group, first column).  Single thread.
+__global__ void k_local_layout(int n_groups, const int32_t* __restrict__ cnt, int32_t* __restrict__ fill,
+                               int32_t* __restrict__ tile_group, int32_t* __restrict__ tile_col,
+                               int32_t* __restrict__ n_tiles) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    int off = 0, nt = 0;
+    for (int g = 0; g < n_groups; ++g) {
+        fill[g] = off;
+        const int c = cnt[g];
+        for (int i = 0; i < c; i += kLocalTile) {
+            tile_group[nt] = g;
+            tile_col[nt] = off + i;
+            ++nt;
+        }
+        off += (c + kLocalTile - 1) / kLocalTile * kLocalTile;
+    }
+    *n_tiles = nt;
+}
+
+__global__ void k_local_scatter(const int32_t* __restrict__ center, int V, int n_groups, int32_t* __restrict__ fill,
+                                int32_t* __restrict__ rows) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= V) return;
+    const int g = center[v];
+    if (g < 0 || g >= n_groups) return;
+    rows[atomicAdd(&fill[g], 1)] = v;
+}
+
+__device__ inline int window_pair(const PlanDev& P, int ca, int cb, int slot) {
+    const int ai = ca - kWinHalf + slot / kWin, bi = cb - kWinHalf + slot % kWin;
+    if (ai < 0 || ai >= P.n_a || bi < 0 || bi >= P.n_b) return -1;
+    return P.lut[ai * P.n_b + bi];
+}
+
+// Energies of the window's pairs.  Block: 5 waves (window rows) over 64
+// columns of one tile half.  Same arithmetic and accumulation schedule as
+// k_energy, so energies are bitwise equal to the full-grid ones.
+__global__ void __launch_bounds__(kWin * 64) k_local_energy(PlanDev P, const float* __restrict__ Yt, int64_t ldY,
+                                                            const int32_t* __restrict__ tile_group,
+                                                            const int32_t* __restrict__ tile_col,
+                                                            const int32_t* __restrict__ n_tiles,
+                                                            float* __restrict__ Eloc) {
+    const int tile = blockIdx.x;
+    if (tile >= *n_tiles) return;
+    const int lane = threadIdx.x & 63;
+    const int r = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int g = tile_group[tile];
+    const int ca = g / P.n_b, cb = g % P.n_b;
+    const int col = tile_col[tile] + blockIdx.y * 64 + lane;
+    const int T = P.T;
+    const float* __restrict__ y = Yt + col;
+    int ks[kWin];
+    bool has_white = false;
+    int kc = -1;
+#pragma unroll
+    for (int c = 0; c < kWin; ++c) {
+        int k = window_pair(P, ca, cb, r * kWin + c);
+        if (k >= 0 && P.white_flag[k]) {
+            has_white = true;
+            k = -2;  // handled by the white accumulator
+        }
+        ks[c] = k;
+        if (k >= 0 && kc < 0) kc = k;
+    }
+    const float2* __restrict__ cf[kWin];
+    float s0[kWin];
+#pragma unroll
+    for (int c = 0; c < kWin; ++c) {
+        cf[c] = ks[c] >= 0 ? P.lp + (size_t)ks[c] * T : P.zero2;
+        s0[c] = ks[c] >= 0 ? P.ls0[ks[c]] : 0.f;
+    }
+    const float* __restrict__ cz = kc >= 0 ? P.lc + (size_t)kc * T : P.zero1;
+    float e[kWin], part[kWin];
+    double acc[kWin];
+    const float y0 = y[0];
+#pragma unroll
+    for (int c = 0; c < kWin; ++c) {
+        e[c] = y0;
+        part[c] = y0 * y0 * s0[c];
+        acc[c] = 0.0;
+    }
+    float wpart = y0 * y0;
+    double wacc = 0.0;
+    float yc = y0;
+    for (int t0 = 0; t0 < T - 1; t0 += 8) {
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const int t = t0 + i;
+            if (t < T - 1) {
+                const float yn = y[(size_t)(t + 1) * ldY];
+                const float z = fmaf(-cz[t], yc, yn);
+#pragma unroll
+                for (int c = 0; c < kWin; ++c) {
+                    const float2 co = cf[c][t];
+                    e[c] = fmaf(co.x, e[c], z);
+                    part[c] = fmaf(e[c] * co.y, e[c], part[c]);
+                }
+                wpart = fmaf(yn * 1.0f, yn, wpart);
+                yc = yn;
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < kWin; ++c) {
+            acc[c] += part[c];
+            part[c] = 0.f;
+        }
+        wacc += wpart;
+        wpart = 0.f;
+    }
+    float* out = Eloc + (size_t)col * kSlots + r * kWin;
+#pragma unroll
+    for (int c = 0; c < kWin; ++c) {
+        if (ks[c] >= 0)
+            out[c] = (float)(acc[c] + part[c]);
+        else if (ks[c] == -2 && has_white)
+            out[c] = (float)(wacc + wpart);
+    }
+}
+
+// Window projection on MFMA (fp32 32x32x2, same k-order as k_project) + score +
+// local argmin + edge test.  Block: 4 waves x 32 voxel rows = one 128-column
+// tile; N = 26 pair slots x kLocalPP columns (slot 25 is padding).
+__global__ void __launch_bounds__(256) k_local_project(PlanDev P, const float* __restrict__ Yt, int64_t ldY,
+                                                       const int32_t* __restrict__ tile_group,
+                                                       const int32_t* __restrict__ tile_col,
+                                                       const int32_t* __restrict__ n_tiles,
+                                                       const int32_t* __restrict__ rows,
+                                                       const float* __restrict__ Eloc, int32_t* __restrict__ best,
+                                                       float* __restrict__ best_score,
+                                                       int32_t* __restrict__ need_full) {
+    constexpr int BM = kLocalTile, BK = 16, NSLOT = kSlots + 1, BN = NSLOT * kLocalPP, NBLK = BN / 32;
+    constexpr int A_ELEMS = BK * BM, B_ELEMS = BK * BN;
+    __shared__ float lds[A_ELEMS + B_ELEMS];
+    __shared__ float scores[BM][kSlots + 1];
+    __shared__ int kslot[NSLOT];
+    const int tile = blockIdx.x;
+    if (tile >= *n_tiles) return;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int g = tile_group[tile];
+    const int ca = g / P.n_b, cb = g % P.n_b;
+    const int col0 = tile_col[tile];
+    if (tid < NSLOT) kslot[tid] = tid < kSlots ? window_pair(P, ca, cb, tid) : -1;
+    __syncthreads();
+    const int K16 = P.K * kLocalPP;
+    float* As = lds;
+    float* Bs = lds + A_ELEMS;
+    constexpr int A_V4 = A_ELEMS / 4 / 256;
+    constexpr int B_V4 = (B_ELEMS / 4 + 255) / 256;
+    float4 ra[A_V4], rb[B_V4];
+    auto gload = [&](int kt) {
+        const int t0 = kt * BK;
+#pragma unroll
+        for (int i = 0; i < A_V4; ++i) {
+            const int idx = tid + i * 256;
+            const int r = idx / (BM / 4), c4 = idx % (BM / 4);
+            ra[i] = *reinterpret_cast<const float4*>(Yt + (size_t)(t0 + r) * ldY + col0 + c4 * 4);
+        }
+#pragma unroll
+        for (int i = 0; i < B_V4; ++i) {
+            const int idx = tid + i * 256;
+            if (idx < B_ELEMS / 4) {
+                const int r = idx / (BN / 4), c4 = idx % (BN / 4);
+                const int slot = c4 / (kLocalPP / 4), part = c4 % (kLocalPP / 4);
+                const int k = kslot[slot];
+                rb[i] = k >= 0 ? *reinterpret_cast<const float4*>(P.Gp + (size_t)(t0 + r) * K16 + (size_t)k * kLocalPP +
+                                                                   part * 4)
+                               : make_float4(0.f, 0.f, 0.f, 0.f);
+            }
+        }
+    };
+    f32x16 acc[NBLK];
+#pragma unroll
+    for (int i = 0; i < NBLK; ++i)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) acc[i][r] = 0.f;
+    const int nk = P.Tpad / BK;
+    gload(0);
+    for (int kt = 0; kt < nk; ++kt) {
+#pragma unroll
+        for (int i = 0; i < A_V4; ++i) *reinterpret_cast<float4*>(As + (tid + i * 256) * 4) = ra[i];
+#pragma unroll
+        for (int i = 0; i < B_V4; ++i) {
+            const int idx = tid + i * 256;
+            if (idx < B_ELEMS / 4) *reinterpret_cast<float4*>(Bs + idx * 4) = rb[i];
+        }
+        __syncthreads();
+        if (kt + 1 < nk) gload(kt + 1);
+        const int ar = lane >> 5, ac = lane & 31;
+#pragma unroll
+        for (int kk = 0; kk < BK / 2; ++kk) {
+            const float av = As[(2 * kk + ar) * BM + w * 32 + ac];
+#pragma unroll
+            for (int n = 0; n < NBLK; ++n) {
+                const float bv = Bs[(2 * kk + ar) * BN + n * 32 + ac];
+                acc[n] = __builtin_amdgcn_mfma_f32_32x32x2f32(av, bv, acc[n], 0, 0, 0);
+            }
+        }
+        __syncthreads();
+    }
+    // Epilogue, one 32x32 accumulator block at a time through a wave-private
+    // LDS patch: lane (row, half) sums its slot's p columns in order (as k_project).
+    float* Cw = lds + w * 32 * 33;
+    const int p = P.p;
+    const double tp = (double)(P.T - p);
+#pragma unroll
+    for (int n = 0; n < NBLK; ++n) {
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const int row = 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+            Cw[row * 33 + (lane & 31)] = acc[n][r];
+        }
+        __syncthreads();
+        const int row = lane & 31, half = lane >> 5;
+        const int slot = 2 * n + half;
+        if (slot < kSlots) {
+            const int k = kslot[slot];
+            float sc = INFINITY;
+            const int col = col0 + w * 32 + row;
+            if (k >= 0) {
+                double pp = 0.0;
+                for (int j = 0; j < p; ++j) {
+                    const double cv = (double)Cw[row * 33 + half * kLocalPP + j];
+                    pp = fma(cv, cv, pp);
+                }
+                const double rss = (double)Eloc[(size_t)col * kSlots + slot] - pp;
+                sc = (float)fma(tp, log(rss + 1e-10), P.ld[k]);
+            }
+            scores[w * 32 + row][slot] = sc;
+        }
+        __syncthreads();
+    }
+    if (tid < BM) {
+        const int v = rows[col0 + tid];
+        if (v >= 0) {
+            float bs = INFINITY;
+            int bslot = -1;
+            for (int s2 = 0; s2 < kSlots; ++s2) {
+                const float sc = scores[tid][s2];
+                if (sc < bs) {
+                    bs = sc;
+                    bslot = s2;
+                }
+            }
+            int edge = 1;
+            int k = -1;
+            if (bslot >= 0) {
+                k = kslot[bslot];
+                const int r = bslot / kWin, c = bslot % kWin;
+                const int ai = ca - kWinHalf + r, bi = cb - kWinHalf + c;
+                edge = (r == 0 && ai > 0) || (r == kWin - 1 && ai < P.n_a - 1) || (c == 0 && bi > 0) ||
+                       (c == kWin - 1 && bi < P.n_b - 1);
+            }
+            best[v] = k;
+            best_score[v] = bs;
+            if (edge) need_full[v] = 1;
+        }
+    }
+}
+
+__global__ void k_fill_i32(int32_t* __restrict__ p, int64_t n, int32_t val) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = val;
+}
+
+// Zero rows [r0, r1) of a row-major float buffer with row stride ld.
+__global__ void k_zero_rows(float* __restrict__ p, int64_t ld, int r0, int r1, int64_t ncols) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t n = (int64_t)(r1 - r0) * ncols;
+    if (i < n) p[(int64_t)(r0 + i / ncols) * ld + i % ncols] = 0.f;
+}
+
+// Compaction of flagged voxels (order within the list is not significant).
+__global__ void k_compact(const int32_t* __restrict__ flags, int V, int32_t* __restrict__ idx,
+                          int32_t* __restrict__ count) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v < V && flags[v]) idx[atomicAdd(count, 1)] = v;
+}
+
+__global__ void k_scatter_best(const int32_t* __restrict__ idx, int n, const int32_t* __restrict__ best_sub,
+                               const float* __restrict__ score_sub, int32_t* __restrict__ best,
+                               float* __restrict__ score) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    best[idx[i]] = best_sub[i];
+    score[idx[i]] = score_sub[i];
 }
 
 }  // namespace remlk

@@ -267,17 +267,22 @@ class RemlGrid(Mapping):
     def vpad(self, V: int) -> int:
         return (V + self.vox_pad - 1) // self.vox_pad * self.vox_pad
 
-    def prep(self, Y: torch.Tensor, normalize: bool = True, want_features: bool = False):
+    def prep(self, Y: torch.Tensor, normalize: bool = True, want_features: bool = False,
+             rows: torch.Tensor | None = None, scale: torch.Tensor | None = None):
         """Normalise + residualise Y (V, T) float32 on the device.  Returns
-        (Yt (Tpad, Vpad) time-major residual, scale (V,), features (V, 10) | None)."""
+        (Yt (Tpad, ncols_pad) time-major residual, scale (V,), features (V, 10) | None).
+        ``rows`` (int32, optional) selects the voxel of each output column."""
         V = Y.shape[0]
-        Vp = self.vpad(V)
-        Yt = torch.zeros(self.Tpad, Vp, dtype=torch.float32, device=self.device)
-        scale = torch.empty(V, dtype=torch.float32, device=self.device)
+        ncols = V if rows is None else rows.shape[0]
+        Vp = self.vpad(ncols)
+        Yt = torch.empty(self.Tpad, Vp, dtype=torch.float32, device=self.device)
+        if scale is None:
+            scale = torch.empty(V, dtype=torch.float32, device=self.device)
         feats = torch.empty(V, 10, dtype=torch.float32, device=self.device) if want_features else None
         lib = _lib.load()
-        _lib.check(lib.ffs_reml_prep(self._handle, _lib.ptr(Y), Y.stride(0), V, _lib.ptr(Yt), Vp, _lib.ptr(scale),
-                                     int(normalize), _lib.ptr(feats), _lib.stream_handle(self.device)), "prep")
+        _lib.check(lib.ffs_reml_prep(self._handle, _lib.ptr(Y), Y.stride(0), V, _lib.ptr(rows), ncols, _lib.ptr(Yt),
+                                     Vp, _lib.ptr(scale), int(normalize), _lib.ptr(feats),
+                                     _lib.stream_handle(self.device)), "prep")
         return Yt, scale, feats
 
     def surface(self, Yt: torch.Tensor, V: int, out: torch.Tensor | None = None) -> torch.Tensor:

@@ -72,12 +72,17 @@ int ffs_reml_plan_dims(const ffs_reml_plan* plan, int32_t* Tpad, int32_t* vox_pa
 
 /*
  * Normalise + residualise Y (V x T float32, row stride ldY) into the
- * time-major buffer Yt (Tpad x ldYt float32, ldYt >= Vpad, rows >= T left
- * untouched), scale[V] = unbiased std clamped at 1e-8 (1 when normalize == 0).
- * feats (optional, V x 10) receives lag-1..10 autocorrelations of the residual.
+ * time-major buffer Yt (Tpad x ldYt float32).  Column c of Yt holds voxel
+ * rows[c] (rows nullable = identity, then ncols = V); a -1 entry, and every
+ * column from ncols up to the next multiple of the voxel padding unit, is zero,
+ * as are rows T..Tpad-1.  ldYt >= ncols rounded up to the padding unit.
+ * scale[v] = unbiased std clamped at 1e-8 (1 when normalize == 0), indexed by
+ * voxel; feats (nullable, V x 10) receives lag-1..10 autocorrelations of the
+ * residual.
  */
-int ffs_reml_prep(const ffs_reml_plan* plan, const float* Y, int64_t ldY, int64_t V, float* Yt, int64_t ldYt,
-                  float* scale, int32_t normalize, float* feats, void* stream);
+int ffs_reml_prep(const ffs_reml_plan* plan, const float* Y, int64_t ldY, int64_t V, const int32_t* rows,
+                  int64_t ncols, float* Yt, int64_t ldYt, float* scale, int32_t normalize, float* feats,
+                  void* stream);
 
 /* (i) E[v*ldE + k] = ||L_k^-1 y_v||^2 for every voxel and pair. */
 int ffs_reml_energy(const ffs_reml_plan* plan, const float* Yt, int64_t ldYt, int64_t V, float* E, int64_t ldE,
@@ -113,6 +118,31 @@ int ffs_reml_gls(const ffs_reml_plan* plan, const float* Y, int64_t ldY, const f
  */
 int ffs_reml_ljung_box(const double* resid_w, int64_t ldR, int64_t V, int32_t n, int32_t nbins,
                        const int32_t* bin_off, const int32_t* pairs, const double* wbin, float* out, void* stream);
+
+/*
+ * Learned-proposal local search (no counterpart in the reference, which always
+ * searches the whole grid, glm/arma.py:1886-2322).  props (V x 3, device) =
+ * (a_hat, b_hat, log-confidence) from the proposal network.  Each voxel is
+ * scored on the 5 x 5 lattice window around its snapped proposal (kept inside
+ * the lattice) with the same arithmetic as the full grid, so the scores equal
+ * the full-grid surface entries bitwise.  Outputs per voxel: best pair (-1 if
+ * no valid pair), its score, need_full = 1 when the optimum is on a window
+ * edge that is not a lattice edge, when the confidence is below conf_min, or
+ * when the proposal is not finite; scale = per-voxel std.  Requires p <= 16
+ * (returns -4 otherwise).  workspace: ffs_reml_local_workspace bytes, first use
+ * zero-filled.
+ */
+int ffs_reml_local_workspace(const ffs_reml_plan* plan, int64_t V, int64_t* bytes);
+int ffs_reml_local_search(const ffs_reml_plan* plan, const float* Y, int64_t ldY, int64_t V, const float* props,
+                          float conf_min, int32_t* best, float* best_score, int32_t* need_full, float* scale,
+                          void* workspace, int64_t ws_bytes, void* stream);
+
+/* idx[0..*count) = voxels with flags[v] != 0 (order unspecified); count is device memory. */
+int ffs_reml_compact(const int32_t* flags, int64_t V, int32_t* idx, int32_t* count, void* stream);
+
+/* best[idx[i]] = best_sub[i], score[idx[i]] = score_sub[i] for i < n. */
+int ffs_reml_scatter_best(const int32_t* idx, int64_t n, const int32_t* best_sub, const float* score_sub,
+                          int32_t* best, float* score, void* stream);
 
 #ifdef __cplusplus
 }
