@@ -1,16 +1,19 @@
-"""Benchmark: ARMA(1,1) REML voxels/s on one MI355X.
+"""Benchmark: ARMA(1,1) REML voxels/s on one MI355X, learned proposal vs full grid.
 
 Workload (default): T=400 acquired TRs with 5% censored (380 retained),
 V=200,000 float32 voxels, p=12 design (4 HRF-convolved task boxcars, 6 AR(1)
 random-walk motion columns, constant, linear), 33x33 (a, b) grid in
-[-0.8, 0.8] (K=561 valid pairs), synthetic seeded data resident in HBM.
+[-0.8, 0.8] (K=561 valid pairs); synthetic seeded data resident in HBM.
 
-One step = one pass of the hot path over the whole batch: normalise +
-residualise, recurrence energy (i), MFMA projection + REML score (ii),
-hierarchical selection (iii), FP64 GLS refit (iv).
-
-Prints one JSON line (rank 0).  ``--gpus N`` with N > 1 exits: this build
-measures one GPU.
+Two paths are timed over the same batch, each step = one full fit pass:
+* full grid: normalise + residualise, recurrence energy (i), MFMA projection +
+  REML score (ii), hierarchical selection (iii), FP64 GLS refit (iv);
+* learned proposal (the reported ``value``): residual-ACF features, MLP
+  proposal, 5x5-window energy + projection + local argmin, full-grid fallback
+  for edge hits / low confidence, FP64 GLS refit.
+The MLP is trained beforehand (untimed) on separately drawn voxels of the same
+design, labelled by the full-grid GPU search.  Prints one JSON line.
+``--gpus N`` with N > 1 exits: this build measures one GPU.
 """
 
 from __future__ import annotations
@@ -40,9 +43,29 @@ def parse():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--voxels", type=int, default=200_000)
     ap.add_argument("--T", type=int, default=400)
+    ap.add_argument("--train-voxels", type=int, default=2_000_000)
+    ap.add_argument("--epochs", type=int, default=4)
+    ap.add_argument("--target-agreement", type=float, default=0.9995)
     ap.add_argument("--cpu-sample", type=int, default=2048, help="voxels in the timed CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     return ap.parse_args()
+
+
+def timed(fn, steps, warmup, stages):
+    """Run fn(marks) warmup + steps times; returns (ms/step wall, per-stage ms, last result)."""
+    for _ in range(warmup):
+        out = fn(None)
+    torch.cuda.synchronize()
+    allm = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        marks = [torch.cuda.Event(enable_timing=True) for _ in range(len(stages) + 1)]
+        out = fn(marks)
+        allm.append(marks)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    per = {n: float(np.mean([m[i].elapsed_time(m[i + 1]) for m in allm])) for i, n in enumerate(stages)}
+    return ms, per, out
 
 
 def main():
@@ -53,9 +76,11 @@ def main():
         sys.exit(2)
     from fastfuncstuff_amd import _lib, synth
     from fastfuncstuff_amd.glm.arma import RemlGrid
+    from fastfuncstuff_amd.learned import (LocalSearcher, ProposalMLP, calibrate_conf_min, features,
+                                           full_grid_labels)
 
     dev = _lib.require_device(None)
-    _lib.load()
+    lib = _lib.load()
     t0 = time.time()
     wl = synth.gpu_config(V=args.voxels, T=args.T)
     t_gen = time.time() - t0
@@ -71,86 +96,112 @@ def main():
     t_plan = time.time() - t0
     K = grid.K
     Y = torch.from_numpy(wl.Y).to(dev)
+    st = _lib.stream_handle(dev)
     stream = torch.cuda.current_stream(dev)
+    S = torch.empty(V, K, dtype=torch.float32, device=dev)
 
-    names = ["prep", "energy", "project", "select", "gls"]
-    ev = {n: [] for n in names}
+    # ---- full-grid path -------------------------------------------------
+    fg_stages = ["prep", "energy", "project", "select", "gls"]
 
-    def step(record: bool):
-        marks = [torch.cuda.Event(enable_timing=True) for _ in range(len(names) + 1)] if record else None
-        if record:
-            marks[0].record(stream)
+    def full_step(marks):
+        def mark(i):
+            if marks is not None:
+                marks[i].record(stream)
+        mark(0)
         Yt, scale, _ = grid.prep(Y, normalize=True)
-        if record:
-            marks[1].record(stream)
-        S = torch.empty(V, K, dtype=torch.float32, device=dev)
-        lib = _lib.load()
-        st = _lib.stream_handle(dev)
+        mark(1)
         _lib.check(lib.ffs_reml_energy(grid.handle, _lib.ptr(Yt), Yt.stride(0), V, _lib.ptr(S), K, st), "energy")
-        if record:
-            marks[2].record(stream)
+        mark(2)
         _lib.check(lib.ffs_reml_project(grid.handle, _lib.ptr(Yt), Yt.stride(0), V, _lib.ptr(S), K, st), "project")
-        if record:
-            marks[3].record(stream)
-        best, score = grid.select(S, hierarchical=True)
-        if record:
-            marks[4].record(stream)
-        out = grid.gls(Y, scale, best)
-        if record:
-            marks[5].record(stream)
-        return marks, best, S
+        mark(3)
+        best, _ = grid.select(S, hierarchical=True)
+        mark(4)
+        grid.gls(Y, scale, best)
+        mark(5)
+        return best
 
-    for _ in range(args.warmup):
-        step(False)
-    torch.cuda.synchronize()
-    all_marks = []
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        marks, best, S = step(True)
-        all_marks.append(marks)
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    for marks in all_marks:
-        for i, n in enumerate(names):
-            ev[n].append(marks[i].elapsed_time(marks[i + 1]))
-    kern_ms = {n: float(np.mean(v)) for n, v in ev.items()}
-    ms_per_step = elapsed * 1e3 / args.steps
-    value = V / (ms_per_step / 1e3)
+    fg_ms, fg_k, best_hier = timed(full_step, args.steps, args.warmup, fg_stages)
+    best_exh, _ = grid.select(S, hierarchical=False)
+    print(f"[bench] full grid {fg_ms:.2f} ms/step", file=sys.stderr, flush=True)
 
-    # Roofline of the dominant kernel.
-    proj_flops = 2.0 * V * T * K * p  # Y (V x T) . G (T x K p)
-    energy_bytes = 4.0 * V * grid.Tpad + 4.0 * V * K  # Y read once + energy written
-    dom = max(["energy", "project"], key=lambda n: kern_ms[n])
-    if dom == "project":
-        ach = proj_flops / (kern_ms["project"] / 1e3)
-        roof = {"kernel": "k_project", "bound": "mfma", "achieved": ach / 1e12, "peak": PEAK_FP32_MFMA / 1e12,
+    # ---- train the proposal network (untimed) --------------------------
+    t0 = time.time()
+    tr = synth.gpu_config(V=args.train_voxels, T=args.T, noise_seed=101, device=dev)
+    val = synth.gpu_config(V=200_000, T=args.T, noise_seed=202, device=dev)
+    labels, _ = full_grid_labels(grid, tr.Y)
+    feats = features(grid, tr.Y)
+    mlp = ProposalMLP(dev, seed=0)
+    log = mlp.train(feats, labels, epochs=args.epochs, batch=4096, lr=3e-3, lr_final=1e-4, log_every=200)
+    _, best_val = full_grid_labels(grid, val.Y)
+    conf_min = calibrate_conf_min(mlp, grid, val.Y, best_val, target=args.target_agreement)
+    del tr, labels, feats, val
+    torch.cuda.synchronize()
+    t_train = time.time() - t0
+    print(f"[bench] trained in {t_train:.1f} s ({log.steps} steps), conf_min {conf_min:.3f}", file=sys.stderr,
+          flush=True)
+
+    # ---- learned-proposal path ------------------------------------------
+    ls = LocalSearcher(grid, mlp, V)
+    lp_stages = list(LocalSearcher.STAGES) + ["gls"]
+
+    def learned_step(marks):
+        res = ls.run(Y, marks=None if marks is None else marks[:len(LocalSearcher.STAGES) + 1])
+        grid.gls(Y, res.scale, res.best)
+        if marks is not None:
+            marks[-1].record(stream)
+        return res
+
+    lp_ms, lp_k, res = timed(learned_step, args.steps, args.warmup, lp_stages)
+    best_l = res.best
+    agree_exh = float((best_l == best_exh).double().mean())
+    agree_hier = float((best_l == best_hier).double().mean())
+    value = V / (lp_ms / 1e3)
+
+    # ---- roofline of the dominant kernel of the reported (learned) path --
+    kern = {"local_project": ("mfma", 2.0 * V * T * 26 * 16, "2*V*T*(26 slots*16 cols)"),
+            "local_energy": ("hbm", 4.0 * grid.Tpad * V + 4.0 * V * 25, "4*Tpad*V + 4*25*V"),
+            "gls": ("hbm", 4.0 * V * T + 4.0 * V * (2 * p + 3), "4*V*T + 4*V*(2p+3)")}
+    dom = max(kern, key=lambda n: lp_k[n])
+    bound, amount, formula = kern[dom]
+    ach = amount / (lp_k[dom] / 1e3)
+    if bound == "mfma":
+        roof = {"kernel": dom, "bound": "mfma", "achieved": ach / 1e12, "peak": PEAK_FP32_MFMA / 1e12,
                 "unit": "TFLOP/s", "frac": ach / PEAK_FP32_MFMA, "traffic": None,
-                "algorithmic": f"2*V*T*K*p = {proj_flops:.4g} flop per launch (fp32 MFMA)"}
+                "algorithmic": f"{formula} = {amount:.4g} flop per launch"}
     else:
-        ach = energy_bytes / (kern_ms["energy"] / 1e3)
-        roof = {"kernel": "k_energy", "bound": "hbm", "achieved": ach / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
+        roof = {"kernel": dom, "bound": "hbm", "achieved": ach / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
                 "frac": ach / PEAK_HBM, "traffic": None,
-                "algorithmic": f"4*V*Tpad + 4*V*K = {energy_bytes:.4g} B per launch"}
+                "algorithmic": f"{formula} = {amount:.4g} B per launch"}
+    proj_flops = 2.0 * V * T * K * p
+    fg_roof = {"kernel": "k_project", "bound": "mfma",
+               "achieved": proj_flops / (fg_k["project"] / 1e3) / 1e12, "peak": PEAK_FP32_MFMA / 1e12,
+               "unit": "TFLOP/s", "frac": proj_flops / (fg_k["project"] / 1e3) / PEAK_FP32_MFMA}
 
     line = {
         "metric": METRIC, "value": value, "unit": "voxels/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
+        "warmup": args.warmup, "ms_per_step": lp_ms, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
         "config": {"workload": f"censored_T{args.T}_V{V}_p{p}_grid33x33", "V": V, "T_acquired": args.T,
-                   "T_retained": T, "p": p, "K": K, "path": "full_grid_hierarchical",
-                   "projection": "fp32 MFMA 32x32x2", "gls": "fp64"},
-        "kernels_ms": kern_ms, "plan_build_s": t_plan, "datagen_s": t_gen,
+                   "T_retained": T, "p": p, "K": K, "path": "learned_proposal_5x5_with_full_grid_fallback",
+                   "projection": "fp32 MFMA 32x32x2", "gls": "fp64", "train_voxels": args.train_voxels},
+        "full_grid": {"value": V / (fg_ms / 1e3), "ms_per_step": fg_ms, "kernels_ms": fg_k, "roofline": fg_roof},
+        "learned": {"kernels_ms": lp_k, "fallback_fraction": res.n_fallback / V, "conf_min": conf_min,
+                    "argmin_match_vs_full_grid_exhaustive": agree_exh,
+                    "argmin_match_vs_full_grid_hierarchical": agree_hier,
+                    "speedup_vs_full_grid": fg_ms / lp_ms, "train_s": t_train, "train_steps": log.steps,
+                    "train_loss_first_last": [log.losses[0][1], log.losses[-1][1]] if log.losses else None},
+        "plan_build_s": t_plan, "datagen_s": t_gen,
         "roofline": roof,
     }
-
     if not args.no_cpu:
-        line["cpu_baseline"], line["argmin_match_sample"] = cpu_baseline(wl, grid, best, args.cpu_sample)
+        line["cpu_baseline"], line["argmin_match_sample"] = cpu_baseline(wl, best_hier, best_exh, best_l,
+                                                                        args.cpu_sample)
     print(json.dumps(line), flush=True)
 
 
-def cpu_baseline(wl, grid, best_gpu, n_sample):
+def cpu_baseline(wl, best_hier, best_exh, best_learned, n_sample):
     """float32 reference-semantics restatement (oracle) timed on the host on a
-    bounded voxel sample; also the GPU's argmin agreement on that sample."""
+    bounded voxel sample, and the GPU paths' argmin agreement with it there."""
     from oracle import reml_oracle as O
 
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
@@ -166,20 +217,26 @@ def cpu_baseline(wl, grid, best_gpu, n_sample):
     fit = O.fit_arma11(Ys, X, run_starts=wl.run_starts, tau=tau, dtype=torch.float32, selector="hierarchical",
                        grid=og)
     t_fit = time.perf_counter() - t0
-    k_cpu = fit.pair_index
-    k_gpu = best_gpu[:n_sample].cpu().long()
     S = fit.surface.double()
-    gap = (S[torch.arange(n_sample), k_gpu] - S[torch.arange(n_sample), k_cpu]).abs()
-    match = float((k_cpu == k_gpu).double().mean())
+    k_exh, _ = O.select_exhaustive(fit.surface)
+    ar = torch.arange(n_sample)
+
+    def agree(k_cpu, k_gpu):
+        k_gpu = k_gpu[:n_sample].cpu().long()
+        gap = (S[ar, k_gpu] - S[ar, k_cpu]).abs()
+        diff = k_cpu != k_gpu
+        return {"match": float((~diff).double().mean()), "near_tie_mismatch": int((diff & (gap <= 2e-3)).sum()),
+                "hard_mismatch": int((diff & (gap > 2e-3)).sum())}
+
     base = {"value": n_sample / t_fit, "unit": "voxels/s", "cores": cores, "kind": "port",
             "sample": f"{n_sample} voxels of the same workload; oracle float32 reference-semantics fit "
                       f"(dense L^-1 whitening GEMM per pair, exhaustive surface + hierarchical, GLS); "
                       f"grid precompute {t_pre:.1f} s excluded",
             "precompute_s": t_pre}
-    agree = {"n": n_sample, "hierarchical_match": match,
-             "near_tie_mismatch": int(((k_cpu != k_gpu) & (gap <= 2e-3)).sum()),
-             "hard_mismatch": int(((k_cpu != k_gpu) & (gap > 2e-3)).sum())}
-    return base, agree
+    agreement = {"n": n_sample, "full_grid_hierarchical_vs_cpu_hierarchical": agree(fit.pair_index, best_hier),
+                 "full_grid_exhaustive_vs_cpu_exhaustive": agree(k_exh, best_exh),
+                 "learned_vs_cpu_exhaustive": agree(k_exh, best_learned)}
+    return base, agreement
 
 
 if __name__ == "__main__":

@@ -281,7 +281,7 @@ static int prep_impl(const ffs_reml_plan* P, const float* Y, int64_t ldY, int64_
                      int64_t ncols, float* Yt, int64_t ldYt, float* scale, int normalize, float* feats,
                      hipStream_t s) {
     const int64_t ncp = vpad_of(ncols);
-    if (ldYt < ncp) return fail(-3, "prep: ldYt must be >= the padded column count");
+    if (Yt && ldYt < ncp) return fail(-3, "prep: ldYt must be >= the padded column count");
     if (ldY < P->d.T) return fail(-3, "prep: ldY must be >= T");
     if (ncp > 0x7fffffff || V > 0x7fffffff) return fail(-3, "prep: batch too large; split it");
     const int p = P->d.p;
@@ -296,7 +296,7 @@ static int prep_impl(const ffs_reml_plan* P, const float* Y, int64_t ldY, int64_
         launch_prep<96>(P, Y, ldY, v, rows, nc, ncpi, Yt, ldYt, scale, normalize, feats, s);
     HIP_TRY(hipGetLastError());
     // rows T..Tpad-1 of the time-major buffer are zero (GEMM k padding)
-    if (P->d.Tpad > P->d.T) {
+    if (Yt && P->d.Tpad > P->d.T) {
         const int64_t n = (int64_t)(P->d.Tpad - P->d.T) * ncp;
         hipLaunchKernelGGL(k_zero_rows, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, Yt, ldYt, P->d.T,
                            P->d.Tpad, ncp);
@@ -460,44 +460,87 @@ extern "C" int ffs_reml_local_workspace(const ffs_reml_plan* P, int64_t V, int64
     return 0;
 }
 
+namespace {
+struct LocalPtrs {
+    int32_t *center, *cnt, *fill, *tg, *tc, *nt, *rows;
+    float *Ytg, *Eloc;
+};
+LocalPtrs local_ptrs(const LocalLayout& L, void* workspace) {
+    char* ws = static_cast<char*>(workspace);
+    LocalPtrs q;
+    q.center = reinterpret_cast<int32_t*>(ws + L.off_center);
+    q.cnt = reinterpret_cast<int32_t*>(ws + L.off_cnt);
+    q.fill = reinterpret_cast<int32_t*>(ws + L.off_fill);
+    q.tg = reinterpret_cast<int32_t*>(ws + L.off_tg);
+    q.tc = reinterpret_cast<int32_t*>(ws + L.off_tc);
+    q.nt = reinterpret_cast<int32_t*>(ws + L.off_nt);
+    q.rows = reinterpret_cast<int32_t*>(ws + L.off_rows);
+    q.Ytg = reinterpret_cast<float*>(ws + L.off_yt);
+    q.Eloc = reinterpret_cast<float*>(ws + L.off_e);
+    return q;
+}
+int local_check(const ffs_reml_plan* P, int64_t V, void* workspace, int64_t ws_bytes, LocalLayout* L) {
+    if (!P) return fail(-3, "local search: null plan");
+    if (!P->local_ok) return fail(-4, "local search: needs p <= 16 (use the full grid)");
+    *L = local_layout(P, V);
+    if (!workspace || ws_bytes < (int64_t)L->total) return fail(-3, "local search: workspace too small");
+    if (L->max_cols > 0x7fffffff) return fail(-3, "local search: batch too large; split it");
+    return 0;
+}
+}  // namespace
+
+extern "C" int ffs_reml_local_prepare(const ffs_reml_plan* P, const float* Y, int64_t ldY, int64_t V,
+                                      const float* props, float conf_min, int32_t* need_full, float* scale,
+                                      void* workspace, int64_t ws_bytes, void* stream) {
+    if (V <= 0) return 0;
+    LocalLayout L;
+    if (int rc = local_check(P, V, workspace, ws_bytes, &L)) return rc;
+    LocalPtrs q = local_ptrs(L, workspace);
+    hipStream_t s = (hipStream_t)stream;
+    const unsigned gv = (unsigned)((V + 255) / 256);
+    hipLaunchKernelGGL(k_local_center, dim3(gv), dim3(256), 0, s, P->d, props, (int)V, conf_min, q.center, need_full);
+    hipLaunchKernelGGL(k_fill_i32, dim3((unsigned)((L.ng + 1 + 255) / 256)), dim3(256), 0, s, q.cnt, L.ng + 1, 0);
+    hipLaunchKernelGGL(k_hist, dim3(gv), dim3(256), 0, s, q.center, (int)V, (int)L.ng, q.cnt);
+    hipLaunchKernelGGL(k_local_layout, dim3(1), dim3(1), 0, s, (int)L.ng, q.cnt, q.fill, q.tg, q.tc, q.nt);
+    hipLaunchKernelGGL(k_fill_i32, dim3((unsigned)((L.max_cols + 255) / 256)), dim3(256), 0, s, q.rows, L.max_cols,
+                       -1);
+    hipLaunchKernelGGL(k_local_scatter, dim3(gv), dim3(256), 0, s, q.center, (int)V, (int)L.ng, q.fill, q.rows);
+    HIP_TRY(hipGetLastError());
+    return prep_impl(P, Y, ldY, V, q.rows, L.max_cols, q.Ytg, L.max_cols, scale, 1, nullptr, s);
+}
+
+extern "C" int ffs_reml_local_energy(const ffs_reml_plan* P, int64_t V, void* workspace, int64_t ws_bytes,
+                                     void* stream) {
+    if (V <= 0) return 0;
+    LocalLayout L;
+    if (int rc = local_check(P, V, workspace, ws_bytes, &L)) return rc;
+    LocalPtrs q = local_ptrs(L, workspace);
+    hipLaunchKernelGGL(k_local_energy, dim3((unsigned)L.max_tiles, 2), dim3(kWin * 64), 0, (hipStream_t)stream, P->d,
+                       q.Ytg, (int64_t)L.max_cols, q.tg, q.tc, q.nt, q.Eloc);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+extern "C" int ffs_reml_local_project(const ffs_reml_plan* P, int64_t V, int32_t* best, float* best_score,
+                                      int32_t* need_full, void* workspace, int64_t ws_bytes, void* stream) {
+    if (V <= 0) return 0;
+    LocalLayout L;
+    if (int rc = local_check(P, V, workspace, ws_bytes, &L)) return rc;
+    LocalPtrs q = local_ptrs(L, workspace);
+    hipLaunchKernelGGL(k_local_project, dim3((unsigned)L.max_tiles), dim3(256), 0, (hipStream_t)stream, P->d, q.Ytg,
+                       (int64_t)L.max_cols, q.tg, q.tc, q.nt, q.rows, q.Eloc, best, best_score, need_full);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
 extern "C" int ffs_reml_local_search(const ffs_reml_plan* P, const float* Y, int64_t ldY, int64_t V,
                                      const float* props, float conf_min, int32_t* best, float* best_score,
                                      int32_t* need_full, float* scale, void* workspace, int64_t ws_bytes,
                                      void* stream) {
-    if (!P) return fail(-3, "local_search: null plan");
-    if (!P->local_ok) return fail(-4, "local_search: needs p <= 16 (use the full grid)");
-    if (V <= 0) return 0;
-    const LocalLayout L = local_layout(P, V);
-    if (!workspace || ws_bytes < (int64_t)L.total) return fail(-3, "local_search: workspace too small");
-    if (L.max_cols > 0x7fffffff) return fail(-3, "local_search: batch too large; split it");
-    hipStream_t s = (hipStream_t)stream;
-    char* ws = static_cast<char*>(workspace);
-    int32_t* center = reinterpret_cast<int32_t*>(ws + L.off_center);
-    int32_t* cnt = reinterpret_cast<int32_t*>(ws + L.off_cnt);
-    int32_t* fill = reinterpret_cast<int32_t*>(ws + L.off_fill);
-    int32_t* tg = reinterpret_cast<int32_t*>(ws + L.off_tg);
-    int32_t* tc = reinterpret_cast<int32_t*>(ws + L.off_tc);
-    int32_t* nt = reinterpret_cast<int32_t*>(ws + L.off_nt);
-    int32_t* rows = reinterpret_cast<int32_t*>(ws + L.off_rows);
-    float* Ytg = reinterpret_cast<float*>(ws + L.off_yt);
-    float* Eloc = reinterpret_cast<float*>(ws + L.off_e);
-    const unsigned gv = (unsigned)((V + 255) / 256);
-    hipLaunchKernelGGL(k_local_center, dim3(gv), dim3(256), 0, s, P->d, props, (int)V, conf_min, center, need_full);
-    hipLaunchKernelGGL(k_fill_i32, dim3((unsigned)((L.ng + 1 + 255) / 256)), dim3(256), 0, s, cnt, L.ng + 1, 0);
-    hipLaunchKernelGGL(k_hist, dim3(gv), dim3(256), 0, s, center, (int)V, (int)L.ng, cnt);
-    hipLaunchKernelGGL(k_local_layout, dim3(1), dim3(1), 0, s, (int)L.ng, cnt, fill, tg, tc, nt);
-    hipLaunchKernelGGL(k_fill_i32, dim3((unsigned)((L.max_cols + 255) / 256)), dim3(256), 0, s, rows, L.max_cols,
-                       -1);
-    hipLaunchKernelGGL(k_local_scatter, dim3(gv), dim3(256), 0, s, center, (int)V, (int)L.ng, fill, rows);
-    HIP_TRY(hipGetLastError());
-    if (int rc = prep_impl(P, Y, ldY, V, rows, L.max_cols, Ytg, L.max_cols, scale, 1, nullptr, s)) return rc;
-    hipLaunchKernelGGL(k_local_energy, dim3((unsigned)L.max_tiles, 2), dim3(kWin * 64), 0, s, P->d, Ytg,
-                       (int64_t)L.max_cols, tg, tc, nt, Eloc);
-    HIP_TRY(hipGetLastError());
-    hipLaunchKernelGGL(k_local_project, dim3((unsigned)L.max_tiles), dim3(256), 0, s, P->d, Ytg, (int64_t)L.max_cols,
-                       tg, tc, nt, rows, Eloc, best, best_score, need_full);
-    HIP_TRY(hipGetLastError());
-    return 0;
+    if (int rc = ffs_reml_local_prepare(P, Y, ldY, V, props, conf_min, need_full, scale, workspace, ws_bytes, stream))
+        return rc;
+    if (int rc = ffs_reml_local_energy(P, V, workspace, ws_bytes, stream)) return rc;
+    return ffs_reml_local_project(P, V, best, best_score, need_full, workspace, ws_bytes, stream);
 }
 
 extern "C" int ffs_reml_compact(const int32_t* flags, int64_t V, int32_t* idx, int32_t* count, void* stream) {

@@ -199,7 +199,8 @@ __global__ void __launch_bounds__(256) k_prep(int T, int p, const float* __restr
     if (col >= ncols_pad) return;
     const int v = (col < ncols) ? (rows ? rows[col] : col) : -1;
     if (v < 0 || v >= V) {
-        for (int t = 0; t < T; ++t) Yt[(size_t)t * ldYt + col] = 0.f;
+        if (Yt)
+            for (int t = 0; t < T; ++t) Yt[(size_t)t * ldYt + col] = 0.f;
         return;
     }
     const float* y = Y + (size_t)v * ldY;
@@ -240,7 +241,7 @@ __global__ void __launch_bounds__(256) k_prep(int T, int p, const float* __restr
         for (int j = 0; j < PM; ++j)
             if (j < p) r -= xr[j] * c[j];
         const float rf = (float)r;
-        Yt[(size_t)t * ldYt + col] = rf;
+        if (Yt) Yt[(size_t)t * ldYt + col] = rf;
         if (NF > 0) {
             acf0 += (double)rf * rf;
 #pragma unroll
@@ -1034,6 +1035,9 @@ __global__ void __launch_bounds__(256) k_local_project(PlanDev P, const float* _
             int k = -1;
             if (bslot >= 0) {
                 k = kslot[bslot];
+                // lambda == 0 pairs score bitwise alike: the global first-minimum
+                // rule picks the first of them.
+                if (P.white_flag[k]) k = P.white[0];
                 const int r = bslot / kWin, c = bslot % kWin;
                 const int ai = ca - kWinHalf + r, bi = cb - kWinHalf + c;
                 edge = (r == 0 && ai > 0) || (r == kWin - 1 && ai < P.n_a - 1) || (c == 0 && bi > 0) ||

@@ -285,6 +285,17 @@ class RemlGrid(Mapping):
                                      _lib.stream_handle(self.device)), "prep")
         return Yt, scale, feats
 
+    def prep_features(self, Y: torch.Tensor, normalize: bool = True):
+        """Per-voxel scale and lag-1..10 residual autocorrelations only (no
+        time-major residual is written).  Returns (None, scale, feats)."""
+        V = Y.shape[0]
+        scale = torch.empty(V, dtype=torch.float32, device=self.device)
+        feats = torch.empty(V, 10, dtype=torch.float32, device=self.device)
+        _lib.check(_lib.load().ffs_reml_prep(self._handle, _lib.ptr(Y), Y.stride(0), V, None, V, None, 0,
+                                             _lib.ptr(scale), int(normalize), _lib.ptr(feats),
+                                             _lib.stream_handle(self.device)), "prep_features")
+        return None, scale, feats
+
     def surface(self, Yt: torch.Tensor, V: int, out: torch.Tensor | None = None) -> torch.Tensor:
         """(V, K) REML score surface: energy kernel then MFMA projection."""
         S = out if out is not None else torch.empty(V, self.K, dtype=torch.float32, device=self.device)

@@ -1,0 +1,239 @@
+"""Learned (a, b) proposal + local REML search.
+
+A small MLP maps the lag-1..10 autocorrelation of each voxel's OLS residual
+(computed by the prep kernel) to (a_hat, b_hat, c), c = log-confidence.  It is
+trained on the GPU by the project's own forward / backward / Adam kernels
+(libffsreml.so, include/ffs_mlp.h) on labels produced by the full-grid GPU
+search itself (the global argmin).  At inference the REML score is evaluated
+only on the 5 x 5 lattice window around the snapped proposal; voxels whose
+local optimum sits on a window edge (away from the lattice edge) or whose
+confidence is below ``conf_min`` are searched on the full grid.  Window scores
+are bitwise equal to the full-grid surface, so an accepted local optimum is
+exactly the global argmin whenever the latter lies inside the window.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import math
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+
+from . import _lib
+from .glm.arma import RemlGrid
+
+N_IN, N_H, N_OUT = 10, 64, 3
+
+
+def init_params(seed: int = 0) -> np.ndarray:
+    """He-style initialisation (zero biases) in the flat layout
+    W1[10][64] b1[64] W2[64][64] b2[64] W3[64][3] b3[3]."""
+    rng = np.random.default_rng(seed)
+    parts = [rng.standard_normal((N_IN, N_H)) * math.sqrt(2.0 / N_IN), np.zeros(N_H),
+             rng.standard_normal((N_H, N_H)) * math.sqrt(2.0 / N_H), np.zeros(N_H),
+             rng.standard_normal((N_H, N_OUT)) * math.sqrt(1.0 / N_H), np.zeros(N_OUT)]
+    return np.concatenate([p.reshape(-1) for p in parts]).astype(np.float32)
+
+
+@dataclass
+class TrainLog:
+    losses: list = field(default_factory=list)
+    seconds: float = 0.0
+    steps: int = 0
+
+
+class ProposalMLP:
+    """The proposal network, parameters resident on the device."""
+
+    def __init__(self, device: torch.device | None = None, seed: int = 0, params: np.ndarray | None = None):
+        self.device = _lib.require_device(device)
+        lib = _lib.load()
+        self.n_params = int(lib.ffs_mlp_param_count())
+        p = init_params(seed) if params is None else np.asarray(params, dtype=np.float32)
+        if p.shape != (self.n_params,):
+            raise ValueError(f"expected {self.n_params} parameters, got {p.shape}")
+        self.params = torch.from_numpy(p.copy()).to(self.device)
+        self.conf_min = -math.inf
+
+    def forward(self, feats: torch.Tensor, idx: torch.Tensor | None = None) -> torch.Tensor:
+        B = feats.shape[0] if idx is None else idx.shape[0]
+        out = torch.empty(B, N_OUT, dtype=torch.float32, device=self.device)
+        rc = _lib.load().ffs_mlp_forward(_lib.ptr(self.params), _lib.ptr(feats), _lib.ptr(idx), B, _lib.ptr(out),
+                                         _lib.stream_handle(self.device))
+        _lib.check(rc, "mlp_forward")
+        return out
+
+    def loss_and_grad(self, feats, target, idx=None):
+        lib = _lib.load()
+        B = feats.shape[0] if idx is None else idx.shape[0]
+        grad = torch.empty(self.n_params, dtype=torch.float32, device=self.device)
+        loss = torch.empty(1, dtype=torch.float32, device=self.device)
+        ws = torch.empty(int(lib.ffs_mlp_workspace_floats(B)), dtype=torch.float32, device=self.device)
+        rc = lib.ffs_mlp_grad(_lib.ptr(self.params), _lib.ptr(feats), _lib.ptr(target), _lib.ptr(idx), B,
+                              _lib.ptr(grad), _lib.ptr(loss), _lib.ptr(ws), _lib.stream_handle(self.device))
+        _lib.check(rc, "mlp_grad")
+        return loss, grad
+
+    def train(self, feats: torch.Tensor, target: torch.Tensor, epochs: int = 8, batch: int = 4096,
+              lr: float = 2e-3, lr_final: float = 1e-4, seed: int = 0, log_every: int = 0) -> TrainLog:
+        """Adam on the device; mini-batches from a seeded host permutation."""
+        lib = _lib.load()
+        dev = self.device
+        n = feats.shape[0]
+        rng = np.random.default_rng(seed)
+        m = torch.zeros_like(self.params)
+        v = torch.zeros_like(self.params)
+        grad = torch.empty_like(self.params)
+        loss = torch.empty(1, dtype=torch.float32, device=dev)
+        ws = torch.empty(int(lib.ffs_mlp_workspace_floats(batch)), dtype=torch.float32, device=dev)
+        steps_per_epoch = max(1, n // batch)
+        total = epochs * steps_per_epoch
+        log = TrainLog()
+        st = _lib.stream_handle(dev)
+        t0 = time.perf_counter()
+        step = 0
+        for ep in range(epochs):
+            perm = torch.from_numpy(rng.permutation(n).astype(np.int32)).to(dev)
+            for s in range(steps_per_epoch):
+                idx = perm[s * batch:(s + 1) * batch]
+                step += 1
+                frac = (step - 1) / max(1, total - 1)
+                lr_t = lr * (lr_final / lr) ** frac
+                _lib.check(lib.ffs_mlp_grad(_lib.ptr(self.params), _lib.ptr(feats), _lib.ptr(target), _lib.ptr(idx),
+                                            batch, _lib.ptr(grad), _lib.ptr(loss), _lib.ptr(ws), st), "mlp_grad")
+                _lib.check(lib.ffs_mlp_adam(_lib.ptr(self.params), _lib.ptr(grad), _lib.ptr(m), _lib.ptr(v),
+                                            self.n_params, lr_t, 0.9, 0.999, 1e-8, step, st), "mlp_adam")
+                if log_every and step % log_every == 0:
+                    log.losses.append((step, float(loss.item())))
+        torch.cuda.synchronize(dev)
+        log.seconds = time.perf_counter() - t0
+        log.steps = step
+        return log
+
+    def state(self) -> np.ndarray:
+        return self.params.detach().cpu().numpy().copy()
+
+
+def features(grid: RemlGrid, Y: torch.Tensor) -> torch.Tensor:
+    """Lag-1..10 autocorrelation of the OLS residual of every voxel (device)."""
+    _, _, feats = grid.prep_features(Y)
+    return feats
+
+
+def full_grid_labels(grid: RemlGrid, Y: torch.Tensor, batch: int = 500_000):
+    """Global-argmin (a, b) of the full-grid GPU search, per voxel."""
+    V = Y.shape[0]
+    labels = torch.empty(V, 2, dtype=torch.float32, device=grid.device)
+    best_all = torch.empty(V, dtype=torch.int32, device=grid.device)
+    params = grid.params.to(grid.device, torch.float32)
+    for b0 in range(0, V, batch):
+        b1 = min(V, b0 + batch)
+        Yt, _, _ = grid.prep(Y[b0:b1])
+        S = grid.surface(Yt, b1 - b0)
+        del Yt
+        best, _ = grid.select(S, hierarchical=False)
+        del S
+        best_all[b0:b1] = best
+        labels[b0:b1] = params[best.long()]
+    return labels, best_all
+
+
+@dataclass
+class LocalResult:
+    best: torch.Tensor
+    score: torch.Tensor
+    scale: torch.Tensor
+    n_fallback: int
+    props: torch.Tensor
+
+
+class LocalSearcher:
+    """Reusable device buffers for the learned-proposal search of one grid."""
+
+    def __init__(self, grid: RemlGrid, mlp: ProposalMLP, V: int):
+        self.grid, self.mlp, self.V = grid, mlp, V
+        lib = _lib.load()
+        nb = ctypes.c_int64()
+        _lib.check(lib.ffs_reml_local_workspace(grid.handle, V, ctypes.byref(nb)), "local_workspace")
+        dev = grid.device
+        self.ws = torch.zeros(int(nb.value), dtype=torch.uint8, device=dev)
+        self.best = torch.empty(V, dtype=torch.int32, device=dev)
+        self.score = torch.empty(V, dtype=torch.float32, device=dev)
+        self.need = torch.empty(V, dtype=torch.int32, device=dev)
+        self.scale = torch.empty(V, dtype=torch.float32, device=dev)
+        self.idx = torch.empty(V, dtype=torch.int32, device=dev)
+        self.count = torch.zeros(1, dtype=torch.int32, device=dev)
+
+    STAGES = ("features", "mlp", "local_prepare", "local_energy", "local_project", "fallback")
+
+    def run(self, Y: torch.Tensor, marks: list | None = None) -> LocalResult:
+        """Learned-proposal search of Y (V, T) float32 on the device.  ``marks``
+        (optional) = len(STAGES)+1 torch.cuda.Events recorded between stages."""
+        grid, lib, dev = self.grid, _lib.load(), self.grid.device
+        V = Y.shape[0]
+        if V != self.V:
+            raise ValueError(f"LocalSearcher was sized for {self.V} voxels, got {V}")
+        st = _lib.stream_handle(dev)
+        stream = torch.cuda.current_stream(dev)
+
+        def mark(i):
+            if marks is not None:
+                marks[i].record(stream)
+
+        mark(0)
+        _, _, feats = grid.prep_features(Y)
+        mark(1)
+        props = self.mlp.forward(feats)
+        mark(2)
+        ws, nb = _lib.ptr(self.ws), self.ws.numel()
+        _lib.check(lib.ffs_reml_local_prepare(grid.handle, _lib.ptr(Y), Y.stride(0), V, _lib.ptr(props),
+                                              float(self.mlp.conf_min), _lib.ptr(self.need), _lib.ptr(self.scale),
+                                              ws, nb, st), "local_prepare")
+        mark(3)
+        _lib.check(lib.ffs_reml_local_energy(grid.handle, V, ws, nb, st), "local_energy")
+        mark(4)
+        _lib.check(lib.ffs_reml_local_project(grid.handle, V, _lib.ptr(self.best), _lib.ptr(self.score),
+                                              _lib.ptr(self.need), ws, nb, st), "local_project")
+        mark(5)
+        _lib.check(lib.ffs_reml_compact(_lib.ptr(self.need), V, _lib.ptr(self.idx), _lib.ptr(self.count), st),
+                   "compact")
+        n_fb = int(self.count.item())
+        if n_fb:
+            rows = self.idx[:n_fb]
+            Yt, _, _ = grid.prep(Y, rows=rows, scale=self.scale)
+            S = grid.surface(Yt, n_fb)
+            del Yt
+            b_fb, s_fb = grid.select(S, hierarchical=False)
+            del S
+            _lib.check(lib.ffs_reml_scatter_best(_lib.ptr(rows), n_fb, _lib.ptr(b_fb), _lib.ptr(s_fb),
+                                                 _lib.ptr(self.best), _lib.ptr(self.score), st), "scatter_best")
+        mark(6)
+        return LocalResult(self.best, self.score, self.scale, n_fb, props)
+
+
+def calibrate_conf_min(mlp: ProposalMLP, grid: RemlGrid, Y_val: torch.Tensor, best_val: torch.Tensor,
+                       target: float = 0.999) -> float:
+    """Smallest-fallback confidence threshold whose local-search agreement with
+    the global argmin on a validation set is >= target.  Sets mlp.conf_min."""
+    mlp.conf_min = -math.inf
+    res = LocalSearcher(grid, mlp, Y_val.shape[0]).run(Y_val)
+    c = res.props[:, 2].detach()
+    ok = (res.best == best_val)
+    # voxels that went to the full grid are already right; find the threshold
+    # on c that also sends enough of the wrong ones there
+    order = torch.argsort(c)
+    wrong = (~ok).to(torch.float64)[order]
+    n = c.numel()
+    cum_wrong_below = torch.cumsum(wrong, 0)  # wrong voxels with c <= c[order[i]]
+    total_wrong = float(wrong.sum())
+    need = total_wrong - (1.0 - target) * n
+    if need <= 0:
+        mlp.conf_min = -math.inf
+        return mlp.conf_min
+    pos = int(torch.searchsorted(cum_wrong_below, torch.tensor(need, dtype=torch.float64, device=c.device)))
+    pos = min(pos, n - 1)
+    mlp.conf_min = float(c[order[pos]].item()) + 1e-6
+    return mlp.conf_min

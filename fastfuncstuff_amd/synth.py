@@ -76,6 +76,26 @@ def arma11_noise(rng: np.random.Generator, a: np.ndarray, b: np.ndarray, T: int,
     return out.astype(dtype)
 
 
+def arma11_noise_torch(gen, a, b, T: int):
+    """arma11_noise on the device with a torch.Generator (same model)."""
+    import torch
+
+    V = a.shape[0]
+    sig2 = (1 - a * a) / (1 + 2 * a * b + b * b)
+    sd = torch.sqrt(sig2)
+    out = torch.empty(V, T, dtype=torch.float32, device=a.device)
+    e_prev = torch.randn(V, generator=gen, device=a.device) * sd
+    x = e_prev + torch.sqrt(torch.clamp(1.0 - sig2, min=0.0)) * torch.randn(V, generator=gen, device=a.device)
+    out[:, 0] = x
+    z = torch.randn(T - 1, V, generator=gen, device=a.device) * sd
+    for t in range(1, T):
+        e = z[t - 1]
+        x = a * x + e + b * e_prev
+        out[:, t] = x
+        e_prev = e
+    return out
+
+
 @dataclass
 class Workload:
     name: str
@@ -124,11 +144,14 @@ def plain_config(V: int = 20_000, T: int = 200, tr: float = 2.0, seed: int = 1, 
     return Workload("plain_T200_p6", Y, X, tr, None, None, a, b, beta, ag, bg)
 
 
-def gpu_config(V: int = 200_000, T: int = 400, tr: float = 2.0, seed: int = 2, censor_frac: float = 0.05) -> Workload:
+def gpu_config(V: int = 200_000, T: int = 400, tr: float = 2.0, seed: int = 2, censor_frac: float = 0.05,
+               noise_seed: int | None = None, chunk: int = 250_000, device=None) -> Workload:
     """p=12: 4 HRF-convolved boxcar task regressors (20 random 10 s blocks each),
     6 AR(1) random-walk motion-like columns, constant + linear; 5% of the TRs
     censored uniformly at random (the retained series keeps its true time
-    index); ARMA(1,1) noise as in the plain config."""
+    index); ARMA(1,1) noise as in the plain config.  ``seed`` fixes the design
+    and censoring; ``noise_seed`` (default: ``seed``) the voxels.  With
+    ``device`` the voxels are drawn by torch on that device (Y stays there)."""
     rng = np.random.default_rng(seed)
     hrf = double_gamma_hrf(tr)
     cols = []
@@ -147,11 +170,36 @@ def gpu_config(V: int = 200_000, T: int = 400, tr: float = 2.0, seed: int = 2, c
     n_cens = int(round(censor_frac * T))
     censored = np.sort(rng.choice(T, n_cens, replace=False)) if n_cens else np.zeros(0, int)
     keep = np.setdiff1d(np.arange(T), censored)
+    Xk = Xf[keep]
+    ag, bg = config_grid()
+    if device is not None:
+        import torch
+
+        g = torch.Generator(device=device)
+        g.manual_seed(seed if noise_seed is None else noise_seed)
+        Y = torch.empty(V, keep.size, dtype=torch.float32, device=device)
+        Xd = torch.from_numpy(Xf).to(device=device, dtype=torch.float32)
+        keep_d = torch.from_numpy(keep).to(device)
+        a_all = torch.empty(V, device=device)
+        b_all = torch.empty(V, device=device)
+        for c0 in range(0, V, chunk):
+            c1 = min(V, c0 + chunk)
+            n = c1 - c0
+            a = torch.rand(n, generator=g, device=device) * 0.7 + 0.1
+            b = torch.rand(n, generator=g, device=device) * 0.8 - 0.5
+            beta = torch.randn(n, Xf.shape[1], generator=g, device=device)
+            noise = arma11_noise_torch(g, a, b, T)
+            Y[c0:c1] = (beta @ Xd.T + noise)[:, keep_d]
+            a_all[c0:c1], b_all[c0:c1] = a, b
+        return Workload(f"censored_T{T}_p12", Y, Xk, tr, [0], keep.astype(np.int64), a_all, b_all, None, ag, bg)
+    if noise_seed is not None:
+        rng = np.random.default_rng(noise_seed)
     a = rng.uniform(0.1, 0.8, V)
     b = rng.uniform(-0.5, 0.3, V)
     beta = rng.standard_normal((V, Xf.shape[1]))
-    noise = arma11_noise(rng, a, b, T)
-    Yf = (beta @ Xf.T + noise).astype(np.float32)
-    ag, bg = config_grid()
-    return Workload(f"censored_T{T}_p12", np.ascontiguousarray(Yf[:, keep]), Xf[keep], tr, [0],
-                    keep.astype(np.int64), a, b, beta, ag, bg)
+    Y = np.empty((V, keep.size), dtype=np.float32)
+    for c0 in range(0, V, chunk):
+        c1 = min(V, c0 + chunk)
+        noise = arma11_noise(rng, a[c0:c1], b[c0:c1], T)
+        Y[c0:c1] = beta[c0:c1] @ Xk.T + noise[:, keep]
+    return Workload(f"censored_T{T}_p12", Y, Xk, tr, [0], keep.astype(np.int64), a, b, beta, ag, bg)

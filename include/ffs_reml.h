@@ -78,7 +78,7 @@ int ffs_reml_plan_dims(const ffs_reml_plan* plan, int32_t* Tpad, int32_t* vox_pa
  * as are rows T..Tpad-1.  ldYt >= ncols rounded up to the padding unit.
  * scale[v] = unbiased std clamped at 1e-8 (1 when normalize == 0), indexed by
  * voxel; feats (nullable, V x 10) receives lag-1..10 autocorrelations of the
- * residual.
+ * residual.  Yt may be null (statistics / features only).
  */
 int ffs_reml_prep(const ffs_reml_plan* plan, const float* Y, int64_t ldY, int64_t V, const int32_t* rows,
                   int64_t ncols, float* Yt, int64_t ldYt, float* scale, int32_t normalize, float* feats,
@@ -133,6 +133,15 @@ int ffs_reml_ljung_box(const double* resid_w, int64_t ldR, int64_t V, int32_t n,
  * zero-filled.
  */
 int ffs_reml_local_workspace(const ffs_reml_plan* plan, int64_t V, int64_t* bytes);
+/* The three stages of ffs_reml_local_search (same workspace, same stream):
+ * window centres + grouping + grouped residualisation; window energies;
+ * window projection + score + local argmin. */
+int ffs_reml_local_prepare(const ffs_reml_plan* plan, const float* Y, int64_t ldY, int64_t V, const float* props,
+                           float conf_min, int32_t* need_full, float* scale, void* workspace, int64_t ws_bytes,
+                           void* stream);
+int ffs_reml_local_energy(const ffs_reml_plan* plan, int64_t V, void* workspace, int64_t ws_bytes, void* stream);
+int ffs_reml_local_project(const ffs_reml_plan* plan, int64_t V, int32_t* best, float* best_score,
+                           int32_t* need_full, void* workspace, int64_t ws_bytes, void* stream);
 int ffs_reml_local_search(const ffs_reml_plan* plan, const float* Y, int64_t ldY, int64_t V, const float* props,
                           float conf_min, int32_t* best, float* best_score, int32_t* need_full, float* scale,
                           void* workspace, int64_t ws_bytes, void* stream);
