@@ -44,7 +44,7 @@ def parse():
     ap.add_argument("--voxels", type=int, default=200_000)
     ap.add_argument("--T", type=int, default=400)
     ap.add_argument("--train-voxels", type=int, default=2_000_000)
-    ap.add_argument("--epochs", type=int, default=4)
+    ap.add_argument("--epochs", type=int, default=12)
     ap.add_argument("--target-agreement", type=float, default=0.9995)
     ap.add_argument("--cpu-sample", type=int, default=2048, help="voxels in the timed CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")

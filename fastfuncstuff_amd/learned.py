@@ -78,8 +78,20 @@ class ProposalMLP:
         return loss, grad
 
     def train(self, feats: torch.Tensor, target: torch.Tensor, epochs: int = 8, batch: int = 4096,
-              lr: float = 2e-3, lr_final: float = 1e-4, seed: int = 0, log_every: int = 0) -> TrainLog:
-        """Adam on the device; mini-batches from a seeded host permutation."""
+              lr: float = 2e-3, lr_final: float = 1e-4, seed: int = 0, log_every: int = 0,
+              standardize: bool = True) -> TrainLog:
+        """Adam on the device; mini-batches from a seeded host permutation.
+
+        With ``standardize`` the network trains on z-scored features and the
+        standardisation is folded exactly into the first layer afterwards
+        (W1 <- W1 / sd, b1 <- b1 - (mean / sd) W1), so inference takes raw
+        features."""
+        if standardize:
+            mu = feats.mean(dim=0)
+            sd = feats.std(dim=0).clamp(min=1e-6)
+            log = self.train((feats - mu) / sd, target, epochs, batch, lr, lr_final, seed, log_every, False)
+            self._fold_standardization(mu.double().cpu().numpy(), sd.double().cpu().numpy())
+            return log
         lib = _lib.load()
         dev = self.device
         n = feats.shape[0]
@@ -112,6 +124,14 @@ class ProposalMLP:
         log.seconds = time.perf_counter() - t0
         log.steps = step
         return log
+
+    def _fold_standardization(self, mu: np.ndarray, sd: np.ndarray) -> None:
+        p = self.params.detach().cpu().double().numpy()
+        W1 = p[:N_IN * N_H].reshape(N_IN, N_H) / sd[:, None]
+        b1 = p[N_IN * N_H:N_IN * N_H + N_H] - mu @ W1
+        p[:N_IN * N_H] = W1.reshape(-1)
+        p[N_IN * N_H:N_IN * N_H + N_H] = b1
+        self.params = torch.from_numpy(p.astype(np.float32)).to(self.device)
 
     def state(self) -> np.ndarray:
         return self.params.detach().cpu().numpy().copy()

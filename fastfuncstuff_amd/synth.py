@@ -120,7 +120,8 @@ def config_grid() -> tuple[np.ndarray, np.ndarray]:
     return g.copy(), g.copy()
 
 
-def plain_config(V: int = 20_000, T: int = 200, tr: float = 2.0, seed: int = 1, snr_range=None) -> Workload:
+def plain_config(V: int = 20_000, T: int = 200, tr: float = 2.0, seed: int = 1, snr_range=None,
+                 noise_seed: int | None = None) -> Workload:
     """Two task boxcar regressors (20 random-onset 10 s blocks each) + Legendre
     0-3 baseline (p=6), ARMA a~U(0.1,0.8), b~U(-0.5,0.3), betas N(0,1).
     ``snr_range=(lo, hi)``: per-voxel task amplitude log-uniform in that range."""
@@ -132,6 +133,8 @@ def plain_config(V: int = 20_000, T: int = 200, tr: float = 2.0, seed: int = 1, 
         on = np.sort(rng.uniform(0, dur - 10.0, 20))
         cols.append(boxcar_regressor(on, 10.0, T, tr, hrf))
     X = np.concatenate([np.stack(cols, 1), legendre_baseline(T, 3)], 1)
+    if noise_seed is not None:
+        rng = np.random.default_rng(noise_seed)
     a = rng.uniform(0.1, 0.8, V)
     b = rng.uniform(-0.5, 0.3, V)
     beta = rng.standard_normal((V, X.shape[1]))
