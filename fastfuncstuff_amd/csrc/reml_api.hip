@@ -164,7 +164,7 @@ extern "C" int ffs_reml_plan_create(ffs_reml_plan** out, int32_t T, int32_t p, c
     if ((rc = P->alloc(&soa, (size_t)K * 3 * T)) || (rc = P->alloc(&coef64, (size_t)K * 3 * T)) ||
         (rc = P->alloc(&Xw64, (size_t)K * T * p)) || (rc = P->alloc(&G64, (size_t)K * T * p)) ||
         (rc = P->alloc(&Lx, (size_t)K * p * p)) || (rc = P->alloc(&Ainv, (size_t)K * p * p)) ||
-        (rc = P->alloc(&Fm, (size_t)K * nt * nt)) || (rc = P->alloc(&work, (size_t)K * 2 * p * p)) ||
+        (rc = P->alloc(&Fm, (size_t)K * p * p)) || (rc = P->alloc(&work, (size_t)K * 2 * p * p)) ||
         (rc = P->alloc(&ld, K)) || (rc = P->alloc(&ldR, K)) || (rc = P->alloc(&ldX, K)) ||
         (rc = P->alloc(&G32, (size_t)d.n_tiles * d.Tpad * BN)) ||
         (rc = P->alloc(&ecoef, (size_t)d.n_chunks * T * 2 * kEnergyNB)) ||
@@ -268,7 +268,7 @@ template <int PM>
 static void launch_prep(const ffs_reml_plan* P, const float* Y, int64_t ldY, int V, const int32_t* rows, int ncols,
                         int ncols_pad, float* Yt, int64_t ldYt, float* scale, int normalize, float* feats,
                         hipStream_t s) {
-    const dim3 grid((ncols_pad + 255) / 256), block(256);
+    const dim3 grid((ncols_pad + 63) / 64), block(64);
     if (feats)
         hipLaunchKernelGGL((k_prep<PM, 10>), grid, block, 0, s, P->d.T, P->d.p, Y, ldY, V, rows, ncols, ncols_pad,
                            P->pinvT, P->d.X, normalize, Yt, ldYt, scale, feats);
@@ -363,7 +363,7 @@ template <int PM>
 static void launch_gls(const ffs_reml_plan* P, const float* Y, int64_t ldY, const float* scale, const int32_t* perm,
                        const int32_t* best, int V, float* betas, float* tstats, float* fstats, float* sigma2,
                        float* r2, double* resid_w, int64_t ldR, float* resid, float* pred, hipStream_t s) {
-    hipLaunchKernelGGL(k_gls<PM>, dim3((V + kGlsBlock - 1) / kGlsBlock), dim3(kGlsBlock), 0, s, P->d, Y, ldY, scale,
+    hipLaunchKernelGGL(k_gls<PM>, dim3((V + 63) / 64), dim3(64), 0, s, P->d, Y, ldY, scale,
                        perm, best, V, betas, tstats, fstats, sigma2, r2, resid_w, ldR, resid, pred);
 }
 

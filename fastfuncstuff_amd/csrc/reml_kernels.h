@@ -47,7 +47,7 @@ struct PlanDev {
     const double* G64;         // [K][T][p]  R^-1 X Lx^-T
     const double* Lx;          // [K][p][p]  chol(X'R^-1X)
     const double* Ainv;        // [K][p][p]  (X'R^-1X)^-1
-    const double* Fm;          // [K][n_task][n_task] F-statistic quadratic form
+    const double* Fm;          // [K][p][p] F-statistic quadratic form (zero outside the task columns)
     const int32_t* task_idx;   // [n_task]
     const float* G32;          // [n_tiles][Tpad][BN] projection operand
     const float* ecoef;        // [n_chunks][T][2*NB] (alpha_t, 1/v_{t+1}) per chunk slot

@@ -76,10 +76,13 @@ __global__ void __launch_bounds__(64) k_tables(int T, int Tpad, int p, int K, co
         L[j * p + i] = s;
     }
     __syncthreads();
-    double* F = Fm + (size_t)k * n_task * n_task;
+    // F-statistic quadratic form, embedded in p x p (zero outside the task columns).
+    double* F = Fm + (size_t)k * p * p;
     if (tid == 0) {
         if (full_f)
             for (int i = 0; i < p * p; ++i) F[i] = L[i];
+        else
+            for (int i = 0; i < p * p; ++i) F[i] = 0.0;
         if (cholesky_inplace(p, L) != 0) {
             sh_fail = 1;
         } else {
@@ -89,12 +92,22 @@ __global__ void __launch_bounds__(64) k_tables(int T, int Tpad, int p, int K, co
             cholesky_inverse(p, L, Ai, wk);
             if (!full_f) {
                 // inv(A^-1[task, task] + 1e-8 I)   (glm/arma.py:4888-4896)
+                // At (wk) -> inverse in F's storage (work: W2) -> copied to W2 ->
+                // F zeroed and the block embedded at the task positions.
                 double* At = wk;
                 double* W2 = wk + p * p;
                 for (int i = 0; i < n_task; ++i)
                     for (int j = 0; j < n_task; ++j)
                         At[i * n_task + j] = Ai[task_idx[i] * p + task_idx[j]] + (i == j ? 1e-8 : 0.0);
-                if (gauss_jordan_inverse(n_task, At, F, W2) != 0) sh_fail = 2;
+                if (gauss_jordan_inverse(n_task, At, F, W2) != 0) {
+                    sh_fail = 2;
+                } else {
+                    for (int q = 0; q < n_task * n_task; ++q) W2[q] = F[q];
+                    for (int q = 0; q < p * p; ++q) F[q] = 0.0;
+                    for (int a2 = 0; a2 < n_task; ++a2)
+                        for (int b2 = 0; b2 < n_task; ++b2)
+                            F[task_idx[a2] * p + task_idx[b2]] = W2[a2 * n_task + b2];
+                }
             }
         }
     }
@@ -189,32 +202,81 @@ __global__ void k_local_tables(int T, int K, const double* __restrict__ soa, flo
 // write the residual time-major (Yt[t][v]) for the search kernels.
 // Optional: lag-1..nfeat autocorrelation of the residual (proposal features).
 // ---------------------------------------------------------------------------
-template <int PM, int NF>
-__global__ void __launch_bounds__(256) k_prep(int T, int p, const float* __restrict__ Y, int64_t ldY, int V,
-                                              const int32_t* __restrict__ rows, int ncols, int ncols_pad,
-                                              const double* __restrict__ pinvT, const double* __restrict__ X,
-                                              int normalize, float* __restrict__ Yt, int64_t ldYt,
-                                              float* __restrict__ scale, float* __restrict__ feats) {
-    const int col = blockIdx.x * blockDim.x + threadIdx.x;
-    if (col >= ncols_pad) return;
-    const int v = (col < ncols) ? (rows ? rows[col] : col) : -1;
-    if (v < 0 || v >= V) {
-        if (Yt)
-            for (int t = 0; t < T; ++t) Yt[(size_t)t * ldYt + col] = 0.f;
-        return;
+constexpr int kStageChunk = 32;
+
+// Stage samples [t0, t0+kStageChunk) of the wave's 64 rows (row index myv per
+// lane, -1 = none) into ys[row][..] (row pitch kStageChunk+1).  Two rows per
+// load instruction, 128 contiguous bytes per row; all 32 loads of a lane are
+// issued before the first LDS write so their latencies overlap.
+__device__ inline void stage_rows(const float* __restrict__ Y, int64_t ldY, int T, int t0, int myv, float* ys) {
+    const int lane = threadIdx.x & 63;
+    const int half = lane / kStageChunk, c = lane % kStageChunk;
+    const int t = t0 + c;
+    float vals[32];
+#pragma unroll
+    for (int r2 = 0; r2 < 32; ++r2) {
+        const int vr = __shfl(myv, 2 * r2 + half);
+        vals[r2] = (vr >= 0 && t < T) ? Y[(size_t)vr * ldY + t] : 0.f;
     }
-    const float* y = Y + (size_t)v * ldY;
+#pragma unroll
+    for (int r2 = 0; r2 < 32; ++r2) ys[(2 * r2 + half) * (kStageChunk + 1) + c] = vals[r2];
+}
+
+// Stage rows [t0, t0+kStageChunk) of a row-major (T x w) float64 table into
+// dst[i*w + j] (all lanes of the wave cooperate; coalesced; w <= WMAX); loads
+// are issued before the LDS writes.
+template <int WMAX>
+__device__ inline void stage_table(const double* __restrict__ src, int T, int w, int t0, double* dst) {
+    const int lane = threadIdx.x & 63;
+    const int n = min(kStageChunk, T - t0) * w;
+    const double* s0 = src + (size_t)t0 * w;
+    constexpr int NI = (kStageChunk * WMAX + 63) / 64;
+    double vals[NI];
+#pragma unroll
+    for (int i = 0; i < NI; ++i) {
+        const int q = lane + 64 * i;
+        vals[i] = q < n ? s0[q] : 0.0;
+    }
+#pragma unroll
+    for (int i = 0; i < NI; ++i) {
+        const int q = lane + 64 * i;
+        if (q < n) dst[q] = vals[i];
+    }
+}
+
+// One wave per 64 output columns; the rows are staged through LDS.
+template <int PM, int NF>
+__global__ void __launch_bounds__(64) k_prep(int T, int p, const float* __restrict__ Y, int64_t ldY, int V,
+                                             const int32_t* __restrict__ rows, int ncols, int ncols_pad,
+                                             const double* __restrict__ pinvT, const double* __restrict__ X,
+                                             int normalize, float* __restrict__ Yt, int64_t ldYt,
+                                             float* __restrict__ scale, float* __restrict__ feats) {
+    __shared__ float ys[64 * (kStageChunk + 1)];
+    __shared__ double tab[kStageChunk * PM];
+    const int lane = threadIdx.x & 63;
+    const int col = blockIdx.x * 64 + lane;
+    if (blockIdx.x * 64 >= ncols_pad) return;
+    int v = (col < ncols) ? (rows ? rows[col] : col) : -1;
+    if (v >= V) v = -1;
+    const float* yrow = ys + lane * (kStageChunk + 1);
     double s1 = 0.0, s2 = 0.0, c[PM];
 #pragma unroll
     for (int j = 0; j < PM; ++j) c[j] = 0.0;
-    for (int t = 0; t < T; ++t) {
-        const double yt = (double)y[t];
-        s1 += yt;
-        s2 += yt * yt;
-        const double* pr = pinvT + (size_t)t * p;
+    for (int t0 = 0; t0 < T; t0 += kStageChunk) {
+        __syncthreads();
+        stage_rows(Y, ldY, T, t0, v, ys);
+        stage_table<PM>(pinvT, T, p, t0, tab);
+        __syncthreads();
+        const int n = min(kStageChunk, T - t0);
+        for (int i = 0; i < n; ++i) {
+            const double yt = (double)yrow[i];
+            s1 += yt;
+            s2 += yt * yt;
+            const double* pr = tab + i * p;
 #pragma unroll
-        for (int j = 0; j < PM; ++j)
-            if (j < p) c[j] += pr[j] * yt;
+            for (int j = 0; j < PM; ++j)
+                if (j < p) c[j] += pr[j] * yt;
+        }
     }
     double s = 1.0;
     if (normalize) {
@@ -234,23 +296,34 @@ __global__ void __launch_bounds__(256) k_prep(int T, int p, const float* __restr
         hist[k] = 0.f;
         acf[k] = 0.0;
     }
-    for (int t = 0; t < T; ++t) {
-        double r = (double)y[t] * inv;
-        const double* xr = X + (size_t)t * p;
+    if (Yt || NF > 0) {
+        for (int t0 = 0; t0 < T; t0 += kStageChunk) {
+            __syncthreads();
+            stage_rows(Y, ldY, T, t0, v, ys);
+            stage_table<PM>(X, T, p, t0, tab);
+            __syncthreads();
+            const int n = min(kStageChunk, T - t0);
+            for (int i = 0; i < n; ++i) {
+                const int t = t0 + i;
+                double r = (double)yrow[i] * inv;
+                const double* xr = tab + i * p;
 #pragma unroll
-        for (int j = 0; j < PM; ++j)
-            if (j < p) r -= xr[j] * c[j];
-        const float rf = (float)r;
-        if (Yt) Yt[(size_t)t * ldYt + col] = rf;
-        if (NF > 0) {
-            acf0 += (double)rf * rf;
+                for (int j = 0; j < PM; ++j)
+                    if (j < p) r -= xr[j] * c[j];
+                const float rf = v >= 0 ? (float)r : 0.f;
+                if (Yt && col < ncols_pad) Yt[(size_t)t * ldYt + col] = rf;
+                if (NF > 0) {
+                    acf0 += (double)rf * rf;
 #pragma unroll
-            for (int k = 0; k < NF; ++k) acf[k] += (double)rf * hist[k];
+                    for (int k = 0; k < NF; ++k) acf[k] += (double)rf * hist[k];
 #pragma unroll
-            for (int k = NF - 1; k > 0; --k) hist[k] = hist[k - 1];
-            hist[0] = rf;
+                    for (int k = NF - 1; k > 0; --k) hist[k] = hist[k - 1];
+                    hist[0] = rf;
+                }
+            }
         }
     }
+    if (v < 0) return;
     if (scale) scale[v] = (float)s;
     if (NF > 0 && feats) {
 #pragma unroll
@@ -605,47 +678,46 @@ __global__ void k_scatter(const int32_t* __restrict__ best, int V, int K, int32_
 //   F = beta_t' Fm beta_t / (sigma2 n_task + 1e-10);  R^2 = CoD(y, X beta).
 // y is the std-normalised series (scale = 1 for the FP64 variant); betas and
 // sigma2 are unscaled on output (glm/arma.py:5827-5862).
+// One wave per 64 consecutive voxels of the pair-sorted order; their rows are
+// staged through LDS in 64-sample chunks with coalesced loads.  When the 64
+// voxels share one pair (the usual case after the sort) the pair's tables are
+// read through wave-uniform (scalar) loads.
 // ---------------------------------------------------------------------------
-template <int PM>
-__global__ void __launch_bounds__(kGlsBlock) k_gls(PlanDev P, const float* __restrict__ Y, int64_t ldY,
-                                             const float* __restrict__ scale, const int32_t* __restrict__ perm,
-                                             const int32_t* __restrict__ best, int V, float* __restrict__ betas,
-                                             float* __restrict__ tstats, float* __restrict__ fstats,
-                                             float* __restrict__ sigma2, float* __restrict__ r2,
-                                             double* __restrict__ resid_w, int64_t ldR, float* __restrict__ resid,
-                                             float* __restrict__ pred_out) {
-    __shared__ double gls_beta[kGlsBlock * PM];
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= V) return;
-    const int v = perm ? perm[i] : i;
-    const int k = best[v];
+constexpr int kGlsChunk = kStageChunk;
+
+template <int PM, bool UNIFORM>
+__device__ inline void gls_body(const PlanDev& P, const float* __restrict__ Y, int64_t ldY, int v, int k, double s,
+                                float* ys, double* tab, float* __restrict__ betas, float* __restrict__ tstats,
+                                float* __restrict__ fstats, float* __restrict__ sigma2, float* __restrict__ r2,
+                                double* __restrict__ resid_w, int64_t ldR, float* __restrict__ resid,
+                                float* __restrict__ pred_out) {
+    const int lane = threadIdx.x & 63;
     const int T = P.T, p = P.p;
-    if (k < 0 || k >= P.K) {  // no valid pair: NaN statistics
-        for (int j = 0; j < p; ++j) {
-            betas[(size_t)v * p + j] = NAN;
-            tstats[(size_t)v * p + j] = NAN;
-        }
-        fstats[v] = sigma2[v] = r2[v] = NAN;
-        return;
-    }
-    const double s = scale ? (double)scale[v] : 1.0;
+    const bool active = v >= 0;
+    const int kk = UNIFORM ? __builtin_amdgcn_readfirstlane(k) : (active ? k : 0);
     const double inv = 1.0 / s;
-    const float* y = Y + (size_t)v * ldY;
-    const double* G = P.G64 + (size_t)k * T * p;
-    const double* Xw = P.Xw64 + (size_t)k * T * p;
-    const double* cf = P.coef64 + (size_t)k * T * 3;
-    const double* L = P.Lx + (size_t)k * p * p;
+    const double* __restrict__ G = P.G64 + (size_t)kk * T * p;
+    const double* __restrict__ Xw = P.Xw64 + (size_t)kk * T * p;
+    const double* __restrict__ cf = P.coef64 + (size_t)kk * T * 3;
+    const double* __restrict__ L = P.Lx + (size_t)kk * p * p;
     double h[PM], be[PM];
 #pragma unroll
     for (int j = 0; j < PM; ++j) h[j] = 0.0;
     double sy = 0.0;
-    for (int t = 0; t < T; ++t) {
-        const double yt = (double)y[t] * inv;
-        sy += yt;
-        const double* gr = G + (size_t)t * p;
+    for (int t0 = 0; t0 < T; t0 += kGlsChunk) {
+        __syncthreads();
+        stage_rows(Y, ldY, T, t0, v, ys);
+        if (UNIFORM) stage_table<PM>(G, T, p, t0, tab);
+        __syncthreads();
+        const int n = min(kGlsChunk, T - t0);
+        for (int i = 0; i < n; ++i) {
+            const double yt = (double)ys[lane * (kGlsChunk + 1) + i] * inv;
+            sy += yt;
+            const double* gr = UNIFORM ? tab + i * p : G + (size_t)(t0 + i) * p;
 #pragma unroll
-        for (int j = 0; j < PM; ++j)
-            if (j < p) h[j] += gr[j] * yt;
+            for (int j = 0; j < PM; ++j)
+                if (j < p) h[j] = fma(gr[j], yt, h[j]);
+        }
     }
     // beta = Lx^-T h  (back substitution on the upper factor Lx')
 #pragma unroll
@@ -663,34 +735,54 @@ __global__ void __launch_bounds__(kGlsBlock) k_gls(PlanDev P, const float* __res
     const double mean = sy / T;
     double ssw = 0.0, ssres = 0.0, sstot = 0.0;
     double yp = 0.0, e = 0.0;
-    for (int t = 0; t < T; ++t) {
-        const double yt = (double)y[t] * inv;
-        if (t == 0 || P.gap[t - 1] <= 0)
-            e = yt;
-        else
-            e = (yt - cf[3 * (t - 1) + 1] * yp) + cf[3 * (t - 1) + 2] * e;
-        yp = yt;
-        double r = e * cf[3 * t];
-        const double* xw = Xw + (size_t)t * p;
-        const double* xr = P.X + (size_t)t * p;
-        double pred = 0.0;
+    double* tx = tab + kGlsChunk * PM;       // X rows
+    double* tc = tab + 2 * kGlsChunk * PM;   // coefficient rows (3 per t)
+    for (int t0 = 0; t0 < T; t0 += kGlsChunk) {
+        __syncthreads();
+        stage_rows(Y, ldY, T, t0, v, ys);
+        stage_table<PM>(P.X, T, p, t0, tx);
+        if (UNIFORM) {
+            stage_table<PM>(Xw, T, p, t0, tab);
+            stage_table<3>(cf, T, 3, t0, tc);
+        }
+        __syncthreads();
+        const int n = min(kGlsChunk, T - t0);
+        for (int i = 0; i < n; ++i) {
+            const int t = t0 + i;
+            const double yt = (double)ys[lane * (kGlsChunk + 1) + i] * inv;
+            // coefficients of step t-1 (previous chunk's last row when i == 0)
+            const double* cprev = (UNIFORM && i > 0) ? tc + 3 * (i - 1) : cf + 3 * (t - 1);
+            const double* ccur = UNIFORM ? tc + 3 * i : cf + 3 * t;
+            if (t == 0 || P.gap[t - 1] <= 0)
+                e = yt;
+            else
+                e = (yt - cprev[1] * yp) + cprev[2] * e;
+            yp = yt;
+            double r = e * ccur[0];
+            const double* xw = UNIFORM ? tab + i * p : Xw + (size_t)t * p;
+            const double* xr = tx + i * p;
+            double pred = 0.0;
 #pragma unroll
-        for (int j = 0; j < PM; ++j)
-            if (j < p) {
-                r -= xw[j] * be[j];
-                pred += xr[j] * be[j];
+            for (int j = 0; j < PM; ++j)
+                if (j < p) {
+                    r -= xw[j] * be[j];
+                    pred += xr[j] * be[j];
+                }
+            ssw += r * r;
+            const double dr = yt - pred, dm = yt - mean;
+            ssres += dr * dr;
+            sstot += dm * dm;
+            if (active) {
+                if (resid_w) resid_w[(size_t)v * ldR + t] = r * s;
+                if (resid) resid[(size_t)v * ldR + t] = (float)(dr * s);
+                if (pred_out) pred_out[(size_t)v * ldR + t] = (float)(pred * s);
             }
-        ssw += r * r;
-        const double dr = yt - pred, dm = yt - mean;
-        ssres += dr * dr;
-        sstot += dm * dm;
-        if (resid_w) resid_w[(size_t)v * ldR + t] = r * s;
-        if (resid) resid[(size_t)v * ldR + t] = (float)(dr * s);
-        if (pred_out) pred_out[(size_t)v * ldR + t] = (float)(pred * s);
+        }
     }
+    if (!active) return;
     const int dof = max(1, T - p);
     const double sig = ssw / dof;
-    const double* Ai = P.Ainv + (size_t)k * p * p;
+    const double* Ai = P.Ainv + (size_t)kk * p * p;
 #pragma unroll
     for (int j = 0; j < PM; ++j)
         if (j < p) {
@@ -699,24 +791,55 @@ __global__ void __launch_bounds__(kGlsBlock) k_gls(PlanDev P, const float* __res
             betas[(size_t)v * p + j] = (float)(be[j] * s);
         }
     const int nt = P.n_task;
-    const double* F = P.Fm + (size_t)k * nt * nt;
-    double* sb = gls_beta + threadIdx.x * PM;
-#pragma unroll
-    for (int j = 0; j < PM; ++j) sb[j] = be[j];
+    const double* F = P.Fm + (size_t)kk * p * p;
     double quad = 0.0;
-    for (int a = 0; a < nt; ++a) {
+#pragma unroll
+    for (int a = 0; a < PM; ++a) {
+        if (a >= p) continue;
         double ra = 0.0;
-        for (int b2 = 0; b2 < nt; ++b2) ra += F[a * nt + b2] * sb[P.task_idx[b2]];
-        quad += sb[P.task_idx[a]] * ra;
+#pragma unroll
+        for (int b2 = 0; b2 < PM; ++b2)
+            if (b2 < p) ra += F[a * p + b2] * be[b2];
+        quad += be[a] * ra;
     }
     fstats[v] = (float)(quad / (sig * nt + 1e-10));
     sigma2[v] = (float)(sig * s * s);
-    double rr;
-    if (sstot <= 1e-10)
-        rr = 0.0;
+    r2[v] = (float)(sstot <= 1e-10 ? 0.0 : fmin(1.0 - ssres / sstot, 1.0));
+}
+
+template <int PM>
+__global__ void __launch_bounds__(64) k_gls(PlanDev P, const float* __restrict__ Y, int64_t ldY,
+                                            const float* __restrict__ scale, const int32_t* __restrict__ perm,
+                                            const int32_t* __restrict__ best, int V, float* __restrict__ betas,
+                                            float* __restrict__ tstats, float* __restrict__ fstats,
+                                            float* __restrict__ sigma2, float* __restrict__ r2,
+                                            double* __restrict__ resid_w, int64_t ldR, float* __restrict__ resid,
+                                            float* __restrict__ pred_out) {
+    __shared__ float ys[64 * (kGlsChunk + 1)];
+    __shared__ double tab[kGlsChunk * (2 * PM + 3)];
+    const int lane = threadIdx.x & 63;
+    const int i = blockIdx.x * 64 + lane;
+    int v = i < V ? (perm ? perm[i] : i) : -1;
+    int k = v >= 0 ? best[v] : -1;
+    if (v >= 0 && (k < 0 || k >= P.K)) {  // no valid pair: NaN statistics
+        for (int j = 0; j < P.p; ++j) {
+            betas[(size_t)v * P.p + j] = NAN;
+            tstats[(size_t)v * P.p + j] = NAN;
+        }
+        fstats[v] = sigma2[v] = r2[v] = NAN;
+        v = -1;
+    }
+    const double s = (v >= 0 && scale) ? (double)scale[v] : 1.0;
+    // all active lanes on one pair?
+    const int k0 = __builtin_amdgcn_readfirstlane(v >= 0 ? k : -1);
+    const bool same = (v < 0) || (k == k0);
+    const bool uniform = __all(same) && k0 >= 0;
+    if (uniform)
+        gls_body<PM, true>(P, Y, ldY, v, k0, s, ys, tab, betas, tstats, fstats, sigma2, r2, resid_w, ldR, resid,
+                           pred_out);
     else
-        rr = fmin(1.0 - ssres / sstot, 1.0);
-    r2[v] = (float)rr;
+        gls_body<PM, false>(P, Y, ldY, v, k, s, ys, tab, betas, tstats, fstats, sigma2, r2, resid_w, ldR, resid,
+                            pred_out);
 }
 
 // ---------------------------------------------------------------------------
