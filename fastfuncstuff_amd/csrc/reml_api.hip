@@ -191,12 +191,23 @@ extern "C" int ffs_reml_plan_create(ffs_reml_plan** out, int32_t T, int32_t p, c
     HIP_TRY(hipMemsetAsync(zero1, 0, sizeof(float) * T, s));
     HIP_TRY(hipMemsetAsync(zero2, 0, sizeof(float2) * T, s));
     P->local_ok = (p <= kLocalPP);
+    uint16_t *Ghf, *Glf, *Ghb, *Glb;
+    float* g16s;
+    const size_t n16 = (size_t)d.n_tiles * d.Tpad * BN;
+    if ((rc = P->alloc(&Ghf, n16)) || (rc = P->alloc(&Glf, n16)) || (rc = P->alloc(&Ghb, n16)) ||
+        (rc = P->alloc(&Glb, n16)) || (rc = P->alloc(&g16s, K)))
+        return rc;
+    HIP_TRY(hipMemsetAsync(Ghf, 0, 2 * n16, s));
+    HIP_TRY(hipMemsetAsync(Glf, 0, 2 * n16, s));
+    HIP_TRY(hipMemsetAsync(Ghb, 0, 2 * n16, s));
+    HIP_TRY(hipMemsetAsync(Glb, 0, 2 * n16, s));
     if (P->local_ok) {
         if ((rc = P->alloc(&Gp, (size_t)d.Tpad * K * kLocalPP))) return rc;
         HIP_TRY(hipMemsetAsync(Gp, 0, sizeof(float) * d.Tpad * K * kLocalPP, s));
     }
     hipLaunchKernelGGL(k_tables, dim3(K), dim3(64), 0, s, T, d.Tpad, p, K, dab, dwhiteflag, dgap, dX, d.n_task, dtask,
-                       full_f, BN, npt, soa, coef64, Xw64, G64, Lx, Ainv, Fm, work, ld, ldR, ldX, G32, Gp, dstatus);
+                       full_f, BN, npt, soa, coef64, Xw64, G64, Lx, Ainv, Fm, work, ld, ldR, ldX, G32, Gp, Ghf, Glf, Ghb, Glb, g16s,
+                       dstatus);
     HIP_TRY(hipGetLastError());
     hipLaunchKernelGGL(k_local_tables, dim3(K), dim3(128), 0, s, T, K, soa, lp, lc, ls0);
     HIP_TRY(hipGetLastError());
@@ -235,6 +246,12 @@ extern "C" int ffs_reml_plan_create(ffs_reml_plan** out, int32_t T, int32_t p, c
     d.Gp = Gp;
     d.zero1 = zero1;
     d.zero2 = zero2;
+    d.Gh_f16 = Ghf;
+    d.Gl_f16 = Glf;
+    d.Gh_bf16 = Ghb;
+    d.Gl_bf16 = Glb;
+    d.g16_unscale = g16s;
+    d.proj_mode = 0;
     P->pinvT = dpinv;
     P->ldR = ldR;
     P->ldX = ldX;
@@ -339,10 +356,23 @@ extern "C" int ffs_reml_project(const ffs_reml_plan* P, const float* Yt, int64_t
     const int64_t nvt = (V + kProjBM - 1) / kProjBM;
     const int64_t nb = nvt * P->d.n_tiles;
     if (nb > 0x7fffffff) return fail(-3, "project: grid too large; split the voxel batch");
-    if (P->d.BN == 64)
-        hipLaunchKernelGGL(k_project<64>, dim3((unsigned)nb), dim3(256), 0, s, P->d, Yt, ldYt, (int)V, S, ldS);
-    else
-        hipLaunchKernelGGL(k_project<96>, dim3((unsigned)nb), dim3(256), 0, s, P->d, Yt, ldYt, (int)V, S, ldS);
+    const dim3 g((unsigned)nb), b(256);
+    const int mode = P->d.proj_mode;
+    if (P->d.BN == 64) {
+        if (mode == 1)
+            hipLaunchKernelGGL((k_project16<64, true>), g, b, 0, s, P->d, Yt, ldYt, (int)V, S, ldS);
+        else if (mode == 2)
+            hipLaunchKernelGGL((k_project16<64, false>), g, b, 0, s, P->d, Yt, ldYt, (int)V, S, ldS);
+        else
+            hipLaunchKernelGGL(k_project<64>, g, b, 0, s, P->d, Yt, ldYt, (int)V, S, ldS);
+    } else {
+        if (mode == 1)
+            hipLaunchKernelGGL((k_project16<96, true>), g, b, 0, s, P->d, Yt, ldYt, (int)V, S, ldS);
+        else if (mode == 2)
+            hipLaunchKernelGGL((k_project16<96, false>), g, b, 0, s, P->d, Yt, ldYt, (int)V, S, ldS);
+        else
+            hipLaunchKernelGGL(k_project<96>, g, b, 0, s, P->d, Yt, ldYt, (int)V, S, ldS);
+    }
     HIP_TRY(hipGetLastError());
     return 0;
 }
@@ -558,5 +588,12 @@ extern "C" int ffs_reml_scatter_best(const int32_t* idx, int64_t n, const int32_
     hipLaunchKernelGGL(k_scatter_best, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, idx,
                        (int)n, best_sub, score_sub, best, score);
     HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+extern "C" int ffs_reml_plan_set_projection(ffs_reml_plan* P, int32_t mode) {
+    if (!P) return fail(-3, "plan_set_projection: null plan");
+    if (mode < 0 || mode > 2) return fail(-3, "plan_set_projection: mode must be 0 (fp32), 1 (bf16x3) or 2 (fp16x3)");
+    P->d.proj_mode = mode;
     return 0;
 }

@@ -50,6 +50,13 @@ struct PlanDev {
     const double* Fm;          // [K][p][p] F-statistic quadratic form (zero outside the task columns)
     const int32_t* task_idx;   // [n_task]
     const float* G32;          // [n_tiles][Tpad][BN] projection operand
+    // split 16-bit projection operand: [n_tiles][Tpad/8][BN][8] hi and lo halves
+    const uint16_t* Gh_f16;
+    const uint16_t* Gl_f16;
+    const uint16_t* Gh_bf16;
+    const uint16_t* Gl_bf16;
+    const float* g16_unscale;  // [K] 2^-2e_k: the fp16 operand of pair k holds G * 2^e_k
+    int proj_mode;             // 0 fp32 MFMA, 1 bf16 x3, 2 fp16 x3
     const float* ecoef;        // [n_chunks][T][2*NB] (alpha_t, 1/v_{t+1}) per chunk slot
     const float* ec;           // [n_chunks][T] c_t of the chunk's AR coefficient
     const float* es0;          // [n_chunks][NB] 1/v_0

@@ -20,6 +20,7 @@ namespace remlk {
 
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
 // ---------------------------------------------------------------------------
 // FP64 per-pair tables.  One 64-thread block per pair.
@@ -36,7 +37,10 @@ __global__ void __launch_bounds__(64) k_tables(int T, int Tpad, int p, int K, co
                                                double* __restrict__ Fm, double* __restrict__ work,
                                                double* __restrict__ ld, double* __restrict__ ldR_out,
                                                double* __restrict__ ldX_out, float* __restrict__ G32,
-                                               float* __restrict__ Gp, int* __restrict__ status) {
+                                               float* __restrict__ Gp, uint16_t* __restrict__ Ghf,
+                                               uint16_t* __restrict__ Glf, uint16_t* __restrict__ Ghb,
+                                               uint16_t* __restrict__ Glb, float* __restrict__ g16_unscale,
+                                               int* __restrict__ status) {
     const int k = blockIdx.x;
     if (k >= K) return;
     const int tid = threadIdx.x;
@@ -137,6 +141,41 @@ __global__ void __launch_bounds__(64) k_tables(int T, int Tpad, int p, int K, co
         const int t = idx / p, j = idx % p;
         Gt[(size_t)t * BN + j] = (float)g[idx];
     }
+    // split 16-bit operands, layout [tile][t/8][BN][8].  The fp16 halves hold
+    // G * 2^e with 2^e * max|G| in [2^13, 2^14): the high half keeps headroom
+    // below the fp16 maximum and the low half stays out of the subnormal range.
+    // The epilogue multiplies the sum of squares by 2^-2e (exact).
+    if (Ghf) {
+        __shared__ double sh_max[64];
+        double m = 0.0;
+        for (int idx = tid; idx < T * p; idx += 64) m = fmax(m, fabs(g[idx]));
+        sh_max[tid] = m;
+        __syncthreads();
+        for (int off = 32; off > 0; off >>= 1) {
+            if (tid < off) sh_max[tid] = fmax(sh_max[tid], sh_max[tid + off]);
+            __syncthreads();
+        }
+        int e2 = 0;
+        if (sh_max[0] > 0.0) frexp(sh_max[0], &e2);  // max in [2^(e2-1), 2^e2)
+        const int e = min(max(14 - e2, -60), 60);
+        if (tid == 0) g16_unscale[k] = (float)ldexp(1.0, -2 * e);
+        const size_t tbase = (size_t)tile * Tpad * BN;
+        for (int idx = tid; idx < T * p; idx += 64) {
+            const int t = idx / p, j = idx % p;
+            const int col = slot * p + j;
+            const size_t o = tbase + ((size_t)(t >> 3) * BN + col) * 8 + (t & 7);
+            const double gv = g[idx];
+            const double gs = ldexp(gv, e);
+            const _Float16 hf = (_Float16)gs;
+            const _Float16 lf = (_Float16)(gs - (double)hf);
+            const __bf16 hb = (__bf16)(float)gv;
+            const __bf16 lb = (__bf16)(float)(gv - (double)(float)hb);
+            Ghf[o] = __builtin_bit_cast(uint16_t, hf);
+            Glf[o] = __builtin_bit_cast(uint16_t, lf);
+            Ghb[o] = __builtin_bit_cast(uint16_t, hb);
+            Glb[o] = __builtin_bit_cast(uint16_t, lb);
+        }
+    }
     if (Gp && p <= kLocalPP) {
         for (int idx = tid; idx < T * p; idx += 64) {
             const int t = idx / p, j = idx % p;
@@ -204,22 +243,27 @@ __global__ void k_local_tables(int T, int K, const double* __restrict__ soa, flo
 // ---------------------------------------------------------------------------
 constexpr int kStageChunk = 32;
 
-// Stage samples [t0, t0+kStageChunk) of the wave's 64 rows (row index myv per
-// lane, -1 = none) into ys[row][..] (row pitch kStageChunk+1).  Two rows per
-// load instruction, 128 contiguous bytes per row; all 32 loads of a lane are
-// issued before the first LDS write so their latencies overlap.
-__device__ inline void stage_rows(const float* __restrict__ Y, int64_t ldY, int T, int t0, int myv, float* ys) {
-    const int lane = threadIdx.x & 63;
-    const int half = lane / kStageChunk, c = lane % kStageChunk;
-    const int t = t0 + c;
-    float vals[32];
+// Samples [t0, t0+kStageChunk) of one row into registers.  Full chunks of a
+// 16-byte aligned row use unconditional float4 loads, all issued before use;
+// the tail chunk and unaligned rows use guarded scalar loads.
+__device__ __forceinline__ void load_chunk(const float* __restrict__ row, int T, int t0, bool valid, bool vec4,
+                                           float (&yv)[kStageChunk]) {
+    if (valid && vec4 && t0 + kStageChunk <= T) {
+        const float4* r4 = reinterpret_cast<const float4*>(row + t0);
+        float4 q[kStageChunk / 4];
 #pragma unroll
-    for (int r2 = 0; r2 < 32; ++r2) {
-        const int vr = __shfl(myv, 2 * r2 + half);
-        vals[r2] = (vr >= 0 && t < T) ? Y[(size_t)vr * ldY + t] : 0.f;
+        for (int i = 0; i < kStageChunk / 4; ++i) q[i] = r4[i];
+#pragma unroll
+        for (int i = 0; i < kStageChunk / 4; ++i) {
+            yv[4 * i] = q[i].x;
+            yv[4 * i + 1] = q[i].y;
+            yv[4 * i + 2] = q[i].z;
+            yv[4 * i + 3] = q[i].w;
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < kStageChunk; ++i) yv[i] = (valid && t0 + i < T) ? row[t0 + i] : 0.f;
     }
-#pragma unroll
-    for (int r2 = 0; r2 < 32; ++r2) ys[(2 * r2 + half) * (kStageChunk + 1) + c] = vals[r2];
 }
 
 // Stage rows [t0, t0+kStageChunk) of a row-major (T x w) float64 table into
@@ -251,24 +295,27 @@ __global__ void __launch_bounds__(64) k_prep(int T, int p, const float* __restri
                                              const double* __restrict__ pinvT, const double* __restrict__ X,
                                              int normalize, float* __restrict__ Yt, int64_t ldYt,
                                              float* __restrict__ scale, float* __restrict__ feats) {
-    __shared__ float ys[64 * (kStageChunk + 1)];
     __shared__ double tab[kStageChunk * PM];
     const int lane = threadIdx.x & 63;
     const int col = blockIdx.x * 64 + lane;
     if (blockIdx.x * 64 >= ncols_pad) return;
     int v = (col < ncols) ? (rows ? rows[col] : col) : -1;
     if (v >= V) v = -1;
-    const float* yrow = ys + lane * (kStageChunk + 1);
+    const float* __restrict__ row = Y + (size_t)(v >= 0 ? v : 0) * ldY;
+    const bool vec4 = (ldY & 3) == 0;
+    float yrow[kStageChunk];
     double s1 = 0.0, s2 = 0.0, c[PM];
 #pragma unroll
     for (int j = 0; j < PM; ++j) c[j] = 0.0;
     for (int t0 = 0; t0 < T; t0 += kStageChunk) {
+        load_chunk(row, T, t0, v >= 0, vec4, yrow);
         __syncthreads();
-        stage_rows(Y, ldY, T, t0, v, ys);
         stage_table<PM>(pinvT, T, p, t0, tab);
         __syncthreads();
         const int n = min(kStageChunk, T - t0);
-        for (int i = 0; i < n; ++i) {
+#pragma unroll
+        for (int i = 0; i < kStageChunk; ++i) {
+            if (i >= n) break;
             const double yt = (double)yrow[i];
             s1 += yt;
             s2 += yt * yt;
@@ -298,12 +345,14 @@ __global__ void __launch_bounds__(64) k_prep(int T, int p, const float* __restri
     }
     if (Yt || NF > 0) {
         for (int t0 = 0; t0 < T; t0 += kStageChunk) {
+            load_chunk(row, T, t0, v >= 0, vec4, yrow);
             __syncthreads();
-            stage_rows(Y, ldY, T, t0, v, ys);
             stage_table<PM>(X, T, p, t0, tab);
             __syncthreads();
             const int n = min(kStageChunk, T - t0);
-            for (int i = 0; i < n; ++i) {
+#pragma unroll
+            for (int i = 0; i < kStageChunk; ++i) {
+                if (i >= n) break;
                 const int t = t0 + i;
                 double r = (double)yrow[i] * inv;
                 const double* xr = tab + i * p;
@@ -438,36 +487,36 @@ __global__ void __launch_bounds__(kEnergyVox* kEnergyWaves) k_energy(PlanDev P, 
 // Global -> register stage of the projection tiles (A: BK x BM from the
 // time-major residual, B: BK x BN from the tile-major operand).
 template <int BM, int BN, int BK, int A_V4, int B_V4>
-__device__ __forceinline__ void proj_gload(float4 (&ra)[A_V4], float4 (&rb)[B_V4], const float* __restrict__ Ag,
+__device__ __forceinline__ void proj_gload(f32x4 (&ra)[A_V4], f32x4 (&rb)[B_V4], const float* __restrict__ Ag,
                                            const float* __restrict__ Bg, int64_t ldY, int kt, int tid) {
     const int t0 = kt * BK;
 #pragma unroll
     for (int i = 0; i < A_V4; ++i) {
         const int idx = tid + i * 256;
         const int r = idx / (BM / 4), c4 = idx % (BM / 4);
-        ra[i] = *reinterpret_cast<const float4*>(Ag + (size_t)(t0 + r) * ldY + c4 * 4);
+        ra[i] = *reinterpret_cast<const f32x4*>(Ag + (size_t)(t0 + r) * ldY + c4 * 4);
     }
 #pragma unroll
     for (int i = 0; i < B_V4; ++i) {
-        const int idx = tid + i * 256;
-        if (idx < BK * BN / 4) {
-            const int r = idx / (BN / 4), c4 = idx % (BN / 4);
-            rb[i] = *reinterpret_cast<const float4*>(Bg + (size_t)(t0 + r) * BN + c4 * 4);
-        }
+        const int idx0 = tid + i * 256;
+        const int idx = idx0 < BK * BN / 4 ? idx0 : BK * BN / 4 - 1;  // clamped: unconditional load
+        const int r = idx / (BN / 4), c4 = idx % (BN / 4);
+        rb[i] = *reinterpret_cast<const f32x4*>(Bg + (size_t)(t0 + r) * BN + c4 * 4);
     }
 }
 
 template <int BM, int BN, int BK, int A_V4, int B_V4>
-__device__ __forceinline__ void proj_sstore(const float4 (&ra)[A_V4], const float4 (&rb)[B_V4], float* stage,
+__device__ __forceinline__ void proj_sstore(const f32x4 (&ra)[A_V4], const f32x4 (&rb)[B_V4], float* stage,
                                             int tid) {
     float* As = stage;
     float* Bs = stage + BK * BM;
 #pragma unroll
-    for (int i = 0; i < A_V4; ++i) *reinterpret_cast<float4*>(As + (tid + i * 256) * 4) = ra[i];
+    for (int i = 0; i < A_V4; ++i) *reinterpret_cast<f32x4*>(As + (tid + i * 256) * 4) = ra[i];
+    constexpr bool FULL = (BK * BN / 4) % 256 == 0;
 #pragma unroll
     for (int i = 0; i < B_V4; ++i) {
         const int idx = tid + i * 256;
-        if (idx < BK * BN / 4) *reinterpret_cast<float4*>(Bs + idx * 4) = rb[i];
+        if (FULL || idx < BK * BN / 4) *reinterpret_cast<f32x4*>(Bs + idx * 4) = rb[i];
     }
 }
 
@@ -515,7 +564,7 @@ __global__ void __launch_bounds__(256) k_project(PlanDev P, const float* __restr
 
     constexpr int A_V4 = A_ELEMS / 4 / 256;  // float4 per thread
     constexpr int B_V4 = (B_ELEMS / 4 + 255) / 256;
-    float4 ra[A_V4], rb[B_V4];
+    f32x4 ra[A_V4], rb[B_V4];
     const int nk = P.Tpad / BK;
     proj_gload<BM, BN, BK, A_V4, B_V4>(ra, rb, Ag, Bg, ldY, 0, tid);
     proj_sstore<BM, BN, BK, A_V4, B_V4>(ra, rb, lds, tid);
@@ -563,6 +612,195 @@ __global__ void __launch_bounds__(256) k_project(PlanDev P, const float* __restr
             const double cv = (double)cr[j];
             pp = fma(cv, cv, pp);
         }
+        float* sp = S + (size_t)v * ldS + k;
+        const double rss = (double)(*sp) - pp;
+        *sp = (float)fma(tp, log(rss + 1e-10), P.ld[k]);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// (ii') Split 16-bit projection: C = Ah Bh + Ah Bl + Al Bh on 32x32x16 MFMA
+// (f16 or bf16 halves, fp32 accumulation).  Same tiling, XCD order and score
+// epilogue as k_project.  A (the residual) is split on the way into LDS and
+// stored [voxel][t] so each lane's 8 time samples are one 16-byte read; B
+// halves come precomputed from the FP64 tables in the [t/8][col][8] layout.
+// ---------------------------------------------------------------------------
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+template <bool BF16>
+__device__ __forceinline__ void split8(const float (&x)[8], uint4& hi, uint4& lo) {
+    uint16_t h[8], l[8];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+        if (BF16) {
+            const __bf16 hb = (__bf16)x[e];
+            const __bf16 lb = (__bf16)(x[e] - (float)hb);
+            h[e] = __builtin_bit_cast(uint16_t, hb);
+            l[e] = __builtin_bit_cast(uint16_t, lb);
+        } else {
+            const _Float16 hf = (_Float16)x[e];
+            const _Float16 lf = (_Float16)(x[e] - (float)hf);
+            h[e] = __builtin_bit_cast(uint16_t, hf);
+            l[e] = __builtin_bit_cast(uint16_t, lf);
+        }
+    }
+    hi = make_uint4(h[0] | (uint32_t)h[1] << 16, h[2] | (uint32_t)h[3] << 16, h[4] | (uint32_t)h[5] << 16,
+                    h[6] | (uint32_t)h[7] << 16);
+    lo = make_uint4(l[0] | (uint32_t)l[1] << 16, l[2] | (uint32_t)l[3] << 16, l[4] | (uint32_t)l[5] << 16,
+                    l[6] | (uint32_t)l[7] << 16);
+}
+
+template <bool BF16>
+__device__ __forceinline__ f32x16 mfma16(uint4 a, uint4 b, f32x16 c) {
+    if (BF16)
+        return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b), c,
+                                                       0, 0, 0);
+    else
+        return __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(f16x8, a), __builtin_bit_cast(f16x8, b), c,
+                                                      0, 0, 0);
+}
+
+template <int BM, int BN, int NTB, int A_U, int B_U>
+__device__ __forceinline__ void proj16_gload(float (&ra)[A_U][8], u32x4 (&rb)[B_U], const float* __restrict__ Yt,
+                                             int64_t ldY, int v0, const uint16_t* __restrict__ gh,
+                                             const uint16_t* __restrict__ gl, int kt, int tid) {
+    const int t0 = kt * NTB * 8;
+#pragma unroll
+    for (int u = 0; u < A_U; ++u) {
+        const int q = tid + u * 256;
+        const int vv = q % BM, tb = q / BM;
+        const float* src = Yt + (size_t)(t0 + tb * 8) * ldY + v0 + vv;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) ra[u][e] = src[(size_t)e * ldY];
+    }
+    constexpr int B_N = 2 * NTB * BN;
+#pragma unroll
+    for (int u = 0; u < B_U; ++u) {
+        const int q0 = tid + u * 256;
+        const int q = q0 < B_N ? q0 : B_N - 1;
+        const int plane = q / (NTB * BN), rr = q % (NTB * BN);
+        const int tb = rr / BN, col = rr % BN;
+        const size_t o = ((size_t)(kt * NTB + tb) * BN + col) * 8;
+        rb[u] = *reinterpret_cast<const u32x4*>((plane ? gl : gh) + o);
+    }
+}
+
+template <int BM, int BN, int NTB, int A_U, int B_U, bool BF16>
+__device__ __forceinline__ void proj16_sstore(const float (&ra)[A_U][8], const u32x4 (&rb)[B_U], uint16_t* Ah,
+                                              uint16_t* Al, uint16_t* Bh, uint16_t* Bl, int RP, int tid) {
+#pragma unroll
+    for (int u = 0; u < A_U; ++u) {
+        const int q = tid + u * 256;
+        const int vv = q % BM, tb = q / BM;
+        uint4 hi, lo;
+        split8<BF16>(ra[u], hi, lo);
+        *reinterpret_cast<uint4*>(Ah + vv * RP + tb * 8) = hi;
+        *reinterpret_cast<uint4*>(Al + vv * RP + tb * 8) = lo;
+    }
+    constexpr int B_N = 2 * NTB * BN;
+    constexpr bool FULL = B_N % 256 == 0;
+#pragma unroll
+    for (int u = 0; u < B_U; ++u) {
+        const int q = tid + u * 256;
+        if (FULL || q < B_N) {
+            const int plane = q / (NTB * BN), rr = q % (NTB * BN);
+            const int tb = rr / BN, col = rr % BN;
+            *reinterpret_cast<u32x4*>((plane ? Bl : Bh) + col * RP + tb * 8) = rb[u];
+        }
+    }
+}
+
+template <int BN, bool BF16>
+__global__ void __launch_bounds__(256) k_project16(PlanDev P, const float* __restrict__ Yt, int64_t ldY, int V,
+                                                   float* __restrict__ S, int64_t ldS) {
+    constexpr int BM = kProjBM, BK = kProjBK, NBLK = BN / 32, NTB = BK / 8;
+    constexpr int RP = NTB * 8 + 8;             // row pitch in halves (padded 16 B)
+    constexpr int A_H = BM * RP, B_H = BN * RP;  // halves per hi (or lo) plane
+    constexpr int C_LD = BN + 1;
+    constexpr int TILE_BYTES = 2 * (A_H + B_H) * 2;
+    constexpr int LDS_BYTES = TILE_BYTES > BM * C_LD * 4 ? TILE_BYTES : BM * C_LD * 4;
+    __shared__ __attribute__((aligned(16))) unsigned char lds_raw[LDS_BYTES];
+    uint16_t* Ah = reinterpret_cast<uint16_t*>(lds_raw);
+    uint16_t* Al = Ah + A_H;
+    uint16_t* Bh = Al + A_H;
+    uint16_t* Bl = Bh + B_H;
+
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int nvt = gridDim.x / P.n_tiles;
+    const int nb = gridDim.x;
+    const int bid = blockIdx.x;
+    const int xcd = bid & 7, local = bid >> 3;
+    const int per_xcd = (nb + 7) >> 3;
+    int lin = xcd * per_xcd + local;
+    if ((nb & 7) != 0) lin = bid;
+    const int vt = lin / P.n_tiles;
+    const int tile = lin % P.n_tiles;
+    if (vt >= nvt) return;
+    const int v0 = vt * BM;
+    const uint16_t* __restrict__ gh = (BF16 ? P.Gh_bf16 : P.Gh_f16) + (size_t)tile * P.Tpad * BN;
+    const uint16_t* __restrict__ gl = (BF16 ? P.Gl_bf16 : P.Gl_f16) + (size_t)tile * P.Tpad * BN;
+
+    // per stage: A units (voxel, t-block) = BM * NTB = 512 -> 2 per thread;
+    // B units (plane, t-block, col) = 2 * NTB * BN -> B_U per thread (clamped, unconditional).
+    constexpr int A_U = BM * NTB / 256;
+    constexpr int B_N = 2 * NTB * BN;
+    constexpr int B_U = (B_N + 255) / 256;
+    float ra[A_U][8];
+    u32x4 rb[B_U];
+    f32x16 acc[NBLK];
+#pragma unroll
+    for (int i = 0; i < NBLK; ++i)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) acc[i][r] = 0.f;
+    const int nk = P.Tpad / BK;
+    proj16_gload<BM, BN, NTB, A_U, B_U>(ra, rb, Yt, ldY, v0, gh, gl, 0, tid);
+    for (int kt = 0; kt < nk; ++kt) {
+        proj16_sstore<BM, BN, NTB, A_U, B_U, BF16>(ra, rb, Ah, Al, Bh, Bl, RP, tid);
+        __syncthreads();
+        if (kt + 1 < nk) proj16_gload<BM, BN, NTB, A_U, B_U>(ra, rb, Yt, ldY, v0, gh, gl, kt + 1, tid);
+        const int row = w * 32 + (lane & 31);
+#pragma unroll
+        for (int ks = 0; ks < BK / 16; ++ks) {
+            const int tb = 2 * ks + (lane >> 5);
+            const uint4 ah = *reinterpret_cast<const uint4*>(Ah + row * RP + tb * 8);
+            const uint4 al = *reinterpret_cast<const uint4*>(Al + row * RP + tb * 8);
+#pragma unroll
+            for (int n = 0; n < NBLK; ++n) {
+                const int col = n * 32 + (lane & 31);
+                const uint4 bh = *reinterpret_cast<const uint4*>(Bh + col * RP + tb * 8);
+                const uint4 bl = *reinterpret_cast<const uint4*>(Bl + col * RP + tb * 8);
+                acc[n] = mfma16<BF16>(al, bh, acc[n]);
+                acc[n] = mfma16<BF16>(ah, bl, acc[n]);
+                acc[n] = mfma16<BF16>(ah, bh, acc[n]);
+            }
+        }
+        __syncthreads();
+    }
+    float* Cs = reinterpret_cast<float*>(lds_raw);
+#pragma unroll
+    for (int n = 0; n < NBLK; ++n)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const int row = w * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+            const int col = n * 32 + (lane & 31);
+            Cs[row * C_LD + col] = acc[n][r];
+        }
+    __syncthreads();
+    const int p = P.p, npt = P.npt, K = P.K;
+    const double tp = (double)(P.T - p);
+    for (int cidx = tid; cidx < BM * npt; cidx += 256) {
+        const int row = cidx / npt, q = cidx % npt;
+        const int k = tile * npt + q;
+        const int v = v0 + row;
+        if (k >= K || v >= V) continue;
+        const float* cr = Cs + row * C_LD + q * p;
+        double pp = 0.0;
+        for (int j = 0; j < p; ++j) {
+            const double cv = (double)cr[j];
+            pp = fma(cv, cv, pp);
+        }
+        if (!BF16) pp *= (double)P.g16_unscale[k];
         float* sp = S + (size_t)v * ldS + k;
         const double rss = (double)(*sp) - pp;
         *sp = (float)fma(tp, log(rss + 1e-10), P.ld[k]);
@@ -687,7 +925,7 @@ constexpr int kGlsChunk = kStageChunk;
 
 template <int PM, bool UNIFORM>
 __device__ inline void gls_body(const PlanDev& P, const float* __restrict__ Y, int64_t ldY, int v, int k, double s,
-                                float* ys, double* tab, float* __restrict__ betas, float* __restrict__ tstats,
+                                double* tab, float* __restrict__ betas, float* __restrict__ tstats,
                                 float* __restrict__ fstats, float* __restrict__ sigma2, float* __restrict__ r2,
                                 double* __restrict__ resid_w, int64_t ldR, float* __restrict__ resid,
                                 float* __restrict__ pred_out) {
@@ -700,18 +938,23 @@ __device__ inline void gls_body(const PlanDev& P, const float* __restrict__ Y, i
     const double* __restrict__ Xw = P.Xw64 + (size_t)kk * T * p;
     const double* __restrict__ cf = P.coef64 + (size_t)kk * T * 3;
     const double* __restrict__ L = P.Lx + (size_t)kk * p * p;
+    const float* __restrict__ row = Y + (size_t)(active ? v : 0) * ldY;
+    const bool vec4 = (ldY & 3) == 0;
+    float yv[kGlsChunk];
     double h[PM], be[PM];
 #pragma unroll
     for (int j = 0; j < PM; ++j) h[j] = 0.0;
     double sy = 0.0;
     for (int t0 = 0; t0 < T; t0 += kGlsChunk) {
+        load_chunk(row, T, t0, active, vec4, yv);
         __syncthreads();
-        stage_rows(Y, ldY, T, t0, v, ys);
         if (UNIFORM) stage_table<PM>(G, T, p, t0, tab);
         __syncthreads();
         const int n = min(kGlsChunk, T - t0);
-        for (int i = 0; i < n; ++i) {
-            const double yt = (double)ys[lane * (kGlsChunk + 1) + i] * inv;
+#pragma unroll
+        for (int i = 0; i < kGlsChunk; ++i) {
+            if (i >= n) break;
+            const double yt = (double)yv[i] * inv;
             sy += yt;
             const double* gr = UNIFORM ? tab + i * p : G + (size_t)(t0 + i) * p;
 #pragma unroll
@@ -738,8 +981,8 @@ __device__ inline void gls_body(const PlanDev& P, const float* __restrict__ Y, i
     double* tx = tab + kGlsChunk * PM;       // X rows
     double* tc = tab + 2 * kGlsChunk * PM;   // coefficient rows (3 per t)
     for (int t0 = 0; t0 < T; t0 += kGlsChunk) {
+        load_chunk(row, T, t0, active, vec4, yv);
         __syncthreads();
-        stage_rows(Y, ldY, T, t0, v, ys);
         stage_table<PM>(P.X, T, p, t0, tx);
         if (UNIFORM) {
             stage_table<PM>(Xw, T, p, t0, tab);
@@ -747,9 +990,11 @@ __device__ inline void gls_body(const PlanDev& P, const float* __restrict__ Y, i
         }
         __syncthreads();
         const int n = min(kGlsChunk, T - t0);
-        for (int i = 0; i < n; ++i) {
+#pragma unroll
+        for (int i = 0; i < kGlsChunk; ++i) {
+            if (i >= n) break;
             const int t = t0 + i;
-            const double yt = (double)ys[lane * (kGlsChunk + 1) + i] * inv;
+            const double yt = (double)yv[i] * inv;
             // coefficients of step t-1 (previous chunk's last row when i == 0)
             const double* cprev = (UNIFORM && i > 0) ? tc + 3 * (i - 1) : cf + 3 * (t - 1);
             const double* ccur = UNIFORM ? tc + 3 * i : cf + 3 * t;
@@ -815,7 +1060,6 @@ __global__ void __launch_bounds__(64) k_gls(PlanDev P, const float* __restrict__
                                             float* __restrict__ sigma2, float* __restrict__ r2,
                                             double* __restrict__ resid_w, int64_t ldR, float* __restrict__ resid,
                                             float* __restrict__ pred_out) {
-    __shared__ float ys[64 * (kGlsChunk + 1)];
     __shared__ double tab[kGlsChunk * (2 * PM + 3)];
     const int lane = threadIdx.x & 63;
     const int i = blockIdx.x * 64 + lane;
@@ -835,10 +1079,10 @@ __global__ void __launch_bounds__(64) k_gls(PlanDev P, const float* __restrict__
     const bool same = (v < 0) || (k == k0);
     const bool uniform = __all(same) && k0 >= 0;
     if (uniform)
-        gls_body<PM, true>(P, Y, ldY, v, k0, s, ys, tab, betas, tstats, fstats, sigma2, r2, resid_w, ldR, resid,
+        gls_body<PM, true>(P, Y, ldY, v, k0, s, tab, betas, tstats, fstats, sigma2, r2, resid_w, ldR, resid,
                            pred_out);
     else
-        gls_body<PM, false>(P, Y, ldY, v, k, s, ys, tab, betas, tstats, fstats, sigma2, r2, resid_w, ldR, resid,
+        gls_body<PM, false>(P, Y, ldY, v, k, s, tab, betas, tstats, fstats, sigma2, r2, resid_w, ldR, resid,
                             pred_out);
 }
 
@@ -1030,6 +1274,29 @@ __global__ void __launch_bounds__(kWin * 64) k_local_energy(PlanDev P, const flo
     }
 }
 
+template <int BM, int BN, int BK, int A_V4, int B_V4>
+__device__ __forceinline__ void local_gload(f32x4 (&ra)[A_V4], f32x4 (&rb)[B_V4], const float* __restrict__ Yt,
+                                            int64_t ldY, int col0, const float* __restrict__ Gp, int K16,
+                                            const int* kslot, int kt, int tid) {
+    const int t0 = kt * BK;
+#pragma unroll
+    for (int i = 0; i < A_V4; ++i) {
+        const int idx = tid + i * 256;
+        const int r = idx / (BM / 4), c4 = idx % (BM / 4);
+        ra[i] = *reinterpret_cast<const f32x4*>(Yt + (size_t)(t0 + r) * ldY + col0 + c4 * 4);
+    }
+#pragma unroll
+    for (int i = 0; i < B_V4; ++i) {
+        const int idx = tid + i * 256;
+        const int r = idx / (BN / 4), c4 = idx % (BN / 4);
+        const int slot = c4 / (kLocalPP / 4), part = c4 % (kLocalPP / 4);
+        const int k = kslot[slot];
+        const f32x4 g =
+            *reinterpret_cast<const f32x4*>(Gp + (size_t)(t0 + r) * K16 + (size_t)(k >= 0 ? k : 0) * kLocalPP + part * 4);
+        rb[i] = k >= 0 ? g : f32x4{0.f, 0.f, 0.f, 0.f};
+    }
+}
+
 // Window projection on MFMA (fp32 32x32x2, same k-order as k_project) + score +
 // local argmin + edge test.  Block: 4 waves x 32 voxel rows = one 128-column
 // tile; N = 26 pair slots x kLocalPP columns (slot 25 is padding).
@@ -1041,7 +1308,7 @@ __global__ void __launch_bounds__(256) k_local_project(PlanDev P, const float* _
                                                        const float* __restrict__ Eloc, int32_t* __restrict__ best,
                                                        float* __restrict__ best_score,
                                                        int32_t* __restrict__ need_full) {
-    constexpr int BM = kLocalTile, BK = 16, NSLOT = kSlots + 1, BN = NSLOT * kLocalPP, NBLK = BN / 32;
+    constexpr int BM = kLocalTile, BK = 16, NSLOT = kSlots + 3, BN = NSLOT * kLocalPP, NBLK = BN / 32;
     constexpr int A_ELEMS = BK * BM, B_ELEMS = BK * BN;
     __shared__ float lds[A_ELEMS + B_ELEMS];
     __shared__ float scores[BM][kSlots + 1];
@@ -1059,45 +1326,22 @@ __global__ void __launch_bounds__(256) k_local_project(PlanDev P, const float* _
     float* Bs = lds + A_ELEMS;
     constexpr int A_V4 = A_ELEMS / 4 / 256;
     constexpr int B_V4 = (B_ELEMS / 4 + 255) / 256;
-    float4 ra[A_V4], rb[B_V4];
-    auto gload = [&](int kt) {
-        const int t0 = kt * BK;
-#pragma unroll
-        for (int i = 0; i < A_V4; ++i) {
-            const int idx = tid + i * 256;
-            const int r = idx / (BM / 4), c4 = idx % (BM / 4);
-            ra[i] = *reinterpret_cast<const float4*>(Yt + (size_t)(t0 + r) * ldY + col0 + c4 * 4);
-        }
-#pragma unroll
-        for (int i = 0; i < B_V4; ++i) {
-            const int idx = tid + i * 256;
-            if (idx < B_ELEMS / 4) {
-                const int r = idx / (BN / 4), c4 = idx % (BN / 4);
-                const int slot = c4 / (kLocalPP / 4), part = c4 % (kLocalPP / 4);
-                const int k = kslot[slot];
-                rb[i] = k >= 0 ? *reinterpret_cast<const float4*>(P.Gp + (size_t)(t0 + r) * K16 + (size_t)k * kLocalPP +
-                                                                   part * 4)
-                               : make_float4(0.f, 0.f, 0.f, 0.f);
-            }
-        }
-    };
+    f32x4 ra[A_V4], rb[B_V4];
+    static_assert((B_ELEMS / 4) % 256 == 0 && (A_ELEMS / 4) % 256 == 0, "local tiles must be exact");
     f32x16 acc[NBLK];
 #pragma unroll
     for (int i = 0; i < NBLK; ++i)
 #pragma unroll
         for (int r = 0; r < 16; ++r) acc[i][r] = 0.f;
     const int nk = P.Tpad / BK;
-    gload(0);
+    local_gload<BM, BN, BK, A_V4, B_V4>(ra, rb, Yt, ldY, col0, P.Gp, K16, kslot, 0, tid);
     for (int kt = 0; kt < nk; ++kt) {
 #pragma unroll
-        for (int i = 0; i < A_V4; ++i) *reinterpret_cast<float4*>(As + (tid + i * 256) * 4) = ra[i];
+        for (int i = 0; i < A_V4; ++i) *reinterpret_cast<f32x4*>(As + (tid + i * 256) * 4) = ra[i];
 #pragma unroll
-        for (int i = 0; i < B_V4; ++i) {
-            const int idx = tid + i * 256;
-            if (idx < B_ELEMS / 4) *reinterpret_cast<float4*>(Bs + idx * 4) = rb[i];
-        }
+        for (int i = 0; i < B_V4; ++i) *reinterpret_cast<f32x4*>(Bs + (tid + i * 256) * 4) = rb[i];
         __syncthreads();
-        if (kt + 1 < nk) gload(kt + 1);
+        if (kt + 1 < nk) local_gload<BM, BN, BK, A_V4, B_V4>(ra, rb, Yt, ldY, col0, P.Gp, K16, kslot, kt + 1, tid);
         const int ar = lane >> 5, ac = lane & 31;
 #pragma unroll
         for (int kk = 0; kk < BK / 2; ++kk) {

@@ -307,6 +307,17 @@ class RemlGrid(Mapping):
                    "project")
         return S
 
+    PROJECTIONS = {"fp32": 0, "bf16x3": 1, "fp16x3": 2}
+
+    def set_projection(self, mode: str):
+        """Arithmetic of the projection GEMM: ``"fp32"`` (fp32 MFMA),
+        ``"fp16x3"`` or ``"bf16x3"`` (hi/lo split operands, three 16-bit MFMA
+        products, fp32 accumulation)."""
+        if mode not in self.PROJECTIONS:
+            raise ValueError(f"projection must be one of {sorted(self.PROJECTIONS)}, got {mode!r}")
+        _lib.check(_lib.load().ffs_reml_plan_set_projection(self._handle, self.PROJECTIONS[mode]), "set_projection")
+        self.projection = mode
+
     def select(self, S: torch.Tensor, hierarchical: bool):
         V = S.shape[0]
         best = torch.empty(V, dtype=torch.int32, device=self.device)

@@ -93,6 +93,11 @@ int ffs_reml_energy(const ffs_reml_plan* plan, const float* Yt, int64_t ldYt, in
 int ffs_reml_project(const ffs_reml_plan* plan, const float* Yt, int64_t ldYt, int64_t V, float* S, int64_t ldS,
                      void* stream);
 
+/* Precision of the full-grid projection (ii): 0 = fp32 MFMA (default),
+ * 1 = split bf16 (hi + lo, three products, fp32 accumulation),
+ * 2 = split fp16 (same, fp16 halves).  The local window search always uses fp32. */
+int ffs_reml_plan_set_projection(ffs_reml_plan* plan, int32_t mode);
+
 /* (iii) mode 0: global argmin (first minimum); mode 1: hierarchical descent. */
 int ffs_reml_select(const ffs_reml_plan* plan, const float* S, int64_t ldS, int64_t V, int32_t mode, int32_t* best,
                     float* best_score, void* stream);

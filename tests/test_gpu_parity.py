@@ -149,3 +149,37 @@ def test_single_voxel_and_ljung_box(dev):
     lb = ljung_box_device(out["resid_w"], lag, lt).cpu().double()
     ref = O.ljung_box(out["resid_w"].cpu(), lag, lt)
     assert float((lb - ref).abs().max()) <= 1e-4 * max(1.0, float(ref.abs().max()))
+
+
+# Split 16-bit projections (hi + lo halves, three MFMA products, fp32 accumulation).
+# Score tolerance |gpu - oracle| <= atol + rtol |score| per mode; argmin mismatches
+# only where the oracle's FP64 scores of the two pairs are within `tie`.
+SPLIT_TOL = {"fp16x3": dict(atol=5e-4, rtol=2e-6, tie=2e-3), "bf16x3": dict(atol=2e-2, rtol=2e-5, tie=2e-2)}
+
+
+@pytest.mark.parametrize("mode", ["fp16x3", "bf16x3"])
+@pytest.mark.parametrize("name", ["plain", "censored", "afni"])
+def test_split_projection_matches_oracle(dev, name, mode):
+    from fastfuncstuff_amd.glm.arma import RemlGrid
+
+    print(f"[parity] split case={name} mode={mode}", flush=True)
+    X, Y, ag, bg, run_starts, tau = _case(name)
+    grid_o = O.precompute_grid(torch.from_numpy(X), ag, bg, run_starts, tau, F64)
+    grid = RemlGrid(torch.from_numpy(X), ag, bg, dev, run_starts=run_starts, tau=tau)
+    grid.set_projection(mode)
+    V = Y.shape[0]
+    Yt, _, _ = grid.prep(torch.from_numpy(Y).to(dev), normalize=True)
+    S = grid.surface(Yt, V).cpu().double()
+    S64, _ = _oracle_on_normalised(X, Y, grid_o)
+    tol = SPLIT_TOL[mode]
+    err = (S - S64).abs()
+    print(f"[parity] {mode} {name}: max score error {float(err.max()):.3e}", flush=True)
+    assert bool((err <= tol["atol"] + tol["rtol"] * S64.abs()).all()), f"max score error {float(err.max()):.3e}"
+    best, _ = grid.select(S.float().to(dev), hierarchical=False)
+    best = best.cpu().long()
+    k_ref, _ = O.select_exhaustive(S64.float())
+    diff = best != k_ref
+    gap = (S64[torch.arange(V), best] - S64[torch.arange(V), k_ref]).abs()
+    assert bool((gap[diff] <= tol["tie"]).all()), f"{int(diff.sum())} mismatches beyond near-ties"
+    with pytest.raises(ValueError):
+        grid.set_projection("fp8")
