@@ -34,6 +34,7 @@ sys.path.insert(0, str(ROOT))
 METRIC = "REML voxels/s (T=400,V=200k): learned-proposal vs full grid; % argmin match"
 PEAK_FP32_MFMA = 157.3e12
 PEAK_HBM = 8.0e12
+PEAK_F16_MFMA = 2.5e15
 
 
 def parse():
@@ -48,6 +49,8 @@ def parse():
     ap.add_argument("--target-agreement", type=float, default=0.9995)
     ap.add_argument("--cpu-sample", type=int, default=2048, help="voxels in the timed CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--projection", default="fp16x3", choices=["fp32", "fp16x3", "bf16x3"],
+                    help="arithmetic of the full-grid projection GEMM")
     return ap.parse_args()
 
 
@@ -94,6 +97,7 @@ def main():
                     tau=tau)
     torch.cuda.synchronize()
     t_plan = time.time() - t0
+    grid.set_projection(args.projection)
     K = grid.K
     Y = torch.from_numpy(wl.Y).to(dev)
     st = _lib.stream_handle(dev)
@@ -158,7 +162,7 @@ def main():
     value = V / (lp_ms / 1e3)
 
     # ---- roofline of the dominant kernel of the reported (learned) path --
-    kern = {"local_project": ("mfma", 2.0 * V * T * 26 * 16, "2*V*T*(26 slots*16 cols)"),
+    kern = {"local_project": ("mfma", 2.0 * V * T * 25 * p, "2*V*T*25*p (25 window pairs x p columns)"),
             "local_energy": ("hbm", 4.0 * grid.Tpad * V + 4.0 * V * 25, "4*Tpad*V + 4*25*V"),
             "gls": ("hbm", 4.0 * V * T + 4.0 * V * (2 * p + 3), "4*V*T + 4*V*(2p+3)")}
     dom = max(kern, key=lambda n: lp_k[n])
@@ -172,10 +176,14 @@ def main():
         roof = {"kernel": dom, "bound": "hbm", "achieved": ach / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
                 "frac": ach / PEAK_HBM, "traffic": None,
                 "algorithmic": f"{formula} = {amount:.4g} B per launch"}
-    proj_flops = 2.0 * V * T * K * p
-    fg_roof = {"kernel": "k_project", "bound": "mfma",
-               "achieved": proj_flops / (fg_k["project"] / 1e3) / 1e12, "peak": PEAK_FP32_MFMA / 1e12,
-               "unit": "TFLOP/s", "frac": proj_flops / (fg_k["project"] / 1e3) / PEAK_FP32_MFMA}
+    # fp32: 2*V*T*K*p flops on fp32 MFMA; split modes: three 16-bit products of that size.
+    split = args.projection != "fp32"
+    proj_flops = (3.0 if split else 1.0) * 2.0 * V * T * K * p
+    proj_peak = PEAK_F16_MFMA if split else PEAK_FP32_MFMA
+    fg_roof = {"kernel": "k_project16" if split else "k_project", "bound": "mfma",
+               "achieved": proj_flops / (fg_k["project"] / 1e3) / 1e12, "peak": proj_peak / 1e12,
+               "unit": "TFLOP/s", "frac": proj_flops / (fg_k["project"] / 1e3) / proj_peak,
+               "algorithmic": f"{'3*' if split else ''}2*V*T*K*p = {proj_flops:.4g} flop per launch"}
 
     line = {
         "metric": METRIC, "value": value, "unit": "voxels/s", "n_gpus": 1, "steps": args.steps,
@@ -183,7 +191,9 @@ def main():
         "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
         "config": {"workload": f"censored_T{args.T}_V{V}_p{p}_grid33x33", "V": V, "T_acquired": args.T,
                    "T_retained": T, "p": p, "K": K, "path": "learned_proposal_5x5_with_full_grid_fallback",
-                   "projection": "fp32 MFMA 32x32x2", "gls": "fp64", "train_voxels": args.train_voxels},
+                   "projection_full_grid": {"fp32": "fp32 MFMA 32x32x2", "fp16x3": "split fp16 hi+lo x3 MFMA 32x32x16",
+                                            "bf16x3": "split bf16 hi+lo x3 MFMA 32x32x16"}[args.projection],
+                   "projection_window": "fp32 MFMA 32x32x2", "gls": "fp64", "train_voxels": args.train_voxels},
         "full_grid": {"value": V / (fg_ms / 1e3), "ms_per_step": fg_ms, "kernels_ms": fg_k, "roofline": fg_roof},
         "learned": {"kernels_ms": lp_k, "fallback_fraction": res.n_fallback / V, "conf_min": conf_min,
                     "argmin_match_vs_full_grid_exhaustive": agree_exh,

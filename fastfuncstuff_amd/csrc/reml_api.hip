@@ -14,6 +14,7 @@
 // The kernels are compiled in this translation unit so that the templated
 // launches below instantiate them.
 #include "reml_kernels.hip"
+#include "reml_sort_gls.hip"
 
 using namespace remlk;
 
@@ -201,13 +202,33 @@ extern "C" int ffs_reml_plan_create(ffs_reml_plan** out, int32_t T, int32_t p, c
     HIP_TRY(hipMemsetAsync(Glf, 0, 2 * n16, s));
     HIP_TRY(hipMemsetAsync(Ghb, 0, 2 * n16, s));
     HIP_TRY(hipMemsetAsync(Glb, 0, 2 * n16, s));
+    // f64 MFMA GLS tables
+    double *G64p = nullptr, *coefR = nullptr, *dXp = nullptr, *dXtX = nullptr;
+    d.PG = p <= kGlsMfmaMaxP ? 16 * ((p + 15) / 16) : 0;
+    if (d.PG) {
+        const int PG = d.PG;
+        std::vector<double> Xp((size_t)d.Tpad * PG, 0.0), XtX((size_t)p * p, 0.0);
+        for (int t = 0; t < T; ++t)
+            for (int j = 0; j < p; ++j) Xp[(size_t)t * PG + j] = X[(size_t)t * p + j];
+        for (int a = 0; a < p; ++a)
+            for (int b = 0; b < p; ++b) {
+                double acc = 0.0;
+                for (int t = 0; t < T; ++t) acc += X[(size_t)t * p + a] * X[(size_t)t * p + b];
+                XtX[(size_t)a * p + b] = acc;
+            }
+        if ((rc = P->alloc(&G64p, (size_t)K * d.Tpad * PG)) || (rc = P->alloc(&coefR, (size_t)K * d.Tpad * 4)) ||
+            (rc = upload(P, &dXp, Xp.data(), Xp.size(), s)) || (rc = upload(P, &dXtX, XtX.data(), XtX.size(), s)))
+            return rc;
+        HIP_TRY(hipMemsetAsync(G64p, 0, sizeof(double) * K * d.Tpad * PG, s));
+        HIP_TRY(hipMemsetAsync(coefR, 0, sizeof(double) * K * d.Tpad * 4, s));
+    }
     if (P->local_ok) {
         if ((rc = P->alloc(&Gp, (size_t)d.Tpad * K * kLocalPP))) return rc;
         HIP_TRY(hipMemsetAsync(Gp, 0, sizeof(float) * d.Tpad * K * kLocalPP, s));
     }
     hipLaunchKernelGGL(k_tables, dim3(K), dim3(64), 0, s, T, d.Tpad, p, K, dab, dwhiteflag, dgap, dX, d.n_task, dtask,
                        full_f, BN, npt, soa, coef64, Xw64, G64, Lx, Ainv, Fm, work, ld, ldR, ldX, G32, Gp, Ghf, Glf, Ghb, Glb, g16s,
-                       dstatus);
+                       d.PG, G64p, coefR, dstatus);
     HIP_TRY(hipGetLastError());
     hipLaunchKernelGGL(k_local_tables, dim3(K), dim3(128), 0, s, T, K, soa, lp, lc, ls0);
     HIP_TRY(hipGetLastError());
@@ -251,6 +272,10 @@ extern "C" int ffs_reml_plan_create(ffs_reml_plan** out, int32_t T, int32_t p, c
     d.Gh_bf16 = Ghb;
     d.Gl_bf16 = Glb;
     d.g16_unscale = g16s;
+    d.G64p = G64p;
+    d.coefR = coefR;
+    d.Xp = dXp;
+    d.XtX = dXtX;
     d.proj_mode = 0;
     P->pinvT = dpinv;
     P->ldR = ldR;
@@ -397,28 +422,87 @@ static void launch_gls(const ffs_reml_plan* P, const float* Y, int64_t ldY, cons
                        perm, best, V, betas, tstats, fstats, sigma2, r2, resid_w, ldR, resid, pred);
 }
 
+// Voxels grouped by key (see k_bucket_count): cnt[nb], off[nb + 1], cursor[nb],
+// perm[cap] with cap >= V + nb * (pad - 1).  Returns 0 or an error status.
+static int group_by_key(const int32_t* keys, int64_t V, int nb, int pad, int32_t* cnt, int32_t* off,
+                        int32_t* cursor, int32_t* perm, int64_t cap, hipStream_t s) {
+    if (nb < 1 || pad < 1) return fail(-3, "group_by_key: bad bucket count or padding");
+    // only non-empty groups are padded: at most min(V, nb) of them
+    if (V + std::min<int64_t>(V, nb) * (pad - 1) > cap) return fail(-3, "group_by_key: permutation buffer too small");
+    hipLaunchKernelGGL(k_fill_i32, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, s, cnt, (int64_t)nb, 0);
+    hipLaunchKernelGGL(k_fill_i32, dim3((unsigned)((cap + 255) / 256)), dim3(256), 0, s, perm, cap, -1);
+    if (V > 0) {
+        if (nb <= kSortLdsMax) {
+            const unsigned g = (unsigned)((V + kSortBlock * kSortItems - 1) / (kSortBlock * kSortItems));
+            hipLaunchKernelGGL(k_bucket_count, dim3(g), dim3(kSortBlock), sizeof(int32_t) * nb, s, keys, (int)V, nb,
+                               cnt);
+            hipLaunchKernelGGL(k_bucket_scan, dim3(1), dim3(1024), 0, s, cnt, nb, pad, off, cursor);
+            hipLaunchKernelGGL(k_bucket_place, dim3(g), dim3(kSortBlock), 2 * sizeof(int32_t) * nb, s, keys, (int)V,
+                               nb, cursor, perm);
+        } else {
+            const unsigned g = (unsigned)((V + 255) / 256);
+            hipLaunchKernelGGL(k_bucket_count_global, dim3(g), dim3(256), 0, s, keys, (int)V, nb, cnt);
+            hipLaunchKernelGGL(k_bucket_scan, dim3(1), dim3(1024), 0, s, cnt, nb, pad, off, cursor);
+            hipLaunchKernelGGL(k_bucket_place_global, dim3(g), dim3(256), 0, s, keys, (int)V, nb, cursor, perm);
+        }
+    } else {
+        hipLaunchKernelGGL(k_bucket_scan, dim3(1), dim3(1024), 0, s, cnt, nb, pad, off, cursor);
+    }
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+// int32 workspace of the GLS call: cnt, off, cursor, perm (64-slot padded groups).
+static int64_t gls_perm_cap(int64_t V, int K) { return (V + std::min<int64_t>(V, K) * 63 + 63) / 64 * 64; }
+static int64_t gls_work_ints(int64_t V, int K) { return 3 * (int64_t)K + 1 + gls_perm_cap(V, K); }
+
+extern "C" int ffs_reml_gls_workspace(const ffs_reml_plan* P, int64_t V, int64_t* n_int32) {
+    if (!P || !n_int32) return fail(-3, "gls_workspace: null argument");
+    *n_int32 = gls_work_ints(std::max<int64_t>(V, 0), P->d.K);
+    return 0;
+}
+
 extern "C" int ffs_reml_gls(const ffs_reml_plan* P, const float* Y, int64_t ldY, const float* scale,
                             const int32_t* best, int64_t V, float* betas, float* tstats, float* fstats,
                             float* sigma2, float* r2, double* resid_w, int64_t ldR, float* resid, float* pred,
                             int32_t* work, void* stream) {
     if (!P) return fail(-3, "gls: null plan");
     if (V <= 0) return 0;
+    if (V > 0x7fffffff - 64 * (int64_t)P->d.K) return fail(-3, "gls: batch too large; split it");
     if (ldY < P->d.T) return fail(-3, "gls: ldY must be >= T");
     if ((resid_w || resid || pred) && ldR < P->d.T) return fail(-3, "gls: ldR must be >= T");
     hipStream_t s = (hipStream_t)stream;
-    const int K = P->d.K;
+    const int K = P->d.K, p = P->d.p;
+    hipLaunchKernelGGL(k_gls_invalid, dim3((unsigned)((V + 255) / 256)), dim3(256), 0, s, best, (int)V, K, p, betas,
+                       tstats, fstats, sigma2, r2);
+    const bool series_out = resid_w || resid || pred;
+    if (work && P->d.PG && !series_out) {
+        // pair-uniform waves: groups padded to 64 slots
+        int32_t* cnt = work;
+        int32_t* off = cnt + K;
+        int32_t* cursor = off + K + 1;
+        int32_t* perm = cursor + K;
+        const int64_t cap = gls_perm_cap(V, K);
+        if (int rc = group_by_key(best, V, K, 64, cnt, off, cursor, perm, cap, s)) return rc;
+        const dim3 g((unsigned)(cap / 64)), b(64);
+        if (P->d.PG == 16)
+            hipLaunchKernelGGL(k_gls_mfma<1>, g, b, 0, s, P->d, Y, ldY, scale, perm, best, betas, tstats, fstats,
+                               sigma2, r2);
+        else
+            hipLaunchKernelGGL(k_gls_mfma<2>, g, b, 0, s, P->d, Y, ldY, scale, perm, best, betas, tstats, fstats,
+                               sigma2, r2);
+        HIP_TRY(hipGetLastError());
+        return 0;
+    }
     int32_t* perm = nullptr;
     if (work) {
-        int32_t* cnt = work + V;
-        perm = work;
-        hipLaunchKernelGGL(k_fill_i32, dim3((K + 1 + 255) / 256), dim3(256), 0, s, cnt, (int64_t)(K + 1), 0);
-        const unsigned g = (unsigned)((V + 255) / 256);
-        hipLaunchKernelGGL(k_hist, dim3(g), dim3(256), 0, s, best, (int)V, K, cnt);
-        hipLaunchKernelGGL(k_scan, dim3(1), dim3(1), 0, s, cnt, K + 1);
-        hipLaunchKernelGGL(k_scatter, dim3(g), dim3(256), 0, s, best, (int)V, K, cnt, perm);
-        HIP_TRY(hipGetLastError());
+        // contiguous groups (no padding); invalid pairs are dropped from the order
+        int32_t* cnt = work;
+        int32_t* off = cnt + K;
+        int32_t* cursor = off + K + 1;
+        perm = cursor + K;
+        if (int rc = group_by_key(best, V, K, 1, cnt, off, cursor, perm, gls_perm_cap(V, K), s)) return rc;
     }
-    const int p = P->d.p;
     if (p <= 8)
         launch_gls<8>(P, Y, ldY, scale, perm, best, (int)V, betas, tstats, fstats, sigma2, r2, resid_w, ldR, resid, pred, s);
     else if (p <= 16)
@@ -449,7 +533,7 @@ extern "C" int ffs_reml_ljung_box(const double* resid_w, int64_t ldR, int64_t V,
 namespace {
 struct LocalLayout {
     int64_t ng, max_cols, max_tiles;
-    size_t off_center, off_cnt, off_fill, off_tg, off_tc, off_nt, off_rows, off_yt, off_e, total;
+    size_t off_center, off_cnt, off_fill, off_cursor, off_tg, off_tc, off_nt, off_rows, off_yt, off_e, total;
 };
 
 size_t align256(size_t x) { return (x + 255) / 256 * 256; }
@@ -466,6 +550,8 @@ LocalLayout local_layout(const ffs_reml_plan* P, int64_t V) {
     L.off_cnt = o;
     o = align256(o + 4 * (size_t)(L.ng + 1));
     L.off_fill = o;
+    o = align256(o + 4 * (size_t)(L.ng + 1));
+    L.off_cursor = o;
     o = align256(o + 4 * (size_t)L.ng);
     L.off_tg = o;
     o = align256(o + 4 * (size_t)L.max_tiles);
@@ -492,7 +578,7 @@ extern "C" int ffs_reml_local_workspace(const ffs_reml_plan* P, int64_t V, int64
 
 namespace {
 struct LocalPtrs {
-    int32_t *center, *cnt, *fill, *tg, *tc, *nt, *rows;
+    int32_t *center, *cnt, *fill, *cursor, *tg, *tc, *nt, *rows;
     float *Ytg, *Eloc;
 };
 LocalPtrs local_ptrs(const LocalLayout& L, void* workspace) {
@@ -501,6 +587,7 @@ LocalPtrs local_ptrs(const LocalLayout& L, void* workspace) {
     q.center = reinterpret_cast<int32_t*>(ws + L.off_center);
     q.cnt = reinterpret_cast<int32_t*>(ws + L.off_cnt);
     q.fill = reinterpret_cast<int32_t*>(ws + L.off_fill);
+    q.cursor = reinterpret_cast<int32_t*>(ws + L.off_cursor);
     q.tg = reinterpret_cast<int32_t*>(ws + L.off_tg);
     q.tc = reinterpret_cast<int32_t*>(ws + L.off_tc);
     q.nt = reinterpret_cast<int32_t*>(ws + L.off_nt);
@@ -529,12 +616,11 @@ extern "C" int ffs_reml_local_prepare(const ffs_reml_plan* P, const float* Y, in
     hipStream_t s = (hipStream_t)stream;
     const unsigned gv = (unsigned)((V + 255) / 256);
     hipLaunchKernelGGL(k_local_center, dim3(gv), dim3(256), 0, s, P->d, props, (int)V, conf_min, q.center, need_full);
-    hipLaunchKernelGGL(k_fill_i32, dim3((unsigned)((L.ng + 1 + 255) / 256)), dim3(256), 0, s, q.cnt, L.ng + 1, 0);
-    hipLaunchKernelGGL(k_hist, dim3(gv), dim3(256), 0, s, q.center, (int)V, (int)L.ng, q.cnt);
-    hipLaunchKernelGGL(k_local_layout, dim3(1), dim3(1), 0, s, (int)L.ng, q.cnt, q.fill, q.tg, q.tc, q.nt);
-    hipLaunchKernelGGL(k_fill_i32, dim3((unsigned)((L.max_cols + 255) / 256)), dim3(256), 0, s, q.rows, L.max_cols,
-                       -1);
-    hipLaunchKernelGGL(k_local_scatter, dim3(gv), dim3(256), 0, s, q.center, (int)V, (int)L.ng, q.fill, q.rows);
+    // voxels grouped by window centre, each group padded to whole tiles
+    if (int rc = group_by_key(q.center, V, (int)L.ng, kLocalTile, q.cnt, q.fill, q.cursor, q.rows, L.max_cols, s))
+        return rc;
+    hipLaunchKernelGGL(k_local_tiles, dim3((unsigned)((L.max_tiles + 255) / 256)), dim3(256), 0, s, q.fill, (int)L.ng,
+                       (int)L.max_tiles, q.tg, q.tc, q.nt);
     HIP_TRY(hipGetLastError());
     return prep_impl(P, Y, ldY, V, q.rows, L.max_cols, q.Ytg, L.max_cols, scale, 1, nullptr, s);
 }
@@ -577,7 +663,7 @@ extern "C" int ffs_reml_compact(const int32_t* flags, int64_t V, int32_t* idx, i
     if (V <= 0) return 0;
     hipStream_t s = (hipStream_t)stream;
     hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, count, (int64_t)1, 0);
-    hipLaunchKernelGGL(k_compact, dim3((unsigned)((V + 255) / 256)), dim3(256), 0, s, flags, (int)V, idx, count);
+    hipLaunchKernelGGL(k_compact_flags, dim3((unsigned)((V + 255) / 256)), dim3(256), 0, s, flags, (int)V, idx, count);
     HIP_TRY(hipGetLastError());
     return 0;
 }

@@ -21,6 +21,8 @@ constexpr int kProjBK = 32;
 constexpr int kVoxPad = 256;
 // Threads per block of the GLS kernel.
 constexpr int kGlsBlock = 64;
+// Largest design width handled by the f64 MFMA GLS kernel (two 16-row tiles).
+constexpr int kGlsMfmaMaxP = 32;
 
 // Device-side description of the pair tables of one plan.
 struct PlanDev {
@@ -56,6 +58,12 @@ struct PlanDev {
     const uint16_t* Gh_bf16;
     const uint16_t* Gl_bf16;
     const float* g16_unscale;  // [K] 2^-2e_k: the fp16 operand of pair k holds G * 2^e_k
+    // GLS on f64 MFMA (p <= kGlsMfmaMaxP): tables padded to PG = 16 * ceil(p / 16) columns.
+    int PG;                    // 0 when p > kGlsMfmaMaxP
+    const double* G64p;        // [K][Tpad][PG]  G, zero outside t < T, j < p
+    const double* Xp;          // [Tpad][PG]     X, zero padded
+    const double* coefR;       // [K][Tpad][4]   (sqrt(1/v_t), c_{t-1}, alpha_{t-1}, 0); c = alpha = 0 at run starts
+    const double* XtX;         // [p][p]
     int proj_mode;             // 0 fp32 MFMA, 1 bf16 x3, 2 fp16 x3
     const float* ecoef;        // [n_chunks][T][2*NB] (alpha_t, 1/v_{t+1}) per chunk slot
     const float* ec;           // [n_chunks][T] c_t of the chunk's AR coefficient

@@ -40,6 +40,7 @@ __global__ void __launch_bounds__(64) k_tables(int T, int Tpad, int p, int K, co
                                                float* __restrict__ Gp, uint16_t* __restrict__ Ghf,
                                                uint16_t* __restrict__ Glf, uint16_t* __restrict__ Ghb,
                                                uint16_t* __restrict__ Glb, float* __restrict__ g16_unscale,
+                                               int PG, double* __restrict__ G64p, double* __restrict__ coefR,
                                                int* __restrict__ status) {
     const int k = blockIdx.x;
     if (k >= K) return;
@@ -174,6 +175,23 @@ __global__ void __launch_bounds__(64) k_tables(int T, int Tpad, int p, int K, co
             Glf[o] = __builtin_bit_cast(uint16_t, lf);
             Ghb[o] = __builtin_bit_cast(uint16_t, hb);
             Glb[o] = __builtin_bit_cast(uint16_t, lb);
+        }
+    }
+    // f64 MFMA GLS operands: G padded to PG columns, recurrence coefficients
+    // shifted so row t holds what step t needs (zeros past T and at run starts).
+    if (G64p) {
+        double* gp = G64p + (size_t)k * Tpad * PG;
+        for (int idx = tid; idx < T * p; idx += 64) {
+            const int t = idx / p, j = idx % p;
+            gp[(size_t)t * PG + j] = g[idx];
+        }
+        double* cr = coefR + (size_t)k * Tpad * 4;
+        for (int t = tid; t < T; t += 64) {
+            const bool start = (t == 0) || gap[t - 1] <= 0;
+            cr[4 * t] = sqrt(s2[t]);
+            cr[4 * t + 1] = start ? 0.0 : cc[t - 1];
+            cr[4 * t + 2] = start ? 0.0 : al[t - 1];
+            cr[4 * t + 3] = 0.0;
         }
     }
     if (Gp && p <= kLocalPP) {
@@ -885,30 +903,6 @@ __global__ void __launch_bounds__(256) k_select(PlanDev P, const float* __restri
     best_score[v] = cur;
 }
 
-// Counting sort of voxels by selected pair (the reference groups voxels by
-// unique (a,b), glm/arma.py:4784-4811), so GLS waves read one pair's tables.
-// Out-of-range pair indices go to the overflow bucket K.
-__device__ inline int bucket_of(int k, int K) { return (k >= 0 && k < K) ? k : K; }
-__global__ void k_hist(const int32_t* __restrict__ best, int V, int K, int32_t* __restrict__ cnt) {
-    const int v = blockIdx.x * blockDim.x + threadIdx.x;
-    if (v < V) atomicAdd(&cnt[bucket_of(best[v], K)], 1);
-}
-__global__ void k_scan(int32_t* __restrict__ cnt, int n) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) {
-        int s = 0;
-        for (int i = 0; i < n; ++i) {
-            const int c = cnt[i];
-            cnt[i] = s;
-            s += c;
-        }
-    }
-}
-__global__ void k_scatter(const int32_t* __restrict__ best, int V, int K, int32_t* __restrict__ off,
-                          int32_t* __restrict__ perm) {
-    const int v = blockIdx.x * blockDim.x + threadIdx.x;
-    if (v < V) perm[atomicAdd(&off[bucket_of(best[v], K)], 1)] = v;
-}
-
 // ---------------------------------------------------------------------------
 // (iv) FP64 GLS refit (glm/arma.py:4784-5601, X'X branch):
 //   beta = (X'R^-1X)^-1 X'R^-1 y = Lx^-T (G' y);  whitened residual
@@ -1152,35 +1146,6 @@ __global__ void k_local_center(PlanDev P, const float* __restrict__ props, int V
     if (P.n_b >= kWin) bi = min(max(bi, kWinHalf), P.n_b - 1 - kWinHalf);
     center[v] = ai * P.n_b + bi;
     flags[v] = (!finite || !(c >= conf_min)) ? 1 : 0;
-}
-
-// Group layout: each centre's voxels get a run of columns padded to
-// kLocalTile; tiles list (group, first column).  Single thread.
-__global__ void k_local_layout(int n_groups, const int32_t* __restrict__ cnt, int32_t* __restrict__ fill,
-                               int32_t* __restrict__ tile_group, int32_t* __restrict__ tile_col,
-                               int32_t* __restrict__ n_tiles) {
-    if (blockIdx.x != 0 || threadIdx.x != 0) return;
-    int off = 0, nt = 0;
-    for (int g = 0; g < n_groups; ++g) {
-        fill[g] = off;
-        const int c = cnt[g];
-        for (int i = 0; i < c; i += kLocalTile) {
-            tile_group[nt] = g;
-            tile_col[nt] = off + i;
-            ++nt;
-        }
-        off += (c + kLocalTile - 1) / kLocalTile * kLocalTile;
-    }
-    *n_tiles = nt;
-}
-
-__global__ void k_local_scatter(const int32_t* __restrict__ center, int V, int n_groups, int32_t* __restrict__ fill,
-                                int32_t* __restrict__ rows) {
-    const int v = blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= V) return;
-    const int g = center[v];
-    if (g < 0 || g >= n_groups) return;
-    rows[atomicAdd(&fill[g], 1)] = v;
 }
 
 __device__ inline int window_pair(const PlanDev& P, int ca, int cb, int slot) {
@@ -1430,12 +1395,6 @@ __global__ void k_zero_rows(float* __restrict__ p, int64_t ld, int r0, int r1, i
 }
 
 // Compaction of flagged voxels (order within the list is not significant).
-__global__ void k_compact(const int32_t* __restrict__ flags, int V, int32_t* __restrict__ idx,
-                          int32_t* __restrict__ count) {
-    const int v = blockIdx.x * blockDim.x + threadIdx.x;
-    if (v < V && flags[v]) idx[atomicAdd(count, 1)] = v;
-}
-
 __global__ void k_scatter_best(const int32_t* __restrict__ idx, int n, const int32_t* __restrict__ best_sub,
                                const float* __restrict__ score_sub, int32_t* __restrict__ best,
                                float* __restrict__ score) {

@@ -1,0 +1,384 @@
+// Voxel grouping (bucket sort) and the FP64 GLS refit on f64 MFMA.
+// Included by reml_api.hip after reml_kernels.hip (one translation unit).
+#pragma once
+
+#include "reml_kernels.h"
+
+namespace remlk {
+
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
+// ---------------------------------------------------------------------------
+// Bucket sort of voxels by an integer key in [0, nb); other keys are dropped.
+// Group b occupies slots [off[b], off[b] + cnt[b]) of perm; off[b] is a
+// multiple of `pad` and the unused slots up to off[b+1] hold -1.  Counting is
+// block-aggregated in LDS so that a heavily populated key costs one global
+// atomic per block instead of one per voxel.  Slot order inside a group is not
+// specified (the consumers compute each voxel independently).
+// ---------------------------------------------------------------------------
+constexpr int kSortBlock = 256;
+constexpr int kSortItems = 16;    // keys per thread
+constexpr int kSortLdsMax = 8192; // buckets counted in LDS (two int32 arrays)
+
+__device__ __forceinline__ bool key_ok(int b, int nb) { return b >= 0 && b < nb; }
+
+__global__ void __launch_bounds__(kSortBlock) k_bucket_count(const int32_t* __restrict__ keys, int V, int nb,
+                                                             int32_t* __restrict__ cnt) {
+    extern __shared__ int32_t sh_cnt[];
+    for (int i = threadIdx.x; i < nb; i += kSortBlock) sh_cnt[i] = 0;
+    __syncthreads();
+    const int base = blockIdx.x * kSortBlock * kSortItems + threadIdx.x;
+#pragma unroll 4
+    for (int it = 0; it < kSortItems; ++it) {
+        const int v = base + it * kSortBlock;
+        if (v < V) {
+            const int b = keys[v];
+            if (key_ok(b, nb)) atomicAdd(&sh_cnt[b], 1);
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < nb; i += kSortBlock)
+        if (sh_cnt[i]) atomicAdd(&cnt[i], sh_cnt[i]);
+}
+
+__global__ void __launch_bounds__(kSortBlock) k_bucket_place(const int32_t* __restrict__ keys, int V, int nb,
+                                                             int32_t* __restrict__ cursor, int32_t* __restrict__ perm) {
+    extern __shared__ int32_t sh_cnt[];
+    int32_t* sh_base = sh_cnt + nb;
+    for (int i = threadIdx.x; i < nb; i += kSortBlock) sh_cnt[i] = 0;
+    __syncthreads();
+    const int base = blockIdx.x * kSortBlock * kSortItems + threadIdx.x;
+    int rank[kSortItems], key[kSortItems];
+#pragma unroll
+    for (int it = 0; it < kSortItems; ++it) {
+        const int v = base + it * kSortBlock;
+        const int b = v < V ? keys[v] : -1;
+        key[it] = key_ok(b, nb) ? b : -1;
+        rank[it] = key[it] >= 0 ? atomicAdd(&sh_cnt[key[it]], 1) : -1;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < nb; i += kSortBlock)
+        if (sh_cnt[i]) sh_base[i] = atomicAdd(&cursor[i], sh_cnt[i]);
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < kSortItems; ++it)
+        if (key[it] >= 0) perm[sh_base[key[it]] + rank[it]] = base + it * kSortBlock;
+}
+
+// Many-bucket variants (nb > kSortLdsMax): one global atomic per voxel.
+__global__ void k_bucket_count_global(const int32_t* __restrict__ keys, int V, int nb, int32_t* __restrict__ cnt) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v < V && key_ok(keys[v], nb)) atomicAdd(&cnt[keys[v]], 1);
+}
+__global__ void k_bucket_place_global(const int32_t* __restrict__ keys, int V, int nb, int32_t* __restrict__ cursor,
+                                      int32_t* __restrict__ perm) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v < V && key_ok(keys[v], nb)) perm[atomicAdd(&cursor[keys[v]], 1)] = v;
+}
+
+// Exclusive scan of the padded counts (one block): off[0..nb], cursor = off.
+__global__ void __launch_bounds__(1024) k_bucket_scan(const int32_t* __restrict__ cnt, int nb, int pad,
+                                                      int32_t* __restrict__ off, int32_t* __restrict__ cursor) {
+    __shared__ int32_t part[1024];
+    const int tid = threadIdx.x;
+    const int per = (nb + 1023) / 1024;
+    const int b0 = tid * per, b1 = min(nb, b0 + per);
+    int s = 0;
+    for (int i = b0; i < b1; ++i) s += (cnt[i] + pad - 1) / pad * pad;
+    part[tid] = s;
+    __syncthreads();
+    for (int d = 1; d < 1024; d <<= 1) {
+        const int x = tid >= d ? part[tid - d] : 0;
+        __syncthreads();
+        part[tid] += x;
+        __syncthreads();
+    }
+    int run = tid ? part[tid - 1] : 0;
+    for (int i = b0; i < b1; ++i) {
+        off[i] = run;
+        cursor[i] = run;
+        run += (cnt[i] + pad - 1) / pad * pad;
+    }
+    if (tid == 1023) off[nb] = part[1023];
+}
+
+// Stream compaction of flagged voxels: one global atomic per wave.
+__global__ void __launch_bounds__(256) k_compact_flags(const int32_t* __restrict__ flags, int V,
+                                                       int32_t* __restrict__ idx, int32_t* __restrict__ count) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool f = v < V && flags[v] != 0;
+    const uint64_t m = __ballot(f);
+    if (m == 0) return;
+    const int lane = threadIdx.x & 63;
+    const int n = __popcll(m);
+    int base = 0;
+    if (lane == __ffsll((unsigned long long)m) - 1) base = atomicAdd(count, n);
+    base = __shfl(base, __ffsll((unsigned long long)m) - 1);
+    if (f) idx[base + __popcll(m & ((1ull << lane) - 1))] = v;
+}
+
+// ---------------------------------------------------------------------------
+// Local-search tiles from the padded group offsets: tile t covers columns
+// [t*kLocalTile, (t+1)*kLocalTile) of the group whose run contains them.
+// ---------------------------------------------------------------------------
+__global__ void k_local_tiles(const int32_t* __restrict__ off, int ng, int max_tiles, int32_t* __restrict__ tile_group,
+                              int32_t* __restrict__ tile_col, int32_t* __restrict__ n_tiles) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int total = off[ng];
+    if (t == 0) *n_tiles = total / kLocalTile;
+    if (t >= max_tiles) return;
+    const int col = t * kLocalTile;
+    if (col >= total) {
+        tile_group[t] = -1;
+        tile_col[t] = col;
+        return;
+    }
+    int lo = 0, hi = ng;  // off[lo] <= col < off[hi]
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (off[mid] <= col)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    tile_group[t] = lo;
+    tile_col[t] = col;
+}
+
+// ---------------------------------------------------------------------------
+// (iv) FP64 GLS refit on f64 MFMA, one wave per 64 voxels of one pair.
+// Per voxel (y normalised by its scale s):
+//   h = G_k' y            (G_k = R^-1 X Lx^-T; MFMA, 16-row tiles)
+//   q = X' y              (MFMA)
+//   E = |L^-1 y|^2        (FP64 innovations recurrence, per lane)
+//   beta = Lx^-T h,  ssw = E - |h|^2 (= |L^-1 (y - X beta)|^2),
+//   ssres = y'y - 2 beta'q + beta' X'X beta,  sstot = y'y - T ybar^2,
+// which are the reference's whitened-residual sums and CoD R^2
+// (glm/arma.py:4784-5601) rewritten as sums over the series.
+// The voxels arrive grouped by pair with every group padded to 64 slots, so
+// the pair and its tables are wave-uniform.  MFMA k-slot q of step m covers
+// time t0 + 8q + m, so each lane reads runs of consecutive samples of a voxel row.
+// 16x16x4 f64 layouts: A lane l = A[l%16][l/16], B lane l = B[l/16][l%16],
+// D lane l element r = D[l/16 + 4r][l%16].
+// ---------------------------------------------------------------------------
+// Samples [t0, t0 + N) of a row; zero past T.  `full` (wave-uniform) says the
+// span lies inside the row and is 16-byte aligned; otherwise each sample is
+// read at a clamped index and zeroed past T (no divergent branch).
+template <int N>
+__device__ __forceinline__ void load_span(const float* __restrict__ row, int T, int t0, bool full, float (&out)[N]) {
+    if (full) {
+#pragma unroll
+        for (int i = 0; i < N / 4; ++i) {
+            const f32x4 q = *reinterpret_cast<const f32x4*>(row + t0 + 4 * i);
+            out[4 * i] = q.x;
+            out[4 * i + 1] = q.y;
+            out[4 * i + 2] = q.z;
+            out[4 * i + 3] = q.w;
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+            const float x = row[min(t0 + i, T - 1)];
+            out[i] = (t0 + i < T) ? x : 0.f;
+        }
+    }
+}
+
+// One 32-sample block of the GLS accumulation (see k_gls_mfma).
+template <int NT>
+__device__ __forceinline__ void gls_block(int t0, int T, bool full, int lane, int j, int kq, const float* const (&rowB)[4],
+                                          const float* rowO, const double* __restrict__ Gk,
+                                          const double* __restrict__ Xp, const double* __restrict__ cR, double* cbuf,
+                                          f64x4 (&accG)[4][NT], f64x4 (&accX)[4][NT], double& E, double& sy,
+                                          double& syy, double& e, double& yprev) {
+    constexpr int PG = 16 * NT;
+    const double c0 = cR[(size_t)t0 * 4 + 2 * lane], c1 = cR[(size_t)t0 * 4 + 2 * lane + 1];
+    __syncthreads();  // the previous block's coefficient reads are done
+    cbuf[2 * lane] = c0;
+    cbuf[2 * lane + 1] = c1;
+    __syncthreads();
+    // two halves of 16 time points: MFMA steps m in [4 hf, 4 hf + 4) and the
+    // recurrence over t0 + 16 hf .. + 16
+#pragma unroll
+    for (int hf = 0; hf < 2; ++hf) {
+        float yb[4][4];
+#pragma unroll
+        for (int g = 0; g < 4; ++g) load_span<4>(rowB[g], T, t0 + 8 * kq + 4 * hf, full, yb[g]);
+        double ag[4][NT], ax[4][NT];
+#pragma unroll
+        for (int m = 0; m < 4; ++m)
+#pragma unroll
+            for (int rt = 0; rt < NT; ++rt) {
+                const size_t o = (size_t)(t0 + 8 * kq + 4 * hf + m) * PG + 16 * rt + j;
+                ag[m][rt] = Gk[o];
+                ax[m][rt] = Xp[o];
+            }
+        float yo[16];
+        load_span<16>(rowO, T, t0 + 16 * hf, full, yo);
+#pragma unroll
+        for (int m = 0; m < 4; ++m)
+#pragma unroll
+            for (int g = 0; g < 4; ++g) {
+                const double b = (double)yb[g][m];
+#pragma unroll
+                for (int rt = 0; rt < NT; ++rt) {
+                    accG[g][rt] = __builtin_amdgcn_mfma_f64_16x16x4f64(ag[m][rt], b, accG[g][rt], 0, 0, 0);
+                    accX[g][rt] = __builtin_amdgcn_mfma_f64_16x16x4f64(ax[m][rt], b, accX[g][rt], 0, 0, 0);
+                }
+            }
+        // Rows t >= T have zero coefficients and zero samples: they add nothing.
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            const int r = 16 * hf + i;
+            const double yt = (double)yo[i];
+            const double gs = cbuf[4 * r], c = cbuf[4 * r + 1], al = cbuf[4 * r + 2];
+            e = fma(al, e, fma(-c, yprev, yt));
+            yprev = yt;
+            const double w = e * gs;
+            E = fma(w, w, E);
+            sy += yt;
+            syy = fma(yt, yt, syy);
+        }
+    }
+}
+
+template <int NT>
+__global__ void __launch_bounds__(64) k_gls_mfma(PlanDev P, const float* __restrict__ Y, int64_t ldY,
+                                                 const float* __restrict__ scale, const int32_t* __restrict__ perm,
+                                                 const int32_t* __restrict__ best, float* __restrict__ betas,
+                                                 float* __restrict__ tstats, float* __restrict__ fstats,
+                                                 float* __restrict__ sigma2, float* __restrict__ r2) {
+    constexpr int PG = 16 * NT;
+    constexpr int TP = PG + 1;  // transpose pitch (doubles)
+    __shared__ double cbuf[32 * 4];
+    __shared__ double tbuf[64 * TP];
+    const int lane = threadIdx.x;
+    const int64_t slot0 = (int64_t)blockIdx.x * 64;
+    const int v0 = __builtin_amdgcn_readfirstlane(perm[slot0]);
+    if (v0 < 0) return;  // padding or beyond the last group
+    const int k = __builtin_amdgcn_readfirstlane(best[v0]);
+    const int v = perm[slot0 + lane];
+    const int T = P.T, Tpad = P.Tpad, p = P.p;
+    const int j = lane & 15, kq = lane >> 4;
+    const double* __restrict__ Gk = P.G64p + (size_t)k * Tpad * PG;
+    const double* __restrict__ Xp = P.Xp;
+    const double* __restrict__ cR = P.coefR + (size_t)k * Tpad * 4;
+    const bool vec4 = (ldY & 3) == 0 && (reinterpret_cast<uintptr_t>(Y) & 15) == 0;
+    const float* rowB[4];
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
+        const int vb = perm[slot0 + 16 * g + j];
+        rowB[g] = Y + (size_t)(vb >= 0 ? vb : v0) * ldY;  // padding columns are computed and discarded
+    }
+    const float* rowO = Y + (size_t)(v >= 0 ? v : v0) * ldY;
+
+    f64x4 accG[4][NT], accX[4][NT];
+#pragma unroll
+    for (int g = 0; g < 4; ++g)
+#pragma unroll
+        for (int rt = 0; rt < NT; ++rt) {
+            accG[g][rt] = f64x4{0.0, 0.0, 0.0, 0.0};
+            accX[g][rt] = f64x4{0.0, 0.0, 0.0, 0.0};
+        }
+    double E = 0.0, sy = 0.0, syy = 0.0, e = 0.0, yprev = 0.0;
+    for (int t0 = 0; t0 < Tpad; t0 += 32)
+        gls_block<NT>(t0, T, vec4 && t0 + 32 <= T, lane, j, kq, rowB, rowO, Gk, Xp, cR, cbuf, accG, accX, E, sy, syy,
+                      e, yprev);
+    // Accumulators -> one voxel per lane through LDS: h first (beta, |h|^2),
+    // then q (read back element by element).
+#pragma unroll
+    for (int g = 0; g < 4; ++g)
+#pragma unroll
+        for (int rt = 0; rt < NT; ++rt)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) tbuf[(16 * g + j) * TP + 16 * rt + kq + 4 * r] = accG[g][rt][r];
+    __syncthreads();
+    const double s = (v >= 0 && scale) ? (double)scale[v] : 1.0;
+    const double inv = 1.0 / s, inv2 = inv * inv;
+    E *= inv2;
+    sy *= inv;
+    syy *= inv2;
+    // beta = Lx^-T h (back substitution on the upper factor Lx'), h = G'y / s
+    const double* __restrict__ L = P.Lx + (size_t)k * p * p;
+    double be[PG];
+    double hh = 0.0;
+#pragma unroll
+    for (int jj = PG - 1; jj >= 0; --jj) {
+        if (jj >= p) {
+            be[jj] = 0.0;
+            continue;
+        }
+        const double hj = tbuf[lane * TP + jj] * inv;
+        hh = fma(hj, hj, hh);
+        double acc = hj;
+#pragma unroll
+        for (int m = jj + 1; m < PG; ++m)
+            if (m < p) acc -= L[m * p + jj] * be[m];
+        be[jj] = acc / L[jj * p + jj];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int g = 0; g < 4; ++g)
+#pragma unroll
+        for (int rt = 0; rt < NT; ++rt)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) tbuf[(16 * g + j) * TP + 16 * rt + kq + 4 * r] = accX[g][rt][r];
+    __syncthreads();
+    if (v < 0) return;
+    double bq = 0.0, bxb = 0.0;
+    const double* __restrict__ XtX = P.XtX;
+#pragma unroll
+    for (int a = 0; a < PG; ++a) {
+        if (a >= p) continue;
+        bq = fma(be[a], tbuf[lane * TP + a] * inv, bq);
+        double ra = 0.0;
+#pragma unroll
+        for (int b2 = 0; b2 < PG; ++b2)
+            if (b2 < p) ra = fma(XtX[a * p + b2], be[b2], ra);
+        bxb = fma(be[a], ra, bxb);
+    }
+    const double ssw = E - hh;
+    const double ssres = syy - 2.0 * bq + bxb;
+    const double sstot = syy - sy * (sy / T);
+    const int dof = max(1, T - p);
+    const double sig = ssw / dof;
+    const double* __restrict__ Ai = P.Ainv + (size_t)k * p * p;
+#pragma unroll
+    for (int jj = 0; jj < PG; ++jj)
+        if (jj < p) {
+            const double se = sqrt(sig * Ai[jj * p + jj]);
+            tstats[(size_t)v * p + jj] = (float)(be[jj] / (se + 1e-10));
+            betas[(size_t)v * p + jj] = (float)(be[jj] * s);
+        }
+    const double* __restrict__ F = P.Fm + (size_t)k * p * p;
+    double quad = 0.0;
+#pragma unroll
+    for (int a = 0; a < PG; ++a) {
+        if (a >= p) continue;
+        double ra = 0.0;
+#pragma unroll
+        for (int b2 = 0; b2 < PG; ++b2)
+            if (b2 < p) ra += F[a * p + b2] * be[b2];
+        quad += be[a] * ra;
+    }
+    fstats[v] = (float)(quad / (sig * P.n_task + 1e-10));
+    sigma2[v] = (float)(sig * s * s);
+    r2[v] = (float)(sstot <= 1e-10 ? 0.0 : fmin(1.0 - ssres / sstot, 1.0));
+}
+
+// NaN statistics for voxels whose pair index is not a valid pair.
+__global__ void k_gls_invalid(const int32_t* __restrict__ best, int V, int K, int p, float* __restrict__ betas,
+                              float* __restrict__ tstats, float* __restrict__ fstats, float* __restrict__ sigma2,
+                              float* __restrict__ r2) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= V) return;
+    const int k = best[v];
+    if (k >= 0 && k < K) return;
+    for (int j = 0; j < p; ++j) {
+        betas[(size_t)v * p + j] = NAN;
+        tstats[(size_t)v * p + j] = NAN;
+    }
+    fstats[v] = sigma2[v] = r2[v] = NAN;
+}
+
+}  // namespace remlk

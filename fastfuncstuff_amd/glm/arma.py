@@ -337,7 +337,9 @@ class RemlGrid(Mapping):
         rw = torch.empty(V, self.T, dtype=torch.float64, device=dev) if want_resid_w else None
         res = torch.empty(V, self.T, **f32) if want_residuals else None
         pred = torch.empty(V, self.T, **f32) if want_predicted else None
-        work = torch.empty(V + self.K + 1, dtype=torch.int32, device=dev)
+        n_work = ctypes.c_int64()
+        _lib.check(_lib.load().ffs_reml_gls_workspace(self._handle, V, ctypes.byref(n_work)), "gls_workspace")
+        work = torch.empty(int(n_work.value), dtype=torch.int32, device=dev)
         _lib.check(_lib.load().ffs_reml_gls(
             self._handle, _lib.ptr(Y), Y.stride(0), _lib.ptr(scale), _lib.ptr(best), V, _lib.ptr(out["betas"]),
             _lib.ptr(out["tstats"]), _lib.ptr(out["fstats"]), _lib.ptr(out["sigma2"]), _lib.ptr(out["r2"]),

@@ -108,9 +108,14 @@ int ffs_reml_select(const ffs_reml_plan* plan, const float* S, int64_t ldS, int6
  * Outputs: betas, tstats (V x p), fstats, sigma2, r2 (V).  Optional (nullable,
  * row stride ldR): resid_w (float64 whitened residuals), resid (float32
  * Y - X beta) and pred (float32 X beta), all in original units.
- * work: caller-provided int32 workspace of V + K + 1 elements (voxel sort;
- * nullable = no sort).
+ * work: caller-provided int32 workspace of ffs_reml_gls_workspace(V) elements
+ * (voxels grouped by pair; nullable = no grouping, slower).  Without series
+ * outputs and with p <= 32 the fit runs on f64 MFMA over pair-uniform waves.
  */
+/* int32 elements of the GLS workspace for V voxels (glm/arma.py:4784-4811 groups
+ * voxels by unique (a, b); the workspace holds that grouping). */
+int ffs_reml_gls_workspace(const ffs_reml_plan* plan, int64_t V, int64_t* n_int32);
+
 int ffs_reml_gls(const ffs_reml_plan* plan, const float* Y, int64_t ldY, const float* scale, const int32_t* best,
                  int64_t V, float* betas, float* tstats, float* fstats, float* sigma2, float* r2, double* resid_w,
                  int64_t ldR, float* resid, float* pred, int32_t* work, void* stream);

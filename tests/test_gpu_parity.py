@@ -21,9 +21,9 @@ F64 = torch.float64
 
 
 def _case(name):
-    rng = np.random.default_rng({"plain": 1, "censored": 2, "afni": 3, "ragged": 4}[name])
-    if name in ("plain", "ragged"):
-        T, p, V = 100, 6, 256 if name == "plain" else 333
+    rng = np.random.default_rng({"plain": 1, "censored": 2, "afni": 3, "ragged": 4, "wide": 5}[name])
+    if name in ("plain", "ragged", "wide"):
+        T, p, V = 100, {"plain": 6, "ragged": 6, "wide": 40}[name], {"plain": 256, "ragged": 333, "wide": 130}[name]
         X = np.column_stack([np.ones(T), np.linspace(-1, 1, T), rng.standard_normal((T, p - 2))])
         run_starts, tau = None, None
         ag = bg = torch.linspace(-0.8, 0.8, 33)
@@ -66,7 +66,7 @@ def dev():
     return torch.device("cuda", 0)
 
 
-@pytest.mark.parametrize("name", ["plain", "censored", "afni", "ragged"])
+@pytest.mark.parametrize("name", ["plain", "censored", "afni", "ragged", "wide"])
 def test_surface_selection_and_gls_match_oracle(dev, name):
     from fastfuncstuff_amd.glm.arma import RemlGrid
 
@@ -99,12 +99,20 @@ def test_surface_selection_and_gls_match_oracle(dev, name):
         assert int(diff.sum()) <= max(1, V // 100)
 
     best, _ = grid.select(S.float().to(dev), hierarchical=True)
-    out = grid.gls(Yd, scale, best, want_resid_w=True)
     idx = best.cpu().long()
     Yn = torch.from_numpy(Y).double() / s_ref[:, None]
-    betas, tst, fst, s2, r2, _, rw = O.gls_refit(torch.from_numpy(X), Yn, grid_o, idx, dtype=F64)
-    betas = betas * s_ref[:, None]
-    s2 = s2 * s_ref**2
+    ref = O.gls_refit(torch.from_numpy(X), Yn, grid_o, idx, dtype=F64)
+    # summary statistics only (f64 MFMA path for p <= 32), then with the
+    # whitened residual series (per-lane path)
+    _check_gls(grid.gls(Yd, scale, best), ref, s_ref)
+    out = grid.gls(Yd, scale, best, want_resid_w=True)
+    _check_gls(out, ref, s_ref)
+    rw_ref = ref[6] * s_ref[:, None]
+    assert float((out["resid_w"].cpu() - rw_ref).abs().max()) <= 1e-6 * float(rw_ref.abs().max())
+
+
+def _check_gls(out, ref, s_ref):
+    betas, tst, fst, s2, r2 = ref[0] * s_ref[:, None], ref[1], ref[2], ref[3] * s_ref**2, ref[4]
     gb = out["betas"].cpu().double()
     denom = torch.maximum(betas.abs(), 1e-3 * s_ref[:, None])
     assert float(((gb - betas).abs() / denom).max()) <= 1e-5
@@ -113,8 +121,6 @@ def test_surface_selection_and_gls_match_oracle(dev, name):
     assert float(((out["fstats"].cpu().double() - fst).abs() / fst.abs().clamp(min=1)).max()) <= 1e-5
     assert float(((out["sigma2"].cpu().double() - s2).abs() / s2).max()) <= 1e-5
     assert float((out["r2"].cpu().double() - r2).abs().max()) <= 1e-6
-    rw_ref = rw * s_ref[:, None]
-    assert float((out["resid_w"].cpu() - rw_ref).abs().max()) <= 1e-6 * float(rw_ref.abs().max())
 
 
 def test_white_pairs_tie_bitwise(dev):
@@ -183,3 +189,31 @@ def test_split_projection_matches_oracle(dev, name, mode):
     assert bool((gap[diff] <= tol["tie"]).all()), f"{int(diff.sum())} mismatches beyond near-ties"
     with pytest.raises(ValueError):
         grid.set_projection("fp8")
+
+
+def test_gls_groups_spanning_waves(dev):
+    """Pair groups larger than one wave, one-voxel groups and invalid pairs in
+    one call: every voxel matches the oracle's refit at its pair."""
+    from fastfuncstuff_amd.glm.arma import RemlGrid
+
+    print("[parity] gls grouping", flush=True)
+    X, Y, ag, bg, run_starts, tau = _case("censored")
+    grid_o = O.precompute_grid(torch.from_numpy(X), ag, bg, run_starts, tau, F64)
+    grid = RemlGrid(torch.from_numpy(X), ag, bg, dev, run_starts=run_starts, tau=tau)
+    V = Y.shape[0]
+    rng = np.random.default_rng(9)
+    idx = np.where(rng.random(V) < 0.7, 17, rng.integers(0, grid.K, V))  # one group of ~140 voxels
+    idx[5] = grid.K + 3  # invalid pair -> NaN statistics
+    idx[6] = -1
+    best = torch.from_numpy(idx.astype(np.int32)).to(dev)
+    Yd = torch.from_numpy(Y).to(dev)
+    _, scale, _ = grid.prep(Yd, normalize=True)
+    out = grid.gls(Yd, scale, best)
+    s_ref = torch.from_numpy(Y).double().std(dim=1).clamp(min=1e-8)
+    ok = torch.ones(V, dtype=torch.bool)
+    ok[5] = ok[6] = False
+    ref = O.gls_refit(torch.from_numpy(X), torch.from_numpy(Y).double()[ok] / s_ref[ok, None], grid_o,
+                      torch.from_numpy(idx)[ok].long(), dtype=F64)
+    _check_gls({k: v[ok.to(dev)] for k, v in out.items() if v is not None}, ref, s_ref[ok])
+    for v in (5, 6):
+        assert bool(torch.isnan(out["betas"][v]).all()) and bool(torch.isnan(out["sigma2"][v]))
