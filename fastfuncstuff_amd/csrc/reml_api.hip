@@ -15,6 +15,7 @@
 // launches below instantiate them.
 #include "reml_kernels.hip"
 #include "reml_sort_gls.hip"
+#include "reml_prep_mfma.hip"
 
 using namespace remlk;
 
@@ -203,13 +204,16 @@ extern "C" int ffs_reml_plan_create(ffs_reml_plan** out, int32_t T, int32_t p, c
     HIP_TRY(hipMemsetAsync(Ghb, 0, 2 * n16, s));
     HIP_TRY(hipMemsetAsync(Glb, 0, 2 * n16, s));
     // f64 MFMA GLS tables
-    double *G64p = nullptr, *coefR = nullptr, *dXp = nullptr, *dXtX = nullptr;
+    double *G64p = nullptr, *coefR = nullptr, *dXp = nullptr, *dXtX = nullptr, *dPp = nullptr;
     d.PG = p <= kGlsMfmaMaxP ? 16 * ((p + 15) / 16) : 0;
     if (d.PG) {
         const int PG = d.PG;
-        std::vector<double> Xp((size_t)d.Tpad * PG, 0.0), XtX((size_t)p * p, 0.0);
+        std::vector<double> Xp((size_t)d.Tpad * PG, 0.0), Pp((size_t)d.Tpad * PG, 0.0), XtX((size_t)p * p, 0.0);
         for (int t = 0; t < T; ++t)
-            for (int j = 0; j < p; ++j) Xp[(size_t)t * PG + j] = X[(size_t)t * p + j];
+            for (int j = 0; j < p; ++j) {
+                Xp[(size_t)t * PG + j] = X[(size_t)t * p + j];
+                Pp[(size_t)t * PG + j] = pinvT[(size_t)t * p + j];
+            }
         for (int a = 0; a < p; ++a)
             for (int b = 0; b < p; ++b) {
                 double acc = 0.0;
@@ -217,7 +221,8 @@ extern "C" int ffs_reml_plan_create(ffs_reml_plan** out, int32_t T, int32_t p, c
                 XtX[(size_t)a * p + b] = acc;
             }
         if ((rc = P->alloc(&G64p, (size_t)K * d.Tpad * PG)) || (rc = P->alloc(&coefR, (size_t)K * d.Tpad * 4)) ||
-            (rc = upload(P, &dXp, Xp.data(), Xp.size(), s)) || (rc = upload(P, &dXtX, XtX.data(), XtX.size(), s)))
+            (rc = upload(P, &dXp, Xp.data(), Xp.size(), s)) || (rc = upload(P, &dPp, Pp.data(), Pp.size(), s)) ||
+            (rc = upload(P, &dXtX, XtX.data(), XtX.size(), s)))
             return rc;
         HIP_TRY(hipMemsetAsync(G64p, 0, sizeof(double) * K * d.Tpad * PG, s));
         HIP_TRY(hipMemsetAsync(coefR, 0, sizeof(double) * K * d.Tpad * 4, s));
@@ -276,6 +281,7 @@ extern "C" int ffs_reml_plan_create(ffs_reml_plan** out, int32_t T, int32_t p, c
     d.coefR = coefR;
     d.Xp = dXp;
     d.XtX = dXtX;
+    d.pinvTp = dPp;
     d.proj_mode = 0;
     P->pinvT = dpinv;
     P->ldR = ldR;
@@ -328,6 +334,27 @@ static int prep_impl(const ffs_reml_plan* P, const float* Y, int64_t ldY, int64_
     if (ncp > 0x7fffffff || V > 0x7fffffff) return fail(-3, "prep: batch too large; split it");
     const int p = P->d.p;
     const int v = (int)V, nc = (int)ncols, ncpi = (int)ncp;
+    if (P->d.PG) {
+        // f64 MFMA path: also writes the zero rows T..Tpad-1
+        const dim3 grid((unsigned)(ncp / 64)), block(64);
+        if (P->d.PG == 16) {
+            if (feats)
+                hipLaunchKernelGGL((k_prep_mfma<1, 10>), grid, block, 0, s, P->d, Y, ldY, v, rows, nc, ncpi, normalize,
+                                   Yt, ldYt, scale, feats);
+            else
+                hipLaunchKernelGGL((k_prep_mfma<1, 0>), grid, block, 0, s, P->d, Y, ldY, v, rows, nc, ncpi, normalize,
+                                   Yt, ldYt, scale, feats);
+        } else {
+            if (feats)
+                hipLaunchKernelGGL((k_prep_mfma<2, 10>), grid, block, 0, s, P->d, Y, ldY, v, rows, nc, ncpi, normalize,
+                                   Yt, ldYt, scale, feats);
+            else
+                hipLaunchKernelGGL((k_prep_mfma<2, 0>), grid, block, 0, s, P->d, Y, ldY, v, rows, nc, ncpi, normalize,
+                                   Yt, ldYt, scale, feats);
+        }
+        HIP_TRY(hipGetLastError());
+        return 0;
+    }
     if (p <= 8)
         launch_prep<8>(P, Y, ldY, v, rows, nc, ncpi, Yt, ldYt, scale, normalize, feats, s);
     else if (p <= 16)

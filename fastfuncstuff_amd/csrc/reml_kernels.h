@@ -62,6 +62,7 @@ struct PlanDev {
     int PG;                    // 0 when p > kGlsMfmaMaxP
     const double* G64p;        // [K][Tpad][PG]  G, zero outside t < T, j < p
     const double* Xp;          // [Tpad][PG]     X, zero padded
+    const double* pinvTp;      // [Tpad][PG]     pinv(X)', zero padded
     const double* coefR;       // [K][Tpad][4]   (sqrt(1/v_t), c_{t-1}, alpha_{t-1}, 0); c = alpha = 0 at run starts
     const double* XtX;         // [p][p]
     int proj_mode;             // 0 fp32 MFMA, 1 bf16 x3, 2 fp16 x3

@@ -1,0 +1,165 @@
+// Normalise + residualise on f64 MFMA (p <= 32).  Included by reml_api.hip
+// after reml_sort_gls.hip (one translation unit).
+#pragma once
+
+#include "reml_kernels.h"
+
+namespace remlk {
+
+__device__ __forceinline__ int prep_voxel(int c, int ncols, const int32_t* __restrict__ rows, int V) {
+    const int v = c < ncols ? (rows ? rows[c] : c) : -1;
+    return v < V ? v : -1;
+}
+
+// ---------------------------------------------------------------------------
+// One wave per 64 output columns (voxel of column c = rows ? rows[c] : c),
+// following glm/arma.py:1554-1571 (std normalisation, projection on the
+// design's pseudo-inverse, residual):
+//   pass 1: c = pinv(X) y on MFMA (k-slot q of step m = time t0 + 4q + m) and
+//           the sums of y, y^2 -> s = max(float(sd), 1e-8);
+//   pass 2: X c on MFMA for 16-row time tiles, r = y / s - X c / s written to
+//           Yt[t][column] (rows T..Tpad-1 come out zero); optional lag-1..NF
+//           autocorrelations of r through an LDS tile.
+// The A operand of pass 2 permutes the tile rows so that a lane's four output
+// rows are four consecutive time points.  Every voxel's arithmetic is the same
+// whatever column it lands in.
+// ---------------------------------------------------------------------------
+template <int NT, int NF>
+__global__ void __launch_bounds__(64) k_prep_mfma(PlanDev P, const float* __restrict__ Y, int64_t ldY, int V,
+                                                  const int32_t* __restrict__ rows, int ncols, int ncols_pad,
+                                                  int normalize, float* __restrict__ Yt, int64_t ldYt,
+                                                  float* __restrict__ scale, float* __restrict__ feats) {
+    constexpr int PG = 16 * NT;
+    constexpr int RB = 65;  // feature tile pitch (floats)
+    __shared__ float rbuf[NF > 0 ? 16 * RB : 1];
+    const int lane = threadIdx.x;
+    const int col0 = blockIdx.x * 64;
+    if (col0 >= ncols_pad) return;
+    const int T = P.T, Tpad = P.Tpad;
+    const int j = lane & 15, kq = lane >> 4;
+    const bool vec4 = (ldY & 3) == 0 && (reinterpret_cast<uintptr_t>(Y) & 15) == 0;
+    int vB[4];
+    const float* rowB[4];
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
+        vB[g] = prep_voxel(col0 + 16 * g + j, ncols, rows, V);
+        rowB[g] = Y + (size_t)(vB[g] >= 0 ? vB[g] : 0) * ldY;
+    }
+    const double* __restrict__ Pp = P.pinvTp;
+    const double* __restrict__ Xp = P.Xp;
+
+    // ---- pass 1 --------------------------------------------------------
+    f64x4 acc[4][NT];
+#pragma unroll
+    for (int g = 0; g < 4; ++g)
+#pragma unroll
+        for (int rt = 0; rt < NT; ++rt) acc[g][rt] = f64x4{0.0, 0.0, 0.0, 0.0};
+    double s1[4] = {0.0, 0.0, 0.0, 0.0}, s2[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int t0 = 0; t0 < Tpad; t0 += 16) {
+        const bool full = vec4 && t0 + 16 <= T;
+        float yb[4][4];
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+            load_span<4>(rowB[g], T, t0 + 4 * kq, full, yb[g]);
+#pragma unroll
+            for (int m = 0; m < 4; ++m) yb[g][m] = vB[g] >= 0 ? yb[g][m] : 0.f;
+        }
+        double a[4][NT];
+#pragma unroll
+        for (int m = 0; m < 4; ++m)
+#pragma unroll
+            for (int rt = 0; rt < NT; ++rt) a[m][rt] = Pp[(size_t)(t0 + 4 * kq + m) * PG + 16 * rt + j];
+#pragma unroll
+        for (int m = 0; m < 4; ++m)
+#pragma unroll
+            for (int g = 0; g < 4; ++g) {
+                const double b = (double)yb[g][m];
+                s1[g] += b;
+                s2[g] = fma(b, b, s2[g]);
+#pragma unroll
+                for (int rt = 0; rt < NT; ++rt)
+                    acc[g][rt] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[m][rt], b, acc[g][rt], 0, 0, 0);
+            }
+    }
+    double inv[4];
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
+        s1[g] += __shfl_xor(s1[g], 16);
+        s1[g] += __shfl_xor(s1[g], 32);
+        s2[g] += __shfl_xor(s2[g], 16);
+        s2[g] += __shfl_xor(s2[g], 32);
+        double s = 1.0;
+        if (normalize) {
+            const double mean = s1[g] / T;
+            const double var = T > 1 ? (s2[g] - T * mean * mean) / (T - 1) : 0.0;
+            const double sd = var > 0.0 ? sqrt(var) : 0.0;
+            s = (double)fmaxf((float)sd, 1e-8f);
+        }
+        inv[g] = 1.0 / s;
+        if (kq == 0 && vB[g] >= 0 && scale) scale[vB[g]] = (float)s;
+    }
+    if (!Yt && NF == 0) return;
+
+    // ---- pass 2 --------------------------------------------------------
+    // A row i of a tile holds time tt + 4 (i % 4) + i / 4, so output element r
+    // of lane (j, kq) is time tt + 4 kq + r.
+    const int arow = 4 * (j & 3) + (j >> 2);
+    float hist[NF > 0 ? NF : 1];
+    double acf[NF > 0 ? NF : 1];
+    double acf0 = 0.0;
+#pragma unroll
+    for (int k = 0; k < (NF > 0 ? NF : 1); ++k) {
+        hist[k] = 0.f;
+        acf[k] = 0.0;
+    }
+    for (int tt = 0; tt < Tpad; tt += 16) {
+        f64x4 d[4];
+#pragma unroll
+        for (int g = 0; g < 4; ++g) d[g] = f64x4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int rt = 0; rt < NT; ++rt)
+#pragma unroll
+            for (int kk = 0; kk < 4; ++kk) {
+                const double a = Xp[(size_t)(tt + arow) * PG + 16 * rt + kq + 4 * kk];
+#pragma unroll
+                for (int g = 0; g < 4; ++g) d[g] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, acc[g][rt][kk], d[g], 0, 0, 0);
+            }
+        const bool full = vec4 && tt + 16 <= T;
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+            float y4[4];
+            load_span<4>(rowB[g], T, tt + 4 * kq, full, y4);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const double y = vB[g] >= 0 ? (double)y4[r] : 0.0;
+                const float rr = (float)((y - d[g][r]) * inv[g]);
+                const int t = tt + 4 * kq + r;
+                if (Yt) Yt[(size_t)t * ldYt + col0 + 16 * g + j] = rr;
+                if (NF > 0) rbuf[(4 * kq + r) * RB + 16 * g + j] = rr;
+            }
+        }
+        if (NF > 0) {
+            __syncthreads();
+#pragma unroll
+            for (int i = 0; i < 16; ++i) {
+                const float x = rbuf[i * RB + lane];  // zero past T
+                acf0 += (double)x * x;
+#pragma unroll
+                for (int k = 0; k < NF; ++k) acf[k] += (double)x * hist[k];
+#pragma unroll
+                for (int k = NF - 1; k > 0; --k) hist[k] = hist[k - 1];
+                hist[0] = x;
+            }
+            __syncthreads();
+        }
+    }
+    if (NF > 0 && feats) {
+        const int v = prep_voxel(col0 + lane, ncols, rows, V);
+        if (v >= 0) {
+#pragma unroll
+            for (int k = 0; k < NF; ++k) feats[(size_t)v * NF + k] = acf0 > 0.0 ? (float)(acf[k] / acf0) : 0.f;
+        }
+    }
+}
+
+}  // namespace remlk
