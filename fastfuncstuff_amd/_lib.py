@@ -36,6 +36,7 @@ SIGNATURES: dict[str, list] = {
     "ffs_reml_gls": [_vp, _vp, _i64, _vp, _vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _vp, _vp, _vp, _vp],
     "ffs_reml_ljung_box": [_vp, _i64, _i64, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp],
     "ffs_reml_local_workspace": [_vp, _i64, _vp],
+    "ffs_reml_local_recenter": [_vp, _vp, _i64, _i64, _vp, _vp, _i64, _vp],
     "ffs_reml_gls_workspace": [_vp, _i64, _vp],
     "ffs_reml_local_prepare": [_vp, _vp, _i64, _i64, _vp, ctypes.c_float, _vp, _vp, _vp, _i64, _vp],
     "ffs_reml_local_energy": [_vp, _i64, _vp, _i64, _vp],

@@ -327,7 +327,7 @@ static void launch_prep(const ffs_reml_plan* P, const float* Y, int64_t ldY, int
 
 static int prep_impl(const ffs_reml_plan* P, const float* Y, int64_t ldY, int64_t V, const int32_t* rows,
                      int64_t ncols, float* Yt, int64_t ldYt, float* scale, int normalize, float* feats,
-                     hipStream_t s) {
+                     hipStream_t s, const int32_t* ncols_live = nullptr) {
     const int64_t ncp = vpad_of(ncols);
     if (Yt && ldYt < ncp) return fail(-3, "prep: ldYt must be >= the padded column count");
     if (ldY < P->d.T) return fail(-3, "prep: ldY must be >= T");
@@ -340,17 +340,17 @@ static int prep_impl(const ffs_reml_plan* P, const float* Y, int64_t ldY, int64_
         if (P->d.PG == 16) {
             if (feats)
                 hipLaunchKernelGGL((k_prep_mfma<1, 10>), grid, block, 0, s, P->d, Y, ldY, v, rows, nc, ncpi, normalize,
-                                   Yt, ldYt, scale, feats);
+                                   Yt, ldYt, scale, feats, ncols_live);
             else
                 hipLaunchKernelGGL((k_prep_mfma<1, 0>), grid, block, 0, s, P->d, Y, ldY, v, rows, nc, ncpi, normalize,
-                                   Yt, ldYt, scale, feats);
+                                   Yt, ldYt, scale, feats, ncols_live);
         } else {
             if (feats)
                 hipLaunchKernelGGL((k_prep_mfma<2, 10>), grid, block, 0, s, P->d, Y, ldY, v, rows, nc, ncpi, normalize,
-                                   Yt, ldYt, scale, feats);
+                                   Yt, ldYt, scale, feats, ncols_live);
             else
                 hipLaunchKernelGGL((k_prep_mfma<2, 0>), grid, block, 0, s, P->d, Y, ldY, v, rows, nc, ncpi, normalize,
-                                   Yt, ldYt, scale, feats);
+                                   Yt, ldYt, scale, feats, ncols_live);
         }
         HIP_TRY(hipGetLastError());
         return 0;
@@ -633,6 +633,21 @@ int local_check(const ffs_reml_plan* P, int64_t V, void* workspace, int64_t ws_b
 }
 }  // namespace
 
+namespace {
+static int local_group_and_prep(const ffs_reml_plan* P, const float* Y, int64_t ldY, int64_t V, float* scale,
+                                const LocalLayout& L, const LocalPtrs& q, hipStream_t s) {
+    // voxels grouped by window centre, each group padded to whole tiles
+    if (int rc = group_by_key(q.center, V, (int)L.ng, kLocalTile, q.cnt, q.fill, q.cursor, q.rows, L.max_cols, s))
+        return rc;
+    hipLaunchKernelGGL(k_local_tiles, dim3((unsigned)((L.max_tiles + 255) / 256)), dim3(256), 0, s, q.fill, (int)L.ng,
+                       (int)L.max_tiles, q.tg, q.tc, q.nt);
+    HIP_TRY(hipGetLastError());
+    // columns past the last group's run (off[ng]) are skipped
+    return prep_impl(P, Y, ldY, V, q.rows, L.max_cols, q.Ytg, L.max_cols, scale, 1, nullptr, s, q.fill + L.ng);
+}
+
+}  // namespace
+
 extern "C" int ffs_reml_local_prepare(const ffs_reml_plan* P, const float* Y, int64_t ldY, int64_t V,
                                       const float* props, float conf_min, int32_t* need_full, float* scale,
                                       void* workspace, int64_t ws_bytes, void* stream) {
@@ -643,13 +658,16 @@ extern "C" int ffs_reml_local_prepare(const ffs_reml_plan* P, const float* Y, in
     hipStream_t s = (hipStream_t)stream;
     const unsigned gv = (unsigned)((V + 255) / 256);
     hipLaunchKernelGGL(k_local_center, dim3(gv), dim3(256), 0, s, P->d, props, (int)V, conf_min, q.center, need_full);
-    // voxels grouped by window centre, each group padded to whole tiles
-    if (int rc = group_by_key(q.center, V, (int)L.ng, kLocalTile, q.cnt, q.fill, q.cursor, q.rows, L.max_cols, s))
-        return rc;
-    hipLaunchKernelGGL(k_local_tiles, dim3((unsigned)((L.max_tiles + 255) / 256)), dim3(256), 0, s, q.fill, (int)L.ng,
-                       (int)L.max_tiles, q.tg, q.tc, q.nt);
     HIP_TRY(hipGetLastError());
-    return prep_impl(P, Y, ldY, V, q.rows, L.max_cols, q.Ytg, L.max_cols, scale, 1, nullptr, s);
+    return local_group_and_prep(P, Y, ldY, V, scale, L, q, s);
+}
+
+extern "C" int ffs_reml_local_recenter(const ffs_reml_plan* P, const float* Y, int64_t ldY, int64_t V, float* scale,
+                                       void* workspace, int64_t ws_bytes, void* stream) {
+    if (V <= 0) return 0;
+    LocalLayout L;
+    if (int rc = local_check(P, V, workspace, ws_bytes, &L)) return rc;
+    return local_group_and_prep(P, Y, ldY, V, scale, L, local_ptrs(L, workspace), (hipStream_t)stream);
 }
 
 extern "C" int ffs_reml_local_energy(const ffs_reml_plan* P, int64_t V, void* workspace, int64_t ws_bytes,
@@ -671,7 +689,7 @@ extern "C" int ffs_reml_local_project(const ffs_reml_plan* P, int64_t V, int32_t
     if (int rc = local_check(P, V, workspace, ws_bytes, &L)) return rc;
     LocalPtrs q = local_ptrs(L, workspace);
     hipLaunchKernelGGL(k_local_project, dim3((unsigned)L.max_tiles), dim3(256), 0, (hipStream_t)stream, P->d, q.Ytg,
-                       (int64_t)L.max_cols, q.tg, q.tc, q.nt, q.rows, q.Eloc, best, best_score, need_full);
+                       (int64_t)L.max_cols, q.tg, q.tc, q.nt, q.rows, q.Eloc, best, best_score, need_full, q.center);
     HIP_TRY(hipGetLastError());
     return 0;
 }

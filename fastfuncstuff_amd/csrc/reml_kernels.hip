@@ -1132,20 +1132,28 @@ __device__ inline int nearest_index(const float* axis, int n, float x) {
 }
 
 // Window centre per voxel from the proposal (a_hat, b_hat, c); the window is
-// kept inside the lattice.  flags[v] = 1 when the confidence c < conf_min or
-// the proposal is not finite (such voxels take the full grid).
+// kept inside the lattice.  Voxels whose confidence c < conf_min or whose
+// proposal is not finite get flags[v] = 2 and no window (centre -1): they take
+// the full grid.  The others get flags[v] = 0.
+__device__ inline int clamp_center(const PlanDev& P, int ai, int bi) {
+    if (P.n_a >= kWin) ai = min(max(ai, kWinHalf), P.n_a - 1 - kWinHalf);
+    if (P.n_b >= kWin) bi = min(max(bi, kWinHalf), P.n_b - 1 - kWinHalf);
+    return ai * P.n_b + bi;
+}
+
 __global__ void k_local_center(PlanDev P, const float* __restrict__ props, int V, float conf_min,
                                int32_t* __restrict__ center, int32_t* __restrict__ flags) {
     const int v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= V) return;
     const float ah = props[3 * v], bh = props[3 * v + 1], c = props[3 * v + 2];
     const bool finite = isfinite(ah) && isfinite(bh) && isfinite(c);
-    int ai = finite ? nearest_index(P.a_axis, P.n_a, ah) : P.zero_a;
-    int bi = finite ? nearest_index(P.b_axis, P.n_b, bh) : P.zero_b;
-    if (P.n_a >= kWin) ai = min(max(ai, kWinHalf), P.n_a - 1 - kWinHalf);
-    if (P.n_b >= kWin) bi = min(max(bi, kWinHalf), P.n_b - 1 - kWinHalf);
-    center[v] = ai * P.n_b + bi;
-    flags[v] = (!finite || !(c >= conf_min)) ? 1 : 0;
+    if (!finite || !(c >= conf_min)) {
+        center[v] = -1;
+        flags[v] = 2;
+        return;
+    }
+    center[v] = clamp_center(P, nearest_index(P.a_axis, P.n_a, ah), nearest_index(P.b_axis, P.n_b, bh));
+    flags[v] = 0;
 }
 
 __device__ inline int window_pair(const PlanDev& P, int ca, int cb, int slot) {
@@ -1264,7 +1272,10 @@ __device__ __forceinline__ void local_gload(f32x4 (&ra)[A_V4], f32x4 (&rb)[B_V4]
 
 // Window projection on MFMA (fp32 32x32x2, same k-order as k_project) + score +
 // local argmin + edge test.  Block: 4 waves x 32 voxel rows = one 128-column
-// tile; N = 26 pair slots x kLocalPP columns (slot 25 is padding).
+// tile; N = 28 pair slots x kLocalPP columns (slots 25..27 are padding).
+// Per voxel: best / best_score; need_full = 1 when the argmin lies on a window
+// side that the lattice continues past (else 0); center = the window around
+// that argmin for a further step (edge) or -1.
 __global__ void __launch_bounds__(256) k_local_project(PlanDev P, const float* __restrict__ Yt, int64_t ldY,
                                                        const int32_t* __restrict__ tile_group,
                                                        const int32_t* __restrict__ tile_col,
@@ -1272,7 +1283,7 @@ __global__ void __launch_bounds__(256) k_local_project(PlanDev P, const float* _
                                                        const int32_t* __restrict__ rows,
                                                        const float* __restrict__ Eloc, int32_t* __restrict__ best,
                                                        float* __restrict__ best_score,
-                                                       int32_t* __restrict__ need_full) {
+                                                       int32_t* __restrict__ need_full, int32_t* __restrict__ center) {
     constexpr int BM = kLocalTile, BK = 16, NSLOT = kSlots + 3, BN = NSLOT * kLocalPP, NBLK = BN / 32;
     constexpr int A_ELEMS = BK * BM, B_ELEMS = BK * BN;
     __shared__ float lds[A_ELEMS + B_ELEMS];
@@ -1365,6 +1376,7 @@ __global__ void __launch_bounds__(256) k_local_project(PlanDev P, const float* _
             }
             int edge = 1;
             int k = -1;
+            int next = -1;
             if (bslot >= 0) {
                 k = kslot[bslot];
                 // lambda == 0 pairs score bitwise alike: the global first-minimum
@@ -1374,10 +1386,12 @@ __global__ void __launch_bounds__(256) k_local_project(PlanDev P, const float* _
                 const int ai = ca - kWinHalf + r, bi = cb - kWinHalf + c;
                 edge = (r == 0 && ai > 0) || (r == kWin - 1 && ai < P.n_a - 1) || (c == 0 && bi > 0) ||
                        (c == kWin - 1 && bi < P.n_b - 1);
+                if (edge) next = clamp_center(P, ai, bi);
             }
             best[v] = k;
             best_score[v] = bs;
-            if (edge) need_full[v] = 1;
+            need_full[v] = edge;
+            center[v] = next;
         }
     }
 }

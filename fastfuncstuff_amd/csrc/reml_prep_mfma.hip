@@ -28,13 +28,15 @@ template <int NT, int NF>
 __global__ void __launch_bounds__(64) k_prep_mfma(PlanDev P, const float* __restrict__ Y, int64_t ldY, int V,
                                                   const int32_t* __restrict__ rows, int ncols, int ncols_pad,
                                                   int normalize, float* __restrict__ Yt, int64_t ldYt,
-                                                  float* __restrict__ scale, float* __restrict__ feats) {
+                                                  float* __restrict__ scale, float* __restrict__ feats,
+                                                  const int32_t* __restrict__ ncols_live) {
     constexpr int PG = 16 * NT;
     constexpr int RB = 65;  // feature tile pitch (floats)
     __shared__ float rbuf[NF > 0 ? 16 * RB : 1];
     const int lane = threadIdx.x;
     const int col0 = blockIdx.x * 64;
     if (col0 >= ncols_pad) return;
+    if (ncols_live && col0 >= *ncols_live) return;  // beyond the columns in use (device-side count)
     const int T = P.T, Tpad = P.Tpad;
     const int j = lane & 15, kq = lane >> 4;
     const bool vec4 = (ldY & 3) == 0 && (reinterpret_cast<uintptr_t>(Y) & 15) == 0;

@@ -187,10 +187,14 @@ class LocalSearcher:
         self.idx = torch.empty(V, dtype=torch.int32, device=dev)
         self.count = torch.zeros(1, dtype=torch.int32, device=dev)
 
-    STAGES = ("features", "mlp", "local_prepare", "local_energy", "local_project", "fallback")
+    STAGES = ("features", "mlp", "local_prepare", "local_energy", "local_project", "local_walk", "fallback")
 
-    def run(self, Y: torch.Tensor, marks: list | None = None) -> LocalResult:
-        """Learned-proposal search of Y (V, T) float32 on the device.  ``marks``
+    def run(self, Y: torch.Tensor, marks: list | None = None, walk: int = 1) -> LocalResult:
+        """Learned-proposal search of Y (V, T) float32 on the device.
+
+        Voxels whose window optimum lies on a window edge take up to ``walk``
+        further 5x5 windows centred on that optimum; those still on an edge,
+        and those with low confidence, take the full grid.  ``marks``
         (optional) = len(STAGES)+1 torch.cuda.Events recorded between stages."""
         grid, lib, dev = self.grid, _lib.load(), self.grid.device
         V = Y.shape[0]
@@ -218,6 +222,13 @@ class LocalSearcher:
         _lib.check(lib.ffs_reml_local_project(grid.handle, V, _lib.ptr(self.best), _lib.ptr(self.score),
                                               _lib.ptr(self.need), ws, nb, st), "local_project")
         mark(5)
+        for _ in range(walk):
+            _lib.check(lib.ffs_reml_local_recenter(grid.handle, _lib.ptr(Y), Y.stride(0), V, _lib.ptr(self.scale), ws,
+                                                   nb, st), "local_recenter")
+            _lib.check(lib.ffs_reml_local_energy(grid.handle, V, ws, nb, st), "local_energy")
+            _lib.check(lib.ffs_reml_local_project(grid.handle, V, _lib.ptr(self.best), _lib.ptr(self.score),
+                                                  _lib.ptr(self.need), ws, nb, st), "local_project")
+        mark(6)
         _lib.check(lib.ffs_reml_compact(_lib.ptr(self.need), V, _lib.ptr(self.idx), _lib.ptr(self.count), st),
                    "compact")
         n_fb = int(self.count.item())
@@ -230,16 +241,16 @@ class LocalSearcher:
             del S
             _lib.check(lib.ffs_reml_scatter_best(_lib.ptr(rows), n_fb, _lib.ptr(b_fb), _lib.ptr(s_fb),
                                                  _lib.ptr(self.best), _lib.ptr(self.score), st), "scatter_best")
-        mark(6)
+        mark(7)
         return LocalResult(self.best, self.score, self.scale, n_fb, props)
 
 
 def calibrate_conf_min(mlp: ProposalMLP, grid: RemlGrid, Y_val: torch.Tensor, best_val: torch.Tensor,
-                       target: float = 0.999) -> float:
+                       target: float = 0.999, walk: int = 1) -> float:
     """Smallest-fallback confidence threshold whose local-search agreement with
     the global argmin on a validation set is >= target.  Sets mlp.conf_min."""
     mlp.conf_min = -math.inf
-    res = LocalSearcher(grid, mlp, Y_val.shape[0]).run(Y_val)
+    res = LocalSearcher(grid, mlp, Y_val.shape[0]).run(Y_val, walk=walk)
     c = res.props[:, 2].detach()
     ok = (res.best == best_val)
     # voxels that went to the full grid are already right; find the threshold

@@ -135,10 +135,11 @@ int ffs_reml_ljung_box(const double* resid_w, int64_t ldR, int64_t V, int32_t n,
  * (a_hat, b_hat, log-confidence) from the proposal network.  Each voxel is
  * scored on the 5 x 5 lattice window around its snapped proposal (kept inside
  * the lattice) with the same arithmetic as the full grid, so the scores equal
- * the full-grid surface entries bitwise.  Outputs per voxel: best pair (-1 if
- * no valid pair), its score, need_full = 1 when the optimum is on a window
- * edge that is not a lattice edge, when the confidence is below conf_min, or
- * when the proposal is not finite; scale = per-voxel std.  Requires p <= 16
+ * the full-grid surface entries bitwise (fp32 projection mode).  Outputs per
+ * voxel: best pair (-1 if no valid pair), its score, need_full = 1 when the
+ * optimum is on a window edge that is not a lattice edge, 2 when the confidence
+ * is below conf_min or the proposal is not finite (no window is searched; best
+ * and score are left as they were); scale = per-voxel std.  Requires p <= 16
  * (returns -4 otherwise).  workspace: ffs_reml_local_workspace bytes, first use
  * zero-filled.
  */
@@ -152,6 +153,14 @@ int ffs_reml_local_prepare(const ffs_reml_plan* plan, const float* Y, int64_t ld
 int ffs_reml_local_energy(const ffs_reml_plan* plan, int64_t V, void* workspace, int64_t ws_bytes, void* stream);
 int ffs_reml_local_project(const ffs_reml_plan* plan, int64_t V, int32_t* best, float* best_score,
                            int32_t* need_full, void* workspace, int64_t ws_bytes, void* stream);
+/* One more window step for the voxels whose last window had its optimum on an
+ * edge (need_full = 1): each is re-grouped and residualised around a window
+ * centred on that optimum (kept inside the lattice); follow with
+ * ffs_reml_local_energy and ffs_reml_local_project, which overwrite best,
+ * best_score and need_full of those voxels only.  Same workspace and stream
+ * as the preceding ffs_reml_local_project. */
+int ffs_reml_local_recenter(const ffs_reml_plan* plan, const float* Y, int64_t ldY, int64_t V, float* scale,
+                            void* workspace, int64_t ws_bytes, void* stream);
 int ffs_reml_local_search(const ffs_reml_plan* plan, const float* Y, int64_t ldY, int64_t V, const float* props,
                           float conf_min, int32_t* best, float* best_score, int32_t* need_full, float* scale,
                           void* workspace, int64_t ws_bytes, void* stream);
