@@ -40,6 +40,10 @@ struct ffs_reml_plan {
     const double* ldR = nullptr;
     const double* ldX = nullptr;
     bool local_ok = false;
+    // second stream + events of ffs_reml_surface (created on first use)
+    static constexpr int kMaxSurfaceChunks = 16;
+    mutable hipStream_t aux = nullptr;
+    mutable hipEvent_t ev[kMaxSurfaceChunks + 1] = {};
     template <class Tp>
     int alloc(Tp** out, size_t n) {
         void* p = nullptr;
@@ -51,6 +55,9 @@ struct ffs_reml_plan {
         return 0;
     }
     ~ffs_reml_plan() {
+        for (hipEvent_t e : ev)
+            if (e) (void)hipEventDestroy(e);
+        if (aux) (void)hipStreamDestroy(aux);
         for (void* p : allocs) (void)hipFree(p);
     }
 };
@@ -227,13 +234,19 @@ extern "C" int ffs_reml_plan_create(ffs_reml_plan** out, int32_t T, int32_t p, c
         HIP_TRY(hipMemsetAsync(G64p, 0, sizeof(double) * K * d.Tpad * PG, s));
         HIP_TRY(hipMemsetAsync(coefR, 0, sizeof(double) * K * d.Tpad * 4, s));
     }
+    uint16_t *Lhf = nullptr, *Llf = nullptr, *Lhb = nullptr, *Llb = nullptr;
     if (P->local_ok) {
+        const size_t nl = (size_t)K * d.Tpad * kLocalPP;
+        if ((rc = P->alloc(&Lhf, nl)) || (rc = P->alloc(&Llf, nl)) || (rc = P->alloc(&Lhb, nl)) ||
+            (rc = P->alloc(&Llb, nl)))
+            return rc;
+        for (uint16_t* q : {Lhf, Llf, Lhb, Llb}) HIP_TRY(hipMemsetAsync(q, 0, 2 * nl, s));
         if ((rc = P->alloc(&Gp, (size_t)d.Tpad * K * kLocalPP))) return rc;
         HIP_TRY(hipMemsetAsync(Gp, 0, sizeof(float) * d.Tpad * K * kLocalPP, s));
     }
     hipLaunchKernelGGL(k_tables, dim3(K), dim3(64), 0, s, T, d.Tpad, p, K, dab, dwhiteflag, dgap, dX, d.n_task, dtask,
                        full_f, BN, npt, soa, coef64, Xw64, G64, Lx, Ainv, Fm, work, ld, ldR, ldX, G32, Gp, Ghf, Glf, Ghb, Glb, g16s,
-                       d.PG, G64p, coefR, dstatus);
+                       d.PG, G64p, coefR, Lhf, Llf, Lhb, Llb, dstatus);
     HIP_TRY(hipGetLastError());
     hipLaunchKernelGGL(k_local_tables, dim3(K), dim3(128), 0, s, T, K, soa, lp, lc, ls0);
     HIP_TRY(hipGetLastError());
@@ -277,6 +290,10 @@ extern "C" int ffs_reml_plan_create(ffs_reml_plan** out, int32_t T, int32_t p, c
     d.Gh_bf16 = Ghb;
     d.Gl_bf16 = Glb;
     d.g16_unscale = g16s;
+    d.Lh_f16 = Lhf;
+    d.Ll_f16 = Llf;
+    d.Lh_bf16 = Lhb;
+    d.Ll_bf16 = Llb;
     d.G64p = G64p;
     d.coefR = coefR;
     d.Xp = dXp;
@@ -426,6 +443,44 @@ extern "C" int ffs_reml_project(const ffs_reml_plan* P, const float* Yt, int64_t
             hipLaunchKernelGGL(k_project<96>, g, b, 0, s, P->d, Yt, ldYt, (int)V, S, ldS);
     }
     HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+// Energy and projection of voxel chunks pipelined over two streams: the
+// energy of chunk i+1 (VALU-bound, auxiliary stream) runs while the projection
+// of chunk i (MFMA-bound, caller's stream) does.  The caller's stream is
+// ordered after all of it on return.
+extern "C" int ffs_reml_surface(const ffs_reml_plan* P, const float* Yt, int64_t ldYt, int64_t V, float* S,
+                                int64_t ldS, int32_t n_chunks, void* stream) {
+    if (!P) return fail(-3, "surface: null plan");
+    if (V <= 0) return 0;
+    if (n_chunks < 1 || n_chunks > ffs_reml_plan::kMaxSurfaceChunks)
+        return fail(-3, "surface: n_chunks must be in [1, 16]");
+    const int64_t chunk = (V + n_chunks - 1) / n_chunks;
+    const int64_t step = (chunk + kVoxPad - 1) / kVoxPad * kVoxPad;  // column offsets stay tile-aligned
+    if (n_chunks == 1 || step >= V) {
+        if (int rc = ffs_reml_energy(P, Yt, ldYt, V, S, ldS, stream)) return rc;
+        return ffs_reml_project(P, Yt, ldYt, V, S, ldS, stream);
+    }
+    hipStream_t s = (hipStream_t)stream;
+    if (!P->aux) {
+        int dev = 0;
+        HIP_TRY(hipGetDevice(&dev));
+        if (dev != P->device) return fail(-3, "surface: the plan belongs to another device");
+        HIP_TRY(hipStreamCreateWithFlags(&P->aux, hipStreamNonBlocking));
+        for (hipEvent_t& e : P->ev) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    }
+    // the auxiliary stream starts after the caller's preceding work (e.g. prep)
+    HIP_TRY(hipEventRecord(P->ev[ffs_reml_plan::kMaxSurfaceChunks], s));
+    HIP_TRY(hipStreamWaitEvent(P->aux, P->ev[ffs_reml_plan::kMaxSurfaceChunks], 0));
+    int i = 0;
+    for (int64_t v0 = 0; v0 < V; v0 += step, ++i) {
+        const int64_t n = std::min(step, V - v0);
+        if (int rc = ffs_reml_energy(P, Yt + v0, ldYt, n, S + v0 * ldS, ldS, P->aux)) return rc;
+        HIP_TRY(hipEventRecord(P->ev[i], P->aux));
+        HIP_TRY(hipStreamWaitEvent(s, P->ev[i], 0));
+        if (int rc = ffs_reml_project(P, Yt + v0, ldYt, n, S + v0 * ldS, ldS, stream)) return rc;
+    }
     return 0;
 }
 
@@ -688,8 +743,17 @@ extern "C" int ffs_reml_local_project(const ffs_reml_plan* P, int64_t V, int32_t
     LocalLayout L;
     if (int rc = local_check(P, V, workspace, ws_bytes, &L)) return rc;
     LocalPtrs q = local_ptrs(L, workspace);
-    hipLaunchKernelGGL(k_local_project, dim3((unsigned)L.max_tiles), dim3(256), 0, (hipStream_t)stream, P->d, q.Ytg,
-                       (int64_t)L.max_cols, q.tg, q.tc, q.nt, q.rows, q.Eloc, best, best_score, need_full, q.center);
+    // same arithmetic as the plan's full-grid projection mode (bitwise-equal scores)
+    const dim3 g((unsigned)L.max_tiles), b(256);
+    if (P->d.proj_mode == 1)
+        hipLaunchKernelGGL(k_local_project16<true>, g, b, 0, (hipStream_t)stream, P->d, q.Ytg, (int64_t)L.max_cols,
+                           q.tg, q.tc, q.nt, q.rows, q.Eloc, best, best_score, need_full, q.center);
+    else if (P->d.proj_mode == 2)
+        hipLaunchKernelGGL(k_local_project16<false>, g, b, 0, (hipStream_t)stream, P->d, q.Ytg, (int64_t)L.max_cols,
+                           q.tg, q.tc, q.nt, q.rows, q.Eloc, best, best_score, need_full, q.center);
+    else
+        hipLaunchKernelGGL(k_local_project, g, b, 0, (hipStream_t)stream, P->d, q.Ytg, (int64_t)L.max_cols, q.tg,
+                           q.tc, q.nt, q.rows, q.Eloc, best, best_score, need_full, q.center);
     HIP_TRY(hipGetLastError());
     return 0;
 }

@@ -80,6 +80,12 @@ struct PlanDev {
     const float* lc;           // [K][T] c_t
     const float* ls0;          // [K] 1/v_0
     const float* Gp;           // [Tpad][K * kLocalPP] pair-padded projection operand (p <= kLocalPP)
+    // pair-padded split 16-bit operands of the window search, same halves as
+    // Gh_* / Gl_*: [K][Tpad/8][kLocalPP][8]
+    const uint16_t* Lh_f16;
+    const uint16_t* Ll_f16;
+    const uint16_t* Lh_bf16;
+    const uint16_t* Ll_bf16;
     const float2* zero2;       // [T] zeros
     const float* zero1;        // [T] zeros
 };

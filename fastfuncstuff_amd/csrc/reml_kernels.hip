@@ -41,6 +41,8 @@ __global__ void __launch_bounds__(64) k_tables(int T, int Tpad, int p, int K, co
                                                uint16_t* __restrict__ Glf, uint16_t* __restrict__ Ghb,
                                                uint16_t* __restrict__ Glb, float* __restrict__ g16_unscale,
                                                int PG, double* __restrict__ G64p, double* __restrict__ coefR,
+                                               uint16_t* __restrict__ Lhf, uint16_t* __restrict__ Llf,
+                                               uint16_t* __restrict__ Lhb, uint16_t* __restrict__ Llb,
                                                int* __restrict__ status) {
     const int k = blockIdx.x;
     if (k >= K) return;
@@ -175,6 +177,13 @@ __global__ void __launch_bounds__(64) k_tables(int T, int Tpad, int p, int K, co
             Glf[o] = __builtin_bit_cast(uint16_t, lf);
             Ghb[o] = __builtin_bit_cast(uint16_t, hb);
             Glb[o] = __builtin_bit_cast(uint16_t, lb);
+            if (Lhf && p <= kLocalPP) {
+                const size_t ol = (((size_t)k * (Tpad / 8) + (t >> 3)) * kLocalPP + j) * 8 + (t & 7);
+                Lhf[ol] = Ghf[o];
+                Llf[ol] = Glf[o];
+                Lhb[ol] = Ghb[o];
+                Llb[ol] = Glb[o];
+            }
         }
     }
     // f64 MFMA GLS operands: G padded to PG columns, recurrence coefficients
@@ -1270,6 +1279,84 @@ __device__ __forceinline__ void local_gload(f32x4 (&ra)[A_V4], f32x4 (&rb)[B_V4]
     }
 }
 
+// Window epilogue shared by the fp32 and split 16-bit kernels: one 32x32
+// accumulator block at a time through a wave-private LDS patch, lane (row,
+// half) sums its slot's p columns in order (as k_project / k_project16), then
+// the local argmin, the edge test and the next window centre.
+template <int NBLK>
+__device__ __forceinline__ void local_finish(const PlanDev& P, f32x16 (&acc)[NBLK], float* lds,
+                                             float (*scores)[kSlots + 1], const int* kslot, int col0, int ca, int cb,
+                                             int tid, const int32_t* __restrict__ rows,
+                                             const float* __restrict__ Eloc, int32_t* __restrict__ best,
+                                             float* __restrict__ best_score, int32_t* __restrict__ need_full,
+                                             int32_t* __restrict__ center, bool unscale) {
+    constexpr int BM = kLocalTile;
+    const int lane = tid & 63, w = tid >> 6;
+    float* Cw = lds + w * 32 * 33;
+    const int p = P.p;
+    const double tp = (double)(P.T - p);
+#pragma unroll
+    for (int n = 0; n < NBLK; ++n) {
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const int row = 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+            Cw[row * 33 + (lane & 31)] = acc[n][r];
+        }
+        __syncthreads();
+        const int row = lane & 31, half = lane >> 5;
+        const int slot = 2 * n + half;
+        if (slot < kSlots) {
+            const int k = kslot[slot];
+            float sc = INFINITY;
+            const int col = col0 + w * 32 + row;
+            if (k >= 0) {
+                double pp = 0.0;
+                for (int j = 0; j < p; ++j) {
+                    const double cv = (double)Cw[row * 33 + half * kLocalPP + j];
+                    pp = fma(cv, cv, pp);
+                }
+                if (unscale) pp *= (double)P.g16_unscale[k];
+                const double rss = (double)Eloc[(size_t)col * kSlots + slot] - pp;
+                sc = (float)fma(tp, log(rss + 1e-10), P.ld[k]);
+            }
+            scores[w * 32 + row][slot] = sc;
+        }
+        __syncthreads();
+    }
+    if (tid < BM) {
+        const int v = rows[col0 + tid];
+        if (v >= 0) {
+            float bs = INFINITY;
+            int bslot = -1;
+            for (int s2 = 0; s2 < kSlots; ++s2) {
+                const float sc = scores[tid][s2];
+                if (sc < bs) {
+                    bs = sc;
+                    bslot = s2;
+                }
+            }
+            int edge = 1;
+            int k = -1;
+            int next = -1;
+            if (bslot >= 0) {
+                k = kslot[bslot];
+                // lambda == 0 pairs score bitwise alike: the global first-minimum
+                // rule picks the first of them.
+                if (P.white_flag[k]) k = P.white[0];
+                const int r = bslot / kWin, c = bslot % kWin;
+                const int ai = ca - kWinHalf + r, bi = cb - kWinHalf + c;
+                edge = (r == 0 && ai > 0) || (r == kWin - 1 && ai < P.n_a - 1) || (c == 0 && bi > 0) ||
+                       (c == kWin - 1 && bi < P.n_b - 1);
+                if (edge) next = clamp_center(P, ai, bi);
+            }
+            best[v] = k;
+            best_score[v] = bs;
+            need_full[v] = edge;
+            center[v] = next;
+        }
+    }
+}
+
 // Window projection on MFMA (fp32 32x32x2, same k-order as k_project) + score +
 // local argmin + edge test.  Block: 4 waves x 32 voxel rows = one 128-column
 // tile; N = 28 pair slots x kLocalPP columns (slots 25..27 are padding).
@@ -1330,70 +1417,109 @@ __global__ void __launch_bounds__(256) k_local_project(PlanDev P, const float* _
         }
         __syncthreads();
     }
-    // Epilogue, one 32x32 accumulator block at a time through a wave-private
-    // LDS patch: lane (row, half) sums its slot's p columns in order (as k_project).
-    float* Cw = lds + w * 32 * 33;
-    const int p = P.p;
-    const double tp = (double)(P.T - p);
+    local_finish<NBLK>(P, acc, lds, scores, kslot, col0, ca, cb, tid, rows, Eloc, best, best_score, need_full, center,
+                       false);
+}
+
+// Window projection with split 16-bit operands (the plan's bf16x3 / fp16x3
+// mode): per 16-sample k-step acc += Al Bh, Ah Bl, Ah Bh on 32x32x16 MFMA in the
+// k order of k_project16, with the same halves (pair-padded copies of its
+// operand) and the same epilogue, so window scores equal that mode's full-grid
+// scores bitwise.  A (the grouped residual) is split on the way into LDS.
+template <bool BF16>
+__global__ void __launch_bounds__(256) k_local_project16(PlanDev P, const float* __restrict__ Yt, int64_t ldY,
+                                                         const int32_t* __restrict__ tile_group,
+                                                         const int32_t* __restrict__ tile_col,
+                                                         const int32_t* __restrict__ n_tiles,
+                                                         const int32_t* __restrict__ rows,
+                                                         const float* __restrict__ Eloc, int32_t* __restrict__ best,
+                                                         float* __restrict__ best_score,
+                                                         int32_t* __restrict__ need_full,
+                                                         int32_t* __restrict__ center) {
+    constexpr int BM = kLocalTile, NSLOT = kSlots + 3, BN = NSLOT * kLocalPP, NBLK = BN / 32;
+    constexpr int NTB = 2;               // 8-sample blocks per 16-sample stage
+    constexpr int RP = NTB * 8 + 8;      // row pitch in halves (48 B: 16-byte aligned rows)
+    constexpr int A_H = BM * RP, B_H = BN * RP;
+    constexpr int B_N = 2 * NTB * BN;    // 16-byte B units per stage (planes x blocks x columns)
+    constexpr int B_U = B_N / 256;
+    static_assert(B_N % 256 == 0 && BM * NTB == 256, "window tiles must be exact");
+    static_assert(2 * (A_H + B_H) * 2 >= 4 * 32 * 33 * 4, "epilogue patch must fit the staging area");
+    __shared__ __attribute__((aligned(16))) uint16_t stage[2 * A_H + 2 * B_H];
+    __shared__ float scores[BM][kSlots + 1];
+    __shared__ int kslot[NSLOT];
+    uint16_t* Ah = stage;
+    uint16_t* Al = Ah + A_H;
+    uint16_t* Bh = Al + A_H;
+    uint16_t* Bl = Bh + B_H;
+    const int tile = blockIdx.x;
+    if (tile >= *n_tiles) return;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int g = tile_group[tile];
+    const int ca = g / P.n_b, cb = g % P.n_b;
+    const int col0 = tile_col[tile];
+    if (tid < NSLOT) kslot[tid] = tid < kSlots ? window_pair(P, ca, cb, tid) : -1;
+    __syncthreads();
+    const uint16_t* __restrict__ Gh = BF16 ? P.Lh_bf16 : P.Lh_f16;
+    const uint16_t* __restrict__ Gl = BF16 ? P.Ll_bf16 : P.Ll_f16;
+    const int TB = P.Tpad / 8;
+    // A unit of this thread: voxel column vv, 8-sample block ta
+    const int vv = tid % BM, ta = tid / BM;
+    float ra[8];
+    u32x4 rb[B_U];
+    auto gload = [&](int kt) __attribute__((always_inline)) {
+        const float* src = Yt + (size_t)(kt * 16 + ta * 8) * ldY + col0 + vv;
 #pragma unroll
-    for (int n = 0; n < NBLK; ++n) {
+        for (int e = 0; e < 8; ++e) ra[e] = src[(size_t)e * ldY];
 #pragma unroll
-        for (int r = 0; r < 16; ++r) {
-            const int row = 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-            Cw[row * 33 + (lane & 31)] = acc[n][r];
+        for (int u = 0; u < B_U; ++u) {
+            const int q = tid + u * 256;
+            const int plane = q / (NTB * BN), rr = q % (NTB * BN);
+            const int tb = rr / BN, col = rr % BN;
+            const int k = kslot[col / kLocalPP];
+            const size_t o = (((size_t)(k >= 0 ? k : 0) * TB + kt * NTB + tb) * kLocalPP + col % kLocalPP) * 8;
+            const u32x4 x = *reinterpret_cast<const u32x4*>((plane ? Gl : Gh) + o);
+            rb[u] = k >= 0 ? x : u32x4{0u, 0u, 0u, 0u};
+        }
+    };
+    f32x16 acc[NBLK];
+#pragma unroll
+    for (int i = 0; i < NBLK; ++i)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) acc[i][r] = 0.f;
+    const int nk = P.Tpad / 16;
+    gload(0);
+    for (int kt = 0; kt < nk; ++kt) {
+        {
+            uint4 hi, lo;
+            split8<BF16>(ra, hi, lo);
+            *reinterpret_cast<uint4*>(Ah + vv * RP + ta * 8) = hi;
+            *reinterpret_cast<uint4*>(Al + vv * RP + ta * 8) = lo;
+        }
+#pragma unroll
+        for (int u = 0; u < B_U; ++u) {
+            const int q = tid + u * 256;
+            const int plane = q / (NTB * BN), rr = q % (NTB * BN);
+            const int tb = rr / BN, col = rr % BN;
+            *reinterpret_cast<u32x4*>((plane ? Bl : Bh) + col * RP + tb * 8) = rb[u];
         }
         __syncthreads();
-        const int row = lane & 31, half = lane >> 5;
-        const int slot = 2 * n + half;
-        if (slot < kSlots) {
-            const int k = kslot[slot];
-            float sc = INFINITY;
-            const int col = col0 + w * 32 + row;
-            if (k >= 0) {
-                double pp = 0.0;
-                for (int j = 0; j < p; ++j) {
-                    const double cv = (double)Cw[row * 33 + half * kLocalPP + j];
-                    pp = fma(cv, cv, pp);
-                }
-                const double rss = (double)Eloc[(size_t)col * kSlots + slot] - pp;
-                sc = (float)fma(tp, log(rss + 1e-10), P.ld[k]);
-            }
-            scores[w * 32 + row][slot] = sc;
+        if (kt + 1 < nk) gload(kt + 1);
+        const int row = w * 32 + (lane & 31), tb = lane >> 5;
+        const uint4 ah = *reinterpret_cast<const uint4*>(Ah + row * RP + tb * 8);
+        const uint4 al = *reinterpret_cast<const uint4*>(Al + row * RP + tb * 8);
+#pragma unroll
+        for (int n = 0; n < NBLK; ++n) {
+            const int col = n * 32 + (lane & 31);
+            const uint4 bh = *reinterpret_cast<const uint4*>(Bh + col * RP + tb * 8);
+            const uint4 bl = *reinterpret_cast<const uint4*>(Bl + col * RP + tb * 8);
+            acc[n] = mfma16<BF16>(al, bh, acc[n]);
+            acc[n] = mfma16<BF16>(ah, bl, acc[n]);
+            acc[n] = mfma16<BF16>(ah, bh, acc[n]);
         }
         __syncthreads();
     }
-    if (tid < BM) {
-        const int v = rows[col0 + tid];
-        if (v >= 0) {
-            float bs = INFINITY;
-            int bslot = -1;
-            for (int s2 = 0; s2 < kSlots; ++s2) {
-                const float sc = scores[tid][s2];
-                if (sc < bs) {
-                    bs = sc;
-                    bslot = s2;
-                }
-            }
-            int edge = 1;
-            int k = -1;
-            int next = -1;
-            if (bslot >= 0) {
-                k = kslot[bslot];
-                // lambda == 0 pairs score bitwise alike: the global first-minimum
-                // rule picks the first of them.
-                if (P.white_flag[k]) k = P.white[0];
-                const int r = bslot / kWin, c = bslot % kWin;
-                const int ai = ca - kWinHalf + r, bi = cb - kWinHalf + c;
-                edge = (r == 0 && ai > 0) || (r == kWin - 1 && ai < P.n_a - 1) || (c == 0 && bi > 0) ||
-                       (c == kWin - 1 && bi < P.n_b - 1);
-                if (edge) next = clamp_center(P, ai, bi);
-            }
-            best[v] = k;
-            best_score[v] = bs;
-            need_full[v] = edge;
-            center[v] = next;
-        }
-    }
+    local_finish<NBLK>(P, acc, reinterpret_cast<float*>(stage), scores, kslot, col0, ca, cb, tid, rows, Eloc, best,
+                       best_score, need_full, center, !BF16);
 }
 
 __global__ void k_fill_i32(int32_t* __restrict__ p, int64_t n, int32_t val) {

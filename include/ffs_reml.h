@@ -98,6 +98,13 @@ int ffs_reml_project(const ffs_reml_plan* plan, const float* Yt, int64_t ldYt, i
  * 2 = split fp16 (same, fp16 halves).  The local window search always uses fp32. */
 int ffs_reml_plan_set_projection(ffs_reml_plan* plan, int32_t mode);
 
+/* (i)+(ii) over n_chunks voxel chunks (1..16), the energy of one chunk
+ * overlapping the projection of the previous one on a second stream owned by
+ * the plan; same result as ffs_reml_energy then ffs_reml_project.  stream is
+ * ordered after all of it on return. */
+int ffs_reml_surface(const ffs_reml_plan* plan, const float* Yt, int64_t ldYt, int64_t V, float* S, int64_t ldS,
+                     int32_t n_chunks, void* stream);
+
 /* (iii) mode 0: global argmin (first minimum); mode 1: hierarchical descent. */
 int ffs_reml_select(const ffs_reml_plan* plan, const float* S, int64_t ldS, int64_t V, int32_t mode, int32_t* best,
                     float* best_score, void* stream);

@@ -3,8 +3,9 @@
 * MLP forward / gradient / Adam kernels vs the FP64 oracle (oracle/mlp_oracle.py):
   forward rtol 1e-5; loss rtol 1e-5; gradient |g - g64| <= 1e-3 |g64| + 1e-4 max|g64|;
   Adam rtol 1e-6.
-* Window scores are bitwise equal to the full-grid surface, and the window
-  argmin equals the argmin of the surface restricted to the window.
+* Window scores are bitwise equal to the full-grid surface in each projection
+  mode, and the window argmin equals the argmin of the surface restricted to
+  the window.
 * Training on the full-grid labels beats an untrained network: fewer pair
   evaluations per voxel and >= 99% agreement with the global argmin.
 """
@@ -76,10 +77,13 @@ def _grid_and_data(dev, V, T=120, seed=0, noise_seed=None):
     return grid, torch.from_numpy(wl.Y).to(dev), wl
 
 
-def test_window_scores_bitwise_equal_full_grid(dev):
+@pytest.mark.parametrize("mode", ["fp32", "fp16x3", "bf16x3"])
+def test_window_scores_bitwise_equal_full_grid(dev, mode):
     from fastfuncstuff_amd.learned import LocalSearcher, ProposalMLP
 
+    print(f"[learned] window bitwise mode={mode}", flush=True)
     grid, Y, _ = _grid_and_data(dev, 700, seed=11)
+    grid.set_projection(mode)
     V = Y.shape[0]
     Yt, _, _ = grid.prep(Y)
     S = grid.surface(Yt, V).cpu()
