@@ -35,6 +35,8 @@ METRIC = "REML voxels/s (T=400,V=200k): learned-proposal vs full grid; % argmin 
 PEAK_FP32_MFMA = 157.3e12
 PEAK_HBM = 8.0e12
 PEAK_F16_MFMA = 2.5e15
+# fp32 VALU without packed math: 256 CUs x 64 lanes/clk x 2.4 GHz
+PEAK_VALU_NONPACKED = 256 * 64 * 2.4e9
 
 
 def parse():
@@ -54,21 +56,28 @@ def parse():
     return ap.parse_args()
 
 
-def timed(fn, steps, warmup, stages):
-    """Run fn(marks) warmup + steps times; returns (ms/step wall, per-stage ms, last result)."""
+def timed(fn, steps, warmup, device):
+    """Run fn(timer) warmup + steps times (timer None during warmup).  Returns
+    (ms/step wall, per-stage ms per step, per-stage launches per step, last result)."""
+    from fastfuncstuff_amd.learned import StageTimer
+
     for _ in range(warmup):
         out = fn(None)
     torch.cuda.synchronize()
-    allm = []
+    timers = []
     t0 = time.perf_counter()
     for _ in range(steps):
-        marks = [torch.cuda.Event(enable_timing=True) for _ in range(len(stages) + 1)]
-        out = fn(marks)
-        allm.append(marks)
+        tm = StageTimer(device)
+        out = fn(tm)
+        tm.mark("end")
+        timers.append(tm)
     torch.cuda.synchronize()
     ms = (time.perf_counter() - t0) * 1e3 / steps
-    per = {n: float(np.mean([m[i].elapsed_time(m[i + 1]) for m in allm])) for i, n in enumerate(stages)}
-    return ms, per, out
+    per: dict = {}
+    for tm in timers:
+        for k, v in tm.durations().items():
+            per[k] = per.get(k, 0.0) + v / steps
+    return ms, per, timers[-1].counts(), out
 
 
 def main():
@@ -105,26 +114,23 @@ def main():
     S = torch.empty(V, K, dtype=torch.float32, device=dev)
 
     # ---- full-grid path -------------------------------------------------
-    fg_stages = ["prep", "energy", "project", "select", "gls"]
-
-    def full_step(marks):
-        def mark(i):
-            if marks is not None:
-                marks[i].record(stream)
-        mark(0)
+    def full_step(tm):
+        def mark(name):
+            if tm is not None:
+                tm.mark(name)
+        mark("prep")
         Yt, scale, _ = grid.prep(Y, normalize=True)
-        mark(1)
+        mark("energy")
         _lib.check(lib.ffs_reml_energy(grid.handle, _lib.ptr(Yt), Yt.stride(0), V, _lib.ptr(S), K, st), "energy")
-        mark(2)
+        mark("project")
         _lib.check(lib.ffs_reml_project(grid.handle, _lib.ptr(Yt), Yt.stride(0), V, _lib.ptr(S), K, st), "project")
-        mark(3)
+        mark("select")
         best, _ = grid.select(S, hierarchical=True)
-        mark(4)
+        mark("gls")
         grid.gls(Y, scale, best)
-        mark(5)
         return best
 
-    fg_ms, fg_k, best_hier = timed(full_step, args.steps, args.warmup, fg_stages)
+    fg_ms, fg_k, _, best_hier = timed(full_step, args.steps, args.warmup, dev)
     best_exh, _ = grid.select(S, hierarchical=False)
     print(f"[bench] full grid {fg_ms:.2f} ms/step", file=sys.stderr, flush=True)
 
@@ -146,40 +152,74 @@ def main():
 
     # ---- learned-proposal path ------------------------------------------
     ls = LocalSearcher(grid, mlp, V)
-    lp_stages = list(LocalSearcher.STAGES) + ["gls"]
 
-    def learned_step(marks):
-        res = ls.run(Y, marks=None if marks is None else marks[:len(LocalSearcher.STAGES) + 1])
+    def learned_step(tm):
+        res = ls.run(Y, timer=tm)
+        if tm is not None:
+            tm.mark("gls")
         grid.gls(Y, res.scale, res.best)
-        if marks is not None:
-            marks[-1].record(stream)
         return res
 
-    lp_ms, lp_k, res = timed(learned_step, args.steps, args.warmup, lp_stages)
+    lp_ms, lp_k, lp_n, res = timed(learned_step, args.steps, args.warmup, dev)
     best_l = res.best
     agree_exh = float((best_l == best_exh).double().mean())
     agree_hier = float((best_l == best_hier).double().mean())
+    # lambda == 0 pairs have identical likelihoods (white noise): compare with
+    # every such pair mapped to the first one
+    from fastfuncstuff_amd.glm.arma import _compute_arma11_lambda
+    lam0 = (_compute_arma11_lambda(grid.params[:, 0].float(), grid.params[:, 1].float()) == 0).to(dev)
+    canon = torch.arange(K, device=dev)
+    canon[lam0] = int(torch.nonzero(lam0)[0])
+
+    def _c(b):
+        return torch.where(b >= 0, canon[b.long().clamp(min=0)], b.long())
+    agree_hier_c = float((_c(best_l) == _c(best_hier)).double().mean())
+    agree_exh_hier_c = float((_c(best_exh) == _c(best_hier)).double().mean())
     value = V / (lp_ms / 1e3)
 
     # ---- roofline of the dominant kernel of the reported (learned) path --
+    # Per-step time of each kernel = sum of its stages; per-launch average =
+    # that / launches per step.  Algorithmic work per launch from its voxels.
     split = args.projection != "fp32"
-    win_flops = (3.0 if split else 1.0) * 2.0 * V * T * 25 * p
-    kern = {"local_project": ("mfma", win_flops,
-                              f"{'3*' if split else ''}2*V*T*25*p (25 window pairs x p columns)"),
-            "local_energy": ("hbm", 4.0 * grid.Tpad * V + 4.0 * V * 25, "4*Tpad*V + 4*25*V"),
-            "gls": ("hbm", 4.0 * V * T + 4.0 * V * (2 * p + 3), "4*V*T + 4*V*(2p+3)")}
-    dom = max(kern, key=lambda n: lp_k[n])
-    bound, amount, formula = kern[dom]
-    ach = amount / (lp_k[dom] / 1e3)
+    n_fb = res.n_fallback
+    wstats: dict = {}
+    ls.run(Y, stats=wstats)  # voxel counts of the window passes (same data, untimed)
+    n_win = [wstats["window"]] + list(wstats.get("walk", []))
+    mul = 3.0 if split else 1.0
+    kerns = {
+        # name: (stages, launches per step, bound, algorithmic amount per launch (avg), formula)
+        "k_local_project16" if split else "k_local_project":
+            (("local_project", "walk_project"), len(n_win), "mfma",
+             mul * 2.0 * T * 25 * p * sum(n_win) / len(n_win),
+             f"{'3*' if split else ''}2*T*25*p per window voxel, voxels per pass {n_win}"),
+        "k_energy": (("fallback_energy",), 1, "hbm", 4.0 * grid.Tpad * n_fb + 4.0 * n_fb * K,
+                     "4*Tpad*n_fb (residual read) + 4*K*n_fb (energy write)"),
+        "k_project16" if split else "k_project":
+            (("fallback_project",), 1, "mfma", mul * 2.0 * T * K * p * n_fb,
+             f"{'3*' if split else ''}2*T*K*p*n_fb"),
+        "k_gls_mfma": (("gls",), 1, "hbm", 4.0 * V * T + 4.0 * V * (2 * p + 3), "4*V*T + 4*V*(2p+3)"),
+    }
+    totals = {n: sum(lp_k.get(st_, 0.0) for st_ in v[0]) for n, v in kerns.items()}
+    dom = max(totals, key=totals.get)
+    stages, launches, bound, amount, formula = kerns[dom]
+    avg_ms = totals[dom] / max(1, launches)
+    ach = amount / (avg_ms / 1e3)
     if bound == "mfma":
         peak = PEAK_F16_MFMA if split else PEAK_FP32_MFMA
         roof = {"kernel": dom, "bound": "mfma", "achieved": ach / 1e12, "peak": peak / 1e12,
-                "unit": "TFLOP/s", "frac": ach / peak, "traffic": None,
+                "unit": "TFLOP/s", "frac": ach / peak, "traffic": None, "avg_launch_ms": avg_ms,
                 "algorithmic": f"{formula} = {amount:.4g} flop per launch"}
     else:
         roof = {"kernel": dom, "bound": "hbm", "achieved": ach / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
-                "frac": ach / PEAK_HBM, "traffic": None,
+                "frac": ach / PEAK_HBM, "traffic": None, "avg_launch_ms": avg_ms,
                 "algorithmic": f"{formula} = {amount:.4g} B per launch"}
+        if dom == "k_energy":
+            # the recurrence is VALU-bound: 3 fp32 ops per (voxel, pair, step) + the
+            # shared AR term; non-packed VALU issue limit of the chip
+            ops = 3.125 * T * K * n_fb
+            roof["valu"] = {"achieved_Tops": ops / (avg_ms / 1e3) / 1e12, "peak_Tops": PEAK_VALU_NONPACKED / 1e12,
+                            "frac": ops / (avg_ms / 1e3) / PEAK_VALU_NONPACKED}
+    roof["per_step_ms"] = {n: round(v, 4) for n, v in totals.items()}
     # fp32: 2*V*T*K*p flops on fp32 MFMA; split modes: three 16-bit products of that size.
     proj_flops = (3.0 if split else 1.0) * 2.0 * V * T * K * p
     proj_peak = PEAK_F16_MFMA if split else PEAK_FP32_MFMA
@@ -201,6 +241,8 @@ def main():
         "learned": {"kernels_ms": lp_k, "fallback_fraction": res.n_fallback / V, "conf_min": conf_min,
                     "argmin_match_vs_full_grid_exhaustive": agree_exh,
                     "argmin_match_vs_full_grid_hierarchical": agree_hier,
+                    "argmin_match_vs_full_grid_hierarchical_white_merged": agree_hier_c,
+                    "full_grid_exhaustive_vs_hierarchical_white_merged": agree_exh_hier_c,
                     "speedup_vs_full_grid": fg_ms / lp_ms, "train_s": t_train, "train_steps": log.steps,
                     "train_loss_first_last": [log.losses[0][1], log.losses[-1][1]] if log.losses else None},
         "plan_build_s": t_plan, "datagen_s": t_gen,

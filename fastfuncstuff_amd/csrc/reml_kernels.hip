@@ -467,33 +467,42 @@ __global__ void __launch_bounds__(kEnergyVox* kEnergyWaves) k_energy(PlanDev P, 
         acc[j] = 0.0;
     }
     float yc = y0;
-    constexpr int U = 8;
+    // One step t -> t+1 for every pair of the chunk (coefficients wave-uniform).
+    auto step = [&](float yn, float czt, const float* c) __attribute__((always_inline)) {
+        const float z = fmaf(-czt, yc, yn);
+#pragma unroll
+        for (int j = 0; j < kEnergyNB; ++j) {
+            e[j] = fmaf(c[2 * j], e[j], z);
+            part[j] = fmaf(e[j] * c[2 * j + 1], e[j], part[j]);
+        }
+        yc = yn;
+    };
+    constexpr int U = 8;  // steps per fp64 flush
+    constexpr int G = 2;  // steps whose coefficients are fetched together
+    const int nsteps = T - 1;
+    const int nfull = nsteps / U * U;
+    // samples of the next block, fetched one block ahead (indices clamped: the
+    // values past T are never used)
     float ybuf[U];
 #pragma unroll
-    for (int i = 0; i < U; ++i) ybuf[i] = (1 + i < T) ? y[(size_t)(1 + i) * ldY] : 0.f;
-    for (int t0 = 0; t0 < T - 1; t0 += U) {
+    for (int i = 0; i < U; ++i) ybuf[i] = Yt[(size_t)min(1 + i, T - 1) * ldY + v];
+    for (int t0 = 0; t0 < nfull; t0 += U) {
         float ycur[U];
 #pragma unroll
         for (int i = 0; i < U; ++i) ycur[i] = ybuf[i];
 #pragma unroll
-        for (int i = 0; i < U; ++i) {
-            const int tn = t0 + U + 1 + i;
-            ybuf[i] = (tn < T) ? y[(size_t)tn * ldY] : 0.f;
-        }
+        for (int i = 0; i < U; ++i) ybuf[i] = Yt[(size_t)min(t0 + U + 1 + i, T - 1) * ldY + v];
 #pragma unroll
-        for (int i = 0; i < U; ++i) {
-            const int t = t0 + i;
-            if (t < T - 1) {
-                const float yn = ycur[i];
-                const float z = fmaf(-cz[t], yc, yn);
-                const float* c = co + (size_t)t * 2 * kEnergyNB;
+        for (int g0 = 0; g0 < U; g0 += G) {
+            float cg[G][2 * kEnergyNB], zg[G];
 #pragma unroll
-                for (int j = 0; j < kEnergyNB; ++j) {
-                    e[j] = fmaf(c[2 * j], e[j], z);
-                    part[j] = fmaf(e[j] * c[2 * j + 1], e[j], part[j]);
-                }
-                yc = yn;
+            for (int i = 0; i < G; ++i) {
+                zg[i] = cz[t0 + g0 + i];
+#pragma unroll
+                for (int q = 0; q < 2 * kEnergyNB; ++q) cg[i][q] = co[(size_t)(t0 + g0 + i) * 2 * kEnergyNB + q];
             }
+#pragma unroll
+            for (int i = 0; i < G; ++i) step(ycur[g0 + i], zg[i], cg[i]);
         }
 #pragma unroll
         for (int j = 0; j < kEnergyNB; ++j) {
@@ -501,6 +510,8 @@ __global__ void __launch_bounds__(kEnergyVox* kEnergyWaves) k_energy(PlanDev P, 
             part[j] = 0.f;
         }
     }
+    // tail (fewer than U steps); the flush matches the blocked schedule
+    for (int t = nfull; t < nsteps; ++t) step(Yt[(size_t)(t + 1) * ldY + v], cz[t], co + (size_t)t * 2 * kEnergyNB);
     if (v < V) {
         const int32_t* slots = P.echunk + chunk * kEnergyNB;
 #pragma unroll

@@ -187,62 +187,100 @@ class LocalSearcher:
         self.idx = torch.empty(V, dtype=torch.int32, device=dev)
         self.count = torch.zeros(1, dtype=torch.int32, device=dev)
 
-    STAGES = ("features", "mlp", "local_prepare", "local_energy", "local_project", "local_walk", "fallback")
-
-    def run(self, Y: torch.Tensor, marks: list | None = None, walk: int = 1) -> LocalResult:
+    def run(self, Y: torch.Tensor, timer: "StageTimer | None" = None, walk: int = 1,
+            stats: dict | None = None) -> LocalResult:
         """Learned-proposal search of Y (V, T) float32 on the device.
 
         Voxels whose window optimum lies on a window edge take up to ``walk``
         further 5x5 windows centred on that optimum; those still on an edge,
-        and those with low confidence, take the full grid.  ``marks``
-        (optional) = len(STAGES)+1 torch.cuda.Events recorded between stages."""
+        and those with low confidence, take the full grid.  ``timer``
+        (optional) records a device event at the start of every stage;
+        ``stats`` (optional, synchronises) receives the voxel count of each
+        window pass ("window": first pass, "walk": list)."""
         grid, lib, dev = self.grid, _lib.load(), self.grid.device
         V = Y.shape[0]
         if V != self.V:
             raise ValueError(f"LocalSearcher was sized for {self.V} voxels, got {V}")
         st = _lib.stream_handle(dev)
-        stream = torch.cuda.current_stream(dev)
 
-        def mark(i):
-            if marks is not None:
-                marks[i].record(stream)
+        def mark(name):
+            if timer is not None:
+                timer.mark(name)
 
-        mark(0)
+        mark("features")
         _, _, feats = grid.prep_features(Y)
-        mark(1)
+        mark("mlp")
         props = self.mlp.forward(feats)
-        mark(2)
+        mark("local_prepare")
         ws, nb = _lib.ptr(self.ws), self.ws.numel()
         _lib.check(lib.ffs_reml_local_prepare(grid.handle, _lib.ptr(Y), Y.stride(0), V, _lib.ptr(props),
                                               float(self.mlp.conf_min), _lib.ptr(self.need), _lib.ptr(self.scale),
                                               ws, nb, st), "local_prepare")
-        mark(3)
+        if stats is not None:
+            stats["window"] = int((self.need != 2).sum())
+        mark("local_energy")
         _lib.check(lib.ffs_reml_local_energy(grid.handle, V, ws, nb, st), "local_energy")
-        mark(4)
+        mark("local_project")
         _lib.check(lib.ffs_reml_local_project(grid.handle, V, _lib.ptr(self.best), _lib.ptr(self.score),
                                               _lib.ptr(self.need), ws, nb, st), "local_project")
-        mark(5)
         for _ in range(walk):
+            if stats is not None:
+                stats.setdefault("walk", []).append(int((self.need == 1).sum()))
+            mark("walk_prepare")
             _lib.check(lib.ffs_reml_local_recenter(grid.handle, _lib.ptr(Y), Y.stride(0), V, _lib.ptr(self.scale), ws,
                                                    nb, st), "local_recenter")
+            mark("walk_energy")
             _lib.check(lib.ffs_reml_local_energy(grid.handle, V, ws, nb, st), "local_energy")
+            mark("walk_project")
             _lib.check(lib.ffs_reml_local_project(grid.handle, V, _lib.ptr(self.best), _lib.ptr(self.score),
                                                   _lib.ptr(self.need), ws, nb, st), "local_project")
-        mark(6)
+        mark("fallback_prep")
         _lib.check(lib.ffs_reml_compact(_lib.ptr(self.need), V, _lib.ptr(self.idx), _lib.ptr(self.count), st),
                    "compact")
         n_fb = int(self.count.item())
         if n_fb:
             rows = self.idx[:n_fb]
             Yt, _, _ = grid.prep(Y, rows=rows, scale=self.scale)
-            S = grid.surface(Yt, n_fb)
+            S = torch.empty(n_fb, grid.K, dtype=torch.float32, device=dev)
+            mark("fallback_energy")
+            _lib.check(lib.ffs_reml_energy(grid.handle, _lib.ptr(Yt), Yt.stride(0), n_fb, _lib.ptr(S), S.stride(0),
+                                           st), "energy")
+            mark("fallback_project")
+            _lib.check(lib.ffs_reml_project(grid.handle, _lib.ptr(Yt), Yt.stride(0), n_fb, _lib.ptr(S), S.stride(0),
+                                            st), "project")
             del Yt
+            mark("fallback_select")
             b_fb, s_fb = grid.select(S, hierarchical=False)
             del S
             _lib.check(lib.ffs_reml_scatter_best(_lib.ptr(rows), n_fb, _lib.ptr(b_fb), _lib.ptr(s_fb),
                                                  _lib.ptr(self.best), _lib.ptr(self.score), st), "scatter_best")
-        mark(7)
         return LocalResult(self.best, self.score, self.scale, n_fb, props)
+
+
+class StageTimer:
+    """Device events at stage starts on the current stream; ``durations()``
+    (after a synchronize) sums the time from each mark to the next by name."""
+
+    def __init__(self, device):
+        self.stream = torch.cuda.current_stream(device)
+        self.events: list = []
+
+    def mark(self, name: str):
+        ev = torch.cuda.Event(enable_timing=True)
+        ev.record(self.stream)
+        self.events.append((name, ev))
+
+    def durations(self) -> dict:
+        out: dict = {}
+        for (name, a), (_, b) in zip(self.events, self.events[1:]):
+            out[name] = out.get(name, 0.0) + a.elapsed_time(b)
+        return out
+
+    def counts(self) -> dict:
+        out: dict = {}
+        for name, _ in self.events[:-1]:
+            out[name] = out.get(name, 0) + 1
+        return out
 
 
 def calibrate_conf_min(mlp: ProposalMLP, grid: RemlGrid, Y_val: torch.Tensor, best_val: torch.Tensor,
