@@ -16,6 +16,7 @@
 #include "reml_kernels.hip"
 #include "reml_sort_gls.hip"
 #include "reml_prep_mfma.hip"
+#include "reml_proj16.hip"
 
 using namespace remlk;
 
@@ -44,6 +45,9 @@ struct ffs_reml_plan {
     static constexpr int kMaxSurfaceChunks = 16;
     mutable hipStream_t aux = nullptr;
     mutable hipEvent_t ev[kMaxSurfaceChunks + 1] = {};
+    // Yt buffer whose rows Tpad..2 Tpad-1 the last ffs_reml_prep filled with
+    // split halves (projection reads them only for this buffer)
+    mutable const float* split_yt = nullptr;
     template <class Tp>
     int alloc(Tp** out, size_t n) {
         void* p = nullptr;
@@ -319,6 +323,16 @@ extern "C" int ffs_reml_plan_logdets(const ffs_reml_plan* P, double* logdet_R, d
     return 0;
 }
 
+// Rows of the time-major buffer that ffs_reml_prep writes and ffs_reml_project
+// reads: Tpad, or 2 Tpad when the projection runs on pre-split halves.
+static int yt_rows(const ffs_reml_plan* P) { return (P->d.PG && P->d.proj_mode != 0) ? 2 * P->d.Tpad : P->d.Tpad; }
+
+extern "C" int ffs_reml_plan_yt_rows(const ffs_reml_plan* P, int32_t* rows) {
+    if (!P || !rows) return fail(-3, "plan_yt_rows: null argument");
+    *rows = yt_rows(P);
+    return 0;
+}
+
 extern "C" int ffs_reml_plan_dims(const ffs_reml_plan* P, int32_t* Tpad, int32_t* vox_pad, int32_t* K) {
     if (!P) return fail(-3, "plan_dims: null plan");
     if (Tpad) *Tpad = P->d.Tpad;
@@ -344,7 +358,7 @@ static void launch_prep(const ffs_reml_plan* P, const float* Y, int64_t ldY, int
 
 static int prep_impl(const ffs_reml_plan* P, const float* Y, int64_t ldY, int64_t V, const int32_t* rows,
                      int64_t ncols, float* Yt, int64_t ldYt, float* scale, int normalize, float* feats,
-                     hipStream_t s, const int32_t* ncols_live = nullptr) {
+                     hipStream_t s, const int32_t* ncols_live = nullptr, bool write_split = true) {
     const int64_t ncp = vpad_of(ncols);
     if (Yt && ldYt < ncp) return fail(-3, "prep: ldYt must be >= the padded column count");
     if (ldY < P->d.T) return fail(-3, "prep: ldY must be >= T");
@@ -352,22 +366,31 @@ static int prep_impl(const ffs_reml_plan* P, const float* Y, int64_t ldY, int64_
     const int p = P->d.p;
     const int v = (int)V, nc = (int)ncols, ncpi = (int)ncp;
     if (P->d.PG) {
-        // f64 MFMA path: also writes the zero rows T..Tpad-1
+        // f64 MFMA path: also writes the zero rows T..Tpad-1; in the split
+        // projection modes also the hi / lo halves into rows Tpad..2 Tpad-1
+        // of Yt (see split_rows_needed)
+        uint16_t *Ysh = nullptr, *Ysl = nullptr;
+        if (write_split && Yt && P->d.proj_mode != 0) {
+            Ysh = reinterpret_cast<uint16_t*>(Yt + (size_t)P->d.Tpad * ldYt);
+            Ysl = Ysh + (size_t)P->d.Tpad * ldYt;
+        }
+        if (write_split) P->split_yt = Ysh ? Yt : nullptr;
+        const int sbf = P->d.proj_mode == 1;
         const dim3 grid((unsigned)(ncp / 64)), block(64);
         if (P->d.PG == 16) {
             if (feats)
                 hipLaunchKernelGGL((k_prep_mfma<1, 10>), grid, block, 0, s, P->d, Y, ldY, v, rows, nc, ncpi, normalize,
-                                   Yt, ldYt, scale, feats, ncols_live);
+                                   Yt, ldYt, scale, feats, ncols_live, Ysh, Ysl, sbf);
             else
                 hipLaunchKernelGGL((k_prep_mfma<1, 0>), grid, block, 0, s, P->d, Y, ldY, v, rows, nc, ncpi, normalize,
-                                   Yt, ldYt, scale, feats, ncols_live);
+                                   Yt, ldYt, scale, feats, ncols_live, Ysh, Ysl, sbf);
         } else {
             if (feats)
                 hipLaunchKernelGGL((k_prep_mfma<2, 10>), grid, block, 0, s, P->d, Y, ldY, v, rows, nc, ncpi, normalize,
-                                   Yt, ldYt, scale, feats, ncols_live);
+                                   Yt, ldYt, scale, feats, ncols_live, Ysh, Ysl, sbf);
             else
                 hipLaunchKernelGGL((k_prep_mfma<2, 0>), grid, block, 0, s, P->d, Y, ldY, v, rows, nc, ncpi, normalize,
-                                   Yt, ldYt, scale, feats, ncols_live);
+                                   Yt, ldYt, scale, feats, ncols_live, Ysh, Ysl, sbf);
         }
         HIP_TRY(hipGetLastError());
         return 0;
@@ -423,19 +446,34 @@ extern "C" int ffs_reml_project(const ffs_reml_plan* P, const float* Yt, int64_t
     if (ldS < P->d.K) return fail(-3, "project: ldS must be >= K");
     hipStream_t s = (hipStream_t)stream;
     const int64_t nvt = (V + kProjBM - 1) / kProjBM;
-    const int64_t nb = nvt * P->d.n_tiles;
+    const int64_t grp8 = 8 * ((P->d.n_tiles + 7) / 8);  // see xcd_tile_map
+    const int64_t nb = nvt * grp8;
     if (nb > 0x7fffffff) return fail(-3, "project: grid too large; split the voxel batch");
     const dim3 g((unsigned)nb), b(256);
     const int mode = P->d.proj_mode;
+    // split modes: 256-voxel blocks
+    const int64_t nb16 = (V + kProj16BM - 1) / kProj16BM * grp8;
+    const dim3 g16((unsigned)nb16);
+    // split modes read the halves that prep wrote when p <= 32 (f64 MFMA
+    // prep); otherwise they split the residual in the kernel
+    const bool presplit = P->d.PG != 0 && Yt == P->split_yt;
     if (P->d.BN == 64) {
-        if (mode == 1)
+        if (mode == 1 && presplit)
+            hipLaunchKernelGGL((k_project16b<64, true>), g16, b, 0, s, P->d, Yt, ldYt, (int)V, S, ldS);
+        else if (mode == 2 && presplit)
+            hipLaunchKernelGGL((k_project16b<64, false>), g16, b, 0, s, P->d, Yt, ldYt, (int)V, S, ldS);
+        else if (mode == 1)
             hipLaunchKernelGGL((k_project16<64, true>), g, b, 0, s, P->d, Yt, ldYt, (int)V, S, ldS);
         else if (mode == 2)
             hipLaunchKernelGGL((k_project16<64, false>), g, b, 0, s, P->d, Yt, ldYt, (int)V, S, ldS);
         else
             hipLaunchKernelGGL(k_project<64>, g, b, 0, s, P->d, Yt, ldYt, (int)V, S, ldS);
     } else {
-        if (mode == 1)
+        if (mode == 1 && presplit)
+            hipLaunchKernelGGL((k_project16b<96, true>), g16, b, 0, s, P->d, Yt, ldYt, (int)V, S, ldS);
+        else if (mode == 2 && presplit)
+            hipLaunchKernelGGL((k_project16b<96, false>), g16, b, 0, s, P->d, Yt, ldYt, (int)V, S, ldS);
+        else if (mode == 1)
             hipLaunchKernelGGL((k_project16<96, true>), g, b, 0, s, P->d, Yt, ldYt, (int)V, S, ldS);
         else if (mode == 2)
             hipLaunchKernelGGL((k_project16<96, false>), g, b, 0, s, P->d, Yt, ldYt, (int)V, S, ldS);
@@ -698,7 +736,8 @@ static int local_group_and_prep(const ffs_reml_plan* P, const float* Y, int64_t 
                        (int)L.max_tiles, q.tg, q.tc, q.nt);
     HIP_TRY(hipGetLastError());
     // columns past the last group's run (off[ng]) are skipped
-    return prep_impl(P, Y, ldY, V, q.rows, L.max_cols, q.Ytg, L.max_cols, scale, 1, nullptr, s, q.fill + L.ng);
+    return prep_impl(P, Y, ldY, V, q.rows, L.max_cols, q.Ytg, L.max_cols, scale, 1, nullptr, s, q.fill + L.ng,
+                     false);
 }
 
 }  // namespace

@@ -21,6 +21,26 @@ constexpr int kProjBK = 32;
 constexpr int kVoxPad = 256;
 // Threads per block of the GLS kernel.
 constexpr int kGlsBlock = 64;
+// Block -> (voxel tile, column tile) map of the projection kernels.  Blocks are
+// dispatched round-robin over the 8 XCDs (block id % 8); each XCD takes a
+// contiguous group of ceil(n_tiles / 8) column tiles, whose operand stays in
+// that XCD's L2, and streams the voxel tiles through it (column tile fastest,
+// so the blocks in flight share their residual tile).  The grid holds
+// 8 * nvt * group blocks; returns false for blocks past the end.
+__device__ inline bool xcd_tile_map(int bid, int nb, int n_tiles, int* vt, int* tile) {
+    const int grp = (n_tiles + 7) >> 3;
+    const int nvt = nb / (8 * grp);
+    const int xcd = bid & 7, local = bid >> 3;
+    const int t0 = xcd * grp;
+    const int ng = min(grp, n_tiles - t0);
+    if (ng <= 0) return false;
+    const int v = local / ng;
+    if (v >= nvt) return false;
+    *vt = v;
+    *tile = t0 + local % ng;
+    return true;
+}
+
 // Largest design width handled by the f64 MFMA GLS kernel (two 16-row tiles).
 constexpr int kGlsMfmaMaxP = 32;
 

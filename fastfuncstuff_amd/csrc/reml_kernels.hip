@@ -578,18 +578,8 @@ __global__ void __launch_bounds__(256) k_project(PlanDev P, const float* __restr
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int w = tid >> 6;
-    // XCD-aware order: consecutive block ids land on different XCDs; give each
-    // XCD a contiguous run of voxel tiles so one tile's column tiles share L2.
-    const int nvt = gridDim.x / P.n_tiles;
-    const int nb = gridDim.x;
-    const int bid = blockIdx.x;
-    const int xcd = bid & 7, local = bid >> 3;
-    const int per_xcd = (nb + 7) >> 3;
-    int lin = xcd * per_xcd + local;
-    if ((nb & 7) != 0) lin = bid;  // fall back to the plain order on ragged grids
-    const int vt = lin / P.n_tiles;
-    const int tile = lin % P.n_tiles;
-    if (vt >= nvt) return;
+    int vt, tile;
+    if (!xcd_tile_map(blockIdx.x, gridDim.x, P.n_tiles, &vt, &tile)) return;
     const int v0 = vt * BM;
     const float* __restrict__ Ag = Yt + v0;
     const float* __restrict__ Bg = P.G32 + (size_t)tile * P.Tpad * BN;
@@ -666,23 +656,27 @@ __global__ void __launch_bounds__(256) k_project(PlanDev P, const float* __restr
 typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 
+// x = hi + lo in two 16-bit halves (round-to-nearest hi, remainder lo).
+template <bool BF16>
+__device__ __forceinline__ void split1(float x, uint16_t& h, uint16_t& l) {
+    if (BF16) {
+        const __bf16 hb = (__bf16)x;
+        const __bf16 lb = (__bf16)(x - (float)hb);
+        h = __builtin_bit_cast(uint16_t, hb);
+        l = __builtin_bit_cast(uint16_t, lb);
+    } else {
+        const _Float16 hf = (_Float16)x;
+        const _Float16 lf = (_Float16)(x - (float)hf);
+        h = __builtin_bit_cast(uint16_t, hf);
+        l = __builtin_bit_cast(uint16_t, lf);
+    }
+}
+
 template <bool BF16>
 __device__ __forceinline__ void split8(const float (&x)[8], uint4& hi, uint4& lo) {
     uint16_t h[8], l[8];
 #pragma unroll
-    for (int e = 0; e < 8; ++e) {
-        if (BF16) {
-            const __bf16 hb = (__bf16)x[e];
-            const __bf16 lb = (__bf16)(x[e] - (float)hb);
-            h[e] = __builtin_bit_cast(uint16_t, hb);
-            l[e] = __builtin_bit_cast(uint16_t, lb);
-        } else {
-            const _Float16 hf = (_Float16)x[e];
-            const _Float16 lf = (_Float16)(x[e] - (float)hf);
-            h[e] = __builtin_bit_cast(uint16_t, hf);
-            l[e] = __builtin_bit_cast(uint16_t, lf);
-        }
-    }
+    for (int e = 0; e < 8; ++e) split1<BF16>(x[e], h[e], l[e]);
     hi = make_uint4(h[0] | (uint32_t)h[1] << 16, h[2] | (uint32_t)h[3] << 16, h[4] | (uint32_t)h[5] << 16,
                     h[6] | (uint32_t)h[7] << 16);
     lo = make_uint4(l[0] | (uint32_t)l[1] << 16, l[2] | (uint32_t)l[3] << 16, l[4] | (uint32_t)l[5] << 16,
@@ -765,16 +759,8 @@ __global__ void __launch_bounds__(256) k_project16(PlanDev P, const float* __res
     uint16_t* Bl = Bh + B_H;
 
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    const int nvt = gridDim.x / P.n_tiles;
-    const int nb = gridDim.x;
-    const int bid = blockIdx.x;
-    const int xcd = bid & 7, local = bid >> 3;
-    const int per_xcd = (nb + 7) >> 3;
-    int lin = xcd * per_xcd + local;
-    if ((nb & 7) != 0) lin = bid;
-    const int vt = lin / P.n_tiles;
-    const int tile = lin % P.n_tiles;
-    if (vt >= nvt) return;
+    int vt, tile;
+    if (!xcd_tile_map(blockIdx.x, gridDim.x, P.n_tiles, &vt, &tile)) return;
     const int v0 = vt * BM;
     const uint16_t* __restrict__ gh = (BF16 ? P.Gh_bf16 : P.Gh_f16) + (size_t)tile * P.Tpad * BN;
     const uint16_t* __restrict__ gl = (BF16 ? P.Gl_bf16 : P.Gl_f16) + (size_t)tile * P.Tpad * BN;

@@ -29,7 +29,8 @@ __global__ void __launch_bounds__(64) k_prep_mfma(PlanDev P, const float* __rest
                                                   const int32_t* __restrict__ rows, int ncols, int ncols_pad,
                                                   int normalize, float* __restrict__ Yt, int64_t ldYt,
                                                   float* __restrict__ scale, float* __restrict__ feats,
-                                                  const int32_t* __restrict__ ncols_live) {
+                                                  const int32_t* __restrict__ ncols_live, uint16_t* __restrict__ Ysh,
+                                                  uint16_t* __restrict__ Ysl, int split_bf16) {
     constexpr int PG = 16 * NT;
     constexpr int RB = 65;  // feature tile pitch (floats)
     __shared__ float rbuf[NF > 0 ? 16 * RB : 1];
@@ -131,13 +132,30 @@ __global__ void __launch_bounds__(64) k_prep_mfma(PlanDev P, const float* __rest
         for (int g = 0; g < 4; ++g) {
             float y4[4];
             load_span<4>(rowB[g], T, tt + 4 * kq, full, y4);
+            float r4[4];
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const double y = vB[g] >= 0 ? (double)y4[r] : 0.0;
                 const float rr = (float)((y - d[g][r]) * inv[g]);
+                r4[r] = rr;
                 const int t = tt + 4 * kq + r;
                 if (Yt) Yt[(size_t)t * ldYt + col0 + 16 * g + j] = rr;
                 if (NF > 0) rbuf[(4 * kq + r) * RB + 16 * g + j] = rr;
+            }
+            if (Ysh) {
+                // split halves, [t/8][column][8] (this lane: one 8-byte run of 4 samples)
+                uint16_t h[4], l[4];
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    if (split_bf16)
+                        split1<true>(r4[r], h[r], l[r]);
+                    else
+                        split1<false>(r4[r], h[r], l[r]);
+                }
+                const int t4 = tt + 4 * kq;
+                const size_t o = ((size_t)(t4 >> 3) * ldYt + col0 + 16 * g + j) * 8 + (t4 & 7);
+                *reinterpret_cast<uint2*>(Ysh + o) = make_uint2(h[0] | (uint32_t)h[1] << 16, h[2] | (uint32_t)h[3] << 16);
+                *reinterpret_cast<uint2*>(Ysl + o) = make_uint2(l[0] | (uint32_t)l[1] << 16, l[2] | (uint32_t)l[3] << 16);
             }
         }
         if (NF > 0) {

@@ -234,6 +234,7 @@ class RemlGrid(Mapping):
         _lib.check(lib.ffs_reml_plan_logdets(handle, ldR.ctypes.data, ldX.ctypes.data), "plan_logdets")
         self.logdet_R, self.logdet_XRX = ldR, ldX
         self.param_list = [(float(a), float(b)) for a, b in self.params.tolist()]
+        self.projection = "fp32"
 
     # Mapping protocol (the reference's dict keyed by (a, b))
     def __getitem__(self, key):
@@ -275,7 +276,11 @@ class RemlGrid(Mapping):
         V = Y.shape[0]
         ncols = V if rows is None else rows.shape[0]
         Vp = self.vpad(ncols)
-        Yt = torch.empty(self.Tpad, Vp, dtype=torch.float32, device=self.device)
+        n_rows = ctypes.c_int32()
+        _lib.check(_lib.load().ffs_reml_plan_yt_rows(self._handle, ctypes.byref(n_rows)), "plan_yt_rows")
+        # rows Tpad.. hold the residual's split halves in the 16-bit projection modes
+        Yt = torch.empty(n_rows.value, Vp, dtype=torch.float32, device=self.device)
+        Yt.ffs_projection = self.projection
         if scale is None:
             scale = torch.empty(V, dtype=torch.float32, device=self.device)
         feats = torch.empty(V, 10, dtype=torch.float32, device=self.device) if want_features else None
@@ -301,6 +306,9 @@ class RemlGrid(Mapping):
         """(V, K) REML score surface: recurrence energy then MFMA projection.
         ``n_chunks`` > 1 pipelines voxel chunks over two streams (measured no
         faster at T=400, V=200k: the energy kernel fills every CU)."""
+        if getattr(Yt, "ffs_projection", self.projection) != self.projection:
+            raise ValueError(f"residual prepared for projection {Yt.ffs_projection!r}, grid is in {self.projection!r}; "
+                             "call prep() again after set_projection()")
         S = out if out is not None else torch.empty(V, self.K, dtype=torch.float32, device=self.device)
         if n_chunks is None:
             n_chunks = 1

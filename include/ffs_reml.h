@@ -70,6 +70,12 @@ int ffs_reml_plan_logdets(const ffs_reml_plan* plan, double* logdet_R, double* l
  * time-major residual buffer (rows >= T must be zero), and the voxel padding unit. */
 int ffs_reml_plan_dims(const ffs_reml_plan* plan, int32_t* Tpad, int32_t* vox_pad, int32_t* K);
 
+/* Rows the Yt buffer of ffs_reml_prep / ffs_reml_project needs: Tpad, or
+ * 2 Tpad in the split 16-bit projection modes with p <= 32, where prep also
+ * writes the residual's hi / lo halves into rows Tpad..2 Tpad-1 and the
+ * projection reads them (prep and project must run in the same mode). */
+int ffs_reml_plan_yt_rows(const ffs_reml_plan* plan, int32_t* rows);
+
 /*
  * Normalise + residualise Y (V x T float32, row stride ldY) into the
  * time-major buffer Yt (Tpad x ldYt float32).  Column c of Yt holds voxel
