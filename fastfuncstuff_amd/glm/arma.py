@@ -553,7 +553,8 @@ def fit_glm_arma11(data, design, tr, a_grid=None, b_grid=None, estimate_per_voxe
                    r2_partial_mode="full", want_r2_semipartial=False, r2_semipartial_mode="full",
                    precomputed_arma_params=None, device=None, verbose=False, use_double=False,
                    force_exhaustive_search=False, task_indices=None, save_profile_likelihoods=False,
-                   run_starts=None, tau=None, grid: RemlGrid | None = None, **kwargs) -> ARMA11Results:
+                   run_starts=None, tau=None, grid: RemlGrid | None = None, projection: str = "fp32",
+                   **kwargs) -> ARMA11Results:
     """glm/arma.py:3650-5878 ``fit_glm_arma11`` on the MI355X engine.
 
     Per-voxel REML search over the (a, b) grid (hierarchical descent by default,
@@ -561,6 +562,8 @@ def fit_glm_arma11(data, design, tr, a_grid=None, b_grid=None, estimate_per_voxe
     the selected pair.  float32 mode divides each voxel by its std before the
     fit and unscales betas / sigma2 / residuals afterwards (glm/arma.py:3947-3954,
     5827-5862); ``use_double`` skips the normalisation.  Outputs are CPU tensors.
+    ``projection`` ("fp32", "fp16x3", "bf16x3") is the arithmetic of the surface
+    GEMM (an engine option; the reference computes it in fp32).
     """
     for key, what in _UNSUPPORTED.items():
         if kwargs.get(key) is not None:
@@ -608,6 +611,7 @@ def fit_glm_arma11(data, design, tr, a_grid=None, b_grid=None, estimate_per_voxe
             b_grid = db if b_grid is None else b_grid
         if grid is None:
             grid = RemlGrid(design, a_grid, b_grid, dev, run_starts=run_starts, tau=tau, task_indices=task_indices)
+        grid.set_projection(projection)
         fixed_best = None
 
     K = grid.K
@@ -676,6 +680,15 @@ def fit_glm_arma11(data, design, tr, a_grid=None, b_grid=None, estimate_per_voxe
                 res.predicted[b0:b1] = g["predicted"].cpu().to(out_dtype)
             if want_ljung_box:
                 res.ljung_box[b0:b1] = ljung_box_device(g["resid_w"], lj_lag, lj_tau).cpu()
+            if want_r2_partial or want_r2_semipartial:
+                part = _partial_r2(g["tstats"].to(out_dtype), g["r2"], res.dof, p, task_indices, want_r2_partial,
+                                   r2_partial_mode, want_r2_semipartial, r2_semipartial_mode)
+                for name, val in part.items():
+                    if val is None:
+                        continue
+                    if getattr(res, name) is None:
+                        setattr(res, name, torch.zeros((V,) + tuple(val.shape[1:]), dtype=val.dtype))
+                    getattr(res, name)[b0:b1] = val.cpu()
     a, b = res.arma_params[:, 0], res.arma_params[:, 1]
     res.arma_lambda = (b + a) * (1 + a * b) / (1 + 2 * a * b + b**2 + 1e-10)  # glm/arma.py:4769-4777
     if fixed_best is not None:
@@ -683,12 +696,112 @@ def fit_glm_arma11(data, design, tr, a_grid=None, b_grid=None, estimate_per_voxe
     if surface_acc is not None:
         res.reml_lklhd_surface = surface_acc
         res.reml_surface_params = list(grid.param_list)
-    if want_r2_partial or want_r2_semipartial:
-        _partial_r2(res, p, task_indices, want_r2_partial, r2_partial_mode, want_r2_semipartial,
-                    r2_semipartial_mode)
     return res
 
 
-def _partial_r2(res, p, task_indices, want_partial, partial_mode, want_semi, semi_mode):
-    """glm/arma.py:5054-5175: r2_partial = t^2/(t^2+dof); semi = partial*(1-R^2)."""
-    raise NotImplementedError("partial / semi-partial R^2 are not provided yet")
+def _partial_r2(t_full, r2, dof, p, task_indices, want_partial, partial_mode, want_semi, semi_mode) -> dict:
+    """glm/arma.py:5054-5175 on one batch: partial R^2 = t^2 / (t^2 + dof) per
+    regressor (all p t-statistics), split into task / nuisance columns; mode
+    "task" divides the task values by max(1 - sum(nuisance), 0.01); semi-partial
+    = partial * max(1 - R^2, 0), same split and rescaling."""
+    t2 = t_full**2
+    full = t2 / (t2 + dof)
+    if task_indices is not None:
+        nuis = sorted(set(range(p)) - set(task_indices))
+        task_part = full[:, list(task_indices)]
+        nuis_part = full[:, nuis] if nuis else None
+    else:
+        task_part, nuis_part = full, None
+    out = {}
+    if want_partial:
+        if partial_mode == "task" and nuis_part is not None:
+            out["r2_partial"] = task_part / torch.clamp(1.0 - nuis_part.sum(dim=1, keepdim=True), min=0.01)
+        else:
+            out["r2_partial"] = task_part
+        out["r2_partial_nuisance"] = nuis_part
+    if want_semi:
+        remain = torch.clamp(1.0 - r2.to(t_full.dtype).unsqueeze(1), min=0.0)
+        semi_task = task_part * remain
+        semi_nuis = nuis_part * remain if nuis_part is not None else None
+        if semi_mode == "task" and semi_nuis is not None:
+            out["r2_semipartial"] = semi_task / torch.clamp(1.0 - semi_nuis.sum(dim=1, keepdim=True), min=0.01)
+        else:
+            out["r2_semipartial"] = semi_task
+        out["r2_semipartial_nuisance"] = semi_nuis
+    return out
+
+
+def fit_glm_arma11_grouped(data, designs_by_group: dict, group_indices, tr: float, group_label: str = "group",
+                           **fit_kwargs) -> ARMA11Results:
+    """glm/arma.py:2836-3183: REML with one design per voxel group (per-voxel
+    HRF, slicewise baseline, ...).  Each group is fitted on its voxel subset
+    with its own plan (the design-dependent tables), on the same (a, b) grid,
+    and the per-voxel outputs are scattered back to the full voxel axis
+    (``VOXEL_SCATTER_ATTRS``, zeros elsewhere), as the reference merges them.
+    ``precomputed_arma_params`` is sliced per group."""
+    group_indices = torch.as_tensor(group_indices).long().cpu()
+    data_t = torch.as_tensor(data)
+    n_voxels = data_t.shape[0]
+    if group_indices.shape[0] != n_voxels:
+        raise ValueError(f"group_indices length {group_indices.shape[0]} != data n_voxels {n_voxels}")
+    groups = sorted(int(h) for h in group_indices.unique().tolist())
+    missing = [h for h in groups if h not in designs_by_group]
+    if missing:
+        raise KeyError(f"{group_label} ids {missing} present in voxels but missing from designs_by_group "
+                       f"(have {sorted(designs_by_group.keys())})")
+    first = torch.as_tensor(designs_by_group[groups[0]])
+    shape = tuple(first.shape)
+    for h in groups[1:]:
+        if tuple(torch.as_tensor(designs_by_group[h]).shape) != shape:
+            raise ValueError(f"designs_by_group[{h}] shape {tuple(torch.as_tensor(designs_by_group[h]).shape)} "
+                             f"!= {shape} — per-{group_label} designs must share dims")
+    kw = dict(fit_kwargs)
+    kw.pop("spatial_metadata", None)
+    pre = kw.pop("precomputed_arma_params", None)
+    if pre is not None:
+        pre = torch.as_tensor(pre).reshape(-1, 2)
+        if pre.shape[0] != n_voxels:
+            raise ValueError(f"precomputed_arma_params has {pre.shape[0]} rows but data has {n_voxels} voxels")
+    if kw.get("a_grid") is None or kw.get("b_grid") is None:
+        da, db = get_default_arma_grids()
+        kw.setdefault("a_grid", da)
+        kw.setdefault("b_grid", db)
+        kw["a_grid"] = da if kw["a_grid"] is None else kw["a_grid"]
+        kw["b_grid"] = db if kw["b_grid"] is None else kw["b_grid"]
+    per, masks = {}, {}
+    for h in groups:
+        m = group_indices == h
+        sub_kw = dict(kw)
+        if pre is not None:
+            sub_kw["precomputed_arma_params"] = pre[m]
+        per[h] = fit_glm_arma11(data_t[m], designs_by_group[h], tr, **sub_kw)
+        masks[h] = m
+    merged = ARMA11Results()
+    tpl = per[groups[0]]
+    for name in ("dof", "tr", "fitted_column_indices", "n_regressors_full", "contrast_labels", "ljung_box_dof"):
+        setattr(merged, name, getattr(tpl, name))
+    for attr in VOXEL_SCATTER_ATTRS + ["pair_index"]:
+        ref = next((getattr(per[h], attr) for h in groups if getattr(per[h], attr, None) is not None), None)
+        if ref is None:
+            continue
+        full = torch.zeros((n_voxels,) + tuple(ref.shape[1:]), dtype=ref.dtype)
+        for h in groups:
+            v = getattr(per[h], attr, None)
+            if v is not None:
+                full[masks[h]] = v.cpu()
+        setattr(merged, attr, full)
+    surf = next((h for h in groups if per[h].reml_lklhd_surface is not None), None)
+    if surf is not None:
+        merged.reml_surface_params = per[surf].reml_surface_params
+        ref = per[surf].reml_lklhd_surface
+        full = torch.zeros((n_voxels, ref.shape[1]), dtype=ref.dtype)
+        for h in groups:
+            if per[h].reml_lklhd_surface is not None:
+                full[masks[h]] = per[h].reml_lklhd_surface
+        merged.reml_lklhd_surface = full
+    return merged
+
+
+def fit_glm_arma11_perhrf(data, designs_by_hrf: dict, hrf_indices, tr: float, **fit_kwargs) -> ARMA11Results:
+    """glm/arma.py:3186-3204: per-voxel-HRF alias of the grouped fit."""
+    return fit_glm_arma11_grouped(data, designs_by_hrf, hrf_indices, tr, group_label="HRF", **fit_kwargs)

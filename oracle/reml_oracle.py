@@ -578,3 +578,30 @@ def fit_arma11(data, design, a_grid=None, b_grid=None, run_starts=None, tau=None
         betas, tst = betas[:, task_indices], tst[:, task_indices]
     return OracleFit(params, idx, best, surf, betas, tst, fst, s2, r2.to(torch.float32),
                      lam, lb, rw, grid, scale)
+
+
+def partial_r2(t_full, r2, dof, task_indices=None, partial_mode="full", semi_mode="full"):
+    """glm/arma.py:5054-5175 restated: partial R^2 = t^2 / (t^2 + dof) for every
+    regressor; with task_indices the task / nuisance columns are split and mode
+    "task" divides the task values by clamp(1 - sum(nuisance), min=0.01);
+    semi-partial = partial * clamp(1 - R^2, min=0) with the same split / mode.
+    Returns (r2_partial, r2_partial_nuisance, r2_semipartial, r2_semipartial_nuisance)."""
+    t_full = torch.as_tensor(t_full)
+    p = t_full.shape[1]
+    t2 = t_full**2
+    full = t2 / (t2 + dof)
+    if task_indices is not None:
+        nuis = sorted(set(range(p)) - set(task_indices))
+        task = full[:, list(task_indices)]
+        nu = full[:, nuis] if nuis else None
+    else:
+        task, nu = full, None
+    part = task / torch.clamp(1.0 - nu.sum(1, keepdim=True), min=0.01) if (partial_mode == "task" and nu is not None) \
+        else task
+    remain = torch.clamp(1.0 - torch.as_tensor(r2).to(t_full.dtype)[:, None], min=0.0)
+    st = task * remain
+    sn = nu * remain if nu is not None else None
+    semi = st / torch.clamp(1.0 - sn.sum(1, keepdim=True), min=0.01) if (semi_mode == "task" and sn is not None) \
+        else st
+    return part, nu, semi, sn
+

@@ -1,0 +1,123 @@
+"""GPU tests of the fit API (fit_glm_arma11 / fit_glm_arma11_grouped) against
+the oracle's restatement of the reference fit.
+
+Tolerances: selected pair equal to the float32 reference-semantics fit's except
+near-ties (score gap <= 2e-3); at the engine's pairs, against the FP64 oracle
+refit: betas rel <= 1e-5 (of max(|beta|, 1e-3 sd)), t rel <= 1e-5; partial /
+semi-partial R^2 abs <= 1e-4 (they come from the float32 t-statistics, and the
+"task" mode divides by 1 - sum(nuisance) >= 0.01).
+"""
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import reml_oracle as O
+
+pytestmark = pytest.mark.gpu
+F64 = torch.float64
+
+
+@pytest.fixture(scope="module")
+def dev():
+    from fastfuncstuff_amd import _lib
+
+    _lib.load()
+    return torch.device("cuda", 0)
+
+
+def _data(rng, T, X, V):
+    a = rng.uniform(0.1, 0.8, V)
+    b = rng.uniform(-0.5, 0.3, V)
+    e = np.zeros((V, T))
+    eps_prev = rng.standard_normal(V)
+    for t in range(T):
+        eps = rng.standard_normal(V)
+        e[:, t] = (a * e[:, t - 1] if t else 0) + eps + b * eps_prev
+        eps_prev = eps
+    return (rng.standard_normal((V, X.shape[1])) @ X.T * 0.5 + 20.0 + e).astype(np.float32)
+
+
+def _design(rng, T, shift=0):
+    box = np.zeros(T)
+    for s in range(5 + shift, T, 23):
+        box[s:s + 6] = 1.0
+    return np.column_stack([box, np.roll(box, 3 + shift), np.ones(T), np.linspace(-1, 1, T),
+                            rng.standard_normal(T) * 0.1])
+
+
+def _compare(res, ref, Y, X, ag, bg, task=None):
+    """Selection vs the float32 reference-semantics fit (near-ties allowed);
+    betas / t vs the FP64 oracle refit at the engine's pairs.  Returns the
+    FP64 refit's full t-statistics and R^2."""
+    V = Y.shape[0]
+    idx = res.pair_index.long()
+    same = idx == ref.pair_index.long()
+    gap = ref.surface[torch.arange(V), idx] - ref.surface[torch.arange(V), ref.pair_index.long()]
+    assert bool((gap[~same].abs() <= 2e-3).all()), f"{int((~same).sum())} mismatches beyond near-ties"
+    s = torch.from_numpy(Y).double().std(dim=1).clamp(min=1e-8)
+    g64 = O.precompute_grid(torch.from_numpy(X).double(), ag, bg, None, None, F64)
+    b64, t64, _, _, r2, _, _ = O.gls_refit(torch.from_numpy(X).double(), torch.from_numpy(Y).double() / s[:, None],
+                                           g64, idx, dtype=F64)
+    b64 = b64 * s[:, None]
+    bt, tt = (b64[:, task], t64[:, task]) if task else (b64, t64)
+    denom = torch.maximum(bt.abs(), 1e-3 * s[:, None])
+    assert float(((res.betas.double() - bt).abs() / denom).max()) <= 1e-5
+    assert float(((res.tstats.double() - tt).abs() / tt.abs().clamp(min=1)).max()) <= 1e-5
+    return t64, r2
+
+
+def test_fit_partial_r2_matches_oracle(dev):
+    from fastfuncstuff_amd.glm.arma import fit_glm_arma11
+
+    print("[fit] partial r2", flush=True)
+    rng = np.random.default_rng(5)
+    T, V = 110, 240
+    X = _design(rng, T)
+    Y = _data(rng, T, X, V)
+    ag = bg = torch.linspace(-0.8, 0.8, 17)
+    task = [0, 1]
+    for pm, sm in (("full", "full"), ("task", "task")):
+        res = fit_glm_arma11(Y, X, 2.0, a_grid=ag, b_grid=bg, device=dev, task_indices=task, want_r2_partial=True,
+                             r2_partial_mode=pm, want_r2_semipartial=True, r2_semipartial_mode=sm)
+        ref = O.fit_arma11(torch.from_numpy(Y), torch.from_numpy(X).float(), ag, bg, dtype=torch.float32,
+                           task_indices=task)
+        t_full, r2 = _compare(res, ref, Y, X, ag, bg, task)
+        # oracle partial R^2 from the FP64 refit's full t-statistics and R^2
+        part, nu, semi, sn = O.partial_r2(t_full, r2.double(), res.dof, task, pm, sm)
+        for got, want in ((res.r2_partial, part), (res.r2_partial_nuisance, nu), (res.r2_semipartial, semi),
+                          (res.r2_semipartial_nuisance, sn)):
+            assert float((got.double() - want).abs().max()) <= 1e-4
+
+
+def test_grouped_fit_matches_per_group_oracle(dev):
+    from fastfuncstuff_amd.glm.arma import fit_glm_arma11_grouped
+
+    print("[fit] grouped", flush=True)
+    rng = np.random.default_rng(6)
+    T, V = 100, 300
+    designs = {g: _design(rng, T, shift=g) for g in (0, 3, 7)}
+    gidx = rng.choice([0, 3, 7], V, p=[0.6, 0.3, 0.1])
+    Y = np.empty((V, T), dtype=np.float32)
+    for g, X in designs.items():
+        m = gidx == g
+        Y[m] = _data(rng, T, X, int(m.sum()))
+    ag = bg = torch.linspace(-0.8, 0.8, 17)
+    res = fit_glm_arma11_grouped(Y, {g: torch.from_numpy(X) for g, X in designs.items()}, torch.from_numpy(gidx),
+                                 2.0, group_label="HRF", a_grid=ag, b_grid=bg, device=dev)
+    assert res.betas.shape == (V, 5) and res.sigma2.shape == (V,)
+    for g, X in designs.items():
+        m = gidx == g
+        ref = O.fit_arma11(torch.from_numpy(Y[m]), torch.from_numpy(X).float(), ag, bg, dtype=torch.float32)
+
+        class Sub:
+            pass
+
+        sub = Sub()
+        for name in ("pair_index", "betas", "tstats"):
+            setattr(sub, name, getattr(res, name)[torch.from_numpy(m)])
+        _compare(sub, ref, Y[m], X, ag, bg)
+    with pytest.raises(KeyError):
+        fit_glm_arma11_grouped(Y, {0: designs[0]}, torch.from_numpy(gidx), 2.0, device=dev)
+    with pytest.raises(ValueError):
+        fit_glm_arma11_grouped(Y, designs, torch.from_numpy(gidx[:-1]), 2.0, device=dev)

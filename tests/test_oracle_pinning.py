@@ -298,3 +298,31 @@ def test_float32_oracle_selects_float64_parameters():
                        selector="exhaustive")
     assert float((f64.pair_index == f32.pair_index).double().mean()) >= 0.95
     np.testing.assert_allclose(f32.sigma2.double().numpy(), f64.sigma2.numpy(), rtol=2e-3)
+
+
+def test_partial_r2_is_squared_partial_correlation():
+    """Textbook identity that pins oracle.partial_r2: with white noise (a = b = 0
+    is GLS = OLS) t_i^2 / (t_i^2 + dof) equals (RSS_without_i - RSS_full) /
+    RSS_without_i from two least-squares fits; semi-partial = partial * (1 - R^2)."""
+    rng = np.random.default_rng(31)
+    T, p, V = 70, 4, 5
+    X = np.column_stack([np.ones(T), rng.standard_normal((T, p - 1))])
+    Y = rng.standard_normal((V, p)) @ X.T * 0.3 + rng.standard_normal((V, T))
+    grid = O.precompute_grid(torch.from_numpy(X), torch.tensor([0.0]), torch.tensor([0.0]), None, None, F64)
+    betas, tst, _, _, r2, _, _ = O.gls_refit(torch.from_numpy(X), torch.from_numpy(Y), grid,
+                                             torch.zeros(V, dtype=torch.long), dtype=F64)
+    dof = T - p
+    part, nu, semi, sn = O.partial_r2(tst, r2.double(), dof)
+    assert nu is None and sn is None
+    for v in range(V):
+        rss_full = np.sum((Y[v] - X @ np.linalg.lstsq(X, Y[v], rcond=None)[0]) ** 2)
+        for i in range(p):
+            Xi = np.delete(X, i, axis=1)
+            rss_i = np.sum((Y[v] - Xi @ np.linalg.lstsq(Xi, Y[v], rcond=None)[0]) ** 2)
+            assert abs(float(part[v, i]) - (rss_i - rss_full) / rss_i) < 1e-8  # t has +1e-10 in its denominator
+    np.testing.assert_allclose(semi.numpy(), part.numpy() * (1 - r2.double().numpy())[:, None], rtol=1e-12)
+    # task split and "task" rescaling
+    part_t, nu_t, _, _ = O.partial_r2(tst, r2.double(), dof, task_indices=[1, 2], partial_mode="task")
+    expect = part[:, [1, 2]] / torch.clamp(1 - part[:, [0, 3]].sum(1, keepdim=True), min=0.01)
+    np.testing.assert_allclose(part_t.numpy(), expect.numpy(), rtol=1e-12)
+    np.testing.assert_allclose(nu_t.numpy(), part[:, [0, 3]].numpy(), rtol=1e-12)
