@@ -525,7 +525,8 @@ extern "C" int ffs_reml_surface(const ffs_reml_plan* P, const float* Yt, int64_t
 extern "C" int ffs_reml_select(const ffs_reml_plan* P, const float* S, int64_t ldS, int64_t V, int32_t mode,
                                int32_t* best, float* best_score, void* stream) {
     if (!P) return fail(-3, "select: null plan");
-    if (mode != 0 && mode != 1) return fail(-3, "select: mode must be 0 (exhaustive) or 1 (hierarchical)");
+    if (mode < 0 || mode > 2)
+        return fail(-3, "select: mode must be 0 (exhaustive), 1 (hierarchical) or 2 (subset descent)");
     if (V <= 0) return 0;
     hipStream_t s = (hipStream_t)stream;
     hipLaunchKernelGGL(k_select, dim3((unsigned)((V + 255) / 256)), dim3(256), 0, s, P->d, S, ldS, (int)V, mode,

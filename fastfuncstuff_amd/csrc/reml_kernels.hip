@@ -834,6 +834,7 @@ __global__ void __launch_bounds__(256) k_project16(PlanDev P, const float* __res
 // ---------------------------------------------------------------------------
 // (iii) Selection over the (V, K) surface.
 //   mode 0: global argmin, strict '<' in pair order (glm/arma.py:2006-2010)
+//   mode 2: subset descent (glm/arma.py:2021-2221), see below
 //   mode 1: hierarchical descent (glm/arma.py:2224-2322): start (0,0); top
 //           level = coarse lattice; lower levels = clamped 3x3 at +/-step;
 //           candidates b-outer / a-inner, first minimum; a NaN candidate
@@ -862,6 +863,43 @@ __global__ void __launch_bounds__(256) k_select(PlanDev P, const float* __restri
         }
         best[v] = bk >= 0 ? bk : k00;
         best_score[v] = bs;
+        return;
+    }
+    if (mode == 2) {
+        // subset descent (glm/arma.py:2021-2221): per level the lattice points
+        // that are multiples of step inside the +/-step window around the best
+        // point at the level's start (whole lattice at the top level), b-outer
+        // / a-inner, each compared with a sequential strict '<' (the best point
+        // moves during the sweep); invalid pairs are skipped.
+        int ba = P.zero_a, bb = P.zero_b;
+        float bst = INFINITY;
+        {
+            const float s0 = lat_val(P, row, ba, bb);
+            if (s0 < bst) bst = s0;
+        }
+        for (int lev = P.ltop; lev >= 0; --lev) {
+            const int step = 1 << lev;
+            int alo = 0, ahi = P.n_a - 1, blo = 0, bhi = P.n_b - 1;
+            if (lev != P.ltop) {
+                alo = max(ba - step, 0);
+                ahi = min(ba + step, P.n_a - 1);
+                blo = max(bb - step, 0);
+                bhi = min(bb + step, P.n_b - 1);
+            }
+            for (int y = (blo + step - 1) / step * step; y <= bhi; y += step)
+                for (int x = (alo + step - 1) / step * step; x <= ahi; x += step) {
+                    const int k = P.lut[x * P.n_b + y];
+                    if (k < 0) continue;
+                    const float sv = row[k];
+                    if (sv < bst) {
+                        bst = sv;
+                        ba = x;
+                        bb = y;
+                    }
+                }
+        }
+        best[v] = P.lut[ba * P.n_b + bb];
+        best_score[v] = bst;
         return;
     }
     int ba = P.zero_a, bb = P.zero_b;

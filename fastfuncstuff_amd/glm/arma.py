@@ -327,11 +327,23 @@ class RemlGrid(Mapping):
         _lib.check(_lib.load().ffs_reml_plan_set_projection(self._handle, self.PROJECTIONS[mode]), "set_projection")
         self.projection = mode
 
-    def select(self, S: torch.Tensor, hierarchical: bool):
+    SELECTORS = {"exhaustive": 0, "hierarchical": 1, "subset": 2}
+
+    def select(self, S: torch.Tensor, hierarchical: bool | str):
+        """Pair index + score per voxel.  ``hierarchical``: True / "hierarchical"
+        (descent on the surface, glm/arma.py:2224-2322), False / "exhaustive"
+        (global first minimum), or "subset" (the subset descent of
+        glm/arma.py:2021-2221)."""
+        if isinstance(hierarchical, str):
+            if hierarchical not in self.SELECTORS:
+                raise ValueError(f"selector must be one of {sorted(self.SELECTORS)}, got {hierarchical!r}")
+            mode = self.SELECTORS[hierarchical]
+        else:
+            mode = int(bool(hierarchical))
         V = S.shape[0]
         best = torch.empty(V, dtype=torch.int32, device=self.device)
         score = torch.empty(V, dtype=torch.float32, device=self.device)
-        _lib.check(_lib.load().ffs_reml_select(self._handle, _lib.ptr(S), S.stride(0), V, int(hierarchical),
+        _lib.check(_lib.load().ffs_reml_select(self._handle, _lib.ptr(S), S.stride(0), V, mode,
                                                _lib.ptr(best), _lib.ptr(score), _lib.stream_handle(self.device)),
                    "select")
         return best, score
@@ -390,6 +402,24 @@ def search_voxels_precomputed_grid(X, Y_batch, precomputed_grid: RemlGrid, devic
     if return_profile:
         return params, score, S, list(grid.param_list)
     return params, score
+
+
+def search_voxels_precomputed_grid_hierarchical(X, Y_batch, precomputed_grid: RemlGrid, device=None,
+                                                verbose=False):
+    """glm/arma.py:2021-2221: per-voxel subset descent (start (0,0), lattice
+    points that are multiples of 2^lev inside +/-2^lev of the running best,
+    sequential strict '<').  The scores of the visited pairs are the
+    surface's, so the descent runs on the device surface.  Returns
+    (best_params (V, 2), best_lik (V,)) on the device."""
+    grid = precomputed_grid
+    Y = _to_device_f32(Y_batch, grid.device)
+    if Y.ndim != 2 or Y.shape[1] != grid.T:
+        raise ValueError(f"Timepoints mismatch: Data has {Y.shape[-1]}, Design has {grid.T}")
+    V = Y.shape[0]
+    Yt, _, _ = grid.prep(Y, normalize=False)
+    S = grid.surface(Yt, V)
+    best, score = grid.select(S, "subset")
+    return grid.params.to(grid.device)[best.long()], score
 
 
 def select_arma_params_hierarchical_from_surface(surface: torch.Tensor, param_list, device=None,
@@ -554,7 +584,7 @@ def fit_glm_arma11(data, design, tr, a_grid=None, b_grid=None, estimate_per_voxe
                    precomputed_arma_params=None, device=None, verbose=False, use_double=False,
                    force_exhaustive_search=False, task_indices=None, save_profile_likelihoods=False,
                    run_starts=None, tau=None, grid: RemlGrid | None = None, projection: str = "fp32",
-                   **kwargs) -> ARMA11Results:
+                   selector: str | None = None, **kwargs) -> ARMA11Results:
     """glm/arma.py:3650-5878 ``fit_glm_arma11`` on the MI355X engine.
 
     Per-voxel REML search over the (a, b) grid (hierarchical descent by default,
@@ -563,7 +593,10 @@ def fit_glm_arma11(data, design, tr, a_grid=None, b_grid=None, estimate_per_voxe
     fit and unscales betas / sigma2 / residuals afterwards (glm/arma.py:3947-3954,
     5827-5862); ``use_double`` skips the normalisation.  Outputs are CPU tensors.
     ``projection`` ("fp32", "fp16x3", "bf16x3") is the arithmetic of the surface
-    GEMM (an engine option; the reference computes it in fp32).
+    GEMM (an engine option; the reference computes it in fp32).  ``selector``
+    picks the search rule: "hierarchical" (default, the reference's GPU path),
+    "subset" (the reference's CPU default, glm/arma.py:2021-2221) or
+    "exhaustive" (same as ``force_exhaustive_search``).
     """
     for key, what in _UNSUPPORTED.items():
         if kwargs.get(key) is not None:
@@ -651,7 +684,7 @@ def fit_glm_arma11(data, design, tr, a_grid=None, b_grid=None, estimate_per_voxe
                 Yt, scale, _ = grid.prep(Yb, normalize=not use_double)
                 S = grid.surface(Yt, Vb)
                 del Yt
-                best, score = grid.select(S, hierarchical=not force_exhaustive_search)
+                best, score = grid.select(S, "exhaustive" if force_exhaustive_search else (selector or "hierarchical"))
                 if surface_acc is not None:
                     surface_acc[b0:b1] = S.cpu()
                 del S

@@ -111,7 +111,10 @@ int ffs_reml_plan_set_projection(ffs_reml_plan* plan, int32_t mode);
 int ffs_reml_surface(const ffs_reml_plan* plan, const float* Yt, int64_t ldYt, int64_t V, float* S, int64_t ldS,
                      int32_t n_chunks, void* stream);
 
-/* (iii) mode 0: global argmin (first minimum); mode 1: hierarchical descent. */
+/* (iii) mode 0: global argmin (first minimum), glm/arma.py:1886-2018;
+ * mode 1: hierarchical descent on the surface, glm/arma.py:2224-2322 (the
+ * reference's GPU default); mode 2: subset descent, glm/arma.py:2021-2221 (the
+ * reference's CPU default), evaluated on the surface. */
 int ffs_reml_select(const ffs_reml_plan* plan, const float* S, int64_t ldS, int64_t V, int32_t mode, int32_t* best,
                     float* best_score, void* stream);
 

@@ -217,3 +217,25 @@ def test_gls_groups_spanning_waves(dev):
     _check_gls({k: v[ok.to(dev)] for k, v in out.items() if v is not None}, ref, s_ref[ok])
     for v in (5, 6):
         assert bool(torch.isnan(out["betas"][v]).all()) and bool(torch.isnan(out["sigma2"][v]))
+
+
+@pytest.mark.parametrize("name", ["plain", "censored", "afni"])
+def test_selectors_exact_on_device_surface(dev, name):
+    """The three selection rules on the engine's own float32 surface give
+    exactly the oracle's pair for every voxel (same inputs, same rules)."""
+    from fastfuncstuff_amd.glm.arma import RemlGrid
+
+    print(f"[parity] selectors case={name}", flush=True)
+    X, Y, ag, bg, run_starts, tau = _case(name)
+    grid = RemlGrid(torch.from_numpy(X), ag, bg, dev, run_starts=run_starts, tau=tau)
+    Yt, _, _ = grid.prep(torch.from_numpy(Y).to(dev))
+    S = grid.surface(Yt, Y.shape[0])
+    Sc = S.cpu()
+    params = grid.params.float()
+    for mode, ref in (("exhaustive", lambda: O.select_exhaustive(Sc)),
+                      ("hierarchical", lambda: O.select_hierarchical(Sc, params)),
+                      ("subset", lambda: O.search_hierarchical_subset(Sc, params))):
+        best, score = grid.select(S, mode)
+        k_ref, s_ref = ref()
+        assert bool((best.cpu().long() == k_ref.long()).all()), mode
+        assert bool((score.cpu() == s_ref.float()).all()), mode
