@@ -25,7 +25,10 @@ SIGNATURES: dict[str, list] = {
     "ffs_reml_last_error": [],
     "ffs_reml_plan_create": [_pp, _c_int, _c_int, _vp, _vp, _vp, _c_int, _vp, _vp, _c_int, _c_int, _vp, _c_int,
                              _c_int, _c_int, _c_int, _vp, _vp],
+    "ffs_reml_plan_create_ridge": [_pp, _c_int, _c_int, _vp, _vp, _vp, _c_int, _vp, _vp, _c_int, _c_int, _vp, _c_int,
+                                   _c_int, _c_int, ctypes.c_double, _vp],
     "ffs_reml_plan_destroy": [_vp],
+    "ffs_reml_dense_likelihood": [_vp, _vp, _c_int, _vp, _c_int, _c_int, _vp, _i64, _vp, _vp],
     "ffs_reml_plan_logdets": [_vp, _vp, _vp],
     "ffs_reml_plan_dims": [_vp, _vp, _vp, _vp],
     "ffs_reml_plan_set_projection": [_vp, _c_int],

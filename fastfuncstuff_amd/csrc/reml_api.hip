@@ -17,6 +17,7 @@
 #include "reml_sort_gls.hip"
 #include "reml_prep_mfma.hip"
 #include "reml_proj16.hip"
+#include "reml_dense.hip"
 
 using namespace remlk;
 
@@ -41,6 +42,7 @@ struct ffs_reml_plan {
     const double* ldR = nullptr;
     const double* ldX = nullptr;
     bool local_ok = false;
+    double ridge = 0.0;  // > 0: search-only plan with the ridge-regularised score
     // second stream + events of ffs_reml_surface (created on first use)
     static constexpr int kMaxSurfaceChunks = 16;
     mutable hipStream_t aux = nullptr;
@@ -96,10 +98,32 @@ static int pick_bn(int p, int* npt) {
     return best;
 }
 
+static int plan_create_impl(ffs_reml_plan** out, int32_t T, int32_t p, const double* X, const double* pinvT,
+                            const int32_t* gap, int32_t K, const double* ab, const int32_t* white, int32_t n_a,
+                            int32_t n_b, const int32_t* lut, int32_t zero_a, int32_t zero_b, int32_t ltop,
+                            int32_t n_task, const int32_t* task_idx, double ridge, void* stream);
+
 extern "C" int ffs_reml_plan_create(ffs_reml_plan** out, int32_t T, int32_t p, const double* X, const double* pinvT,
                                     const int32_t* gap, int32_t K, const double* ab, const int32_t* white,
                                     int32_t n_a, int32_t n_b, const int32_t* lut, int32_t zero_a, int32_t zero_b,
                                     int32_t ltop, int32_t n_task, const int32_t* task_idx, void* stream) {
+    return plan_create_impl(out, T, p, X, pinvT, gap, K, ab, white, n_a, n_b, lut, zero_a, zero_b, ltop, n_task,
+                            task_idx, 0.0, stream);
+}
+
+extern "C" int ffs_reml_plan_create_ridge(ffs_reml_plan** out, int32_t T, int32_t p, const double* X,
+                                          const double* pinvT, const int32_t* gap, int32_t K, const double* ab,
+                                          const int32_t* white, int32_t n_a, int32_t n_b, const int32_t* lut,
+                                          int32_t zero_a, int32_t zero_b, int32_t ltop, double ridge, void* stream) {
+    if (!(ridge > 0.0)) return fail(-3, "plan_create_ridge: ridge must be > 0");
+    return plan_create_impl(out, T, p, X, pinvT, gap, K, ab, white, n_a, n_b, lut, zero_a, zero_b, ltop, 0, nullptr,
+                            ridge, stream);
+}
+
+static int plan_create_impl(ffs_reml_plan** out, int32_t T, int32_t p, const double* X, const double* pinvT,
+                            const int32_t* gap, int32_t K, const double* ab, const int32_t* white, int32_t n_a,
+                            int32_t n_b, const int32_t* lut, int32_t zero_a, int32_t zero_b, int32_t ltop,
+                            int32_t n_task, const int32_t* task_idx, double ridge, void* stream) {
     if (!out) return fail(-3, "plan_create: null output handle");
     *out = nullptr;
     if (T < 2 || p < 1 || K < 1) return fail(-3, "plan_create: need T >= 2, p >= 1, K >= 1");
@@ -250,7 +274,7 @@ extern "C" int ffs_reml_plan_create(ffs_reml_plan** out, int32_t T, int32_t p, c
     }
     hipLaunchKernelGGL(k_tables, dim3(K), dim3(64), 0, s, T, d.Tpad, p, K, dab, dwhiteflag, dgap, dX, d.n_task, dtask,
                        full_f, BN, npt, soa, coef64, Xw64, G64, Lx, Ainv, Fm, work, ld, ldR, ldX, G32, Gp, Ghf, Glf, Ghb, Glb, g16s,
-                       d.PG, G64p, coefR, Lhf, Llf, Lhb, Llb, dstatus);
+                       d.PG, G64p, coefR, Lhf, Llf, Lhb, Llb, ridge, dstatus);
     HIP_TRY(hipGetLastError());
     hipLaunchKernelGGL(k_local_tables, dim3(K), dim3(128), 0, s, T, K, soa, lp, lc, ls0);
     HIP_TRY(hipGetLastError());
@@ -305,6 +329,7 @@ extern "C" int ffs_reml_plan_create(ffs_reml_plan** out, int32_t T, int32_t p, c
     d.pinvTp = dPp;
     d.proj_mode = 0;
     P->pinvT = dpinv;
+    P->ridge = ridge;
     P->ldR = ldR;
     P->ldX = ldX;
     *out = guard.release();
@@ -588,6 +613,7 @@ extern "C" int ffs_reml_gls(const ffs_reml_plan* P, const float* Y, int64_t ldY,
                             float* sigma2, float* r2, double* resid_w, int64_t ldR, float* resid, float* pred,
                             int32_t* work, void* stream) {
     if (!P) return fail(-3, "gls: null plan");
+    if (P->ridge > 0.0) return fail(-3, "gls: a ridge plan is search-only");
     if (V <= 0) return 0;
     if (V > 0x7fffffff - 64 * (int64_t)P->d.K) return fail(-3, "gls: batch too large; split it");
     if (ldY < P->d.T) return fail(-3, "gls: ldY must be >= T");
@@ -832,3 +858,15 @@ extern "C" int ffs_reml_plan_set_projection(ffs_reml_plan* P, int32_t mode) {
     P->d.proj_mode = mode;
     return 0;
 }
+
+extern "C" int ffs_reml_dense_likelihood(const double* X, const double* Y, int32_t m, const double* R, int32_t T,
+                                         int32_t p, double* work, int64_t work_doubles, double* out, void* stream) {
+    if (T < 1 || p < 1 || m < 1 || p > T) return fail(-3, "dense_likelihood: need 1 <= p <= T and m >= 1");
+    const int64_t need = (int64_t)T * T + (int64_t)T * (p + m) + (int64_t)p * p + (int64_t)p * m;
+    if (!work || work_doubles < need) return fail(-3, "dense_likelihood: workspace too small");
+    hipLaunchKernelGGL(k_dense_reml, dim3(1), dim3(kDenseThreads), 0, (hipStream_t)stream, X, Y, (int)m, R, (int)T,
+                       (int)p, work, out);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+

@@ -43,7 +43,7 @@ __global__ void __launch_bounds__(64) k_tables(int T, int Tpad, int p, int K, co
                                                int PG, double* __restrict__ G64p, double* __restrict__ coefR,
                                                uint16_t* __restrict__ Lhf, uint16_t* __restrict__ Llf,
                                                uint16_t* __restrict__ Lhb, uint16_t* __restrict__ Llb,
-                                               int* __restrict__ status) {
+                                               double ridge, int* __restrict__ status) {
     const int k = blockIdx.x;
     if (k >= K) return;
     const int tid = threadIdx.x;
@@ -90,13 +90,33 @@ __global__ void __launch_bounds__(64) k_tables(int T, int Tpad, int p, int K, co
             for (int i = 0; i < p * p; ++i) F[i] = L[i];
         else
             for (int i = 0; i < p * p; ++i) F[i] = 0.0;
+        // ridge plans (glm/arma.py:1160-1176, 2724-2755): A + ridge I in the
+        // log-determinant and in the solve for beta
+        if (ridge > 0.0)
+            for (int j = 0; j < p; ++j) L[j * p + j] += ridge;
         if (cholesky_inplace(p, L) != 0) {
             sh_fail = 1;
         } else {
             double s = 0.0;
-            for (int j = 0; j < p; ++j) s += 2.0 * log(fabs(L[j * p + j]) + 1e-10);
+            if (ridge > 0.0)
+                for (int j = 0; j < p; ++j) s += 2.0 * log(L[j * p + j]);
+            else
+                for (int j = 0; j < p; ++j) s += 2.0 * log(fabs(L[j * p + j]) + 1e-10);
             sh_ldX = s;
             cholesky_inverse(p, L, Ai, wk);
+            if (ridge > 0.0) {
+                // |y_w - X_w beta|^2 = E - c' W c, c = X_w' y_w, M = (A + rI)^-1,
+                // W = M (A + 2rI) M = M + r M M; L <- chol(W)
+                double* Wm = wk;
+                for (int i = 0; i < p; ++i)
+                    for (int j = 0; j < p; ++j) {
+                        double mm = 0.0;
+                        for (int q = 0; q < p; ++q) mm += Ai[i * p + q] * Ai[q * p + j];
+                        Wm[i * p + j] = Ai[i * p + j] + ridge * mm;
+                    }
+                for (int i = 0; i < p * p; ++i) L[i] = Wm[i];
+                if (cholesky_inplace(p, L) != 0) sh_fail = 1;
+            }
             if (!full_f) {
                 // inv(A^-1[task, task] + 1e-8 I)   (glm/arma.py:4888-4896)
                 // At (wk) -> inverse in F's storage (work: W2) -> copied to W2 ->
@@ -123,10 +143,19 @@ __global__ void __launch_bounds__(64) k_tables(int T, int Tpad, int p, int K, co
         if (tid == 0) atomicMax(status, sh_fail);
         return;
     }
-    // Q rows: q_t = Lx^-1 xw_t  (forward substitution), stored in g.
+    // Q rows: q_t = Lx^-1 xw_t (forward substitution), stored in g; ridge
+    // plans: q_t = chol(W)' xw_t.
     for (int t = tid; t < T; t += 64) {
         const double* x = xw + (size_t)t * p;
         double* q = g + (size_t)t * p;
+        if (ridge > 0.0) {
+            for (int j = 0; j < p; ++j) {
+                double s = 0.0;
+                for (int i = j; i < p; ++i) s += L[i * p + j] * x[i];
+                q[j] = s;
+            }
+            continue;
+        }
         for (int i = 0; i < p; ++i) {
             double s = x[i];
             for (int m = 0; m < i; ++m) s -= L[i * p + m] * q[m];
@@ -353,7 +382,7 @@ __global__ void __launch_bounds__(64) k_prep(int T, int p, const float* __restri
         }
     }
     double s = 1.0;
-    if (normalize) {
+    if (normalize & 1) {
         const double mean = s1 / T;
         double var = T > 1 ? (s2 - T * mean * mean) / (T - 1) : 0.0;
         double sd = var > 0.0 ? sqrt(var) : 0.0;
@@ -361,7 +390,7 @@ __global__ void __launch_bounds__(64) k_prep(int T, int p, const float* __restri
     }
     const double inv = 1.0 / s;
 #pragma unroll
-    for (int j = 0; j < PM; ++j) c[j] *= inv;
+    for (int j = 0; j < PM; ++j) c[j] = (normalize & 2) ? 0.0 : c[j] * inv;  // bit 1: no residualisation
     float hist[NF > 0 ? NF : 1];
     double acf[NF > 0 ? NF : 1];
     double acf0 = 0.0;

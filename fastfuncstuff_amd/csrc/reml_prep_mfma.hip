@@ -16,7 +16,8 @@ __device__ __forceinline__ int prep_voxel(int c, int ncols, const int32_t* __res
 // following glm/arma.py:1554-1571 (std normalisation, projection on the
 // design's pseudo-inverse, residual):
 //   pass 1: c = pinv(X) y on MFMA (k-slot q of step m = time t0 + 4q + m) and
-//           the sums of y, y^2 -> s = max(float(sd), 1e-8);
+//           the sums of y, y^2 -> s = max(float(sd), 1e-8) (normalize bit 0;
+//           bit 1 keeps c = 0, i.e. no residualisation);
 //   pass 2: X c on MFMA for 16-row time tiles, r = y / s - X c / s written to
 //           Yt[t][column] (rows T..Tpad-1 come out zero); optional lag-1..NF
 //           autocorrelations of r through an LDS tile.
@@ -92,7 +93,7 @@ __global__ void __launch_bounds__(64) k_prep_mfma(PlanDev P, const float* __rest
         s2[g] += __shfl_xor(s2[g], 16);
         s2[g] += __shfl_xor(s2[g], 32);
         double s = 1.0;
-        if (normalize) {
+        if (normalize & 1) {
             const double mean = s1[g] / T;
             const double var = T > 1 ? (s2[g] - T * mean * mean) / (T - 1) : 0.0;
             const double sd = var > 0.0 ? sqrt(var) : 0.0;
@@ -102,6 +103,12 @@ __global__ void __launch_bounds__(64) k_prep_mfma(PlanDev P, const float* __rest
         if (kq == 0 && vB[g] >= 0 && scale) scale[vB[g]] = (float)s;
     }
     if (!Yt && NF == 0) return;
+    if (normalize & 2) {  // keep the design component (no residualisation)
+#pragma unroll
+        for (int g = 0; g < 4; ++g)
+#pragma unroll
+            for (int rt = 0; rt < NT; ++rt) acc[g][rt] = f64x4{0.0, 0.0, 0.0, 0.0};
+    }
 
     // ---- pass 2 --------------------------------------------------------
     // A row i of a tile holds time tt + 4 (i % 4) + i / 4, so output element r

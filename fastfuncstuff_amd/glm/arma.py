@@ -175,7 +175,8 @@ class RemlGrid(Mapping):
     reference's dict; the whiteners, whitened designs and projection operands
     live on the device inside the plan."""
 
-    def __init__(self, X, a_grid, b_grid, device, run_starts=None, tau=None, task_indices=None, pairs=None):
+    def __init__(self, X, a_grid, b_grid, device, run_starts=None, tau=None, task_indices=None, pairs=None,
+                 ridge: float = 0.0, ensure_zero: bool = True):
         self.device = _lib.require_device(device)
         lib = _lib.load()
         X = torch.as_tensor(X)
@@ -184,7 +185,9 @@ class RemlGrid(Mapping):
         self.X64 = np.ascontiguousarray(X.detach().cpu().to(torch.float64).numpy())
         self.T, self.p = self.X64.shape
         if pairs is None:
-            a_grid, b_grid = ensure_zero_in_grid(_grid_tensor(a_grid, "a_grid"), _grid_tensor(b_grid, "b_grid"))
+            a_grid, b_grid = _grid_tensor(a_grid, "a_grid"), _grid_tensor(b_grid, "b_grid")
+            if ensure_zero:
+                a_grid, b_grid = ensure_zero_in_grid(a_grid, b_grid)
             pairs = valid_grid_pairs(a_grid, b_grid)
         self.params = pairs.to(torch.float32) if pairs.dtype != torch.float64 else pairs
         if self.params.shape[0] == 0:
@@ -203,6 +206,9 @@ class RemlGrid(Mapping):
         for k in range(self.K):
             lut[ai[a_np[k]], bi[b_np[k]]] = k
         self.lut = lut
+        self.ridge = float(ridge)
+        if self.ridge > 0 and task_indices is not None:
+            raise ValueError("a ridge plan is search-only (no task subset)")
         za = next((i for i, v in enumerate(self.a_axis) if abs(v) < 1e-6), None)
         zb = next((i for i, v in enumerate(self.b_axis) if abs(v) < 1e-6), None)
         self.has_zero = za is not None and zb is not None and lut[za, zb] >= 0
@@ -219,11 +225,17 @@ class RemlGrid(Mapping):
         self.task_indices = None if task_indices is None else list(task_indices)
         handle = ctypes.c_void_p()
         with torch.cuda.device(self.device):
-            rc = lib.ffs_reml_plan_create(
-                ctypes.byref(handle), self.T, self.p, self.X64.ctypes.data, pinvT.ctypes.data,
-                self.gap.ctypes.data, self.K, ab.ctypes.data, np.ascontiguousarray(white).ctypes.data,
-                n_a, n_b, lut.ctypes.data, za, zb, self.ltop, int(task.size),
-                task.ctypes.data if task.size else None, _lib.stream_handle(self.device))
+            if self.ridge > 0:
+                rc = lib.ffs_reml_plan_create_ridge(
+                    ctypes.byref(handle), self.T, self.p, self.X64.ctypes.data, pinvT.ctypes.data,
+                    self.gap.ctypes.data, self.K, ab.ctypes.data, np.ascontiguousarray(white).ctypes.data,
+                    n_a, n_b, lut.ctypes.data, za, zb, self.ltop, self.ridge, _lib.stream_handle(self.device))
+            else:
+                rc = lib.ffs_reml_plan_create(
+                    ctypes.byref(handle), self.T, self.p, self.X64.ctypes.data, pinvT.ctypes.data,
+                    self.gap.ctypes.data, self.K, ab.ctypes.data, np.ascontiguousarray(white).ctypes.data,
+                    n_a, n_b, lut.ctypes.data, za, zb, self.ltop, int(task.size),
+                    task.ctypes.data if task.size else None, _lib.stream_handle(self.device))
         _lib.check(rc, "precompute_reml_grid")
         self._handle = handle
         tp, vp, _ = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
@@ -269,7 +281,7 @@ class RemlGrid(Mapping):
         return (V + self.vox_pad - 1) // self.vox_pad * self.vox_pad
 
     def prep(self, Y: torch.Tensor, normalize: bool = True, want_features: bool = False,
-             rows: torch.Tensor | None = None, scale: torch.Tensor | None = None):
+             rows: torch.Tensor | None = None, scale: torch.Tensor | None = None, residualize: bool = True):
         """Normalise + residualise Y (V, T) float32 on the device.  Returns
         (Yt (Tpad, ncols_pad) time-major residual, scale (V,), features (V, 10) | None).
         ``rows`` (int32, optional) selects the voxel of each output column."""
@@ -285,8 +297,9 @@ class RemlGrid(Mapping):
             scale = torch.empty(V, dtype=torch.float32, device=self.device)
         feats = torch.empty(V, 10, dtype=torch.float32, device=self.device) if want_features else None
         lib = _lib.load()
+        flags = int(bool(normalize)) | (0 if residualize else 2)
         _lib.check(lib.ffs_reml_prep(self._handle, _lib.ptr(Y), Y.stride(0), V, _lib.ptr(rows), ncols, _lib.ptr(Yt),
-                                     Vp, _lib.ptr(scale), int(normalize), _lib.ptr(feats),
+                                     Vp, _lib.ptr(scale), flags, _lib.ptr(feats),
                                      _lib.stream_handle(self.device)), "prep")
         return Yt, scale, feats
 
@@ -431,6 +444,126 @@ def select_arma_params_hierarchical_from_surface(surface: torch.Tensor, param_li
     S = surface.to(device=grid.device, dtype=torch.float32).contiguous()
     best, score = grid.select(S, hierarchical=True)
     return grid.params.to(grid.device)[best.long()], score
+
+
+# ---------------------------------------------------------------------------
+# Public search variants (glm/arma.py:1106-1303, 1574-1883, 2577-2833)
+# ---------------------------------------------------------------------------
+
+
+def compute_reml_likelihood(X: torch.Tensor, Y: torch.Tensor, R: torch.Tensor, device=None) -> float:
+    """glm/arma.py:1106-1186: REML score of one dense covariance R (T x T):
+    2 sum log(diag chol R + 1e-10) + log det(X_w'X_w + 1e-6 I)
+    + (T - p) log(||Y_w - X_w beta||^2 + 1e-10), beta solved with the same
+    ridge; 1e10 when R is not positive definite.  Y: (T,) or (T, m)."""
+    dev = _lib.require_device(device if device is not None else (X.device if X.is_cuda else None))
+    f64 = dict(dtype=torch.float64, device=dev)
+    Xd = torch.as_tensor(X).to(**f64).contiguous()
+    Yd = torch.as_tensor(Y).to(**f64)
+    Yd = (Yd[:, None] if Yd.ndim == 1 else Yd).contiguous()
+    Rd = torch.as_tensor(R).to(**f64).contiguous()
+    T, p = Xd.shape
+    m = Yd.shape[1]
+    if Yd.shape[0] != T or tuple(Rd.shape) != (T, T):
+        raise ValueError(f"Timepoints mismatch: X has {T}, Y has {Yd.shape[0]}, R is {tuple(Rd.shape)}")
+    n = T * T + T * (p + m) + p * p + p * m
+    work = torch.empty(n, **f64)
+    out = torch.empty(1, **f64)
+    _lib.check(_lib.load().ffs_reml_dense_likelihood(_lib.ptr(Xd), _lib.ptr(Yd), m, _lib.ptr(Rd), T, p,
+                                                      _lib.ptr(work), n, _lib.ptr(out), _lib.stream_handle(dev)),
+               "compute_reml_likelihood")
+    return float(out.item())
+
+
+def _default_axes(a_grid, b_grid):
+    """The reference's grid defaulting (glm/arma.py:1253-1266)."""
+    da, db = get_default_arma_grids()
+    a = da if a_grid is None else _grid_tensor(a_grid, "a_grid")
+    b = db if b_grid is None else _grid_tensor(b_grid, "b_grid")
+    return a, b
+
+
+def reml_grid_search_batched(data, design, a_grid, b_grid, device=None, verbose=False, dtype=torch.float32,
+                             grid_chunk_size=1, voxel_batch_size=None, y_scale=None, run_starts=None, tau=None):
+    """glm/arma.py:2577-2833: exhaustive search over the valid pairs of the given
+    grids (no zero insertion) on the raw data (optionally / y_scale, no
+    residualisation) with the ridge score (X_w'X_w + 1e-6 I in the log
+    determinant and in beta); sequential strict '<' in pair order; voxels that
+    never improve keep (0, 0) and +inf.  Returns CPU (best_params, best_lik).
+    The chunking arguments only shape memory use in the reference and are
+    accepted for compatibility."""
+    dev = _lib.require_device(device)
+    Y = torch.as_tensor(data)
+    if y_scale is not None:
+        Y = Y / torch.as_tensor(y_scale).to(Y.dtype)[:, None]
+    X = torch.as_tensor(design)
+    V = Y.shape[0]
+    pairs = valid_grid_pairs(_grid_tensor(a_grid, "a_grid"), _grid_tensor(b_grid, "b_grid"))
+    best_params = torch.zeros(V, 2, dtype=dtype)
+    best_lik = torch.full((V,), float("inf"), dtype=dtype)
+    if pairs.shape[0] == 0:
+        return best_params, best_lik
+    grid = RemlGrid(X, None, None, dev, run_starts=run_starts, tau=tau, pairs=pairs, ridge=1e-6)
+    Yd = _to_device_f32(Y, dev)
+    Yt, _, _ = grid.prep(Yd, normalize=False, residualize=False)
+    S = grid.surface(Yt, V)
+    best, score = grid.select(S, "exhaustive")
+    score = score.cpu()
+    ok = torch.isfinite(score)
+    best_params[ok] = grid.params[best.cpu().long()[ok]].to(dtype)
+    best_lik[ok] = score[ok].to(dtype)
+    return best_params, best_lik
+
+
+def reml_grid_search(X, Y, a_grid=None, b_grid=None, device=None, run_starts=None, tau=None):
+    """glm/arma.py:1189-1303: single-voxel exhaustive search with the dense-R
+    score of compute_reml_likelihood (ridge 1e-6, raw data), a-outer / b-inner,
+    strict '<'; (0.5, 0.0, inf) when no pair is valid.  Returns (a, b, lik)."""
+    a_g, b_g = _default_axes(a_grid, b_grid)
+    Yv = torch.as_tensor(Y).reshape(1, -1)
+    params, lik = reml_grid_search_batched(Yv, X, a_g, b_g, device, run_starts=run_starts, tau=tau)
+    if not torch.isfinite(lik[0]):
+        return 0.5, 0.0, float("inf")
+    return float(params[0, 0]), float(params[0, 1]), float(lik[0])
+
+
+# glm/arma.py:1754-1765: pairs evaluated first by batch_reml_grid_search
+_PRIORITY_PARAMS = [(0.0, 0.3), (0.0, 0.2), (0.0, 0.4), (0.0, 0.1), (0.0, 0.5), (0.8, -0.6), (0.1, 0.3),
+                    (0.8, -0.5), (0.8, -0.4), (0.0, 0.0)]
+
+
+def batch_reml_grid_search(X, Y_batch, a_grid=None, b_grid=None, device=None, precomputed=None,
+                           dtype=torch.float32, enable_early_stopping=False, run_starts=None, tau=None):
+    """glm/arma.py:1574-1883: exhaustive search of Y_batch (T x V), residualised
+    on X (no normalisation), zero inserted into both grids, with the pairs of
+    the reference's priority list evaluated first (so exact ties go to the
+    first pair in that order; the list is matched by exact float equality as
+    in the reference); voxels with residual variance < 1e-6 get (0, 0) and
+    likelihood -1.  Returns (best_params (V, 2), best_lik (V,)) on the device."""
+    if enable_early_stopping:
+        raise NotImplementedError("chunk-level early stopping is not supported (it skips pairs)")
+    if dtype != torch.float32:
+        raise NotImplementedError("the engine searches in float32")
+    dev = _lib.require_device(device)
+    a_g, b_g = _default_axes(a_grid, b_grid)
+    grid = precomputed if isinstance(precomputed, RemlGrid) else RemlGrid(X, a_g, b_g, dev, run_starts=run_starts,
+                                                                          tau=tau)
+    Y = _to_device_f32(torch.as_tensor(Y_batch).T, dev)
+    V = Y.shape[0]
+    Yt, _, _ = grid.prep(Y, normalize=False)
+    S = grid.surface(Yt, V)
+    order = [k for k, pr in enumerate(grid.param_list) if pr in _PRIORITY_PARAMS]
+    order += [k for k in range(grid.K) if k not in set(order)]
+    perm = torch.tensor(order, dtype=torch.long, device=dev)
+    Sp = S[:, perm].contiguous()
+    j, score = grid.select(Sp, "exhaustive")
+    best = perm[j.long()]
+    params = grid.params.to(device=dev, dtype=torch.float32)[best]
+    var = Yt[:grid.T, :V].var(dim=0)
+    bg = var < 1e-6
+    params[bg] = 0.0
+    score = torch.where(bg, torch.full_like(score, -1.0), score)
+    return params, score
 
 
 # ---------------------------------------------------------------------------

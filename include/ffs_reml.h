@@ -63,6 +63,16 @@ int ffs_reml_plan_create(ffs_reml_plan** out, int32_t T, int32_t p, const double
 
 int ffs_reml_plan_destroy(ffs_reml_plan* plan);
 
+/* Search-only plan whose score uses A + ridge I (A = X'R^-1X) in the
+ * log-determinant and in the solve for beta, with the whitened residual of
+ * that beta: the score of compute_reml_likelihood / reml_grid_search_batched
+ * (glm/arma.py:1160-1176, 2724-2755; ridge = 1e-6 there).  Same arguments as
+ * ffs_reml_plan_create without the task subset.  ffs_reml_gls refuses it. */
+int ffs_reml_plan_create_ridge(ffs_reml_plan** out, int32_t T, int32_t p, const double* X, const double* pinvT,
+                               const int32_t* gap, int32_t K, const double* ab, const int32_t* white, int32_t n_a,
+                               int32_t n_b, const int32_t* lut, int32_t zero_a, int32_t zero_b, int32_t ltop,
+                               double ridge, void* stream);
+
 /* Host copies of logdet R and logdet X'R^-1X per pair (synchronous). */
 int ffs_reml_plan_logdets(const ffs_reml_plan* plan, double* logdet_R, double* logdet_XRX);
 
@@ -82,9 +92,10 @@ int ffs_reml_plan_yt_rows(const ffs_reml_plan* plan, int32_t* rows);
  * rows[c] (rows nullable = identity, then ncols = V); a -1 entry, and every
  * column from ncols up to the next multiple of the voxel padding unit, is zero,
  * as are rows T..Tpad-1.  ldYt >= ncols rounded up to the padding unit.
- * scale[v] = unbiased std clamped at 1e-8 (1 when normalize == 0), indexed by
- * voxel; feats (nullable, V x 10) receives lag-1..10 autocorrelations of the
- * residual.  Yt may be null (statistics / features only).
+ * normalize bit 0: divide by scale[v] = unbiased std clamped at 1e-8 (else
+ * scale = 1); bit 1: skip the residualisation (Yt holds y / scale).  scale is
+ * indexed by voxel; feats (nullable, V x 10) receives lag-1..10
+ * autocorrelations of the residual.  Yt may be null (statistics / features only).
  */
 int ffs_reml_prep(const ffs_reml_plan* plan, const float* Y, int64_t ldY, int64_t V, const int32_t* rows,
                   int64_t ncols, float* Yt, int64_t ldYt, float* scale, int32_t normalize, float* feats,
@@ -187,6 +198,15 @@ int ffs_reml_compact(const int32_t* flags, int64_t V, int32_t* idx, int32_t* cou
 /* best[idx[i]] = best_sub[i], score[idx[i]] = score_sub[i] for i < n. */
 int ffs_reml_scatter_best(const int32_t* idx, int64_t n, const int32_t* best_sub, const float* score_sub,
                           int32_t* best, float* score, void* stream);
+
+/* REML score of one dense covariance (compute_reml_likelihood,
+ * glm/arma.py:1106-1186): X (T x p), Y (T x m), R (T x T), row-major float64
+ * device arrays; work >= T*T + T*(p+m) + p*p + p*m doubles; *out (device) =
+ * 2 sum log(diag chol R + 1e-10) + log det(X_w'X_w + 1e-6 I)
+ * + (T - p) log(sum of squared whitened residuals + 1e-10), or 1e10 when R is
+ * not positive definite. */
+int ffs_reml_dense_likelihood(const double* X, const double* Y, int32_t m, const double* R, int32_t T, int32_t p,
+                              double* work, int64_t work_doubles, double* out, void* stream);
 
 #ifdef __cplusplus
 }

@@ -605,3 +605,52 @@ def partial_r2(t_full, r2, dof, task_indices=None, partial_mode="full", semi_mod
         else st
     return part, nu, semi, sn
 
+
+
+def reml_likelihood_dense(X, Y, R):
+    """glm/arma.py:1106-1186 ``compute_reml_likelihood`` restated in FP64:
+    L = chol(R); 2 sum log(diag L + 1e-10) + log det(X_w'X_w + 1e-6 I)
+    + (T - p) log(||Y_w - X_w beta||^2 + 1e-10), beta = (X_w'X_w + 1e-6 I)^-1 X_w'Y_w."""
+    X = torch.as_tensor(X, dtype=torch.float64)
+    Y = torch.as_tensor(Y, dtype=torch.float64)
+    Y = Y[:, None] if Y.ndim == 1 else Y
+    R = torch.as_tensor(R, dtype=torch.float64)
+    T, p = X.shape
+    L = torch.linalg.cholesky(R)
+    term1 = 2 * torch.sum(torch.log(torch.diag(L) + 1e-10))
+    Xw = torch.linalg.solve_triangular(L, X, upper=False)
+    Yw = torch.linalg.solve_triangular(L, Y, upper=False)
+    A = Xw.T @ Xw + 1e-6 * torch.eye(p, dtype=torch.float64)
+    sign, ld = torch.linalg.slogdet(A)
+    term2 = ld if sign > 0 else torch.tensor(1e10, dtype=torch.float64)
+    beta = torch.linalg.solve(A, Xw.T @ Yw)
+    rss = torch.sum((Yw - Xw @ beta) ** 2)
+    return float(term1 + term2 + (T - p) * torch.log(rss + 1e-10))
+
+
+def ridge_grid_search(X, Y, params, run_starts=None, tau=None):
+    """glm/arma.py:2577-2833 ``reml_grid_search_batched`` restated per voxel:
+    dense ARMA(1,1) covariance per pair (run blocks, tau gaps), the ridge score
+    of reml_likelihood_dense on the raw data, sequential strict '<' in pair
+    order.  Returns (pair index, score), index -1 / score inf when nothing
+    improves on +inf."""
+    Y = torch.as_tensor(Y, dtype=torch.float64)
+    V, T = Y.shape
+    best_k = torch.full((V,), -1, dtype=torch.long)
+    best_s = torch.full((V,), float("inf"), dtype=torch.float64)
+    for k, (a, b) in enumerate(torch.as_tensor(params, dtype=torch.float64).tolist()):
+        R = covariance(a, b, T, run_starts, tau, torch.float64)
+        for v in range(V):
+            s = reml_likelihood_dense(X, Y[v], R)
+            if s < best_s[v]:
+                best_s[v] = s
+                best_k[v] = k
+    return best_k, best_s
+
+
+def select_in_order(surface: torch.Tensor, order):
+    """Exhaustive first-minimum selection visiting the pairs in ``order``
+    (glm/arma.py:1754-1776 evaluation order of batch_reml_grid_search)."""
+    S = surface[:, list(order)]
+    j = torch.argmin(S, dim=1)  # first minimum in the visiting order
+    return torch.as_tensor(list(order))[j], S[torch.arange(S.shape[0]), j]

@@ -1,0 +1,111 @@
+"""GPU tests of the public search variants against the oracle restatements.
+
+Tolerances: compute_reml_likelihood rel 1e-9 (FP64 both sides); the grid
+searches compute their surface in float32: selected pair equal to the oracle's
+except near-ties (FP64 score gap <= 2e-3), scores |diff| <= 5e-4 + 2e-6 |score|.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import reml_oracle as O
+
+pytestmark = pytest.mark.gpu
+F64 = torch.float64
+
+
+@pytest.fixture(scope="module")
+def dev():
+    from fastfuncstuff_amd import _lib
+
+    _lib.load()
+    return torch.device("cuda", 0)
+
+
+def _voxels(rng, T, X, V, mean=3.0):
+    e = np.zeros((V, T))
+    a = rng.uniform(0.1, 0.8, V)
+    b = rng.uniform(-0.5, 0.3, V)
+    ep = rng.standard_normal(V)
+    for t in range(T):
+        eps = rng.standard_normal(V)
+        e[:, t] = (a * e[:, t - 1] if t else 0) + eps + b * ep
+        ep = eps
+    return (rng.standard_normal((V, X.shape[1])) @ X.T * 0.5 + mean + e).astype(np.float32)
+
+
+def _check_search(k, s, k_ref, S_ref):
+    V = S_ref.shape[0]
+    diff = k != k_ref
+    gap = (S_ref[torch.arange(V), k] - S_ref[torch.arange(V), k_ref]).abs()
+    assert bool((gap[diff] <= 2e-3).all()), f"{int(diff.sum())} mismatches beyond near-ties"
+    ref = S_ref[torch.arange(V), k]
+    assert bool(((s.double() - ref).abs() <= 5e-4 + 2e-6 * ref.abs()).all())
+
+
+def test_compute_reml_likelihood(dev):
+    from fastfuncstuff_amd.glm.arma import compute_reml_likelihood
+
+    print("[api] dense likelihood", flush=True)
+    rng = np.random.default_rng(1)
+    T, p = 90, 4
+    X = np.column_stack([np.ones(T), rng.standard_normal((T, p - 1))])
+    y = rng.standard_normal(T)
+    for R in (O.covariance(0.6, -0.3, T, [0, 40], None, F64), torch.eye(T, dtype=F64) * 2.0):
+        got = compute_reml_likelihood(torch.from_numpy(X), torch.from_numpy(y), R, device=dev)
+        want = O.reml_likelihood_dense(torch.from_numpy(X), torch.from_numpy(y), R)
+        assert abs(got - want) <= 1e-9 * abs(want)
+    Y2 = rng.standard_normal((T, 3))
+    R = O.covariance(0.3, 0.2, T, dtype=F64)
+    got = compute_reml_likelihood(torch.from_numpy(X), torch.from_numpy(Y2), R, device=dev)
+    assert abs(got - O.reml_likelihood_dense(torch.from_numpy(X), torch.from_numpy(Y2), R)) <= 1e-9 * abs(got)
+    bad = torch.diag(torch.cat([torch.ones(T - 1, dtype=F64), -torch.ones(1, dtype=F64)]))
+    assert compute_reml_likelihood(torch.from_numpy(X), torch.from_numpy(y), bad, device=dev) == 1e10
+
+
+def test_ridge_grid_searches(dev):
+    from fastfuncstuff_amd.glm.arma import reml_grid_search, reml_grid_search_batched, valid_grid_pairs
+
+    print("[api] ridge searches", flush=True)
+    rng = np.random.default_rng(2)
+    T, V = 60, 12
+    X = np.column_stack([np.ones(T), np.linspace(-1, 1, T), rng.standard_normal(T)])
+    Y = _voxels(rng, T, X, V)
+    ag, bg = torch.linspace(0.1, 0.8, 8), torch.linspace(-0.4, 0.3, 8)
+    params, lik = reml_grid_search_batched(torch.from_numpy(Y), torch.from_numpy(X), ag, bg, dev)
+    pairs = valid_grid_pairs(ag, bg)
+    k_ref, s_ref = O.ridge_grid_search(X, Y, pairs)
+    # oracle surface for the near-tie check
+    S_ref = torch.stack([torch.tensor([O.reml_likelihood_dense(X, Y[v], O.covariance(float(a), float(b), T,
+                                                                                      dtype=F64))
+                                       for a, b in pairs.double().tolist()]) for v in range(V)])
+    k = torch.tensor([next(i for i, pr in enumerate(pairs.tolist()) if pr == prm) for prm in params.tolist()])
+    _check_search(k, lik, k_ref, S_ref)
+    a, b, l1 = reml_grid_search(torch.from_numpy(X), torch.from_numpy(Y[0]), ag, bg, dev)
+    assert (a, b) == tuple(params[0].tolist()) and abs(l1 - float(lik[0])) <= 1e-6 * abs(l1)
+
+
+def test_batch_grid_search_priority_order_and_background(dev):
+    from fastfuncstuff_amd.glm.arma import RemlGrid, _PRIORITY_PARAMS, batch_reml_grid_search
+
+    print("[api] batch search", flush=True)
+    rng = np.random.default_rng(3)
+    T, V = 100, 200
+    X = np.column_stack([np.ones(T), rng.standard_normal((T, 2))])
+    Y = _voxels(rng, T, X, V)
+    Y[7] = 5.0  # constant series: background
+    ag, bg = torch.linspace(0, 0.8, 9), torch.linspace(-0.8, 0.8, 17)
+    params, lik = batch_reml_grid_search(torch.from_numpy(X), torch.from_numpy(Y).T, ag, bg, dev)
+    grid = RemlGrid(torch.from_numpy(X), ag, bg, dev)
+    g64 = O.precompute_grid(torch.from_numpy(X).double(), ag, bg, None, None, F64)
+    S_ref = O.score_surface(torch.from_numpy(X).double(), torch.from_numpy(Y).double(), g64)
+    order = [k for k, pr in enumerate(grid.param_list) if pr in _PRIORITY_PARAMS]
+    order += [k for k in range(grid.K) if k not in order]
+    k_ref, _ = O.select_in_order(S_ref.float(), order)
+    pl = grid.param_list
+    k = torch.tensor([pl.index(tuple(pr)) for pr in params.cpu().tolist()])
+    live = torch.ones(V, dtype=torch.bool)
+    live[7] = False
+    _check_search(k[live], lik.cpu()[live], k_ref[live], S_ref[live])
+    assert params[7].tolist() == [0.0, 0.0] and float(lik[7]) == -1.0

@@ -326,3 +326,25 @@ def test_partial_r2_is_squared_partial_correlation():
     expect = part[:, [1, 2]] / torch.clamp(1 - part[:, [0, 3]].sum(1, keepdim=True), min=0.01)
     np.testing.assert_allclose(part_t.numpy(), expect.numpy(), rtol=1e-12)
     np.testing.assert_allclose(nu_t.numpy(), part[:, [0, 3]].numpy(), rtol=1e-12)
+
+
+def test_dense_likelihood_identity_is_ridge_regression():
+    """Pins oracle.reml_likelihood_dense: with R = I it is
+    log det(X'X + 1e-6 I) + (T - p) log(||y - X b_ridge||^2 + 1e-10) (closed
+    form with numpy), and it is invariant to rescaling R by c only through
+    T log c in term 1 plus the whitening (checked: score(cR) - score(R) =
+    T log c - p log c + (T - p) log(1/c) with the ridge made negligible)."""
+    rng = np.random.default_rng(41)
+    T, p = 50, 3
+    X = np.column_stack([np.ones(T), rng.standard_normal((T, p - 1))])
+    y = rng.standard_normal(T)
+    A = X.T @ X + 1e-6 * np.eye(p)
+    b = np.linalg.solve(A, X.T @ y)
+    expect = np.linalg.slogdet(A)[1] + (T - p) * np.log(np.sum((y - X @ b) ** 2) + 1e-10)
+    got = O.reml_likelihood_dense(torch.from_numpy(X), torch.from_numpy(y), torch.eye(T, dtype=F64))
+    assert abs(got - expect) < 1e-10 * abs(expect) + 1e-6  # the +1e-10 inside log(diag) adds ~T*1e-10
+    R = O.covariance(0.5, -0.2, T, dtype=F64)
+    c = 4.0
+    s1 = O.reml_likelihood_dense(torch.from_numpy(X), torch.from_numpy(y), R)
+    s2 = O.reml_likelihood_dense(torch.from_numpy(X), torch.from_numpy(y), c * R)
+    assert abs((s2 - s1) - (T * np.log(c) - p * np.log(c) - (T - p) * np.log(c))) < 1e-5
