@@ -53,6 +53,9 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--projection", default="fp16x3", choices=["fp32", "fp16x3", "bf16x3"],
                     help="arithmetic of the full-grid projection GEMM")
+    ap.add_argument("--workload", default="metric", choices=["metric", "learned_eval"],
+                    help="metric: the headline line; learned_eval: learned path on a 100x100x50 volume with "
+                         "spatially smooth ARMA parameters and 2%% t(3) voxels (T=600), its own JSON line")
     return ap.parse_args()
 
 
@@ -80,12 +83,80 @@ def timed(fn, steps, warmup, device):
     return ms, per, timers[-1].counts(), out
 
 
+def learned_eval(args):
+    """Learned proposal trained on uniform-parameter Gaussian voxels of the p=12
+    design at T=600, evaluated on a 100x100x50 volume whose ARMA parameters are
+    spatially smooth (Gaussian random fields, FWHM 4 voxels) with 2% of voxels
+    driven by Student-t(3) innovations; agreement against the full-grid
+    exhaustive argmin of the same data."""
+    from fastfuncstuff_amd import _lib, synth
+    from fastfuncstuff_amd.glm.arma import RemlGrid
+    from fastfuncstuff_amd.learned import (LocalSearcher, ProposalMLP, calibrate_conf_min, features,
+                                           full_grid_labels)
+
+    dev = _lib.require_device(None)
+    T_acq = 600
+    ev = synth.learned_eval_config(T=T_acq, device=dev)
+    grid = RemlGrid(torch.from_numpy(ev.X), torch.from_numpy(ev.a_grid), torch.from_numpy(ev.b_grid), dev,
+                    run_starts=ev.run_starts, tau=torch.from_numpy(ev.tau))
+    grid.set_projection(args.projection)
+    V = ev.Y.shape[0]
+    t0 = time.time()
+    tr = synth.gpu_config(V=args.train_voxels, T=T_acq, seed=5, noise_seed=101, device=dev)
+    val = synth.gpu_config(V=200_000, T=T_acq, seed=5, noise_seed=202, device=dev)
+    labels, _ = full_grid_labels(grid, tr.Y)
+    feats = features(grid, tr.Y)
+    mlp = ProposalMLP(dev, seed=0)
+    log = mlp.train(feats, labels, epochs=args.epochs, batch=4096, lr=3e-3, lr_final=1e-4, log_every=200)
+    _, best_val = full_grid_labels(grid, val.Y)
+    conf_min = calibrate_conf_min(mlp, grid, val.Y, best_val, target=args.target_agreement)
+    del tr, labels, feats, val
+    t_train = time.time() - t0
+    _, best_full = full_grid_labels(grid, ev.Y)
+    ls = LocalSearcher(grid, mlp, V)
+
+    def step(tm):
+        res = ls.run(ev.Y, timer=tm)
+        if tm is not None:
+            tm.mark("gls")
+        grid.gls(ev.Y, res.scale, res.best)
+        return res
+
+    lp_ms, lp_k, _, res = timed(step, args.steps, args.warmup, dev)
+    S = torch.empty(V, grid.K, dtype=torch.float32, device=dev)
+
+    def full(tm):
+        Yt, scale, _ = grid.prep(ev.Y)
+        grid.surface(Yt, V, out=S)
+        best, _ = grid.select(S, "exhaustive")
+        grid.gls(ev.Y, scale, best)
+        return best
+
+    fg_ms, _, _, _ = timed(full, args.steps, args.warmup, dev)
+    line = {
+        "metric": "REML voxels/s, learned proposal on spatially smooth ARMA parameters (T=600, V=500k)",
+        "value": V / (lp_ms / 1e3), "unit": "voxels/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": lp_ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
+        "data": "synthetic",
+        "config": {"workload": f"grf_fwhm4_100x100x50_T{T_acq}_p12_t3frac0.02", "V": V, "T_retained": grid.T,
+                   "p": grid.p, "K": grid.K, "projection": args.projection, "train_voxels": args.train_voxels},
+        "full_grid_exhaustive_ms_per_step": fg_ms, "speedup_vs_full_grid": fg_ms / lp_ms,
+        "argmin_match_vs_full_grid_exhaustive": float((res.best == best_full).double().mean()),
+        "fallback_fraction": res.n_fallback / V, "conf_min": conf_min, "stages_ms": lp_k,
+        "train_s": t_train, "train_steps": log.steps,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU; multi-GPU runs are not part of this build "
               "(the path shards by voxel with no collective in the hot loop).", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "learned_eval":
+        learned_eval(args)
+        return
     from fastfuncstuff_amd import _lib, synth
     from fastfuncstuff_amd.glm.arma import RemlGrid
     from fastfuncstuff_amd.learned import (LocalSearcher, ProposalMLP, calibrate_conf_min, features,

@@ -9,6 +9,7 @@ stationary ARMA(1,1) with unit marginal variance.
 
 from __future__ import annotations
 
+import math
 from dataclasses import dataclass
 
 import numpy as np
@@ -147,15 +148,9 @@ def plain_config(V: int = 20_000, T: int = 200, tr: float = 2.0, seed: int = 1, 
     return Workload("plain_T200_p6", Y, X, tr, None, None, a, b, beta, ag, bg)
 
 
-def gpu_config(V: int = 200_000, T: int = 400, tr: float = 2.0, seed: int = 2, censor_frac: float = 0.05,
-               noise_seed: int | None = None, chunk: int = 250_000, device=None) -> Workload:
-    """p=12: 4 HRF-convolved boxcar task regressors (20 random 10 s blocks each),
-    6 AR(1) random-walk motion-like columns, constant + linear; 5% of the TRs
-    censored uniformly at random (the retained series keeps its true time
-    index); ARMA(1,1) noise as in the plain config.  ``seed`` fixes the design
-    and censoring; ``noise_seed`` (default: ``seed``) the voxels.  With
-    ``device`` the voxels are drawn by torch on that device (Y stays there)."""
-    rng = np.random.default_rng(seed)
+def _design_p12(rng, T: int, tr: float, censor_frac: float):
+    """The p=12 design of gpu_config over all T acquired TRs and the retained
+    TR indices (draws from ``rng`` in a fixed order)."""
     hrf = double_gamma_hrf(tr)
     cols = []
     for _ in range(4):
@@ -173,6 +168,19 @@ def gpu_config(V: int = 200_000, T: int = 400, tr: float = 2.0, seed: int = 2, c
     n_cens = int(round(censor_frac * T))
     censored = np.sort(rng.choice(T, n_cens, replace=False)) if n_cens else np.zeros(0, int)
     keep = np.setdiff1d(np.arange(T), censored)
+    return Xf, keep
+
+
+def gpu_config(V: int = 200_000, T: int = 400, tr: float = 2.0, seed: int = 2, censor_frac: float = 0.05,
+               noise_seed: int | None = None, chunk: int = 250_000, device=None) -> Workload:
+    """p=12: 4 HRF-convolved boxcar task regressors (20 random 10 s blocks each),
+    6 AR(1) random-walk motion-like columns, constant + linear; 5% of the TRs
+    censored uniformly at random (the retained series keeps its true time
+    index); ARMA(1,1) noise as in the plain config.  ``seed`` fixes the design
+    and censoring; ``noise_seed`` (default: ``seed``) the voxels.  With
+    ``device`` the voxels are drawn by torch on that device (Y stays there)."""
+    rng = np.random.default_rng(seed)
+    Xf, keep = _design_p12(rng, T, tr, censor_frac)
     Xk = Xf[keep]
     ag, bg = config_grid()
     if device is not None:
@@ -206,3 +214,78 @@ def gpu_config(V: int = 200_000, T: int = 400, tr: float = 2.0, seed: int = 2, c
         noise = arma11_noise(rng, a[c0:c1], b[c0:c1], T)
         Y[c0:c1] = beta[c0:c1] @ Xk.T + noise[:, keep]
     return Workload(f"censored_T{T}_p12", Y, Xk, tr, [0], keep.astype(np.int64), a, b, beta, ag, bg)
+
+
+def _gaussian_smooth3d(field, fwhm: float):
+    """Separable Gaussian smoothing of a (nx, ny, nz) torch tensor (FWHM in voxels)."""
+    import torch
+    import torch.nn.functional as F
+
+    sigma = fwhm / 2.3548200450309493
+    r = int(math.ceil(3 * sigma))
+    x = torch.arange(-r, r + 1, dtype=field.dtype, device=field.device)
+    k = torch.exp(-0.5 * (x / sigma) ** 2)
+    k = (k / k.sum()).view(1, 1, -1)
+    f = field
+    for dim in range(3):
+        f = f.movedim(dim, -1)
+        sh = f.shape
+        f = F.conv1d(f.reshape(-1, 1, sh[-1]), k, padding=r).reshape(sh).movedim(-1, dim)
+    return f
+
+
+def learned_eval_config(shape=(100, 100, 50), T: int = 600, tr: float = 2.0, seed: int = 5, heavy_frac: float = 0.02,
+                        fwhm: float = 4.0, censor_frac: float = 0.05, device=None) -> Workload:
+    """Evaluation volume for the learned proposal: the p=12 design of
+    gpu_config at T acquired TRs (5% censored), ARMA parameters that are
+    spatially smooth over the voxel grid (two Gaussian random fields, FWHM in
+    voxels, mapped through the normal CDF to a in [0.1, 0.8] and b in
+    [-0.5, 0.3]), and a fraction of voxels with Student-t(3) innovations
+    (scaled to unit variance) instead of Gaussian ones.  Voxels in C order of
+    ``shape``; Y is drawn on ``device``."""
+    import torch
+
+    Xf, keep_np = _design_p12(np.random.default_rng(seed), T, tr, censor_frac)
+    ag, bg = config_grid()
+    V = int(np.prod(shape))
+    g = torch.Generator(device=device)
+    g.manual_seed(seed + 1000)
+    u = []
+    for _ in range(2):
+        f = _gaussian_smooth3d(torch.randn(*shape, generator=g, device=device), fwhm)
+        f = (f - f.mean()) / f.std()
+        u.append((0.5 * (1 + torch.erf(f / math.sqrt(2.0)))).reshape(-1))
+    a = 0.1 + 0.7 * u[0]
+    b = -0.5 + 0.8 * u[1]
+    heavy = torch.rand(V, generator=g, device=device) < heavy_frac
+    Xf_full = torch.from_numpy(Xf).to(device=device, dtype=torch.float32)
+    keep = torch.from_numpy(keep_np).to(device)
+    Y = torch.empty(V, keep.numel(), dtype=torch.float32, device=device)
+    chunk = 125_000
+    for c0 in range(0, V, chunk):
+        c1 = min(V, c0 + chunk)
+        n = c1 - c0
+        aa, bb, hh = a[c0:c1], b[c0:c1], heavy[c0:c1]
+        sig2 = (1 - aa * aa) / (1 + 2 * aa * bb + bb * bb)
+        sd = torch.sqrt(sig2)
+
+        def innov(shape_):
+            z = torch.randn(*shape_, generator=g, device=device)
+            chi = (torch.randn(3, *shape_, generator=g, device=device) ** 2).sum(0)
+            t3 = z / torch.sqrt(chi / 3.0) / math.sqrt(3.0)
+            return torch.where(hh if len(shape_) == 1 else hh[None, :], t3, z)
+
+        noise = torch.empty(n, T, dtype=torch.float32, device=device)
+        e_prev = innov((n,)) * sd
+        x = e_prev + torch.sqrt(torch.clamp(1.0 - sig2, min=0.0)) * innov((n,))
+        noise[:, 0] = x
+        zz = innov((T - 1, n)) * sd
+        for t in range(1, T):
+            e = zz[t - 1]
+            x = aa * x + e + bb * e_prev
+            noise[:, t] = x
+            e_prev = e
+        beta = torch.randn(n, Xf_full.shape[1], generator=g, device=device)
+        Y[c0:c1] = (beta @ Xf_full.T + noise)[:, keep]
+    return Workload(f"grf_T{T}_p12", Y, Xf[keep_np], tr, [0], keep_np.astype(np.int64), a, b, None, ag, bg)
+
