@@ -291,6 +291,15 @@ def main():
             roof["valu"] = {"achieved_Tops": ops / (avg_ms / 1e3) / 1e12, "peak_Tops": PEAK_VALU_NONPACKED / 1e12,
                             "frac": ops / (avg_ms / 1e3) / PEAK_VALU_NONPACKED}
     roof["per_step_ms"] = {n: round(v, 4) for n, v in totals.items()}
+    # HBM bytes per launch of that kernel, measured with rocprofv3 FETCH_SIZE /
+    # WRITE_SIZE on this workload (calibrated against a plain copy) and kept
+    # with the profiles; used when the run is that workload
+    tfile = ROOT / "profiles" / "traffic_bench_T400_V200k.json"
+    if tfile.exists() and V == 200_000 and args.T == 400 and args.projection == "fp16x3":
+        tk = json.loads(tfile.read_text())["kernels"].get(dom)
+        if tk is not None:
+            roof["traffic"] = tk["bytes"]
+            roof["traffic_source"] = f"profiles/{tfile.name} ({tk['launch']})"
     # fp32: 2*V*T*K*p flops on fp32 MFMA; split modes: three 16-bit products of that size.
     proj_flops = (3.0 if split else 1.0) * 2.0 * V * T * K * p
     proj_peak = PEAK_F16_MFMA if split else PEAK_FP32_MFMA

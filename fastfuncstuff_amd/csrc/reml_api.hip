@@ -456,8 +456,12 @@ extern "C" int ffs_reml_energy(const ffs_reml_plan* P, const float* Yt, int64_t 
     if (ldYt < Vpad) return fail(-3, "energy: ldYt must be >= the padded voxel count");
     if (ldE < P->d.K) return fail(-3, "energy: ldE must be >= K");
     hipStream_t s = (hipStream_t)stream;
-    const dim3 grid((unsigned)(Vpad / kEnergyVox), (P->d.n_chunks + kEnergyWaves - 1) / kEnergyWaves);
-    hipLaunchKernelGGL(k_energy, grid, dim3(kEnergyVox * kEnergyWaves), 0, s, P->d, Yt, ldYt, (int)V, E, ldE);
+    const int64_t n_xt = (V + kEnergyVox - 1) / kEnergyVox;
+    const int64_t n_yg = (P->d.n_chunks + kEnergyWaves - 1) / kEnergyWaves;
+    const int64_t nblk = 8 * ((n_xt + 7) / 8) * n_yg;  // see k_energy's block map
+    if (nblk > 0x7fffffff) return fail(-3, "energy: grid too large; split the voxel batch");
+    hipLaunchKernelGGL(k_energy, dim3((unsigned)nblk), dim3(kEnergyVox * kEnergyWaves), 0, s, P->d, Yt, ldYt, (int)V,
+                       E, ldE);
     HIP_TRY(hipGetLastError());
     return 0;
 }

@@ -468,9 +468,18 @@ __global__ void __launch_bounds__(kEnergyVox* kEnergyWaves) k_energy(PlanDev P, 
                                                                      int64_t ldE) {
     const int lane = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int chunk = blockIdx.y * kEnergyWaves + w;
+    // (voxel tile, chunk group) of this block: the chunk groups of one voxel
+    // tile run together on one XCD so its residual columns are fetched once
+    // into that XCD's L2 (same map as the projection, chunk group fastest)
+    const int n_xt = (V + kEnergyVox - 1) / kEnergyVox;
+    const int n_yg = (P.n_chunks + kEnergyWaves - 1) / kEnergyWaves;
+    const int bid = blockIdx.x, xcd = bid & 7, local = bid >> 3;
+    const int per = (n_xt + 7) >> 3;  // voxel tiles per XCD
+    const int xt = xcd * per + local / n_yg;
+    if (local / n_yg >= per || xt >= n_xt) return;
+    const int chunk = (local % n_yg) * kEnergyWaves + w;
     if (chunk >= P.n_chunks) return;
-    const int v = blockIdx.x * kEnergyVox + lane;
+    const int v = xt * kEnergyVox + lane;
     const int T = P.T;
     const float* __restrict__ y = Yt + v;
 
