@@ -112,7 +112,8 @@ int ffs_reml_project(const ffs_reml_plan* plan, const float* Yt, int64_t ldYt, i
 
 /* Precision of the full-grid projection (ii): 0 = fp32 MFMA (default),
  * 1 = split bf16 (hi + lo, three products, fp32 accumulation),
- * 2 = split fp16 (same, fp16 halves).  The local window search always uses fp32. */
+ * 2 = split fp16 (same, fp16 halves).  The local window search uses the same
+ * mode, so its scores stay equal to the full-grid surface. */
 int ffs_reml_plan_set_projection(ffs_reml_plan* plan, int32_t mode);
 
 /* (i)+(ii) over n_chunks voxel chunks (1..16), the energy of one chunk
@@ -162,7 +163,7 @@ int ffs_reml_ljung_box(const double* resid_w, int64_t ldR, int64_t V, int32_t n,
  * (a_hat, b_hat, log-confidence) from the proposal network.  Each voxel is
  * scored on the 5 x 5 lattice window around its snapped proposal (kept inside
  * the lattice) with the same arithmetic as the full grid, so the scores equal
- * the full-grid surface entries bitwise (fp32 projection mode).  Outputs per
+ * the full-grid surface entries bitwise (in every projection mode).  Outputs per
  * voxel: best pair (-1 if no valid pair), its score, need_full = 1 when the
  * optimum is on a window edge that is not a lattice edge, 2 when the confidence
  * is below conf_min or the proposal is not finite (no window is searched; best
