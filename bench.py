@@ -53,6 +53,10 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--projection", default="fp16x3", choices=["fp32", "fp16x3", "bf16x3"],
                     help="arithmetic of the full-grid projection GEMM")
+    ap.add_argument("--band-width", type=float, default=0.14,
+                    help="half-width of the lambda bands of the learned search (0: no band search)")
+    ap.add_argument("--band-bin", type=float, default=0.02, help="spacing of the lambda band centres")
+    ap.add_argument("--walk", type=int, default=1, help="window steps after the first window")
     ap.add_argument("--workload", default="metric", choices=["metric", "learned_eval"],
                     help="metric: the headline line; learned_eval: learned path on a 100x100x50 volume with "
                          "spatially smooth ARMA parameters and 2%% t(3) voxels (T=600), its own JSON line")
@@ -100,6 +104,8 @@ def learned_eval(args):
     grid = RemlGrid(torch.from_numpy(ev.X), torch.from_numpy(ev.a_grid), torch.from_numpy(ev.b_grid), dev,
                     run_starts=ev.run_starts, tau=torch.from_numpy(ev.tau))
     grid.set_projection(args.projection)
+    if args.projection != "fp32" and args.band_width > 0:
+        grid.set_bands(args.band_width, args.band_bin)
     V = ev.Y.shape[0]
     t0 = time.time()
     tr = synth.gpu_config(V=args.train_voxels, T=T_acq, seed=5, noise_seed=101, device=dev)
@@ -116,7 +122,7 @@ def learned_eval(args):
     ls = LocalSearcher(grid, mlp, V)
 
     def step(tm):
-        res = ls.run(ev.Y, timer=tm)
+        res = ls.run(ev.Y, timer=tm, walk=args.walk)
         if tm is not None:
             tm.mark("gls")
         grid.gls(ev.Y, res.scale, res.best)
@@ -178,6 +184,8 @@ def main():
     torch.cuda.synchronize()
     t_plan = time.time() - t0
     grid.set_projection(args.projection)
+    if args.projection != "fp32" and args.band_width > 0:
+        grid.set_bands(args.band_width, args.band_bin)
     K = grid.K
     Y = torch.from_numpy(wl.Y).to(dev)
     st = _lib.stream_handle(dev)
@@ -225,7 +233,7 @@ def main():
     ls = LocalSearcher(grid, mlp, V)
 
     def learned_step(tm):
-        res = ls.run(Y, timer=tm)
+        res = ls.run(Y, timer=tm, walk=args.walk)
         if tm is not None:
             tm.mark("gls")
         grid.gls(Y, res.scale, res.best)
@@ -254,7 +262,7 @@ def main():
     split = args.projection != "fp32"
     n_fb = res.n_fallback
     wstats: dict = {}
-    ls.run(Y, stats=wstats)  # voxel counts of the window passes (same data, untimed)
+    ls.run(Y, stats=wstats, walk=args.walk)  # voxel counts of the window passes (same data, untimed)
     n_win = [wstats["window"]] + list(wstats.get("walk", []))
     mul = 3.0 if split else 1.0
     kerns = {
@@ -270,6 +278,13 @@ def main():
              f"{'3*' if split else ''}2*T*K*p*n_fb"),
         "k_gls_mfma": (("gls",), 1, "hbm", 4.0 * V * T + 4.0 * V * (2 * p + 3), "4*V*T + 4*V*(2p+3)"),
     }
+    n_bv, n_bs = wstats.get("band", 0), wstats.get("band_slots", 0)
+    if n_bv:
+        kerns["k_band_energy"] = (("band_energy",), 1, "hbm", 4.0 * grid.Tpad * n_bv + 4.0 * n_bs,
+                                  f"4*Tpad*n_band (residual read) + 4*sum of band sizes (energy write); "
+                                  f"{n_bv} voxels, {n_bs} pairs")
+        kerns["k_band_project16"] = (("band_project",), 1, "mfma", mul * 2.0 * T * p * n_bs,
+                                     f"3*2*T*p*sum of band sizes ({n_bs} pairs)")
     totals = {n: sum(lp_k.get(st_, 0.0) for st_ in v[0]) for n, v in kerns.items()}
     dom = max(totals, key=totals.get)
     stages, launches, bound, amount, formula = kerns[dom]
@@ -284,10 +299,11 @@ def main():
         roof = {"kernel": dom, "bound": "hbm", "achieved": ach / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
                 "frac": ach / PEAK_HBM, "traffic": None, "avg_launch_ms": avg_ms,
                 "algorithmic": f"{formula} = {amount:.4g} B per launch"}
-        if dom == "k_energy":
+        if dom in ("k_energy", "k_band_energy"):
             # the recurrence is VALU-bound: 3 fp32 ops per (voxel, pair, step) + the
-            # shared AR term; non-packed VALU issue limit of the chip
-            ops = 3.125 * T * K * n_fb
+            # AR term (shared by 8 pairs in k_energy, per pair in the band kernel);
+            # non-packed VALU issue limit of the chip
+            ops = 3.125 * T * K * n_fb if dom == "k_energy" else 4.0 * T * n_bs
             roof["valu"] = {"achieved_Tops": ops / (avg_ms / 1e3) / 1e12, "peak_Tops": PEAK_VALU_NONPACKED / 1e12,
                             "frac": ops / (avg_ms / 1e3) / PEAK_VALU_NONPACKED}
     roof["per_step_ms"] = {n: round(v, 4) for n, v in totals.items()}
@@ -313,7 +329,8 @@ def main():
         "warmup": args.warmup, "ms_per_step": lp_ms, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
         "config": {"workload": f"censored_T{args.T}_V{V}_p{p}_grid33x33", "V": V, "T_acquired": args.T,
-                   "T_retained": T, "p": p, "K": K, "path": "learned_proposal_5x5_with_full_grid_fallback",
+                   "T_retained": T, "p": p, "K": K, "path": ("learned_proposal_5x5_window_then_lambda_band_then_full_grid" if grid.bands
+                            else "learned_proposal_5x5_with_full_grid_fallback"),
                    "projection_full_grid": {"fp32": "fp32 MFMA 32x32x2", "fp16x3": "split fp16 hi+lo x3 MFMA 32x32x16",
                                             "bf16x3": "split bf16 hi+lo x3 MFMA 32x32x16"}[args.projection],
                    "projection_window": "same arithmetic as the full grid (bitwise-equal scores)", "gls": "fp64", "train_voxels": args.train_voxels},
@@ -324,6 +341,9 @@ def main():
                     "argmin_match_vs_full_grid_hierarchical_white_merged": agree_hier_c,
                     "full_grid_exhaustive_vs_hierarchical_white_merged": agree_exh_hier_c,
                     "speedup_vs_full_grid": fg_ms / lp_ms, "train_s": t_train, "train_steps": log.steps,
+                    "walk": args.walk, "window_voxels_per_pass": n_win,
+                    "band": ({"half_width": grid.bands[0], "bin": grid.bands[1], "voxels": n_bv,
+                              "pairs_per_voxel": n_bs / max(1, n_bv)} if grid.bands else None),
                     "train_loss_first_last": [log.losses[0][1], log.losses[-1][1]] if log.losses else None},
         "plan_build_s": t_plan, "datagen_s": t_gen,
         "roofline": roof,

@@ -48,6 +48,13 @@ SIGNATURES: dict[str, list] = {
     "ffs_reml_local_project": [_vp, _i64, _vp, _vp, _vp, _vp, _i64, _vp],
     "ffs_reml_local_search": [_vp, _vp, _i64, _i64, _vp, ctypes.c_float, _vp, _vp, _vp, _vp, _vp, _i64, _vp],
     "ffs_reml_compact": [_vp, _i64, _vp, _vp, _vp],
+    "ffs_reml_plan_set_bands": [_vp, ctypes.c_double, ctypes.c_double, _vp],
+    "ffs_reml_plan_band_info": [_vp, _vp, _vp, _vp, _vp, _vp],
+    "ffs_reml_band_workspace": [_vp, _i64, _vp],
+    "ffs_reml_band_prepare": [_vp, _vp, _i64, _i64, _vp, _vp, _vp, _vp, _i64, _vp],
+    "ffs_reml_band_energy": [_vp, _i64, _vp, _i64, _vp],
+    "ffs_reml_band_project": [_vp, _i64, _vp, _vp, _vp, _vp, _i64, _vp],
+    "ffs_reml_band_search": [_vp, _vp, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _vp],
     "ffs_reml_scatter_best": [_vp, _i64, _vp, _vp, _vp, _vp, _vp],
     "ffs_mlp_param_count": [],
     "ffs_mlp_workspace_floats": [_i64],

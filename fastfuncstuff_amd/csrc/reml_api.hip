@@ -2,6 +2,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
+#include <numeric>
 #include <cstdio>
 #include <cstring>
 #include <memory>
@@ -17,6 +19,7 @@
 #include "reml_sort_gls.hip"
 #include "reml_prep_mfma.hip"
 #include "reml_proj16.hip"
+#include "reml_band.hip"
 #include "reml_dense.hip"
 
 using namespace remlk;
@@ -50,6 +53,9 @@ struct ffs_reml_plan {
     // Yt buffer whose rows Tpad..2 Tpad-1 the last ffs_reml_prep filled with
     // split halves (projection reads them only for this buffer)
     mutable const float* split_yt = nullptr;
+    // host copies of the pair list and lattice (band tables are built from them)
+    std::vector<double> h_ab;
+    std::vector<int32_t> h_lut;
     template <class Tp>
     int alloc(Tp** out, size_t n) {
         void* p = nullptr;
@@ -136,6 +142,8 @@ static int plan_create_impl(ffs_reml_plan** out, int32_t T, int32_t p, const dou
     std::unique_ptr<ffs_reml_plan> guard(P);
     HIP_TRY(hipGetDevice(&P->device));
     PlanDev& d = P->d;
+    P->h_ab.assign(ab, ab + (size_t)2 * K);
+    P->h_lut.assign(lut, lut + (size_t)n_a * n_b);
     d.T = T;
     d.Tpad = (T + kProjBK - 1) / kProjBK * kProjBK;
     d.p = p;
@@ -383,7 +391,8 @@ static void launch_prep(const ffs_reml_plan* P, const float* Y, int64_t ldY, int
 
 static int prep_impl(const ffs_reml_plan* P, const float* Y, int64_t ldY, int64_t V, const int32_t* rows,
                      int64_t ncols, float* Yt, int64_t ldYt, float* scale, int normalize, float* feats,
-                     hipStream_t s, const int32_t* ncols_live = nullptr, bool write_split = true) {
+                     hipStream_t s, const int32_t* ncols_live = nullptr, bool write_split = true,
+                     bool record_split = true) {
     const int64_t ncp = vpad_of(ncols);
     if (Yt && ldYt < ncp) return fail(-3, "prep: ldYt must be >= the padded column count");
     if (ldY < P->d.T) return fail(-3, "prep: ldY must be >= T");
@@ -399,7 +408,7 @@ static int prep_impl(const ffs_reml_plan* P, const float* Y, int64_t ldY, int64_
             Ysh = reinterpret_cast<uint16_t*>(Yt + (size_t)P->d.Tpad * ldYt);
             Ysl = Ysh + (size_t)P->d.Tpad * ldYt;
         }
-        if (write_split) P->split_yt = Ysh ? Yt : nullptr;
+        if (write_split && record_split) P->split_yt = Ysh ? Yt : nullptr;
         const int sbf = P->d.proj_mode == 1;
         const dim3 grid((unsigned)(ncp / 64)), block(64);
         if (P->d.PG == 16) {
@@ -763,8 +772,8 @@ static int local_group_and_prep(const ffs_reml_plan* P, const float* Y, int64_t 
     // voxels grouped by window centre, each group padded to whole tiles
     if (int rc = group_by_key(q.center, V, (int)L.ng, kLocalTile, q.cnt, q.fill, q.cursor, q.rows, L.max_cols, s))
         return rc;
-    hipLaunchKernelGGL(k_local_tiles, dim3((unsigned)((L.max_tiles + 255) / 256)), dim3(256), 0, s, q.fill, (int)L.ng,
-                       (int)L.max_tiles, q.tg, q.tc, q.nt);
+    hipLaunchKernelGGL(k_group_tiles, dim3((unsigned)((L.max_tiles + 255) / 256)), dim3(256), 0, s, q.fill, (int)L.ng,
+                       (int)L.max_tiles, kLocalTile, q.tg, q.tc, q.nt);
     HIP_TRY(hipGetLastError());
     // columns past the last group's run (off[ng]) are skipped
     return prep_impl(P, Y, ldY, V, q.rows, L.max_cols, q.Ytg, L.max_cols, scale, 1, nullptr, s, q.fill + L.ng,
@@ -854,6 +863,250 @@ extern "C" int ffs_reml_scatter_best(const int32_t* idx, int64_t n, const int32_
                        (int)n, best_sub, score_sub, best, score);
     HIP_TRY(hipGetLastError());
     return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Band search (lambda bands)
+// ---------------------------------------------------------------------------
+extern "C" int ffs_reml_plan_set_bands(ffs_reml_plan* P, double half_width, double bin_width, void* stream) {
+    if (!P) return fail(-3, "plan_set_bands: null plan");
+    PlanDev& d = P->d;
+    if (!(half_width > 0.0) || !(bin_width > 0.0)) {
+        d.n_bands = 0;
+        return 0;
+    }
+    if (!d.PG) return fail(-4, "plan_set_bands: the band search needs p <= 32");
+    if (bin_width < 1e-3 || bin_width > 2.0) return fail(-3, "plan_set_bands: bin_width must be in [0.001, 2]");
+    const int K = d.K, n_a = d.n_a, n_b = d.n_b;
+    const int nbins = (int)std::floor(2.0 / bin_width + 0.5) + 1;
+    bin_width = (double)(float)bin_width;  // the centres the kernels use (fp32 step)
+    std::vector<double> lam(K);
+    for (int k = 0; k < K; ++k) {
+        const double a = P->h_ab[2 * k], b = P->h_ab[2 * k + 1];
+        const double den = 1.0 + 2.0 * a * b + b * b;
+        lam[k] = den > 0.0 ? (1.0 + a * b) * (a + b) / den : 0.0;
+    }
+    std::vector<int> ai(K, -1), bi(K, -1);
+    for (int x = 0; x < n_a; ++x)
+        for (int y = 0; y < n_b; ++y) {
+            const int q = P->h_lut[(size_t)x * n_b + y];
+            if (q >= 0 && q < K) {
+                ai[q] = x;
+                bi[q] = y;
+            }
+        }
+    std::vector<std::vector<int32_t>> bands(nbins);
+    size_t maxn = 0;
+    for (int i = 0; i < nbins; ++i) {
+        const double c = -1.0 + i * bin_width;
+        for (int k = 0; k < K; ++k)
+            if (std::fabs(lam[k] - c) <= half_width) bands[i].push_back(k);
+        maxn = std::max(maxn, bands[i].size());
+    }
+    if (maxn == 0) return fail(-3, "plan_set_bands: no pair lies in any band");
+    const int unit = std::lcm(kEnergyNB, d.npt);
+    const int SB = (int)((maxn + unit - 1) / unit * unit);
+    std::vector<int32_t> pairs((size_t)nbins * SB, -1), edge((size_t)nbins * SB, 0), cnt(nbins, 0);
+    for (int i = 0; i < nbins; ++i) {
+        const double c = -1.0 + i * bin_width;
+        cnt[i] = (int32_t)bands[i].size();
+        for (size_t sl = 0; sl < bands[i].size(); ++sl) {
+            const int k = bands[i][sl];
+            pairs[(size_t)i * SB + sl] = k;
+            int e = 0;
+            if (ai[k] >= 0) {
+                const int nx[4] = {ai[k] - 1, ai[k] + 1, ai[k], ai[k]}, ny[4] = {bi[k], bi[k], bi[k] - 1, bi[k] + 1};
+                for (int q = 0; q < 4; ++q) {
+                    if (nx[q] < 0 || nx[q] >= n_a || ny[q] < 0 || ny[q] >= n_b) continue;
+                    const int kn = P->h_lut[(size_t)nx[q] * n_b + ny[q]];
+                    if (kn >= 0 && kn < K && !(std::fabs(lam[kn] - c) <= half_width)) e = 1;
+                }
+            }
+            edge[(size_t)i * SB + sl] = e;
+        }
+    }
+    hipStream_t s = (hipStream_t)stream;
+    int rc = 0;
+    int32_t *dpairs, *dedge, *dcnt;
+    float *coef, *s0;
+    const size_t nchunks = (size_t)nbins * (SB / kEnergyNB);
+    if ((rc = upload(P, &dpairs, pairs.data(), pairs.size(), s)) || (rc = upload(P, &dedge, edge.data(), edge.size(), s)) ||
+        (rc = upload(P, &dcnt, cnt.data(), cnt.size(), s)) ||
+        (rc = P->alloc(&coef, nchunks * d.T * 3 * kEnergyNB)) || (rc = P->alloc(&s0, (size_t)nbins * SB)))
+        return rc;
+    d.n_bands = nbins;
+    d.band_sb = SB;
+    d.band_lo = -1.f;
+    d.band_h = (float)bin_width;
+    d.band_n = dcnt;
+    d.band_pairs = dpairs;
+    d.band_edge = dedge;
+    d.band_coef = coef;
+    d.band_s0 = s0;
+    hipLaunchKernelGGL(k_band_tables, dim3((unsigned)nchunks), dim3(128), 0, s, d, coef, s0);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(s));  // the host vectors above are uploaded
+    return 0;
+}
+
+extern "C" int ffs_reml_plan_band_info(const ffs_reml_plan* P, int32_t* n_bands, int32_t* slots, int32_t* pairs,
+                                       int32_t* counts, int32_t* edge) {
+    if (!P) return fail(-3, "plan_band_info: null plan");
+    const PlanDev& d = P->d;
+    if (n_bands) *n_bands = d.n_bands;
+    if (slots) *slots = d.band_sb;
+    const size_t n = (size_t)d.n_bands * d.band_sb;
+    if (d.n_bands > 0) {
+        if (pairs) HIP_TRY(hipMemcpy(pairs, d.band_pairs, 4 * n, hipMemcpyDeviceToHost));
+        if (edge) HIP_TRY(hipMemcpy(edge, d.band_edge, 4 * n, hipMemcpyDeviceToHost));
+        if (counts) HIP_TRY(hipMemcpy(counts, d.band_n, 4 * (size_t)d.n_bands, hipMemcpyDeviceToHost));
+    }
+    return 0;
+}
+
+namespace {
+struct BandLayout {
+    int64_t nb, max_cols, max_vb;
+    size_t off_key, off_cnt, off_fill, off_cursor, off_vbin, off_vcol, off_nvb, off_rows, off_yt, off_s, total;
+};
+BandLayout band_layout(const ffs_reml_plan* P, int64_t V) {
+    BandLayout L{};
+    L.nb = std::max(P->d.n_bands, 1);
+    const int64_t groups = std::min<int64_t>(V, L.nb);
+    L.max_cols = vpad_of(V + (kBandTile - 1) * groups);
+    L.max_vb = L.max_cols / kBandTile;
+    size_t o = 0;
+    L.off_key = o;
+    o = align256(o + 4 * (size_t)V);
+    L.off_cnt = o;
+    o = align256(o + 4 * (size_t)L.nb);
+    L.off_fill = o;
+    o = align256(o + 4 * (size_t)(L.nb + 1));
+    L.off_cursor = o;
+    o = align256(o + 4 * (size_t)L.nb);
+    L.off_vbin = o;
+    o = align256(o + 4 * (size_t)L.max_vb);
+    L.off_vcol = o;
+    o = align256(o + 4 * (size_t)L.max_vb);
+    L.off_nvb = o;
+    o = align256(o + 4);
+    L.off_rows = o;
+    o = align256(o + 4 * (size_t)L.max_cols);
+    L.off_yt = o;
+    o = align256(o + 4 * (size_t)2 * P->d.Tpad * L.max_cols);
+    L.off_s = o;
+    o = align256(o + 4 * (size_t)L.max_cols * std::max(P->d.band_sb, 1));
+    L.total = o;
+    return L;
+}
+struct BandPtrs {
+    int32_t *key, *cnt, *fill, *cursor, *vbin, *vcol, *nvb, *rows;
+    float *Yt, *S;
+};
+BandPtrs band_ptrs(const BandLayout& L, void* workspace) {
+    char* ws = static_cast<char*>(workspace);
+    BandPtrs q;
+    q.key = reinterpret_cast<int32_t*>(ws + L.off_key);
+    q.cnt = reinterpret_cast<int32_t*>(ws + L.off_cnt);
+    q.fill = reinterpret_cast<int32_t*>(ws + L.off_fill);
+    q.cursor = reinterpret_cast<int32_t*>(ws + L.off_cursor);
+    q.vbin = reinterpret_cast<int32_t*>(ws + L.off_vbin);
+    q.vcol = reinterpret_cast<int32_t*>(ws + L.off_vcol);
+    q.nvb = reinterpret_cast<int32_t*>(ws + L.off_nvb);
+    q.rows = reinterpret_cast<int32_t*>(ws + L.off_rows);
+    q.Yt = reinterpret_cast<float*>(ws + L.off_yt);
+    q.S = reinterpret_cast<float*>(ws + L.off_s);
+    return q;
+}
+int band_check(const ffs_reml_plan* P, int64_t V, void* workspace, int64_t ws_bytes, BandLayout* L) {
+    if (!P) return fail(-3, "band search: null plan");
+    if (P->d.n_bands <= 0) return fail(-4, "band search: no bands on this plan (ffs_reml_plan_set_bands)");
+    if (P->d.proj_mode == 0) return fail(-4, "band search: needs a split projection mode (bf16x3 or fp16x3)");
+    *L = band_layout(P, V);
+    if (!workspace || ws_bytes < (int64_t)L->total) return fail(-3, "band search: workspace too small");
+    if (L->max_cols > 0x7fffffff) return fail(-3, "band search: batch too large; split it");
+    return 0;
+}
+}  // namespace
+
+extern "C" int ffs_reml_band_workspace(const ffs_reml_plan* P, int64_t V, int64_t* bytes) {
+    if (!P || !bytes) return fail(-3, "band_workspace: null argument");
+    *bytes = (int64_t)band_layout(P, std::max<int64_t>(V, 0)).total;
+    return 0;
+}
+
+extern "C" int ffs_reml_band_prepare(const ffs_reml_plan* P, const float* Y, int64_t ldY, int64_t V,
+                                     const float* props, const int32_t* need_full, float* scale, void* workspace,
+                                     int64_t ws_bytes, void* stream) {
+    if (V <= 0) return 0;
+    BandLayout L;
+    if (int rc = band_check(P, V, workspace, ws_bytes, &L)) return rc;
+    BandPtrs q = band_ptrs(L, workspace);
+    hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_band_key, dim3((unsigned)((V + 255) / 256)), dim3(256), 0, s, P->d, props, (int)V, need_full,
+                       q.key);
+    HIP_TRY(hipGetLastError());
+    if (int rc = group_by_key(q.key, V, (int)L.nb, kBandTile, q.cnt, q.fill, q.cursor, q.rows, L.max_cols, s))
+        return rc;
+    hipLaunchKernelGGL(k_group_tiles, dim3((unsigned)((L.max_vb + 255) / 256)), dim3(256), 0, s, q.fill, (int)L.nb,
+                       (int)L.max_vb, kBandTile, q.vbin, q.vcol, q.nvb);
+    HIP_TRY(hipGetLastError());
+    // split halves into rows Tpad..2 Tpad-1 of the band buffer; columns past the
+    // last group's run (fill[nb]) are skipped
+    return prep_impl(P, Y, ldY, V, q.rows, L.max_cols, q.Yt, L.max_cols, scale, 1, nullptr, s, q.fill + L.nb, true,
+                     false);
+}
+
+extern "C" int ffs_reml_band_energy(const ffs_reml_plan* P, int64_t V, void* workspace, int64_t ws_bytes,
+                                    void* stream) {
+    if (V <= 0) return 0;
+    BandLayout L;
+    if (int rc = band_check(P, V, workspace, ws_bytes, &L)) return rc;
+    BandPtrs q = band_ptrs(L, workspace);
+    const int64_t n_cg = (P->d.band_sb / kEnergyNB + kBandWaves - 1) / kBandWaves;
+    const int64_t items = L.max_cols / 64 * n_cg;
+    const int64_t nblk = std::min<int64_t>(2048, std::max<int64_t>(8, (items + 7) / 8 * 8));
+    hipLaunchKernelGGL(k_band_energy, dim3((unsigned)nblk), dim3(64 * kBandWaves), 0, (hipStream_t)stream, P->d, q.Yt,
+                       (int64_t)L.max_cols, q.vbin, q.nvb, q.fill, q.cnt, q.S);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+extern "C" int ffs_reml_band_project(const ffs_reml_plan* P, int64_t V, int32_t* best, float* best_score,
+                                     int32_t* need_full, void* workspace, int64_t ws_bytes, void* stream) {
+    if (V <= 0) return 0;
+    BandLayout L;
+    if (int rc = band_check(P, V, workspace, ws_bytes, &L)) return rc;
+    BandPtrs q = band_ptrs(L, workspace);
+    hipStream_t s = (hipStream_t)stream;
+    const int64_t items = L.max_vb * (P->d.band_sb / P->d.npt);
+    const dim3 g((unsigned)std::min<int64_t>(512, std::max<int64_t>(8, (items + 7) / 8 * 8))), b(256);
+    const int nc = (int)L.max_cols;
+    const int64_t ld = L.max_cols;
+    if (P->d.BN == 64) {
+        if (P->d.proj_mode == 1)
+            hipLaunchKernelGGL((k_band_project16<64, true>), g, b, 0, s, P->d, q.Yt, ld, nc, q.vbin, q.nvb, q.S);
+        else
+            hipLaunchKernelGGL((k_band_project16<64, false>), g, b, 0, s, P->d, q.Yt, ld, nc, q.vbin, q.nvb, q.S);
+    } else {
+        if (P->d.proj_mode == 1)
+            hipLaunchKernelGGL((k_band_project16<96, true>), g, b, 0, s, P->d, q.Yt, ld, nc, q.vbin, q.nvb, q.S);
+        else
+            hipLaunchKernelGGL((k_band_project16<96, false>), g, b, 0, s, P->d, q.Yt, ld, nc, q.vbin, q.nvb, q.S);
+    }
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(k_band_select, dim3((unsigned)(L.max_cols / 256)), dim3(256), 0, s, P->d, q.rows, q.vbin, q.nvb,
+                       q.S, best, best_score, need_full);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+extern "C" int ffs_reml_band_search(const ffs_reml_plan* P, const float* Y, int64_t ldY, int64_t V,
+                                    const float* props, int32_t* best, float* best_score, int32_t* need_full,
+                                    float* scale, void* workspace, int64_t ws_bytes, void* stream) {
+    if (int rc = ffs_reml_band_prepare(P, Y, ldY, V, props, need_full, scale, workspace, ws_bytes, stream)) return rc;
+    if (int rc = ffs_reml_band_energy(P, V, workspace, ws_bytes, stream)) return rc;
+    return ffs_reml_band_project(P, V, best, best_score, need_full, workspace, ws_bytes, stream);
 }
 
 extern "C" int ffs_reml_plan_set_projection(ffs_reml_plan* P, int32_t mode) {

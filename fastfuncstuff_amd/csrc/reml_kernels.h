@@ -43,6 +43,8 @@ __device__ inline bool xcd_tile_map(int bid, int nb, int n_tiles, int* vt, int* 
 
 // Largest design width handled by the f64 MFMA GLS kernel (two 16-row tiles).
 constexpr int kGlsMfmaMaxP = 32;
+// Band search: voxel groups padded to the projection block (256 columns).
+constexpr int kBandTile = 256;
 
 // Device-side description of the pair tables of one plan.
 struct PlanDev {
@@ -108,6 +110,16 @@ struct PlanDev {
     const uint16_t* Ll_bf16;
     const float2* zero2;       // [T] zeros
     const float* zero1;        // [T] zeros
+    // Lag-1 correlation bands (0 bands = off): band i holds, in pair order,
+    // the pairs whose lambda lies within the half-width of band_lo + i*band_h.
+    int n_bands;
+    int band_sb;               // slots per band (multiple of kEnergyNB and npt)
+    float band_lo, band_h;
+    const int32_t* band_n;     // [n_bands] pairs in the band
+    const int32_t* band_pairs; // [n_bands][band_sb] pair index (-1 = empty slot)
+    const int32_t* band_edge;  // [n_bands][band_sb] 1: a lattice neighbour is a pair outside the band
+    const float* band_coef;    // [n_bands * band_sb / kEnergyNB][T][3 * kEnergyNB] (c_t, alpha_t, 1/v_{t+1})
+    const float* band_s0;      // [n_bands][band_sb] 1/v_0
 };
 
 // Local search: window half-width, slots, pair column padding, voxel tile.

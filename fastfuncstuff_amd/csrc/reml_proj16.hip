@@ -33,35 +33,65 @@ __device__ __forceinline__ f32x16 mfma16u(u32x4 a, u32x4 b, f32x16 c) {
                                                       0, 0, 0);
 }
 
-template <int BN, bool BF16>
-__global__ void __launch_bounds__(256) k_project16b(PlanDev P, const float* __restrict__ Yt, int64_t ldY, int V,
-                                                    float* __restrict__ S, int64_t ldS) {
-    constexpr int BM = kProj16BM, BK = kProjBK, NBLK = BN / 32, NTB = BK / 8;
-    constexpr int RP = NTB * 8 + 8;  // B row pitch in halves (16-byte aligned rows)
-    constexpr int B_H = BN * RP;     // halves per plane
-    constexpr int C_LD = BN + 1;     // epilogue patch pitch (floats)
-    constexpr int STAGE_BYTES = 2 * B_H * 2;
-    constexpr int PATCH_BYTES = 4 * 32 * C_LD * 4;
-    constexpr int LDS_BYTES = STAGE_BYTES > PATCH_BYTES ? STAGE_BYTES : PATCH_BYTES;
+template <int BN>
+struct Proj16Lds {
+    static constexpr int NTB = kProjBK / 8;
+    static constexpr int RP = NTB * 8 + 8;  // B row pitch in halves (16-byte aligned rows)
+    static constexpr int B_H = BN * RP;     // halves per plane
+    static constexpr int C_LD = BN + 1;     // epilogue patch pitch (floats)
+    static constexpr int STAGE_BYTES = 2 * B_H * 2;
+    static constexpr int PATCH_BYTES = 4 * 32 * C_LD * 4;
+    static constexpr int BYTES = STAGE_BYTES > PATCH_BYTES ? STAGE_BYTES : PATCH_BYTES;
+};
+
+// One (256-voxel block, column tile) of the split projection.  Full grid
+// (BAND = false): column tile `tile` of the plan, S[v * ldS + k].  Band
+// (BAND = true): the tile's npt pairs are kp[0..npt) (-1 = empty), gathered
+// column by column from their tiles of the same operand, and the scores go to
+// S[v * ldS + slot0 + q] for the pair in slot q.  The arithmetic of every
+// output column is the same in both cases.
+template <int BN, bool BF16, bool BAND>
+__device__ __forceinline__ void proj16_item(const PlanDev& P, const float* __restrict__ Yt, int64_t ldY, int V,
+                                            float* __restrict__ S, int64_t ldS, int vt, int tile,
+                                            const int32_t* __restrict__ kp, int slot0, unsigned char* lds_raw) {
+    using L = Proj16Lds<BN>;
+    constexpr int BM = kProj16BM, NBLK = BN / 32, NTB = L::NTB, RP = L::RP, C_LD = L::C_LD;
     constexpr int B_N = 2 * NTB * BN;  // 16-byte B units per stage
     constexpr int B_U = B_N / 256;
     static_assert(B_N % 256 == 0, "B stage must split evenly over the block");
-    __shared__ __attribute__((aligned(16))) unsigned char lds_raw[LDS_BYTES];
     uint16_t* Bh = reinterpret_cast<uint16_t*>(lds_raw);
-    uint16_t* Bl = Bh + B_H;
+    uint16_t* Bl = Bh + L::B_H;
 
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    int vt, tile;
-    if (!xcd_tile_map(blockIdx.x, gridDim.x, P.n_tiles, &vt, &tile)) return;
     const int v0 = vt * BM;
-    const uint16_t* __restrict__ gh = (BF16 ? P.Gh_bf16 : P.Gh_f16) + (size_t)tile * P.Tpad * BN;
-    const uint16_t* __restrict__ gl = (BF16 ? P.Gl_bf16 : P.Gl_f16) + (size_t)tile * P.Tpad * BN;
+    const uint16_t* __restrict__ gh = BF16 ? P.Gh_bf16 : P.Gh_f16;
+    const uint16_t* __restrict__ gl = BF16 ? P.Gl_bf16 : P.Gl_f16;
     // this lane's A column (voxel) of row block r: v0 + 64 w + 32 r + lane % 32;
     // k-slot half: samples 8 (lane / 32) .. + 8 of each 16-sample k-step
     const uint16_t* __restrict__ Ash = reinterpret_cast<const uint16_t*>(Yt + (size_t)P.Tpad * ldY);
     const uint16_t* __restrict__ Asl = Ash + (size_t)P.Tpad * ldY;
     const int a_col = v0 + 64 * w + (lane & 31);
     const int a_tb = lane >> 5;
+    // source offset (halves, stage 0) of each B unit this thread stages
+    size_t ub[B_U];
+    bool uok[B_U];
+#pragma unroll
+    for (int u = 0; u < B_U; ++u) {
+        const int q = tid + u * 256;
+        const int rr = q % (NTB * BN);
+        const int tb = rr / BN, col = rr % BN;
+        int st = tile, cc = col;
+        bool ok = true;
+        if (BAND) {
+            const int slot = col / P.p, j = col - slot * P.p;
+            const int k = slot < P.npt ? kp[slot] : -1;
+            ok = k >= 0;
+            st = ok ? k / P.npt : 0;
+            cc = ok ? (k - st * P.npt) * P.p + j : 0;
+        }
+        ub[u] = (size_t)st * P.Tpad * BN + ((size_t)tb * BN + cc) * 8;
+        uok[u] = ok;
+    }
 
     u32x4 ah[2][2], al[2][2];  // [row block][k-step of the stage]
     u32x4 rb[B_U];
@@ -78,11 +108,9 @@ __global__ void __launch_bounds__(256) k_project16b(PlanDev P, const float* __re
     auto load_b = [&](int kt) __attribute__((always_inline)) {
 #pragma unroll
         for (int u = 0; u < B_U; ++u) {
-            const int q = tid + u * 256;
-            const int plane = q / (NTB * BN), rr = q % (NTB * BN);
-            const int tb = rr / BN, col = rr % BN;
-            const size_t o = ((size_t)(kt * NTB + tb) * BN + col) * 8;
-            rb[u] = *reinterpret_cast<const u32x4*>((plane ? gl : gh) + o);
+            const int plane = (tid + u * 256) / (NTB * BN);
+            const u32x4 x = *reinterpret_cast<const u32x4*>((plane ? gl : gh) + ub[u] + (size_t)kt * NTB * BN * 8);
+            rb[u] = uok[u] ? x : u32x4{0u, 0u, 0u, 0u};
         }
     };
 
@@ -93,7 +121,7 @@ __global__ void __launch_bounds__(256) k_project16b(PlanDev P, const float* __re
         for (int i = 0; i < NBLK; ++i)
 #pragma unroll
             for (int q = 0; q < 16; ++q) acc[r][i][q] = 0.f;
-    const int nk = P.Tpad / BK;
+    const int nk = P.Tpad / kProjBK;
     load_a(0);
     load_b(0);
     for (int kt = 0; kt < nk; ++kt) {
@@ -160,9 +188,9 @@ __global__ void __launch_bounds__(256) k_project16b(PlanDev P, const float* __re
         __syncthreads();
         for (int it = lane; it < 32 * npt; it += 64) {
             const int row = it / npt, qq = it % npt;
-            const int k = tile * npt + qq;
+            const int k = BAND ? kp[qq] : tile * npt + qq;
             const int v = v0 + 64 * w + 32 * r + row;
-            if (k >= K || v >= V) continue;
+            if (k < 0 || k >= K || v >= V) continue;
             const float* cr = Cw + row * C_LD + qq * p;
             double pp = 0.0;
             for (int j = 0; j < p; ++j) {
@@ -170,12 +198,21 @@ __global__ void __launch_bounds__(256) k_project16b(PlanDev P, const float* __re
                 pp = fma(cv, cv, pp);
             }
             if (!BF16) pp *= (double)P.g16_unscale[k];
-            float* sp = S + (size_t)v * ldS + k;
+            float* sp = S + (size_t)v * ldS + (BAND ? slot0 + qq : k);
             const double rss = (double)(*sp) - pp;
             *sp = (float)fma(tp, log(rss + 1e-10), P.ld[k]);
         }
         __syncthreads();
     }
+}
+
+template <int BN, bool BF16>
+__global__ void __launch_bounds__(256) k_project16b(PlanDev P, const float* __restrict__ Yt, int64_t ldY, int V,
+                                                    float* __restrict__ S, int64_t ldS) {
+    __shared__ __attribute__((aligned(16))) unsigned char lds_raw[Proj16Lds<BN>::BYTES];
+    int vt, tile;
+    if (!xcd_tile_map(blockIdx.x, gridDim.x, P.n_tiles, &vt, &tile)) return;
+    proj16_item<BN, BF16, false>(P, Yt, ldY, V, S, ldS, vt, tile, nullptr, 0, lds_raw);
 }
 
 }  // namespace remlk

@@ -118,16 +118,17 @@ __global__ void __launch_bounds__(256) k_compact_flags(const int32_t* __restrict
 }
 
 // ---------------------------------------------------------------------------
-// Local-search tiles from the padded group offsets: tile t covers columns
-// [t*kLocalTile, (t+1)*kLocalTile) of the group whose run contains them.
+// Tiles of a padded grouping (groups padded to multiples of `width`): tile t
+// covers columns [t*width, (t+1)*width) of the group whose run contains them.
 // ---------------------------------------------------------------------------
-__global__ void k_local_tiles(const int32_t* __restrict__ off, int ng, int max_tiles, int32_t* __restrict__ tile_group,
-                              int32_t* __restrict__ tile_col, int32_t* __restrict__ n_tiles) {
+__global__ void k_group_tiles(const int32_t* __restrict__ off, int ng, int max_tiles, int width,
+                              int32_t* __restrict__ tile_group, int32_t* __restrict__ tile_col,
+                              int32_t* __restrict__ n_tiles) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     const int total = off[ng];
-    if (t == 0) *n_tiles = total / kLocalTile;
+    if (t == 0) *n_tiles = total / width;
     if (t >= max_tiles) return;
-    const int col = t * kLocalTile;
+    const int col = t * width;
     if (col >= total) {
         tile_group[t] = -1;
         tile_col[t] = col;

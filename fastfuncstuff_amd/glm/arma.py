@@ -247,6 +247,7 @@ class RemlGrid(Mapping):
         self.logdet_R, self.logdet_XRX = ldR, ldX
         self.param_list = [(float(a), float(b)) for a, b in self.params.tolist()]
         self.projection = "fp32"
+        self.bands = None
 
     # Mapping protocol (the reference's dict keyed by (a, b))
     def __getitem__(self, key):
@@ -339,6 +340,31 @@ class RemlGrid(Mapping):
             raise ValueError(f"projection must be one of {sorted(self.PROJECTIONS)}, got {mode!r}")
         _lib.check(_lib.load().ffs_reml_plan_set_projection(self._handle, self.PROJECTIONS[mode]), "set_projection")
         self.projection = mode
+
+    def set_bands(self, half_width: float = 0.14, bin_width: float = 0.02):
+        """Lag-1 correlation bands for the learned search: band i holds the
+        pairs whose lambda lies within ``half_width`` of ``-1 + i * bin_width``.
+        ``half_width <= 0`` removes them.  Needs p <= 32."""
+        lib = _lib.load()
+        _lib.check(lib.ffs_reml_plan_set_bands(self._handle, float(half_width), float(bin_width),
+                                               _lib.stream_handle(self.device)), "set_bands")
+        self.bands = (float(half_width), float(bin_width)) if half_width > 0 and bin_width > 0 else None
+
+    def band_info(self):
+        """(pairs, counts, edge) of the bands as CPU int32 tensors
+        (n_bands x slots, n_bands, n_bands x slots), or None without bands."""
+        lib = _lib.load()
+        nb, sb = ctypes.c_int32(), ctypes.c_int32()
+        _lib.check(lib.ffs_reml_plan_band_info(self._handle, ctypes.byref(nb), ctypes.byref(sb), None, None, None),
+                   "band_info")
+        if nb.value <= 0:
+            return None
+        pairs = torch.empty(nb.value, sb.value, dtype=torch.int32)
+        edge = torch.empty(nb.value, sb.value, dtype=torch.int32)
+        cnt = torch.empty(nb.value, dtype=torch.int32)
+        _lib.check(lib.ffs_reml_plan_band_info(self._handle, None, None, _lib.ptr(pairs), _lib.ptr(cnt),
+                                               _lib.ptr(edge)), "band_info")
+        return pairs, cnt, edge
 
     SELECTORS = {"exhaustive": 0, "hierarchical": 1, "subset": 2}
 

@@ -186,17 +186,24 @@ class LocalSearcher:
         self.scale = torch.empty(V, dtype=torch.float32, device=dev)
         self.idx = torch.empty(V, dtype=torch.int32, device=dev)
         self.count = torch.zeros(1, dtype=torch.int32, device=dev)
+        self.bws = None
+        if grid.bands is not None and grid.projection != "fp32":
+            _lib.check(lib.ffs_reml_band_workspace(grid.handle, V, ctypes.byref(nb)), "band_workspace")
+            self.bws = torch.zeros(int(nb.value), dtype=torch.uint8, device=dev)
 
     def run(self, Y: torch.Tensor, timer: "StageTimer | None" = None, walk: int = 1,
-            stats: dict | None = None) -> LocalResult:
+            stats: dict | None = None, band: bool = True) -> LocalResult:
         """Learned-proposal search of Y (V, T) float32 on the device.
 
         Voxels whose window optimum lies on a window edge take up to ``walk``
-        further 5x5 windows centred on that optimum; those still on an edge,
-        and those with low confidence, take the full grid.  ``timer``
-        (optional) records a device event at the start of every stage;
-        ``stats`` (optional, synchronises) receives the voxel count of each
-        window pass ("window": first pass, "walk": list)."""
+        further 5x5 windows centred on that optimum.  Those still on an edge,
+        and those with low confidence, then take the band search when the grid
+        has lambda bands (``RemlGrid.set_bands``) and ``band`` is set: every
+        pair whose lambda lies near the proposal's.  What remains (band optimum
+        on the band's edge, non-finite proposals) takes the full grid.
+        ``timer`` (optional) records a device event at the start of every
+        stage; ``stats`` (optional, synchronises) receives the voxel count of
+        each pass ("window": first pass, "walk": list, "band")."""
         grid, lib, dev = self.grid, _lib.load(), self.grid.device
         V = Y.shape[0]
         if V != self.V:
@@ -234,6 +241,20 @@ class LocalSearcher:
             mark("walk_project")
             _lib.check(lib.ffs_reml_local_project(grid.handle, V, _lib.ptr(self.best), _lib.ptr(self.score),
                                                   _lib.ptr(self.need), ws, nb, st), "local_project")
+        if band and self.bws is not None:
+            if stats is not None:
+                n_vox, n_slots = band_load(grid, props, self.need)
+                stats["band"], stats["band_slots"] = n_vox, n_slots
+            bws, bnb = _lib.ptr(self.bws), self.bws.numel()
+            mark("band_prepare")
+            _lib.check(lib.ffs_reml_band_prepare(grid.handle, _lib.ptr(Y), Y.stride(0), V, _lib.ptr(props),
+                                                 _lib.ptr(self.need), _lib.ptr(self.scale), bws, bnb, st),
+                       "band_prepare")
+            mark("band_energy")
+            _lib.check(lib.ffs_reml_band_energy(grid.handle, V, bws, bnb, st), "band_energy")
+            mark("band_project")
+            _lib.check(lib.ffs_reml_band_project(grid.handle, V, _lib.ptr(self.best), _lib.ptr(self.score),
+                                                 _lib.ptr(self.need), bws, bnb, st), "band_project")
         mark("fallback_prep")
         _lib.check(lib.ffs_reml_compact(_lib.ptr(self.need), V, _lib.ptr(self.idx), _lib.ptr(self.count), st),
                    "compact")
@@ -255,6 +276,27 @@ class LocalSearcher:
             _lib.check(lib.ffs_reml_scatter_best(_lib.ptr(rows), n_fb, _lib.ptr(b_fb), _lib.ptr(s_fb),
                                                  _lib.ptr(self.best), _lib.ptr(self.score), st), "scatter_best")
         return LocalResult(self.best, self.score, self.scale, n_fb, props)
+
+
+def band_load(grid: RemlGrid, props: torch.Tensor, need: torch.Tensor) -> tuple[int, int]:
+    """(voxels, total pair evaluations) the band search takes for these
+    proposals and flags: the band choice of the library (lambda of the
+    proposal clamped to the lattice box, nearest band centre), restated."""
+    info = grid.band_info()
+    if info is None:
+        return 0, 0
+    _, cnt, _ = info
+    hw, h = grid.bands
+    a = props[:, 0].double().clamp(min(grid.a_axis), max(grid.a_axis))
+    b = props[:, 1].double().clamp(min(grid.b_axis), max(grid.b_axis))
+    ok = (need != 0) & torch.isfinite(props[:, 0]) & torch.isfinite(props[:, 1])
+    den = 1.0 + 2.0 * a * b + b * b
+    lam = torch.where(den > 0, (1.0 + a * b) * (a + b) / den.clamp(min=1e-300), torch.zeros_like(a)).clamp(-1.0, 1.0)
+    h32 = float(np.float32(h))
+    bins = torch.round((lam + 1.0) / h32).long().clamp(0, cnt.numel() - 1)
+    n = cnt.to(props.device).long()[bins]
+    ok = ok & (n > 0)
+    return int(ok.sum()), int(n[ok].sum())
 
 
 class StageTimer:

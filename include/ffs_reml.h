@@ -193,6 +193,38 @@ int ffs_reml_local_search(const ffs_reml_plan* plan, const float* Y, int64_t ldY
                           float conf_min, int32_t* best, float* best_score, int32_t* need_full, float* scale,
                           void* workspace, int64_t ws_bytes, void* stream);
 
+/*
+ * Band search (no counterpart in the reference, which searches the whole grid,
+ * glm/arma.py:1886-2018).  Band i holds, in pair order, the pairs whose lag-1
+ * correlation lambda = (1 + ab)(a + b) / (1 + 2ab + b^2) lies within
+ * half_width of -1 + i * bin_width.  A voxel with need_full[v] != 0 and a
+ * finite proposal (a_hat, b_hat) is scored on every pair of the band nearest
+ * lambda(a_hat, b_hat) (clamped to the lattice box), with the full grid's
+ * arithmetic, so its scores equal the full-grid surface entries bitwise; best
+ * = first minimum in pair order (lambda == 0 pairs mapped to the first of
+ * them), need_full = 1 when that pair has a lattice neighbour outside the band,
+ * else 0.  Other voxels are left as they were.  Needs p <= 32 and a split
+ * projection mode (returns -4 otherwise); scale = per-voxel std of the band
+ * voxels.  half_width <= 0 removes the bands.
+ */
+int ffs_reml_plan_set_bands(ffs_reml_plan* plan, double half_width, double bin_width, void* stream);
+/* Host copies (synchronous; each pointer nullable): band count, slots per band,
+ * pairs [n_bands][slots] (-1 = empty), counts [n_bands], edge flags [n_bands][slots]. */
+int ffs_reml_plan_band_info(const ffs_reml_plan* plan, int32_t* n_bands, int32_t* slots, int32_t* pairs,
+                            int32_t* counts, int32_t* edge);
+int ffs_reml_band_workspace(const ffs_reml_plan* plan, int64_t V, int64_t* bytes);
+/* Stages of ffs_reml_band_search (same workspace and stream): band choice,
+ * grouping and grouped residualisation; band energies; band projection, score
+ * and band argmin. */
+int ffs_reml_band_prepare(const ffs_reml_plan* plan, const float* Y, int64_t ldY, int64_t V, const float* props,
+                          const int32_t* need_full, float* scale, void* workspace, int64_t ws_bytes, void* stream);
+int ffs_reml_band_energy(const ffs_reml_plan* plan, int64_t V, void* workspace, int64_t ws_bytes, void* stream);
+int ffs_reml_band_project(const ffs_reml_plan* plan, int64_t V, int32_t* best, float* best_score,
+                          int32_t* need_full, void* workspace, int64_t ws_bytes, void* stream);
+int ffs_reml_band_search(const ffs_reml_plan* plan, const float* Y, int64_t ldY, int64_t V, const float* props,
+                         int32_t* best, float* best_score, int32_t* need_full, float* scale, void* workspace,
+                         int64_t ws_bytes, void* stream);
+
 /* idx[0..*count) = voxels with flags[v] != 0 (order unspecified); count is device memory. */
 int ffs_reml_compact(const int32_t* flags, int64_t V, int32_t* idx, int32_t* count, void* stream);
 
