@@ -104,7 +104,7 @@ def learned_eval(args):
     grid = RemlGrid(torch.from_numpy(ev.X), torch.from_numpy(ev.a_grid), torch.from_numpy(ev.b_grid), dev,
                     run_starts=ev.run_starts, tau=torch.from_numpy(ev.tau))
     grid.set_projection(args.projection)
-    if args.projection != "fp32" and args.band_width > 0:
+    if args.band_width > 0:
         grid.set_bands(args.band_width, args.band_bin)
     V = ev.Y.shape[0]
     t0 = time.time()
@@ -184,7 +184,7 @@ def main():
     torch.cuda.synchronize()
     t_plan = time.time() - t0
     grid.set_projection(args.projection)
-    if args.projection != "fp32" and args.band_width > 0:
+    if args.band_width > 0:
         grid.set_bands(args.band_width, args.band_bin)
     K = grid.K
     Y = torch.from_numpy(wl.Y).to(dev)
@@ -271,11 +271,6 @@ def main():
             (("local_project", "walk_project"), len(n_win), "mfma",
              mul * 2.0 * T * 25 * p * sum(n_win) / len(n_win),
              f"{'3*' if split else ''}2*T*25*p per window voxel, voxels per pass {n_win}"),
-        "k_energy": (("fallback_energy",), 1, "hbm", 4.0 * grid.Tpad * n_fb + 4.0 * n_fb * K,
-                     "4*Tpad*n_fb (residual read) + 4*K*n_fb (energy write)"),
-        "k_project16" if split else "k_project":
-            (("fallback_project",), 1, "mfma", mul * 2.0 * T * K * p * n_fb,
-             f"{'3*' if split else ''}2*T*K*p*n_fb"),
         "k_gls_mfma": (("gls",), 1, "hbm", 4.0 * V * T + 4.0 * V * (2 * p + 3), "4*V*T + 4*V*(2p+3)"),
     }
     n_bv, n_bs = wstats.get("band", 0), wstats.get("band_slots", 0)

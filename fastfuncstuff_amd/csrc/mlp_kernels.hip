@@ -52,11 +52,12 @@ __device__ inline void layer2(const float* sp, const float* x, float* z2, float*
     }
 }
 
+// Inference: the weights are read straight from global memory at wave-uniform
+// addresses (scalar loads, used as scalar operands of the FMAs).
 __global__ void __launch_bounds__(256) k_mlp_forward(const float* __restrict__ params, const float* __restrict__ X,
                                                      const int32_t* __restrict__ idx, int64_t B,
                                                      float* __restrict__ out) {
-    __shared__ float sp[NPARAM];
-    load_params(params, sp);
+    const float* __restrict__ sp = params;
     const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= B) return;
     const int64_t r = idx ? idx[b] : b;

@@ -19,7 +19,7 @@
 #include "reml_sort_gls.hip"
 #include "reml_prep_mfma.hip"
 #include "reml_proj16.hip"
-#include "reml_band.hip"
+#include "reml_sets.hip"
 #include "reml_dense.hip"
 
 using namespace remlk;
@@ -44,7 +44,6 @@ struct ffs_reml_plan {
     const double* pinvT = nullptr;
     const double* ldR = nullptr;
     const double* ldX = nullptr;
-    bool local_ok = false;
     double ridge = 0.0;  // > 0: search-only plan with the ridge-regularised score
     // second stream + events of ffs_reml_surface (created on first use)
     static constexpr int kMaxSurfaceChunks = 16;
@@ -108,6 +107,10 @@ static int plan_create_impl(ffs_reml_plan** out, int32_t T, int32_t p, const dou
                             const int32_t* gap, int32_t K, const double* ab, const int32_t* white, int32_t n_a,
                             int32_t n_b, const int32_t* lut, int32_t zero_a, int32_t zero_b, int32_t ltop,
                             int32_t n_task, const int32_t* task_idx, double ridge, void* stream);
+namespace {
+int build_windows(ffs_reml_plan* P, hipStream_t s);
+size_t align256(size_t x) { return (x + 255) / 256 * 256; }
+}  // namespace
 
 extern "C" int ffs_reml_plan_create(ffs_reml_plan** out, int32_t T, int32_t p, const double* X, const double* pinvT,
                                     const int32_t* gap, int32_t K, const double* ab, const int32_t* white,
@@ -226,16 +229,12 @@ static int plan_create_impl(ffs_reml_plan** out, int32_t T, int32_t p, const dou
                 b_axis[bi] = (float)ab[2 * q + 1];
             }
         }
-    float *da_axis, *db_axis, *lc, *ls0, *Gp = nullptr, *zero1;
-    float2 *lp, *zero2;
+    float *da_axis, *db_axis, *lc, *ls0;
+    float2* lp;
     if ((rc = upload(P, &da_axis, a_axis.data(), a_axis.size(), s)) ||
         (rc = upload(P, &db_axis, b_axis.data(), b_axis.size(), s)) || (rc = P->alloc(&lp, (size_t)K * T)) ||
-        (rc = P->alloc(&lc, (size_t)K * T)) || (rc = P->alloc(&ls0, K)) || (rc = P->alloc(&zero1, T)) ||
-        (rc = P->alloc(&zero2, T)))
+        (rc = P->alloc(&lc, (size_t)K * T)) || (rc = P->alloc(&ls0, K)))
         return rc;
-    HIP_TRY(hipMemsetAsync(zero1, 0, sizeof(float) * T, s));
-    HIP_TRY(hipMemsetAsync(zero2, 0, sizeof(float2) * T, s));
-    P->local_ok = (p <= kLocalPP);
     uint16_t *Ghf, *Glf, *Ghb, *Glb;
     float* g16s;
     const size_t n16 = (size_t)d.n_tiles * d.Tpad * BN;
@@ -270,19 +269,9 @@ static int plan_create_impl(ffs_reml_plan** out, int32_t T, int32_t p, const dou
         HIP_TRY(hipMemsetAsync(G64p, 0, sizeof(double) * K * d.Tpad * PG, s));
         HIP_TRY(hipMemsetAsync(coefR, 0, sizeof(double) * K * d.Tpad * 4, s));
     }
-    uint16_t *Lhf = nullptr, *Llf = nullptr, *Lhb = nullptr, *Llb = nullptr;
-    if (P->local_ok) {
-        const size_t nl = (size_t)K * d.Tpad * kLocalPP;
-        if ((rc = P->alloc(&Lhf, nl)) || (rc = P->alloc(&Llf, nl)) || (rc = P->alloc(&Lhb, nl)) ||
-            (rc = P->alloc(&Llb, nl)))
-            return rc;
-        for (uint16_t* q : {Lhf, Llf, Lhb, Llb}) HIP_TRY(hipMemsetAsync(q, 0, 2 * nl, s));
-        if ((rc = P->alloc(&Gp, (size_t)d.Tpad * K * kLocalPP))) return rc;
-        HIP_TRY(hipMemsetAsync(Gp, 0, sizeof(float) * d.Tpad * K * kLocalPP, s));
-    }
     hipLaunchKernelGGL(k_tables, dim3(K), dim3(64), 0, s, T, d.Tpad, p, K, dab, dwhiteflag, dgap, dX, d.n_task, dtask,
-                       full_f, BN, npt, soa, coef64, Xw64, G64, Lx, Ainv, Fm, work, ld, ldR, ldX, G32, Gp, Ghf, Glf, Ghb, Glb, g16s,
-                       d.PG, G64p, coefR, Lhf, Llf, Lhb, Llb, ridge, dstatus);
+                       full_f, BN, npt, soa, coef64, Xw64, G64, Lx, Ainv, Fm, work, ld, ldR, ldX, G32, Ghf, Glf, Ghb, Glb, g16s,
+                       d.PG, G64p, coefR, ridge, dstatus);
     HIP_TRY(hipGetLastError());
     hipLaunchKernelGGL(k_local_tables, dim3(K), dim3(128), 0, s, T, K, soa, lp, lc, ls0);
     HIP_TRY(hipGetLastError());
@@ -318,24 +307,18 @@ static int plan_create_impl(ffs_reml_plan** out, int32_t T, int32_t p, const dou
     d.lp = lp;
     d.lc = lc;
     d.ls0 = ls0;
-    d.Gp = Gp;
-    d.zero1 = zero1;
-    d.zero2 = zero2;
     d.Gh_f16 = Ghf;
     d.Gl_f16 = Glf;
     d.Gh_bf16 = Ghb;
     d.Gl_bf16 = Glb;
     d.g16_unscale = g16s;
-    d.Lh_f16 = Lhf;
-    d.Ll_f16 = Llf;
-    d.Lh_bf16 = Lhb;
-    d.Ll_bf16 = Llb;
     d.G64p = G64p;
     d.coefR = coefR;
     d.Xp = dXp;
     d.XtX = dXtX;
     d.pinvTp = dPp;
     d.proj_mode = 0;
+    if (int rc2 = build_windows(P, s)) return rc2;
     P->pinvT = dpinv;
     P->ridge = ridge;
     P->ldR = ldR;
@@ -457,32 +440,34 @@ extern "C" int ffs_reml_prep(const ffs_reml_plan* P, const float* Y, int64_t ldY
     return prep_impl(P, Y, ldY, V, rows, ncols, Yt, ldYt, scale, normalize, feats, (hipStream_t)stream);
 }
 
-extern "C" int ffs_reml_energy(const ffs_reml_plan* P, const float* Yt, int64_t ldYt, int64_t V, float* E,
-                               int64_t ldE, void* stream) {
-    if (!P) return fail(-3, "energy: null plan");
-    if (V <= 0) return 0;
+// n_live (nullable, device): voxels past *n_live are skipped (grid sized for V).
+static int energy_impl(const ffs_reml_plan* P, const float* Yt, int64_t ldYt, int64_t V, float* E, int64_t ldE,
+                       const int32_t* n_live, hipStream_t s) {
     const int64_t Vpad = vpad_of(V);
     if (ldYt < Vpad) return fail(-3, "energy: ldYt must be >= the padded voxel count");
     if (ldE < P->d.K) return fail(-3, "energy: ldE must be >= K");
-    hipStream_t s = (hipStream_t)stream;
     const int64_t n_xt = (V + kEnergyVox - 1) / kEnergyVox;
     const int64_t n_yg = (P->d.n_chunks + kEnergyWaves - 1) / kEnergyWaves;
     const int64_t nblk = 8 * ((n_xt + 7) / 8) * n_yg;  // see k_energy's block map
     if (nblk > 0x7fffffff) return fail(-3, "energy: grid too large; split the voxel batch");
     hipLaunchKernelGGL(k_energy, dim3((unsigned)nblk), dim3(kEnergyVox * kEnergyWaves), 0, s, P->d, Yt, ldYt, (int)V,
-                       E, ldE);
+                       E, ldE, n_live);
     HIP_TRY(hipGetLastError());
     return 0;
 }
 
-extern "C" int ffs_reml_project(const ffs_reml_plan* P, const float* Yt, int64_t ldYt, int64_t V, float* S,
-                                int64_t ldS, void* stream) {
-    if (!P) return fail(-3, "project: null plan");
+extern "C" int ffs_reml_energy(const ffs_reml_plan* P, const float* Yt, int64_t ldYt, int64_t V, float* E,
+                               int64_t ldE, void* stream) {
+    if (!P) return fail(-3, "energy: null plan");
     if (V <= 0) return 0;
+    return energy_impl(P, Yt, ldYt, V, E, ldE, nullptr, (hipStream_t)stream);
+}
+
+static int project_impl(const ffs_reml_plan* P, const float* Yt, int64_t ldYt, int64_t V, float* S, int64_t ldS,
+                        const int32_t* n_live, bool presplit, hipStream_t s) {
     const int64_t Vpad = vpad_of(V);
     if (ldYt < Vpad) return fail(-3, "project: ldYt must be >= the padded voxel count");
     if (ldS < P->d.K) return fail(-3, "project: ldS must be >= K");
-    hipStream_t s = (hipStream_t)stream;
     const int64_t nvt = (V + kProjBM - 1) / kProjBM;
     const int64_t grp8 = 8 * ((P->d.n_tiles + 7) / 8);  // see xcd_tile_map
     const int64_t nb = nvt * grp8;
@@ -492,34 +477,42 @@ extern "C" int ffs_reml_project(const ffs_reml_plan* P, const float* Yt, int64_t
     // split modes: 256-voxel blocks
     const int64_t nb16 = (V + kProj16BM - 1) / kProj16BM * grp8;
     const dim3 g16((unsigned)nb16);
-    // split modes read the halves that prep wrote when p <= 32 (f64 MFMA
-    // prep); otherwise they split the residual in the kernel
-    const bool presplit = P->d.PG != 0 && Yt == P->split_yt;
+    const int v = (int)V;
     if (P->d.BN == 64) {
         if (mode == 1 && presplit)
-            hipLaunchKernelGGL((k_project16b<64, true>), g16, b, 0, s, P->d, Yt, ldYt, (int)V, S, ldS);
+            hipLaunchKernelGGL((k_project16b<64, true>), g16, b, 0, s, P->d, Yt, ldYt, v, S, ldS, n_live);
         else if (mode == 2 && presplit)
-            hipLaunchKernelGGL((k_project16b<64, false>), g16, b, 0, s, P->d, Yt, ldYt, (int)V, S, ldS);
+            hipLaunchKernelGGL((k_project16b<64, false>), g16, b, 0, s, P->d, Yt, ldYt, v, S, ldS, n_live);
         else if (mode == 1)
-            hipLaunchKernelGGL((k_project16<64, true>), g, b, 0, s, P->d, Yt, ldYt, (int)V, S, ldS);
+            hipLaunchKernelGGL((k_project16<64, true>), g, b, 0, s, P->d, Yt, ldYt, v, S, ldS, n_live);
         else if (mode == 2)
-            hipLaunchKernelGGL((k_project16<64, false>), g, b, 0, s, P->d, Yt, ldYt, (int)V, S, ldS);
+            hipLaunchKernelGGL((k_project16<64, false>), g, b, 0, s, P->d, Yt, ldYt, v, S, ldS, n_live);
         else
-            hipLaunchKernelGGL(k_project<64>, g, b, 0, s, P->d, Yt, ldYt, (int)V, S, ldS);
+            hipLaunchKernelGGL(k_project<64>, g, b, 0, s, P->d, Yt, ldYt, v, S, ldS, n_live);
     } else {
         if (mode == 1 && presplit)
-            hipLaunchKernelGGL((k_project16b<96, true>), g16, b, 0, s, P->d, Yt, ldYt, (int)V, S, ldS);
+            hipLaunchKernelGGL((k_project16b<96, true>), g16, b, 0, s, P->d, Yt, ldYt, v, S, ldS, n_live);
         else if (mode == 2 && presplit)
-            hipLaunchKernelGGL((k_project16b<96, false>), g16, b, 0, s, P->d, Yt, ldYt, (int)V, S, ldS);
+            hipLaunchKernelGGL((k_project16b<96, false>), g16, b, 0, s, P->d, Yt, ldYt, v, S, ldS, n_live);
         else if (mode == 1)
-            hipLaunchKernelGGL((k_project16<96, true>), g, b, 0, s, P->d, Yt, ldYt, (int)V, S, ldS);
+            hipLaunchKernelGGL((k_project16<96, true>), g, b, 0, s, P->d, Yt, ldYt, v, S, ldS, n_live);
         else if (mode == 2)
-            hipLaunchKernelGGL((k_project16<96, false>), g, b, 0, s, P->d, Yt, ldYt, (int)V, S, ldS);
+            hipLaunchKernelGGL((k_project16<96, false>), g, b, 0, s, P->d, Yt, ldYt, v, S, ldS, n_live);
         else
-            hipLaunchKernelGGL(k_project<96>, g, b, 0, s, P->d, Yt, ldYt, (int)V, S, ldS);
+            hipLaunchKernelGGL(k_project<96>, g, b, 0, s, P->d, Yt, ldYt, v, S, ldS, n_live);
     }
     HIP_TRY(hipGetLastError());
     return 0;
+}
+
+extern "C" int ffs_reml_project(const ffs_reml_plan* P, const float* Yt, int64_t ldYt, int64_t V, float* S,
+                                int64_t ldS, void* stream) {
+    if (!P) return fail(-3, "project: null plan");
+    if (V <= 0) return 0;
+    // split modes read the halves that prep wrote when p <= 32 (f64 MFMA
+    // prep); otherwise they split the residual in the kernel
+    const bool presplit = P->d.PG != 0 && Yt == P->split_yt;
+    return project_impl(P, Yt, ldYt, V, S, ldS, nullptr, presplit, (hipStream_t)stream);
 }
 
 // Energy and projection of voxel chunks pipelined over two streams: the
@@ -567,8 +560,12 @@ extern "C" int ffs_reml_select(const ffs_reml_plan* P, const float* S, int64_t l
         return fail(-3, "select: mode must be 0 (exhaustive), 1 (hierarchical) or 2 (subset descent)");
     if (V <= 0) return 0;
     hipStream_t s = (hipStream_t)stream;
-    hipLaunchKernelGGL(k_select, dim3((unsigned)((V + 255) / 256)), dim3(256), 0, s, P->d, S, ldS, (int)V, mode,
-                       best, best_score);
+    if (mode == 0)
+        hipLaunchKernelGGL(k_select_first, dim3((unsigned)((V + 3) / 4)), dim3(256), 0, s, P->d, S, ldS, (int)V,
+                           nullptr, best, best_score);
+    else
+        hipLaunchKernelGGL(k_select, dim3((unsigned)((V + 255) / 256)), dim3(256), 0, s, P->d, S, ldS, (int)V, mode,
+                           best, best_score);
     HIP_TRY(hipGetLastError());
     return 0;
 }
@@ -688,153 +685,237 @@ extern "C" int ffs_reml_ljung_box(const double* resid_w, int64_t ldR, int64_t V,
 }
 
 // ---------------------------------------------------------------------------
-// Learned-proposal local search
+// Pair-set search (learned path): 5 x 5 windows and lambda bands
 // ---------------------------------------------------------------------------
 namespace {
-struct LocalLayout {
-    int64_t ng, max_cols, max_tiles;
-    size_t off_center, off_cnt, off_fill, off_cursor, off_tg, off_tc, off_nt, off_rows, off_yt, off_e, total;
+int host_clamp_center(int n_a, int n_b, int ai, int bi) {
+    if (n_a >= kWin) ai = std::min(std::max(ai, kWinHalf), n_a - 1 - kWinHalf);
+    if (n_b >= kWin) bi = std::min(std::max(bi, kWinHalf), n_b - 1 - kWinHalf);
+    return ai * n_b + bi;
+}
+
+// Uploads a family of pair sets (pairs in pair order) with their edge / next
+// flags and builds the energy chunk tables.
+int build_set_table(ffs_reml_plan* P, const std::vector<std::vector<int32_t>>& sets,
+                    const std::vector<std::vector<int32_t>>& edges, const std::vector<std::vector<int32_t>>* nexts,
+                    SetTable* out, hipStream_t s) {
+    PlanDev& d = P->d;
+    size_t maxn = 0;
+    for (const auto& v : sets) maxn = std::max(maxn, v.size());
+    const int unit = std::lcm(kEnergyNB, kSetWaves * d.npt);
+    const int SB = (int)std::max<size_t>(unit, (maxn + unit - 1) / unit * unit);
+    const int n = (int)sets.size();
+    std::vector<int32_t> pairs((size_t)n * SB, -1), edge((size_t)n * SB, 0), next, cnt(n, 0);
+    if (nexts) next.assign((size_t)n * SB, -1);
+    for (int i = 0; i < n; ++i) {
+        cnt[i] = (int32_t)sets[i].size();
+        for (size_t q = 0; q < sets[i].size(); ++q) {
+            pairs[(size_t)i * SB + q] = sets[i][q];
+            edge[(size_t)i * SB + q] = edges[i][q];
+            if (nexts) next[(size_t)i * SB + q] = (*nexts)[i][q];
+        }
+    }
+    int32_t *dpairs, *dedge, *dcnt, *dnext = nullptr;
+    float *coef, *s0;
+    const size_t nchunks = (size_t)n * (SB / kEnergyNB);
+    int rc = 0;
+    if ((rc = upload(P, &dpairs, pairs.data(), pairs.size(), s)) ||
+        (rc = upload(P, &dedge, edge.data(), edge.size(), s)) || (rc = upload(P, &dcnt, cnt.data(), cnt.size(), s)) ||
+        (nexts && (rc = upload(P, &dnext, next.data(), next.size(), s))) ||
+        (rc = P->alloc(&coef, nchunks * d.T * 3 * kEnergyNB)) || (rc = P->alloc(&s0, (size_t)n * SB)))
+        return rc;
+    SetTable t{};
+    t.n = n;
+    t.sb = SB;
+    t.cnt = dcnt;
+    t.pairs = dpairs;
+    t.edge = dedge;
+    t.next = dnext;
+    t.coef = coef;
+    t.s0 = s0;
+    hipLaunchKernelGGL(k_set_tables, dim3((unsigned)nchunks), dim3(128), 0, s, d, t, coef, s0);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(s));  // the host vectors above are uploaded
+    *out = t;
+    return 0;
+}
+
+// Windows of every lattice point: slots in (a, b) order (= pair order), edge =
+// the slot lies on a window side that the lattice continues past, next = the
+// window around that slot's pair.
+int build_windows(ffs_reml_plan* P, hipStream_t s) {
+    const int n_a = P->d.n_a, n_b = P->d.n_b;
+    std::vector<std::vector<int32_t>> sets((size_t)n_a * n_b), edges(sets.size()), nexts(sets.size());
+    for (int ca = 0; ca < n_a; ++ca)
+        for (int cb = 0; cb < n_b; ++cb) {
+            const size_t w = (size_t)ca * n_b + cb;
+            for (int r = 0; r < kWin; ++r)
+                for (int c = 0; c < kWin; ++c) {
+                    const int ai = ca - kWinHalf + r, bi = cb - kWinHalf + c;
+                    if (ai < 0 || ai >= n_a || bi < 0 || bi >= n_b) continue;
+                    const int k = P->h_lut[(size_t)ai * n_b + bi];
+                    if (k < 0) continue;
+                    const int e = (r == 0 && ai > 0) || (r == kWin - 1 && ai < n_a - 1) || (c == 0 && bi > 0) ||
+                                  (c == kWin - 1 && bi < n_b - 1);
+                    sets[w].push_back(k);
+                    edges[w].push_back(e);
+                    nexts[w].push_back(e ? host_clamp_center(n_a, n_b, ai, bi) : -1);
+                }
+        }
+    return build_set_table(P, sets, edges, &nexts, &P->d.win, s);
+}
+
+struct SetLayout {
+    int64_t n, max_cols, max_tiles, yt_rows;
+    size_t off_key, off_cnt, off_fill, off_cursor, off_tset, off_tcol, off_nt, off_rows, off_yt, off_s, total;
 };
-
-size_t align256(size_t x) { return (x + 255) / 256 * 256; }
-
-LocalLayout local_layout(const ffs_reml_plan* P, int64_t V) {
-    LocalLayout L{};
-    L.ng = (int64_t)P->d.n_a * P->d.n_b;
-    const int64_t groups = std::min<int64_t>(V, L.ng);
-    L.max_cols = vpad_of(V + (kLocalTile - 1) * groups);
-    L.max_tiles = L.max_cols / kLocalTile;
+SetLayout set_layout(const ffs_reml_plan* P, const SetTable& ST, int64_t V) {
+    SetLayout L{};
+    L.n = std::max(ST.n, 1);
+    const int64_t groups = std::min<int64_t>(V, L.n);
+    L.max_cols = vpad_of(V + (kSetTile - 1) * groups);
+    L.max_tiles = L.max_cols / kSetTile;
+    L.yt_rows = (P->d.PG && P->d.proj_mode != 0) ? 2 * (int64_t)P->d.Tpad : P->d.Tpad;
     size_t o = 0;
-    L.off_center = o;
-    o = align256(o + 4 * (size_t)V);
-    L.off_cnt = o;
-    o = align256(o + 4 * (size_t)(L.ng + 1));
-    L.off_fill = o;
-    o = align256(o + 4 * (size_t)(L.ng + 1));
-    L.off_cursor = o;
-    o = align256(o + 4 * (size_t)L.ng);
-    L.off_tg = o;
-    o = align256(o + 4 * (size_t)L.max_tiles);
-    L.off_tc = o;
-    o = align256(o + 4 * (size_t)L.max_tiles);
-    L.off_nt = o;
-    o = align256(o + 4);
-    L.off_rows = o;
-    o = align256(o + 4 * (size_t)L.max_cols);
-    L.off_yt = o;
-    o = align256(o + 4 * (size_t)P->d.Tpad * L.max_cols);
-    L.off_e = o;
-    o = align256(o + 4 * (size_t)L.max_cols * kSlots);
+    auto take = [&](size_t bytes) {
+        const size_t at = o;
+        o = align256(o + bytes);
+        return at;
+    };
+    L.off_key = take(4 * (size_t)V);
+    L.off_cnt = take(4 * (size_t)L.n);
+    L.off_fill = take(4 * (size_t)(L.n + 1));
+    L.off_cursor = take(4 * (size_t)L.n);
+    L.off_tset = take(4 * (size_t)L.max_tiles);
+    L.off_tcol = take(4 * (size_t)L.max_tiles);
+    L.off_nt = take(4);
+    L.off_rows = take(4 * (size_t)L.max_cols);
+    L.off_yt = take(4 * (size_t)L.yt_rows * L.max_cols);
+    L.off_s = take(4 * (size_t)L.max_cols * std::max(ST.sb, 1));
     L.total = o;
     return L;
+}
+struct SetPtrs {
+    int32_t *key, *cnt, *fill, *cursor, *tset, *tcol, *nt, *rows;
+    float *Yt, *S;
+};
+SetPtrs set_ptrs(const SetLayout& L, void* workspace) {
+    char* ws = static_cast<char*>(workspace);
+    SetPtrs q;
+    q.key = reinterpret_cast<int32_t*>(ws + L.off_key);
+    q.cnt = reinterpret_cast<int32_t*>(ws + L.off_cnt);
+    q.fill = reinterpret_cast<int32_t*>(ws + L.off_fill);
+    q.cursor = reinterpret_cast<int32_t*>(ws + L.off_cursor);
+    q.tset = reinterpret_cast<int32_t*>(ws + L.off_tset);
+    q.tcol = reinterpret_cast<int32_t*>(ws + L.off_tcol);
+    q.nt = reinterpret_cast<int32_t*>(ws + L.off_nt);
+    q.rows = reinterpret_cast<int32_t*>(ws + L.off_rows);
+    q.Yt = reinterpret_cast<float*>(ws + L.off_yt);
+    q.S = reinterpret_cast<float*>(ws + L.off_s);
+    return q;
+}
+int set_check(const ffs_reml_plan* P, const SetTable& ST, const char* what, int64_t V, void* workspace,
+              int64_t ws_bytes, SetLayout* L) {
+    if (!P) return fail(-3, std::string(what) + ": null plan");
+    if (ST.n <= 0) return fail(-4, std::string(what) + ": no sets on this plan");
+    if (P->d.proj_mode != 0 && !P->d.PG)
+        return fail(-4, std::string(what) + ": the split projection modes need p <= 32 here (use fp32)");
+    *L = set_layout(P, ST, V);
+    if (!workspace || ws_bytes < (int64_t)L->total) return fail(-3, std::string(what) + ": workspace too small");
+    if (L->max_cols > 0x7fffffff) return fail(-3, std::string(what) + ": batch too large; split it");
+    return 0;
+}
+// voxels grouped by q.key, residualised (with the split halves in the split
+// modes) into the set layout
+int set_group_prep(const ffs_reml_plan* P, const SetTable& ST, const float* Y, int64_t ldY, int64_t V, float* scale,
+                   const SetLayout& L, const SetPtrs& q, hipStream_t s) {
+    if (int rc = group_by_key(q.key, V, (int)L.n, kSetTile, q.cnt, q.fill, q.cursor, q.rows, L.max_cols, s)) return rc;
+    hipLaunchKernelGGL(k_group_tiles, dim3((unsigned)((L.max_tiles + 255) / 256)), dim3(256), 0, s, q.fill, (int)L.n,
+                       (int)L.max_tiles, kSetTile, q.tset, q.tcol, q.nt);
+    HIP_TRY(hipGetLastError());
+    const bool split = P->d.PG && P->d.proj_mode != 0;
+    return prep_impl(P, Y, ldY, V, q.rows, L.max_cols, q.Yt, L.max_cols, scale, 1, nullptr, s, q.fill + L.n, split,
+                     false);
+}
+int set_energy(const ffs_reml_plan* P, const SetTable& ST, const SetLayout& L, const SetPtrs& q, hipStream_t s) {
+    const int64_t n_cg = (ST.sb / kEnergyNB + kSetWaves - 1) / kSetWaves;
+    const int64_t items = L.max_tiles * n_cg;
+    const unsigned nblk = (unsigned)std::min<int64_t>(2048, std::max<int64_t>(8, (items + 7) / 8 * 8));
+    hipLaunchKernelGGL(k_set_energy, dim3(nblk), dim3(64 * kSetWaves), 0, s, P->d, ST, q.Yt, (int64_t)L.max_cols,
+                       q.tset, q.nt, q.S);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+int set_project(const ffs_reml_plan* P, const SetTable& ST, const SetLayout& L, const SetPtrs& q, int32_t* best,
+                float* best_score, int32_t* need, int32_t* next_key, hipStream_t s) {
+    const int64_t items = L.max_tiles * (ST.sb / (kSetWaves * P->d.npt));
+    const dim3 g((unsigned)std::min<int64_t>(1024, std::max<int64_t>(8, (items + 7) / 8 * 8))), b(64 * kSetWaves);
+    const int64_t ld = L.max_cols;
+    const int mode = P->d.proj_mode;
+#define FFS_SET_PROJ(BN_, MODE_) \
+    hipLaunchKernelGGL((k_set_project<BN_, MODE_>), g, b, 0, s, P->d, ST, q.Yt, ld, q.tset, q.nt, q.S)
+    if (P->d.BN == 64) {
+        if (mode == 1) FFS_SET_PROJ(64, 1);
+        else if (mode == 2) FFS_SET_PROJ(64, 2);
+        else FFS_SET_PROJ(64, 0);
+    } else {
+        if (mode == 1) FFS_SET_PROJ(96, 1);
+        else if (mode == 2) FFS_SET_PROJ(96, 2);
+        else FFS_SET_PROJ(96, 0);
+    }
+#undef FFS_SET_PROJ
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(k_set_select, dim3((unsigned)(L.max_cols / 256)), dim3(256), 0, s, P->d, ST, q.rows, q.tset,
+                       q.nt, q.S, best, best_score, need, next_key);
+    HIP_TRY(hipGetLastError());
+    return 0;
 }
 }  // namespace
 
 extern "C" int ffs_reml_local_workspace(const ffs_reml_plan* P, int64_t V, int64_t* bytes) {
     if (!P || !bytes) return fail(-3, "local_workspace: null argument");
-    *bytes = (int64_t)local_layout(P, V).total;
+    *bytes = (int64_t)set_layout(P, P->d.win, std::max<int64_t>(V, 0)).total;
     return 0;
 }
-
-namespace {
-struct LocalPtrs {
-    int32_t *center, *cnt, *fill, *cursor, *tg, *tc, *nt, *rows;
-    float *Ytg, *Eloc;
-};
-LocalPtrs local_ptrs(const LocalLayout& L, void* workspace) {
-    char* ws = static_cast<char*>(workspace);
-    LocalPtrs q;
-    q.center = reinterpret_cast<int32_t*>(ws + L.off_center);
-    q.cnt = reinterpret_cast<int32_t*>(ws + L.off_cnt);
-    q.fill = reinterpret_cast<int32_t*>(ws + L.off_fill);
-    q.cursor = reinterpret_cast<int32_t*>(ws + L.off_cursor);
-    q.tg = reinterpret_cast<int32_t*>(ws + L.off_tg);
-    q.tc = reinterpret_cast<int32_t*>(ws + L.off_tc);
-    q.nt = reinterpret_cast<int32_t*>(ws + L.off_nt);
-    q.rows = reinterpret_cast<int32_t*>(ws + L.off_rows);
-    q.Ytg = reinterpret_cast<float*>(ws + L.off_yt);
-    q.Eloc = reinterpret_cast<float*>(ws + L.off_e);
-    return q;
-}
-int local_check(const ffs_reml_plan* P, int64_t V, void* workspace, int64_t ws_bytes, LocalLayout* L) {
-    if (!P) return fail(-3, "local search: null plan");
-    if (!P->local_ok) return fail(-4, "local search: needs p <= 16 (use the full grid)");
-    *L = local_layout(P, V);
-    if (!workspace || ws_bytes < (int64_t)L->total) return fail(-3, "local search: workspace too small");
-    if (L->max_cols > 0x7fffffff) return fail(-3, "local search: batch too large; split it");
-    return 0;
-}
-}  // namespace
-
-namespace {
-static int local_group_and_prep(const ffs_reml_plan* P, const float* Y, int64_t ldY, int64_t V, float* scale,
-                                const LocalLayout& L, const LocalPtrs& q, hipStream_t s) {
-    // voxels grouped by window centre, each group padded to whole tiles
-    if (int rc = group_by_key(q.center, V, (int)L.ng, kLocalTile, q.cnt, q.fill, q.cursor, q.rows, L.max_cols, s))
-        return rc;
-    hipLaunchKernelGGL(k_group_tiles, dim3((unsigned)((L.max_tiles + 255) / 256)), dim3(256), 0, s, q.fill, (int)L.ng,
-                       (int)L.max_tiles, kLocalTile, q.tg, q.tc, q.nt);
-    HIP_TRY(hipGetLastError());
-    // columns past the last group's run (off[ng]) are skipped
-    return prep_impl(P, Y, ldY, V, q.rows, L.max_cols, q.Ytg, L.max_cols, scale, 1, nullptr, s, q.fill + L.ng,
-                     false);
-}
-
-}  // namespace
 
 extern "C" int ffs_reml_local_prepare(const ffs_reml_plan* P, const float* Y, int64_t ldY, int64_t V,
                                       const float* props, float conf_min, int32_t* need_full, float* scale,
                                       void* workspace, int64_t ws_bytes, void* stream) {
     if (V <= 0) return 0;
-    LocalLayout L;
-    if (int rc = local_check(P, V, workspace, ws_bytes, &L)) return rc;
-    LocalPtrs q = local_ptrs(L, workspace);
+    SetLayout L;
+    if (int rc = set_check(P, P->d.win, "local search", V, workspace, ws_bytes, &L)) return rc;
+    SetPtrs q = set_ptrs(L, workspace);
     hipStream_t s = (hipStream_t)stream;
-    const unsigned gv = (unsigned)((V + 255) / 256);
-    hipLaunchKernelGGL(k_local_center, dim3(gv), dim3(256), 0, s, P->d, props, (int)V, conf_min, q.center, need_full);
+    hipLaunchKernelGGL(k_win_key, dim3((unsigned)((V + 255) / 256)), dim3(256), 0, s, P->d, props, (int)V, conf_min,
+                       q.key, need_full);
     HIP_TRY(hipGetLastError());
-    return local_group_and_prep(P, Y, ldY, V, scale, L, q, s);
+    return set_group_prep(P, P->d.win, Y, ldY, V, scale, L, q, s);
 }
 
 extern "C" int ffs_reml_local_recenter(const ffs_reml_plan* P, const float* Y, int64_t ldY, int64_t V, float* scale,
                                        void* workspace, int64_t ws_bytes, void* stream) {
     if (V <= 0) return 0;
-    LocalLayout L;
-    if (int rc = local_check(P, V, workspace, ws_bytes, &L)) return rc;
-    return local_group_and_prep(P, Y, ldY, V, scale, L, local_ptrs(L, workspace), (hipStream_t)stream);
+    SetLayout L;
+    if (int rc = set_check(P, P->d.win, "local search", V, workspace, ws_bytes, &L)) return rc;
+    return set_group_prep(P, P->d.win, Y, ldY, V, scale, L, set_ptrs(L, workspace), (hipStream_t)stream);
 }
 
 extern "C" int ffs_reml_local_energy(const ffs_reml_plan* P, int64_t V, void* workspace, int64_t ws_bytes,
                                      void* stream) {
     if (V <= 0) return 0;
-    LocalLayout L;
-    if (int rc = local_check(P, V, workspace, ws_bytes, &L)) return rc;
-    LocalPtrs q = local_ptrs(L, workspace);
-    hipLaunchKernelGGL(k_local_energy, dim3((unsigned)L.max_tiles, 2), dim3(kWin * 64), 0, (hipStream_t)stream, P->d,
-                       q.Ytg, (int64_t)L.max_cols, q.tg, q.tc, q.nt, q.Eloc);
-    HIP_TRY(hipGetLastError());
-    return 0;
+    SetLayout L;
+    if (int rc = set_check(P, P->d.win, "local search", V, workspace, ws_bytes, &L)) return rc;
+    return set_energy(P, P->d.win, L, set_ptrs(L, workspace), (hipStream_t)stream);
 }
 
 extern "C" int ffs_reml_local_project(const ffs_reml_plan* P, int64_t V, int32_t* best, float* best_score,
                                       int32_t* need_full, void* workspace, int64_t ws_bytes, void* stream) {
     if (V <= 0) return 0;
-    LocalLayout L;
-    if (int rc = local_check(P, V, workspace, ws_bytes, &L)) return rc;
-    LocalPtrs q = local_ptrs(L, workspace);
-    // same arithmetic as the plan's full-grid projection mode (bitwise-equal scores)
-    const dim3 g((unsigned)L.max_tiles), b(256);
-    if (P->d.proj_mode == 1)
-        hipLaunchKernelGGL(k_local_project16<true>, g, b, 0, (hipStream_t)stream, P->d, q.Ytg, (int64_t)L.max_cols,
-                           q.tg, q.tc, q.nt, q.rows, q.Eloc, best, best_score, need_full, q.center);
-    else if (P->d.proj_mode == 2)
-        hipLaunchKernelGGL(k_local_project16<false>, g, b, 0, (hipStream_t)stream, P->d, q.Ytg, (int64_t)L.max_cols,
-                           q.tg, q.tc, q.nt, q.rows, q.Eloc, best, best_score, need_full, q.center);
-    else
-        hipLaunchKernelGGL(k_local_project, g, b, 0, (hipStream_t)stream, P->d, q.Ytg, (int64_t)L.max_cols, q.tg,
-                           q.tc, q.nt, q.rows, q.Eloc, best, best_score, need_full, q.center);
-    HIP_TRY(hipGetLastError());
-    return 0;
+    SetLayout L;
+    if (int rc = set_check(P, P->d.win, "local search", V, workspace, ws_bytes, &L)) return rc;
+    SetPtrs q = set_ptrs(L, workspace);
+    // the window to continue with (walk) replaces the voxel's key
+    return set_project(P, P->d.win, L, q, best, best_score, need_full, q.key, (hipStream_t)stream);
 }
 
 extern "C" int ffs_reml_local_search(const ffs_reml_plan* P, const float* Y, int64_t ldY, int64_t V,
@@ -860,22 +941,21 @@ extern "C" int ffs_reml_scatter_best(const int32_t* idx, int64_t n, const int32_
                                      int32_t* best, float* score, void* stream) {
     if (n <= 0) return 0;
     hipLaunchKernelGGL(k_scatter_best, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, idx,
-                       (int)n, best_sub, score_sub, best, score);
+                       (int)n, nullptr, best_sub, score_sub, best, score);
     HIP_TRY(hipGetLastError());
     return 0;
 }
 
 // ---------------------------------------------------------------------------
-// Band search (lambda bands)
+// Lambda bands
 // ---------------------------------------------------------------------------
 extern "C" int ffs_reml_plan_set_bands(ffs_reml_plan* P, double half_width, double bin_width, void* stream) {
     if (!P) return fail(-3, "plan_set_bands: null plan");
     PlanDev& d = P->d;
     if (!(half_width > 0.0) || !(bin_width > 0.0)) {
-        d.n_bands = 0;
+        d.band = SetTable{};
         return 0;
     }
-    if (!d.PG) return fail(-4, "plan_set_bands: the band search needs p <= 32");
     if (bin_width < 1e-3 || bin_width > 2.0) return fail(-3, "plan_set_bands: bin_width must be in [0.001, 2]");
     const int K = d.K, n_a = d.n_a, n_b = d.n_b;
     const int nbins = (int)std::floor(2.0 / bin_width + 0.5) + 1;
@@ -895,143 +975,54 @@ extern "C" int ffs_reml_plan_set_bands(ffs_reml_plan* P, double half_width, doub
                 bi[q] = y;
             }
         }
-    std::vector<std::vector<int32_t>> bands(nbins);
+    std::vector<std::vector<int32_t>> bands(nbins), edges(nbins);
     size_t maxn = 0;
     for (int i = 0; i < nbins; ++i) {
         const double c = -1.0 + i * bin_width;
-        for (int k = 0; k < K; ++k)
-            if (std::fabs(lam[k] - c) <= half_width) bands[i].push_back(k);
-        maxn = std::max(maxn, bands[i].size());
-    }
-    if (maxn == 0) return fail(-3, "plan_set_bands: no pair lies in any band");
-    const int unit = std::lcm(kEnergyNB, d.npt);
-    const int SB = (int)((maxn + unit - 1) / unit * unit);
-    std::vector<int32_t> pairs((size_t)nbins * SB, -1), edge((size_t)nbins * SB, 0), cnt(nbins, 0);
-    for (int i = 0; i < nbins; ++i) {
-        const double c = -1.0 + i * bin_width;
-        cnt[i] = (int32_t)bands[i].size();
-        for (size_t sl = 0; sl < bands[i].size(); ++sl) {
-            const int k = bands[i][sl];
-            pairs[(size_t)i * SB + sl] = k;
+        auto inside = [&](int k) { return std::fabs(lam[k] - c) <= half_width; };
+        for (int k = 0; k < K; ++k) {
+            if (!inside(k)) continue;
             int e = 0;
             if (ai[k] >= 0) {
                 const int nx[4] = {ai[k] - 1, ai[k] + 1, ai[k], ai[k]}, ny[4] = {bi[k], bi[k], bi[k] - 1, bi[k] + 1};
                 for (int q = 0; q < 4; ++q) {
                     if (nx[q] < 0 || nx[q] >= n_a || ny[q] < 0 || ny[q] >= n_b) continue;
                     const int kn = P->h_lut[(size_t)nx[q] * n_b + ny[q]];
-                    if (kn >= 0 && kn < K && !(std::fabs(lam[kn] - c) <= half_width)) e = 1;
+                    if (kn >= 0 && kn < K && !inside(kn)) e = 1;
                 }
             }
-            edge[(size_t)i * SB + sl] = e;
+            bands[i].push_back(k);
+            edges[i].push_back(e);
         }
+        maxn = std::max(maxn, bands[i].size());
     }
-    hipStream_t s = (hipStream_t)stream;
-    int rc = 0;
-    int32_t *dpairs, *dedge, *dcnt;
-    float *coef, *s0;
-    const size_t nchunks = (size_t)nbins * (SB / kEnergyNB);
-    if ((rc = upload(P, &dpairs, pairs.data(), pairs.size(), s)) || (rc = upload(P, &dedge, edge.data(), edge.size(), s)) ||
-        (rc = upload(P, &dcnt, cnt.data(), cnt.size(), s)) ||
-        (rc = P->alloc(&coef, nchunks * d.T * 3 * kEnergyNB)) || (rc = P->alloc(&s0, (size_t)nbins * SB)))
-        return rc;
-    d.n_bands = nbins;
-    d.band_sb = SB;
+    if (maxn == 0) return fail(-3, "plan_set_bands: no pair lies in any band");
+    SetTable t{};
+    if (int rc = build_set_table(P, bands, edges, nullptr, &t, (hipStream_t)stream)) return rc;
+    d.band = t;
     d.band_lo = -1.f;
     d.band_h = (float)bin_width;
-    d.band_n = dcnt;
-    d.band_pairs = dpairs;
-    d.band_edge = dedge;
-    d.band_coef = coef;
-    d.band_s0 = s0;
-    hipLaunchKernelGGL(k_band_tables, dim3((unsigned)nchunks), dim3(128), 0, s, d, coef, s0);
-    HIP_TRY(hipGetLastError());
-    HIP_TRY(hipStreamSynchronize(s));  // the host vectors above are uploaded
     return 0;
 }
 
 extern "C" int ffs_reml_plan_band_info(const ffs_reml_plan* P, int32_t* n_bands, int32_t* slots, int32_t* pairs,
                                        int32_t* counts, int32_t* edge) {
     if (!P) return fail(-3, "plan_band_info: null plan");
-    const PlanDev& d = P->d;
-    if (n_bands) *n_bands = d.n_bands;
-    if (slots) *slots = d.band_sb;
-    const size_t n = (size_t)d.n_bands * d.band_sb;
-    if (d.n_bands > 0) {
-        if (pairs) HIP_TRY(hipMemcpy(pairs, d.band_pairs, 4 * n, hipMemcpyDeviceToHost));
-        if (edge) HIP_TRY(hipMemcpy(edge, d.band_edge, 4 * n, hipMemcpyDeviceToHost));
-        if (counts) HIP_TRY(hipMemcpy(counts, d.band_n, 4 * (size_t)d.n_bands, hipMemcpyDeviceToHost));
+    const SetTable& t = P->d.band;
+    if (n_bands) *n_bands = t.n;
+    if (slots) *slots = t.sb;
+    const size_t n = (size_t)t.n * t.sb;
+    if (t.n > 0) {
+        if (pairs) HIP_TRY(hipMemcpy(pairs, t.pairs, 4 * n, hipMemcpyDeviceToHost));
+        if (edge) HIP_TRY(hipMemcpy(edge, t.edge, 4 * n, hipMemcpyDeviceToHost));
+        if (counts) HIP_TRY(hipMemcpy(counts, t.cnt, 4 * (size_t)t.n, hipMemcpyDeviceToHost));
     }
     return 0;
 }
 
-namespace {
-struct BandLayout {
-    int64_t nb, max_cols, max_vb;
-    size_t off_key, off_cnt, off_fill, off_cursor, off_vbin, off_vcol, off_nvb, off_rows, off_yt, off_s, total;
-};
-BandLayout band_layout(const ffs_reml_plan* P, int64_t V) {
-    BandLayout L{};
-    L.nb = std::max(P->d.n_bands, 1);
-    const int64_t groups = std::min<int64_t>(V, L.nb);
-    L.max_cols = vpad_of(V + (kBandTile - 1) * groups);
-    L.max_vb = L.max_cols / kBandTile;
-    size_t o = 0;
-    L.off_key = o;
-    o = align256(o + 4 * (size_t)V);
-    L.off_cnt = o;
-    o = align256(o + 4 * (size_t)L.nb);
-    L.off_fill = o;
-    o = align256(o + 4 * (size_t)(L.nb + 1));
-    L.off_cursor = o;
-    o = align256(o + 4 * (size_t)L.nb);
-    L.off_vbin = o;
-    o = align256(o + 4 * (size_t)L.max_vb);
-    L.off_vcol = o;
-    o = align256(o + 4 * (size_t)L.max_vb);
-    L.off_nvb = o;
-    o = align256(o + 4);
-    L.off_rows = o;
-    o = align256(o + 4 * (size_t)L.max_cols);
-    L.off_yt = o;
-    o = align256(o + 4 * (size_t)2 * P->d.Tpad * L.max_cols);
-    L.off_s = o;
-    o = align256(o + 4 * (size_t)L.max_cols * std::max(P->d.band_sb, 1));
-    L.total = o;
-    return L;
-}
-struct BandPtrs {
-    int32_t *key, *cnt, *fill, *cursor, *vbin, *vcol, *nvb, *rows;
-    float *Yt, *S;
-};
-BandPtrs band_ptrs(const BandLayout& L, void* workspace) {
-    char* ws = static_cast<char*>(workspace);
-    BandPtrs q;
-    q.key = reinterpret_cast<int32_t*>(ws + L.off_key);
-    q.cnt = reinterpret_cast<int32_t*>(ws + L.off_cnt);
-    q.fill = reinterpret_cast<int32_t*>(ws + L.off_fill);
-    q.cursor = reinterpret_cast<int32_t*>(ws + L.off_cursor);
-    q.vbin = reinterpret_cast<int32_t*>(ws + L.off_vbin);
-    q.vcol = reinterpret_cast<int32_t*>(ws + L.off_vcol);
-    q.nvb = reinterpret_cast<int32_t*>(ws + L.off_nvb);
-    q.rows = reinterpret_cast<int32_t*>(ws + L.off_rows);
-    q.Yt = reinterpret_cast<float*>(ws + L.off_yt);
-    q.S = reinterpret_cast<float*>(ws + L.off_s);
-    return q;
-}
-int band_check(const ffs_reml_plan* P, int64_t V, void* workspace, int64_t ws_bytes, BandLayout* L) {
-    if (!P) return fail(-3, "band search: null plan");
-    if (P->d.n_bands <= 0) return fail(-4, "band search: no bands on this plan (ffs_reml_plan_set_bands)");
-    if (P->d.proj_mode == 0) return fail(-4, "band search: needs a split projection mode (bf16x3 or fp16x3)");
-    *L = band_layout(P, V);
-    if (!workspace || ws_bytes < (int64_t)L->total) return fail(-3, "band search: workspace too small");
-    if (L->max_cols > 0x7fffffff) return fail(-3, "band search: batch too large; split it");
-    return 0;
-}
-}  // namespace
-
 extern "C" int ffs_reml_band_workspace(const ffs_reml_plan* P, int64_t V, int64_t* bytes) {
     if (!P || !bytes) return fail(-3, "band_workspace: null argument");
-    *bytes = (int64_t)band_layout(P, std::max<int64_t>(V, 0)).total;
+    *bytes = (int64_t)set_layout(P, P->d.band, std::max<int64_t>(V, 0)).total;
     return 0;
 }
 
@@ -1039,66 +1030,31 @@ extern "C" int ffs_reml_band_prepare(const ffs_reml_plan* P, const float* Y, int
                                      const float* props, const int32_t* need_full, float* scale, void* workspace,
                                      int64_t ws_bytes, void* stream) {
     if (V <= 0) return 0;
-    BandLayout L;
-    if (int rc = band_check(P, V, workspace, ws_bytes, &L)) return rc;
-    BandPtrs q = band_ptrs(L, workspace);
+    SetLayout L;
+    if (int rc = set_check(P, P->d.band, "band search", V, workspace, ws_bytes, &L)) return rc;
+    SetPtrs q = set_ptrs(L, workspace);
     hipStream_t s = (hipStream_t)stream;
     hipLaunchKernelGGL(k_band_key, dim3((unsigned)((V + 255) / 256)), dim3(256), 0, s, P->d, props, (int)V, need_full,
                        q.key);
     HIP_TRY(hipGetLastError());
-    if (int rc = group_by_key(q.key, V, (int)L.nb, kBandTile, q.cnt, q.fill, q.cursor, q.rows, L.max_cols, s))
-        return rc;
-    hipLaunchKernelGGL(k_group_tiles, dim3((unsigned)((L.max_vb + 255) / 256)), dim3(256), 0, s, q.fill, (int)L.nb,
-                       (int)L.max_vb, kBandTile, q.vbin, q.vcol, q.nvb);
-    HIP_TRY(hipGetLastError());
-    // split halves into rows Tpad..2 Tpad-1 of the band buffer; columns past the
-    // last group's run (fill[nb]) are skipped
-    return prep_impl(P, Y, ldY, V, q.rows, L.max_cols, q.Yt, L.max_cols, scale, 1, nullptr, s, q.fill + L.nb, true,
-                     false);
+    return set_group_prep(P, P->d.band, Y, ldY, V, scale, L, q, s);
 }
 
 extern "C" int ffs_reml_band_energy(const ffs_reml_plan* P, int64_t V, void* workspace, int64_t ws_bytes,
                                     void* stream) {
     if (V <= 0) return 0;
-    BandLayout L;
-    if (int rc = band_check(P, V, workspace, ws_bytes, &L)) return rc;
-    BandPtrs q = band_ptrs(L, workspace);
-    const int64_t n_cg = (P->d.band_sb / kEnergyNB + kBandWaves - 1) / kBandWaves;
-    const int64_t items = L.max_cols / 64 * n_cg;
-    const int64_t nblk = std::min<int64_t>(2048, std::max<int64_t>(8, (items + 7) / 8 * 8));
-    hipLaunchKernelGGL(k_band_energy, dim3((unsigned)nblk), dim3(64 * kBandWaves), 0, (hipStream_t)stream, P->d, q.Yt,
-                       (int64_t)L.max_cols, q.vbin, q.nvb, q.fill, q.cnt, q.S);
-    HIP_TRY(hipGetLastError());
-    return 0;
+    SetLayout L;
+    if (int rc = set_check(P, P->d.band, "band search", V, workspace, ws_bytes, &L)) return rc;
+    return set_energy(P, P->d.band, L, set_ptrs(L, workspace), (hipStream_t)stream);
 }
 
 extern "C" int ffs_reml_band_project(const ffs_reml_plan* P, int64_t V, int32_t* best, float* best_score,
                                      int32_t* need_full, void* workspace, int64_t ws_bytes, void* stream) {
     if (V <= 0) return 0;
-    BandLayout L;
-    if (int rc = band_check(P, V, workspace, ws_bytes, &L)) return rc;
-    BandPtrs q = band_ptrs(L, workspace);
-    hipStream_t s = (hipStream_t)stream;
-    const int64_t items = L.max_vb * (P->d.band_sb / P->d.npt);
-    const dim3 g((unsigned)std::min<int64_t>(512, std::max<int64_t>(8, (items + 7) / 8 * 8))), b(256);
-    const int nc = (int)L.max_cols;
-    const int64_t ld = L.max_cols;
-    if (P->d.BN == 64) {
-        if (P->d.proj_mode == 1)
-            hipLaunchKernelGGL((k_band_project16<64, true>), g, b, 0, s, P->d, q.Yt, ld, nc, q.vbin, q.nvb, q.S);
-        else
-            hipLaunchKernelGGL((k_band_project16<64, false>), g, b, 0, s, P->d, q.Yt, ld, nc, q.vbin, q.nvb, q.S);
-    } else {
-        if (P->d.proj_mode == 1)
-            hipLaunchKernelGGL((k_band_project16<96, true>), g, b, 0, s, P->d, q.Yt, ld, nc, q.vbin, q.nvb, q.S);
-        else
-            hipLaunchKernelGGL((k_band_project16<96, false>), g, b, 0, s, P->d, q.Yt, ld, nc, q.vbin, q.nvb, q.S);
-    }
-    HIP_TRY(hipGetLastError());
-    hipLaunchKernelGGL(k_band_select, dim3((unsigned)(L.max_cols / 256)), dim3(256), 0, s, P->d, q.rows, q.vbin, q.nvb,
-                       q.S, best, best_score, need_full);
-    HIP_TRY(hipGetLastError());
-    return 0;
+    SetLayout L;
+    if (int rc = set_check(P, P->d.band, "band search", V, workspace, ws_bytes, &L)) return rc;
+    return set_project(P, P->d.band, L, set_ptrs(L, workspace), best, best_score, need_full, nullptr,
+                       (hipStream_t)stream);
 }
 
 extern "C" int ffs_reml_band_search(const ffs_reml_plan* P, const float* Y, int64_t ldY, int64_t V,
@@ -1107,6 +1063,70 @@ extern "C" int ffs_reml_band_search(const ffs_reml_plan* P, const float* Y, int6
     if (int rc = ffs_reml_band_prepare(P, Y, ldY, V, props, need_full, scale, workspace, ws_bytes, stream)) return rc;
     if (int rc = ffs_reml_band_energy(P, V, workspace, ws_bytes, stream)) return rc;
     return ffs_reml_band_project(P, V, best, best_score, need_full, workspace, ws_bytes, stream);
+}
+
+// ---------------------------------------------------------------------------
+// Full-grid search of a device-counted voxel subset
+// ---------------------------------------------------------------------------
+namespace {
+struct SubsetLayout {
+    int64_t cap_pad, rows;
+    size_t off_yt, off_s, off_b, off_sc, total;
+};
+SubsetLayout subset_layout(const ffs_reml_plan* P, int64_t cap) {
+    SubsetLayout L{};
+    L.cap_pad = vpad_of(std::max<int64_t>(cap, 1));
+    L.rows = (P->d.PG && P->d.proj_mode != 0) ? 2 * (int64_t)P->d.Tpad : P->d.Tpad;
+    size_t o = 0;
+    L.off_yt = o;
+    o = align256(o + 4 * (size_t)L.rows * L.cap_pad);
+    L.off_s = o;
+    o = align256(o + 4 * (size_t)L.cap_pad * P->d.K);
+    L.off_b = o;
+    o = align256(o + 4 * (size_t)L.cap_pad);
+    L.off_sc = o;
+    o = align256(o + 4 * (size_t)L.cap_pad);
+    L.total = o;
+    return L;
+}
+}  // namespace
+
+extern "C" int ffs_reml_subset_workspace(const ffs_reml_plan* P, int64_t cap, int64_t* bytes) {
+    if (!P || !bytes) return fail(-3, "subset_workspace: null argument");
+    *bytes = (int64_t)subset_layout(P, cap).total;
+    return 0;
+}
+
+extern "C" int ffs_reml_subset_search(const ffs_reml_plan* P, const float* Y, int64_t ldY, int64_t V,
+                                      const int32_t* idx, const int32_t* n_live, int64_t cap, int32_t* best,
+                                      float* best_score, float* scale, void* workspace, int64_t ws_bytes,
+                                      void* stream) {
+    if (!P) return fail(-3, "subset_search: null plan");
+    if (V <= 0 || cap <= 0) return 0;
+    if (!idx || !n_live) return fail(-3, "subset_search: idx and n_live are required");
+    if (cap > V) cap = V;
+    const SubsetLayout L = subset_layout(P, cap);
+    if (!workspace || ws_bytes < (int64_t)L.total) return fail(-3, "subset_search: workspace too small");
+    if (L.cap_pad > 0x7fffffff) return fail(-3, "subset_search: cap too large");
+    char* ws = static_cast<char*>(workspace);
+    float* Yt = reinterpret_cast<float*>(ws + L.off_yt);
+    float* S = reinterpret_cast<float*>(ws + L.off_s);
+    int32_t* bsub = reinterpret_cast<int32_t*>(ws + L.off_b);
+    float* ssub = reinterpret_cast<float*>(ws + L.off_sc);
+    hipStream_t s = (hipStream_t)stream;
+    const int K = P->d.K;
+    const bool split = P->d.PG && P->d.proj_mode != 0;
+    if (int rc = prep_impl(P, Y, ldY, V, idx, cap, Yt, L.cap_pad, scale, 1, nullptr, s, n_live, split, false))
+        return rc;
+    if (int rc = energy_impl(P, Yt, L.cap_pad, cap, S, K, n_live, s)) return rc;
+    if (int rc = project_impl(P, Yt, L.cap_pad, cap, S, K, n_live, split, s)) return rc;
+    hipLaunchKernelGGL(k_select_first, dim3((unsigned)((cap + 3) / 4)), dim3(256), 0, s, P->d, S, (int64_t)K,
+                       (int)cap, n_live, bsub, ssub);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(k_scatter_best, dim3((unsigned)((cap + 255) / 256)), dim3(256), 0, s, idx, (int)cap, n_live,
+                       bsub, ssub, best, best_score);
+    HIP_TRY(hipGetLastError());
+    return 0;
 }
 
 extern "C" int ffs_reml_plan_set_projection(ffs_reml_plan* P, int32_t mode) {

@@ -43,8 +43,27 @@ __device__ inline bool xcd_tile_map(int bid, int nb, int n_tiles, int* vt, int* 
 
 // Largest design width handled by the f64 MFMA GLS kernel (two 16-row tiles).
 constexpr int kGlsMfmaMaxP = 32;
-// Band search: voxel groups padded to the projection block (256 columns).
-constexpr int kBandTile = 256;
+// Pair-set search: voxel groups padded to 64-column tiles; blocks of 4 waves.
+constexpr int kSetTile = 64;
+constexpr int kSetWaves = 4;
+
+// A family of pair sets (lambda bands, lattice windows).  Set s holds cnt[s]
+// pairs, in pair order, in slots [s*sb, s*sb + cnt[s]) (-1 past them); sb is a
+// multiple of kEnergyNB and of 4 projection tiles.  Per slot: edge = 1 when
+// the pair has a lattice neighbour that the set misses (the set's optimum is
+// then not trusted), next = the window to continue with from that pair
+// (windows only; -1 = none).  coef / s0: interleaved energy coefficients of
+// each 8-slot chunk (c_t, alpha_t, 1/v_{t+1}) and 1/v_0.
+struct SetTable {
+    int n;
+    int sb;
+    const int32_t* cnt;    // [n]
+    const int32_t* pairs;  // [n][sb]
+    const int32_t* edge;   // [n][sb]
+    const int32_t* next;   // [n][sb] or null
+    const float* coef;     // [n * sb / kEnergyNB][T][3 * kEnergyNB]
+    const float* s0;       // [n][sb]
+};
 
 // Device-side description of the pair tables of one plan.
 struct PlanDev {
@@ -101,32 +120,15 @@ struct PlanDev {
     const float2* lp;          // [K][T] (alpha_t, 1/v_{t+1})
     const float* lc;           // [K][T] c_t
     const float* ls0;          // [K] 1/v_0
-    const float* Gp;           // [Tpad][K * kLocalPP] pair-padded projection operand (p <= kLocalPP)
-    // pair-padded split 16-bit operands of the window search, same halves as
-    // Gh_* / Gl_*: [K][Tpad/8][kLocalPP][8]
-    const uint16_t* Lh_f16;
-    const uint16_t* Ll_f16;
-    const uint16_t* Lh_bf16;
-    const uint16_t* Ll_bf16;
-    const float2* zero2;       // [T] zeros
-    const float* zero1;        // [T] zeros
-    // Lag-1 correlation bands (0 bands = off): band i holds, in pair order,
-    // the pairs whose lambda lies within the half-width of band_lo + i*band_h.
-    int n_bands;
-    int band_sb;               // slots per band (multiple of kEnergyNB and npt)
-    float band_lo, band_h;
-    const int32_t* band_n;     // [n_bands] pairs in the band
-    const int32_t* band_pairs; // [n_bands][band_sb] pair index (-1 = empty slot)
-    const int32_t* band_edge;  // [n_bands][band_sb] 1: a lattice neighbour is a pair outside the band
-    const float* band_coef;    // [n_bands * band_sb / kEnergyNB][T][3 * kEnergyNB] (c_t, alpha_t, 1/v_{t+1})
-    const float* band_s0;      // [n_bands][band_sb] 1/v_0
+    // Pair sets scored for voxel groups (the full grid's arithmetic per pair):
+    // lambda bands (band.n = 0: off) and 5 x 5 lattice windows.
+    SetTable band;
+    float band_lo, band_h;     // centre of band i = band_lo + i * band_h
+    SetTable win;              // set of window centre c = ai * n_b + bi
 };
 
-// Local search: window half-width, slots, pair column padding, voxel tile.
+// Lattice window of the learned search: kWin x kWin pairs around the proposal.
 constexpr int kWin = 5;
 constexpr int kWinHalf = 2;
-constexpr int kSlots = kWin * kWin;
-constexpr int kLocalPP = 16;
-constexpr int kLocalTile = 128;
 
 }  // namespace remlk

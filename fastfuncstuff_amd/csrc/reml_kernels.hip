@@ -37,12 +37,10 @@ __global__ void __launch_bounds__(64) k_tables(int T, int Tpad, int p, int K, co
                                                double* __restrict__ Fm, double* __restrict__ work,
                                                double* __restrict__ ld, double* __restrict__ ldR_out,
                                                double* __restrict__ ldX_out, float* __restrict__ G32,
-                                               float* __restrict__ Gp, uint16_t* __restrict__ Ghf,
+                                               uint16_t* __restrict__ Ghf,
                                                uint16_t* __restrict__ Glf, uint16_t* __restrict__ Ghb,
                                                uint16_t* __restrict__ Glb, float* __restrict__ g16_unscale,
                                                int PG, double* __restrict__ G64p, double* __restrict__ coefR,
-                                               uint16_t* __restrict__ Lhf, uint16_t* __restrict__ Llf,
-                                               uint16_t* __restrict__ Lhb, uint16_t* __restrict__ Llb,
                                                double ridge, int* __restrict__ status) {
     const int k = blockIdx.x;
     if (k >= K) return;
@@ -206,13 +204,6 @@ __global__ void __launch_bounds__(64) k_tables(int T, int Tpad, int p, int K, co
             Glf[o] = __builtin_bit_cast(uint16_t, lf);
             Ghb[o] = __builtin_bit_cast(uint16_t, hb);
             Glb[o] = __builtin_bit_cast(uint16_t, lb);
-            if (Lhf && p <= kLocalPP) {
-                const size_t ol = (((size_t)k * (Tpad / 8) + (t >> 3)) * kLocalPP + j) * 8 + (t & 7);
-                Lhf[ol] = Ghf[o];
-                Llf[ol] = Glf[o];
-                Lhb[ol] = Ghb[o];
-                Llb[ol] = Glb[o];
-            }
         }
     }
     // f64 MFMA GLS operands: G padded to PG columns, recurrence coefficients
@@ -230,12 +221,6 @@ __global__ void __launch_bounds__(64) k_tables(int T, int Tpad, int p, int K, co
             cr[4 * t + 1] = start ? 0.0 : cc[t - 1];
             cr[4 * t + 2] = start ? 0.0 : al[t - 1];
             cr[4 * t + 3] = 0.0;
-        }
-    }
-    if (Gp && p <= kLocalPP) {
-        for (int idx = tid; idx < T * p; idx += 64) {
-            const int t = idx / p, j = idx % p;
-            Gp[(size_t)t * K * kLocalPP + (size_t)k * kLocalPP + j] = (float)g[idx];
         }
     }
     if (tid == 0) {
@@ -465,7 +450,7 @@ __device__ inline float white_energy(const float* __restrict__ y, int64_t ldY, i
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(kEnergyVox* kEnergyWaves) k_energy(PlanDev P, const float* __restrict__ Yt,
                                                                      int64_t ldY, int V, float* __restrict__ E,
-                                                                     int64_t ldE) {
+                                                                     int64_t ldE, const int32_t* __restrict__ n_live) {
     const int lane = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     // (voxel tile, chunk group) of this block: the chunk groups of one voxel
@@ -479,6 +464,8 @@ __global__ void __launch_bounds__(kEnergyVox* kEnergyWaves) k_energy(PlanDev P, 
     if (local / n_yg >= per || xt >= n_xt) return;
     const int chunk = (local % n_yg) * kEnergyWaves + w;
     if (chunk >= P.n_chunks) return;
+    if (n_live) V = min(V, *n_live);  // voxels past the live count are not needed
+    if (xt * kEnergyVox >= V) return;
     const int v = xt * kEnergyVox + lane;
     const int T = P.T;
     const float* __restrict__ y = Yt + v;
@@ -605,7 +592,7 @@ __device__ __forceinline__ void proj_sstore(const f32x4 (&ra)[A_V4], const f32x4
 // ---------------------------------------------------------------------------
 template <int BN>
 __global__ void __launch_bounds__(256) k_project(PlanDev P, const float* __restrict__ Yt, int64_t ldY, int V,
-                                                 float* __restrict__ S, int64_t ldS) {
+                                                 float* __restrict__ S, int64_t ldS, const int32_t* __restrict__ n_live) {
     constexpr int BM = kProjBM, BK = kProjBK, NBLK = BN / 32;
     constexpr int A_ELEMS = BK * BM, B_ELEMS = BK * BN;
     constexpr int STAGE = A_ELEMS + B_ELEMS;
@@ -618,6 +605,7 @@ __global__ void __launch_bounds__(256) k_project(PlanDev P, const float* __restr
     const int w = tid >> 6;
     int vt, tile;
     if (!xcd_tile_map(blockIdx.x, gridDim.x, P.n_tiles, &vt, &tile)) return;
+    if (n_live && vt * BM >= *n_live) return;  // past the live voxels
     const int v0 = vt * BM;
     const float* __restrict__ Ag = Yt + v0;
     const float* __restrict__ Bg = P.G32 + (size_t)tile * P.Tpad * BN;
@@ -783,7 +771,8 @@ __device__ __forceinline__ void proj16_sstore(const float (&ra)[A_U][8], const u
 
 template <int BN, bool BF16>
 __global__ void __launch_bounds__(256) k_project16(PlanDev P, const float* __restrict__ Yt, int64_t ldY, int V,
-                                                   float* __restrict__ S, int64_t ldS) {
+                                                   float* __restrict__ S, int64_t ldS,
+                                                   const int32_t* __restrict__ n_live) {
     constexpr int BM = kProjBM, BK = kProjBK, NBLK = BN / 32, NTB = BK / 8;
     constexpr int RP = NTB * 8 + 8;             // row pitch in halves (padded 16 B)
     constexpr int A_H = BM * RP, B_H = BN * RP;  // halves per hi (or lo) plane
@@ -799,6 +788,7 @@ __global__ void __launch_bounds__(256) k_project16(PlanDev P, const float* __res
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     int vt, tile;
     if (!xcd_tile_map(blockIdx.x, gridDim.x, P.n_tiles, &vt, &tile)) return;
+    if (n_live && vt * BM >= *n_live) return;  // past the live voxels
     const int v0 = vt * BM;
     const uint16_t* __restrict__ gh = (BF16 ? P.Gh_bf16 : P.Gh_f16) + (size_t)tile * P.Tpad * BN;
     const uint16_t* __restrict__ gl = (BF16 ? P.Gl_bf16 : P.Gl_f16) + (size_t)tile * P.Tpad * BN;
@@ -881,6 +871,44 @@ __global__ void __launch_bounds__(256) k_project16(PlanDev P, const float* __res
 __device__ inline float lat_val(const PlanDev& P, const float* row, int ai, int bi) {
     const int k = P.lut[ai * P.n_b + bi];
     return k >= 0 ? row[k] : INFINITY;
+}
+
+// (iii) mode 0, one wave per voxel: lane l takes pairs l, l + 64, ... with a
+// strict '<' (its first minimum), then the wave keeps the smallest score and,
+// among equal scores, the smallest pair index: the first minimum over all K
+// pairs, as the sequential scan (glm/arma.py:2008-2018).  Pairs whose score
+// is never below +inf (NaN, inf) are never chosen; a voxel with none takes
+// the (0, 0) pair.  n_live (nullable, device) bounds the voxels processed.
+__global__ void __launch_bounds__(256) k_select_first(PlanDev P, const float* __restrict__ S, int64_t ldS, int V,
+                                                      const int32_t* __restrict__ n_live, int32_t* __restrict__ best,
+                                                      float* __restrict__ best_score) {
+    const int lane = threadIdx.x & 63;
+    const int v = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int Vl = n_live ? min(*n_live, V) : V;
+    if (v >= Vl) return;
+    const float* row = S + (size_t)v * ldS;
+    float bs = INFINITY;
+    int bk = -1;
+    for (int k = lane; k < P.K; k += 64) {
+        const float s = row[k];
+        if (s < bs) {
+            bs = s;
+            bk = k;
+        }
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const float os = __shfl_xor(bs, o);
+        const int ok = __shfl_xor(bk, o);
+        if (ok >= 0 && (os < bs || (os == bs && (bk < 0 || ok < bk)))) {
+            bs = os;
+            bk = ok;
+        }
+    }
+    if (lane == 0) {
+        best[v] = bk >= 0 ? bk : P.lut[P.zero_a * P.n_b + P.zero_b];
+        best_score[v] = bs;
+    }
 }
 
 __global__ void __launch_bounds__(256) k_select(PlanDev P, const float* __restrict__ S, int64_t ldS, int V, int mode,
@@ -1197,404 +1225,6 @@ __global__ void __launch_bounds__(256) k_ljung_box(const double* __restrict__ rw
     }
 }
 
-// ---------------------------------------------------------------------------
-// Learned-proposal local search over a 5 x 5 lattice window.
-// ---------------------------------------------------------------------------
-__device__ inline int nearest_index(const float* axis, int n, float x) {
-    int best = 0;
-    float bd = fabsf(axis[0] - x);
-    for (int i = 1; i < n; ++i) {
-        const float d = fabsf(axis[i] - x);
-        if (d < bd) {
-            bd = d;
-            best = i;
-        }
-    }
-    return best;
-}
-
-// Window centre per voxel from the proposal (a_hat, b_hat, c); the window is
-// kept inside the lattice.  Voxels whose confidence c < conf_min or whose
-// proposal is not finite get flags[v] = 2 and no window (centre -1): they take
-// the full grid.  The others get flags[v] = 0.
-__device__ inline int clamp_center(const PlanDev& P, int ai, int bi) {
-    if (P.n_a >= kWin) ai = min(max(ai, kWinHalf), P.n_a - 1 - kWinHalf);
-    if (P.n_b >= kWin) bi = min(max(bi, kWinHalf), P.n_b - 1 - kWinHalf);
-    return ai * P.n_b + bi;
-}
-
-__global__ void k_local_center(PlanDev P, const float* __restrict__ props, int V, float conf_min,
-                               int32_t* __restrict__ center, int32_t* __restrict__ flags) {
-    const int v = blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= V) return;
-    const float ah = props[3 * v], bh = props[3 * v + 1], c = props[3 * v + 2];
-    const bool finite = isfinite(ah) && isfinite(bh) && isfinite(c);
-    if (!finite || !(c >= conf_min)) {
-        center[v] = -1;
-        flags[v] = 2;
-        return;
-    }
-    center[v] = clamp_center(P, nearest_index(P.a_axis, P.n_a, ah), nearest_index(P.b_axis, P.n_b, bh));
-    flags[v] = 0;
-}
-
-__device__ inline int window_pair(const PlanDev& P, int ca, int cb, int slot) {
-    const int ai = ca - kWinHalf + slot / kWin, bi = cb - kWinHalf + slot % kWin;
-    if (ai < 0 || ai >= P.n_a || bi < 0 || bi >= P.n_b) return -1;
-    return P.lut[ai * P.n_b + bi];
-}
-
-// Energies of the window's pairs.  Block: 5 waves (window rows) over 64
-// columns of one tile half.  Same arithmetic and accumulation schedule as
-// k_energy, so energies are bitwise equal to the full-grid ones.
-__global__ void __launch_bounds__(kWin * 64) k_local_energy(PlanDev P, const float* __restrict__ Yt, int64_t ldY,
-                                                            const int32_t* __restrict__ tile_group,
-                                                            const int32_t* __restrict__ tile_col,
-                                                            const int32_t* __restrict__ n_tiles,
-                                                            float* __restrict__ Eloc) {
-    const int tile = blockIdx.x;
-    if (tile >= *n_tiles) return;
-    const int lane = threadIdx.x & 63;
-    const int r = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int g = tile_group[tile];
-    const int ca = g / P.n_b, cb = g % P.n_b;
-    const int col = tile_col[tile] + blockIdx.y * 64 + lane;
-    const int T = P.T;
-    const float* __restrict__ y = Yt + col;
-    int ks[kWin];
-    bool has_white = false;
-    int kc = -1;
-#pragma unroll
-    for (int c = 0; c < kWin; ++c) {
-        int k = window_pair(P, ca, cb, r * kWin + c);
-        if (k >= 0 && P.white_flag[k]) {
-            has_white = true;
-            k = -2;  // handled by the white accumulator
-        }
-        ks[c] = k;
-        if (k >= 0 && kc < 0) kc = k;
-    }
-    const float2* __restrict__ cf[kWin];
-    float s0[kWin];
-#pragma unroll
-    for (int c = 0; c < kWin; ++c) {
-        cf[c] = ks[c] >= 0 ? P.lp + (size_t)ks[c] * T : P.zero2;
-        s0[c] = ks[c] >= 0 ? P.ls0[ks[c]] : 0.f;
-    }
-    const float* __restrict__ cz = kc >= 0 ? P.lc + (size_t)kc * T : P.zero1;
-    float e[kWin], part[kWin];
-    double acc[kWin];
-    const float y0 = y[0];
-#pragma unroll
-    for (int c = 0; c < kWin; ++c) {
-        e[c] = y0;
-        part[c] = y0 * y0 * s0[c];
-        acc[c] = 0.0;
-    }
-    float wpart = y0 * y0;
-    double wacc = 0.0;
-    float yc = y0;
-    for (int t0 = 0; t0 < T - 1; t0 += 8) {
-#pragma unroll
-        for (int i = 0; i < 8; ++i) {
-            const int t = t0 + i;
-            if (t < T - 1) {
-                const float yn = y[(size_t)(t + 1) * ldY];
-                const float z = fmaf(-cz[t], yc, yn);
-#pragma unroll
-                for (int c = 0; c < kWin; ++c) {
-                    const float2 co = cf[c][t];
-                    e[c] = fmaf(co.x, e[c], z);
-                    part[c] = fmaf(e[c] * co.y, e[c], part[c]);
-                }
-                wpart = fmaf(yn * 1.0f, yn, wpart);
-                yc = yn;
-            }
-        }
-#pragma unroll
-        for (int c = 0; c < kWin; ++c) {
-            acc[c] += part[c];
-            part[c] = 0.f;
-        }
-        wacc += wpart;
-        wpart = 0.f;
-    }
-    float* out = Eloc + (size_t)col * kSlots + r * kWin;
-#pragma unroll
-    for (int c = 0; c < kWin; ++c) {
-        if (ks[c] >= 0)
-            out[c] = (float)(acc[c] + part[c]);
-        else if (ks[c] == -2 && has_white)
-            out[c] = (float)(wacc + wpart);
-    }
-}
-
-template <int BM, int BN, int BK, int A_V4, int B_V4>
-__device__ __forceinline__ void local_gload(f32x4 (&ra)[A_V4], f32x4 (&rb)[B_V4], const float* __restrict__ Yt,
-                                            int64_t ldY, int col0, const float* __restrict__ Gp, int K16,
-                                            const int* kslot, int kt, int tid) {
-    const int t0 = kt * BK;
-#pragma unroll
-    for (int i = 0; i < A_V4; ++i) {
-        const int idx = tid + i * 256;
-        const int r = idx / (BM / 4), c4 = idx % (BM / 4);
-        ra[i] = *reinterpret_cast<const f32x4*>(Yt + (size_t)(t0 + r) * ldY + col0 + c4 * 4);
-    }
-#pragma unroll
-    for (int i = 0; i < B_V4; ++i) {
-        const int idx = tid + i * 256;
-        const int r = idx / (BN / 4), c4 = idx % (BN / 4);
-        const int slot = c4 / (kLocalPP / 4), part = c4 % (kLocalPP / 4);
-        const int k = kslot[slot];
-        const f32x4 g =
-            *reinterpret_cast<const f32x4*>(Gp + (size_t)(t0 + r) * K16 + (size_t)(k >= 0 ? k : 0) * kLocalPP + part * 4);
-        rb[i] = k >= 0 ? g : f32x4{0.f, 0.f, 0.f, 0.f};
-    }
-}
-
-// Window epilogue shared by the fp32 and split 16-bit kernels: one 32x32
-// accumulator block at a time through a wave-private LDS patch, lane (row,
-// half) sums its slot's p columns in order (as k_project / k_project16), then
-// the local argmin, the edge test and the next window centre.
-template <int NBLK>
-__device__ __forceinline__ void local_finish(const PlanDev& P, f32x16 (&acc)[NBLK], float* lds,
-                                             float (*scores)[kSlots + 1], const int* kslot, int col0, int ca, int cb,
-                                             int tid, const int32_t* __restrict__ rows,
-                                             const float* __restrict__ Eloc, int32_t* __restrict__ best,
-                                             float* __restrict__ best_score, int32_t* __restrict__ need_full,
-                                             int32_t* __restrict__ center, bool unscale) {
-    constexpr int BM = kLocalTile;
-    const int lane = tid & 63, w = tid >> 6;
-    float* Cw = lds + w * 32 * 33;
-    const int p = P.p;
-    const double tp = (double)(P.T - p);
-#pragma unroll
-    for (int n = 0; n < NBLK; ++n) {
-#pragma unroll
-        for (int r = 0; r < 16; ++r) {
-            const int row = 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-            Cw[row * 33 + (lane & 31)] = acc[n][r];
-        }
-        __syncthreads();
-        const int row = lane & 31, half = lane >> 5;
-        const int slot = 2 * n + half;
-        if (slot < kSlots) {
-            const int k = kslot[slot];
-            float sc = INFINITY;
-            const int col = col0 + w * 32 + row;
-            if (k >= 0) {
-                double pp = 0.0;
-                for (int j = 0; j < p; ++j) {
-                    const double cv = (double)Cw[row * 33 + half * kLocalPP + j];
-                    pp = fma(cv, cv, pp);
-                }
-                if (unscale) pp *= (double)P.g16_unscale[k];
-                const double rss = (double)Eloc[(size_t)col * kSlots + slot] - pp;
-                sc = (float)fma(tp, log(rss + 1e-10), P.ld[k]);
-            }
-            scores[w * 32 + row][slot] = sc;
-        }
-        __syncthreads();
-    }
-    if (tid < BM) {
-        const int v = rows[col0 + tid];
-        if (v >= 0) {
-            float bs = INFINITY;
-            int bslot = -1;
-            for (int s2 = 0; s2 < kSlots; ++s2) {
-                const float sc = scores[tid][s2];
-                if (sc < bs) {
-                    bs = sc;
-                    bslot = s2;
-                }
-            }
-            int edge = 1;
-            int k = -1;
-            int next = -1;
-            if (bslot >= 0) {
-                k = kslot[bslot];
-                // lambda == 0 pairs score bitwise alike: the global first-minimum
-                // rule picks the first of them.
-                if (P.white_flag[k]) k = P.white[0];
-                const int r = bslot / kWin, c = bslot % kWin;
-                const int ai = ca - kWinHalf + r, bi = cb - kWinHalf + c;
-                edge = (r == 0 && ai > 0) || (r == kWin - 1 && ai < P.n_a - 1) || (c == 0 && bi > 0) ||
-                       (c == kWin - 1 && bi < P.n_b - 1);
-                if (edge) next = clamp_center(P, ai, bi);
-            }
-            best[v] = k;
-            best_score[v] = bs;
-            need_full[v] = edge;
-            center[v] = next;
-        }
-    }
-}
-
-// Window projection on MFMA (fp32 32x32x2, same k-order as k_project) + score +
-// local argmin + edge test.  Block: 4 waves x 32 voxel rows = one 128-column
-// tile; N = 28 pair slots x kLocalPP columns (slots 25..27 are padding).
-// Per voxel: best / best_score; need_full = 1 when the argmin lies on a window
-// side that the lattice continues past (else 0); center = the window around
-// that argmin for a further step (edge) or -1.
-__global__ void __launch_bounds__(256) k_local_project(PlanDev P, const float* __restrict__ Yt, int64_t ldY,
-                                                       const int32_t* __restrict__ tile_group,
-                                                       const int32_t* __restrict__ tile_col,
-                                                       const int32_t* __restrict__ n_tiles,
-                                                       const int32_t* __restrict__ rows,
-                                                       const float* __restrict__ Eloc, int32_t* __restrict__ best,
-                                                       float* __restrict__ best_score,
-                                                       int32_t* __restrict__ need_full, int32_t* __restrict__ center) {
-    constexpr int BM = kLocalTile, BK = 16, NSLOT = kSlots + 3, BN = NSLOT * kLocalPP, NBLK = BN / 32;
-    constexpr int A_ELEMS = BK * BM, B_ELEMS = BK * BN;
-    __shared__ float lds[A_ELEMS + B_ELEMS];
-    __shared__ float scores[BM][kSlots + 1];
-    __shared__ int kslot[NSLOT];
-    const int tile = blockIdx.x;
-    if (tile >= *n_tiles) return;
-    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    const int g = tile_group[tile];
-    const int ca = g / P.n_b, cb = g % P.n_b;
-    const int col0 = tile_col[tile];
-    if (tid < NSLOT) kslot[tid] = tid < kSlots ? window_pair(P, ca, cb, tid) : -1;
-    __syncthreads();
-    const int K16 = P.K * kLocalPP;
-    float* As = lds;
-    float* Bs = lds + A_ELEMS;
-    constexpr int A_V4 = A_ELEMS / 4 / 256;
-    constexpr int B_V4 = (B_ELEMS / 4 + 255) / 256;
-    f32x4 ra[A_V4], rb[B_V4];
-    static_assert((B_ELEMS / 4) % 256 == 0 && (A_ELEMS / 4) % 256 == 0, "local tiles must be exact");
-    f32x16 acc[NBLK];
-#pragma unroll
-    for (int i = 0; i < NBLK; ++i)
-#pragma unroll
-        for (int r = 0; r < 16; ++r) acc[i][r] = 0.f;
-    const int nk = P.Tpad / BK;
-    local_gload<BM, BN, BK, A_V4, B_V4>(ra, rb, Yt, ldY, col0, P.Gp, K16, kslot, 0, tid);
-    for (int kt = 0; kt < nk; ++kt) {
-#pragma unroll
-        for (int i = 0; i < A_V4; ++i) *reinterpret_cast<f32x4*>(As + (tid + i * 256) * 4) = ra[i];
-#pragma unroll
-        for (int i = 0; i < B_V4; ++i) *reinterpret_cast<f32x4*>(Bs + (tid + i * 256) * 4) = rb[i];
-        __syncthreads();
-        if (kt + 1 < nk) local_gload<BM, BN, BK, A_V4, B_V4>(ra, rb, Yt, ldY, col0, P.Gp, K16, kslot, kt + 1, tid);
-        const int ar = lane >> 5, ac = lane & 31;
-#pragma unroll
-        for (int kk = 0; kk < BK / 2; ++kk) {
-            const float av = As[(2 * kk + ar) * BM + w * 32 + ac];
-#pragma unroll
-            for (int n = 0; n < NBLK; ++n) {
-                const float bv = Bs[(2 * kk + ar) * BN + n * 32 + ac];
-                acc[n] = __builtin_amdgcn_mfma_f32_32x32x2f32(av, bv, acc[n], 0, 0, 0);
-            }
-        }
-        __syncthreads();
-    }
-    local_finish<NBLK>(P, acc, lds, scores, kslot, col0, ca, cb, tid, rows, Eloc, best, best_score, need_full, center,
-                       false);
-}
-
-// Window projection with split 16-bit operands (the plan's bf16x3 / fp16x3
-// mode): per 16-sample k-step acc += Al Bh, Ah Bl, Ah Bh on 32x32x16 MFMA in the
-// k order of k_project16, with the same halves (pair-padded copies of its
-// operand) and the same epilogue, so window scores equal that mode's full-grid
-// scores bitwise.  A (the grouped residual) is split on the way into LDS.
-template <bool BF16>
-__global__ void __launch_bounds__(256) k_local_project16(PlanDev P, const float* __restrict__ Yt, int64_t ldY,
-                                                         const int32_t* __restrict__ tile_group,
-                                                         const int32_t* __restrict__ tile_col,
-                                                         const int32_t* __restrict__ n_tiles,
-                                                         const int32_t* __restrict__ rows,
-                                                         const float* __restrict__ Eloc, int32_t* __restrict__ best,
-                                                         float* __restrict__ best_score,
-                                                         int32_t* __restrict__ need_full,
-                                                         int32_t* __restrict__ center) {
-    constexpr int BM = kLocalTile, NSLOT = kSlots + 3, BN = NSLOT * kLocalPP, NBLK = BN / 32;
-    constexpr int NTB = 2;               // 8-sample blocks per 16-sample stage
-    constexpr int RP = NTB * 8 + 8;      // row pitch in halves (48 B: 16-byte aligned rows)
-    constexpr int A_H = BM * RP, B_H = BN * RP;
-    constexpr int B_N = 2 * NTB * BN;    // 16-byte B units per stage (planes x blocks x columns)
-    constexpr int B_U = B_N / 256;
-    static_assert(B_N % 256 == 0 && BM * NTB == 256, "window tiles must be exact");
-    static_assert(2 * (A_H + B_H) * 2 >= 4 * 32 * 33 * 4, "epilogue patch must fit the staging area");
-    __shared__ __attribute__((aligned(16))) uint16_t stage[2 * A_H + 2 * B_H];
-    __shared__ float scores[BM][kSlots + 1];
-    __shared__ int kslot[NSLOT];
-    uint16_t* Ah = stage;
-    uint16_t* Al = Ah + A_H;
-    uint16_t* Bh = Al + A_H;
-    uint16_t* Bl = Bh + B_H;
-    const int tile = blockIdx.x;
-    if (tile >= *n_tiles) return;
-    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    const int g = tile_group[tile];
-    const int ca = g / P.n_b, cb = g % P.n_b;
-    const int col0 = tile_col[tile];
-    if (tid < NSLOT) kslot[tid] = tid < kSlots ? window_pair(P, ca, cb, tid) : -1;
-    __syncthreads();
-    const uint16_t* __restrict__ Gh = BF16 ? P.Lh_bf16 : P.Lh_f16;
-    const uint16_t* __restrict__ Gl = BF16 ? P.Ll_bf16 : P.Ll_f16;
-    const int TB = P.Tpad / 8;
-    // A unit of this thread: voxel column vv, 8-sample block ta
-    const int vv = tid % BM, ta = tid / BM;
-    float ra[8];
-    u32x4 rb[B_U];
-    auto gload = [&](int kt) __attribute__((always_inline)) {
-        const float* src = Yt + (size_t)(kt * 16 + ta * 8) * ldY + col0 + vv;
-#pragma unroll
-        for (int e = 0; e < 8; ++e) ra[e] = src[(size_t)e * ldY];
-#pragma unroll
-        for (int u = 0; u < B_U; ++u) {
-            const int q = tid + u * 256;
-            const int plane = q / (NTB * BN), rr = q % (NTB * BN);
-            const int tb = rr / BN, col = rr % BN;
-            const int k = kslot[col / kLocalPP];
-            const size_t o = (((size_t)(k >= 0 ? k : 0) * TB + kt * NTB + tb) * kLocalPP + col % kLocalPP) * 8;
-            const u32x4 x = *reinterpret_cast<const u32x4*>((plane ? Gl : Gh) + o);
-            rb[u] = k >= 0 ? x : u32x4{0u, 0u, 0u, 0u};
-        }
-    };
-    f32x16 acc[NBLK];
-#pragma unroll
-    for (int i = 0; i < NBLK; ++i)
-#pragma unroll
-        for (int r = 0; r < 16; ++r) acc[i][r] = 0.f;
-    const int nk = P.Tpad / 16;
-    gload(0);
-    for (int kt = 0; kt < nk; ++kt) {
-        {
-            uint4 hi, lo;
-            split8<BF16>(ra, hi, lo);
-            *reinterpret_cast<uint4*>(Ah + vv * RP + ta * 8) = hi;
-            *reinterpret_cast<uint4*>(Al + vv * RP + ta * 8) = lo;
-        }
-#pragma unroll
-        for (int u = 0; u < B_U; ++u) {
-            const int q = tid + u * 256;
-            const int plane = q / (NTB * BN), rr = q % (NTB * BN);
-            const int tb = rr / BN, col = rr % BN;
-            *reinterpret_cast<u32x4*>((plane ? Bl : Bh) + col * RP + tb * 8) = rb[u];
-        }
-        __syncthreads();
-        if (kt + 1 < nk) gload(kt + 1);
-        const int row = w * 32 + (lane & 31), tb = lane >> 5;
-        const uint4 ah = *reinterpret_cast<const uint4*>(Ah + row * RP + tb * 8);
-        const uint4 al = *reinterpret_cast<const uint4*>(Al + row * RP + tb * 8);
-#pragma unroll
-        for (int n = 0; n < NBLK; ++n) {
-            const int col = n * 32 + (lane & 31);
-            const uint4 bh = *reinterpret_cast<const uint4*>(Bh + col * RP + tb * 8);
-            const uint4 bl = *reinterpret_cast<const uint4*>(Bl + col * RP + tb * 8);
-            acc[n] = mfma16<BF16>(al, bh, acc[n]);
-            acc[n] = mfma16<BF16>(ah, bl, acc[n]);
-            acc[n] = mfma16<BF16>(ah, bh, acc[n]);
-        }
-        __syncthreads();
-    }
-    local_finish<NBLK>(P, acc, reinterpret_cast<float*>(stage), scores, kslot, col0, ca, cb, tid, rows, Eloc, best,
-                       best_score, need_full, center, !BF16);
-}
-
 __global__ void k_fill_i32(int32_t* __restrict__ p, int64_t n, int32_t val) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = val;
@@ -1608,11 +1238,12 @@ __global__ void k_zero_rows(float* __restrict__ p, int64_t ld, int r0, int r1, i
 }
 
 // Compaction of flagged voxels (order within the list is not significant).
-__global__ void k_scatter_best(const int32_t* __restrict__ idx, int n, const int32_t* __restrict__ best_sub,
-                               const float* __restrict__ score_sub, int32_t* __restrict__ best,
-                               float* __restrict__ score) {
+// n_live (nullable, device) bounds n.
+__global__ void k_scatter_best(const int32_t* __restrict__ idx, int n, const int32_t* __restrict__ n_live,
+                               const int32_t* __restrict__ best_sub, const float* __restrict__ score_sub,
+                               int32_t* __restrict__ best, float* __restrict__ score) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    if (i >= (n_live ? min(*n_live, n) : n)) return;
     best[idx[i]] = best_sub[i];
     score[idx[i]] = score_sub[i];
 }

@@ -44,16 +44,11 @@ struct Proj16Lds {
     static constexpr int BYTES = STAGE_BYTES > PATCH_BYTES ? STAGE_BYTES : PATCH_BYTES;
 };
 
-// One (256-voxel block, column tile) of the split projection.  Full grid
-// (BAND = false): column tile `tile` of the plan, S[v * ldS + k].  Band
-// (BAND = true): the tile's npt pairs are kp[0..npt) (-1 = empty), gathered
-// column by column from their tiles of the same operand, and the scores go to
-// S[v * ldS + slot0 + q] for the pair in slot q.  The arithmetic of every
-// output column is the same in both cases.
-template <int BN, bool BF16, bool BAND>
+// One (256-voxel block, column tile) of the split projection: S[v * ldS + k].
+template <int BN, bool BF16>
 __device__ __forceinline__ void proj16_item(const PlanDev& P, const float* __restrict__ Yt, int64_t ldY, int V,
                                             float* __restrict__ S, int64_t ldS, int vt, int tile,
-                                            const int32_t* __restrict__ kp, int slot0, unsigned char* lds_raw) {
+                                            unsigned char* lds_raw) {
     using L = Proj16Lds<BN>;
     constexpr int BM = kProj16BM, NBLK = BN / 32, NTB = L::NTB, RP = L::RP, C_LD = L::C_LD;
     constexpr int B_N = 2 * NTB * BN;  // 16-byte B units per stage
@@ -74,23 +69,10 @@ __device__ __forceinline__ void proj16_item(const PlanDev& P, const float* __res
     const int a_tb = lane >> 5;
     // source offset (halves, stage 0) of each B unit this thread stages
     size_t ub[B_U];
-    bool uok[B_U];
 #pragma unroll
     for (int u = 0; u < B_U; ++u) {
-        const int q = tid + u * 256;
-        const int rr = q % (NTB * BN);
-        const int tb = rr / BN, col = rr % BN;
-        int st = tile, cc = col;
-        bool ok = true;
-        if (BAND) {
-            const int slot = col / P.p, j = col - slot * P.p;
-            const int k = slot < P.npt ? kp[slot] : -1;
-            ok = k >= 0;
-            st = ok ? k / P.npt : 0;
-            cc = ok ? (k - st * P.npt) * P.p + j : 0;
-        }
-        ub[u] = (size_t)st * P.Tpad * BN + ((size_t)tb * BN + cc) * 8;
-        uok[u] = ok;
+        const int rr = (tid + u * 256) % (NTB * BN);
+        ub[u] = (size_t)tile * P.Tpad * BN + ((size_t)(rr / BN) * BN + rr % BN) * 8;
     }
 
     u32x4 ah[2][2], al[2][2];  // [row block][k-step of the stage]
@@ -109,8 +91,7 @@ __device__ __forceinline__ void proj16_item(const PlanDev& P, const float* __res
 #pragma unroll
         for (int u = 0; u < B_U; ++u) {
             const int plane = (tid + u * 256) / (NTB * BN);
-            const u32x4 x = *reinterpret_cast<const u32x4*>((plane ? gl : gh) + ub[u] + (size_t)kt * NTB * BN * 8);
-            rb[u] = uok[u] ? x : u32x4{0u, 0u, 0u, 0u};
+            rb[u] = *reinterpret_cast<const u32x4*>((plane ? gl : gh) + ub[u] + (size_t)kt * NTB * BN * 8);
         }
     };
 
@@ -188,9 +169,9 @@ __device__ __forceinline__ void proj16_item(const PlanDev& P, const float* __res
         __syncthreads();
         for (int it = lane; it < 32 * npt; it += 64) {
             const int row = it / npt, qq = it % npt;
-            const int k = BAND ? kp[qq] : tile * npt + qq;
+            const int k = tile * npt + qq;
             const int v = v0 + 64 * w + 32 * r + row;
-            if (k < 0 || k >= K || v >= V) continue;
+            if (k >= K || v >= V) continue;
             const float* cr = Cw + row * C_LD + qq * p;
             double pp = 0.0;
             for (int j = 0; j < p; ++j) {
@@ -198,7 +179,7 @@ __device__ __forceinline__ void proj16_item(const PlanDev& P, const float* __res
                 pp = fma(cv, cv, pp);
             }
             if (!BF16) pp *= (double)P.g16_unscale[k];
-            float* sp = S + (size_t)v * ldS + (BAND ? slot0 + qq : k);
+            float* sp = S + (size_t)v * ldS + k;
             const double rss = (double)(*sp) - pp;
             *sp = (float)fma(tp, log(rss + 1e-10), P.ld[k]);
         }
@@ -208,11 +189,13 @@ __device__ __forceinline__ void proj16_item(const PlanDev& P, const float* __res
 
 template <int BN, bool BF16>
 __global__ void __launch_bounds__(256) k_project16b(PlanDev P, const float* __restrict__ Yt, int64_t ldY, int V,
-                                                    float* __restrict__ S, int64_t ldS) {
+                                                    float* __restrict__ S, int64_t ldS,
+                                                    const int32_t* __restrict__ n_live) {
     __shared__ __attribute__((aligned(16))) unsigned char lds_raw[Proj16Lds<BN>::BYTES];
     int vt, tile;
     if (!xcd_tile_map(blockIdx.x, gridDim.x, P.n_tiles, &vt, &tile)) return;
-    proj16_item<BN, BF16, false>(P, Yt, ldY, V, S, ldS, vt, tile, nullptr, 0, lds_raw);
+    if (n_live && vt * kProj16BM >= *n_live) return;  // past the live voxels
+    proj16_item<BN, BF16>(P, Yt, ldY, V, S, ldS, vt, tile, lds_raw);
 }
 
 }  // namespace remlk

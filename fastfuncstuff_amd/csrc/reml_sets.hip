@@ -1,0 +1,424 @@
+// Pair-set search: voxel groups scored on a set of pairs with the arithmetic of
+// the full grid.  Included by reml_api.hip after reml_proj16.hip (one
+// translation unit).
+//
+// Two families of sets use it:
+//   windows  the 5 x 5 lattice window around a voxel's proposal (learned path);
+//   bands    the pairs whose lag-1 correlation
+//            lambda = (1 + ab)(a + b) / (1 + 2ab + b^2) lies near the
+//            proposal's.  The REML surface of a weakly correlated series is a
+//            long valley along a lambda contour: the proposal network places
+//            such a voxel's lambda well but not its position along the valley.
+// Voxels are grouped by set (groups padded to 64 columns of a time-major
+// residual buffer) and every pair of their set is scored so that each score
+// equals the full-grid surface entry bitwise:
+//   energy      k_set_energy: the k_energy recurrence and accumulation
+//               schedule, 8 slots per wave; the AR-filtered sample is formed per
+//               pair, which for pairs sharing a is the value k_energy shares;
+//   projection  k_set_project: one wave per 8-pair tile of the set, the same
+//               MFMA sequence per output column as k_project (fp32) or
+//               k_project16b (split 16-bit), B gathered column by column from
+//               the full-grid operand;
+//   argmin      k_set_select: first minimum in pair order, lambda == 0 pairs
+//               mapped to the first of them (the global first-minimum rule).
+#pragma once
+
+#include "reml_kernels.h"
+
+namespace remlk {
+
+// Interleaved per-chunk energy coefficients of a set table:
+// coef[chunk][t][3 j + (0, 1, 2)] = (c_t, alpha_t, 1/v_{t+1}) of slot j, the
+// same fp32 roundings as the full-grid chunk tables; s0 = 1/v_0.
+__global__ void k_set_tables(PlanDev P, SetTable ST, float* __restrict__ coef, float* __restrict__ s0) {
+    const int MC = ST.sb / kEnergyNB;
+    const int cb = blockIdx.x;
+    if (cb >= ST.n * MC) return;
+    const int set = cb / MC, chunk = cb % MC;
+    const int T = P.T;
+    const int32_t* slots = ST.pairs + (size_t)set * ST.sb + chunk * kEnergyNB;
+    for (int t = threadIdx.x; t < T; t += blockDim.x) {
+        float* o = coef + ((size_t)cb * T + t) * 3 * kEnergyNB;
+#pragma unroll
+        for (int j = 0; j < kEnergyNB; ++j) {
+            const int k = slots[j];
+            float c = 0.f, al = 0.f, sn = 0.f;
+            if (k >= 0) {
+                const float2 q = P.lp[(size_t)k * T + t];
+                c = P.lc[(size_t)k * T + t];
+                al = q.x;
+                sn = q.y;
+            }
+            o[3 * j] = c;
+            o[3 * j + 1] = al;
+            o[3 * j + 2] = sn;
+        }
+    }
+    if (threadIdx.x < kEnergyNB) {
+        const int k = slots[threadIdx.x];
+        s0[(size_t)set * ST.sb + chunk * kEnergyNB + threadIdx.x] = k >= 0 ? P.ls0[k] : 0.f;
+    }
+}
+
+__device__ inline float axis_clamp(const float* axis, int n, float x) {
+    float lo = axis[0], hi = axis[0];
+    for (int i = 1; i < n; ++i) {
+        lo = fminf(lo, axis[i]);
+        hi = fmaxf(hi, axis[i]);
+    }
+    return fminf(fmaxf(x, lo), hi);
+}
+
+__device__ inline int nearest_index(const float* axis, int n, float x) {
+    int best = 0;
+    float bd = fabsf(axis[0] - x);
+    for (int i = 1; i < n; ++i) {
+        const float d = fabsf(axis[i] - x);
+        if (d < bd) {
+            bd = d;
+            best = i;
+        }
+    }
+    return best;
+}
+
+// Window centre kept inside the lattice.
+__device__ inline int clamp_center(const PlanDev& P, int ai, int bi) {
+    if (P.n_a >= kWin) ai = min(max(ai, kWinHalf), P.n_a - 1 - kWinHalf);
+    if (P.n_b >= kWin) bi = min(max(bi, kWinHalf), P.n_b - 1 - kWinHalf);
+    return ai * P.n_b + bi;
+}
+
+// Window of each voxel from its proposal (a_hat, b_hat, c): the window around
+// the nearest lattice point, kept inside the lattice.  Voxels whose confidence
+// c < conf_min or whose proposal is not finite get need = 2 and no window
+// (key -1); the others need = 0.
+__global__ void k_win_key(PlanDev P, const float* __restrict__ props, int V, float conf_min,
+                          int32_t* __restrict__ key, int32_t* __restrict__ need) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= V) return;
+    const float ah = props[3 * v], bh = props[3 * v + 1], c = props[3 * v + 2];
+    const bool finite = isfinite(ah) && isfinite(bh) && isfinite(c);
+    if (!finite || !(c >= conf_min)) {
+        key[v] = -1;
+        need[v] = 2;
+        return;
+    }
+    key[v] = clamp_center(P, nearest_index(P.a_axis, P.n_a, ah), nearest_index(P.b_axis, P.n_b, bh));
+    need[v] = 0;
+}
+
+// Band of each voxel still flagged (need[v] != 0) whose proposal is finite:
+// lambda of the proposal clamped to the lattice box, nearest band centre; -1
+// for every other voxel and for empty bands.
+__global__ void k_band_key(PlanDev P, const float* __restrict__ props, int V, const int32_t* __restrict__ need,
+                           int32_t* __restrict__ key) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= V) return;
+    int out = -1;
+    if (need[v] != 0) {
+        const float ah = props[3 * v], bh = props[3 * v + 1];
+        if (isfinite(ah) && isfinite(bh)) {
+            const double a = axis_clamp(P.a_axis, P.n_a, ah), b = axis_clamp(P.b_axis, P.n_b, bh);
+            const double den = 1.0 + 2.0 * a * b + b * b;
+            double lam = den > 0.0 ? (1.0 + a * b) * (a + b) / den : 0.0;
+            lam = fmin(fmax(lam, -1.0), 1.0);
+            int bin = (int)rint((lam - (double)P.band_lo) / (double)P.band_h);
+            bin = min(max(bin, 0), P.band.n - 1);
+            if (P.band.cnt[bin] > 0) out = bin;
+        }
+    }
+    key[v] = out;
+}
+
+// Work items of a set pass are split into 8 contiguous tile ranges, one per
+// XCD (block id % 8), whose blocks stride over them: the items of one voxel
+// tile share that XCD's L2.
+struct SetItems {
+    int lo, n, lb, nlb;
+};
+__device__ inline SetItems set_items(int n_tiles, int per_tile) {
+    SetItems r;
+    const int xcd = blockIdx.x & 7;
+    const int per = (n_tiles + 7) >> 3;
+    r.lo = xcd * per;
+    r.n = max(0, min(n_tiles, r.lo + per) - r.lo) * per_tile;
+    r.lb = blockIdx.x >> 3;
+    r.nlb = gridDim.x >> 3;
+    return r;
+}
+
+// Energies E[col][slot] of every live 64-column tile and every 8-slot chunk of
+// its set (one chunk per wave).
+__global__ void __launch_bounds__(64 * kSetWaves) k_set_energy(PlanDev P, SetTable ST, const float* __restrict__ Yt,
+                                                               int64_t ldY, const int32_t* __restrict__ tset,
+                                                               const int32_t* __restrict__ n_tiles,
+                                                               float* __restrict__ Eb) {
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int SB = ST.sb, MC = SB / kEnergyNB;
+    const int n_cg = (MC + kSetWaves - 1) / kSetWaves;
+    const SetItems I = set_items(*n_tiles, n_cg);
+    const int T = P.T;
+    constexpr int U = 8;  // steps per fp64 flush
+    const int nsteps = T - 1;
+    const int nfull = nsteps / U * U;
+    for (int it = I.lb; it < I.n; it += I.nlb) {
+        const int tt = I.lo + it / n_cg;
+        const int chunk = (it % n_cg) * kSetWaves + w;
+        const int set = tset[tt];
+        if (chunk * kEnergyNB >= ST.cnt[set]) continue;  // past the set
+        const int col = tt * kSetTile + lane;
+        const float* __restrict__ y = Yt + col;
+        const float* __restrict__ co = ST.coef + ((size_t)set * MC + chunk) * T * 3 * kEnergyNB;
+        const float* __restrict__ s0 = ST.s0 + (size_t)set * SB + chunk * kEnergyNB;
+        float e[kEnergyNB], part[kEnergyNB];
+        double acc[kEnergyNB];
+        const float y0 = y[0];
+#pragma unroll
+        for (int j = 0; j < kEnergyNB; ++j) {
+            e[j] = y0;
+            part[j] = y0 * y0 * s0[j];
+            acc[j] = 0.0;
+        }
+        float yc = y0;
+        auto step = [&](float yn, const float* c) __attribute__((always_inline)) {
+#pragma unroll
+            for (int j = 0; j < kEnergyNB; ++j) {
+                const float z = fmaf(-c[3 * j], yc, yn);
+                e[j] = fmaf(c[3 * j + 1], e[j], z);
+                part[j] = fmaf(e[j] * c[3 * j + 2], e[j], part[j]);
+            }
+            yc = yn;
+        };
+        float ybuf[U];
+#pragma unroll
+        for (int i = 0; i < U; ++i) ybuf[i] = y[(size_t)min(1 + i, T - 1) * ldY];
+        for (int t0 = 0; t0 < nfull; t0 += U) {
+            float ycur[U];
+#pragma unroll
+            for (int i = 0; i < U; ++i) ycur[i] = ybuf[i];
+#pragma unroll
+            for (int i = 0; i < U; ++i) ybuf[i] = y[(size_t)min(t0 + U + 1 + i, T - 1) * ldY];
+#pragma unroll
+            for (int i = 0; i < U; ++i) {
+                float cg[3 * kEnergyNB];
+#pragma unroll
+                for (int q = 0; q < 3 * kEnergyNB; ++q) cg[q] = co[(size_t)(t0 + i) * 3 * kEnergyNB + q];
+                step(ycur[i], cg);
+            }
+#pragma unroll
+            for (int j = 0; j < kEnergyNB; ++j) {
+                acc[j] += part[j];
+                part[j] = 0.f;
+            }
+        }
+        for (int t = nfull; t < nsteps; ++t) step(y[(size_t)(t + 1) * ldY], co + (size_t)t * 3 * kEnergyNB);
+        float* out = Eb + (size_t)col * SB + chunk * kEnergyNB;
+#pragma unroll
+        for (int j = 0; j < kEnergyNB; ++j) out[j] = (float)(acc[j] + part[j]);
+    }
+}
+
+// Projection + score of every live 64-column tile over its set: items
+// (tile, quad of 4 set tiles); wave w takes set tile 4 quad + w (npt pairs,
+// BN columns) for all 64 columns.  MODE 0: fp32 MFMA 32x32x2 over the fp32
+// residual rows and the fp32 operand (k_project's k-order); MODE 1 / 2: split
+// bf16 / fp16 halves on 32x32x16 (k_project16b's order: per 16-sample k-step
+// al*bh, ah*bl, ah*bh).  Scores go over the energies in Sb[col][slot].
+template <int BN, int MODE>
+__global__ void __launch_bounds__(64 * kSetWaves) k_set_project(PlanDev P, SetTable ST, const float* __restrict__ Yt,
+                                                                int64_t ldY, const int32_t* __restrict__ tset,
+                                                                const int32_t* __restrict__ n_tiles,
+                                                                float* __restrict__ Sb) {
+    constexpr int NBLK = BN / 32;
+    constexpr int C_LD = BN + 1;
+    __shared__ float patch[kSetWaves][32 * C_LD];
+    const int lane = threadIdx.x & 63;
+    const int w = threadIdx.x >> 6;
+    const int SB = ST.sb, npt = P.npt, p = P.p;
+    const int QT = SB / (kSetWaves * npt);  // quads per set
+    const SetItems I = set_items(*n_tiles, QT);
+    const double tp = (double)(P.T - p);
+    float* Cw = patch[w];
+    for (int it = I.lb; it < I.n; it += I.nlb) {
+        const int tt = I.lo + it / QT, quad = it % QT;
+        const int set = tset[tt];
+        if (quad * kSetWaves * npt >= ST.cnt[set]) continue;  // block-uniform
+        const int stile = quad * kSetWaves + w;
+        const int32_t* __restrict__ kp = ST.pairs + (size_t)set * SB + stile * npt;
+        const int col0 = tt * kSetTile;
+        // this lane's B column in n-block n: source tile and column of its pair
+        int bst[NBLK], bcc[NBLK];
+        bool bok[NBLK];
+#pragma unroll
+        for (int n = 0; n < NBLK; ++n) {
+            const int c = n * 32 + (lane & 31);
+            const int q = c / p, j = c - q * p;
+            const int k = q < npt ? kp[q] : -1;
+            bok[n] = k >= 0;
+            bst[n] = k >= 0 ? k / npt : 0;
+            bcc[n] = k >= 0 ? (k - bst[n] * npt) * p + j : 0;
+        }
+        f32x16 acc[2][NBLK];
+#pragma unroll
+        for (int r = 0; r < 2; ++r)
+#pragma unroll
+            for (int n = 0; n < NBLK; ++n)
+#pragma unroll
+                for (int q = 0; q < 16; ++q) acc[r][n][q] = 0.f;
+        if (MODE == 0) {
+            const float* __restrict__ G = P.G32;
+            size_t boff[NBLK];
+#pragma unroll
+            for (int n = 0; n < NBLK; ++n) boff[n] = (size_t)bst[n] * P.Tpad * BN + bcc[n];
+            const int ar = lane >> 5;
+            const float* __restrict__ ya = Yt + col0 + (lane & 31);
+            for (int t0 = 0; t0 < P.Tpad; t0 += 8) {
+                float av[4][2], bv[4][NBLK];
+#pragma unroll
+                for (int s = 0; s < 4; ++s) {
+                    const int t = t0 + 2 * s + ar;
+#pragma unroll
+                    for (int r = 0; r < 2; ++r) av[s][r] = ya[(size_t)t * ldY + 32 * r];
+#pragma unroll
+                    for (int n = 0; n < NBLK; ++n) {
+                        const float x = G[boff[n] + (size_t)t * BN];
+                        bv[s][n] = bok[n] ? x : 0.f;
+                    }
+                }
+#pragma unroll
+                for (int s = 0; s < 4; ++s)
+#pragma unroll
+                    for (int n = 0; n < NBLK; ++n)
+#pragma unroll
+                        for (int r = 0; r < 2; ++r)
+                            acc[r][n] = __builtin_amdgcn_mfma_f32_32x32x2f32(av[s][r], bv[s][n], acc[r][n], 0, 0, 0);
+            }
+        } else {
+            constexpr bool BF16 = MODE == 1;
+            const uint16_t* __restrict__ gh = BF16 ? P.Gh_bf16 : P.Gh_f16;
+            const uint16_t* __restrict__ gl = BF16 ? P.Gl_bf16 : P.Gl_f16;
+            const uint16_t* __restrict__ Ash = reinterpret_cast<const uint16_t*>(Yt + (size_t)P.Tpad * ldY);
+            const uint16_t* __restrict__ Asl = Ash + (size_t)P.Tpad * ldY;
+            size_t boff[NBLK];
+#pragma unroll
+            for (int n = 0; n < NBLK; ++n) boff[n] = (size_t)bst[n] * P.Tpad * BN + (size_t)bcc[n] * 8;
+            const int nks = P.Tpad / 16;
+            auto load = [&](int ks, u32x4 (&ah)[2], u32x4 (&al)[2], u32x4 (&bh)[NBLK], u32x4 (&bl)[NBLK])
+                            __attribute__((always_inline)) {
+                const int tb = 2 * ks + (lane >> 5);
+#pragma unroll
+                for (int r = 0; r < 2; ++r) {
+                    const size_t o = ((size_t)tb * ldY + col0 + 32 * r + (lane & 31)) * 8;
+                    ah[r] = *reinterpret_cast<const u32x4*>(Ash + o);
+                    al[r] = *reinterpret_cast<const u32x4*>(Asl + o);
+                }
+#pragma unroll
+                for (int n = 0; n < NBLK; ++n) {
+                    const size_t o = boff[n] + (size_t)tb * BN * 8;
+                    const u32x4 xh = *reinterpret_cast<const u32x4*>(gh + o);
+                    const u32x4 xl = *reinterpret_cast<const u32x4*>(gl + o);
+                    bh[n] = bok[n] ? xh : u32x4{0u, 0u, 0u, 0u};
+                    bl[n] = bok[n] ? xl : u32x4{0u, 0u, 0u, 0u};
+                }
+            };
+            u32x4 ah[2], al[2], bh[NBLK], bl[NBLK];
+            load(0, ah, al, bh, bl);
+            for (int ks = 0; ks < nks; ++ks) {
+                u32x4 ahc[2] = {ah[0], ah[1]}, alc[2] = {al[0], al[1]}, bhc[NBLK], blc[NBLK];
+#pragma unroll
+                for (int n = 0; n < NBLK; ++n) {
+                    bhc[n] = bh[n];
+                    blc[n] = bl[n];
+                }
+                if (ks + 1 < nks) load(ks + 1, ah, al, bh, bl);
+                // per accumulator: al*bh, then ah*bl, then ah*bh
+#pragma unroll
+                for (int n = 0; n < NBLK; ++n)
+#pragma unroll
+                    for (int r = 0; r < 2; ++r) acc[r][n] = mfma16u<BF16>(alc[r], bhc[n], acc[r][n]);
+#pragma unroll
+                for (int n = 0; n < NBLK; ++n)
+#pragma unroll
+                    for (int r = 0; r < 2; ++r) acc[r][n] = mfma16u<BF16>(ahc[r], blc[n], acc[r][n]);
+#pragma unroll
+                for (int n = 0; n < NBLK; ++n)
+#pragma unroll
+                    for (int r = 0; r < 2; ++r) acc[r][n] = mfma16u<BF16>(ahc[r], bhc[n], acc[r][n]);
+            }
+        }
+        // scores, one 32-row block at a time through the wave's patch
+#pragma unroll
+        for (int r = 0; r < 2; ++r) {
+            __syncthreads();  // the patch is free
+#pragma unroll
+            for (int n = 0; n < NBLK; ++n)
+#pragma unroll
+                for (int q = 0; q < 16; ++q) {
+                    const int row = 8 * (q >> 2) + 4 * (lane >> 5) + (q & 3);
+                    Cw[row * C_LD + n * 32 + (lane & 31)] = acc[r][n][q];
+                }
+            __syncthreads();
+            for (int i = lane; i < 32 * npt; i += 64) {
+                const int row = i / npt, qq = i % npt;
+                const int k = kp[qq];
+                if (k < 0) continue;
+                const float* cr = Cw + row * C_LD + qq * p;
+                double pp = 0.0;
+                for (int j = 0; j < p; ++j) {
+                    const double cv = (double)cr[j];
+                    pp = fma(cv, cv, pp);
+                }
+                if (MODE == 2) pp *= (double)P.g16_unscale[k];
+                float* sp = Sb + (size_t)(col0 + 32 * r + row) * SB + stile * npt + qq;
+                const double rss = (double)(*sp) - pp;
+                *sp = (float)fma(tp, log(rss + 1e-10), P.ld[k]);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// First minimum over the set's slots (pair order), lambda == 0 pairs mapped
+// to the first of them; need = 1 when that pair's edge flag is set or no slot
+// has a finite score, else 0; next_key (nullable) = the window to continue
+// with (edge) or -1.
+__global__ void k_set_select(PlanDev P, SetTable ST, const int32_t* __restrict__ rows,
+                             const int32_t* __restrict__ tset, const int32_t* __restrict__ n_tiles,
+                             const float* __restrict__ Sb, int32_t* __restrict__ best, float* __restrict__ best_score,
+                             int32_t* __restrict__ need, int32_t* __restrict__ next_key) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= *n_tiles * kSetTile) return;
+    const int v = rows[c];
+    if (v < 0) return;
+    const int set = tset[c / kSetTile];
+    const int SB = ST.sb, nb = ST.cnt[set];
+    const float* sr = Sb + (size_t)c * SB;
+    float bs = INFINITY;
+    int bslot = -1;
+    for (int s = 0; s < nb; ++s) {
+        const float sc = sr[s];
+        if (sc < bs) {
+            bs = sc;
+            bslot = s;
+        }
+    }
+    if (bslot < 0) {
+        best[v] = -1;
+        best_score[v] = INFINITY;
+        need[v] = 1;
+        if (next_key) next_key[v] = -1;
+        return;
+    }
+    const size_t o = (size_t)set * SB + bslot;
+    int k = ST.pairs[o];
+    const int edge = ST.edge[o];
+    if (P.white_flag[k]) k = P.white[0];
+    best[v] = k;
+    best_score[v] = bs;
+    need[v] = edge ? 1 : 0;
+    if (next_key) next_key[v] = edge && ST.next ? ST.next[o] : -1;
+}
+
+}  // namespace remlk

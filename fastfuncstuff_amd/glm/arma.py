@@ -344,7 +344,7 @@ class RemlGrid(Mapping):
     def set_bands(self, half_width: float = 0.14, bin_width: float = 0.02):
         """Lag-1 correlation bands for the learned search: band i holds the
         pairs whose lambda lies within ``half_width`` of ``-1 + i * bin_width``.
-        ``half_width <= 0`` removes them.  Needs p <= 32."""
+        ``half_width <= 0`` removes them."""
         lib = _lib.load()
         _lib.check(lib.ffs_reml_plan_set_bands(self._handle, float(half_width), float(bin_width),
                                                _lib.stream_handle(self.device)), "set_bands")

@@ -187,9 +187,13 @@ class LocalSearcher:
         self.idx = torch.empty(V, dtype=torch.int32, device=dev)
         self.count = torch.zeros(1, dtype=torch.int32, device=dev)
         self.bws = None
-        if grid.bands is not None and grid.projection != "fp32":
+        if grid.bands is not None:
             _lib.check(lib.ffs_reml_band_workspace(grid.handle, V, ctypes.byref(nb)), "band_workspace")
             self.bws = torch.zeros(int(nb.value), dtype=torch.uint8, device=dev)
+        # full-grid voxels handled without a host round trip (more take a second pass)
+        self.cap = max(1, min(V, max(4096, V // 16) if self.bws is not None else max(4096, V // 3)))
+        _lib.check(lib.ffs_reml_subset_workspace(grid.handle, self.cap, ctypes.byref(nb)), "subset_workspace")
+        self.sws = torch.empty(int(nb.value), dtype=torch.uint8, device=dev)
 
     def run(self, Y: torch.Tensor, timer: "StageTimer | None" = None, walk: int = 1,
             stats: dict | None = None, band: bool = True) -> LocalResult:
@@ -255,25 +259,30 @@ class LocalSearcher:
             mark("band_project")
             _lib.check(lib.ffs_reml_band_project(grid.handle, V, _lib.ptr(self.best), _lib.ptr(self.score),
                                                  _lib.ptr(self.need), bws, bnb, st), "band_project")
-        mark("fallback_prep")
+        mark("fallback")
         _lib.check(lib.ffs_reml_compact(_lib.ptr(self.need), V, _lib.ptr(self.idx), _lib.ptr(self.count), st),
                    "compact")
+        # full grid for the first `cap` flagged voxels without reading the count
+        # on the host; the count is read once everything is queued
+        _lib.check(lib.ffs_reml_subset_search(grid.handle, _lib.ptr(Y), Y.stride(0), V, _lib.ptr(self.idx),
+                                              _lib.ptr(self.count), self.cap, _lib.ptr(self.best),
+                                              _lib.ptr(self.score), _lib.ptr(self.scale), _lib.ptr(self.sws),
+                                              self.sws.numel(), st), "subset_search")
         n_fb = int(self.count.item())
-        if n_fb:
-            rows = self.idx[:n_fb]
+        if n_fb > self.cap:
+            mark("fallback_overflow")
+            rows = self.idx[self.cap:n_fb]
+            n_rest = n_fb - self.cap
             Yt, _, _ = grid.prep(Y, rows=rows, scale=self.scale)
-            S = torch.empty(n_fb, grid.K, dtype=torch.float32, device=dev)
-            mark("fallback_energy")
-            _lib.check(lib.ffs_reml_energy(grid.handle, _lib.ptr(Yt), Yt.stride(0), n_fb, _lib.ptr(S), S.stride(0),
-                                           st), "energy")
-            mark("fallback_project")
-            _lib.check(lib.ffs_reml_project(grid.handle, _lib.ptr(Yt), Yt.stride(0), n_fb, _lib.ptr(S), S.stride(0),
-                                            st), "project")
+            S = torch.empty(n_rest, grid.K, dtype=torch.float32, device=dev)
+            _lib.check(lib.ffs_reml_energy(grid.handle, _lib.ptr(Yt), Yt.stride(0), n_rest, _lib.ptr(S),
+                                           S.stride(0), st), "energy")
+            _lib.check(lib.ffs_reml_project(grid.handle, _lib.ptr(Yt), Yt.stride(0), n_rest, _lib.ptr(S),
+                                            S.stride(0), st), "project")
             del Yt
-            mark("fallback_select")
             b_fb, s_fb = grid.select(S, hierarchical=False)
             del S
-            _lib.check(lib.ffs_reml_scatter_best(_lib.ptr(rows), n_fb, _lib.ptr(b_fb), _lib.ptr(s_fb),
+            _lib.check(lib.ffs_reml_scatter_best(_lib.ptr(rows), n_rest, _lib.ptr(b_fb), _lib.ptr(s_fb),
                                                  _lib.ptr(self.best), _lib.ptr(self.score), st), "scatter_best")
         return LocalResult(self.best, self.score, self.scale, n_fb, props)
 

@@ -167,9 +167,9 @@ int ffs_reml_ljung_box(const double* resid_w, int64_t ldR, int64_t V, int32_t n,
  * voxel: best pair (-1 if no valid pair), its score, need_full = 1 when the
  * optimum is on a window edge that is not a lattice edge, 2 when the confidence
  * is below conf_min or the proposal is not finite (no window is searched; best
- * and score are left as they were); scale = per-voxel std.  Requires p <= 16
- * (returns -4 otherwise).  workspace: ffs_reml_local_workspace bytes, first use
- * zero-filled.
+ * and score are left as they were); scale = per-voxel std.  The split
+ * projection modes need p <= 32 (returns -4 otherwise).  workspace:
+ * ffs_reml_local_workspace bytes for the plan's current projection mode.
  */
 int ffs_reml_local_workspace(const ffs_reml_plan* plan, int64_t V, int64_t* bytes);
 /* The three stages of ffs_reml_local_search (same workspace, same stream):
@@ -203,9 +203,10 @@ int ffs_reml_local_search(const ffs_reml_plan* plan, const float* Y, int64_t ldY
  * arithmetic, so its scores equal the full-grid surface entries bitwise; best
  * = first minimum in pair order (lambda == 0 pairs mapped to the first of
  * them), need_full = 1 when that pair has a lattice neighbour outside the band,
- * else 0.  Other voxels are left as they were.  Needs p <= 32 and a split
- * projection mode (returns -4 otherwise); scale = per-voxel std of the band
- * voxels.  half_width <= 0 removes the bands.
+ * else 0.  Other voxels are left as they were.  Returns -4 without bands, or
+ * in a split projection mode with p > 32; scale = per-voxel std of the band
+ * voxels.  half_width <= 0 removes the bands.  The workspace size depends on
+ * the projection mode.
  */
 int ffs_reml_plan_set_bands(ffs_reml_plan* plan, double half_width, double bin_width, void* stream);
 /* Host copies (synchronous; each pointer nullable): band count, slots per band,
@@ -224,6 +225,20 @@ int ffs_reml_band_project(const ffs_reml_plan* plan, int64_t V, int32_t* best, f
 int ffs_reml_band_search(const ffs_reml_plan* plan, const float* Y, int64_t ldY, int64_t V, const float* props,
                          int32_t* best, float* best_score, int32_t* need_full, float* scale, void* workspace,
                          int64_t ws_bytes, void* stream);
+
+/*
+ * Full-grid search of the voxels idx[0 .. min(*n_live, cap)) (idx and n_live
+ * device memory, e.g. from ffs_reml_compact) without the host reading the
+ * count: normalise + residualise, energies and projection over every pair in
+ * the plan's projection mode, first minimum over the grid (the exhaustive
+ * rule of glm/arma.py:1886-2018); writes best[idx[i]], best_score[idx[i]] and
+ * scale[idx[i]].  Voxels past cap are left untouched (the caller compares
+ * *n_live with cap afterwards).  workspace: ffs_reml_subset_workspace(cap) bytes.
+ */
+int ffs_reml_subset_workspace(const ffs_reml_plan* plan, int64_t cap, int64_t* bytes);
+int ffs_reml_subset_search(const ffs_reml_plan* plan, const float* Y, int64_t ldY, int64_t V, const int32_t* idx,
+                           const int32_t* n_live, int64_t cap, int32_t* best, float* best_score, float* scale,
+                           void* workspace, int64_t ws_bytes, void* stream);
 
 /* idx[0..*count) = voxels with flags[v] != 0 (order unspecified); count is device memory. */
 int ffs_reml_compact(const int32_t* flags, int64_t V, int32_t* idx, int32_t* count, void* stream);

@@ -4,8 +4,8 @@ window-edge voxels).
 * Band membership from the library equals a restatement here: pairs with
   |lambda_k - (-1 + i h)| <= w, lambda = (1 + ab)(a + b) / (1 + 2ab + b^2) in
   FP64; edge flag = a lattice 4-neighbour that is a pair outside the band.
-* Band scores are bitwise equal to the full-grid surface (split projection
-  modes), the band argmin is the first minimum of the surface restricted to the
+* Band scores are bitwise equal to the full-grid surface (each projection
+  mode), the band argmin is the first minimum of the surface restricted to the
   band (lambda == 0 pairs mapped to the first of them), need = the edge flag of
   that pair, and voxels not flagged (or with non-finite proposals) are left
   untouched.
@@ -88,7 +88,7 @@ def test_band_tables_match_restatement(dev):
     assert grid.band_info() is None
 
 
-@pytest.mark.parametrize("mode", ["fp16x3", "bf16x3"])
+@pytest.mark.parametrize("mode", ["fp32", "fp16x3", "bf16x3"])
 def test_band_scores_bitwise_equal_full_grid(dev, mode):
     from fastfuncstuff_amd import _lib
 
@@ -150,14 +150,13 @@ def test_band_scores_bitwise_equal_full_grid(dev, mode):
     assert n_band > V // 3
 
 
-def test_band_refused_without_split_projection(dev):
+def test_band_search_refused_without_bands(dev):
     from fastfuncstuff_amd import _lib
 
     grid, Y = _grid(dev, 64)
-    grid.set_bands(W, H)
     lib = _lib.load()
     rc = lib.ffs_reml_band_energy(grid.handle, 64, None, 0, _lib.stream_handle(dev))
-    assert rc == -4  # fp32 projection mode
+    assert rc == -4  # no bands on this plan
 
 
 def test_learned_search_with_bands(dev):
@@ -184,3 +183,44 @@ def test_learned_search_with_bands(dev):
           f"band voxels {stats.get('band')}", flush=True)
     assert a1 >= 0.99
     assert res1.n_fallback < res0.n_fallback
+
+
+@pytest.mark.parametrize("mode", ["fp32", "fp16x3"])
+def test_subset_search_matches_exhaustive(dev, mode):
+    """Full-grid search of a device-counted subset: the first `cap` listed
+    voxels get the exhaustive first minimum of the full surface (bitwise
+    score); the others are untouched."""
+    from fastfuncstuff_amd import _lib
+
+    print(f"[band] subset search mode={mode}", flush=True)
+    lib = _lib.load()
+    grid, Y = _grid(dev, 900)
+    grid.set_projection(mode)
+    V = Y.shape[0]
+    Yt, scale_ref, _ = grid.prep(Y)
+    S = grid.surface(Yt, V)
+    b_ref, s_ref = grid.select(S, "exhaustive")
+    b_ref, s_ref = b_ref.cpu(), s_ref.cpu()
+    g = torch.Generator().manual_seed(3)
+    idx = torch.randperm(V, generator=g)[:300].to(torch.int32)
+    cap = 256
+    idx_d = torch.full((V,), -5, dtype=torch.int32, device=dev)
+    idx_d[:300] = idx.to(dev)
+    count = torch.tensor([300], dtype=torch.int32, device=dev)
+    best = torch.full((V,), -7, dtype=torch.int32, device=dev)
+    score = torch.full((V,), 123.0, dtype=torch.float32, device=dev)
+    scale = torch.full((V,), -1.0, dtype=torch.float32, device=dev)
+    nb = ctypes.c_int64()
+    _lib.check(lib.ffs_reml_subset_workspace(grid.handle, cap, ctypes.byref(nb)), "ws")
+    ws = torch.empty(int(nb.value), dtype=torch.uint8, device=dev)
+    _lib.check(lib.ffs_reml_subset_search(grid.handle, _lib.ptr(Y), Y.stride(0), V, _lib.ptr(idx_d), _lib.ptr(count),
+                                          cap, _lib.ptr(best), _lib.ptr(score), _lib.ptr(scale), _lib.ptr(ws),
+                                          ws.numel(), _lib.stream_handle(dev)), "subset")
+    best, score, scale = best.cpu(), score.cpu(), scale.cpu()
+    done = idx[:cap].long()
+    assert torch.equal(best[done], b_ref[done])
+    assert torch.equal(score[done], s_ref[done])  # bitwise
+    assert torch.equal(scale[done], scale_ref.cpu()[done])
+    rest = torch.ones(V, dtype=torch.bool)
+    rest[done] = False
+    assert bool((best[rest] == -7).all()) and bool((score[rest] == 123.0).all())
