@@ -56,7 +56,8 @@ def parse():
     ap.add_argument("--band-width", type=float, default=0.14,
                     help="half-width of the lambda bands of the learned search (0: no band search)")
     ap.add_argument("--band-bin", type=float, default=0.02, help="spacing of the lambda band centres")
-    ap.add_argument("--walk", type=int, default=1, help="window steps after the first window")
+    ap.add_argument("--walk", type=int, default=None,
+                    help="window steps after the first window (default: 0 with bands, 1 without)")
     ap.add_argument("--workload", default="metric", choices=["metric", "learned_eval"],
                     help="metric: the headline line; learned_eval: learned path on a 100x100x50 volume with "
                          "spatially smooth ARMA parameters and 2%% t(3) voxels (T=600), its own JSON line")
@@ -336,7 +337,7 @@ def main():
                     "argmin_match_vs_full_grid_hierarchical_white_merged": agree_hier_c,
                     "full_grid_exhaustive_vs_hierarchical_white_merged": agree_exh_hier_c,
                     "speedup_vs_full_grid": fg_ms / lp_ms, "train_s": t_train, "train_steps": log.steps,
-                    "walk": args.walk, "window_voxels_per_pass": n_win,
+                    "walk": len(n_win) - 1, "window_voxels_per_pass": n_win,
                     "band": ({"half_width": grid.bands[0], "bin": grid.bands[1], "voxels": n_bv,
                               "pairs_per_voxel": n_bs / max(1, n_bv)} if grid.bands else None),
                     "train_loss_first_last": [log.losses[0][1], log.losses[-1][1]] if log.losses else None},

@@ -52,27 +52,48 @@ __device__ inline void layer2(const float* sp, const float* x, float* z2, float*
     }
 }
 
-// Inference: the weights are read straight from global memory at wave-uniform
-// addresses (scalar loads, used as scalar operands of the FMAs).
+// Inference, one sample per lane.  Layer-1 weights are staged transposed in
+// LDS (a unit's 10 inputs in one broadcast row); layer-2 and -3 weights are
+// read straight from global memory at wave-uniform addresses (scalar loads,
+// scalar FMA operands).  Same operation order as layer2 / z1_of.
+constexpr int W1T_LD = 12;  // padded row of the transposed layer-1 weights
 __global__ void __launch_bounds__(256) k_mlp_forward(const float* __restrict__ params, const float* __restrict__ X,
                                                      const int32_t* __restrict__ idx, int64_t B,
                                                      float* __restrict__ out) {
+    __shared__ float w1t[NH * W1T_LD];
+    for (int q = threadIdx.x; q < NH * NIN; q += blockDim.x) {
+        const int k = q / NH, i = q % NH;
+        w1t[i * W1T_LD + k] = params[OW1 + k * NH + i];
+    }
+    __syncthreads();
     const float* __restrict__ sp = params;
     const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= B) return;
     const int64_t r = idx ? idx[b] : b;
-    float x[NIN], z2[NH];
+    float x[NIN];
 #pragma unroll
     for (int i = 0; i < NIN; ++i) x[i] = X[r * NIN + i];
-    layer2(sp, x, z2, nullptr);
+    float z2[NH];
+#pragma unroll
+    for (int j = 0; j < NH; ++j) z2[j] = sp[OB2 + j];
+#pragma unroll 2
+    for (int i = 0; i < NH; ++i) {
+        const float* wr = w1t + i * W1T_LD;
+        float zs = sp[OB1 + i];
+#pragma unroll
+        for (int k = 0; k < NIN; ++k) zs = fmaf(x[k], wr[k], zs);
+        const float h = gelu(zs);
+#pragma unroll
+        for (int j = 0; j < NH; ++j) z2[j] = fmaf(h, sp[OW2 + i * NH + j], z2[j]);
+    }
     float o[NOUT];
 #pragma unroll
     for (int k = 0; k < NOUT; ++k) o[k] = sp[OB3 + k];
 #pragma unroll
     for (int j = 0; j < NH; ++j) {
-        const float h = gelu(z2[j]);
+        const float g = gelu(z2[j]);
 #pragma unroll
-        for (int k = 0; k < NOUT; ++k) o[k] = fmaf(h, sp[OW3 + j * NOUT + k], o[k]);
+        for (int k = 0; k < NOUT; ++k) o[k] = fmaf(g, sp[OW3 + j * NOUT + k], o[k]);
     }
 #pragma unroll
     for (int k = 0; k < NOUT; ++k) out[b * NOUT + k] = o[k];

@@ -455,7 +455,10 @@ __global__ void __launch_bounds__(kEnergyVox* kEnergyWaves) k_energy(PlanDev P, 
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     // (voxel tile, chunk group) of this block: the chunk groups of one voxel
     // tile run together on one XCD so its residual columns are fetched once
-    // into that XCD's L2 (same map as the projection, chunk group fastest)
+    // into that XCD's L2 (same map as the projection, chunk group fastest).
+    // With a live count the map covers the live voxels only (the grid is
+    // sized for V), so a small count still spreads over every XCD.
+    if (n_live) V = min(V, *n_live);
     const int n_xt = (V + kEnergyVox - 1) / kEnergyVox;
     const int n_yg = (P.n_chunks + kEnergyWaves - 1) / kEnergyWaves;
     const int bid = blockIdx.x, xcd = bid & 7, local = bid >> 3;
@@ -464,8 +467,6 @@ __global__ void __launch_bounds__(kEnergyVox* kEnergyWaves) k_energy(PlanDev P, 
     if (local / n_yg >= per || xt >= n_xt) return;
     const int chunk = (local % n_yg) * kEnergyWaves + w;
     if (chunk >= P.n_chunks) return;
-    if (n_live) V = min(V, *n_live);  // voxels past the live count are not needed
-    if (xt * kEnergyVox >= V) return;
     const int v = xt * kEnergyVox + lane;
     const int T = P.T;
     const float* __restrict__ y = Yt + v;

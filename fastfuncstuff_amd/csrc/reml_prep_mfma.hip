@@ -59,20 +59,36 @@ __global__ void __launch_bounds__(64) k_prep_mfma(PlanDev P, const float* __rest
 #pragma unroll
         for (int rt = 0; rt < NT; ++rt) acc[g][rt] = f64x4{0.0, 0.0, 0.0, 0.0};
     double s1[4] = {0.0, 0.0, 0.0, 0.0}, s2[4] = {0.0, 0.0, 0.0, 0.0};
-    for (int t0 = 0; t0 < Tpad; t0 += 16) {
+    // operands of one 16-sample step; the next step's are fetched before the
+    // current step's MFMAs (one wave per block: the loads must overlap)
+    auto load1 = [&](int t0, float (&yb)[4][4], double (&a)[4][NT]) __attribute__((always_inline)) {
         const bool full = vec4 && t0 + 16 <= T;
-        float yb[4][4];
 #pragma unroll
         for (int g = 0; g < 4; ++g) {
             load_span<4>(rowB[g], T, t0 + 4 * kq, full, yb[g]);
 #pragma unroll
             for (int m = 0; m < 4; ++m) yb[g][m] = vB[g] >= 0 ? yb[g][m] : 0.f;
         }
-        double a[4][NT];
 #pragma unroll
         for (int m = 0; m < 4; ++m)
 #pragma unroll
             for (int rt = 0; rt < NT; ++rt) a[m][rt] = Pp[(size_t)(t0 + 4 * kq + m) * PG + 16 * rt + j];
+    };
+    float ybn[4][4];
+    double an[4][NT];
+    load1(0, ybn, an);
+    for (int t0 = 0; t0 < Tpad; t0 += 16) {
+        float yb[4][4];
+        double a[4][NT];
+#pragma unroll
+        for (int g = 0; g < 4; ++g)
+#pragma unroll
+            for (int m = 0; m < 4; ++m) yb[g][m] = ybn[g][m];
+#pragma unroll
+        for (int m = 0; m < 4; ++m)
+#pragma unroll
+            for (int rt = 0; rt < NT; ++rt) a[m][rt] = an[m][rt];
+        if (t0 + 16 < Tpad) load1(t0 + 16, ybn, an);
 #pragma unroll
         for (int m = 0; m < 4; ++m)
 #pragma unroll
@@ -122,7 +138,30 @@ __global__ void __launch_bounds__(64) k_prep_mfma(PlanDev P, const float* __rest
         hist[k] = 0.f;
         acf[k] = 0.0;
     }
+    auto load2 = [&](int tt, double (&xa)[NT][4], float (&y4)[4][4]) __attribute__((always_inline)) {
+#pragma unroll
+        for (int rt = 0; rt < NT; ++rt)
+#pragma unroll
+            for (int kk = 0; kk < 4; ++kk) xa[rt][kk] = Xp[(size_t)(tt + arow) * PG + 16 * rt + kq + 4 * kk];
+        const bool full = vec4 && tt + 16 <= T;
+#pragma unroll
+        for (int g = 0; g < 4; ++g) load_span<4>(rowB[g], T, tt + 4 * kq, full, y4[g]);
+    };
+    double xan[NT][4];
+    float y4n[4][4];
+    load2(0, xan, y4n);
     for (int tt = 0; tt < Tpad; tt += 16) {
+        double xa[NT][4];
+        float y4c[4][4];
+#pragma unroll
+        for (int rt = 0; rt < NT; ++rt)
+#pragma unroll
+            for (int kk = 0; kk < 4; ++kk) xa[rt][kk] = xan[rt][kk];
+#pragma unroll
+        for (int g = 0; g < 4; ++g)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) y4c[g][r] = y4n[g][r];
+        if (tt + 16 < Tpad) load2(tt + 16, xan, y4n);
         f64x4 d[4];
 #pragma unroll
         for (int g = 0; g < 4; ++g) d[g] = f64x4{0.0, 0.0, 0.0, 0.0};
@@ -130,19 +169,16 @@ __global__ void __launch_bounds__(64) k_prep_mfma(PlanDev P, const float* __rest
         for (int rt = 0; rt < NT; ++rt)
 #pragma unroll
             for (int kk = 0; kk < 4; ++kk) {
-                const double a = Xp[(size_t)(tt + arow) * PG + 16 * rt + kq + 4 * kk];
+                const double a = xa[rt][kk];
 #pragma unroll
                 for (int g = 0; g < 4; ++g) d[g] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, acc[g][rt][kk], d[g], 0, 0, 0);
             }
-        const bool full = vec4 && tt + 16 <= T;
 #pragma unroll
         for (int g = 0; g < 4; ++g) {
-            float y4[4];
-            load_span<4>(rowB[g], T, tt + 4 * kq, full, y4);
             float r4[4];
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const double y = vB[g] >= 0 ? (double)y4[r] : 0.0;
+                const double y = vB[g] >= 0 ? (double)y4c[g][r] : 0.0;
                 const float rr = (float)((y - d[g][r]) * inv[g]);
                 r4[r] = rr;
                 const int t = tt + 4 * kq + r;

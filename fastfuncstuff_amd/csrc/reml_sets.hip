@@ -296,24 +296,38 @@ __global__ void __launch_bounds__(64 * kSetWaves) k_set_project(PlanDev P, SetTa
                             acc[r][n] = __builtin_amdgcn_mfma_f32_32x32x2f32(av[s][r], bv[s][n], acc[r][n], 0, 0, 0);
             }
         } else {
+            // A (the block's 64 residual columns) is shared by the 4 waves:
+            // staged through LDS per 32-sample stage (double-buffered); each
+            // wave's B columns come straight from the operand into registers,
+            // one stage ahead.
             constexpr bool BF16 = MODE == 1;
             const uint16_t* __restrict__ gh = BF16 ? P.Gh_bf16 : P.Gh_f16;
             const uint16_t* __restrict__ gl = BF16 ? P.Gl_bf16 : P.Gl_f16;
             const uint16_t* __restrict__ Ash = reinterpret_cast<const uint16_t*>(Yt + (size_t)P.Tpad * ldY);
             const uint16_t* __restrict__ Asl = Ash + (size_t)P.Tpad * ldY;
+            uint16_t* As = reinterpret_cast<uint16_t*>(&patch[0][0]);  // [buf][plane][tb][col][8]
+            constexpr int A_BUF = 2 * 4 * kSetTile * 8;                // halves per buffer
             size_t boff[NBLK];
 #pragma unroll
             for (int n = 0; n < NBLK; ++n) boff[n] = (size_t)bst[n] * P.Tpad * BN + (size_t)bcc[n] * 8;
-            const int nks = P.Tpad / 16;
-            auto load = [&](int ks, u32x4 (&ah)[2], u32x4 (&al)[2], u32x4 (&bh)[NBLK], u32x4 (&bl)[NBLK])
-                            __attribute__((always_inline)) {
-                const int tb = 2 * ks + (lane >> 5);
+            const int nst = P.Tpad / 32;
+            const int tid = threadIdx.x;
+            auto load_a = [&](int kt, u32x4 (&ra)[2]) __attribute__((always_inline)) {
 #pragma unroll
-                for (int r = 0; r < 2; ++r) {
-                    const size_t o = ((size_t)tb * ldY + col0 + 32 * r + (lane & 31)) * 8;
-                    ah[r] = *reinterpret_cast<const u32x4*>(Ash + o);
-                    al[r] = *reinterpret_cast<const u32x4*>(Asl + o);
+                for (int u = 0; u < 2; ++u) {
+                    const int q = tid + u * 256;  // plane = q / 256, t-block = (q / 64) % 4, column = q % 64
+                    const size_t o = ((size_t)(kt * 4 + ((q >> 6) & 3)) * ldY + col0 + (q & 63)) * 8;
+                    ra[u] = *reinterpret_cast<const u32x4*>(((q >> 8) ? Asl : Ash) + o);
                 }
+            };
+            auto store_a = [&](int buf, const u32x4 (&ra)[2]) __attribute__((always_inline)) {
+#pragma unroll
+                for (int u = 0; u < 2; ++u)
+                    *reinterpret_cast<u32x4*>(As + buf * A_BUF + (tid + u * 256) * 8) = ra[u];
+            };
+            // B of one 16-sample k-step (this lane's columns)
+            auto load_b = [&](int ks, u32x4 (&bh)[NBLK], u32x4 (&bl)[NBLK]) __attribute__((always_inline)) {
+                const int tb = 2 * ks + (lane >> 5);
 #pragma unroll
                 for (int n = 0; n < NBLK; ++n) {
                     const size_t o = boff[n] + (size_t)tb * BN * 8;
@@ -323,16 +337,35 @@ __global__ void __launch_bounds__(64 * kSetWaves) k_set_project(PlanDev P, SetTa
                     bl[n] = bok[n] ? xl : u32x4{0u, 0u, 0u, 0u};
                 }
             };
-            u32x4 ah[2], al[2], bh[NBLK], bl[NBLK];
-            load(0, ah, al, bh, bl);
+            __syncthreads();  // the patch (aliased by the A stages) is free
+            {
+                u32x4 ra[2];
+                load_a(0, ra);
+                store_a(0, ra);
+            }
+            u32x4 bh[NBLK], bl[NBLK];
+            load_b(0, bh, bl);
+            __syncthreads();
+            const int nks = 2 * nst;
+            u32x4 ra[2];
             for (int ks = 0; ks < nks; ++ks) {
-                u32x4 ahc[2] = {ah[0], ah[1]}, alc[2] = {al[0], al[1]}, bhc[NBLK], blc[NBLK];
+                const int buf = (ks >> 1) & 1, half = ks & 1;
+                if (half == 0 && ks + 2 < nks) load_a((ks >> 1) + 1, ra);  // next stage's A
+                u32x4 bhc[NBLK], blc[NBLK];
 #pragma unroll
                 for (int n = 0; n < NBLK; ++n) {
                     bhc[n] = bh[n];
                     blc[n] = bl[n];
                 }
-                if (ks + 1 < nks) load(ks + 1, ah, al, bh, bl);
+                if (ks + 1 < nks) load_b(ks + 1, bh, bl);
+                const int tb = 2 * half + (lane >> 5);
+                u32x4 ahc[2], alc[2];
+#pragma unroll
+                for (int r = 0; r < 2; ++r) {
+                    const int c = 32 * r + (lane & 31);
+                    ahc[r] = *reinterpret_cast<const u32x4*>(As + buf * A_BUF + ((0 * 4 + tb) * kSetTile + c) * 8);
+                    alc[r] = *reinterpret_cast<const u32x4*>(As + buf * A_BUF + ((1 * 4 + tb) * kSetTile + c) * 8);
+                }
                 // per accumulator: al*bh, then ah*bl, then ah*bh
 #pragma unroll
                 for (int n = 0; n < NBLK; ++n)
@@ -346,6 +379,10 @@ __global__ void __launch_bounds__(64 * kSetWaves) k_set_project(PlanDev P, SetTa
                 for (int n = 0; n < NBLK; ++n)
 #pragma unroll
                     for (int r = 0; r < 2; ++r) acc[r][n] = mfma16u<BF16>(ahc[r], bhc[n], acc[r][n]);
+                if (half == 1) {
+                    if (ks + 1 < nks) store_a(buf ^ 1, ra);
+                    __syncthreads();
+                }
             }
         }
         // scores, one 32-row block at a time through the wave's patch

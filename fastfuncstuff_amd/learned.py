@@ -195,7 +195,7 @@ class LocalSearcher:
         _lib.check(lib.ffs_reml_subset_workspace(grid.handle, self.cap, ctypes.byref(nb)), "subset_workspace")
         self.sws = torch.empty(int(nb.value), dtype=torch.uint8, device=dev)
 
-    def run(self, Y: torch.Tensor, timer: "StageTimer | None" = None, walk: int = 1,
+    def run(self, Y: torch.Tensor, timer: "StageTimer | None" = None, walk: int | None = None,
             stats: dict | None = None, band: bool = True) -> LocalResult:
         """Learned-proposal search of Y (V, T) float32 on the device.
 
@@ -205,10 +205,15 @@ class LocalSearcher:
         has lambda bands (``RemlGrid.set_bands``) and ``band`` is set: every
         pair whose lambda lies near the proposal's.  What remains (band optimum
         on the band's edge, non-finite proposals) takes the full grid.
-        ``timer`` (optional) records a device event at the start of every
-        stage; ``stats`` (optional, synchronises) receives the voxel count of
-        each pass ("window": first pass, "walk": list, "band")."""
+        ``walk`` defaults to 0 with bands (the band search handles window
+        edges better than further windows) and 1 without.  ``timer``
+        (optional) records a device event at the start of every stage;
+        ``stats`` (optional, synchronises) receives the voxel count of each
+        pass ("window": first pass, "walk": list, "band")."""
         grid, lib, dev = self.grid, _lib.load(), self.grid.device
+        use_band = band and self.bws is not None
+        if walk is None:
+            walk = 0 if use_band else 1
         V = Y.shape[0]
         if V != self.V:
             raise ValueError(f"LocalSearcher was sized for {self.V} voxels, got {V}")
@@ -245,7 +250,7 @@ class LocalSearcher:
             mark("walk_project")
             _lib.check(lib.ffs_reml_local_project(grid.handle, V, _lib.ptr(self.best), _lib.ptr(self.score),
                                                   _lib.ptr(self.need), ws, nb, st), "local_project")
-        if band and self.bws is not None:
+        if use_band:
             if stats is not None:
                 n_vox, n_slots = band_load(grid, props, self.need)
                 stats["band"], stats["band_slots"] = n_vox, n_slots
@@ -335,7 +340,7 @@ class StageTimer:
 
 
 def calibrate_conf_min(mlp: ProposalMLP, grid: RemlGrid, Y_val: torch.Tensor, best_val: torch.Tensor,
-                       target: float = 0.999, walk: int = 1) -> float:
+                       target: float = 0.999, walk: int | None = None) -> float:
     """Smallest-fallback confidence threshold whose local-search agreement with
     the global argmin on a validation set is >= target.  Sets mlp.conf_min."""
     mlp.conf_min = -math.inf
