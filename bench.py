@@ -266,21 +266,23 @@ def main():
     ls.run(Y, stats=wstats, walk=args.walk)  # voxel counts of the window passes (same data, untimed)
     n_win = [wstats["window"]] + list(wstats.get("walk", []))
     mul = 3.0 if split else 1.0
+    n_bv, n_bs = wstats.get("band", 0), wstats.get("band_slots", 0)
+    n_wv, n_ws = wstats["window"], wstats["window_slots"]
+    n_walk = sum(wstats.get("walk", []))
+    pairs = n_ws + 25 * n_walk + n_bs  # pair evaluations of the set passes (walk windows counted as 25)
+    vox = n_wv + n_walk + n_bv
+    launches = len(n_win) + (1 if n_bv else 0)
     kerns = {
         # name: (stages, launches per step, bound, algorithmic amount per launch (avg), formula)
-        "k_local_project16" if split else "k_local_project":
-            (("local_project", "walk_project"), len(n_win), "mfma",
-             mul * 2.0 * T * 25 * p * sum(n_win) / len(n_win),
-             f"{'3*' if split else ''}2*T*25*p per window voxel, voxels per pass {n_win}"),
+        "k_set_energy": (("local_energy", "walk_energy", "band_energy"), launches, "hbm",
+                         (4.0 * grid.Tpad * vox + 4.0 * pairs) / launches,
+                         f"4*Tpad per set voxel (residual read) + 4 per pair (energy write); {vox} voxels, "
+                         f"{pairs} pairs over {launches} launches (window, band)"),
+        "k_set_project": (("local_project", "walk_project", "band_project"), launches, "mfma",
+                          mul * 2.0 * T * p * pairs / launches,
+                          f"{'3*' if split else ''}2*T*p per pair evaluation; {pairs} pairs over {launches} launches"),
         "k_gls_mfma": (("gls",), 1, "hbm", 4.0 * V * T + 4.0 * V * (2 * p + 3), "4*V*T + 4*V*(2p+3)"),
     }
-    n_bv, n_bs = wstats.get("band", 0), wstats.get("band_slots", 0)
-    if n_bv:
-        kerns["k_band_energy"] = (("band_energy",), 1, "hbm", 4.0 * grid.Tpad * n_bv + 4.0 * n_bs,
-                                  f"4*Tpad*n_band (residual read) + 4*sum of band sizes (energy write); "
-                                  f"{n_bv} voxels, {n_bs} pairs")
-        kerns["k_band_project16"] = (("band_project",), 1, "mfma", mul * 2.0 * T * p * n_bs,
-                                     f"3*2*T*p*sum of band sizes ({n_bs} pairs)")
     totals = {n: sum(lp_k.get(st_, 0.0) for st_ in v[0]) for n, v in kerns.items()}
     dom = max(totals, key=totals.get)
     stages, launches, bound, amount, formula = kerns[dom]
@@ -295,11 +297,11 @@ def main():
         roof = {"kernel": dom, "bound": "hbm", "achieved": ach / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
                 "frac": ach / PEAK_HBM, "traffic": None, "avg_launch_ms": avg_ms,
                 "algorithmic": f"{formula} = {amount:.4g} B per launch"}
-        if dom in ("k_energy", "k_band_energy"):
-            # the recurrence is VALU-bound: 3 fp32 ops per (voxel, pair, step) + the
-            # AR term (shared by 8 pairs in k_energy, per pair in the band kernel);
+        if dom == "k_set_energy":
+            # the recurrence is VALU-bound: per (voxel, pair, step) 3 fp32 ops and
+            # the AR-filtered sample (formed per pair in the set kernel);
             # non-packed VALU issue limit of the chip
-            ops = 3.125 * T * K * n_fb if dom == "k_energy" else 4.0 * T * n_bs
+            ops = 4.0 * T * pairs / launches
             roof["valu"] = {"achieved_Tops": ops / (avg_ms / 1e3) / 1e12, "peak_Tops": PEAK_VALU_NONPACKED / 1e12,
                             "frac": ops / (avg_ms / 1e3) / PEAK_VALU_NONPACKED}
     roof["per_step_ms"] = {n: round(v, 4) for n, v in totals.items()}
@@ -315,7 +317,7 @@ def main():
     # fp32: 2*V*T*K*p flops on fp32 MFMA; split modes: three 16-bit products of that size.
     proj_flops = (3.0 if split else 1.0) * 2.0 * V * T * K * p
     proj_peak = PEAK_F16_MFMA if split else PEAK_FP32_MFMA
-    fg_roof = {"kernel": "k_project16" if split else "k_project", "bound": "mfma",
+    fg_roof = {"kernel": "k_project16b" if split else "k_project", "bound": "mfma",
                "achieved": proj_flops / (fg_k["project"] / 1e3) / 1e12, "peak": proj_peak / 1e12,
                "unit": "TFLOP/s", "frac": proj_flops / (fg_k["project"] / 1e3) / proj_peak,
                "algorithmic": f"{'3*' if split else ''}2*V*T*K*p = {proj_flops:.4g} flop per launch"}

@@ -233,7 +233,7 @@ class LocalSearcher:
                                               float(self.mlp.conf_min), _lib.ptr(self.need), _lib.ptr(self.scale),
                                               ws, nb, st), "local_prepare")
         if stats is not None:
-            stats["window"] = int((self.need != 2).sum())
+            stats["window"], stats["window_slots"] = window_load(grid, props, float(self.mlp.conf_min))
         mark("local_energy")
         _lib.check(lib.ffs_reml_local_energy(grid.handle, V, ws, nb, st), "local_energy")
         mark("local_project")
@@ -290,6 +290,30 @@ class LocalSearcher:
             _lib.check(lib.ffs_reml_scatter_best(_lib.ptr(rows), n_rest, _lib.ptr(b_fb), _lib.ptr(s_fb),
                                                  _lib.ptr(self.best), _lib.ptr(self.score), st), "scatter_best")
         return LocalResult(self.best, self.score, self.scale, n_fb, props)
+
+
+def window_load(grid: RemlGrid, props: torch.Tensor, conf_min: float) -> tuple[int, int]:
+    """(voxels, total pair evaluations) of the first window pass for these
+    proposals: the library's window choice (nearest lattice point, window kept
+    inside the lattice; confidence >= conf_min and finite proposal), restated."""
+    lut = torch.as_tensor(np.asarray(grid.lut))
+    n_a, n_b = lut.shape
+    valid = (lut >= 0).to(torch.int64)
+    half = 2
+    pad = torch.nn.functional.pad(valid[None, None].double(), (half, half, half, half))
+    counts = torch.nn.functional.avg_pool2d(pad, 2 * half + 1, stride=1)[0, 0] * (2 * half + 1) ** 2
+    counts = counts.round().long().to(props.device)  # valid pairs of the window centred at (ai, bi)
+    a_ax = torch.tensor(grid.a_axis, dtype=torch.float32, device=props.device)
+    b_ax = torch.tensor(grid.b_axis, dtype=torch.float32, device=props.device)
+    ok = torch.isfinite(props).all(dim=1) & (props[:, 2] >= conf_min)
+    ai = (props[:, 0:1] - a_ax[None]).abs().argmin(dim=1)
+    bi = (props[:, 1:2] - b_ax[None]).abs().argmin(dim=1)
+    if n_a >= 2 * half + 1:
+        ai = ai.clamp(half, n_a - 1 - half)
+    if n_b >= 2 * half + 1:
+        bi = bi.clamp(half, n_b - 1 - half)
+    n = counts[ai, bi]
+    return int(ok.sum()), int(n[ok].sum())
 
 
 def band_load(grid: RemlGrid, props: torch.Tensor, need: torch.Tensor) -> tuple[int, int]:
