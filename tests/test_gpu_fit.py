@@ -121,3 +121,33 @@ def test_grouped_fit_matches_per_group_oracle(dev):
         fit_glm_arma11_grouped(Y, {0: designs[0]}, torch.from_numpy(gidx), 2.0, device=dev)
     with pytest.raises(ValueError):
         fit_glm_arma11_grouped(Y, designs, torch.from_numpy(gidx[:-1]), 2.0, device=dev)
+
+
+def test_rvar_write_load_refit(dev, tmp_path):
+    """-Rvar round trip (glm/arma.py:6181-6454): the (a, b) written by
+    save_arma_rvar and read back by load_arma_params reproduce the fit when
+    passed as precomputed_arma_params; the -Rlklhd surface file holds the
+    fit's surface with one labelled sub-brick per pair."""
+    from fastfuncstuff_amd import outputs
+    from fastfuncstuff_amd.glm.arma import fit_glm_arma11
+
+    print("[fit] rvar round trip", flush=True)
+    rng = np.random.default_rng(8)
+    T, V = 90, 60
+    X = _design(rng, T)
+    Y = _data(rng, T, X, V)
+    ag = bg = torch.linspace(-0.8, 0.8, 17)
+    res = fit_glm_arma11(Y.reshape(3, 4, 5, T), X, 2.0, a_grid=ag, b_grid=bg, device=dev, want_ljung_box=True,
+                         save_profile_likelihoods=True)
+    f = outputs.save_arma_rvar(res, tmp_path / "rvar.nii.gz", volume_shape=(3, 4, 5))
+    img = outputs.read_nifti(f)
+    assert img.labels == ["a", "b", "lam", "StDev", "-LogLik", "LjungBox"]
+    assert img.stataux[5] == (6, (float(res.ljung_box_dof),))
+    ab = outputs.load_arma_params(f)
+    assert np.array_equal(ab, res.arma_params.numpy())
+    res2 = fit_glm_arma11(Y, X, 2.0, a_grid=ag, b_grid=bg, device=dev, precomputed_arma_params=ab)
+    b1, b2 = res.betas.reshape(V, -1).double(), res2.betas.double()
+    assert float(((b2 - b1).abs() / b1.abs().clamp(min=1e-3)).max()) <= 1e-6
+    lk = outputs.read_nifti(outputs.save_reml_likelihood_surface(res, tmp_path / "lk.nii", volume_shape=(3, 4, 5)))
+    assert len(lk.labels) == res.reml_lklhd_surface.shape[1]
+    assert np.array_equal(lk.data.reshape(V, -1), res.reml_lklhd_surface.float().numpy())
