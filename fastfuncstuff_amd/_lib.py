@@ -30,6 +30,7 @@ SIGNATURES: dict[str, list] = {
     "ffs_reml_plan_destroy": [_vp],
     "ffs_reml_dense_likelihood": [_vp, _vp, _c_int, _vp, _c_int, _c_int, _vp, _i64, _vp, _vp],
     "ffs_reml_plan_logdets": [_vp, _vp, _vp],
+    "ffs_reml_plan_ainv": [_vp, _vp],
     "ffs_reml_plan_dims": [_vp, _vp, _vp, _vp],
     "ffs_reml_plan_set_projection": [_vp, _c_int],
     "ffs_reml_prep": [_vp, _vp, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _c_int, _vp, _vp],

@@ -339,6 +339,12 @@ extern "C" int ffs_reml_plan_logdets(const ffs_reml_plan* P, double* logdet_R, d
     return 0;
 }
 
+extern "C" int ffs_reml_plan_ainv(const ffs_reml_plan* P, double* ainv) {
+    if (!P || !ainv) return fail(-3, "plan_ainv: null argument");
+    HIP_TRY(hipMemcpy(ainv, P->d.Ainv, sizeof(double) * P->d.K * P->d.p * P->d.p, hipMemcpyDeviceToHost));
+    return 0;
+}
+
 // Rows of the time-major buffer that ffs_reml_prep writes and ffs_reml_project
 // reads: Tpad, or 2 Tpad when the projection runs on pre-split halves.
 static int yt_rows(const ffs_reml_plan* P) { return (P->d.PG && P->d.proj_mode != 0) ? 2 * P->d.Tpad : P->d.Tpad; }

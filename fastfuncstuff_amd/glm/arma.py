@@ -366,6 +366,12 @@ class RemlGrid(Mapping):
                                                _lib.ptr(edge)), "band_info")
         return pairs, cnt, edge
 
+    def ainv(self) -> torch.Tensor:
+        """(X'R^-1X)^-1 per pair, (K, p, p) float64 on the CPU."""
+        out = torch.empty(self.K, self.p, self.p, dtype=torch.float64)
+        _lib.check(_lib.load().ffs_reml_plan_ainv(self._handle, _lib.ptr(out)), "plan_ainv")
+        return out
+
     SELECTORS = {"exhaustive": 0, "hierarchical": 1, "subset": 2}
 
     def select(self, S: torch.Tensor, hierarchical: bool | str):
@@ -733,7 +739,6 @@ class ARMA11Results:
 _UNSUPPORTED = {
     "dsort": "voxel-wise -dsort regressors",
     "column_sets": "-nuisance_summary column sets",
-    "glt_matrices": "in-loop GLT contrasts",
 }
 
 
@@ -833,6 +838,34 @@ def fit_glm_arma11(data, design, tr, a_grid=None, b_grid=None, estimate_per_voxe
         lj_tau = build_ljung_box_tau(T, run_starts, tau)
         res.ljung_box = torch.zeros(V, dtype=torch.float32)
         res.ljung_box_dof = max(1, _resolve_ljung_box_lag(T, lj_lag) - 2)
+    glt_labels, glt_matrices = kwargs.get("glt_labels"), kwargs.get("glt_matrices")
+    C = Qc = None
+    if glt_labels and glt_matrices is not None:
+        # glm/arma.py:4201-4253: single-row contrasts over the full design
+        rows = []
+        for m in glt_matrices:
+            c = torch.as_tensor(m, dtype=torch.float64)
+            if c.ndim == 2 and c.shape[0] == 1:
+                c = c[0]
+            if c.ndim != 1:
+                raise NotImplementedError(
+                    f"Multi-row GLT contrasts (F-tests) not yet supported. "
+                    f"Got shape {tuple(c.shape)}, expected (n_regressors,) or (1, n_regressors)")
+            if c.shape[0] != p:
+                raise ValueError(f"GLT has {c.shape[0]} columns, design has {p}")
+            rows.append(c)
+        C = torch.stack(rows)
+        # Var(c'beta) / sigma^2 = c' (X'R^-1X)^-1 c per pair (glm/arma.py:5030-5043, 5454-5460)
+        Qc = torch.einsum("cr,krs,cs->kc", C, grid.ainv(), C).to(dev)
+        C = C.to(dev)
+        n_c = C.shape[0]
+        res.contrast_labels = list(glt_labels)
+        res.contrast_betas = torch.zeros(V, n_c, dtype=out_dtype)
+        res.contrast_tstats = torch.zeros(V, n_c, dtype=out_dtype)
+        if want_r2_partial:
+            res.contrast_r2_partial = torch.zeros(V, n_c, dtype=out_dtype)
+        if want_r2_semipartial:
+            res.contrast_r2_semipartial = torch.zeros(V, n_c, dtype=out_dtype)
     params_dev = grid.params.to(dev)
     with torch.cuda.device(dev):
         for b0 in range(0, V, nb):
@@ -872,6 +905,22 @@ def fit_glm_arma11(data, design, tr, a_grid=None, b_grid=None, estimate_per_voxe
                 res.predicted[b0:b1] = g["predicted"].cpu().to(out_dtype)
             if want_ljung_box:
                 res.ljung_box[b0:b1] = ljung_box_device(g["resid_w"], lj_lag, lj_tau).cpu()
+            if C is not None:
+                # contrast estimates and t in the normalised units of the fit
+                # (glm/arma.py:5429-5471), estimates unscaled like betas (:5835)
+                sb = scale.double() if scale is not None else torch.ones(Vb, dtype=torch.float64, device=dev)
+                cb = (g["betas"].double() / sb[:, None]) @ C.T
+                var = (g["sigma2"].double() / sb**2)[:, None] * Qc[best.long()]
+                ct = cb / (var.clamp(min=0.0).sqrt() + 1e-10)
+                res.contrast_betas[b0:b1] = (cb * sb[:, None]).cpu().to(out_dtype)
+                res.contrast_tstats[b0:b1] = ct.cpu().to(out_dtype)
+                if want_r2_partial or want_r2_semipartial:
+                    cp = ct**2 / (ct**2 + res.dof)
+                    if want_r2_partial:
+                        res.contrast_r2_partial[b0:b1] = cp.cpu().to(out_dtype)
+                    if want_r2_semipartial:
+                        rem = (1.0 - g["r2"].double()).clamp(min=0.0)[:, None]
+                        res.contrast_r2_semipartial[b0:b1] = (cp * rem).cpu().to(out_dtype)
             if want_r2_partial or want_r2_semipartial:
                 part = _partial_r2(g["tstats"].to(out_dtype), g["r2"], res.dof, p, task_indices, want_r2_partial,
                                    r2_partial_mode, want_r2_semipartial, r2_semipartial_mode)

@@ -76,6 +76,10 @@ int ffs_reml_plan_create_ridge(ffs_reml_plan** out, int32_t T, int32_t p, const 
 /* Host copies of logdet R and logdet X'R^-1X per pair (synchronous). */
 int ffs_reml_plan_logdets(const ffs_reml_plan* plan, double* logdet_R, double* logdet_XRX);
 
+/* Host copy (synchronous) of (X'R^-1X)^-1 per pair, K x p x p row-major: the
+ * XwTXw_inv of glm/arma.py:5030-5043 from which GLT variances c' A c follow. */
+int ffs_reml_plan_ainv(const ffs_reml_plan* plan, double* ainv);
+
 /* Padded sizes the search buffers must have: Tpad rows of Vpad floats for the
  * time-major residual buffer (rows >= T must be zero), and the voxel padding unit. */
 int ffs_reml_plan_dims(const ffs_reml_plan* plan, int32_t* Tpad, int32_t* vox_pad, int32_t* K);

@@ -524,6 +524,26 @@ def gls_refit(X, Y, grid: Grid, pair_index, task_indices=None, dtype=torch.float
     return betas, tst, fst, s2, r2, lb, rw
 
 
+def glt_contrasts(X, grid: Grid, pair_index, betas, sigma2, C, dof, r2=None, dtype=torch.float64):
+    """glm/arma.py:5421-5509: single-row GLTs per voxel.  contrast beta = C beta;
+    Var = sigma2 c' (X_w'X_w)^-1 c with X_w = L^-1 X of the voxel's pair;
+    t = cb / (sqrt(max(Var, 0)) + 1e-10); partial R^2 = t^2 / (t^2 + dof);
+    semi-partial = partial * max(1 - R^2, 0)."""
+    X = torch.as_tensor(X).to(dtype)
+    C = torch.as_tensor(C).to(dtype)
+    cb = betas.to(dtype) @ C.T
+    var = torch.zeros_like(cb)
+    for k in torch.unique(pair_index).tolist():
+        m = (pair_index == k).nonzero().flatten()
+        Xw = grid.Linv[k].to(dtype) @ X
+        Ainv = torch.linalg.inv(Xw.T @ Xw)
+        var[m] = sigma2.to(dtype)[m, None] * torch.einsum("cr,rs,cs->c", C, Ainv, C)[None, :]
+    t = cb / (var.clamp(min=0.0).sqrt() + 1e-10)
+    part = t**2 / (t**2 + dof)
+    semi = part * (1.0 - r2.to(dtype)).clamp(min=0.0)[:, None] if r2 is not None else None
+    return cb, t, part, semi
+
+
 def fit_arma11(data, design, a_grid=None, b_grid=None, run_starts=None, tau=None,
                dtype=torch.float64, selector="hierarchical", task_indices=None,
                want_ljung_box=False, grid: Grid | None = None) -> OracleFit:

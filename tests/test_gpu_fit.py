@@ -151,3 +151,38 @@ def test_rvar_write_load_refit(dev, tmp_path):
     lk = outputs.read_nifti(outputs.save_reml_likelihood_surface(res, tmp_path / "lk.nii", volume_shape=(3, 4, 5)))
     assert len(lk.labels) == res.reml_lklhd_surface.shape[1]
     assert np.array_equal(lk.data.reshape(V, -1), res.reml_lklhd_surface.float().numpy())
+
+
+def test_glt_contrasts_match_oracle(dev):
+    """In-loop single-row GLTs (glm/arma.py:4201-4253, 5421-5509) against the
+    FP64 oracle at the engine's pairs: contrast estimates rel <= 1e-5 (of
+    max(|cb|, 1e-3 sd)), t rel <= 1e-5, partial / semi-partial R^2 <= 1e-4;
+    multi-row GLTs raise as in the reference."""
+    from fastfuncstuff_amd.glm.arma import fit_glm_arma11
+
+    print("[fit] glt", flush=True)
+    rng = np.random.default_rng(9)
+    T, V = 100, 200
+    X = _design(rng, T)
+    Y = _data(rng, T, X, V)
+    ag = bg = torch.linspace(-0.8, 0.8, 17)
+    C = [np.array([1.0, -1.0, 0, 0, 0]), np.array([[0.5, 0.5, 0, 0, 0]])]
+    res = fit_glm_arma11(Y, X, 2.0, a_grid=ag, b_grid=bg, device=dev, glt_labels=["diff", "mean"],
+                         glt_matrices=C, want_r2_partial=True, want_r2_semipartial=True)
+    assert res.contrast_labels == ["diff", "mean"] and res.contrast_betas.shape == (V, 2)
+    idx = res.pair_index.long()
+    s = torch.from_numpy(Y).double().std(dim=1).clamp(min=1e-8)
+    g64 = O.precompute_grid(torch.from_numpy(X).double(), ag, bg, None, None, F64)
+    Yn = torch.from_numpy(Y).double() / s[:, None]
+    b64, _, _, s2, r2, _, _ = O.gls_refit(torch.from_numpy(X).double(), Yn, g64, idx, dtype=F64)
+    Cm = torch.tensor(np.stack([C[0], C[1][0]]))
+    cb, ct, part, semi = O.glt_contrasts(torch.from_numpy(X).double(), g64, idx, b64, s2, Cm, T - 5, r2)
+    cb = cb * s[:, None]
+    den = torch.maximum(cb.abs(), 1e-3 * s[:, None])
+    assert float(((res.contrast_betas.double() - cb).abs() / den).max()) <= 1e-5
+    assert float(((res.contrast_tstats.double() - ct).abs() / ct.abs().clamp(min=1)).max()) <= 1e-5
+    assert float((res.contrast_r2_partial.double() - part).abs().max()) <= 1e-4
+    assert float((res.contrast_r2_semipartial.double() - semi).abs().max()) <= 1e-4
+    with pytest.raises(NotImplementedError):
+        fit_glm_arma11(Y, X, 2.0, a_grid=ag, b_grid=bg, device=dev, glt_labels=["f"],
+                       glt_matrices=[np.eye(5)[:2]])

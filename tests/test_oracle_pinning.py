@@ -348,3 +348,29 @@ def test_dense_likelihood_identity_is_ridge_regression():
     s1 = O.reml_likelihood_dense(torch.from_numpy(X), torch.from_numpy(y), R)
     s2 = O.reml_likelihood_dense(torch.from_numpy(X), torch.from_numpy(y), c * R)
     assert abs((s2 - s1) - (T * np.log(c) - p * np.log(c) - (T - p) * np.log(c))) < 1e-5
+
+
+def test_glt_contrast_equals_reparameterised_coefficient():
+    """The GLT t of c = e0 - e1 equals the t of the first coefficient of the
+    same GLS on the reparameterised design Z = X M (gamma = M^-1 beta with
+    gamma_0 = beta_0 - beta_1); a unit contrast reproduces the GLS t."""
+    torch.manual_seed(3)
+    T, p, V = 50, 3, 6
+    X = torch.randn(T, p, dtype=torch.float64)
+    X[:, 2] = 1.0
+    Y = torch.randn(V, T, dtype=torch.float64) + X[:, 0] * 0.5
+    ag = bg = torch.tensor([0.0, 0.4], dtype=torch.float32)
+    g = O.precompute_grid(X, ag, bg)
+    k = torch.tensor([1, 0, 2, 1, 2, 0])[:V] % g.params.shape[0]
+    b, t, _, s2, r2, _, _ = O.gls_refit(X, Y, g, k)
+    C = torch.tensor([[1.0, -1.0, 0.0], [0.0, 1.0, 0.0]], dtype=torch.float64)
+    cb, ct, part, semi = O.glt_contrasts(X, g, k, b, s2, C, T - p, r2)
+    torch.testing.assert_close(ct[:, 1], t[:, 1], rtol=1e-10, atol=1e-12)
+    Minv = torch.eye(p, dtype=torch.float64)
+    Minv[0, 1] = -1.0
+    Z = X @ torch.linalg.inv(Minv)
+    gz = O.precompute_grid(Z, ag, bg)
+    bz, tz, _, _, _, _, _ = O.gls_refit(Z, Y, gz, k)
+    torch.testing.assert_close(bz[:, 0], cb[:, 0], rtol=1e-9, atol=1e-10)
+    torch.testing.assert_close(tz[:, 0], ct[:, 0], rtol=1e-8, atol=1e-10)
+    torch.testing.assert_close(part, ct**2 / (ct**2 + T - p))
