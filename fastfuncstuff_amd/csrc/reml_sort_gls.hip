@@ -185,61 +185,63 @@ __device__ __forceinline__ void load_span(const float* __restrict__ row, int T, 
     }
 }
 
-// One 32-sample block of the GLS accumulation (see k_gls_mfma).
+// Operands of one 16-sample half of a 32-sample block of the GLS accumulation
+// (see k_gls_mfma): MFMA samples t0 + 8 kq + 4 hf + m of the four 16-voxel
+// groups with their G / X rows, and the recurrence samples t0 + 16 hf .. + 16
+// of this lane's voxel.
 template <int NT>
-__device__ __forceinline__ void gls_block(int t0, int T, bool full, int lane, int j, int kq, const float* const (&rowB)[4],
-                                          const float* rowO, const double* __restrict__ Gk,
-                                          const double* __restrict__ Xp, const double* __restrict__ cR, double* cbuf,
-                                          f64x4 (&accG)[4][NT], f64x4 (&accX)[4][NT], double& E, double& sy,
-                                          double& syy, double& e, double& yprev) {
+struct GlsHalf {
+    float yb[4][4];
+    double ag[4][NT], ax[4][NT];
+    float yo[16];
+};
+
+template <int NT>
+__device__ __forceinline__ void gls_load_half(int t0, int hf, int T, bool full, int j, int kq,
+                                              const float* const (&rowB)[4], const float* rowO,
+                                              const double* __restrict__ Gk, const double* __restrict__ Xp,
+                                              GlsHalf<NT>& h) {
     constexpr int PG = 16 * NT;
-    const double c0 = cR[(size_t)t0 * 4 + 2 * lane], c1 = cR[(size_t)t0 * 4 + 2 * lane + 1];
-    __syncthreads();  // the previous block's coefficient reads are done
-    cbuf[2 * lane] = c0;
-    cbuf[2 * lane + 1] = c1;
-    __syncthreads();
-    // two halves of 16 time points: MFMA steps m in [4 hf, 4 hf + 4) and the
-    // recurrence over t0 + 16 hf .. + 16
 #pragma unroll
-    for (int hf = 0; hf < 2; ++hf) {
-        float yb[4][4];
+    for (int g = 0; g < 4; ++g) load_span<4>(rowB[g], T, t0 + 8 * kq + 4 * hf, full, h.yb[g]);
 #pragma unroll
-        for (int g = 0; g < 4; ++g) load_span<4>(rowB[g], T, t0 + 8 * kq + 4 * hf, full, yb[g]);
-        double ag[4][NT], ax[4][NT];
+    for (int m = 0; m < 4; ++m)
 #pragma unroll
-        for (int m = 0; m < 4; ++m)
+        for (int rt = 0; rt < NT; ++rt) {
+            const size_t o = (size_t)(t0 + 8 * kq + 4 * hf + m) * PG + 16 * rt + j;
+            h.ag[m][rt] = Gk[o];
+            h.ax[m][rt] = Xp[o];
+        }
+    load_span<16>(rowO, T, t0 + 16 * hf, full, h.yo);
+}
+
+template <int NT>
+__device__ __forceinline__ void gls_half(int hf, const GlsHalf<NT>& h, const double* cbuf, f64x4 (&accG)[4][NT],
+                                         f64x4 (&accX)[4][NT], double& E, double& sy, double& syy, double& e,
+                                         double& yprev) {
+#pragma unroll
+    for (int m = 0; m < 4; ++m)
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+            const double b = (double)h.yb[g][m];
 #pragma unroll
             for (int rt = 0; rt < NT; ++rt) {
-                const size_t o = (size_t)(t0 + 8 * kq + 4 * hf + m) * PG + 16 * rt + j;
-                ag[m][rt] = Gk[o];
-                ax[m][rt] = Xp[o];
+                accG[g][rt] = __builtin_amdgcn_mfma_f64_16x16x4f64(h.ag[m][rt], b, accG[g][rt], 0, 0, 0);
+                accX[g][rt] = __builtin_amdgcn_mfma_f64_16x16x4f64(h.ax[m][rt], b, accX[g][rt], 0, 0, 0);
             }
-        float yo[16];
-        load_span<16>(rowO, T, t0 + 16 * hf, full, yo);
-#pragma unroll
-        for (int m = 0; m < 4; ++m)
-#pragma unroll
-            for (int g = 0; g < 4; ++g) {
-                const double b = (double)yb[g][m];
-#pragma unroll
-                for (int rt = 0; rt < NT; ++rt) {
-                    accG[g][rt] = __builtin_amdgcn_mfma_f64_16x16x4f64(ag[m][rt], b, accG[g][rt], 0, 0, 0);
-                    accX[g][rt] = __builtin_amdgcn_mfma_f64_16x16x4f64(ax[m][rt], b, accX[g][rt], 0, 0, 0);
-                }
-            }
-        // Rows t >= T have zero coefficients and zero samples: they add nothing.
-#pragma unroll
-        for (int i = 0; i < 16; ++i) {
-            const int r = 16 * hf + i;
-            const double yt = (double)yo[i];
-            const double gs = cbuf[4 * r], c = cbuf[4 * r + 1], al = cbuf[4 * r + 2];
-            e = fma(al, e, fma(-c, yprev, yt));
-            yprev = yt;
-            const double w = e * gs;
-            E = fma(w, w, E);
-            sy += yt;
-            syy = fma(yt, yt, syy);
         }
+    // Rows t >= T have zero coefficients and zero samples: they add nothing.
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+        const int r = 16 * hf + i;
+        const double yt = (double)h.yo[i];
+        const double gs = cbuf[4 * r], c = cbuf[4 * r + 1], al = cbuf[4 * r + 2];
+        e = fma(al, e, fma(-c, yprev, yt));
+        yprev = yt;
+        const double w = e * gs;
+        E = fma(w, w, E);
+        sy += yt;
+        syy = fma(yt, yt, syy);
     }
 }
 
@@ -282,9 +284,32 @@ __global__ void __launch_bounds__(64) k_gls_mfma(PlanDev P, const float* __restr
             accX[g][rt] = f64x4{0.0, 0.0, 0.0, 0.0};
         }
     double E = 0.0, sy = 0.0, syy = 0.0, e = 0.0, yprev = 0.0;
-    for (int t0 = 0; t0 < Tpad; t0 += 32)
-        gls_block<NT>(t0, T, vec4 && t0 + 32 <= T, lane, j, kq, rowB, rowO, Gk, Xp, cR, cbuf, accG, accX, E, sy, syy,
-                      e, yprev);
+    // 32-sample blocks, each two halves; the operands of the next half (and
+    // the next block's recurrence coefficients) are fetched before the
+    // current half is computed (one wave per block: the loads must overlap).
+    GlsHalf<NT> hn;
+    gls_load_half<NT>(0, 0, T, vec4 && 32 <= T, j, kq, rowB, rowO, Gk, Xp, hn);
+    double c0 = cR[2 * lane], c1 = cR[2 * lane + 1];
+    for (int t0 = 0; t0 < Tpad; t0 += 32) {
+        const bool full = vec4 && t0 + 32 <= T;
+        __syncthreads();  // the previous block's coefficient reads are done
+        cbuf[2 * lane] = c0;
+        cbuf[2 * lane + 1] = c1;
+        __syncthreads();
+        if (t0 + 32 < Tpad) {
+            c0 = cR[(size_t)(t0 + 32) * 4 + 2 * lane];
+            c1 = cR[(size_t)(t0 + 32) * 4 + 2 * lane + 1];
+        }
+#pragma unroll
+        for (int hf = 0; hf < 2; ++hf) {
+            const GlsHalf<NT> h = hn;
+            if (hf == 0)
+                gls_load_half<NT>(t0, 1, T, full, j, kq, rowB, rowO, Gk, Xp, hn);
+            else if (t0 + 32 < Tpad)
+                gls_load_half<NT>(t0 + 32, 0, T, vec4 && t0 + 64 <= T, j, kq, rowB, rowO, Gk, Xp, hn);
+            gls_half<NT>(hf, h, cbuf, accG, accX, E, sy, syy, e, yprev);
+        }
+    }
     // Accumulators -> one voxel per lane through LDS: h first (beta, |h|^2),
     // then q (read back element by element).
 #pragma unroll
