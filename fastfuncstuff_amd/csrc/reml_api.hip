@@ -399,7 +399,7 @@ static int prep_impl(const ffs_reml_plan* P, const float* Y, int64_t ldY, int64_
         }
         if (write_split && record_split) P->split_yt = Ysh ? Yt : nullptr;
         const int sbf = P->d.proj_mode == 1;
-        const dim3 grid((unsigned)(ncp / 64)), block(64);
+        const dim3 grid((unsigned)(ncp / 64)), block(64 * kPrepWaves);
         if (P->d.PG == 16) {
             if (feats)
                 hipLaunchKernelGGL((k_prep_mfma<1, 10>), grid, block, 0, s, P->d, Y, ldY, v, rows, nc, ncpi, normalize,

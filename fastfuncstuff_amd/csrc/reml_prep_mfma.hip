@@ -6,40 +6,61 @@
 
 namespace remlk {
 
+// Waves per 64-column block (each takes a contiguous range of time steps).
+constexpr int kPrepWaves = 4;
+// Steps of operands in flight per wave: as deep as the register file allows
+// two waves per SIMD.
+constexpr int prep_depth(int NT, int NF) { return NT == 1 && NF == 0 ? 2 : 1; }
+
 __device__ __forceinline__ int prep_voxel(int c, int ncols, const int32_t* __restrict__ rows, int V) {
     const int v = c < ncols ? (rows ? rows[c] : c) : -1;
     return v < V ? v : -1;
 }
 
 // ---------------------------------------------------------------------------
-// One wave per 64 output columns (voxel of column c = rows ? rows[c] : c),
-// following glm/arma.py:1554-1571 (std normalisation, projection on the
-// design's pseudo-inverse, residual):
-//   pass 1: c = pinv(X) y on MFMA (k-slot q of step m = time t0 + 4q + m) and
-//           the sums of y, y^2 -> s = max(float(sd), 1e-8) (normalize bit 0;
-//           bit 1 keeps c = 0, i.e. no residualisation);
-//   pass 2: X c on MFMA for 16-row time tiles, r = y / s - X c / s written to
-//           Yt[t][column] (rows T..Tpad-1 come out zero); optional lag-1..NF
-//           autocorrelations of r through an LDS tile.
+// One block of kPrepWaves waves per 64 output columns (voxel of column
+// c = rows ? rows[c] : c), following glm/arma.py:1554-1571 (std
+// normalisation, projection on the design's pseudo-inverse, residual).  The
+// 16-sample time steps are split into kPrepWaves contiguous ranges, one per
+// wave, so that each wave's chain of dependent loads is short:
+//   pass 1: partial c = pinv(X) y on MFMA (k-slot q of step m = time
+//           t0 + 4q + m) and partial sums of y, y^2 over the wave's range,
+//           added in wave order through LDS (every wave forms the same sums);
+//           s = max(float(sd), 1e-8) (normalize bit 0; bit 1 keeps c = 0,
+//           i.e. no residualisation);
+//   pass 2: X c on MFMA for the wave's 16-row time tiles, r = y / s - X c / s
+//           written to Yt[t][column] (rows T..Tpad-1 come out zero); optional
+//           lag-1..NF autocorrelations of r through the wave's LDS tile (a
+//           wave first replays the step before its range to seed the lags),
+//           added in wave order.
 // The A operand of pass 2 permutes the tile rows so that a lane's four output
 // rows are four consecutive time points.  Every voxel's arithmetic is the same
 // whatever column it lands in.
 // ---------------------------------------------------------------------------
 template <int NT, int NF>
-__global__ void __launch_bounds__(64) k_prep_mfma(PlanDev P, const float* __restrict__ Y, int64_t ldY, int V,
-                                                  const int32_t* __restrict__ rows, int ncols, int ncols_pad,
-                                                  int normalize, float* __restrict__ Yt, int64_t ldYt,
-                                                  float* __restrict__ scale, float* __restrict__ feats,
-                                                  const int32_t* __restrict__ ncols_live, uint16_t* __restrict__ Ysh,
-                                                  uint16_t* __restrict__ Ysl, int split_bf16) {
+__global__ void __launch_bounds__(64 * kPrepWaves) k_prep_mfma(PlanDev P, const float* __restrict__ Y, int64_t ldY,
+                                                               int V, const int32_t* __restrict__ rows, int ncols,
+                                                               int ncols_pad, int normalize, float* __restrict__ Yt,
+                                                               int64_t ldYt, float* __restrict__ scale,
+                                                               float* __restrict__ feats,
+                                                               const int32_t* __restrict__ ncols_live,
+                                                               uint16_t* __restrict__ Ysh, uint16_t* __restrict__ Ysl,
+                                                               int split_bf16) {
     constexpr int PG = 16 * NT;
     constexpr int RB = 65;  // feature tile pitch (floats)
-    __shared__ float rbuf[NF > 0 ? 16 * RB : 1];
-    const int lane = threadIdx.x;
+    constexpr int W = kPrepWaves;
+    constexpr int RN = 16 * NT + 8;  // partial sums per lane: c (4 x NT x 4), then s1, s2 (4 each)
+    static_assert(NF <= 16 && NF + 1 <= RN, "feature lags must fit one step and the reduction buffer");
+    __shared__ double red[W][RN][64];
+    __shared__ float rbuf[NF > 0 ? W * 16 * RB : 1];
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int col0 = blockIdx.x * 64;
     if (col0 >= ncols_pad) return;
     if (ncols_live && col0 >= *ncols_live) return;  // beyond the columns in use (device-side count)
     const int T = P.T, Tpad = P.Tpad;
+    const int nst = Tpad / 16;
+    const int tb = 16 * (w * nst / W), te = 16 * ((w + 1) * nst / W);  // this wave's time range
     const int j = lane & 15, kq = lane >> 4;
     const bool vec4 = (ldY & 3) == 0 && (reinterpret_cast<uintptr_t>(Y) & 15) == 0;
     int vB[4];
@@ -59,8 +80,7 @@ __global__ void __launch_bounds__(64) k_prep_mfma(PlanDev P, const float* __rest
 #pragma unroll
         for (int rt = 0; rt < NT; ++rt) acc[g][rt] = f64x4{0.0, 0.0, 0.0, 0.0};
     double s1[4] = {0.0, 0.0, 0.0, 0.0}, s2[4] = {0.0, 0.0, 0.0, 0.0};
-    // operands of one 16-sample step; the next step's are fetched before the
-    // current step's MFMAs (one wave per block: the loads must overlap)
+    // operands of one 16-sample step, fetched D steps ahead
     auto load1 = [&](int t0, float (&yb)[4][4], double (&a)[4][NT]) __attribute__((always_inline)) {
         const bool full = vec4 && t0 + 16 <= T;
 #pragma unroll
@@ -74,40 +94,89 @@ __global__ void __launch_bounds__(64) k_prep_mfma(PlanDev P, const float* __rest
 #pragma unroll
             for (int rt = 0; rt < NT; ++rt) a[m][rt] = Pp[(size_t)(t0 + 4 * kq + m) * PG + 16 * rt + j];
     };
-    float ybn[4][4];
-    double an[4][NT];
-    load1(0, ybn, an);
-    for (int t0 = 0; t0 < Tpad; t0 += 16) {
-        float yb[4][4];
-        double a[4][NT];
+    constexpr int D = prep_depth(NT, NF);
+    {
+        float yq[D][4][4];
+        double aq[D][4][NT];
 #pragma unroll
-        for (int g = 0; g < 4; ++g)
+        for (int u = 0; u < D; ++u)
+            if (tb + 16 * u < te) load1(tb + 16 * u, yq[u], aq[u]);
+        for (int t0b = tb; t0b < te; t0b += 16 * D) {
 #pragma unroll
-            for (int m = 0; m < 4; ++m) yb[g][m] = ybn[g][m];
+            for (int u = 0; u < D; ++u) {
+                const int t0 = t0b + 16 * u;
+                if (t0 >= te) break;
+                float yb[4][4];
+                double a[4][NT];
 #pragma unroll
-        for (int m = 0; m < 4; ++m)
+                for (int g = 0; g < 4; ++g)
 #pragma unroll
-            for (int rt = 0; rt < NT; ++rt) a[m][rt] = an[m][rt];
-        if (t0 + 16 < Tpad) load1(t0 + 16, ybn, an);
+                    for (int m = 0; m < 4; ++m) yb[g][m] = yq[u][g][m];
 #pragma unroll
-        for (int m = 0; m < 4; ++m)
+                for (int m = 0; m < 4; ++m)
 #pragma unroll
-            for (int g = 0; g < 4; ++g) {
-                const double b = (double)yb[g][m];
-                s1[g] += b;
-                s2[g] = fma(b, b, s2[g]);
+                    for (int rt = 0; rt < NT; ++rt) a[m][rt] = aq[u][m][rt];
+                if (t0 + 16 * D < te) load1(t0 + 16 * D, yq[u], aq[u]);
 #pragma unroll
-                for (int rt = 0; rt < NT; ++rt)
-                    acc[g][rt] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[m][rt], b, acc[g][rt], 0, 0, 0);
+                for (int m = 0; m < 4; ++m)
+#pragma unroll
+                    for (int g = 0; g < 4; ++g) {
+                        const double b = (double)yb[g][m];
+                        s1[g] += b;
+                        s2[g] = fma(b, b, s2[g]);
+#pragma unroll
+                        for (int rt = 0; rt < NT; ++rt)
+                            acc[g][rt] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[m][rt], b, acc[g][rt], 0, 0, 0);
+                    }
             }
+        }
     }
-    double inv[4];
+    // partial sums of the block's waves, added in wave order by every wave
 #pragma unroll
     for (int g = 0; g < 4; ++g) {
         s1[g] += __shfl_xor(s1[g], 16);
         s1[g] += __shfl_xor(s1[g], 32);
         s2[g] += __shfl_xor(s2[g], 16);
         s2[g] += __shfl_xor(s2[g], 32);
+    }
+    {
+        int q = 0;
+#pragma unroll
+        for (int g = 0; g < 4; ++g)
+#pragma unroll
+            for (int rt = 0; rt < NT; ++rt)
+#pragma unroll
+                for (int i = 0; i < 4; ++i) red[w][q++][lane] = acc[g][rt][i];
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+            red[w][q++][lane] = s1[g];
+            red[w][q++][lane] = s2[g];
+        }
+    }
+    __syncthreads();
+    {
+        auto sum = [&](int q) __attribute__((always_inline)) {
+            double x = red[0][q][lane];
+#pragma unroll
+            for (int ww = 1; ww < W; ++ww) x += red[ww][q][lane];
+            return x;
+        };
+        int q = 0;
+#pragma unroll
+        for (int g = 0; g < 4; ++g)
+#pragma unroll
+            for (int rt = 0; rt < NT; ++rt)
+#pragma unroll
+                for (int i = 0; i < 4; ++i) acc[g][rt][i] = sum(q++);
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+            s1[g] = sum(q++);
+            s2[g] = sum(q++);
+        }
+    }
+    double inv[4];
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
         double s = 1.0;
         if (normalize & 1) {
             const double mean = s1[g] / T;
@@ -116,7 +185,7 @@ __global__ void __launch_bounds__(64) k_prep_mfma(PlanDev P, const float* __rest
             s = (double)fmaxf((float)sd, 1e-8f);
         }
         inv[g] = 1.0 / s;
-        if (kq == 0 && vB[g] >= 0 && scale) scale[vB[g]] = (float)s;
+        if (w == 0 && kq == 0 && vB[g] >= 0 && scale) scale[vB[g]] = (float)s;
     }
     if (!Yt && NF == 0) return;
     if (normalize & 2) {  // keep the design component (no residualisation)
@@ -138,6 +207,7 @@ __global__ void __launch_bounds__(64) k_prep_mfma(PlanDev P, const float* __rest
         hist[k] = 0.f;
         acf[k] = 0.0;
     }
+    float* rb = rbuf + (NF > 0 ? w * 16 * RB : 0);
     auto load2 = [&](int tt, double (&xa)[NT][4], float (&y4)[4][4]) __attribute__((always_inline)) {
 #pragma unroll
         for (int rt = 0; rt < NT; ++rt)
@@ -147,21 +217,9 @@ __global__ void __launch_bounds__(64) k_prep_mfma(PlanDev P, const float* __rest
 #pragma unroll
         for (int g = 0; g < 4; ++g) load_span<4>(rowB[g], T, tt + 4 * kq, full, y4[g]);
     };
-    double xan[NT][4];
-    float y4n[4][4];
-    load2(0, xan, y4n);
-    for (int tt = 0; tt < Tpad; tt += 16) {
-        double xa[NT][4];
-        float y4c[4][4];
-#pragma unroll
-        for (int rt = 0; rt < NT; ++rt)
-#pragma unroll
-            for (int kk = 0; kk < 4; ++kk) xa[rt][kk] = xan[rt][kk];
-#pragma unroll
-        for (int g = 0; g < 4; ++g)
-#pragma unroll
-            for (int r = 0; r < 4; ++r) y4c[g][r] = y4n[g][r];
-        if (tt + 16 < Tpad) load2(tt + 16, xan, y4n);
+    // residuals of one step; emit = false only updates the lag history
+    auto step2 = [&](int tt, const double (&xa)[NT][4], const float (&y4c)[4][4], bool emit)
+                     __attribute__((always_inline)) {
         f64x4 d[4];
 #pragma unroll
         for (int g = 0; g < 4; ++g) d[g] = f64x4{0.0, 0.0, 0.0, 0.0};
@@ -182,10 +240,10 @@ __global__ void __launch_bounds__(64) k_prep_mfma(PlanDev P, const float* __rest
                 const float rr = (float)((y - d[g][r]) * inv[g]);
                 r4[r] = rr;
                 const int t = tt + 4 * kq + r;
-                if (Yt) Yt[(size_t)t * ldYt + col0 + 16 * g + j] = rr;
-                if (NF > 0) rbuf[(4 * kq + r) * RB + 16 * g + j] = rr;
+                if (emit && Yt) Yt[(size_t)t * ldYt + col0 + 16 * g + j] = rr;
+                if (NF > 0) rb[(4 * kq + r) * RB + 16 * g + j] = rr;
             }
-            if (Ysh) {
+            if (emit && Ysh) {
                 // split halves, [t/8][column][8] (this lane: one 8-byte run of 4 samples)
                 uint16_t h[4], l[4];
 #pragma unroll
@@ -202,25 +260,79 @@ __global__ void __launch_bounds__(64) k_prep_mfma(PlanDev P, const float* __rest
             }
         }
         if (NF > 0) {
-            __syncthreads();
+            // the tile is the wave's own: order its LDS writes and reads within the wave
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 #pragma unroll
             for (int i = 0; i < 16; ++i) {
-                const float x = rbuf[i * RB + lane];  // zero past T
-                acf0 += (double)x * x;
+                const float x = rb[i * RB + lane];  // zero past T
+                if (emit) {
+                    acf0 += (double)x * x;
 #pragma unroll
-                for (int k = 0; k < NF; ++k) acf[k] += (double)x * hist[k];
+                    for (int k = 0; k < NF; ++k) acf[k] += (double)x * hist[k];
+                }
 #pragma unroll
                 for (int k = NF - 1; k > 0; --k) hist[k] = hist[k - 1];
                 hist[0] = x;
             }
-            __syncthreads();
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        }
+    };
+    if (NF > 0 && tb > 0 && tb < te) {  // seed the lags with the step before the range
+        double xa[NT][4];
+        float y4[4][4];
+        load2(tb - 16, xa, y4);
+        step2(tb - 16, xa, y4, false);
+    }
+    {
+        double xq[D][NT][4];
+        float y4q[D][4][4];
+#pragma unroll
+        for (int u = 0; u < D; ++u)
+            if (tb + 16 * u < te) load2(tb + 16 * u, xq[u], y4q[u]);
+        for (int ttb = tb; ttb < te; ttb += 16 * D) {
+#pragma unroll
+            for (int u = 0; u < D; ++u) {
+                const int tt = ttb + 16 * u;
+                if (tt >= te) break;
+                double xa[NT][4];
+                float y4c[4][4];
+#pragma unroll
+                for (int rt = 0; rt < NT; ++rt)
+#pragma unroll
+                    for (int kk = 0; kk < 4; ++kk) xa[rt][kk] = xq[u][rt][kk];
+#pragma unroll
+                for (int g = 0; g < 4; ++g)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) y4c[g][r] = y4q[u][g][r];
+                if (tt + 16 * D < te) load2(tt + 16 * D, xq[u], y4q[u]);
+                step2(tt, xa, y4c, true);
+            }
         }
     }
-    if (NF > 0 && feats) {
-        const int v = prep_voxel(col0 + lane, ncols, rows, V);
-        if (v >= 0) {
+    if (NF > 0) {
+        __syncthreads();  // every wave has read the pass-1 partials
+        red[w][0][lane] = acf0;
 #pragma unroll
-            for (int k = 0; k < NF; ++k) feats[(size_t)v * NF + k] = acf0 > 0.0 ? (float)(acf[k] / acf0) : 0.f;
+        for (int k = 0; k < NF; ++k) red[w][1 + k][lane] = acf[k];
+        __syncthreads();
+        if (w == 0 && feats) {
+            const int v = prep_voxel(col0 + lane, ncols, rows, V);
+            if (v >= 0) {
+                double a0 = red[0][0][lane];
+#pragma unroll
+                for (int ww = 1; ww < W; ++ww) a0 += red[ww][0][lane];
+#pragma unroll
+                for (int k = 0; k < NF; ++k) {
+                    double ak = red[0][1 + k][lane];
+#pragma unroll
+                    for (int ww = 1; ww < W; ++ww) ak += red[ww][1 + k][lane];
+                    feats[(size_t)v * NF + k] = a0 > 0.0 ? (float)(ak / a0) : 0.f;
+                }
+            }
         }
     }
 }

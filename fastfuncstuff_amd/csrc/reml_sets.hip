@@ -260,6 +260,13 @@ __global__ void __launch_bounds__(64 * kSetWaves) k_set_project(PlanDev P, SetTa
             bst[n] = k >= 0 ? k / npt : 0;
             bcc[n] = k >= 0 ? (k - bst[n] * npt) * p + j : 0;
         }
+        // Pairs of this lane's first kPf (row, pair) scores per 32-row block.
+        int kq[kPf];
+#pragma unroll
+        for (int jj = 0; jj < kPf; ++jj) {
+            const int i = lane + 64 * jj;
+            kq[jj] = i < 32 * npt ? kp[i % npt] : -1;
+        }
         f32x16 acc[2][NBLK];
 #pragma unroll
         for (int r = 0; r < 2; ++r)
@@ -385,7 +392,21 @@ __global__ void __launch_bounds__(64 * kSetWaves) k_set_project(PlanDev P, SetTa
                 }
             }
         }
-        // scores, one 32-row block at a time through the wave's patch
+        // scores, one 32-row block at a time through the wave's patch; the
+        // operands of the first kPf per block are all fetched up front
+        float eq[2][kPf], usq[kPf];
+        double ldq[kPf];
+#pragma unroll
+        for (int jj = 0; jj < kPf; ++jj) {
+            const int k = kq[jj];
+            const int i = lane + 64 * jj;
+            const int row = i / npt, qq = i - row * npt;
+#pragma unroll
+            for (int r = 0; r < 2; ++r)
+                eq[r][jj] = k >= 0 ? Sb[(size_t)(col0 + 32 * r + row) * SB + stile * npt + qq] : 0.f;
+            ldq[jj] = k >= 0 ? P.ld[k] : 0.0;
+            usq[jj] = MODE == 2 && k >= 0 ? P.g16_unscale[k] : 1.f;
+        }
 #pragma unroll
         for (int r = 0; r < 2; ++r) {
             __syncthreads();  // the patch is free
@@ -397,7 +418,42 @@ __global__ void __launch_bounds__(64 * kSetWaves) k_set_project(PlanDev P, SetTa
                     Cw[row * C_LD + n * 32 + (lane & 31)] = acc[r][n][q];
                 }
             __syncthreads();
-            for (int i = lane; i < 32 * npt; i += 64) {
+            // one score at a time (not unrolled: the log is long); the operand
+            // arrays rotate so that their indexing stays static
+            int kc[kPf];
+            float ec[kPf], uc[kPf];
+            double lc[kPf];
+#pragma unroll
+            for (int jj = 0; jj < kPf; ++jj) {
+                kc[jj] = kq[jj];
+                ec[jj] = eq[r][jj];
+                uc[jj] = usq[jj];
+                lc[jj] = ldq[jj];
+            }
+#pragma unroll 1
+            for (int jj = 0; jj < kPf; ++jj) {
+                if (kc[0] >= 0) {  // else past the tile or an empty slot
+                    const int i = lane + 64 * jj;
+                    const int row = i / npt, qq = i - row * npt;
+                    const float* cr = Cw + row * C_LD + qq * p;
+                    double pp = 0.0;
+                    for (int j = 0; j < p; ++j) {
+                        const double cv = (double)cr[j];
+                        pp = fma(cv, cv, pp);
+                    }
+                    if (MODE == 2) pp *= (double)uc[0];
+                    const double rss = (double)ec[0] - pp;
+                    Sb[(size_t)(col0 + 32 * r + row) * SB + stile * npt + qq] = (float)fma(tp, log(rss + 1e-10), lc[0]);
+                }
+#pragma unroll
+                for (int q = 0; q + 1 < kPf; ++q) {
+                    kc[q] = kc[q + 1];
+                    ec[q] = ec[q + 1];
+                    uc[q] = uc[q + 1];
+                    lc[q] = lc[q + 1];
+                }
+            }
+            for (int i = lane + 64 * kPf; i < 32 * npt; i += 64) {
                 const int row = i / npt, qq = i % npt;
                 const int k = kp[qq];
                 if (k < 0) continue;

@@ -107,3 +107,19 @@ def adam_step(theta, grad, m, v, step, lr=1e-3, beta1=0.9, beta2=0.999, eps=1e-8
     mh = m / (1 - beta1**step)
     vh = v / (1 - beta2**step)
     return theta - lr * mh / (np.sqrt(vh) + eps), m, v
+
+
+def residual_acf(Y: np.ndarray, X: np.ndarray, nlags: int = N_IN) -> np.ndarray:
+    """Network inputs: lag-1..nlags autocorrelations of each voxel's OLS residual
+    r = y - X pinv(X) y (rows of Y are voxels), acf_k = sum_t r_t r_{t-k} /
+    sum_t r_t^2 (0 where the residual is identically zero).  The per-voxel std
+    normalisation that the device applies first cancels in the ratio."""
+    Y = np.asarray(Y, dtype=np.float64)
+    X = np.asarray(X, dtype=np.float64)
+    R = Y - (Y @ np.linalg.pinv(X).T) @ X.T
+    den = (R * R).sum(axis=1)
+    out = np.zeros((Y.shape[0], nlags))
+    for k in range(1, nlags + 1):
+        num = (R[:, k:] * R[:, :-k]).sum(axis=1)
+        out[:, k - 1] = np.where(den > 0, num / np.where(den > 0, den, 1.0), 0.0)
+    return out

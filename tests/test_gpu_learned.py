@@ -3,6 +3,7 @@
 * MLP forward / gradient / Adam kernels vs the FP64 oracle (oracle/mlp_oracle.py):
   forward rtol 1e-5; loss rtol 1e-5; gradient |g - g64| <= 1e-3 |g64| + 1e-4 max|g64|;
   Adam rtol 1e-6.
+* Residual-autocorrelation features vs oracle.residual_acf: |d| <= 2e-5.
 * Window scores are bitwise equal to the full-grid surface in each projection
   mode, and the window argmin equals the argmin of the surface restricted to
   the window.
@@ -25,6 +26,22 @@ def dev():
 
     _lib.load()
     return torch.device("cuda", 0)
+
+
+@pytest.mark.parametrize("T", [80, 200, 400])
+def test_features_match_oracle(dev, T):
+    """Residual autocorrelation features (the time range of each 64-voxel group
+    is split over the waves of its block) vs the FP64 oracle: |d| <= 2e-5."""
+    from fastfuncstuff_amd import synth
+    from fastfuncstuff_amd.glm.arma import RemlGrid
+    from fastfuncstuff_amd.learned import features
+
+    wl = synth.plain_config(V=300, T=T, seed=11)
+    grid = RemlGrid(torch.from_numpy(wl.X), torch.from_numpy(wl.a_grid), torch.from_numpy(wl.b_grid), dev)
+    got = features(grid, torch.from_numpy(wl.Y).to(dev)).cpu().numpy().astype(np.float64)
+    ref = M.residual_acf(wl.Y, wl.X, M.N_IN)
+    assert got.shape == ref.shape
+    assert np.abs(got - ref).max() <= 2e-5
 
 
 def test_mlp_forward_grad_adam_match_oracle(dev):

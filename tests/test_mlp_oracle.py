@@ -52,3 +52,20 @@ def test_adam_closed_form_first_step():
     th, m, v = M.adam_step(theta, g, np.zeros(2), np.zeros(2), 1, lr=0.1)
     # first bias-corrected step moves by lr * sign(g) (up to eps)
     np.testing.assert_allclose(th, theta - 0.1 * np.sign(g), rtol=0, atol=1e-7)
+
+
+def test_residual_acf_closed_forms():
+    # intercept-only design, y_t = (-1)^t with T even: the residual is y itself
+    # and acf_k = (-1)^k (T - k) / T
+    T = 40
+    X = np.ones((T, 1))
+    y = (-1.0) ** np.arange(T)
+    got = M.residual_acf(y[None, :], X, 10)[0]
+    want = np.array([(-1.0) ** k * (T - k) / T for k in range(1, 11)])
+    assert np.abs(got - want).max() < 1e-12
+    t = np.arange(T, dtype=np.float64)
+    # a step series orthogonal to the intercept: r_t = +1 then -1 (T/2 each),
+    # acf_k = (T - 3k) / T for k <= T/2
+    z = np.where(t < T // 2, 1.0, -1.0)
+    want = np.array([(T - 3 * k) / T for k in range(1, 11)])
+    assert np.abs(M.residual_acf(z[None, :], X, 10)[0] - want).max() < 1e-12
