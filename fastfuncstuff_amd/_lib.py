@@ -34,30 +34,32 @@ SIGNATURES: dict[str, list] = {
     "ffs_reml_plan_dims": [_vp, _vp, _vp, _vp],
     "ffs_reml_plan_set_projection": [_vp, _c_int],
     "ffs_reml_prep": [_vp, _vp, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _c_int, _vp, _vp],
+    "ffs_reml_prep_features": [_vp, _vp, _i64, _i64, _vp, _c_int, _vp, _vp, _vp],
+    "ffs_reml_plan_coef_stride": [_vp, _vp],
     "ffs_reml_energy": [_vp, _vp, _i64, _i64, _vp, _i64, _vp],
     "ffs_reml_project": [_vp, _vp, _i64, _i64, _vp, _i64, _vp],
     "ffs_reml_select": [_vp, _vp, _i64, _i64, _c_int, _vp, _vp, _vp],
     "ffs_reml_gls": [_vp, _vp, _i64, _vp, _vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _vp, _vp, _vp, _vp],
     "ffs_reml_ljung_box": [_vp, _i64, _i64, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp],
     "ffs_reml_local_workspace": [_vp, _i64, _vp],
-    "ffs_reml_local_recenter": [_vp, _vp, _i64, _i64, _vp, _vp, _i64, _vp],
+    "ffs_reml_local_recenter": [_vp, _vp, _i64, _i64, _vp, _vp, _vp, _i64, _vp],
     "ffs_reml_gls_workspace": [_vp, _i64, _vp],
     "ffs_reml_plan_yt_rows": [_vp, _vp],
     "ffs_reml_surface": [_vp, _vp, _i64, _i64, _vp, _i64, _c_int, _vp],
-    "ffs_reml_local_prepare": [_vp, _vp, _i64, _i64, _vp, ctypes.c_float, _vp, _vp, _vp, _i64, _vp],
+    "ffs_reml_local_prepare": [_vp, _vp, _i64, _i64, _vp, ctypes.c_float, _vp, _vp, _vp, _vp, _i64, _vp],
     "ffs_reml_local_energy": [_vp, _i64, _vp, _i64, _vp],
     "ffs_reml_local_project": [_vp, _i64, _vp, _vp, _vp, _vp, _i64, _vp],
-    "ffs_reml_local_search": [_vp, _vp, _i64, _i64, _vp, ctypes.c_float, _vp, _vp, _vp, _vp, _vp, _i64, _vp],
+    "ffs_reml_local_search": [_vp, _vp, _i64, _i64, _vp, ctypes.c_float, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _vp],
     "ffs_reml_compact": [_vp, _i64, _vp, _vp, _vp],
     "ffs_reml_subset_workspace": [_vp, _i64, _vp],
-    "ffs_reml_subset_search": [_vp, _vp, _i64, _i64, _vp, _vp, _i64, _vp, _vp, _vp, _vp, _i64, _vp],
+    "ffs_reml_subset_search": [_vp, _vp, _i64, _i64, _vp, _vp, _i64, _vp, _vp, _vp, _vp, _vp, _i64, _vp],
     "ffs_reml_plan_set_bands": [_vp, ctypes.c_double, ctypes.c_double, _vp],
     "ffs_reml_plan_band_info": [_vp, _vp, _vp, _vp, _vp, _vp],
     "ffs_reml_band_workspace": [_vp, _i64, _vp],
-    "ffs_reml_band_prepare": [_vp, _vp, _i64, _i64, _vp, _vp, _vp, _vp, _i64, _vp],
+    "ffs_reml_band_prepare": [_vp, _vp, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _i64, _vp],
     "ffs_reml_band_energy": [_vp, _i64, _vp, _i64, _vp],
     "ffs_reml_band_project": [_vp, _i64, _vp, _vp, _vp, _vp, _i64, _vp],
-    "ffs_reml_band_search": [_vp, _vp, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _vp],
+    "ffs_reml_band_search": [_vp, _vp, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _vp],
     "ffs_reml_scatter_best": [_vp, _i64, _vp, _vp, _vp, _vp, _vp],
     "ffs_mlp_param_count": [],
     "ffs_mlp_workspace_floats": [_i64],

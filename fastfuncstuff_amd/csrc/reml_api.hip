@@ -381,7 +381,7 @@ static void launch_prep(const ffs_reml_plan* P, const float* Y, int64_t ldY, int
 static int prep_impl(const ffs_reml_plan* P, const float* Y, int64_t ldY, int64_t V, const int32_t* rows,
                      int64_t ncols, float* Yt, int64_t ldYt, float* scale, int normalize, float* feats,
                      hipStream_t s, const int32_t* ncols_live = nullptr, bool write_split = true,
-                     bool record_split = true) {
+                     bool record_split = true, const double* coef_in = nullptr, double* coef_out = nullptr) {
     const int64_t ncp = vpad_of(ncols);
     if (Yt && ldYt < ncp) return fail(-3, "prep: ldYt must be >= the padded column count");
     if (ldY < P->d.T) return fail(-3, "prep: ldY must be >= T");
@@ -400,24 +400,24 @@ static int prep_impl(const ffs_reml_plan* P, const float* Y, int64_t ldY, int64_
         if (write_split && record_split) P->split_yt = Ysh ? Yt : nullptr;
         const int sbf = P->d.proj_mode == 1;
         const dim3 grid((unsigned)(ncp / 64)), block(64 * kPrepWaves);
+        if (coef_in && (feats || !scale)) return fail(-3, "prep: given coefficients need the scales and no features");
+#define FFS_PREP(NT_, NF_, CI_)                                                                                     \
+    hipLaunchKernelGGL((k_prep_mfma<NT_, NF_, CI_>), grid, block, 0, s, P->d, Y, ldY, v, rows, nc, ncpi, normalize, \
+                       Yt, ldYt, scale, feats, ncols_live, Ysh, Ysl, sbf, coef_in, coef_out)
         if (P->d.PG == 16) {
-            if (feats)
-                hipLaunchKernelGGL((k_prep_mfma<1, 10>), grid, block, 0, s, P->d, Y, ldY, v, rows, nc, ncpi, normalize,
-                                   Yt, ldYt, scale, feats, ncols_live, Ysh, Ysl, sbf);
-            else
-                hipLaunchKernelGGL((k_prep_mfma<1, 0>), grid, block, 0, s, P->d, Y, ldY, v, rows, nc, ncpi, normalize,
-                                   Yt, ldYt, scale, feats, ncols_live, Ysh, Ysl, sbf);
+            if (feats) FFS_PREP(1, 10, false);
+            else if (coef_in) FFS_PREP(1, 0, true);
+            else FFS_PREP(1, 0, false);
         } else {
-            if (feats)
-                hipLaunchKernelGGL((k_prep_mfma<2, 10>), grid, block, 0, s, P->d, Y, ldY, v, rows, nc, ncpi, normalize,
-                                   Yt, ldYt, scale, feats, ncols_live, Ysh, Ysl, sbf);
-            else
-                hipLaunchKernelGGL((k_prep_mfma<2, 0>), grid, block, 0, s, P->d, Y, ldY, v, rows, nc, ncpi, normalize,
-                                   Yt, ldYt, scale, feats, ncols_live, Ysh, Ysl, sbf);
+            if (feats) FFS_PREP(2, 10, false);
+            else if (coef_in) FFS_PREP(2, 0, true);
+            else FFS_PREP(2, 0, false);
         }
+#undef FFS_PREP
         HIP_TRY(hipGetLastError());
         return 0;
     }
+    if (coef_in || coef_out) return fail(-4, "prep: residual coefficients are kept only for p <= 32");
     if (p <= 8)
         launch_prep<8>(P, Y, ldY, v, rows, nc, ncpi, Yt, ldYt, scale, normalize, feats, s);
     else if (p <= 16)
@@ -444,6 +444,21 @@ extern "C" int ffs_reml_prep(const ffs_reml_plan* P, const float* Y, int64_t ldY
     if (!rows) ncols = V;
     if (ncols <= 0) return 0;
     return prep_impl(P, Y, ldY, V, rows, ncols, Yt, ldYt, scale, normalize, feats, (hipStream_t)stream);
+}
+
+extern "C" int ffs_reml_plan_coef_stride(const ffs_reml_plan* P, int32_t* stride) {
+    if (!P || !stride) return fail(-3, "plan_coef_stride: null argument");
+    *stride = P->d.PG;
+    return 0;
+}
+
+extern "C" int ffs_reml_prep_features(const ffs_reml_plan* P, const float* Y, int64_t ldY, int64_t V, float* scale,
+                                      int32_t normalize, float* feats, double* coef, void* stream) {
+    if (!P) return fail(-3, "prep_features: null plan");
+    if (!scale || !feats) return fail(-3, "prep_features: scale and feats are required");
+    if (V <= 0) return 0;
+    return prep_impl(P, Y, ldY, V, nullptr, V, nullptr, 0, scale, normalize, feats, (hipStream_t)stream, nullptr,
+                     false, false, nullptr, coef);
 }
 
 // n_live (nullable, device): voxels past *n_live are skipped (grid sized for V).
@@ -834,14 +849,14 @@ int set_check(const ffs_reml_plan* P, const SetTable& ST, const char* what, int6
 // voxels grouped by q.key, residualised (with the split halves in the split
 // modes) into the set layout
 int set_group_prep(const ffs_reml_plan* P, const SetTable& ST, const float* Y, int64_t ldY, int64_t V, float* scale,
-                   const SetLayout& L, const SetPtrs& q, hipStream_t s) {
+                   const double* coef, const SetLayout& L, const SetPtrs& q, hipStream_t s) {
     if (int rc = group_by_key(q.key, V, (int)L.n, kSetTile, q.cnt, q.fill, q.cursor, q.rows, L.max_cols, s)) return rc;
     hipLaunchKernelGGL(k_group_tiles, dim3((unsigned)((L.max_tiles + 255) / 256)), dim3(256), 0, s, q.fill, (int)L.n,
                        (int)L.max_tiles, kSetTile, q.tset, q.tcol, q.nt);
     HIP_TRY(hipGetLastError());
     const bool split = P->d.PG && P->d.proj_mode != 0;
     return prep_impl(P, Y, ldY, V, q.rows, L.max_cols, q.Yt, L.max_cols, scale, 1, nullptr, s, q.fill + L.n, split,
-                     false);
+                     false, coef);
 }
 int set_energy(const ffs_reml_plan* P, const SetTable& ST, const SetLayout& L, const SetPtrs& q, hipStream_t s) {
     const int64_t n_cg = (ST.sb / kEnergyNB + kSetWaves - 1) / kSetWaves;
@@ -886,7 +901,7 @@ extern "C" int ffs_reml_local_workspace(const ffs_reml_plan* P, int64_t V, int64
 
 extern "C" int ffs_reml_local_prepare(const ffs_reml_plan* P, const float* Y, int64_t ldY, int64_t V,
                                       const float* props, float conf_min, int32_t* need_full, float* scale,
-                                      void* workspace, int64_t ws_bytes, void* stream) {
+                                      const double* coef, void* workspace, int64_t ws_bytes, void* stream) {
     if (V <= 0) return 0;
     SetLayout L;
     if (int rc = set_check(P, P->d.win, "local search", V, workspace, ws_bytes, &L)) return rc;
@@ -895,15 +910,15 @@ extern "C" int ffs_reml_local_prepare(const ffs_reml_plan* P, const float* Y, in
     hipLaunchKernelGGL(k_win_key, dim3((unsigned)((V + 255) / 256)), dim3(256), 0, s, P->d, props, (int)V, conf_min,
                        q.key, need_full);
     HIP_TRY(hipGetLastError());
-    return set_group_prep(P, P->d.win, Y, ldY, V, scale, L, q, s);
+    return set_group_prep(P, P->d.win, Y, ldY, V, scale, coef, L, q, s);
 }
 
 extern "C" int ffs_reml_local_recenter(const ffs_reml_plan* P, const float* Y, int64_t ldY, int64_t V, float* scale,
-                                       void* workspace, int64_t ws_bytes, void* stream) {
+                                       const double* coef, void* workspace, int64_t ws_bytes, void* stream) {
     if (V <= 0) return 0;
     SetLayout L;
     if (int rc = set_check(P, P->d.win, "local search", V, workspace, ws_bytes, &L)) return rc;
-    return set_group_prep(P, P->d.win, Y, ldY, V, scale, L, set_ptrs(L, workspace), (hipStream_t)stream);
+    return set_group_prep(P, P->d.win, Y, ldY, V, scale, coef, L, set_ptrs(L, workspace), (hipStream_t)stream);
 }
 
 extern "C" int ffs_reml_local_energy(const ffs_reml_plan* P, int64_t V, void* workspace, int64_t ws_bytes,
@@ -926,9 +941,10 @@ extern "C" int ffs_reml_local_project(const ffs_reml_plan* P, int64_t V, int32_t
 
 extern "C" int ffs_reml_local_search(const ffs_reml_plan* P, const float* Y, int64_t ldY, int64_t V,
                                      const float* props, float conf_min, int32_t* best, float* best_score,
-                                     int32_t* need_full, float* scale, void* workspace, int64_t ws_bytes,
-                                     void* stream) {
-    if (int rc = ffs_reml_local_prepare(P, Y, ldY, V, props, conf_min, need_full, scale, workspace, ws_bytes, stream))
+                                     int32_t* need_full, float* scale, const double* coef, void* workspace,
+                                     int64_t ws_bytes, void* stream) {
+    if (int rc = ffs_reml_local_prepare(P, Y, ldY, V, props, conf_min, need_full, scale, coef, workspace, ws_bytes,
+                                        stream))
         return rc;
     if (int rc = ffs_reml_local_energy(P, V, workspace, ws_bytes, stream)) return rc;
     return ffs_reml_local_project(P, V, best, best_score, need_full, workspace, ws_bytes, stream);
@@ -1033,8 +1049,8 @@ extern "C" int ffs_reml_band_workspace(const ffs_reml_plan* P, int64_t V, int64_
 }
 
 extern "C" int ffs_reml_band_prepare(const ffs_reml_plan* P, const float* Y, int64_t ldY, int64_t V,
-                                     const float* props, const int32_t* need_full, float* scale, void* workspace,
-                                     int64_t ws_bytes, void* stream) {
+                                     const float* props, const int32_t* need_full, float* scale,
+                                     const double* coef, void* workspace, int64_t ws_bytes, void* stream) {
     if (V <= 0) return 0;
     SetLayout L;
     if (int rc = set_check(P, P->d.band, "band search", V, workspace, ws_bytes, &L)) return rc;
@@ -1043,7 +1059,7 @@ extern "C" int ffs_reml_band_prepare(const ffs_reml_plan* P, const float* Y, int
     hipLaunchKernelGGL(k_band_key, dim3((unsigned)((V + 255) / 256)), dim3(256), 0, s, P->d, props, (int)V, need_full,
                        q.key);
     HIP_TRY(hipGetLastError());
-    return set_group_prep(P, P->d.band, Y, ldY, V, scale, L, q, s);
+    return set_group_prep(P, P->d.band, Y, ldY, V, scale, coef, L, q, s);
 }
 
 extern "C" int ffs_reml_band_energy(const ffs_reml_plan* P, int64_t V, void* workspace, int64_t ws_bytes,
@@ -1065,8 +1081,10 @@ extern "C" int ffs_reml_band_project(const ffs_reml_plan* P, int64_t V, int32_t*
 
 extern "C" int ffs_reml_band_search(const ffs_reml_plan* P, const float* Y, int64_t ldY, int64_t V,
                                     const float* props, int32_t* best, float* best_score, int32_t* need_full,
-                                    float* scale, void* workspace, int64_t ws_bytes, void* stream) {
-    if (int rc = ffs_reml_band_prepare(P, Y, ldY, V, props, need_full, scale, workspace, ws_bytes, stream)) return rc;
+                                    float* scale, const double* coef, void* workspace, int64_t ws_bytes,
+                                    void* stream) {
+    if (int rc = ffs_reml_band_prepare(P, Y, ldY, V, props, need_full, scale, coef, workspace, ws_bytes, stream))
+        return rc;
     if (int rc = ffs_reml_band_energy(P, V, workspace, ws_bytes, stream)) return rc;
     return ffs_reml_band_project(P, V, best, best_score, need_full, workspace, ws_bytes, stream);
 }
@@ -1105,8 +1123,8 @@ extern "C" int ffs_reml_subset_workspace(const ffs_reml_plan* P, int64_t cap, in
 
 extern "C" int ffs_reml_subset_search(const ffs_reml_plan* P, const float* Y, int64_t ldY, int64_t V,
                                       const int32_t* idx, const int32_t* n_live, int64_t cap, int32_t* best,
-                                      float* best_score, float* scale, void* workspace, int64_t ws_bytes,
-                                      void* stream) {
+                                      float* best_score, float* scale, const double* coef, void* workspace,
+                                      int64_t ws_bytes, void* stream) {
     if (!P) return fail(-3, "subset_search: null plan");
     if (V <= 0 || cap <= 0) return 0;
     if (!idx || !n_live) return fail(-3, "subset_search: idx and n_live are required");
@@ -1122,7 +1140,7 @@ extern "C" int ffs_reml_subset_search(const ffs_reml_plan* P, const float* Y, in
     hipStream_t s = (hipStream_t)stream;
     const int K = P->d.K;
     const bool split = P->d.PG && P->d.proj_mode != 0;
-    if (int rc = prep_impl(P, Y, ldY, V, idx, cap, Yt, L.cap_pad, scale, 1, nullptr, s, n_live, split, false))
+    if (int rc = prep_impl(P, Y, ldY, V, idx, cap, Yt, L.cap_pad, scale, 1, nullptr, s, n_live, split, false, coef))
         return rc;
     if (int rc = energy_impl(P, Yt, L.cap_pad, cap, S, K, n_live, s)) return rc;
     if (int rc = project_impl(P, Yt, L.cap_pad, cap, S, K, n_live, split, s)) return rc;

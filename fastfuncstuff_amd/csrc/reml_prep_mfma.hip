@@ -35,9 +35,10 @@ __device__ __forceinline__ int prep_voxel(int c, int ncols, const int32_t* __res
 //           added in wave order.
 // The A operand of pass 2 permutes the tile rows so that a lane's four output
 // rows are four consecutive time points.  Every voxel's arithmetic is the same
-// whatever column it lands in.
+// whatever column it lands in.  CI: pass 1 is replaced by coef_in (c of each
+// voxel, as coef_out stores it) and scale (read, not written).
 // ---------------------------------------------------------------------------
-template <int NT, int NF>
+template <int NT, int NF, bool CI>
 __global__ void __launch_bounds__(64 * kPrepWaves) k_prep_mfma(PlanDev P, const float* __restrict__ Y, int64_t ldY,
                                                                int V, const int32_t* __restrict__ rows, int ncols,
                                                                int ncols_pad, int normalize, float* __restrict__ Yt,
@@ -45,7 +46,8 @@ __global__ void __launch_bounds__(64 * kPrepWaves) k_prep_mfma(PlanDev P, const 
                                                                float* __restrict__ feats,
                                                                const int32_t* __restrict__ ncols_live,
                                                                uint16_t* __restrict__ Ysh, uint16_t* __restrict__ Ysl,
-                                                               int split_bf16) {
+                                                               int split_bf16, const double* __restrict__ coef_in,
+                                                               double* __restrict__ coef_out) {
     constexpr int PG = 16 * NT;
     constexpr int RB = 65;  // feature tile pitch (floats)
     constexpr int W = kPrepWaves;
@@ -95,7 +97,7 @@ __global__ void __launch_bounds__(64 * kPrepWaves) k_prep_mfma(PlanDev P, const 
             for (int rt = 0; rt < NT; ++rt) a[m][rt] = Pp[(size_t)(t0 + 4 * kq + m) * PG + 16 * rt + j];
     };
     constexpr int D = prep_depth(NT, NF);
-    {
+    if (!CI) {
         float yq[D][4][4];
         double aq[D][4][NT];
 #pragma unroll
@@ -131,61 +133,86 @@ __global__ void __launch_bounds__(64 * kPrepWaves) k_prep_mfma(PlanDev P, const 
             }
         }
     }
-    // partial sums of the block's waves, added in wave order by every wave
-#pragma unroll
-    for (int g = 0; g < 4; ++g) {
-        s1[g] += __shfl_xor(s1[g], 16);
-        s1[g] += __shfl_xor(s1[g], 32);
-        s2[g] += __shfl_xor(s2[g], 16);
-        s2[g] += __shfl_xor(s2[g], 32);
-    }
-    {
-        int q = 0;
-#pragma unroll
-        for (int g = 0; g < 4; ++g)
-#pragma unroll
-            for (int rt = 0; rt < NT; ++rt)
-#pragma unroll
-                for (int i = 0; i < 4; ++i) red[w][q++][lane] = acc[g][rt][i];
-#pragma unroll
-        for (int g = 0; g < 4; ++g) {
-            red[w][q++][lane] = s1[g];
-            red[w][q++][lane] = s2[g];
-        }
-    }
-    __syncthreads();
-    {
-        auto sum = [&](int q) __attribute__((always_inline)) {
-            double x = red[0][q][lane];
-#pragma unroll
-            for (int ww = 1; ww < W; ++ww) x += red[ww][q][lane];
-            return x;
-        };
-        int q = 0;
-#pragma unroll
-        for (int g = 0; g < 4; ++g)
-#pragma unroll
-            for (int rt = 0; rt < NT; ++rt)
-#pragma unroll
-                for (int i = 0; i < 4; ++i) acc[g][rt][i] = sum(q++);
-#pragma unroll
-        for (int g = 0; g < 4; ++g) {
-            s1[g] = sum(q++);
-            s2[g] = sum(q++);
-        }
-    }
     double inv[4];
+    // partial sums of the block's waves, added in wave order by every wave
+    // (CI: the coefficients and scales of an earlier call are given instead)
+    if (!CI) {
 #pragma unroll
-    for (int g = 0; g < 4; ++g) {
-        double s = 1.0;
-        if (normalize & 1) {
-            const double mean = s1[g] / T;
-            const double var = T > 1 ? (s2[g] - T * mean * mean) / (T - 1) : 0.0;
-            const double sd = var > 0.0 ? sqrt(var) : 0.0;
-            s = (double)fmaxf((float)sd, 1e-8f);
+        for (int g = 0; g < 4; ++g) {
+            s1[g] += __shfl_xor(s1[g], 16);
+            s1[g] += __shfl_xor(s1[g], 32);
+            s2[g] += __shfl_xor(s2[g], 16);
+            s2[g] += __shfl_xor(s2[g], 32);
         }
-        inv[g] = 1.0 / s;
-        if (w == 0 && kq == 0 && vB[g] >= 0 && scale) scale[vB[g]] = (float)s;
+        {
+            int q = 0;
+#pragma unroll
+            for (int g = 0; g < 4; ++g)
+#pragma unroll
+                for (int rt = 0; rt < NT; ++rt)
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) red[w][q++][lane] = acc[g][rt][i];
+#pragma unroll
+            for (int g = 0; g < 4; ++g) {
+                red[w][q++][lane] = s1[g];
+                red[w][q++][lane] = s2[g];
+            }
+        }
+        __syncthreads();
+        {
+            auto sum = [&](int q) __attribute__((always_inline)) {
+                double x = red[0][q][lane];
+#pragma unroll
+                for (int ww = 1; ww < W; ++ww) x += red[ww][q][lane];
+                return x;
+            };
+            int q = 0;
+#pragma unroll
+            for (int g = 0; g < 4; ++g)
+#pragma unroll
+                for (int rt = 0; rt < NT; ++rt)
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) acc[g][rt][i] = sum(q++);
+#pragma unroll
+            for (int g = 0; g < 4; ++g) {
+                s1[g] = sum(q++);
+                s2[g] = sum(q++);
+            }
+        }
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+            double s = 1.0;
+            if (normalize & 1) {
+                const double mean = s1[g] / T;
+                const double var = T > 1 ? (s2[g] - T * mean * mean) / (T - 1) : 0.0;
+                const double sd = var > 0.0 ? sqrt(var) : 0.0;
+                s = (double)fmaxf((float)sd, 1e-8f);
+            }
+            inv[g] = 1.0 / s;
+            if (w == 0 && kq == 0 && vB[g] >= 0 && scale) scale[vB[g]] = (float)s;
+        }
+        if (coef_out) {
+            // coefficient 16 rt + 4 kq + i of the voxel in column 16 g + j; wave g % W writes group g
+#pragma unroll
+            for (int g = 0; g < 4; ++g)
+                if (g % W == w && vB[g] >= 0)
+#pragma unroll
+                    for (int rt = 0; rt < NT; ++rt)
+#pragma unroll
+                        for (int i = 0; i < 4; ++i) coef_out[(size_t)vB[g] * PG + 16 * rt + 4 * kq + i] = acc[g][rt][i];
+        }
+    } else {
+        // c and s of an earlier call with the same data (bitwise the values it
+        // computed: s is a float widened to double)
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+            const bool ok = vB[g] >= 0;
+#pragma unroll
+            for (int rt = 0; rt < NT; ++rt)
+#pragma unroll
+                for (int i = 0; i < 4; ++i) acc[g][rt][i] = ok ? coef_in[(size_t)vB[g] * PG + 16 * rt + 4 * kq + i] : 0.0;
+            inv[g] = ok ? 1.0 / (double)scale[vB[g]] : 1.0;
+        }
     }
     if (!Yt && NF == 0) return;
     if (normalize & 2) {  // keep the design component (no residualisation)

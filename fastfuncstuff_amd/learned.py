@@ -186,6 +186,12 @@ class LocalSearcher:
         self.scale = torch.empty(V, dtype=torch.float32, device=dev)
         self.idx = torch.empty(V, dtype=torch.int32, device=dev)
         self.count = torch.zeros(1, dtype=torch.int32, device=dev)
+        self.feats = torch.empty(V, 10, dtype=torch.float32, device=dev)
+        # residualisation coefficients of the features pass, reused by the
+        # window / band / subset preparations (None where the plan keeps none)
+        stride = ctypes.c_int32()
+        _lib.check(lib.ffs_reml_plan_coef_stride(grid.handle, ctypes.byref(stride)), "plan_coef_stride")
+        self.coef = (torch.empty(V, stride.value, dtype=torch.float64, device=dev) if stride.value > 0 else None)
         self.bws = None
         if grid.bands is not None:
             _lib.check(lib.ffs_reml_band_workspace(grid.handle, V, ctypes.byref(nb)), "band_workspace")
@@ -224,14 +230,16 @@ class LocalSearcher:
                 timer.mark(name)
 
         mark("features")
-        _, _, feats = grid.prep_features(Y)
+        coef = _lib.ptr(self.coef) if self.coef is not None else None
+        _lib.check(lib.ffs_reml_prep_features(grid.handle, _lib.ptr(Y), Y.stride(0), V, _lib.ptr(self.scale), 1,
+                                              _lib.ptr(self.feats), coef, st), "prep_features")
         mark("mlp")
-        props = self.mlp.forward(feats)
+        props = self.mlp.forward(self.feats)
         mark("local_prepare")
         ws, nb = _lib.ptr(self.ws), self.ws.numel()
         _lib.check(lib.ffs_reml_local_prepare(grid.handle, _lib.ptr(Y), Y.stride(0), V, _lib.ptr(props),
                                               float(self.mlp.conf_min), _lib.ptr(self.need), _lib.ptr(self.scale),
-                                              ws, nb, st), "local_prepare")
+                                              coef, ws, nb, st), "local_prepare")
         if stats is not None:
             stats["window"], stats["window_slots"] = window_load(grid, props, float(self.mlp.conf_min))
         mark("local_energy")
@@ -243,8 +251,8 @@ class LocalSearcher:
             if stats is not None:
                 stats.setdefault("walk", []).append(int((self.need == 1).sum()))
             mark("walk_prepare")
-            _lib.check(lib.ffs_reml_local_recenter(grid.handle, _lib.ptr(Y), Y.stride(0), V, _lib.ptr(self.scale), ws,
-                                                   nb, st), "local_recenter")
+            _lib.check(lib.ffs_reml_local_recenter(grid.handle, _lib.ptr(Y), Y.stride(0), V, _lib.ptr(self.scale),
+                                                   coef, ws, nb, st), "local_recenter")
             mark("walk_energy")
             _lib.check(lib.ffs_reml_local_energy(grid.handle, V, ws, nb, st), "local_energy")
             mark("walk_project")
@@ -257,7 +265,7 @@ class LocalSearcher:
             bws, bnb = _lib.ptr(self.bws), self.bws.numel()
             mark("band_prepare")
             _lib.check(lib.ffs_reml_band_prepare(grid.handle, _lib.ptr(Y), Y.stride(0), V, _lib.ptr(props),
-                                                 _lib.ptr(self.need), _lib.ptr(self.scale), bws, bnb, st),
+                                                 _lib.ptr(self.need), _lib.ptr(self.scale), coef, bws, bnb, st),
                        "band_prepare")
             mark("band_energy")
             _lib.check(lib.ffs_reml_band_energy(grid.handle, V, bws, bnb, st), "band_energy")
@@ -271,7 +279,7 @@ class LocalSearcher:
         # on the host; the count is read once everything is queued
         _lib.check(lib.ffs_reml_subset_search(grid.handle, _lib.ptr(Y), Y.stride(0), V, _lib.ptr(self.idx),
                                               _lib.ptr(self.count), self.cap, _lib.ptr(self.best),
-                                              _lib.ptr(self.score), _lib.ptr(self.scale), _lib.ptr(self.sws),
+                                              _lib.ptr(self.score), _lib.ptr(self.scale), coef, _lib.ptr(self.sws),
                                               self.sws.numel(), st), "subset_search")
         n_fb = int(self.count.item())
         if n_fb > self.cap:

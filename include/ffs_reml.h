@@ -105,6 +105,20 @@ int ffs_reml_prep(const ffs_reml_plan* plan, const float* Y, int64_t ldY, int64_
                   int64_t ncols, float* Yt, int64_t ldYt, float* scale, int32_t normalize, float* feats,
                   void* stream);
 
+/* Features of every voxel for the learned proposal (no counterpart in the
+ * reference): scale[v] and feats[v*10 + k] as ffs_reml_prep with rows = NULL
+ * and Yt = NULL, and, when coef is not NULL, each voxel's residualisation
+ * coefficients pinv(X) y / scale (normalised units) in
+ * coef[v*stride .. v*stride + stride), stride from ffs_reml_plan_coef_stride.
+ * The window, band and subset preparations given that coef and scale (same Y,
+ * normalize = 1) skip their first pass and produce bitwise the residuals they
+ * would compute themselves.  Returns -4 with coef when stride is 0 (p > 32). */
+int ffs_reml_prep_features(const ffs_reml_plan* plan, const float* Y, int64_t ldY, int64_t V, float* scale,
+                           int32_t normalize, float* feats, double* coef, void* stream);
+/* Doubles per voxel of the coefficient array of ffs_reml_prep_features (16 or
+ * 32; 0 when the plan's design is wider than 32 columns). */
+int ffs_reml_plan_coef_stride(const ffs_reml_plan* plan, int32_t* stride);
+
 /* (i) E[v*ldE + k] = ||L_k^-1 y_v||^2 for every voxel and pair. */
 int ffs_reml_energy(const ffs_reml_plan* plan, const float* Yt, int64_t ldYt, int64_t V, float* E, int64_t ldE,
                     void* stream);
@@ -171,7 +185,9 @@ int ffs_reml_ljung_box(const double* resid_w, int64_t ldR, int64_t V, int32_t n,
  * voxel: best pair (-1 if no valid pair), its score, need_full = 1 when the
  * optimum is on a window edge that is not a lattice edge, 2 when the confidence
  * is below conf_min or the proposal is not finite (no window is searched; best
- * and score are left as they were); scale = per-voxel std.  The split
+ * and score are left as they were); scale = per-voxel std, or, when coef
+ * (nullable) is given, the scale written by ffs_reml_prep_features with that
+ * coef, which is then read instead of recomputed.  The split
  * projection modes need p <= 32 (returns -4 otherwise).  workspace:
  * ffs_reml_local_workspace bytes for the plan's current projection mode.
  */
@@ -180,8 +196,8 @@ int ffs_reml_local_workspace(const ffs_reml_plan* plan, int64_t V, int64_t* byte
  * window centres + grouping + grouped residualisation; window energies;
  * window projection + score + local argmin. */
 int ffs_reml_local_prepare(const ffs_reml_plan* plan, const float* Y, int64_t ldY, int64_t V, const float* props,
-                           float conf_min, int32_t* need_full, float* scale, void* workspace, int64_t ws_bytes,
-                           void* stream);
+                           float conf_min, int32_t* need_full, float* scale, const double* coef, void* workspace,
+                           int64_t ws_bytes, void* stream);
 int ffs_reml_local_energy(const ffs_reml_plan* plan, int64_t V, void* workspace, int64_t ws_bytes, void* stream);
 int ffs_reml_local_project(const ffs_reml_plan* plan, int64_t V, int32_t* best, float* best_score,
                            int32_t* need_full, void* workspace, int64_t ws_bytes, void* stream);
@@ -192,10 +208,10 @@ int ffs_reml_local_project(const ffs_reml_plan* plan, int64_t V, int32_t* best, 
  * best_score and need_full of those voxels only.  Same workspace and stream
  * as the preceding ffs_reml_local_project. */
 int ffs_reml_local_recenter(const ffs_reml_plan* plan, const float* Y, int64_t ldY, int64_t V, float* scale,
-                            void* workspace, int64_t ws_bytes, void* stream);
+                            const double* coef, void* workspace, int64_t ws_bytes, void* stream);
 int ffs_reml_local_search(const ffs_reml_plan* plan, const float* Y, int64_t ldY, int64_t V, const float* props,
                           float conf_min, int32_t* best, float* best_score, int32_t* need_full, float* scale,
-                          void* workspace, int64_t ws_bytes, void* stream);
+                          const double* coef, void* workspace, int64_t ws_bytes, void* stream);
 
 /*
  * Band search (no counterpart in the reference, which searches the whole grid,
@@ -222,13 +238,14 @@ int ffs_reml_band_workspace(const ffs_reml_plan* plan, int64_t V, int64_t* bytes
  * grouping and grouped residualisation; band energies; band projection, score
  * and band argmin. */
 int ffs_reml_band_prepare(const ffs_reml_plan* plan, const float* Y, int64_t ldY, int64_t V, const float* props,
-                          const int32_t* need_full, float* scale, void* workspace, int64_t ws_bytes, void* stream);
+                          const int32_t* need_full, float* scale, const double* coef, void* workspace,
+                          int64_t ws_bytes, void* stream);
 int ffs_reml_band_energy(const ffs_reml_plan* plan, int64_t V, void* workspace, int64_t ws_bytes, void* stream);
 int ffs_reml_band_project(const ffs_reml_plan* plan, int64_t V, int32_t* best, float* best_score,
                           int32_t* need_full, void* workspace, int64_t ws_bytes, void* stream);
 int ffs_reml_band_search(const ffs_reml_plan* plan, const float* Y, int64_t ldY, int64_t V, const float* props,
-                         int32_t* best, float* best_score, int32_t* need_full, float* scale, void* workspace,
-                         int64_t ws_bytes, void* stream);
+                         int32_t* best, float* best_score, int32_t* need_full, float* scale, const double* coef,
+                         void* workspace, int64_t ws_bytes, void* stream);
 
 /*
  * Full-grid search of the voxels idx[0 .. min(*n_live, cap)) (idx and n_live
@@ -236,13 +253,14 @@ int ffs_reml_band_search(const ffs_reml_plan* plan, const float* Y, int64_t ldY,
  * count: normalise + residualise, energies and projection over every pair in
  * the plan's projection mode, first minimum over the grid (the exhaustive
  * rule of glm/arma.py:1886-2018); writes best[idx[i]], best_score[idx[i]] and
- * scale[idx[i]].  Voxels past cap are left untouched (the caller compares
+ * scale[idx[i]] (read instead, with coef, as in ffs_reml_local_search).
+ * Voxels past cap are left untouched (the caller compares
  * *n_live with cap afterwards).  workspace: ffs_reml_subset_workspace(cap) bytes.
  */
 int ffs_reml_subset_workspace(const ffs_reml_plan* plan, int64_t cap, int64_t* bytes);
 int ffs_reml_subset_search(const ffs_reml_plan* plan, const float* Y, int64_t ldY, int64_t V, const int32_t* idx,
                            const int32_t* n_live, int64_t cap, int32_t* best, float* best_score, float* scale,
-                           void* workspace, int64_t ws_bytes, void* stream);
+                           const double* coef, void* workspace, int64_t ws_bytes, void* stream);
 
 /* idx[0..*count) = voxels with flags[v] != 0 (order unspecified); count is device memory. */
 int ffs_reml_compact(const int32_t* flags, int64_t V, int32_t* idx, int32_t* count, void* stream);

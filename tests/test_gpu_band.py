@@ -116,8 +116,8 @@ def test_band_scores_bitwise_equal_full_grid(dev, mode):
     _lib.check(lib.ffs_reml_band_workspace(grid.handle, V, ctypes.byref(nb)), "ws")
     ws = torch.zeros(int(nb.value), dtype=torch.uint8, device=dev)
     _lib.check(lib.ffs_reml_band_search(grid.handle, _lib.ptr(Y), Y.stride(0), V, _lib.ptr(props), _lib.ptr(best),
-                                        _lib.ptr(score), _lib.ptr(need), _lib.ptr(scale), _lib.ptr(ws), ws.numel(),
-                                        _lib.stream_handle(dev)), "band_search")
+                                        _lib.ptr(score), _lib.ptr(need), _lib.ptr(scale), None, _lib.ptr(ws),
+                                        ws.numel(), _lib.stream_handle(dev)), "band_search")
     best, score, need, scale = best.cpu().long(), score.cpu(), need.cpu(), scale.cpu()
     need_in = need_in.cpu()
     bands, edges = _bands_restated(grid)
@@ -185,14 +185,15 @@ def test_learned_search_with_bands(dev):
     assert res1.n_fallback < res0.n_fallback
 
 
-@pytest.mark.parametrize("mode", ["fp32", "fp16x3"])
-def test_subset_search_matches_exhaustive(dev, mode):
+@pytest.mark.parametrize("mode,use_coef", [("fp32", False), ("fp16x3", False), ("fp16x3", True)])
+def test_subset_search_matches_exhaustive(dev, mode, use_coef):
     """Full-grid search of a device-counted subset: the first `cap` listed
     voxels get the exhaustive first minimum of the full surface (bitwise
-    score); the others are untouched."""
+    score); the others are untouched.  use_coef: the residualisation
+    coefficients and scales come from ffs_reml_prep_features instead."""
     from fastfuncstuff_amd import _lib
 
-    print(f"[band] subset search mode={mode}", flush=True)
+    print(f"[band] subset search mode={mode} coef={use_coef}", flush=True)
     lib = _lib.load()
     grid, Y = _grid(dev, 900)
     grid.set_projection(mode)
@@ -210,12 +211,24 @@ def test_subset_search_matches_exhaustive(dev, mode):
     best = torch.full((V,), -7, dtype=torch.int32, device=dev)
     score = torch.full((V,), 123.0, dtype=torch.float32, device=dev)
     scale = torch.full((V,), -1.0, dtype=torch.float32, device=dev)
+    coef = None
+    if use_coef:
+        stride = ctypes.c_int32()
+        _lib.check(lib.ffs_reml_plan_coef_stride(grid.handle, ctypes.byref(stride)), "stride")
+        assert stride.value == 16
+        coef = torch.full((V, stride.value), float("nan"), dtype=torch.float64, device=dev)
+        feats = torch.empty(V, 10, dtype=torch.float32, device=dev)
+        _lib.check(lib.ffs_reml_prep_features(grid.handle, _lib.ptr(Y), Y.stride(0), V, _lib.ptr(scale), 1,
+                                              _lib.ptr(feats), _lib.ptr(coef), _lib.stream_handle(dev)), "features")
+        assert torch.equal(scale.cpu(), scale_ref.cpu())
+        assert bool(torch.isfinite(coef).all())
     nb = ctypes.c_int64()
     _lib.check(lib.ffs_reml_subset_workspace(grid.handle, cap, ctypes.byref(nb)), "ws")
     ws = torch.empty(int(nb.value), dtype=torch.uint8, device=dev)
     _lib.check(lib.ffs_reml_subset_search(grid.handle, _lib.ptr(Y), Y.stride(0), V, _lib.ptr(idx_d), _lib.ptr(count),
-                                          cap, _lib.ptr(best), _lib.ptr(score), _lib.ptr(scale), _lib.ptr(ws),
-                                          ws.numel(), _lib.stream_handle(dev)), "subset")
+                                          cap, _lib.ptr(best), _lib.ptr(score), _lib.ptr(scale),
+                                          _lib.ptr(coef) if use_coef else None, _lib.ptr(ws), ws.numel(),
+                                          _lib.stream_handle(dev)), "subset")
     best, score, scale = best.cpu(), score.cpu(), scale.cpu()
     done = idx[:cap].long()
     assert torch.equal(best[done], b_ref[done])
@@ -224,3 +237,5 @@ def test_subset_search_matches_exhaustive(dev, mode):
     rest = torch.ones(V, dtype=torch.bool)
     rest[done] = False
     assert bool((best[rest] == -7).all()) and bool((score[rest] == 123.0).all())
+    if not use_coef:
+        assert bool((scale[rest] == -1.0).all())

@@ -94,11 +94,14 @@ def _grid_and_data(dev, V, T=120, seed=0, noise_seed=None):
     return grid, torch.from_numpy(wl.Y).to(dev), wl
 
 
-@pytest.mark.parametrize("mode", ["fp32", "fp16x3", "bf16x3"])
-def test_window_scores_bitwise_equal_full_grid(dev, mode):
+@pytest.mark.parametrize("mode,use_coef", [("fp32", False), ("fp16x3", False), ("bf16x3", False),
+                                           ("fp32", True), ("fp16x3", True)])
+def test_window_scores_bitwise_equal_full_grid(dev, mode, use_coef):
+    """use_coef: the window preparation takes each voxel's residualisation
+    coefficients and scale from ffs_reml_prep_features (no first pass)."""
     from fastfuncstuff_amd.learned import LocalSearcher, ProposalMLP
 
-    print(f"[learned] window bitwise mode={mode}", flush=True)
+    print(f"[learned] window bitwise mode={mode} coef={use_coef}", flush=True)
     grid, Y, _ = _grid_and_data(dev, 700, seed=11)
     grid.set_projection(mode)
     V = Y.shape[0]
@@ -124,8 +127,15 @@ def test_window_scores_bitwise_equal_full_grid(dev, mode):
     score = torch.empty(V, dtype=torch.float32, device=dev)
     need = torch.empty(V, dtype=torch.int32, device=dev)
     scale = torch.empty(V, dtype=torch.float32, device=dev)
+    coef = None
+    if use_coef:
+        assert ls.coef is not None
+        feats = torch.empty(V, 10, dtype=torch.float32, device=dev)
+        _lib.check(lib.ffs_reml_prep_features(grid.handle, _lib.ptr(Y), Y.stride(0), V, _lib.ptr(scale), 1,
+                                              _lib.ptr(feats), _lib.ptr(ls.coef), _lib.stream_handle(dev)), "feats")
+        coef = _lib.ptr(ls.coef)
     _lib.check(lib.ffs_reml_local_search(grid.handle, _lib.ptr(Y), Y.stride(0), V, _lib.ptr(props), -1e30,
-                                         _lib.ptr(best), _lib.ptr(score), _lib.ptr(need), _lib.ptr(scale),
+                                         _lib.ptr(best), _lib.ptr(score), _lib.ptr(need), _lib.ptr(scale), coef,
                                          _lib.ptr(ls.ws), ls.ws.numel(), _lib.stream_handle(dev)), "local")
     best, score = best.cpu().long(), score.cpu()
     lut = grid.lut
