@@ -291,17 +291,24 @@ __global__ void __launch_bounds__(64 * kPrepWaves) k_prep_mfma(PlanDev P, const 
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            // fp32 sums over the step's 16 samples, added to the fp64 totals once per step
+            float p0 = 0.f, pk[NF > 0 ? NF : 1];
+#pragma unroll
+            for (int k = 0; k < NF; ++k) pk[k] = 0.f;
 #pragma unroll
             for (int i = 0; i < 16; ++i) {
                 const float x = rb[i * RB + lane];  // zero past T
-                if (emit) {
-                    acf0 += (double)x * x;
+                p0 = fmaf(x, x, p0);
 #pragma unroll
-                    for (int k = 0; k < NF; ++k) acf[k] += (double)x * hist[k];
-                }
+                for (int k = 0; k < NF; ++k) pk[k] = fmaf(x, hist[k], pk[k]);
 #pragma unroll
                 for (int k = NF - 1; k > 0; --k) hist[k] = hist[k - 1];
                 hist[0] = x;
+            }
+            if (emit) {
+                acf0 += (double)p0;
+#pragma unroll
+                for (int k = 0; k < NF; ++k) acf[k] += (double)pk[k];
             }
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
