@@ -310,7 +310,7 @@ def main():
     # with the profiles; used when the run is that workload
     tfile = ROOT / "profiles" / "traffic_bench_T400_V200k.json"
     if tfile.exists() and V == 200_000 and args.T == 400 and args.projection == "fp16x3":
-        tk = json.loads(tfile.read_text())["kernels"].get(dom)
+        tk = json.loads(tfile.read_text()).get("kernels", {}).get(dom)
         if tk is not None:
             roof["traffic"] = tk["bytes"]
             roof["traffic_source"] = f"profiles/{tfile.name} ({tk['launch']})"
