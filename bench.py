@@ -8,9 +8,11 @@ random-walk motion columns, constant, linear), 33x33 (a, b) grid in
 Two paths are timed over the same batch, each step = one full fit pass:
 * full grid: normalise + residualise, recurrence energy (i), MFMA projection +
   REML score (ii), hierarchical selection (iii), FP64 GLS refit (iv);
-* learned proposal (the reported ``value``): residual-ACF features, MLP
-  proposal, 5x5-window energy + projection + local argmin, full-grid fallback
-  for edge hits / low confidence, FP64 GLS refit.
+* learned proposal (the reported ``value``): residual-ACF features (which
+  also keep each voxel's residualisation coefficients), MLP proposal, 5x5-window
+  search, lambda-band search for low-confidence and window-edge voxels,
+  full-grid search of the voxels left (band edges, non-finite proposals), FP64
+  GLS refit.
 The MLP is trained beforehand (untimed) on separately drawn voxels of the same
 design, labelled by the full-grid GPU search.  Prints one JSON line.
 ``--gpus N`` with N > 1 exits: this build measures one GPU.
@@ -53,7 +55,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--projection", default="fp16x3", choices=["fp32", "fp16x3", "bf16x3"],
                     help="arithmetic of the full-grid projection GEMM")
-    ap.add_argument("--band-width", type=float, default=0.14,
+    ap.add_argument("--band-width", type=float, default=0.10,
                     help="half-width of the lambda bands of the learned search (0: no band search)")
     ap.add_argument("--band-bin", type=float, default=0.02, help="spacing of the lambda band centres")
     ap.add_argument("--walk", type=int, default=None,

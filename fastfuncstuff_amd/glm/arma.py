@@ -341,7 +341,7 @@ class RemlGrid(Mapping):
         _lib.check(_lib.load().ffs_reml_plan_set_projection(self._handle, self.PROJECTIONS[mode]), "set_projection")
         self.projection = mode
 
-    def set_bands(self, half_width: float = 0.14, bin_width: float = 0.02):
+    def set_bands(self, half_width: float = 0.10, bin_width: float = 0.02):
         """Lag-1 correlation bands for the learned search: band i holds the
         pairs whose lambda lies within ``half_width`` of ``-1 + i * bin_width``.
         ``half_width <= 0`` removes them."""
