@@ -149,7 +149,10 @@ __device__ inline SetItems set_items(int n_tiles, int per_tile) {
 }
 
 // Energies E[col][slot] of every live 64-column tile and every 8-slot chunk of
-// its set (one chunk per wave).
+// its set.  An item is (chunk, quad of 4 consecutive tiles of the XCD's range),
+// wave w taking tile 4 quad + w: the waves of a block, which mostly share a set,
+// stream the same chunk coefficients through the scalar cache, and the items
+// run chunk-major, so the blocks of an XCD share them in L2.
 __global__ void __launch_bounds__(64 * kSetWaves) k_set_energy(PlanDev P, SetTable ST, const float* __restrict__ Yt,
                                                                int64_t ldY, const int32_t* __restrict__ tset,
                                                                const int32_t* __restrict__ n_tiles,
@@ -157,15 +160,19 @@ __global__ void __launch_bounds__(64 * kSetWaves) k_set_energy(PlanDev P, SetTab
     const int lane = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int SB = ST.sb, MC = SB / kEnergyNB;
-    const int n_cg = (MC + kSetWaves - 1) / kSetWaves;
-    const SetItems I = set_items(*n_tiles, n_cg);
+    const int nt = *n_tiles, per = (nt + 7) >> 3;
+    const int lo = (blockIdx.x & 7) * per;
+    const int n_xt = max(0, min(nt, lo + per) - lo);  // tiles of this XCD
+    const int nq = (n_xt + kSetWaves - 1) / kSetWaves;
+    const int n_items = nq * MC;
     const int T = P.T;
     constexpr int U = 8;  // steps per fp64 flush
     const int nsteps = T - 1;
     const int nfull = nsteps / U * U;
-    for (int it = I.lb; it < I.n; it += I.nlb) {
-        const int tt = I.lo + it / n_cg;
-        const int chunk = (it % n_cg) * kSetWaves + w;
+    for (int it = blockIdx.x >> 3; it < n_items; it += gridDim.x >> 3) {
+        const int chunk = it / nq, tl = (it % nq) * kSetWaves + w;
+        if (tl >= n_xt) continue;
+        const int tt = lo + tl;
         const int set = tset[tt];
         if (chunk * kEnergyNB >= ST.cnt[set]) continue;  // past the set
         const int col = tt * kSetTile + lane;
