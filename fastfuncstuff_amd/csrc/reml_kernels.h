@@ -46,9 +46,6 @@ constexpr int kGlsMfmaMaxP = 32;
 // Pair-set search: voxel groups padded to 64-column tiles; blocks of 4 waves.
 constexpr int kSetTile = 64;
 constexpr int kSetWaves = 4;
-// (row, pair) scores per lane and 32-row block whose operands the set
-// projection fetches ahead of its MFMA loop (all of them for npt <= 8).
-constexpr int kPf = 4;
 
 // A family of pair sets (lambda bands, lattice windows).  Set s holds cnt[s]
 // pairs, in pair order, in slots [s*sb, s*sb + cnt[s]) (-1 past them); sb is a

@@ -267,13 +267,6 @@ __global__ void __launch_bounds__(64 * kSetWaves) k_set_project(PlanDev P, SetTa
             bst[n] = k >= 0 ? k / npt : 0;
             bcc[n] = k >= 0 ? (k - bst[n] * npt) * p + j : 0;
         }
-        // Pairs of this lane's first kPf (row, pair) scores per 32-row block.
-        int kq[kPf];
-#pragma unroll
-        for (int jj = 0; jj < kPf; ++jj) {
-            const int i = lane + 64 * jj;
-            kq[jj] = i < 32 * npt ? kp[i % npt] : -1;
-        }
         f32x16 acc[2][NBLK];
 #pragma unroll
         for (int r = 0; r < 2; ++r)
@@ -352,19 +345,19 @@ __global__ void __launch_bounds__(64 * kSetWaves) k_set_project(PlanDev P, SetTa
                 }
             };
             __syncthreads();  // the patch (aliased by the A stages) is free
-            {
-                u32x4 ra[2];
-                load_a(0, ra);
-                store_a(0, ra);
-            }
+            // A stage st + 1 is fetched into registers during stage st - 1 and
+            // stored to LDS at the end of stage st (two stages of latency cover)
+            u32x4 ra[2], ra2[2];
+            load_a(0, ra);
+            store_a(0, ra);
+            if (nst > 1) load_a(1, ra);
             u32x4 bh[NBLK], bl[NBLK];
             load_b(0, bh, bl);
             __syncthreads();
             const int nks = 2 * nst;
-            u32x4 ra[2];
             for (int ks = 0; ks < nks; ++ks) {
-                const int buf = (ks >> 1) & 1, half = ks & 1;
-                if (half == 0 && ks + 2 < nks) load_a((ks >> 1) + 1, ra);  // next stage's A
+                const int st = ks >> 1, buf = st & 1, half = ks & 1;
+                if (half == 0 && st + 2 < nst) load_a(st + 2, ra2);
                 u32x4 bhc[NBLK], blc[NBLK];
 #pragma unroll
                 for (int n = 0; n < NBLK; ++n) {
@@ -394,26 +387,14 @@ __global__ void __launch_bounds__(64 * kSetWaves) k_set_project(PlanDev P, SetTa
 #pragma unroll
                     for (int r = 0; r < 2; ++r) acc[r][n] = mfma16u<BF16>(ahc[r], bhc[n], acc[r][n]);
                 if (half == 1) {
-                    if (ks + 1 < nks) store_a(buf ^ 1, ra);
+                    if (st + 1 < nst) store_a(buf ^ 1, ra);
+#pragma unroll
+                    for (int u = 0; u < 2; ++u) ra[u] = ra2[u];
                     __syncthreads();
                 }
             }
         }
-        // scores, one 32-row block at a time through the wave's patch; the
-        // operands of the first kPf per block are all fetched up front
-        float eq[2][kPf], usq[kPf];
-        double ldq[kPf];
-#pragma unroll
-        for (int jj = 0; jj < kPf; ++jj) {
-            const int k = kq[jj];
-            const int i = lane + 64 * jj;
-            const int row = i / npt, qq = i - row * npt;
-#pragma unroll
-            for (int r = 0; r < 2; ++r)
-                eq[r][jj] = k >= 0 ? Sb[(size_t)(col0 + 32 * r + row) * SB + stile * npt + qq] : 0.f;
-            ldq[jj] = k >= 0 ? P.ld[k] : 0.0;
-            usq[jj] = MODE == 2 && k >= 0 ? P.g16_unscale[k] : 1.f;
-        }
+        // scores, one 32-row block at a time through the wave's patch
 #pragma unroll
         for (int r = 0; r < 2; ++r) {
             __syncthreads();  // the patch is free
@@ -425,42 +406,7 @@ __global__ void __launch_bounds__(64 * kSetWaves) k_set_project(PlanDev P, SetTa
                     Cw[row * C_LD + n * 32 + (lane & 31)] = acc[r][n][q];
                 }
             __syncthreads();
-            // one score at a time (not unrolled: the log is long); the operand
-            // arrays rotate so that their indexing stays static
-            int kc[kPf];
-            float ec[kPf], uc[kPf];
-            double lc[kPf];
-#pragma unroll
-            for (int jj = 0; jj < kPf; ++jj) {
-                kc[jj] = kq[jj];
-                ec[jj] = eq[r][jj];
-                uc[jj] = usq[jj];
-                lc[jj] = ldq[jj];
-            }
-#pragma unroll 1
-            for (int jj = 0; jj < kPf; ++jj) {
-                if (kc[0] >= 0) {  // else past the tile or an empty slot
-                    const int i = lane + 64 * jj;
-                    const int row = i / npt, qq = i - row * npt;
-                    const float* cr = Cw + row * C_LD + qq * p;
-                    double pp = 0.0;
-                    for (int j = 0; j < p; ++j) {
-                        const double cv = (double)cr[j];
-                        pp = fma(cv, cv, pp);
-                    }
-                    if (MODE == 2) pp *= (double)uc[0];
-                    const double rss = (double)ec[0] - pp;
-                    Sb[(size_t)(col0 + 32 * r + row) * SB + stile * npt + qq] = (float)fma(tp, log(rss + 1e-10), lc[0]);
-                }
-#pragma unroll
-                for (int q = 0; q + 1 < kPf; ++q) {
-                    kc[q] = kc[q + 1];
-                    ec[q] = ec[q + 1];
-                    uc[q] = uc[q + 1];
-                    lc[q] = lc[q + 1];
-                }
-            }
-            for (int i = lane + 64 * kPf; i < 32 * npt; i += 64) {
+            for (int i = lane; i < 32 * npt; i += 64) {
                 const int row = i / npt, qq = i % npt;
                 const int k = kp[qq];
                 if (k < 0) continue;
