@@ -738,7 +738,6 @@ class ARMA11Results:
 
 _UNSUPPORTED = {
     "dsort": "voxel-wise -dsort regressors",
-    "column_sets": "-nuisance_summary column sets",
 }
 
 
@@ -866,6 +865,29 @@ def fit_glm_arma11(data, design, tr, a_grid=None, b_grid=None, estimate_per_voxe
             res.contrast_r2_partial = torch.zeros(V, n_c, dtype=out_dtype)
         if want_r2_semipartial:
             res.contrast_r2_semipartial = torch.zeros(V, n_c, dtype=out_dtype)
+    column_sets = kwargs.get("column_sets")
+    set_mats = None
+    if column_sets:
+        # glm/arma.py:3543-3562: per set, (A_SS + 1e-8 I)^-1 of every pair (A =
+        # (X'R^-1X)^-1) and the covariance of its centred design columns; None
+        # for a set naming a column the design does not have (a -dsort column)
+        Ai = grid.ainv()
+        Xd = torch.as_tensor(design, dtype=torch.float64)
+        Xc = Xd - Xd.mean(0, keepdim=True)
+        set_mats = []
+        for cols in column_sets:
+            keep = [int(c) for c in cols if int(c) < p]
+            if len(keep) != len(cols) or not keep:
+                set_mats.append(None)
+                continue
+            ix = torch.as_tensor(keep, dtype=torch.long)
+            A = Ai[:, ix][:, :, ix]
+            Minv = torch.linalg.inv(A + 1e-8 * torch.eye(len(keep), dtype=torch.float64))
+            Cs = Xc[:, ix].T @ Xc[:, ix] / Xd.shape[0]
+            set_mats.append((ix.to(dev), Minv.to(dev), Cs.to(dev)))
+        res.set_fstats = torch.full((V, len(column_sets)), float("nan"), dtype=out_dtype)
+        res.set_sd = torch.full_like(res.set_fstats, float("nan"))
+        res.set_coef = torch.full_like(res.set_fstats, float("nan"))
     params_dev = grid.params.to(dev)
     with torch.cuda.device(dev):
         for b0 in range(0, V, nb):
@@ -921,6 +943,23 @@ def fit_glm_arma11(data, design, tr, a_grid=None, b_grid=None, estimate_per_voxe
                     if want_r2_semipartial:
                         rem = (1.0 - g["r2"].double()).clamp(min=0.0)[:, None]
                         res.contrast_r2_semipartial[b0:b1] = (cp * rem).cpu().to(out_dtype)
+            if set_mats is not None:
+                # glm/arma.py:3564-3585 in the normalised units of the fit; SD and
+                # coef unscaled like betas (:5845-5848)
+                sb = scale.double() if scale is not None else torch.ones(Vb, dtype=torch.float64, device=dev)
+                bn = g["betas"].double() / sb[:, None]
+                s2n = g["sigma2"].double() / sb**2
+                for gi, m in enumerate(set_mats):
+                    if m is None:
+                        continue
+                    ix, Minv, Cs = m
+                    bs = bn.index_select(1, ix)
+                    Fv = torch.einsum("bi,bij,bj->b", bs, Minv[best.long()], bs) / (ix.numel() * s2n + 1e-30)
+                    sdv = torch.einsum("bi,ij,bj->b", bs, Cs, bs).clamp(min=0).sqrt() * sb
+                    res.set_fstats[b0:b1, gi] = Fv.cpu().to(out_dtype)
+                    res.set_sd[b0:b1, gi] = sdv.cpu().to(out_dtype)
+                    if ix.numel() == 1:
+                        res.set_coef[b0:b1, gi] = (bs[:, 0] * sb).cpu().to(out_dtype)
             if want_r2_partial or want_r2_semipartial:
                 part = _partial_r2(g["tstats"].to(out_dtype), g["r2"], res.dof, p, task_indices, want_r2_partial,
                                    r2_partial_mode, want_r2_semipartial, r2_semipartial_mode)

@@ -544,6 +544,43 @@ def glt_contrasts(X, grid: Grid, pair_index, betas, sigma2, C, dof, r2=None, dty
     return cb, t, part, semi
 
 
+def column_set_stats(X, grid: Grid, pair_index, betas, sigma2, column_sets, dtype=torch.float64):
+    """glm/arma.py:3543-3585 (-nuisance_summary column sets) per voxel, in the
+    units of betas / sigma2: for a set S of q columns, F = b_S' (A_SS + 1e-8 I)^-1
+    b_S / (q sigma2 + 1e-30) with A = (X_w'X_w)^-1 of the voxel's pair (X_w =
+    L^-1 X), SD = sqrt(max(b_S' C_S b_S, 0)) with C_S the covariance over time
+    of the centred unwhitened columns (divisor T), coef = b for a one-column set.
+    NaN for a set naming a column past the design (and coef for wider sets)."""
+    X = torch.as_tensor(X).to(dtype)
+    T, p = X.shape
+    V, G = betas.shape[0], len(column_sets)
+    F = torch.full((V, G), float("nan"), dtype=dtype)
+    sd = torch.full_like(F, float("nan"))
+    coef = torch.full_like(F, float("nan"))
+    Xc = X - X.mean(0, keepdim=True)
+    b_all = betas.to(dtype)
+    s2 = sigma2.to(dtype)
+    inv_by_pair = {}
+    for k in torch.unique(pair_index).tolist():
+        Xw = grid.Linv[k].to(dtype) @ X
+        inv_by_pair[k] = torch.linalg.inv(Xw.T @ Xw)
+    for g, cols in enumerate(column_sets):
+        if not cols or any(c >= p for c in cols):
+            continue
+        ix = torch.as_tensor(cols, dtype=torch.long)
+        q = len(cols)
+        Cs = Xc[:, ix].T @ Xc[:, ix] / T
+        b = b_all[:, ix]
+        for k, Ainv in inv_by_pair.items():
+            m = (pair_index == k).nonzero().flatten()
+            Minv = torch.linalg.inv(Ainv[ix][:, ix] + 1e-8 * torch.eye(q, dtype=dtype))
+            F[m, g] = torch.einsum("bi,ij,bj->b", b[m], Minv, b[m]) / (q * s2[m] + 1e-30)
+        sd[:, g] = torch.einsum("bi,ij,bj->b", b, Cs, b).clamp(min=0).sqrt()
+        if q == 1:
+            coef[:, g] = b[:, 0]
+    return F, sd, coef
+
+
 def fit_arma11(data, design, a_grid=None, b_grid=None, run_starts=None, tau=None,
                dtype=torch.float64, selector="hierarchical", task_indices=None,
                want_ljung_box=False, grid: Grid | None = None) -> OracleFit:

@@ -186,3 +186,35 @@ def test_glt_contrasts_match_oracle(dev):
     with pytest.raises(NotImplementedError):
         fit_glm_arma11(Y, X, 2.0, a_grid=ag, b_grid=bg, device=dev, glt_labels=["f"],
                        glt_matrices=[np.eye(5)[:2]])
+
+
+def test_column_sets_match_oracle(dev):
+    """-nuisance_summary column sets (glm/arma.py:3543-3585, 4190-4196,
+    5522-5527, 5845-5848) against the FP64 oracle at the engine's pairs: Wald F
+    rel <= 1e-5, fitted-signal SD and one-column coefficients rel <= 1e-5 (of
+    max(|x|, 1e-3 sd)); NaN where the reference leaves NaN."""
+    from fastfuncstuff_amd.glm.arma import fit_glm_arma11
+
+    print("[fit] column sets", flush=True)
+    rng = np.random.default_rng(13)
+    T, V = 100, 200
+    X = _design(rng, T)
+    Y = _data(rng, T, X, V)
+    ag = bg = torch.linspace(-0.8, 0.8, 17)
+    sets = [[0], [1, 2], [2, 3, 4], [0, 9]]
+    res = fit_glm_arma11(Y, X, 2.0, a_grid=ag, b_grid=bg, device=dev, column_sets=sets, task_indices=[0, 1])
+    assert res.set_fstats.shape == (V, 4) and res.betas.shape == (V, 2)
+    idx = res.pair_index.long()
+    s = torch.from_numpy(Y).double().std(dim=1).clamp(min=1e-8)
+    g64 = O.precompute_grid(torch.from_numpy(X).double(), ag, bg, None, None, F64)
+    Yn = torch.from_numpy(Y).double() / s[:, None]
+    b64, _, _, s2, _, _, _ = O.gls_refit(torch.from_numpy(X).double(), Yn, g64, idx, dtype=F64)
+    F, sd, coef = O.column_set_stats(torch.from_numpy(X).double(), g64, idx, b64, s2, sets)
+    sd, coef = sd * s[:, None], coef * s[:, None]
+    ok = torch.tensor([True, True, True, False])
+    assert float(((res.set_fstats.double()[:, ok] - F[:, ok]).abs() / F[:, ok].abs().clamp(min=1)).max()) <= 1e-5
+    den = torch.maximum(sd[:, ok].abs(), 1e-3 * s[:, None])
+    assert float(((res.set_sd.double()[:, ok] - sd[:, ok]).abs() / den).max()) <= 1e-5
+    den0 = torch.maximum(coef[:, 0].abs(), 1e-3 * s)
+    assert float(((res.set_coef.double()[:, 0] - coef[:, 0]).abs() / den0).max()) <= 1e-5
+    assert bool(torch.isnan(res.set_fstats[:, 3]).all()) and bool(torch.isnan(res.set_coef[:, 1:]).all())

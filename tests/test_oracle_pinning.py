@@ -374,3 +374,37 @@ def test_glt_contrast_equals_reparameterised_coefficient():
     torch.testing.assert_close(bz[:, 0], cb[:, 0], rtol=1e-9, atol=1e-10)
     torch.testing.assert_close(tz[:, 0], ct[:, 0], rtol=1e-8, atol=1e-10)
     torch.testing.assert_close(part, ct**2 / (ct**2 + T - p))
+
+
+def test_column_set_f_is_nested_model_f_test():
+    """A column set's Wald F equals the extra-sum-of-squares F of the nested
+    whitened fits (drop the set's columns) at the voxel's pair; a one-column
+    set gives F = t^2 and SD = |b| times the column's population SD; a set
+    naming a column past the design is NaN."""
+    torch.manual_seed(5)
+    T, p, V = 60, 4, 5
+    X = torch.randn(T, p, dtype=torch.float64)
+    X[:, 3] = 1.0
+    Y = torch.randn(V, T, dtype=torch.float64) + X[:, 0] * 0.7 - X[:, 1] * 0.3
+    ag = bg = torch.tensor([0.0, 0.5], dtype=torch.float32)
+    g = O.precompute_grid(X, ag, bg)
+    k = torch.tensor([0, 1, 2, 1, 0]) % g.params.shape[0]
+    b, t, _, s2, _, _, _ = O.gls_refit(X, Y, g, k)
+    sets = [[0], [0, 1], [1, 2], [2, p + 1]]
+    F, sd, coef = O.column_set_stats(X, g, k, b, s2, sets)
+    for v in range(V):
+        L = g.Linv[int(k[v])]
+        Xw, yw = L @ X, L @ Y[v]
+        rss_f = float(((yw - Xw @ torch.linalg.lstsq(Xw, yw).solution) ** 2).sum())
+        sig = rss_f / (T - p)
+        for gi, cols in enumerate(sets[:3]):
+            keep = [c for c in range(p) if c not in cols]
+            Xr = Xw[:, keep]
+            rss_r = float(((yw - Xr @ torch.linalg.lstsq(Xr, yw).solution) ** 2).sum())
+            f_nested = (rss_r - rss_f) / len(cols) / sig
+            assert abs(float(F[v, gi]) - f_nested) <= 1e-6 * max(1.0, abs(f_nested))
+    torch.testing.assert_close(F[:, 0], t[:, 0] ** 2, rtol=1e-6, atol=1e-9)
+    torch.testing.assert_close(sd[:, 0], b[:, 0].abs() * X[:, 0].std(correction=0), rtol=1e-10, atol=1e-12)
+    torch.testing.assert_close(coef[:, 0], b[:, 0])
+    assert bool(torch.isnan(coef[:, 1]).all()) and bool(torch.isnan(F[:, 3]).all())
+    assert bool(torch.isnan(sd[:, 3]).all()) and bool(torch.isnan(coef[:, 3]).all())
