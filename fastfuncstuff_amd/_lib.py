@@ -35,6 +35,7 @@ SIGNATURES: dict[str, list] = {
     "ffs_reml_plan_set_projection": [_vp, _c_int],
     "ffs_reml_prep": [_vp, _vp, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _c_int, _vp, _vp],
     "ffs_reml_prep_features": [_vp, _vp, _i64, _i64, _vp, _c_int, _vp, _vp, _vp],
+    "ffs_reml_whiten_cols": [_vp, _vp, _i64, _c_int, _vp, _i64, _vp, _vp],
     "ffs_reml_plan_coef_stride": [_vp, _vp],
     "ffs_reml_energy": [_vp, _vp, _i64, _i64, _vp, _i64, _vp],
     "ffs_reml_project": [_vp, _vp, _i64, _i64, _vp, _i64, _vp],

@@ -21,6 +21,7 @@
 #include "reml_proj16.hip"
 #include "reml_sets.hip"
 #include "reml_dense.hip"
+#include "reml_dsort.hip"
 
 using namespace remlk;
 
@@ -1149,6 +1150,21 @@ extern "C" int ffs_reml_subset_search(const ffs_reml_plan* P, const float* Y, in
     HIP_TRY(hipGetLastError());
     hipLaunchKernelGGL(k_scatter_best, dim3((unsigned)((cap + 255) / 256)), dim3(256), 0, s, idx, (int)cap, n_live,
                        bsub, ssub, best, best_score);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+extern "C" int ffs_reml_whiten_cols(const ffs_reml_plan* P, const int32_t* pair, int64_t V, int32_t m,
+                                    const double* cols, int64_t ld, double* out, void* stream) {
+    if (!P) return fail(-3, "whiten_cols: null plan");
+    if (V <= 0 || m <= 0) return 0;
+    if (!pair || !cols || !out) return fail(-3, "whiten_cols: null argument");
+    if (ld < P->d.T) return fail(-3, "whiten_cols: ld must be >= T");
+    if (!P->d.coefR) return fail(-4, "whiten_cols: the recurrence tables are kept only for p <= 32");
+    const int64_t n = V * (int64_t)m;
+    if (n > 0x7fffffffLL * 64 || V > 0x7fffffff) return fail(-3, "whiten_cols: batch too large; split it");
+    hipLaunchKernelGGL(k_whiten_cols, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, P->d,
+                       pair, (int)V, (int)m, cols, ld, out);
     HIP_TRY(hipGetLastError());
     return 0;
 }

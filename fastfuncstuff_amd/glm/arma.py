@@ -13,6 +13,7 @@ from __future__ import annotations
 
 import ctypes
 import math
+import warnings
 from collections.abc import Mapping
 from typing import Any
 
@@ -737,7 +738,7 @@ class ARMA11Results:
 
 
 _UNSUPPORTED = {
-    "dsort": "voxel-wise -dsort regressors",
+    "dsort_run_bounds": "compact per-run -dsort columns",
 }
 
 
@@ -888,6 +889,18 @@ def fit_glm_arma11(data, design, tr, a_grid=None, b_grid=None, estimate_per_voxe
         res.set_fstats = torch.full((V, len(column_sets)), float("nan"), dtype=out_dtype)
         res.set_sd = torch.full_like(res.set_fstats, float("nan"))
         res.set_coef = torch.full_like(res.set_fstats, float("nan"))
+    dsort = kwargs.get("dsort")
+    scale_all = None
+    if dsort is not None:
+        if kwargs.get("want_dsort_nods"):
+            raise NotImplementedError("want_dsort_nods (the no-dsort snapshot) is not supported by the MI355X engine")
+        dsort = torch.as_tensor(dsort)
+        if dsort.ndim == 2:
+            dsort = dsort.unsqueeze(1)
+        if dsort.shape[0] != V or dsort.shape[2] != T:
+            raise ValueError(f"dsort must have shape ({V}, n_dsort, {T}), got {tuple(dsort.shape)}")
+        dsort = _dsort_constant_guard(dsort)
+        scale_all = torch.ones(V, dtype=torch.float64)
     params_dev = grid.params.to(dev)
     with torch.cuda.device(dev):
         for b0 in range(0, V, nb):
@@ -908,6 +921,8 @@ def fit_glm_arma11(data, design, tr, a_grid=None, b_grid=None, estimate_per_voxe
                 scale = None
                 if not use_double:
                     _, scale, _ = grid.prep(Yb, normalize=True)
+            if scale_all is not None and scale is not None and not use_double:
+                scale_all[b0:b1] = scale.double().cpu()
             g = grid.gls(Yb, None if use_double else scale, best, want_resid_w=want_ljung_box or want_residuals,
                          want_residuals=want_residuals, want_predicted=want_predicted)
             ti = task_indices
@@ -969,6 +984,13 @@ def fit_glm_arma11(data, design, tr, a_grid=None, b_grid=None, estimate_per_voxe
                     if getattr(res, name) is None:
                         setattr(res, name, torch.zeros((V,) + tuple(val.shape[1:]), dtype=val.dtype))
                     getattr(res, name)[b0:b1] = val.cpu()
+    if dsort is not None:
+        _dsort_pass(grid, data, design, dsort, res, scale_all, dev, out_dtype, task_indices=task_indices,
+                    labels=kwargs.get("dsort_labels"), C=C, column_sets=column_sets,
+                    want_r2_partial=want_r2_partial, r2_partial_mode=r2_partial_mode,
+                    want_r2_semipartial=want_r2_semipartial, r2_semipartial_mode=r2_semipartial_mode,
+                    want_residuals=want_residuals, want_predicted=want_predicted,
+                    lj=(lj_lag, lj_tau) if want_ljung_box else None, batch=nb)
     a, b = res.arma_params[:, 0], res.arma_params[:, 1]
     res.arma_lambda = (b + a) * (1 + a * b) / (1 + 2 * a * b + b**2 + 1e-10)  # glm/arma.py:4769-4777
     if fixed_best is not None:
@@ -977,6 +999,145 @@ def fit_glm_arma11(data, design, tr, a_grid=None, b_grid=None, estimate_per_voxe
         res.reml_lklhd_surface = surface_acc
         res.reml_surface_params = list(grid.param_list)
     return res
+
+
+def _dsort_constant_guard(dsort: torch.Tensor) -> torch.Tensor:
+    """glm/arma.py:3207-3245: a -dsort voxel series that is constant and nonzero
+    through time (std < 1e-8, mean |x| > 1e-8) is replaced by its dataset's mean
+    series over the other voxels (all voxels when none is left), with a warning."""
+    stds = dsort.std(dim=2)
+    means = dsort.abs().mean(dim=2)
+    bad = (stds < 1e-8) & (means > 1e-8)
+    n_bad = int(bad.sum().item())
+    if n_bad == 0:
+        return dsort
+    dsort = dsort.clone()
+    for d in range(dsort.shape[1]):
+        bad_d = bad[:, d]
+        if not bool(bad_d.any()):
+            continue
+        good_d = ~bad_d
+        mean_ts = dsort[good_d, d, :].mean(dim=0) if bool(good_d.any()) else dsort[:, d, :].mean(dim=0)
+        dsort[bad_d, d, :] = mean_ts
+    warnings.warn(f"-dsort: {n_bad} voxel timeseries were ~constant through time and were replaced by their "
+                  "dataset mean (matches 3dREMLfit).", stacklevel=3)
+    return dsort
+
+
+def _dsort_pass(grid: RemlGrid, data, design, dsort, res, scale_all, dev, out_dtype, *, task_indices, labels, C,
+                column_sets, want_r2_partial, r2_partial_mode, want_r2_semipartial, r2_semipartial_mode,
+                want_residuals, want_predicted, lj, batch) -> None:
+    """glm/arma.py:3248-3539: the final GLS redone per voxel on [X | dsort_v]
+    at the (a, b) selected without dsort.  The voxel's columns (design, its
+    dsort series, its normalised data) are whitened on the device by the
+    plan's recurrence of its pair (ffs_reml_whiten_cols), then solved per voxel
+    in fp64 (A + 1e-8 I by Cholesky).  Overwrites betas / tstats / sigma2 / r2 /
+    fstats / dof (and the GLT, partial R^2, column-set, Ljung-Box, residual and
+    prediction outputs), fills dsort_betas / dsort_tstats; everything computed
+    in the normalised units of the fit and unscaled like betas (:5827-5848)."""
+    lib = _lib.load()
+    st = _lib.stream_handle(dev)
+    f64 = torch.float64
+    V, q, T = dsort.shape
+    Xd = torch.as_tensor(design, dtype=f64).to(dev)
+    p = Xd.shape[1]
+    pe = p + q
+    dof = max(1, T - pe)
+    res.dof = dof
+    res.dsort_labels = list(labels) if labels is not None else None
+    res.dsort_betas = torch.zeros(V, q, dtype=out_dtype)
+    res.dsort_tstats = torch.zeros(V, q, dtype=out_dtype)
+    fci = list(task_indices) if task_indices else list(range(p))
+    fci_d = torch.as_tensor(fci, dtype=torch.long, device=dev)
+    eye_f = torch.eye(len(fci), dtype=f64, device=dev)
+    eye_e = torch.eye(pe, dtype=f64, device=dev)
+    m = pe + 1
+    nb = max(1, min(batch, max(1, (1 << 28) // (m * T * 8 * 3))))
+    for b0 in range(0, V, nb):
+        b1 = min(V, b0 + nb)
+        B = b1 - b0
+        s = scale_all[b0:b1].to(dev)
+        Y = torch.as_tensor(data[b0:b1]).to(dev, f64) / s[:, None]
+        D = dsort[b0:b1].to(dev, f64)
+        cols = torch.cat([Xd.T.unsqueeze(0).expand(B, p, T), D, Y.unsqueeze(1)], dim=1).contiguous()
+        W = torch.empty_like(cols)
+        pair = res.pair_index[b0:b1].to(dev, torch.int32).contiguous()
+        _lib.check(lib.ffs_reml_whiten_cols(grid.handle, _lib.ptr(pair), B, m, _lib.ptr(cols), T, _lib.ptr(W), st),
+                   "whiten_cols")
+        Xw = W[:, :pe, :].transpose(1, 2)  # (B, T, pe)
+        yw = W[:, pe, :]
+        A = Xw.transpose(1, 2) @ Xw + 1e-8 * eye_e
+        Lc, info = torch.linalg.cholesky_ex(A)
+        Ai = torch.cholesky_inverse(Lc)
+        if bool((info != 0).any()):
+            badv = info != 0
+            Ai[badv] = torch.linalg.pinv(A[badv])
+        beta = (Ai @ (Xw.transpose(1, 2) @ yw.unsqueeze(2))).squeeze(2)
+        rw = yw - (Xw @ beta.unsqueeze(2)).squeeze(2)
+        sig = rw.pow(2).sum(1) / dof
+        tst = beta / (torch.sqrt(sig[:, None] * torch.diagonal(Ai, dim1=1, dim2=2)) + 1e-10)
+        pred = beta[:, :p] @ Xd.T + torch.einsum("bq,bqt->bt", beta[:, p:], D)
+        ss_res = (Y - pred).pow(2).sum(1)
+        ss_tot = (Y - Y.mean(1, keepdim=True)).pow(2).sum(1)
+        r2 = torch.where(ss_tot <= 1e-10, torch.zeros_like(ss_tot),
+                         (1.0 - ss_res / ss_tot.clamp(min=1e-10)).clamp(max=1.0))
+        At = Ai.index_select(1, fci_d).index_select(2, fci_d)
+        bt = beta.index_select(1, fci_d)
+        quad = torch.einsum("bi,bij,bj->b", bt, torch.linalg.inv(At + 1e-8 * eye_f), bt)
+        Fv = quad / (sig * len(fci) + 1e-30)
+        sc = s[:, None]
+        bb, tb = beta[:, :p], tst[:, :p]
+        res.betas[b0:b1] = ((bb[:, task_indices] if task_indices else bb) * sc).cpu().to(out_dtype)
+        res.tstats[b0:b1] = (tb[:, task_indices] if task_indices else tb).cpu().to(out_dtype)
+        res.dsort_betas[b0:b1] = (beta[:, p:] * sc).cpu().to(out_dtype)
+        res.dsort_tstats[b0:b1] = tst[:, p:].cpu().to(out_dtype)
+        res.sigma2[b0:b1] = (sig * s**2).cpu().to(out_dtype)
+        res.fstats[b0:b1] = Fv.cpu().to(out_dtype)
+        res.r2[b0:b1] = r2.cpu().to(res.r2.dtype)
+        if column_sets:
+            # per voxel: the extended design's (X'X)^-1 block (glm/arma.py:3461-3480)
+            Xo = torch.cat([Xd.unsqueeze(0).expand(B, T, p), D.transpose(1, 2)], 2)
+            Xo = Xo - Xo.mean(1, keepdim=True)
+            for gi, cset in enumerate(column_sets):
+                ix = torch.as_tensor([int(c) for c in cset], dtype=torch.long, device=dev)
+                Ag = Ai.index_select(1, ix).index_select(2, ix)
+                bg = beta.index_select(1, ix)
+                eye_g = torch.eye(ix.numel(), dtype=f64, device=dev)
+                qg = torch.einsum("bi,bi->b", bg, torch.linalg.solve(Ag + 1e-8 * eye_g, bg.unsqueeze(2)).squeeze(2))
+                res.set_fstats[b0:b1, gi] = (qg / (ix.numel() * sig + 1e-30)).cpu().to(out_dtype)
+                sigt = torch.einsum("bti,bi->bt", Xo.index_select(2, ix), bg)
+                res.set_sd[b0:b1, gi] = (sigt.std(dim=1, correction=0) * s).cpu().to(out_dtype)
+                if ix.numel() == 1:
+                    res.set_coef[b0:b1, gi] = (bg[:, 0] * s).cpu().to(out_dtype)
+        if want_r2_partial or want_r2_semipartial:
+            part = _partial_r2(tb.to(out_dtype), r2.float(), dof, p, task_indices, want_r2_partial, r2_partial_mode,
+                               want_r2_semipartial, r2_semipartial_mode)
+            for name, val in part.items():
+                if val is None:
+                    continue
+                if getattr(res, name) is None:
+                    setattr(res, name, torch.zeros((V,) + tuple(val.shape[1:]), dtype=val.dtype))
+                getattr(res, name)[b0:b1] = val.cpu()
+        if C is not None:
+            # GLTs on the base-design block (dsort betas cannot enter a GLT)
+            Cd = C.to(dev, f64)
+            cb = bb @ Cd.T
+            cv = sig[:, None] * torch.einsum("cr,brs,cs->bc", Cd, Ai[:, :p, :p], Cd)
+            ct = cb / (cv.clamp(min=0.0).sqrt() + 1e-10)
+            res.contrast_betas[b0:b1] = (cb * sc).cpu().to(out_dtype)
+            res.contrast_tstats[b0:b1] = ct.cpu().to(out_dtype)
+            if want_r2_partial and res.contrast_r2_partial is not None:
+                res.contrast_r2_partial[b0:b1] = (ct**2 / (ct**2 + dof)).cpu().to(out_dtype)
+            if want_r2_semipartial and res.contrast_r2_semipartial is not None:
+                rem = (1.0 - r2).clamp(min=0.0)[:, None]
+                res.contrast_r2_semipartial[b0:b1] = (ct**2 / (ct**2 + dof) * rem).cpu().to(out_dtype)
+        if lj is not None and res.ljung_box is not None:
+            res.ljung_box[b0:b1] = ljung_box_device(rw.float(), lj[0], lj[1]).cpu()
+        if want_residuals:
+            res.residuals[b0:b1] = ((Y - pred) * sc).cpu().to(out_dtype)
+            res.residuals_whitened[b0:b1] = (rw * sc).cpu().to(out_dtype)
+        if want_predicted:
+            res.predicted[b0:b1] = (pred * sc).cpu().to(out_dtype)
 
 
 def _partial_r2(t_full, r2, dof, p, task_indices, want_partial, partial_mode, want_semi, semi_mode) -> dict:

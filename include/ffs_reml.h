@@ -262,6 +262,16 @@ int ffs_reml_subset_search(const ffs_reml_plan* plan, const float* Y, int64_t ld
                            const int32_t* n_live, int64_t cap, int32_t* best, float* best_score, float* scale,
                            const double* coef, void* workspace, int64_t ws_bytes, void* stream);
 
+/* Whitening of per-voxel columns for the -dsort pass (glm/arma.py:3248-3539
+ * whitens the base design and each voxel's dsort columns with
+ * arma11_whitener(a, b).cols): out[(v*m + j)*ld + t] = (L_k^-1 x)_t, t < T, for
+ * x = cols[(v*m + j)*ld + 0 .. T) and k = pair[v] (device arrays, fp64; a pair
+ * outside [0, K) gives zeros).  L_k is the plan's whitener of pair k (censored
+ * samples and run starts included), the one ffs_reml_gls applies.  Returns -4
+ * when the plan's design is wider than 32 columns. */
+int ffs_reml_whiten_cols(const ffs_reml_plan* plan, const int32_t* pair, int64_t V, int32_t m, const double* cols,
+                         int64_t ld, double* out, void* stream);
+
 /* idx[0..*count) = voxels with flags[v] != 0 (order unspecified); count is device memory. */
 int ffs_reml_compact(const int32_t* flags, int64_t V, int32_t* idx, int32_t* count, void* stream);
 

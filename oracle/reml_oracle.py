@@ -581,6 +581,53 @@ def column_set_stats(X, grid: Grid, pair_index, betas, sigma2, column_sets, dtyp
     return F, sd, coef
 
 
+def dsort_gls(X, D, Y, grid: Grid, pair_index, task_indices=None, dtype=torch.float64):
+    """glm/arma.py:3248-3460 (-dsort pass) restated per voxel: extended design
+    [X | D_v'] (D (V, q, T)), whitened by the voxel's pair (dense L^-1);
+    A = X_ext,w' X_ext,w + 1e-8 I, beta = A^-1 X_ext,w' y_w, A^-1 by Cholesky;
+    sigma2 = |resid_w|^2 / (T - p - q); t = beta / (sqrt(sigma2 diag A^-1) + 1e-10);
+    R^2 = CoD(y, X beta_base + D' beta_dsort); F over the task columns (all
+    base columns without a task subset) = b' (A^-1_SS + 1e-8 I)^-1 b /
+    (sigma2 p_f + 1e-30).  Returns (betas (V, p+q), tstats, sigma2, r2, F,
+    resid_w, pred, A^-1 (V, p+q, p+q))."""
+    X = torch.as_tensor(X).to(dtype)
+    D = torch.as_tensor(D).to(dtype)
+    Y = torch.as_tensor(Y).to(dtype)
+    T, p = X.shape
+    V, q = D.shape[0], D.shape[1]
+    pe = p + q
+    dof = max(1, T - pe)
+    fci = list(task_indices) if task_indices is not None else list(range(p))
+    out_b = torch.zeros(V, pe, dtype=dtype)
+    out_t = torch.zeros_like(out_b)
+    out_s2 = torch.zeros(V, dtype=dtype)
+    out_r2 = torch.zeros(V, dtype=dtype)
+    out_F = torch.zeros(V, dtype=dtype)
+    out_rw = torch.zeros(V, T, dtype=dtype)
+    out_pred = torch.zeros(V, T, dtype=dtype)
+    out_Ai = torch.zeros(V, pe, pe, dtype=dtype)
+    for v in range(V):
+        L = grid.Linv[int(pair_index[v])].to(dtype)
+        Xe = torch.cat([X, D[v].T], dim=1)
+        Xw, yw = L @ Xe, L @ Y[v]
+        A = Xw.T @ Xw + 1e-8 * torch.eye(pe, dtype=dtype)
+        Ai = torch.cholesky_inverse(torch.linalg.cholesky(A))
+        b = Ai @ (Xw.T @ yw)
+        rw = yw - Xw @ b
+        sig = float(rw.pow(2).sum()) / dof
+        t = b / (torch.sqrt(sig * torch.diagonal(Ai)) + 1e-10)
+        pred = Xe @ b
+        ix = torch.as_tensor(fci)
+        At = Ai[ix][:, ix]
+        bt = b[ix]
+        Ati = torch.linalg.inv(At + 1e-8 * torch.eye(len(fci), dtype=dtype))
+        out_b[v], out_t[v], out_s2[v] = b, t, sig
+        out_F[v] = (bt @ Ati @ bt) / (sig * len(fci) + 1e-30)
+        out_r2[v] = cod_r2(Y[v][None], pred[None])[0]
+        out_rw[v], out_pred[v], out_Ai[v] = rw, pred, Ai
+    return out_b, out_t, out_s2, out_r2, out_F, out_rw, out_pred, out_Ai
+
+
 def fit_arma11(data, design, a_grid=None, b_grid=None, run_starts=None, tau=None,
                dtype=torch.float64, selector="hierarchical", task_indices=None,
                want_ljung_box=False, grid: Grid | None = None) -> OracleFit:

@@ -218,3 +218,44 @@ def test_column_sets_match_oracle(dev):
     den0 = torch.maximum(coef[:, 0].abs(), 1e-3 * s)
     assert float(((res.set_coef.double()[:, 0] - coef[:, 0]).abs() / den0).max()) <= 1e-5
     assert bool(torch.isnan(res.set_fstats[:, 3]).all()) and bool(torch.isnan(res.set_coef[:, 1:]).all())
+
+
+def test_dsort_pass_matches_oracle(dev):
+    """-dsort voxel-wise regressors (glm/arma.py:3248-3539): the final GLS per
+    voxel on [X | D_v] at the pair selected without them, whitened on the
+    device, against the FP64 oracle at the engine's pairs: base and dsort
+    betas rel <= 1e-5 (of max(|b|, 1e-3 sd)), t and F rel <= 1e-5, sigma2 rel
+    <= 1e-6, R^2 <= 1e-6; dof drops by the dsort count; the search (pairs) is
+    the one without dsort."""
+    from fastfuncstuff_amd.glm.arma import fit_glm_arma11
+
+    print("[fit] dsort", flush=True)
+    rng = np.random.default_rng(17)
+    T, V, q = 100, 150, 2
+    X = _design(rng, T)
+    Y = _data(rng, T, X, V)
+    D = rng.standard_normal((V, q, T)).astype(np.float32)
+    Y = Y + 0.5 * D[:, 0, :]
+    ag = bg = torch.linspace(-0.8, 0.8, 17)
+    base = fit_glm_arma11(Y, X, 2.0, a_grid=ag, b_grid=bg, device=dev)
+    res = fit_glm_arma11(Y, X, 2.0, a_grid=ag, b_grid=bg, device=dev, dsort=D, dsort_labels=["wm", "csf"],
+                         task_indices=[0, 1], column_sets=[[0], [5], [2, 6]])
+    assert torch.equal(res.pair_index, base.pair_index)
+    assert res.dof == T - 5 - q and res.dsort_labels == ["wm", "csf"]
+    idx = res.pair_index.long()
+    s = torch.from_numpy(Y).double().std(dim=1).clamp(min=1e-8)
+    g64 = O.precompute_grid(torch.from_numpy(X).double(), ag, bg, None, None, F64)
+    Yn = torch.from_numpy(Y).double() / s[:, None]
+    b, t, s2, r2, F, _, _, Ai = O.dsort_gls(torch.from_numpy(X).double(), torch.from_numpy(D).double(), Yn, g64,
+                                            idx, task_indices=[0, 1])
+    bu = b * s[:, None]
+    den = torch.maximum(bu.abs(), 1e-3 * s[:, None])
+    assert float(((res.betas.double() - bu[:, :2]).abs() / den[:, :2]).max()) <= 1e-5
+    assert float(((res.dsort_betas.double() - bu[:, 5:]).abs() / den[:, 5:]).max()) <= 1e-5
+    rel = lambda a, r: float(((a.double() - r).abs() / r.abs().clamp(min=1)).max())
+    assert rel(res.tstats, t[:, :2]) <= 1e-5 and rel(res.dsort_tstats, t[:, 5:]) <= 1e-5
+    assert rel(res.fstats, F) <= 1e-5
+    assert float(((res.sigma2.double() - s2 * s**2).abs() / (s2 * s**2)).max()) <= 1e-6
+    assert float((res.r2.double() - r2).abs().max()) <= 1e-6
+    # the one-column set of a dsort column: F = t^2 of that column
+    assert rel(res.set_fstats[:, 1], t[:, 5] ** 2) <= 1e-4

@@ -408,3 +408,51 @@ def test_column_set_f_is_nested_model_f_test():
     torch.testing.assert_close(coef[:, 0], b[:, 0])
     assert bool(torch.isnan(coef[:, 1]).all()) and bool(torch.isnan(F[:, 3]).all())
     assert bool(torch.isnan(sd[:, 3]).all()) and bool(torch.isnan(coef[:, 3]).all())
+
+
+def test_dsort_gls_with_shared_columns_is_extended_design_gls():
+    """-dsort columns that are the same for every voxel reduce the per-voxel
+    extended GLS to the ordinary GLS of the design [X | Z] (betas, t, sigma2,
+    R^2, and the F over the base columns), up to the 1e-8 ridge."""
+    torch.manual_seed(7)
+    T, p, q, V = 70, 3, 2, 4
+    X = torch.randn(T, p, dtype=torch.float64)
+    X[:, 2] = 1.0
+    Z = torch.randn(T, q, dtype=torch.float64)
+    Y = torch.randn(V, T, dtype=torch.float64) + 0.6 * X[:, 0] + 0.4 * Z[:, 1]
+    ag = bg = torch.tensor([0.0, 0.5], dtype=torch.float32)
+    XZ = torch.cat([X, Z], dim=1)
+    g = O.precompute_grid(X, ag, bg)
+    gz = O.precompute_grid(XZ, ag, bg)
+    k = torch.tensor([0, 1, 2, 1]) % g.params.shape[0]
+    D = Z.T[None].expand(V, q, T).contiguous()
+    b, t, s2, r2, F, rw, pred, Ai = O.dsort_gls(X, D, Y, g, k)
+    bz, tz, fz, s2z, r2z, _, rwz = O.gls_refit(XZ, Y, gz, k, task_indices=list(range(p)))
+    torch.testing.assert_close(b, bz, rtol=1e-7, atol=1e-9)
+    torch.testing.assert_close(t, tz, rtol=1e-6, atol=1e-8)
+    torch.testing.assert_close(s2, s2z, rtol=1e-7, atol=1e-12)
+    torch.testing.assert_close(r2, r2z, rtol=1e-7, atol=1e-10)
+    torch.testing.assert_close(F, fz, rtol=1e-6, atol=1e-8)
+    torch.testing.assert_close(rw.pow(2).sum(1), rwz.pow(2).sum(1), rtol=1e-7, atol=1e-10)
+    torch.testing.assert_close(pred, b @ XZ.T)
+
+
+def test_dsort_constant_guard():
+    """glm/arma.py:3207-3245: constant nonzero dsort series are replaced by
+    the dataset's mean series over the other voxels; all-zero series stay."""
+    import warnings
+
+    from fastfuncstuff_amd.glm.arma import _dsort_constant_guard
+
+    torch.manual_seed(2)
+    D = torch.randn(5, 2, 30, dtype=torch.float64)
+    D[1, 0] = 3.0
+    D[2, 1] = 0.0
+    with warnings.catch_warnings(record=True) as w:
+        warnings.simplefilter("always")
+        out = _dsort_constant_guard(D)
+    assert any("constant" in str(x.message) for x in w)
+    good = [0, 2, 3, 4]
+    torch.testing.assert_close(out[1, 0], D[good, 0].mean(0))
+    assert bool((out[2, 1] == 0).all())
+    torch.testing.assert_close(out[[0, 3, 4]], D[[0, 3, 4]])
