@@ -984,6 +984,17 @@ def fit_glm_arma11(data, design, tr, a_grid=None, b_grid=None, estimate_per_voxe
                     if getattr(res, name) is None:
                         setattr(res, name, torch.zeros((V,) + tuple(val.shape[1:]), dtype=val.dtype))
                     getattr(res, name)[b0:b1] = val.cpu()
+    if kwargs.get("want_ols"):
+        # glm/arma.py:4288-4323: the OLS baseline on the same data and design
+        from .core import fit_glm
+
+        res.ols_results = fit_glm(data, design, tr, device=dev, use_double=use_double, max_poly_degree=-1,
+                                  want_residuals=bool(kwargs.get("want_ols_residuals")),
+                                  want_predicted=bool(kwargs.get("want_ols_predicted")),
+                                  want_r2_partial=want_r2_partial, r2_partial_mode=r2_partial_mode,
+                                  want_r2_semipartial=want_r2_semipartial, r2_semipartial_mode=r2_semipartial_mode,
+                                  glt_labels=glt_labels, glt_matrices=glt_matrices, task_indices=task_indices)
+        res.ols_results.original_shape = original_shape
     if dsort is not None:
         _dsort_pass(grid, data, design, dsort, res, scale_all, dev, out_dtype, task_indices=task_indices,
                     labels=kwargs.get("dsort_labels"), C=C, column_sets=column_sets,
@@ -1231,6 +1242,25 @@ def fit_glm_arma11_grouped(data, designs_by_group: dict, group_indices, tr: floa
             if v is not None:
                 full[masks[h]] = v.cpu()
         setattr(merged, attr, full)
+    ols = [h for h in groups if getattr(per[h], "ols_results", None) is not None]
+    if ols:
+        # glm/arma.py:3139-3175: the per-group OLS results scattered to the voxel axis
+        from .core import GLMResults
+
+        om = GLMResults()
+        tpl_o = per[ols[0]].ols_results
+        om.dof, om.contrast_labels = tpl_o.dof, tpl_o.contrast_labels
+        for attr in ("betas", "tstats", "stderr", "sigma2", "fstats", "r2", "meanvol", "r2_run", "residuals",
+                     "predicted", "r2_partial", "r2_partial_nuisance", "r2_semipartial", "r2_semipartial_nuisance",
+                     "contrast_betas", "contrast_tstats"):
+            ref = getattr(tpl_o, attr, None)
+            if ref is None:
+                continue
+            full = torch.zeros((n_voxels,) + tuple(ref.shape[1:]), dtype=ref.dtype)
+            for h in ols:
+                full[masks[h]] = getattr(per[h].ols_results, attr)
+            setattr(om, attr, full)
+        merged.ols_results = om
     surf = next((h for h in groups if per[h].reml_lklhd_surface is not None), None)
     if surf is not None:
         merged.reml_surface_params = per[surf].reml_surface_params

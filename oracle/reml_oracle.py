@@ -628,6 +628,43 @@ def dsort_gls(X, D, Y, grid: Grid, pair_index, task_indices=None, dtype=torch.fl
     return out_b, out_t, out_s2, out_r2, out_F, out_rw, out_pred, out_Ai
 
 
+def ols_fit(X, Y, task_indices=None, C=None, dtype=torch.float64):
+    """glm/core.py:293-385, 880-1110 (one run, complete design): OLS betas by
+    the normal equations; sigma2 = max(RSS / (T - p), 0); stderr and t on the
+    task columns; F = t^2 for one task column, else b_S' ((X'X)^-1_SS)^-1 b_S /
+    (|S| sigma2 + 1e-10); R^2 = CoD; r2_run from the task-column prediction;
+    GLT c'b and its t.  Returns a dict of (V, ...) tensors."""
+    X = torch.as_tensor(X).to(dtype)
+    Y = torch.as_tensor(Y).to(dtype)
+    T, p = X.shape
+    task = list(task_indices) if task_indices is not None else list(range(p))
+    dof = max(1, T - p)
+    Ai = torch.linalg.inv(X.T @ X)
+    b = Y @ X @ Ai.T
+    resid = Y - b @ X.T
+    sig = (resid.pow(2).sum(1) / dof).clamp(min=0.0)
+    ti = torch.as_tensor(task)
+    bt = b[:, ti]
+    se = torch.sqrt(sig[:, None] * torch.diagonal(Ai)[ti][None, :])
+    t = bt / (se + 1e-10)
+    if len(task) == 1:
+        F = t[:, 0] ** 2
+    else:
+        F = torch.einsum("bi,ij,bj->b", bt, torch.linalg.inv(Ai[ti][:, ti]), bt) / (len(task) * sig + 1e-10)
+    ss_tot = (Y - Y.mean(1, keepdim=True)).pow(2).sum(1)
+    r2 = cod_r2(Y, b @ X.T)
+    pred_t = bt @ X[:, ti].T
+    r2_run = (1 - (Y - pred_t).pow(2).sum(1) / (ss_tot + 1e-10)).clamp(0, 1)
+    out = {"betas": bt, "betas_full": b, "tstats": t, "stderr": se, "sigma2": sig, "fstats": F, "r2": r2,
+           "r2_run": r2_run, "meanvol": Y.mean(1), "resid": resid}
+    if C is not None:
+        C = torch.as_tensor(C).to(dtype)
+        cb = b @ C.T
+        cv = sig[:, None] * torch.einsum("cr,rs,cs->c", C, Ai, C)[None, :]
+        out["contrast_betas"], out["contrast_tstats"] = cb, cb / (cv.clamp(min=0.0).sqrt() + 1e-10)
+    return out
+
+
 def fit_arma11(data, design, a_grid=None, b_grid=None, run_starts=None, tau=None,
                dtype=torch.float64, selector="hierarchical", task_indices=None,
                want_ljung_box=False, grid: Grid | None = None) -> OracleFit:

@@ -259,3 +259,39 @@ def test_dsort_pass_matches_oracle(dev):
     assert float((res.r2.double() - r2).abs().max()) <= 1e-6
     # the one-column set of a dsort column: F = t^2 of that column
     assert rel(res.set_fstats[:, 1], t[:, 5] ** 2) <= 1e-4
+
+
+def test_ols_baseline_matches_oracle(dev):
+    """want_ols (glm/arma.py:4288-4323 -> glm/core.py fit_glm): the OLS fit
+    that runs alongside, on the engine's GLS kernel at the white pair, vs the
+    FP64 OLS oracle: betas / stderr / t / F rel <= 1e-5, sigma2 rel <= 1e-6,
+    R^2 and r2_run <= 1e-6, GLT estimate and t rel <= 1e-5; a 2-D list
+    (multi-run) input raises as unsupported."""
+    from fastfuncstuff_amd.glm.arma import fit_glm_arma11
+    from fastfuncstuff_amd.glm.core import fit_glm
+
+    print("[fit] ols baseline", flush=True)
+    rng = np.random.default_rng(21)
+    T, V = 100, 200
+    X = _design(rng, T)
+    Y = _data(rng, T, X, V)
+    ag = bg = torch.linspace(-0.8, 0.8, 9)
+    C = [np.array([1.0, -1.0, 0, 0, 0])]
+    res = fit_glm_arma11(Y, X, 2.0, a_grid=ag, b_grid=bg, device=dev, want_ols=True, want_ols_residuals=True,
+                         task_indices=[0, 1], glt_labels=["d"], glt_matrices=C)
+    o = res.ols_results
+    ref = O.ols_fit(torch.from_numpy(X).double(), torch.from_numpy(Y).double(), task_indices=[0, 1],
+                    C=torch.tensor(np.stack(C)))
+    rel = lambda a, r: float(((a.double() - r).abs() / r.abs().clamp(min=1e-3)).max())
+    assert o.dof == T - 5 and o.betas.shape == (V, 2)
+    assert rel(o.betas, ref["betas"]) <= 1e-5 and rel(o.stderr, ref["stderr"]) <= 1e-5
+    assert rel(o.tstats, ref["tstats"]) <= 1e-5 and rel(o.fstats, ref["fstats"]) <= 1e-5
+    assert float(((o.sigma2.double() - ref["sigma2"]).abs() / ref["sigma2"]).max()) <= 1e-6
+    assert float((o.r2.double() - ref["r2"]).abs().max()) <= 1e-6
+    assert float((o.r2_run[:, 0].double() - ref["r2_run"]).abs().max()) <= 1e-6
+    assert float((o.meanvol.double() - ref["meanvol"]).abs().max()) <= 1e-4
+    assert rel(o.contrast_betas, ref["contrast_betas"]) <= 1e-5
+    assert rel(o.contrast_tstats, ref["contrast_tstats"]) <= 1e-5
+    assert float((o.residuals.double() - ref["resid"]).abs().max()) <= 1e-4
+    with pytest.raises(NotImplementedError):
+        fit_glm([Y, Y], [X, X], 2.0, device=dev)

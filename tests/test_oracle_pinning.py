@@ -456,3 +456,28 @@ def test_dsort_constant_guard():
     torch.testing.assert_close(out[1, 0], D[good, 0].mean(0))
     assert bool((out[2, 1] == 0).all())
     torch.testing.assert_close(out[[0, 3, 4]], D[[0, 3, 4]])
+
+
+def test_ols_oracle_against_lstsq_and_nested_f():
+    """OLS oracle: betas equal numpy's least squares; the task-column F equals
+    the nested-model F test (drop the task columns); one task column gives
+    F = t^2; R^2 = 1 - RSS / TSS."""
+    rng = np.random.default_rng(4)
+    T, p, V = 80, 4, 6
+    X = rng.standard_normal((T, p))
+    X[:, 3] = 1.0
+    Y = rng.standard_normal((V, T)) + 0.8 * X[:, 0] - 0.5 * X[:, 1]
+    o = O.ols_fit(torch.from_numpy(X), torch.from_numpy(Y), task_indices=[0, 1])
+    ref = np.linalg.lstsq(X, Y.T, rcond=None)[0].T
+    np.testing.assert_allclose(o["betas_full"].numpy(), ref, rtol=1e-10, atol=1e-12)
+    for v in range(V):
+        rss_f = float(((Y[v] - X @ ref[v]) ** 2).sum())
+        Xr = X[:, 2:]
+        br = np.linalg.lstsq(Xr, Y[v], rcond=None)[0]
+        rss_r = float(((Y[v] - Xr @ br) ** 2).sum())
+        f_nested = (rss_r - rss_f) / 2 / (rss_f / (T - p))
+        assert abs(float(o["fstats"][v]) - f_nested) <= 1e-8 * max(1.0, f_nested)
+        tss = float(((Y[v] - Y[v].mean()) ** 2).sum())
+        assert abs(float(o["r2"][v]) - (1 - rss_f / tss)) <= 1e-12
+    o1 = O.ols_fit(torch.from_numpy(X), torch.from_numpy(Y), task_indices=[1])
+    torch.testing.assert_close(o1["fstats"], o1["tstats"][:, 0] ** 2)
