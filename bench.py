@@ -60,9 +60,10 @@ def parse():
     ap.add_argument("--band-bin", type=float, default=0.02, help="spacing of the lambda band centres")
     ap.add_argument("--walk", type=int, default=None,
                     help="window steps after the first window (default: 0 with bands, 1 without)")
-    ap.add_argument("--workload", default="metric", choices=["metric", "learned_eval"],
+    ap.add_argument("--workload", default="metric", choices=["metric", "learned_eval", "grouped"],
                     help="metric: the headline line; learned_eval: learned path on a 100x100x50 volume with "
-                         "spatially smooth ARMA parameters and 2%% t(3) voxels (T=600), its own JSON line")
+                         "spatially smooth ARMA parameters and 2%% t(3) voxels (T=600), its own JSON line; "
+                         "grouped: 20 uneven HRF groups with a 64-event single-trial design (p=68), full grid")
     return ap.parse_args()
 
 
@@ -157,6 +158,40 @@ def learned_eval(args):
     print(json.dumps(line), flush=True)
 
 
+def grouped(args):
+    """Batched uneven designs: V voxels in 20 HRF groups (Dirichlet(0.5) sizes),
+    single-trial design of 64 events + Legendre 0-3 (p=68), T=400, 33x33 grid;
+    one step = fit_glm_arma11_grouped over all voxels (one plan per group, full
+    grid, hierarchical selection, GLS).  Data start on the host (the grouped
+    API takes host arrays), so the step includes each group's upload and plan."""
+    from fastfuncstuff_amd import _lib, synth
+    from fastfuncstuff_amd.glm.arma import fit_glm_arma11_grouped
+
+    dev = _lib.require_device(None)
+    t0 = time.time()
+    Y, designs, groups, ag, bg = synth.grouped_config(V=args.voxels, T=args.T)
+    gen_s = time.time() - t0
+    print(f"[bench] grouped workload generated in {gen_s:.1f} s", file=sys.stderr, flush=True)
+
+    def step(tm):
+        return fit_glm_arma11_grouped(Y, designs, groups, 2.0, a_grid=torch.from_numpy(ag),
+                                      b_grid=torch.from_numpy(bg), device=dev)
+
+    ms, _, _, res = timed(step, args.steps, args.warmup, dev)
+    sizes = np.bincount(groups, minlength=len(designs))
+    line = {
+        "metric": "REML voxels/s, batched uneven designs (20 HRF groups, p=68, T=400, V=200k)",
+        "value": args.voxels / (ms / 1e3), "unit": "voxels/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp32", "data": "synthetic",
+        "config": {"workload": f"grouped20_dirichlet0.5_T{args.T}_p68_grid33x33", "V": args.voxels, "T": args.T,
+                   "p": 68, "groups": len(designs), "group_sizes_min_max": [int(sizes.min()), int(sizes.max())],
+                   "path": "one plan per group, full grid, hierarchical", "projection": "fp32"},
+        "host_resident_input": True, "datagen_s": gen_s,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -165,6 +200,9 @@ def main():
         sys.exit(2)
     if args.workload == "learned_eval":
         learned_eval(args)
+        return
+    if args.workload == "grouped":
+        grouped(args)
         return
     from fastfuncstuff_amd import _lib, synth
     from fastfuncstuff_amd.glm.arma import RemlGrid

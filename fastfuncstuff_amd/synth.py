@@ -289,3 +289,41 @@ def learned_eval_config(shape=(100, 100, 50), T: int = 600, tr: float = 2.0, see
         Y[c0:c1] = (beta @ Xf_full.T + noise)[:, keep]
     return Workload(f"grf_T{T}_p12", Y, Xf[keep_np], tr, [0], keep_np.astype(np.int64), a, b, None, ag, bg)
 
+
+
+def grouped_config(V: int = 200_000, T: int = 400, tr: float = 2.0, n_groups: int = 20, n_events: int = 64,
+                   seed: int = 4, chunk: int = 100_000):
+    """Uneven per-group designs: ``n_groups`` double-gamma HRFs (peak delay
+    U(4, 7) s, undershoot at peak + 10 s with ratio U(0.1, 0.35)), one shared
+    single-trial design of ``n_events`` 1 s events at random onsets (one
+    regressor each, convolved with the group's HRF) + Legendre 0-3 (p = n_events
+    + 4); Dirichlet(0.5) group sizes; ARMA(1,1) noise as in the plain config.
+    Returns (Y (V, T) float32, designs_by_group {g: (T, p) float64},
+    group_indices (V,) int64, a_grid, b_grid)."""
+    rng = np.random.default_rng(seed)
+    onsets = np.sort(rng.uniform(0.0, (T - 15) * tr, n_events))
+    base = legendre_baseline(T, 3)
+    designs = {}
+    for g in range(n_groups):
+        peak = rng.uniform(4.0, 7.0)
+        hrf = double_gamma_hrf(tr, peak=peak, under=peak + 10.0, ratio=rng.uniform(0.1, 0.35))
+        cols = [boxcar_regressor(np.array([o]), 1.0, T, tr, hrf) for o in onsets]
+        designs[g] = np.concatenate([np.stack(cols, axis=1), base], axis=1)
+    w = rng.dirichlet(np.full(n_groups, 0.5))
+    groups = rng.choice(n_groups, size=V, p=w).astype(np.int64)
+    a = rng.uniform(0.1, 0.8, V)
+    b = rng.uniform(-0.5, 0.3, V)
+    Y = np.empty((V, T), dtype=np.float32)
+    p = n_events + 4
+    for c0 in range(0, V, chunk):
+        c1 = min(V, c0 + chunk)
+        noise = arma11_noise(rng, a[c0:c1], b[c0:c1], T)
+        beta = rng.standard_normal((c1 - c0, p))
+        sig = np.empty((c1 - c0, T))
+        for g in range(n_groups):
+            m = groups[c0:c1] == g
+            if m.any():
+                sig[m] = beta[m] @ designs[g].T
+        Y[c0:c1] = (sig + noise).astype(np.float32)
+    ag, bg = config_grid()
+    return Y, designs, groups, ag, bg
