@@ -737,11 +737,6 @@ class ARMA11Results:
         self.pair_index = None
 
 
-_UNSUPPORTED = {
-    "dsort_run_bounds": "compact per-run -dsort columns",
-}
-
-
 def fit_glm_arma11(data, design, tr, a_grid=None, b_grid=None, estimate_per_voxel=True, batch_size=None,
                    want_residuals=False, want_predicted=False, want_ljung_box=False, want_r2_partial=False,
                    r2_partial_mode="full", want_r2_semipartial=False, r2_semipartial_mode="full",
@@ -762,9 +757,6 @@ def fit_glm_arma11(data, design, tr, a_grid=None, b_grid=None, estimate_per_voxe
     "subset" (the reference's CPU default, glm/arma.py:2021-2221) or
     "exhaustive" (same as ``force_exhaustive_search``).
     """
-    for key, what in _UNSUPPORTED.items():
-        if kwargs.get(key) is not None:
-            raise NotImplementedError(f"{what} ({key}=...) is not supported by the MI355X engine")
     dev = _lib.require_device(device)
     out_dtype = torch.float64 if use_double else torch.float32
     data = torch.as_tensor(data)
@@ -892,8 +884,6 @@ def fit_glm_arma11(data, design, tr, a_grid=None, b_grid=None, estimate_per_voxe
     dsort = kwargs.get("dsort")
     scale_all = None
     if dsort is not None:
-        if kwargs.get("want_dsort_nods"):
-            raise NotImplementedError("want_dsort_nods (the no-dsort snapshot) is not supported by the MI355X engine")
         dsort = torch.as_tensor(dsort)
         if dsort.ndim == 2:
             dsort = dsort.unsqueeze(1)
@@ -995,15 +985,24 @@ def fit_glm_arma11(data, design, tr, a_grid=None, b_grid=None, estimate_per_voxe
                                   want_r2_semipartial=want_r2_semipartial, r2_semipartial_mode=r2_semipartial_mode,
                                   glt_labels=glt_labels, glt_matrices=glt_matrices, task_indices=task_indices)
         res.ols_results.original_shape = original_shape
+    a, b = res.arma_params[:, 0], res.arma_params[:, 1]
+    res.arma_lambda = (b + a) * (1 + a * b) / (1 + 2 * a * b + b**2 + 1e-10)  # glm/arma.py:4769-4777
     if dsort is not None:
+        # glm/arma.py:5780-5825: the base fit is the no-dsort fit; snapshot it
+        # when asked, then redo the final GLS per voxel with the extended design
+        nods = None
+        if kwargs.get("want_dsort_nods"):
+            nods = ARMA11Results()
+            for attr, val in vars(res).items():
+                setattr(nods, attr, val.clone() if isinstance(val, torch.Tensor) else val)
+        run_bounds = kwargs.get("dsort_run_bounds")
         _dsort_pass(grid, data, design, dsort, res, scale_all, dev, out_dtype, task_indices=task_indices,
                     labels=kwargs.get("dsort_labels"), C=C, column_sets=column_sets,
                     want_r2_partial=want_r2_partial, r2_partial_mode=r2_partial_mode,
                     want_r2_semipartial=want_r2_semipartial, r2_semipartial_mode=r2_semipartial_mode,
                     want_residuals=want_residuals, want_predicted=want_predicted,
-                    lj=(lj_lag, lj_tau) if want_ljung_box else None, batch=nb)
-    a, b = res.arma_params[:, 0], res.arma_params[:, 1]
-    res.arma_lambda = (b + a) * (1 + a * b) / (1 + 2 * a * b + b**2 + 1e-10)  # glm/arma.py:4769-4777
+                    lj=(lj_lag, lj_tau) if want_ljung_box else None, batch=nb, run_bounds=run_bounds)
+        res.nods_results = nods
     if fixed_best is not None:
         res.reml_likelihood = None
     if surface_acc is not None:
@@ -1037,7 +1036,7 @@ def _dsort_constant_guard(dsort: torch.Tensor) -> torch.Tensor:
 
 def _dsort_pass(grid: RemlGrid, data, design, dsort, res, scale_all, dev, out_dtype, *, task_indices, labels, C,
                 column_sets, want_r2_partial, r2_partial_mode, want_r2_semipartial, r2_semipartial_mode,
-                want_residuals, want_predicted, lj, batch) -> None:
+                want_residuals, want_predicted, lj, batch, run_bounds=None) -> None:
     """glm/arma.py:3248-3539: the final GLS redone per voxel on [X | dsort_v]
     at the (a, b) selected without dsort.  The voxel's columns (design, its
     dsort series, its normalised data) are whitened on the device by the
@@ -1045,17 +1044,22 @@ def _dsort_pass(grid: RemlGrid, data, design, dsort, res, scale_all, dev, out_dt
     in fp64 (A + 1e-8 I by Cholesky).  Overwrites betas / tstats / sigma2 / r2 /
     fstats / dof (and the GLT, partial R^2, column-set, Ljung-Box, residual and
     prediction outputs), fills dsort_betas / dsort_tstats; everything computed
-    in the normalised units of the fit and unscaled like betas (:5827-5848)."""
+    in the normalised units of the fit and unscaled like betas (:5827-5848).
+    ``run_bounds`` ([t0, t1, ..., T]): ``dsort`` holds one run-concatenated
+    series per set and each set becomes one zero-padded column per run
+    (run-major within the set, :3284-3303, 3389-3398), expanded per batch."""
     lib = _lib.load()
     st = _lib.stream_handle(dev)
     f64 = torch.float64
-    V, q, T = dsort.shape
+    V, n_sets, T = dsort.shape
+    n_runs = len(run_bounds) - 1 if run_bounds is not None else 1
+    q = n_sets * n_runs
     Xd = torch.as_tensor(design, dtype=f64).to(dev)
     p = Xd.shape[1]
     pe = p + q
     dof = max(1, T - pe)
     res.dof = dof
-    res.dsort_labels = list(labels) if labels is not None else None
+    res.dsort_labels = list(labels) if labels else [f"dsort#{k}" for k in range(q)]
     res.dsort_betas = torch.zeros(V, q, dtype=out_dtype)
     res.dsort_tstats = torch.zeros(V, q, dtype=out_dtype)
     fci = list(task_indices) if task_indices else list(range(p))
@@ -1069,7 +1073,15 @@ def _dsort_pass(grid: RemlGrid, data, design, dsort, res, scale_all, dev, out_dt
         B = b1 - b0
         s = scale_all[b0:b1].to(dev)
         Y = torch.as_tensor(data[b0:b1]).to(dev, f64) / s[:, None]
-        D = dsort[b0:b1].to(dev, f64)
+        if run_bounds is not None:
+            Dc = dsort[b0:b1].to(dev, f64)
+            D = torch.zeros(B, q, T, dtype=f64, device=dev)
+            for si in range(n_sets):
+                for r in range(n_runs):
+                    t0, t1 = int(run_bounds[r]), int(run_bounds[r + 1])
+                    D[:, si * n_runs + r, t0:t1] = Dc[:, si, t0:t1]
+        else:
+            D = dsort[b0:b1].to(dev, f64)
         cols = torch.cat([Xd.T.unsqueeze(0).expand(B, p, T), D, Y.unsqueeze(1)], dim=1).contiguous()
         W = torch.empty_like(cols)
         pair = res.pair_index[b0:b1].to(dev, torch.int32).contiguous()
@@ -1214,6 +1226,14 @@ def fit_glm_arma11_grouped(data, designs_by_group: dict, group_indices, tr: floa
         pre = torch.as_tensor(pre).reshape(-1, 2)
         if pre.shape[0] != n_voxels:
             raise ValueError(f"precomputed_arma_params has {pre.shape[0]} rows but data has {n_voxels} voxels")
+    # glm/arma.py:2931-2940, 3035-3036: voxel-wise regressors sliced per group
+    dsort_full = kw.pop("dsort", None)
+    if dsort_full is not None:
+        dsort_full = torch.as_tensor(dsort_full)
+        if dsort_full.ndim == 2:
+            dsort_full = dsort_full.unsqueeze(1)
+        if dsort_full.shape[0] != n_voxels:
+            raise ValueError(f"dsort has {dsort_full.shape[0]} voxels but data has {n_voxels}")
     if kw.get("a_grid") is None or kw.get("b_grid") is None:
         da, db = get_default_arma_grids()
         kw.setdefault("a_grid", da)
@@ -1226,11 +1246,14 @@ def fit_glm_arma11_grouped(data, designs_by_group: dict, group_indices, tr: floa
         sub_kw = dict(kw)
         if pre is not None:
             sub_kw["precomputed_arma_params"] = pre[m]
+        if dsort_full is not None:
+            sub_kw["dsort"] = dsort_full[m]
         per[h] = fit_glm_arma11(data_t[m], designs_by_group[h], tr, **sub_kw)
         masks[h] = m
     merged = ARMA11Results()
     tpl = per[groups[0]]
-    for name in ("dof", "tr", "fitted_column_indices", "n_regressors_full", "contrast_labels", "ljung_box_dof"):
+    for name in ("dof", "tr", "fitted_column_indices", "n_regressors_full", "contrast_labels", "ljung_box_dof",
+                 "dsort_labels"):
         setattr(merged, name, getattr(tpl, name))
     for attr in VOXEL_SCATTER_ATTRS + ["pair_index"]:
         ref = next((getattr(per[h], attr) for h in groups if getattr(per[h], attr, None) is not None), None)

@@ -295,3 +295,53 @@ def test_ols_baseline_matches_oracle(dev):
     assert float((o.residuals.double() - ref["resid"]).abs().max()) <= 1e-4
     with pytest.raises(NotImplementedError):
         fit_glm([Y, Y], [X, X], 2.0, device=dev)
+
+
+def test_dsort_nods_snapshot_and_run_bounds(dev):
+    """want_dsort_nods keeps the no-dsort fit (equal to a fit without dsort);
+    the compact per-run form (dsort_run_bounds) equals the explicitly expanded
+    zero-padded per-run columns (glm/arma.py:3284-3303, 3389-3398, 5780-5825);
+    default labels dsort#k."""
+    from fastfuncstuff_amd.glm.arma import fit_glm_arma11
+
+    print("[fit] dsort nods / run bounds", flush=True)
+    rng = np.random.default_rng(23)
+    T, V = 100, 120
+    X = _design(rng, T)
+    Y = _data(rng, T, X, V)
+    Dc = rng.standard_normal((V, 1, T)).astype(np.float32)
+    Y = Y + 0.4 * Dc[:, 0, :]
+    ag = bg = torch.linspace(-0.8, 0.8, 9)
+    base = fit_glm_arma11(Y, X, 2.0, a_grid=ag, b_grid=bg, device=dev)
+    rb = [0, 60, T]
+    res = fit_glm_arma11(Y, X, 2.0, a_grid=ag, b_grid=bg, device=dev, dsort=Dc, dsort_run_bounds=rb,
+                         want_dsort_nods=True)
+    assert torch.equal(res.nods_results.betas, base.betas) and res.nods_results.dof == T - 5
+    assert res.dsort_labels == ["dsort#0", "dsort#1"] and res.dof == T - 5 - 2
+    De = np.zeros((V, 2, T), dtype=np.float32)
+    De[:, 0, :60] = Dc[:, 0, :60]
+    De[:, 1, 60:] = Dc[:, 0, 60:]
+    ref = fit_glm_arma11(Y, X, 2.0, a_grid=ag, b_grid=bg, device=dev, dsort=De)
+    assert torch.equal(res.betas, ref.betas) and torch.equal(res.dsort_betas, ref.dsort_betas)
+    assert torch.equal(res.sigma2, ref.sigma2)
+
+
+def test_grouped_dsort_matches_per_group_fits(dev):
+    """Grouped fit with voxel-wise regressors: dsort is sliced per group and the
+    merged outputs equal the per-group fits (glm/arma.py:2931-2940, 3035-3036)."""
+    from fastfuncstuff_amd.glm.arma import fit_glm_arma11, fit_glm_arma11_grouped
+
+    print("[fit] grouped dsort", flush=True)
+    rng = np.random.default_rng(29)
+    T, V = 80, 90
+    X0, X1 = _design(rng, T), _design(rng, T, shift=2)
+    grp = rng.integers(0, 2, V)
+    Y = _data(rng, T, X0, V)
+    D = rng.standard_normal((V, T)).astype(np.float32)
+    ag = bg = torch.linspace(-0.8, 0.8, 9)
+    res = fit_glm_arma11_grouped(Y, {0: X0, 1: X1}, grp, 2.0, a_grid=ag, b_grid=bg, device=dev, dsort=D)
+    for g, Xg in ((0, X0), (1, X1)):
+        m = grp == g
+        ref = fit_glm_arma11(Y[m], Xg, 2.0, a_grid=ag, b_grid=bg, device=dev, dsort=D[m])
+        assert torch.equal(res.betas[torch.from_numpy(m)], ref.betas)
+        assert torch.equal(res.dsort_betas[torch.from_numpy(m)], ref.dsort_betas)
