@@ -185,15 +185,14 @@ __device__ __forceinline__ void load_span(const float* __restrict__ row, int T, 
     }
 }
 
-// Operands of one 16-sample half of a 32-sample block of the GLS accumulation
+// Samples of one 16-sample half of a 32-sample block of the GLS accumulation
 // (see k_gls_mfma): MFMA samples t0 + 8 kq + 4 hf + m of the four 16-voxel
-// groups with their G / X rows, and the recurrence samples t0 + 16 hf .. + 16
-// of this lane's voxel.
+// groups.  gls_half fetches the rest when it runs: the half's G / X rows
+// (L2-resident tables) and the recurrence samples t0 + 16 hf .. + 16 of the
+// lane's voxel (lines the MFMA loads of the block have just brought in).
 template <int NT>
 struct GlsHalf {
     float yb[4][4];
-    double ag[4][NT], ax[4][NT];
-    float yo[16];
 };
 
 template <int NT>
@@ -204,37 +203,32 @@ __device__ __forceinline__ void gls_load_half(int t0, int hf, int T, bool full, 
     constexpr int PG = 16 * NT;
 #pragma unroll
     for (int g = 0; g < 4; ++g) load_span<4>(rowB[g], T, t0 + 8 * kq + 4 * hf, full, h.yb[g]);
+}
+
+template <int NT>
+__device__ __forceinline__ void gls_half(int t0, int hf, int j, int kq, int T, bool full, const float* rowO,
+                                         const double* __restrict__ Gk, const double* __restrict__ Xp,
+                                         const GlsHalf<NT>& h, const double* cbuf, f64x4 (&accG)[4][NT],
+                                         f64x4 (&accX)[4][NT], double& E, double& sy, double& syy, double& e,
+                                         double& yprev) {
+    constexpr int PG = 16 * NT;
+    float yo[16];
+    load_span<16>(rowO, T, t0 + 16 * hf, full, yo);
+    // the half's G / X rows (L2-resident tables), in flight during the recurrence
+    double ag[4][NT], ax[4][NT];
 #pragma unroll
     for (int m = 0; m < 4; ++m)
 #pragma unroll
         for (int rt = 0; rt < NT; ++rt) {
             const size_t o = (size_t)(t0 + 8 * kq + 4 * hf + m) * PG + 16 * rt + j;
-            h.ag[m][rt] = Gk[o];
-            h.ax[m][rt] = Xp[o];
-        }
-    load_span<16>(rowO, T, t0 + 16 * hf, full, h.yo);
-}
-
-template <int NT>
-__device__ __forceinline__ void gls_half(int hf, const GlsHalf<NT>& h, const double* cbuf, f64x4 (&accG)[4][NT],
-                                         f64x4 (&accX)[4][NT], double& E, double& sy, double& syy, double& e,
-                                         double& yprev) {
-#pragma unroll
-    for (int m = 0; m < 4; ++m)
-#pragma unroll
-        for (int g = 0; g < 4; ++g) {
-            const double b = (double)h.yb[g][m];
-#pragma unroll
-            for (int rt = 0; rt < NT; ++rt) {
-                accG[g][rt] = __builtin_amdgcn_mfma_f64_16x16x4f64(h.ag[m][rt], b, accG[g][rt], 0, 0, 0);
-                accX[g][rt] = __builtin_amdgcn_mfma_f64_16x16x4f64(h.ax[m][rt], b, accX[g][rt], 0, 0, 0);
-            }
+            ag[m][rt] = Gk[o];
+            ax[m][rt] = Xp[o];
         }
     // Rows t >= T have zero coefficients and zero samples: they add nothing.
-#pragma unroll
+#pragma unroll 4
     for (int i = 0; i < 16; ++i) {
         const int r = 16 * hf + i;
-        const double yt = (double)h.yo[i];
+        const double yt = (double)yo[i];
         const double gs = cbuf[4 * r], c = cbuf[4 * r + 1], al = cbuf[4 * r + 2];
         e = fma(al, e, fma(-c, yprev, yt));
         yprev = yt;
@@ -243,10 +237,21 @@ __device__ __forceinline__ void gls_half(int hf, const GlsHalf<NT>& h, const dou
         sy += yt;
         syy = fma(yt, yt, syy);
     }
+#pragma unroll
+    for (int m = 0; m < 4; ++m)
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+            const double b = (double)h.yb[g][m];
+#pragma unroll
+            for (int rt = 0; rt < NT; ++rt) {
+                accG[g][rt] = __builtin_amdgcn_mfma_f64_16x16x4f64(ag[m][rt], b, accG[g][rt], 0, 0, 0);
+                accX[g][rt] = __builtin_amdgcn_mfma_f64_16x16x4f64(ax[m][rt], b, accX[g][rt], 0, 0, 0);
+            }
+        }
 }
 
 template <int NT>
-__global__ void __launch_bounds__(64) k_gls_mfma(PlanDev P, const float* __restrict__ Y, int64_t ldY,
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) k_gls_mfma(PlanDev P, const float* __restrict__ Y, int64_t ldY,
                                                  const float* __restrict__ scale, const int32_t* __restrict__ perm,
                                                  const int32_t* __restrict__ best, float* __restrict__ betas,
                                                  float* __restrict__ tstats, float* __restrict__ fstats,
@@ -255,6 +260,7 @@ __global__ void __launch_bounds__(64) k_gls_mfma(PlanDev P, const float* __restr
     constexpr int TP = PG + 1;  // transpose pitch (doubles)
     __shared__ double cbuf[32 * 4];
     __shared__ double tbuf[64 * TP];
+    __shared__ double bes[64 * TP];
     const int lane = threadIdx.x;
     const int64_t slot0 = (int64_t)blockIdx.x * 64;
     const int v0 = __builtin_amdgcn_readfirstlane(perm[slot0]);
@@ -307,7 +313,7 @@ __global__ void __launch_bounds__(64) k_gls_mfma(PlanDev P, const float* __restr
                 gls_load_half<NT>(t0, 1, T, full, j, kq, rowB, rowO, Gk, Xp, hn);
             else if (t0 + 32 < Tpad)
                 gls_load_half<NT>(t0 + 32, 0, T, vec4 && t0 + 64 <= T, j, kq, rowB, rowO, Gk, Xp, hn);
-            gls_half<NT>(hf, h, cbuf, accG, accX, E, sy, syy, e, yprev);
+            gls_half<NT>(t0, hf, j, kq, T, full, rowO, Gk, Xp, h, cbuf, accG, accX, E, sy, syy, e, yprev);
         }
     }
     // Accumulators -> one voxel per lane through LDS: h first (beta, |h|^2),
@@ -324,22 +330,19 @@ __global__ void __launch_bounds__(64) k_gls_mfma(PlanDev P, const float* __restr
     E *= inv2;
     sy *= inv;
     syy *= inv2;
-    // beta = Lx^-T h (back substitution on the upper factor Lx'), h = G'y / s
+    // beta = Lx^-T h (back substitution on the upper factor Lx'), h = G'y / s;
+    // beta lives in the lane's LDS row (runtime loops keep the register count
+    // of the epilogue below the main loop's)
     const double* __restrict__ L = P.Lx + (size_t)k * p * p;
-    double be[PG];
+    double* be = bes + lane * TP;
     double hh = 0.0;
-#pragma unroll
-    for (int jj = PG - 1; jj >= 0; --jj) {
-        if (jj >= p) {
-            be[jj] = 0.0;
-            continue;
-        }
+#pragma unroll 1
+    for (int jj = p - 1; jj >= 0; --jj) {
         const double hj = tbuf[lane * TP + jj] * inv;
         hh = fma(hj, hj, hh);
         double acc = hj;
-#pragma unroll
-        for (int m = jj + 1; m < PG; ++m)
-            if (m < p) acc -= L[m * p + jj] * be[m];
+#pragma unroll 1
+        for (int m = jj + 1; m < p; ++m) acc -= L[m * p + jj] * be[m];
         be[jj] = acc / L[jj * p + jj];
     }
     __syncthreads();
@@ -353,14 +356,12 @@ __global__ void __launch_bounds__(64) k_gls_mfma(PlanDev P, const float* __restr
     if (v < 0) return;
     double bq = 0.0, bxb = 0.0;
     const double* __restrict__ XtX = P.XtX;
-#pragma unroll
-    for (int a = 0; a < PG; ++a) {
-        if (a >= p) continue;
+#pragma unroll 1
+    for (int a = 0; a < p; ++a) {
         bq = fma(be[a], tbuf[lane * TP + a] * inv, bq);
         double ra = 0.0;
-#pragma unroll
-        for (int b2 = 0; b2 < PG; ++b2)
-            if (b2 < p) ra = fma(XtX[a * p + b2], be[b2], ra);
+#pragma unroll 1
+        for (int b2 = 0; b2 < p; ++b2) ra = fma(XtX[a * p + b2], be[b2], ra);
         bxb = fma(be[a], ra, bxb);
     }
     const double ssw = E - hh;
@@ -369,22 +370,19 @@ __global__ void __launch_bounds__(64) k_gls_mfma(PlanDev P, const float* __restr
     const int dof = max(1, T - p);
     const double sig = ssw / dof;
     const double* __restrict__ Ai = P.Ainv + (size_t)k * p * p;
-#pragma unroll
-    for (int jj = 0; jj < PG; ++jj)
-        if (jj < p) {
-            const double se = sqrt(sig * Ai[jj * p + jj]);
-            tstats[(size_t)v * p + jj] = (float)(be[jj] / (se + 1e-10));
-            betas[(size_t)v * p + jj] = (float)(be[jj] * s);
-        }
+#pragma unroll 1
+    for (int jj = 0; jj < p; ++jj) {
+        const double se = sqrt(sig * Ai[jj * p + jj]);
+        tstats[(size_t)v * p + jj] = (float)(be[jj] / (se + 1e-10));
+        betas[(size_t)v * p + jj] = (float)(be[jj] * s);
+    }
     const double* __restrict__ F = P.Fm + (size_t)k * p * p;
     double quad = 0.0;
-#pragma unroll
-    for (int a = 0; a < PG; ++a) {
-        if (a >= p) continue;
+#pragma unroll 1
+    for (int a = 0; a < p; ++a) {
         double ra = 0.0;
-#pragma unroll
-        for (int b2 = 0; b2 < PG; ++b2)
-            if (b2 < p) ra += F[a * p + b2] * be[b2];
+#pragma unroll 1
+        for (int b2 = 0; b2 < p; ++b2) ra += F[a * p + b2] * be[b2];
         quad += be[a] * ra;
     }
     fstats[v] = (float)(quad / (sig * P.n_task + 1e-10));
