@@ -1265,6 +1265,23 @@ def fit_glm_arma11_grouped(data, designs_by_group: dict, group_indices, tr: floa
             if v is not None:
                 full[masks[h]] = v.cpu()
         setattr(merged, attr, full)
+    nods = [h for h in groups if getattr(per[h], "nods_results", None) is not None]
+    if nods:
+        # glm/arma.py:3087-3115: the per-group no-dsort snapshots scattered likewise
+        nm = ARMA11Results()
+        tpl_n = per[nods[0]].nods_results
+        for name in ("dof", "tr", "fitted_column_indices", "n_regressors_full", "contrast_labels",
+                     "ljung_box_dof"):
+            setattr(nm, name, getattr(tpl_n, name, None))
+        for attr in VOXEL_SCATTER_ATTRS + ["pair_index"]:
+            ref = getattr(tpl_n, attr, None)
+            if ref is None:
+                continue
+            full = torch.zeros((n_voxels,) + tuple(ref.shape[1:]), dtype=ref.dtype)
+            for h in nods:
+                full[masks[h]] = getattr(per[h].nods_results, attr)
+            setattr(nm, attr, full)
+        merged.nods_results = nm
     ols = [h for h in groups if getattr(per[h], "ols_results", None) is not None]
     if ols:
         # glm/arma.py:3139-3175: the per-group OLS results scattered to the voxel axis

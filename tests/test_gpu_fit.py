@@ -339,9 +339,11 @@ def test_grouped_dsort_matches_per_group_fits(dev):
     Y = _data(rng, T, X0, V)
     D = rng.standard_normal((V, T)).astype(np.float32)
     ag = bg = torch.linspace(-0.8, 0.8, 9)
-    res = fit_glm_arma11_grouped(Y, {0: X0, 1: X1}, grp, 2.0, a_grid=ag, b_grid=bg, device=dev, dsort=D)
+    res = fit_glm_arma11_grouped(Y, {0: X0, 1: X1}, grp, 2.0, a_grid=ag, b_grid=bg, device=dev, dsort=D,
+                                 want_dsort_nods=True)
     for g, Xg in ((0, X0), (1, X1)):
         m = grp == g
-        ref = fit_glm_arma11(Y[m], Xg, 2.0, a_grid=ag, b_grid=bg, device=dev, dsort=D[m])
+        ref = fit_glm_arma11(Y[m], Xg, 2.0, a_grid=ag, b_grid=bg, device=dev, dsort=D[m], want_dsort_nods=True)
         assert torch.equal(res.betas[torch.from_numpy(m)], ref.betas)
         assert torch.equal(res.dsort_betas[torch.from_numpy(m)], ref.dsort_betas)
+        assert torch.equal(res.nods_results.betas[torch.from_numpy(m)], ref.nods_results.betas)
