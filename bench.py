@@ -126,10 +126,13 @@ def learned_eval(args):
     ls = LocalSearcher(grid, mlp, V)
 
     def step(tm):
-        res = ls.run(ev.Y, timer=tm, walk=args.walk)
+        res = ls.run(ev.Y, timer=tm, walk=args.walk, defer_count=True)
         if tm is not None:
             tm.mark("gls")
         grid.gls(ev.Y, res.scale, res.best)
+        if ls.settle(ev.Y, tm) > ls.cap:
+            grid.gls(ev.Y, res.scale, res.best)
+        res.n_fallback = ls.last_n_fallback
         return res
 
     lp_ms, lp_k, _, res = timed(step, args.steps, args.warmup, dev)
@@ -274,10 +277,16 @@ def main():
     ls = LocalSearcher(grid, mlp, V)
 
     def learned_step(tm):
-        res = ls.run(Y, timer=tm, walk=args.walk)
+        # the full-grid voxel count is read after the GLS is queued (no host
+        # wait inside the step); past the device-side capacity the voxels are
+        # searched then and the GLS is redone
+        res = ls.run(Y, timer=tm, walk=args.walk, defer_count=True)
         if tm is not None:
             tm.mark("gls")
         grid.gls(Y, res.scale, res.best)
+        if ls.settle(Y, tm) > ls.cap:
+            grid.gls(Y, res.scale, res.best)
+        res.n_fallback = ls.last_n_fallback
         return res
 
     lp_ms, lp_k, lp_n, res = timed(learned_step, args.steps, args.warmup, dev)

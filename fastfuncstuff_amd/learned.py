@@ -202,7 +202,7 @@ class LocalSearcher:
         self.sws = torch.empty(int(nb.value), dtype=torch.uint8, device=dev)
 
     def run(self, Y: torch.Tensor, timer: "StageTimer | None" = None, walk: int | None = None,
-            stats: dict | None = None, band: bool = True) -> LocalResult:
+            stats: dict | None = None, band: bool = True, defer_count: bool = False) -> LocalResult:
         """Learned-proposal search of Y (V, T) float32 on the device.
 
         Voxels whose window optimum lies on a window edge take up to ``walk``
@@ -215,7 +215,11 @@ class LocalSearcher:
         edges better than further windows) and 1 without.  ``timer``
         (optional) records a device event at the start of every stage;
         ``stats`` (optional, synchronises) receives the voxel count of each
-        pass ("window": first pass, "walk": list, "band")."""
+        pass ("window": first pass, "walk": list, "band").  ``defer_count``:
+        return without reading the full-grid voxel count on the host (no
+        synchronisation; ``n_fallback`` is -1), so the caller can queue its
+        next work first and then call ``settle``, which reads the count and
+        searches any voxels past the device-side capacity."""
         grid, lib, dev = self.grid, _lib.load(), self.grid.device
         use_band = band and self.bws is not None
         if walk is None:
@@ -281,9 +285,23 @@ class LocalSearcher:
                                               _lib.ptr(self.count), self.cap, _lib.ptr(self.best),
                                               _lib.ptr(self.score), _lib.ptr(self.scale), coef, _lib.ptr(self.sws),
                                               self.sws.numel(), st), "subset_search")
+        if defer_count:
+            return LocalResult(self.best, self.score, self.scale, -1, props)
+        n_fb = self.settle(Y, timer)
+        return LocalResult(self.best, self.score, self.scale, n_fb, props)
+
+    def settle(self, Y: torch.Tensor, timer: "StageTimer | None" = None) -> int:
+        """Full-grid voxel count of the last ``run`` (reads it on the host);
+        voxels past the capacity of the device-counted search are searched now
+        (best / score updated in place).  Returns the count; a count above
+        ``cap`` means ``best`` changed after a deferred ``run``."""
+        grid, lib, dev = self.grid, _lib.load(), self.grid.device
+        st = _lib.stream_handle(dev)
         n_fb = int(self.count.item())
+        self.last_n_fallback = n_fb
         if n_fb > self.cap:
-            mark("fallback_overflow")
+            if timer is not None:
+                timer.mark("fallback_overflow")
             rows = self.idx[self.cap:n_fb]
             n_rest = n_fb - self.cap
             Yt, _, _ = grid.prep(Y, rows=rows, scale=self.scale)
@@ -297,7 +315,7 @@ class LocalSearcher:
             del S
             _lib.check(lib.ffs_reml_scatter_best(_lib.ptr(rows), n_rest, _lib.ptr(b_fb), _lib.ptr(s_fb),
                                                  _lib.ptr(self.best), _lib.ptr(self.score), st), "scatter_best")
-        return LocalResult(self.best, self.score, self.scale, n_fb, props)
+        return n_fb
 
 
 def window_load(grid: RemlGrid, props: torch.Tensor, conf_min: float) -> tuple[int, int]:
