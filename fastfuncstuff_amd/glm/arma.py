@@ -130,6 +130,54 @@ def run_bounds(n: int, run_starts=None) -> list[tuple[int, int]]:
     return [(x, y) for x, y in zip(s, e) if y > x]
 
 
+def _run_block_mask(run_starts, n: int, device=None) -> torch.Tensor:
+    """glm/arma.py:732-749: (n, n) bool, True within a run."""
+    rid = torch.zeros(n, dtype=torch.long)
+    for r, (s, e) in enumerate(run_bounds(n, run_starts)):
+        rid[s:e] = r
+    return (rid[None, :] == rid[:, None]).to(device)
+
+
+def build_arma11_covariance_batch(a_grid, b_grid, n: int, device=None, dtype: torch.dtype = torch.float32,
+                                  run_starts=None, tau=None):
+    """glm/arma.py:939-1103 (without the AFNI band): the dense correlation
+    matrices R[i, j] = corr[|tau_i - tau_j|], corr = [1, lam, lam a, lam a^2, ...],
+    zeroed across runs, of every valid pair of the grid (the search's filter and
+    order).  Returns (R (K, n, n), params (K, 2), param_list).  A utility for
+    callers that need R itself; the engine whitens by recurrence instead."""
+    a_grid, b_grid = _grid_tensor(a_grid, "a_grid"), _grid_tensor(b_grid, "b_grid")
+    pairs = valid_grid_pairs(a_grid.to(dtype), b_grid.to(dtype))
+    if pairs.shape[0] == 0:
+        return torch.empty(0, n, n, device=device), torch.empty(0, 2, device=device), []
+    a, b = pairs[:, 0], pairs[:, 1]
+    lam = _compute_arma11_lambda(a, b).to(dtype)
+    t = torch.arange(n) if tau is None else torch.as_tensor(tau, dtype=torch.long).flatten()
+    dist = torch.abs(t[None, :] - t[:, None])
+    max_lag = int(dist.max().item()) if n > 0 else 0
+    corr = torch.zeros(pairs.shape[0], max_lag + 1, dtype=dtype)
+    corr[:, 0] = 1.0
+    if max_lag >= 1:
+        corr[:, 1] = lam
+        if max_lag >= 2:
+            ex = torch.arange(1, max_lag, dtype=dtype)
+            corr[:, 2:] = lam[:, None] * a[:, None].to(dtype) ** ex[None, :]
+    R = corr[:, dist]
+    if run_starts is not None:
+        R = R * _run_block_mask(run_starts, n).to(dtype)
+    R = R.contiguous().to(device)
+    params = pairs.to(device)
+    return R, params, [(float(x), float(y)) for x, y in pairs.tolist()]
+
+
+def build_arma11_covariance(a: float, b: float, n: int, device=None, dtype: torch.dtype = torch.float32,
+                            run_starts=None, tau=None):
+    """glm/arma.py:823-936: the dense R of one (a, b), or None when it is not a
+    valid ARMA(1,1) correlation."""
+    R, _, plist = build_arma11_covariance_batch(torch.tensor([a], dtype=dtype), torch.tensor([b], dtype=dtype), n,
+                                                device, dtype, run_starts=run_starts, tau=tau)
+    return R[0] if plist else None
+
+
 def timeline_gaps(n: int, run_starts=None, tau=None) -> np.ndarray:
     """Per retained index: tau[t+1]-tau[t] inside a run, 0 at each run's last point."""
     gap = np.zeros(n, dtype=np.int32)
@@ -413,6 +461,25 @@ class RemlGrid(Mapping):
             _lib.ptr(rw), self.T, _lib.ptr(res), _lib.ptr(pred), _lib.ptr(work), _lib.stream_handle(dev)), "gls")
         out["resid_w"], out["residuals"], out["predicted"] = rw, res, pred
         return out
+
+
+def precompute_autocorr_grid(n_timepoints: int, a_grid, b_grid, device=None, *, verbose: bool = False,
+                             cholesky_on_cpu: bool = True, dtype: torch.dtype = torch.float32, run_starts=None,
+                             tau=None) -> dict:
+    """glm/arma.py:1306-1368: the design-independent half of the grid — one
+    whitener per valid (a, b) (after zero insertion) and log det R — as
+    ``{"param_list", "whiteners", "logdet_Rcorr_batch", "dtype"}``.  The
+    whiteners are recurrence handles on the device (``cholesky_on_cpu`` has
+    nothing to choose); ``precompute_reml_grid`` accepts the dict as
+    ``autocorr_cache`` (its tables are built on the device per design either way)."""
+    from .whiten import arma11_whiteners
+
+    a_grid, b_grid = ensure_zero_in_grid(_grid_tensor(a_grid, "a_grid"), _grid_tensor(b_grid, "b_grid"))
+    ws, logdet, plist = arma11_whiteners(a_grid.to(dtype), b_grid.to(dtype), n_timepoints, run_starts=run_starts,
+                                         tau=tau, device=device, dtype=dtype)
+    if verbose:
+        print(f"  {len(plist)} whiteners (recurrence) for {len(a_grid) * len(b_grid)} grid cells", flush=True)
+    return {"param_list": plist, "whiteners": ws, "logdet_Rcorr_batch": logdet, "dtype": dtype}
 
 
 def precompute_reml_grid(X, n_timepoints, a_grid, b_grid, device=None, verbose=False, cholesky_on_cpu=True,

@@ -149,3 +149,38 @@ def test_estimate_arma11_per_voxel_matches_oracle(dev):
     kg = torch.tensor([lut[(float(a), float(b))] for a, b in got.tolist()])
     sg = S.gather(1, kg[:, None])[:, 0]
     assert float(((sg - srt[:, 0]).abs() / srt[:, 0].abs().clamp(min=1.0)).max()) <= 1e-5
+
+
+def test_whiteners_match_oracle(dev):
+    """glm/whiten.py whiteners on the engine's recurrence: dense L^-1 of a
+    censored two-run timeline vs the oracle's Cholesky inverse (1e-10), rows /
+    cols / stack_cols consistent with it, log det R per pair (1e-9), the
+    autocorr grid's pairs equal the search grid's (zero inserted), and an
+    invalid pair raising as in the reference."""
+    from fastfuncstuff_amd.glm.arma import RemlGrid, precompute_autocorr_grid
+    from fastfuncstuff_amd.glm.whiten import arma11_whitener, arma11_whiteners, stack_cols
+
+    print("[api] whiteners", flush=True)
+    n = 40
+    tau = torch.tensor(list(range(15)) + list(range(17, 42)))
+    rs = [0, 15]
+    w = arma11_whitener(0.6, 0.2, n, run_starts=rs, tau=tau, device=dev, dtype=torch.float64)
+    Li, ld = O.whitener(float(torch.tensor(0.6, dtype=torch.float64)), 0.2, n, run_starts=rs, tau=tau)
+    D = w.dense().cpu()
+    assert float((D - Li).abs().max()) <= 1e-10 * float(Li.abs().max())
+    rng = np.random.default_rng(3)
+    M = torch.from_numpy(rng.standard_normal((n, 3)))
+    torch.testing.assert_close(w.cols(M.to(dev)).cpu(), Li @ M, rtol=1e-10, atol=1e-10)
+    torch.testing.assert_close(w.rows(M.T.contiguous().to(dev)).cpu(), M.T @ Li.T, rtol=1e-10, atol=1e-10)
+    g = torch.linspace(-0.8, 0.8, 5)
+    ws, lds, plist = arma11_whiteners(g, g, n, run_starts=rs, tau=tau, device=dev, dtype=torch.float64)
+    for k in (0, len(plist) - 1):
+        a, b = plist[k]
+        Lk, ldk = O.whitener(a, b, n, run_starts=rs, tau=tau)
+        assert abs(float(lds[k]) - float(ldk)) <= 1e-9 * max(1.0, abs(float(ldk)))
+    S = stack_cols(ws[:2], M.to(dev)).cpu()
+    assert S.shape == (2, n, 3)
+    cache = precompute_autocorr_grid(n, g, g, dev, run_starts=rs, tau=tau)
+    assert cache["param_list"] == RemlGrid(torch.ones(n, 1), g, g, dev, run_starts=rs, tau=tau).param_list
+    with pytest.raises(ValueError):
+        arma11_whitener(0.2, -0.6, n, device=dev)
