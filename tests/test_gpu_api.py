@@ -184,3 +184,35 @@ def test_whiteners_match_oracle(dev):
     assert cache["param_list"] == RemlGrid(torch.ones(n, 1), g, g, dev, run_starts=rs, tau=tau).param_list
     with pytest.raises(ValueError):
         arma11_whitener(0.2, -0.6, n, device=dev)
+
+
+def test_prewhiten_ljung_box_and_comparison_helpers(dev):
+    """prewhiten_with_arma11 vs the oracle's dense whitening (1e-10); the
+    Ljung-Box helper on fp32 and fp64 residuals equals the oracle's statistic;
+    the -dsort pass with want_ljung_box fills it; compare_ols_vs_arma11 runs."""
+    from fastfuncstuff_amd.glm import compare_ols_vs_arma11, compute_ljung_box_statistic, prewhiten_with_arma11
+    from fastfuncstuff_amd.glm.arma import fit_glm_arma11
+
+    print("[api] prewhiten / ljung-box / compare", flush=True)
+    rng = np.random.default_rng(12)
+    T, V = 60, 40
+    X = torch.from_numpy(np.column_stack([np.ones(T), np.linspace(-1, 1, T), rng.standard_normal(T)]))
+    Y = torch.from_numpy(rng.standard_normal((T, V)))
+    Xw, Yw, L = prewhiten_with_arma11(X, Y, 0.5, 0.1, run_starts=[0, 25])
+    Li, _ = O.whitener(0.5, 0.1, T, run_starts=[0, 25])  # (a, b) taken in fp64, as the covariance builder takes them
+    torch.testing.assert_close(Xw, Li @ X, rtol=1e-10, atol=1e-10)
+    torch.testing.assert_close(Yw, Li @ Y, rtol=1e-10, atol=1e-10)
+    R = L.double() @ L.double().T
+    torch.testing.assert_close(torch.linalg.inv(R), Li.T @ Li, rtol=1e-8, atol=1e-8)
+    res = Yw.T.contiguous()
+    ref = O.ljung_box(res, None, None).numpy()
+    for r in (res, res.float()):
+        got = compute_ljung_box_statistic(r)
+        assert np.allclose(got, ref, rtol=1e-4, atol=1e-4)
+    Yv = (Y.T.numpy() + 0.3 * X[:, 1].numpy()).astype(np.float32)
+    D = rng.standard_normal((V, T)).astype(np.float32)
+    ag = bg = torch.linspace(-0.8, 0.8, 9)
+    fit = fit_glm_arma11(Yv, X.numpy(), 2.0, a_grid=ag, b_grid=bg, device=dev, dsort=D, want_ljung_box=True)
+    assert fit.ljung_box is not None and bool(torch.isfinite(fit.ljung_box).all())
+    out = compare_ols_vs_arma11(Yv, X.numpy(), 2.0, device=dev)
+    assert set(out) >= {"ols", "arma11", "tstat_ratio", "r2_improvement", "summary"}
