@@ -263,13 +263,14 @@ static int plan_create_impl(ffs_reml_plan** out, int32_t T, int32_t p, const dou
                 for (int t = 0; t < T; ++t) acc += X[(size_t)t * p + a] * X[(size_t)t * p + b];
                 XtX[(size_t)a * p + b] = acc;
             }
-        if ((rc = P->alloc(&G64p, (size_t)K * d.Tpad * PG)) || (rc = P->alloc(&coefR, (size_t)K * d.Tpad * 4)) ||
-            (rc = upload(P, &dXp, Xp.data(), Xp.size(), s)) || (rc = upload(P, &dPp, Pp.data(), Pp.size(), s)) ||
-            (rc = upload(P, &dXtX, XtX.data(), XtX.size(), s)))
+        if ((rc = P->alloc(&G64p, (size_t)K * d.Tpad * PG)) || (rc = upload(P, &dXp, Xp.data(), Xp.size(), s)) ||
+            (rc = upload(P, &dPp, Pp.data(), Pp.size(), s)) || (rc = upload(P, &dXtX, XtX.data(), XtX.size(), s)))
             return rc;
         HIP_TRY(hipMemsetAsync(G64p, 0, sizeof(double) * K * d.Tpad * PG, s));
-        HIP_TRY(hipMemsetAsync(coefR, 0, sizeof(double) * K * d.Tpad * 4, s));
     }
+    // recurrence tables for every p: K * Tpad * 32 bytes
+    if ((rc = P->alloc(&coefR, (size_t)K * d.Tpad * 4))) return rc;
+    HIP_TRY(hipMemsetAsync(coefR, 0, sizeof(double) * K * d.Tpad * 4, s));
     hipLaunchKernelGGL(k_tables, dim3(K), dim3(64), 0, s, T, d.Tpad, p, K, dab, dwhiteflag, dgap, dX, d.n_task, dtask,
                        full_f, BN, npt, soa, coef64, Xw64, G64, Lx, Ainv, Fm, work, ld, ldR, ldX, G32, Ghf, Glf, Ghb, Glb, g16s,
                        d.PG, G64p, coefR, ridge, dstatus);
@@ -1160,7 +1161,7 @@ extern "C" int ffs_reml_whiten_cols(const ffs_reml_plan* P, const int32_t* pair,
     if (V <= 0 || m <= 0) return 0;
     if (!pair || !cols || !out) return fail(-3, "whiten_cols: null argument");
     if (ld < P->d.T) return fail(-3, "whiten_cols: ld must be >= T");
-    if (!P->d.coefR) return fail(-4, "whiten_cols: the recurrence tables are kept only for p <= 32");
+    if (!P->d.coefR) return fail(-4, "whiten_cols: plan has no recurrence tables");
     const int64_t n = V * (int64_t)m;
     if (n > 0x7fffffffLL * 64 || V > 0x7fffffff) return fail(-3, "whiten_cols: batch too large; split it");
     hipLaunchKernelGGL(k_whiten_cols, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, P->d,

@@ -206,14 +206,17 @@ __global__ void __launch_bounds__(64) k_tables(int T, int Tpad, int p, int K, co
             Glb[o] = __builtin_bit_cast(uint16_t, lb);
         }
     }
-    // f64 MFMA GLS operands: G padded to PG columns, recurrence coefficients
-    // shifted so row t holds what step t needs (zeros past T and at run starts).
+    // f64 MFMA GLS operand: G padded to PG columns (p <= 32 only).
     if (G64p) {
         double* gp = G64p + (size_t)k * Tpad * PG;
         for (int idx = tid; idx < T * p; idx += 64) {
             const int t = idx / p, j = idx % p;
             gp[(size_t)t * PG + j] = g[idx];
         }
+    }
+    // Recurrence coefficients for every p (f64 GLS, -dsort and the whiteners),
+    // shifted so row t holds what step t needs (zeros past T and at run starts).
+    {
         double* cr = coefR + (size_t)k * Tpad * 4;
         for (int t = tid; t < T; t += 64) {
             const bool start = (t == 0) || gap[t - 1] <= 0;

@@ -267,8 +267,8 @@ int ffs_reml_subset_search(const ffs_reml_plan* plan, const float* Y, int64_t ld
  * arma11_whitener(a, b).cols): out[(v*m + j)*ld + t] = (L_k^-1 x)_t, t < T, for
  * x = cols[(v*m + j)*ld + 0 .. T) and k = pair[v] (device arrays, fp64; a pair
  * outside [0, K) gives zeros).  L_k is the plan's whitener of pair k (censored
- * samples and run starts included), the one ffs_reml_gls applies.  Returns -4
- * when the plan's design is wider than 32 columns. */
+ * samples and run starts included), the one ffs_reml_gls applies; any design
+ * width. */
 int ffs_reml_whiten_cols(const ffs_reml_plan* plan, const int32_t* pair, int64_t V, int32_t m, const double* cols,
                          int64_t ld, double* out, void* stream);
 

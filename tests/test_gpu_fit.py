@@ -347,3 +347,38 @@ def test_grouped_dsort_matches_per_group_fits(dev):
         assert torch.equal(res.betas[torch.from_numpy(m)], ref.betas)
         assert torch.equal(res.dsort_betas[torch.from_numpy(m)], ref.dsort_betas)
         assert torch.equal(res.nods_results.betas[torch.from_numpy(m)], ref.nods_results.betas)
+
+
+def test_dsort_wide_design_matches_oracle(dev):
+    """-dsort with a 40-column design (wider than the f64-MFMA GLS tiles; the
+    per-lane GLS and the recurrence tables serve it): base and dsort betas rel
+    <= 1e-5 of max(|b|, 1e-3 sd), t rel <= 1e-5, sigma2 rel <= 1e-6 vs the
+    FP64 oracle at the engine's pairs."""
+    from fastfuncstuff_amd.glm.arma import fit_glm_arma11
+
+    print("[fit] dsort p=40", flush=True)
+    rng = np.random.default_rng(23)
+    T, V, q = 160, 64, 1
+    base = _design(rng, T)
+    extra = np.column_stack([np.cos(np.pi * (j + 1) * np.arange(T) / T) for j in range(35)])
+    X = np.column_stack([base, extra])
+    assert X.shape[1] == 40
+    Y = _data(rng, T, base, V)
+    D = rng.standard_normal((V, q, T)).astype(np.float32)
+    Y = Y + 0.5 * D[:, 0, :]
+    ag = bg = torch.linspace(-0.8, 0.8, 9)
+    res = fit_glm_arma11(Y, X, 2.0, a_grid=ag, b_grid=bg, device=dev, dsort=D, task_indices=[0, 1])
+    assert res.dof == T - 40 - q
+    idx = res.pair_index.long()
+    s = torch.from_numpy(Y).double().std(dim=1).clamp(min=1e-8)
+    g64 = O.precompute_grid(torch.from_numpy(X).double(), ag, bg, None, None, F64)
+    Yn = torch.from_numpy(Y).double() / s[:, None]
+    b, t, s2, _, _, _, _, _ = O.dsort_gls(torch.from_numpy(X).double(), torch.from_numpy(D).double(), Yn, g64, idx,
+                                          task_indices=[0, 1])
+    bu = b * s[:, None]
+    den = torch.maximum(bu.abs(), 1e-3 * s[:, None])
+    assert float(((res.betas.double() - bu[:, :2]).abs() / den[:, :2]).max()) <= 1e-5
+    assert float(((res.dsort_betas.double() - bu[:, 40:]).abs() / den[:, 40:]).max()) <= 1e-5
+    rel = lambda a, r: float(((a.double() - r).abs() / r.abs().clamp(min=1)).max())
+    assert rel(res.tstats, t[:, :2]) <= 1e-5 and rel(res.dsort_tstats, t[:, 40:]) <= 1e-5
+    assert float(((res.sigma2.double() - s2 * s**2).abs() / (s2 * s**2)).max()) <= 1e-6
