@@ -586,8 +586,8 @@ def _default_axes(a_grid, b_grid):
 def reml_grid_search_batched(data, design, a_grid, b_grid, device=None, verbose=False, dtype=torch.float32,
                              grid_chunk_size=1, voxel_batch_size=None, y_scale=None, run_starts=None, tau=None):
     """glm/arma.py:2577-2833: exhaustive search over the valid pairs of the given
-    grids (no zero insertion) on the raw data (optionally / y_scale, no
-    residualisation) with the ridge score (X_w'X_w + 1e-6 I in the log
+    grids (no zero insertion) on the raw data (optionally / y_scale) with the
+    ridge score (X_w'X_w + 1e-6 I in the log
     determinant and in beta); sequential strict '<' in pair order; voxels that
     never improve keep (0, 0) and +inf.  Returns CPU (best_params, best_lik).
     The chunking arguments only shape memory use in the reference and are
@@ -605,7 +605,11 @@ def reml_grid_search_batched(data, design, a_grid, b_grid, device=None, verbose=
         return best_params, best_lik
     grid = RemlGrid(X, None, None, dev, run_starts=run_starts, tau=tau, pairs=pairs, ridge=1e-6)
     Yd = _to_device_f32(Y, dev)
-    Yt, _, _ = grid.prep(Yd, normalize=False, residualize=False)
+    # The score is computed as |y_w|^2 - |projection|^2 in fp32; on raw data
+    # with a large mean that difference loses most of its digits.  Removing the
+    # OLS fit first changes the ridge residual by O(1e-6 |c|) for y -> y - X c
+    # (the reference's own float32 residual carries more error than that).
+    Yt, _, _ = grid.prep(Yd, normalize=False, residualize=True)
     S = grid.surface(Yt, V)
     best, score = grid.select(S, "exhaustive")
     score = score.cpu()
@@ -842,7 +846,19 @@ def fit_glm_arma11(data, design, tr, a_grid=None, b_grid=None, estimate_per_voxe
         raise ValueError(f"Timepoints mismatch: Data has {T}, Design has {design.shape[0]}")
     p = design.shape[1]
     if not estimate_per_voxel:
-        raise NotImplementedError("global (mean time series) ARMA estimation is not supported")
+        unsupported = [k for k, on in (("precomputed_arma_params", precomputed_arma_params is not None),
+                                       ("want_r2_semipartial", want_r2_semipartial),
+                                       ("save_profile_likelihoods", save_profile_likelihoods),
+                                       ("glt_matrices", bool(kwargs.get("glt_matrices"))),
+                                       ("dsort", kwargs.get("dsort") is not None),
+                                       ("want_ols", bool(kwargs.get("want_ols")))) if on]
+        if unsupported:
+            raise NotImplementedError(f"estimate_per_voxel=False with {', '.join(unsupported)}")
+        return _fit_global(data, design, tr, a_grid, b_grid, dev, original_shape=original_shape,
+                           task_indices=task_indices, want_residuals=want_residuals, want_predicted=want_predicted,
+                           want_ljung_box=want_ljung_box, want_r2_partial=want_r2_partial,
+                           r2_partial_mode=r2_partial_mode, use_double=use_double, run_starts=run_starts, tau=tau,
+                           column_sets=kwargs.get("column_sets"), batch_size=batch_size)
 
     res = ARMA11Results()
     res.original_shape = original_shape
@@ -1078,6 +1094,146 @@ def fit_glm_arma11(data, design, tr, a_grid=None, b_grid=None, estimate_per_voxe
     if surface_acc is not None:
         res.reml_lklhd_surface = surface_acc
         res.reml_surface_params = list(grid.param_list)
+    return res
+
+
+def _fit_global(data, design, tr, a_grid, b_grid, dev, *, original_shape, task_indices, want_residuals, want_predicted,
+                want_ljung_box, want_r2_partial, r2_partial_mode, use_double, run_starts, tau, column_sets,
+                batch_size) -> ARMA11Results:
+    """glm/arma.py:5603-5780, ``estimate_per_voxel=False``: one (a, b) for every
+    voxel, found by ``reml_grid_search`` on the mean time series of the raw
+    data, then one GLS with the shared whitener: the design and each voxel's
+    (std-normalised) series go through the pair's recurrence on the device
+    (ffs_reml_whiten_cols) and the solve uses A = X_w'X_w + 1e-6 I in fp64.
+    Per voxel: beta over all p columns, sigma2 = |r_w|^2 / dof, t = beta /
+    (sqrt(sigma2 diag A^-1) + 1e-10), F = beta_S' (sigma2 A^-1_SS + 1e-8 I)^-1
+    beta_S / |S|, R^2 = CoD against X beta, partial R^2, column sets with A^-1,
+    Ljung-Box on r_w; betas / sigma2 / series unscaled as in the per-voxel fit.
+    The reference stores the full beta row in this mode, so ``task_indices``
+    must name every design column (a shorter list fails there too)."""
+    from .whiten import arma11_whitener
+
+    f64 = torch.float64
+    out_dtype = f64 if use_double else torch.float32
+    V, T = data.shape
+    p = design.shape[1]
+    if task_indices and len(task_indices) != p:
+        raise ValueError(f"estimate_per_voxel=False stores all {p} design columns; task_indices names "
+                         f"{len(task_indices)} of them")
+    Yc = data.to(out_dtype)
+    Xc = design.to(out_dtype)
+    a_opt, b_opt, lik = reml_grid_search(Xc, Yc.mean(dim=0), a_grid, b_grid, dev, run_starts=run_starts, tau=tau)
+    res = ARMA11Results()
+    res.original_shape = original_shape
+    res.fitted_column_indices = task_indices
+    res.n_regressors_full = p
+    dof = max(1, T - p)
+    res.dof = dof
+    res.tr = tr
+    res.arma_params = torch.empty(V, 2, dtype=out_dtype)
+    res.arma_params[:, 0] = a_opt
+    res.arma_params[:, 1] = b_opt
+    res.reml_likelihood = torch.full((V,), float(lik), dtype=out_dtype)
+    res.arma_lambda = torch.full((V,), (b_opt + a_opt) * (1 + a_opt * b_opt) / (1 + 2 * a_opt * b_opt + b_opt**2 + 1e-10),
+                                 dtype=out_dtype)
+    res.pair_index = torch.zeros(V, dtype=torch.int64)
+    w = arma11_whitener(a_opt, b_opt, T, run_starts=run_starts, tau=tau, device=dev, dtype=f64)
+    Xd = Xc.to(dev, f64)
+    Xw = w.cols(Xd)
+    Ai = torch.linalg.inv(Xw.T @ Xw + 1e-6 * torch.eye(p, dtype=f64, device=dev))
+    S = list(task_indices) if task_indices else list(range(p))
+    ti = torch.as_tensor(S, dtype=torch.long, device=dev)
+    A_SS = Ai.index_select(0, ti).index_select(1, ti)
+    eye_S = torch.eye(len(S), dtype=f64, device=dev)
+    dAi = torch.diagonal(Ai)
+    res.betas = torch.zeros(V, p, dtype=out_dtype)
+    res.tstats = torch.zeros(V, p, dtype=out_dtype)
+    res.sigma2 = torch.zeros(V, dtype=out_dtype)
+    res.fstats = torch.zeros(V, dtype=out_dtype)
+    res.r2 = torch.zeros(V, dtype=torch.float32)
+    if want_residuals:
+        res.residuals = torch.zeros(V, T, dtype=out_dtype)
+        res.residuals_whitened = torch.zeros(V, T, dtype=out_dtype)
+    if want_predicted:
+        res.predicted = torch.zeros(V, T, dtype=out_dtype)
+    lj = None
+    if want_ljung_box:
+        starts = sorted({int(x) for x in (run_starts.tolist() if isinstance(run_starts, torch.Tensor)
+                                          else run_starts)} if run_starts is not None else {0})
+        min_run = (min(e - b for b, e in zip(starts, starts[1:] + [T])) if len(starts) > 1 else T)
+        lj = (ljung_box_max_lag(T, p, min_run), build_ljung_box_tau(T, run_starts, tau))
+        res.ljung_box = torch.zeros(V, dtype=torch.float32)
+        res.ljung_box_dof = max(1, _resolve_ljung_box_lag(T, lj[0]) - 2)
+    set_mats = None
+    if column_sets:
+        Xcen = Xd - Xd.mean(0, keepdim=True)
+        set_mats = []
+        for cols in column_sets:
+            keep = [int(c) for c in cols if int(c) < p]
+            if len(keep) != len(cols) or not keep:
+                set_mats.append(None)
+                continue
+            ix = torch.as_tensor(keep, dtype=torch.long, device=dev)
+            Minv = torch.linalg.inv(Ai.index_select(0, ix).index_select(1, ix)
+                                    + 1e-8 * torch.eye(len(keep), dtype=f64, device=dev))
+            set_mats.append((ix, Minv, Xcen[:, ix].T @ Xcen[:, ix] / T))
+        res.set_fstats = torch.full((V, len(column_sets)), float("nan"), dtype=out_dtype)
+        res.set_sd = torch.full_like(res.set_fstats, float("nan"))
+        res.set_coef = torch.full_like(res.set_fstats, float("nan"))
+    scale = None if use_double else Yc.std(dim=1).clamp(min=1e-8)
+    nb = max(1, min(batch_size or (1 << 18), (1 << 28) // (T * 8 * 4)))
+    for b0 in range(0, V, nb):
+        b1 = min(V, b0 + nb)
+        Y = Yc[b0:b1].to(dev, f64)
+        sc = torch.ones(b1 - b0, 1, dtype=f64, device=dev)
+        if scale is not None:
+            sc = scale[b0:b1].to(dev, f64)[:, None]
+            Y = Y / sc
+        Yw = w.rows(Y)
+        beta = (Yw @ Xw) @ Ai
+        rw = Yw - beta @ Xw.T
+        pred = beta @ Xd.T
+        sig = rw.pow(2).sum(1) / dof
+        t = beta / (torch.sqrt(sig[:, None] * dAi[None, :]) + 1e-10)
+        bS = beta.index_select(1, ti)
+        F = torch.einsum("bi,bi->b", bS, torch.linalg.solve(sig[:, None, None] * A_SS + 1e-8 * eye_S,
+                                                              bS.unsqueeze(2)).squeeze(2)) / len(S)
+        ss_tot = (Y - Y.mean(1, keepdim=True)).pow(2).sum(1)
+        r2 = torch.where(ss_tot <= 1e-10, torch.zeros_like(ss_tot),
+                         (1.0 - (Y - pred).pow(2).sum(1) / ss_tot.clamp(min=1e-10)).clamp(max=1.0))
+        res.betas[b0:b1] = (beta * sc).cpu().to(out_dtype)
+        res.tstats[b0:b1] = t.cpu().to(out_dtype)
+        res.sigma2[b0:b1] = (sig * sc[:, 0] ** 2).cpu().to(out_dtype)
+        res.fstats[b0:b1] = F.cpu().to(out_dtype)
+        res.r2[b0:b1] = r2.cpu().to(torch.float32)
+        if set_mats is not None:
+            for gi, m in enumerate(set_mats):
+                if m is None:
+                    continue
+                ix, Minv, Cs = m
+                bs = beta.index_select(1, ix)
+                res.set_fstats[b0:b1, gi] = (torch.einsum("bi,ij,bj->b", bs, Minv, bs)
+                                             / (ix.numel() * sig + 1e-30)).cpu().to(out_dtype)
+                res.set_sd[b0:b1, gi] = (torch.einsum("bi,ij,bj->b", bs, Cs, bs).clamp(min=0).sqrt()
+                                         * sc[:, 0]).cpu().to(out_dtype)
+                if ix.numel() == 1:
+                    res.set_coef[b0:b1, gi] = (bs[:, 0] * sc[:, 0]).cpu().to(out_dtype)
+        if want_r2_partial:
+            part = _partial_r2(t.to(out_dtype), r2.float(), dof, p, task_indices, True, r2_partial_mode, False,
+                               "full")
+            for name, val in part.items():
+                if val is None:
+                    continue
+                if getattr(res, name) is None:
+                    setattr(res, name, torch.zeros((V,) + tuple(val.shape[1:]), dtype=val.dtype))
+                getattr(res, name)[b0:b1] = val.cpu()
+        if lj is not None:
+            res.ljung_box[b0:b1] = ljung_box_device(rw, lj[0], lj[1]).cpu()
+        if want_residuals:
+            res.residuals[b0:b1] = ((Y - pred) * sc).cpu().to(out_dtype)
+            res.residuals_whitened[b0:b1] = (rw * sc).cpu().to(out_dtype)
+        if want_predicted:
+            res.predicted[b0:b1] = (pred * sc).cpu().to(out_dtype)
     return res
 
 

@@ -795,3 +795,34 @@ def select_in_order(surface: torch.Tensor, order):
     S = surface[:, list(order)]
     j = torch.argmin(S, dim=1)  # first minimum in the visiting order
     return torch.as_tensor(list(order))[j], S[torch.arange(S.shape[0]), j]
+
+
+def global_fit(X, Y, a, b, run_starts=None, tau=None, task_indices=None, dtype=torch.float64):
+    """glm/arma.py:5603-5780 (``estimate_per_voxel=False``) after the search:
+    one whitener L^-1 of (a, b) for every voxel, A = X_w'X_w + 1e-6 I, beta =
+    A^-1 X_w'y_w, sigma2 = |y_w - X_w beta|^2 / (T - p), t = beta / (sqrt(sigma2
+    diag A^-1) + 1e-10) over all p columns, F = beta_S' (sigma2 A^-1_SS +
+    1e-8 I)^-1 beta_S / |S| (S = task columns, all when None), R^2 = CoD of y
+    against X beta.  Y (V, T) in the units the fit sees.  Returns a dict."""
+    X = torch.as_tensor(X).to(dtype)
+    Y = torch.as_tensor(Y).to(dtype)
+    T, p = X.shape
+    Li, _ = whitener(a, b, T, run_starts, tau, dtype)
+    Xw = Li @ X
+    Yw = Y @ Li.T
+    Ai = torch.linalg.inv(Xw.T @ Xw + 1e-6 * torch.eye(p, dtype=dtype))
+    beta = Yw @ Xw @ Ai
+    rw = Yw - beta @ Xw.T
+    dof = max(1, T - p)
+    sig = rw.pow(2).sum(1) / dof
+    t = beta / (torch.sqrt(sig[:, None] * torch.diagonal(Ai)[None, :]) + 1e-10)
+    S = list(task_indices) if task_indices else list(range(p))
+    ti = torch.as_tensor(S)
+    F = torch.empty(Y.shape[0], dtype=dtype)
+    for v in range(Y.shape[0]):
+        M = sig[v] * Ai[ti][:, ti] + 1e-8 * torch.eye(len(S), dtype=dtype)
+        bs = beta[v, ti]
+        F[v] = bs @ torch.linalg.inv(M) @ bs / len(S)
+    pred = beta @ X.T
+    return {"betas": beta, "tstats": t, "sigma2": sig, "fstats": F, "r2": cod_r2(Y, pred), "resid_w": rw,
+            "resid": Y - pred, "pred": pred, "Ainv": Ai, "dof": dof}

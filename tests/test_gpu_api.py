@@ -84,6 +84,16 @@ def test_ridge_grid_searches(dev):
     _check_search(k, lik, k_ref, S_ref)
     a, b, l1 = reml_grid_search(torch.from_numpy(X), torch.from_numpy(Y[0]), ag, bg, dev)
     assert (a, b) == tuple(params[0].tolist()) and abs(l1 - float(lik[0])) <= 1e-6 * abs(l1)
+    # raw series at a large level (the search sees the data unnormalised):
+    # the score must not lose its digits to the mean
+    Yo = (Y + 500.0).astype(np.float32)
+    params, lik = reml_grid_search_batched(torch.from_numpy(Yo), torch.from_numpy(X), ag, bg, dev)
+    k_ref, _ = O.ridge_grid_search(X, Yo, pairs)
+    S_ref = torch.stack([torch.tensor([O.reml_likelihood_dense(X, torch.from_numpy(Yo[v]).double(),
+                                                               O.covariance(float(a), float(b), T, dtype=F64))
+                                       for a, b in pairs.double().tolist()]) for v in range(V)])
+    k = torch.tensor([next(i for i, pr in enumerate(pairs.tolist()) if pr == prm) for prm in params.tolist()])
+    _check_search(k, lik, k_ref, S_ref)
 
 
 def test_batch_grid_search_priority_order_and_background(dev):

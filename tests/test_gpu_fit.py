@@ -382,3 +382,50 @@ def test_dsort_wide_design_matches_oracle(dev):
     rel = lambda a, r: float(((a.double() - r).abs() / r.abs().clamp(min=1)).max())
     assert rel(res.tstats, t[:, :2]) <= 1e-5 and rel(res.dsort_tstats, t[:, 40:]) <= 1e-5
     assert float(((res.sigma2.double() - s2 * s**2).abs() / (s2 * s**2)).max()) <= 1e-6
+
+
+def test_global_estimation_matches_oracle(dev):
+    """estimate_per_voxel=False (glm/arma.py:5603-5780): the pair is the ridge
+    search of the mean series (== oracle.ridge_grid_search), every voxel gets
+    it; GLS vs oracle.global_fit on the std-normalised data: betas rel <= 1e-5
+    of max(|b|, 1e-3 sd), t / F rel <= 1e-5, sigma2 rel <= 1e-6, R^2 <= 1e-6,
+    Ljung-Box of the whitened residuals, residual series; a task list shorter
+    than the design is refused."""
+    from fastfuncstuff_amd.glm.arma import fit_glm_arma11
+
+    print("[fit] global estimation", flush=True)
+    rng = np.random.default_rng(31)
+    T, V = 120, 90
+    X = _design(rng, T)
+    Y = _data(rng, T, X, V)
+    ag = bg = torch.linspace(-0.8, 0.8, 9)
+    res = fit_glm_arma11(Y, X, 2.0, a_grid=ag, b_grid=bg, device=dev, estimate_per_voxel=False, want_residuals=True,
+                         want_predicted=True, want_ljung_box=True, want_r2_partial=True, column_sets=[[0, 1], [3]],
+                         run_starts=[0, 70])
+    Yt = torch.from_numpy(Y)
+    pairs = O.valid_pairs(ag, bg)
+    k, _ = O.ridge_grid_search(torch.from_numpy(X).float().double(), Yt.float().mean(0).double()[None], pairs,
+                               run_starts=[0, 70])
+    a, b = pairs[int(k[0])].tolist()
+    assert torch.all(res.arma_params[:, 0] == a) and torch.all(res.arma_params[:, 1] == b)
+    s = Yt.float().std(dim=1).clamp(min=1e-8).double()
+    g = O.global_fit(torch.from_numpy(X).float().double(), Yt.float().double() / s[:, None], a, b, run_starts=[0, 70])
+    bu = g["betas"] * s[:, None]
+    den = torch.maximum(bu.abs(), 1e-3 * s[:, None])
+    assert float(((res.betas.double() - bu).abs() / den).max()) <= 1e-5
+    rel = lambda x, r: float(((x.double() - r).abs() / r.abs().clamp(min=1)).max())
+    assert rel(res.tstats, g["tstats"]) <= 1e-5 and rel(res.fstats, g["fstats"]) <= 1e-5
+    s2 = g["sigma2"] * s**2
+    assert float(((res.sigma2.double() - s2).abs() / s2).max()) <= 1e-6
+    assert float((res.r2.double() - g["r2"]).abs().max()) <= 1e-6
+    assert res.dof == T - X.shape[1]
+    t2 = g["tstats"] ** 2
+    assert float((res.r2_partial.double() - t2 / (t2 + res.dof)).abs().max()) <= 1e-5
+    rw = g["resid_w"] * s[:, None]
+    assert float((res.residuals_whitened.double() - rw).abs().max()) <= 1e-4 * float(s.max())
+    assert float((res.residuals.double() - g["resid"] * s[:, None]).abs().max()) <= 1e-4 * float(s.max())
+    lb = O.ljung_box(g["resid_w"], O.ljung_box_max_lag(T, X.shape[1], 50), O.ljung_box_tau(T, [0, 70]))
+    assert np.allclose(res.ljung_box.numpy(), lb.numpy(), rtol=1e-4, atol=1e-4)
+    assert rel(res.set_fstats[:, 1], g["tstats"][:, 3] ** 2) <= 1e-4
+    with pytest.raises(ValueError):
+        fit_glm_arma11(Y, X, 2.0, a_grid=ag, b_grid=bg, device=dev, estimate_per_voxel=False, task_indices=[0, 1])

@@ -481,3 +481,30 @@ def test_ols_oracle_against_lstsq_and_nested_f():
         assert abs(float(o["r2"][v]) - (1 - rss_f / tss)) <= 1e-12
     o1 = O.ols_fit(torch.from_numpy(X), torch.from_numpy(Y), task_indices=[1])
     torch.testing.assert_close(o1["fstats"], o1["tstats"][:, 0] ** 2)
+
+
+def test_global_fit_oracle_against_ols_and_whitened_ols():
+    """oracle.global_fit (estimate_per_voxel=False): at (a, b) = (0, 0) it is
+    OLS up to the 1e-6 ridge (betas, t, sigma2, R^2 vs the lstsq-pinned OLS
+    oracle; F over all columns = OLS F); at (0.6, -0.2) on two runs its betas,
+    t and sigma2 are OLS of L^-1 X, L^-1 y with the whitener pinned against
+    closed forms, and R^2 is taken on the unwhitened series."""
+    rng = np.random.default_rng(8)
+    T, p, V = 80, 4, 6
+    X = torch.from_numpy(np.column_stack([np.ones(T), np.linspace(-1, 1, T), rng.standard_normal((T, 2))]))
+    Y = torch.from_numpy(rng.standard_normal((V, T))) + (X @ torch.tensor([1.0, 2.0, -0.5, 0.3], dtype=F64))
+    g = O.global_fit(X, Y, 0.0, 0.0)
+    o = O.ols_fit(X, Y)
+    rel = lambda a, b: float(((a - b).abs() / b.abs().clamp(min=1e-3)).max())
+    assert rel(g["betas"], o["betas_full"]) < 1e-6 and rel(g["tstats"], o["tstats"]) < 1e-6
+    assert rel(g["sigma2"], o["sigma2"]) < 1e-6 and float((g["r2"] - o["r2"]).abs().max()) < 1e-9
+    assert rel(g["fstats"], o["fstats"]) < 1e-6
+    Li, _ = O.whitener(0.6, -0.2, T, run_starts=[0, 50])
+    g = O.global_fit(X, Y, 0.6, -0.2, run_starts=[0, 50])
+    ow = O.ols_fit(Li @ X, Y @ Li.T)
+    assert rel(g["betas"], ow["betas_full"]) < 1e-6 and rel(g["tstats"], ow["tstats"]) < 1e-6
+    assert rel(g["sigma2"], ow["sigma2"]) < 1e-6
+    assert torch.allclose(g["r2"], O.cod_r2(Y, g["betas"] @ X.T), atol=1e-12)
+    # one task column: F = t^2 up to the 1e-8 regulariser
+    g1 = O.global_fit(X, Y, 0.6, -0.2, run_starts=[0, 50], task_indices=[1])
+    assert rel(g1["fstats"], g1["tstats"][:, 1] ** 2) < 1e-6
