@@ -201,11 +201,23 @@ def hierarchical_ltop(n_a: int, n_b: int) -> int:
     return max(0, min(n_a, n_b).bit_length() - 2)
 
 
+try:
+    from threadpoolctl import threadpool_limits as _threadpool_limits
+except ImportError:  # optional: only bounds the BLAS pool around small SVDs
+    _threadpool_limits = None
+
+
 def design_pinv(X: np.ndarray, tol_dtype=np.float32) -> np.ndarray:
     """glm/core.py:248-272: rank-truncated SVD pseudo-inverse in float64 with the
     design dtype's tolerance.  Returns pinv (p x T)."""
     Xd = np.asarray(X, dtype=np.float64)
-    U, S, Vh = np.linalg.svd(Xd, full_matrices=False)
+    if _threadpool_limits is not None:
+        # a T x p SVD is too small for a threaded BLAS: one thread is several
+        # times faster than a contended pool
+        with _threadpool_limits(1):
+            U, S, Vh = np.linalg.svd(Xd, full_matrices=False)
+    else:
+        U, S, Vh = np.linalg.svd(Xd, full_matrices=False)
     tol = max(Xd.shape) * np.finfo(tol_dtype).eps * (S[0] if S.size else 0.0)
     r = int((S > tol).sum())
     return (Vh[:r].T / S[:r]) @ U[:, :r].T
@@ -1467,14 +1479,19 @@ def fit_glm_arma11_grouped(data, designs_by_group: dict, group_indices, tr: floa
         kw["a_grid"] = da if kw["a_grid"] is None else kw["a_grid"]
         kw["b_grid"] = db if kw["b_grid"] is None else kw["b_grid"]
     per, masks = {}, {}
+    order = torch.argsort(group_indices, stable=True)
+    counts = torch.bincount(group_indices - group_indices.min()) if n_voxels else torch.zeros(0, dtype=torch.long)
+    g0 = int(group_indices.min()) if n_voxels else 0
+    offs = torch.cumsum(counts, 0) - counts
     for h in groups:
-        m = group_indices == h
+        # voxel ids of the group in ascending order (the order a boolean mask gives)
+        m = order[int(offs[h - g0]):int(offs[h - g0]) + int(counts[h - g0])]
         sub_kw = dict(kw)
         if pre is not None:
-            sub_kw["precomputed_arma_params"] = pre[m]
+            sub_kw["precomputed_arma_params"] = pre.index_select(0, m)
         if dsort_full is not None:
-            sub_kw["dsort"] = dsort_full[m]
-        per[h] = fit_glm_arma11(data_t[m], designs_by_group[h], tr, **sub_kw)
+            sub_kw["dsort"] = dsort_full.index_select(0, m)
+        per[h] = fit_glm_arma11(data_t.index_select(0, m.to(data_t.device)), designs_by_group[h], tr, **sub_kw)
         masks[h] = m
     merged = ARMA11Results()
     tpl = per[groups[0]]
@@ -1489,7 +1506,7 @@ def fit_glm_arma11_grouped(data, designs_by_group: dict, group_indices, tr: floa
         for h in groups:
             v = getattr(per[h], attr, None)
             if v is not None:
-                full[masks[h]] = v.cpu()
+                full.index_copy_(0, masks[h], v.cpu().to(full.dtype))
         setattr(merged, attr, full)
     nods = [h for h in groups if getattr(per[h], "nods_results", None) is not None]
     if nods:
@@ -1505,7 +1522,7 @@ def fit_glm_arma11_grouped(data, designs_by_group: dict, group_indices, tr: floa
                 continue
             full = torch.zeros((n_voxels,) + tuple(ref.shape[1:]), dtype=ref.dtype)
             for h in nods:
-                full[masks[h]] = getattr(per[h].nods_results, attr)
+                full.index_copy_(0, masks[h], getattr(per[h].nods_results, attr).cpu().to(full.dtype))
             setattr(nm, attr, full)
         merged.nods_results = nm
     ols = [h for h in groups if getattr(per[h], "ols_results", None) is not None]
@@ -1524,7 +1541,7 @@ def fit_glm_arma11_grouped(data, designs_by_group: dict, group_indices, tr: floa
                 continue
             full = torch.zeros((n_voxels,) + tuple(ref.shape[1:]), dtype=ref.dtype)
             for h in ols:
-                full[masks[h]] = getattr(per[h].ols_results, attr)
+                full.index_copy_(0, masks[h], getattr(per[h].ols_results, attr).cpu().to(full.dtype))
             setattr(om, attr, full)
         merged.ols_results = om
     surf = next((h for h in groups if per[h].reml_lklhd_surface is not None), None)
@@ -1534,7 +1551,7 @@ def fit_glm_arma11_grouped(data, designs_by_group: dict, group_indices, tr: floa
         full = torch.zeros((n_voxels, ref.shape[1]), dtype=ref.dtype)
         for h in groups:
             if per[h].reml_lklhd_surface is not None:
-                full[masks[h]] = per[h].reml_lklhd_surface
+                full.index_copy_(0, masks[h], per[h].reml_lklhd_surface.cpu().to(full.dtype))
         merged.reml_lklhd_surface = full
     return merged
 
