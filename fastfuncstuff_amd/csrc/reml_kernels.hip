@@ -27,6 +27,119 @@ typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 //   scratch: [K][3][T]  s2 / c / alpha (SoA, kept for the energy tables)
 //   work:    [K][2*p*p] Gauss-Jordan / Cholesky-inverse workspace
 // ---------------------------------------------------------------------------
+// Block-cooperative forms of cholesky_inplace / cholesky_inverse /
+// gauss_jordan_inverse (reml_math.h) for the 64-lane k_tables block.  Every
+// element is computed by one lane with the serial routine's expression and
+// summation order; only independent rows / columns are spread over lanes, with
+// a barrier between dependent steps.  `flag` is block-shared and starts at 0;
+// every lane reads it after the same barrier, so all lanes leave together.
+
+// A (p x p, row-major, symmetric) -> lower factor L (upper part zeroed).
+// Returns -1 (flag = 1) when a pivot is not positive.
+__device__ int block_cholesky(int p, double* A, int* flag) {
+    const int tid = threadIdx.x;
+    for (int j = 0; j < p; ++j) {
+        if (tid == 0) {
+            double d = A[j * p + j];
+            for (int k = 0; k < j; ++k) d -= A[j * p + k] * A[j * p + k];
+            if (!(d > 0.0))
+                *flag = 1;
+            else
+                A[j * p + j] = sqrt(d);
+        }
+        __syncthreads();
+        if (*flag) return -1;
+        const double d = A[j * p + j];
+        for (int i = j + 1 + tid; i < p; i += 64) {
+            double s = A[i * p + j];
+            for (int k = 0; k < j; ++k) s -= A[i * p + k] * A[j * p + k];
+            A[i * p + j] = s / d;
+        }
+        for (int k = j + 1 + tid; k < p; k += 64) A[j * p + k] = 0.0;
+        __syncthreads();
+    }
+    return 0;
+}
+
+// Ainv = (L L')^-1 = L^-T L^-1; tmp (p x p) receives L^-1, one column per lane.
+__device__ void block_cholesky_inverse(int p, const double* L, double* Ainv, double* tmp) {
+    const int tid = threadIdx.x;
+    for (int j = tid; j < p; j += 64) {
+        for (int i = 0; i < p; ++i) tmp[i * p + j] = 0.0;
+        tmp[j * p + j] = 1.0 / L[j * p + j];
+        for (int i = j + 1; i < p; ++i) {
+            double s = 0.0;
+            for (int k = j; k < i; ++k) s += L[i * p + k] * tmp[k * p + j];
+            tmp[i * p + j] = -s / L[i * p + i];
+        }
+    }
+    __syncthreads();
+    for (int idx = tid; idx < p * p; idx += 64) {
+        const int i = idx / p, j = idx % p;
+        if (j > i) continue;
+        double s = 0.0;
+        for (int k = i; k < p; ++k) s += tmp[k * p + i] * tmp[k * p + j];
+        Ainv[i * p + j] = s;
+        Ainv[j * p + i] = s;
+    }
+    __syncthreads();
+}
+
+// General inverse with partial pivoting; -1 (flag = fail_code) if singular.
+__device__ int block_gauss_jordan_inverse(int p, const double* A, double* inv, double* work, int* piv_sh, int* flag,
+                                          int fail_code) {
+    const int tid = threadIdx.x;
+    for (int i = tid; i < p * p; i += 64) {
+        work[i] = A[i];
+        inv[i] = (i / p == i % p) ? 1.0 : 0.0;
+    }
+    __syncthreads();
+    for (int col = 0; col < p; ++col) {
+        if (tid == 0) {
+            int piv = col;
+            double best = fabs(work[col * p + col]);
+            for (int r = col + 1; r < p; ++r)
+                if (fabs(work[r * p + col]) > best) {
+                    best = fabs(work[r * p + col]);
+                    piv = r;
+                }
+            if (!(best > 0.0)) *flag = fail_code;
+            *piv_sh = piv;
+        }
+        __syncthreads();
+        if (*flag) return -1;
+        const int piv = *piv_sh;
+        if (piv != col)
+            for (int j = tid; j < p; j += 64) {
+                double t = work[col * p + j];
+                work[col * p + j] = work[piv * p + j];
+                work[piv * p + j] = t;
+                t = inv[col * p + j];
+                inv[col * p + j] = inv[piv * p + j];
+                inv[piv * p + j] = t;
+            }
+        __syncthreads();
+        const double d = work[col * p + col];
+        __syncthreads();
+        for (int j = tid; j < p; j += 64) {
+            work[col * p + j] /= d;
+            inv[col * p + j] /= d;
+        }
+        __syncthreads();
+        for (int r = tid; r < p; r += 64) {
+            if (r == col) continue;
+            const double f = work[r * p + col];
+            if (f == 0.0) continue;
+            for (int j = 0; j < p; ++j) {
+                work[r * p + j] -= f * work[col * p + j];
+                inv[r * p + j] -= f * inv[col * p + j];
+            }
+        }
+        __syncthreads();
+    }
+    return 0;
+}
+
 __global__ void __launch_bounds__(64) k_tables(int T, int Tpad, int p, int K, const double* __restrict__ ab,
                                                const int32_t* __restrict__ white_flag,
                                                const int32_t* __restrict__ gap, const double* __restrict__ X,
@@ -55,7 +168,7 @@ __global__ void __launch_bounds__(64) k_tables(int T, int Tpad, int p, int K, co
     double* Ai = Ainv + (size_t)k * p * p;
     double* wk = work + (size_t)k * 2 * p * p;
     __shared__ double sh_ldR, sh_ldX;
-    __shared__ int sh_fail;
+    __shared__ int sh_fail, sh_piv;
 
     if (tid == 0) {
         sh_ldR = pair_coefficients(a, b, T, gap, white_flag[k], s2, cc, al);
@@ -82,56 +195,61 @@ __global__ void __launch_bounds__(64) k_tables(int T, int Tpad, int p, int K, co
     }
     __syncthreads();
     // F-statistic quadratic form, embedded in p x p (zero outside the task columns).
+    // The factorisations below run on the whole block (block_* helpers); each
+    // element gets the arithmetic of the one-lane reml_math.h routines in the
+    // same order, so the tables are identical to a serial build.
     double* F = Fm + (size_t)k * p * p;
-    if (tid == 0) {
-        if (full_f)
-            for (int i = 0; i < p * p; ++i) F[i] = L[i];
-        else
-            for (int i = 0; i < p * p; ++i) F[i] = 0.0;
-        // ridge plans (glm/arma.py:1160-1176, 2724-2755): A + ridge I in the
-        // log-determinant and in the solve for beta
-        if (ridge > 0.0)
-            for (int j = 0; j < p; ++j) L[j * p + j] += ridge;
-        if (cholesky_inplace(p, L) != 0) {
-            sh_fail = 1;
-        } else {
+    for (int i = tid; i < p * p; i += 64) F[i] = full_f ? L[i] : 0.0;
+    __syncthreads();
+    // ridge plans (glm/arma.py:1160-1176, 2724-2755): A + ridge I in the
+    // log-determinant and in the solve for beta
+    if (ridge > 0.0)
+        for (int j = tid; j < p; j += 64) L[j * p + j] += ridge;
+    __syncthreads();
+    if (block_cholesky(p, L, &sh_fail) == 0) {
+        if (tid == 0) {
             double s = 0.0;
             if (ridge > 0.0)
                 for (int j = 0; j < p; ++j) s += 2.0 * log(L[j * p + j]);
             else
                 for (int j = 0; j < p; ++j) s += 2.0 * log(fabs(L[j * p + j]) + 1e-10);
             sh_ldX = s;
-            cholesky_inverse(p, L, Ai, wk);
-            if (ridge > 0.0) {
-                // |y_w - X_w beta|^2 = E - c' W c, c = X_w' y_w, M = (A + rI)^-1,
-                // W = M (A + 2rI) M = M + r M M; L <- chol(W)
-                double* Wm = wk;
-                for (int i = 0; i < p; ++i)
-                    for (int j = 0; j < p; ++j) {
-                        double mm = 0.0;
-                        for (int q = 0; q < p; ++q) mm += Ai[i * p + q] * Ai[q * p + j];
-                        Wm[i * p + j] = Ai[i * p + j] + ridge * mm;
-                    }
-                for (int i = 0; i < p * p; ++i) L[i] = Wm[i];
-                if (cholesky_inplace(p, L) != 0) sh_fail = 1;
+        }
+        block_cholesky_inverse(p, L, Ai, wk);
+        if (ridge > 0.0) {
+            // |y_w - X_w beta|^2 = E - c' W c, c = X_w' y_w, M = (A + rI)^-1,
+            // W = M (A + 2rI) M = M + r M M; L <- chol(W)
+            double* Wm = wk;
+            for (int idx = tid; idx < p * p; idx += 64) {
+                const int i = idx / p, j = idx % p;
+                double mm = 0.0;
+                for (int q = 0; q < p; ++q) mm += Ai[i * p + q] * Ai[q * p + j];
+                Wm[idx] = Ai[idx] + ridge * mm;
             }
-            if (!full_f) {
-                // inv(A^-1[task, task] + 1e-8 I)   (glm/arma.py:4888-4896)
-                // At (wk) -> inverse in F's storage (work: W2) -> copied to W2 ->
-                // F zeroed and the block embedded at the task positions.
-                double* At = wk;
-                double* W2 = wk + p * p;
-                for (int i = 0; i < n_task; ++i)
-                    for (int j = 0; j < n_task; ++j)
-                        At[i * n_task + j] = Ai[task_idx[i] * p + task_idx[j]] + (i == j ? 1e-8 : 0.0);
-                if (gauss_jordan_inverse(n_task, At, F, W2) != 0) {
-                    sh_fail = 2;
-                } else {
-                    for (int q = 0; q < n_task * n_task; ++q) W2[q] = F[q];
-                    for (int q = 0; q < p * p; ++q) F[q] = 0.0;
-                    for (int a2 = 0; a2 < n_task; ++a2)
-                        for (int b2 = 0; b2 < n_task; ++b2)
-                            F[task_idx[a2] * p + task_idx[b2]] = W2[a2 * n_task + b2];
+            __syncthreads();
+            for (int i = tid; i < p * p; i += 64) L[i] = Wm[i];
+            __syncthreads();
+            block_cholesky(p, L, &sh_fail);
+        }
+        if (!full_f && !sh_fail) {
+            // inv(A^-1[task, task] + 1e-8 I)   (glm/arma.py:4888-4896)
+            // At (wk) -> inverse in F's storage (work: W2) -> copied to W2 ->
+            // F zeroed and the block embedded at the task positions.
+            double* At = wk;
+            double* W2 = wk + p * p;
+            for (int idx = tid; idx < n_task * n_task; idx += 64) {
+                const int i = idx / n_task, j = idx % n_task;
+                At[idx] = Ai[task_idx[i] * p + task_idx[j]] + (i == j ? 1e-8 : 0.0);
+            }
+            __syncthreads();
+            if (block_gauss_jordan_inverse(n_task, At, F, W2, &sh_piv, &sh_fail, 2) == 0) {
+                for (int q = tid; q < n_task * n_task; q += 64) W2[q] = F[q];
+                __syncthreads();
+                for (int q = tid; q < p * p; q += 64) F[q] = 0.0;
+                __syncthreads();
+                for (int idx = tid; idx < n_task * n_task; idx += 64) {
+                    const int a2 = idx / n_task, b2 = idx % n_task;
+                    F[task_idx[a2] * p + task_idx[b2]] = W2[idx];
                 }
             }
         }
