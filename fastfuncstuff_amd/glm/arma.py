@@ -1001,7 +1001,7 @@ def fit_glm_arma11(data, design, tr, a_grid=None, b_grid=None, estimate_per_voxe
                 del Yt
                 best, score = grid.select(S, "exhaustive" if force_exhaustive_search else (selector or "hierarchical"))
                 if surface_acc is not None:
-                    surface_acc[b0:b1] = S.cpu()
+                    surface_acc[b0:b1].copy_(S)
                 del S
             else:
                 best = fixed_best[b0:b1].to(dev)
@@ -1010,26 +1010,26 @@ def fit_glm_arma11(data, design, tr, a_grid=None, b_grid=None, estimate_per_voxe
                 if not use_double:
                     _, scale, _ = grid.prep(Yb, normalize=True)
             if scale_all is not None and scale is not None and not use_double:
-                scale_all[b0:b1] = scale.double().cpu()
+                scale_all[b0:b1].copy_(scale.double())
             g = grid.gls(Yb, None if use_double else scale, best, want_resid_w=want_ljung_box or want_residuals,
                          want_residuals=want_residuals, want_predicted=want_predicted)
             ti = task_indices
-            res.betas[b0:b1] = (g["betas"][:, ti] if ti else g["betas"]).cpu().to(out_dtype)
-            res.tstats[b0:b1] = (g["tstats"][:, ti] if ti else g["tstats"]).cpu().to(out_dtype)
-            res.fstats[b0:b1] = g["fstats"].cpu().to(out_dtype)
-            res.sigma2[b0:b1] = g["sigma2"].cpu().to(out_dtype)
-            res.r2[b0:b1] = g["r2"].cpu()
+            res.betas[b0:b1].copy_(g["betas"][:, ti] if ti else g["betas"])
+            res.tstats[b0:b1].copy_(g["tstats"][:, ti] if ti else g["tstats"])
+            res.fstats[b0:b1].copy_(g["fstats"])
+            res.sigma2[b0:b1].copy_(g["sigma2"])
+            res.r2[b0:b1].copy_(g["r2"])
             sel = params_dev[best.long()]
-            res.arma_params[b0:b1] = sel.cpu().to(out_dtype)
-            res.reml_likelihood[b0:b1] = score.cpu().to(out_dtype)
-            res.pair_index[b0:b1] = best.cpu().long()
+            res.arma_params[b0:b1].copy_(sel)
+            res.reml_likelihood[b0:b1].copy_(score)
+            res.pair_index[b0:b1].copy_(best)
             if want_residuals:
-                res.residuals[b0:b1] = g["residuals"].cpu().to(out_dtype)
-                res.residuals_whitened[b0:b1] = g["resid_w"].cpu().to(out_dtype)
+                res.residuals[b0:b1].copy_(g["residuals"])
+                res.residuals_whitened[b0:b1].copy_(g["resid_w"])
             if want_predicted:
-                res.predicted[b0:b1] = g["predicted"].cpu().to(out_dtype)
+                res.predicted[b0:b1].copy_(g["predicted"])
             if want_ljung_box:
-                res.ljung_box[b0:b1] = ljung_box_device(g["resid_w"], lj_lag, lj_tau).cpu()
+                res.ljung_box[b0:b1].copy_(ljung_box_device(g["resid_w"], lj_lag, lj_tau))
             if C is not None:
                 # contrast estimates and t in the normalised units of the fit
                 # (glm/arma.py:5429-5471), estimates unscaled like betas (:5835)
@@ -1037,15 +1037,15 @@ def fit_glm_arma11(data, design, tr, a_grid=None, b_grid=None, estimate_per_voxe
                 cb = (g["betas"].double() / sb[:, None]) @ C.T
                 var = (g["sigma2"].double() / sb**2)[:, None] * Qc[best.long()]
                 ct = cb / (var.clamp(min=0.0).sqrt() + 1e-10)
-                res.contrast_betas[b0:b1] = (cb * sb[:, None]).cpu().to(out_dtype)
-                res.contrast_tstats[b0:b1] = ct.cpu().to(out_dtype)
+                res.contrast_betas[b0:b1].copy_(cb * sb[:, None])
+                res.contrast_tstats[b0:b1].copy_(ct)
                 if want_r2_partial or want_r2_semipartial:
                     cp = ct**2 / (ct**2 + res.dof)
                     if want_r2_partial:
-                        res.contrast_r2_partial[b0:b1] = cp.cpu().to(out_dtype)
+                        res.contrast_r2_partial[b0:b1].copy_(cp)
                     if want_r2_semipartial:
                         rem = (1.0 - g["r2"].double()).clamp(min=0.0)[:, None]
-                        res.contrast_r2_semipartial[b0:b1] = (cp * rem).cpu().to(out_dtype)
+                        res.contrast_r2_semipartial[b0:b1].copy_(cp * rem)
             if set_mats is not None:
                 # glm/arma.py:3564-3585 in the normalised units of the fit; SD and
                 # coef unscaled like betas (:5845-5848)
@@ -1059,10 +1059,10 @@ def fit_glm_arma11(data, design, tr, a_grid=None, b_grid=None, estimate_per_voxe
                     bs = bn.index_select(1, ix)
                     Fv = torch.einsum("bi,bij,bj->b", bs, Minv[best.long()], bs) / (ix.numel() * s2n + 1e-30)
                     sdv = torch.einsum("bi,ij,bj->b", bs, Cs, bs).clamp(min=0).sqrt() * sb
-                    res.set_fstats[b0:b1, gi] = Fv.cpu().to(out_dtype)
-                    res.set_sd[b0:b1, gi] = sdv.cpu().to(out_dtype)
+                    res.set_fstats[b0:b1, gi].copy_(Fv)
+                    res.set_sd[b0:b1, gi].copy_(sdv)
                     if ix.numel() == 1:
-                        res.set_coef[b0:b1, gi] = (bs[:, 0] * sb).cpu().to(out_dtype)
+                        res.set_coef[b0:b1, gi].copy_(bs[:, 0] * sb)
             if want_r2_partial or want_r2_semipartial:
                 part = _partial_r2(g["tstats"].to(out_dtype), g["r2"], res.dof, p, task_indices, want_r2_partial,
                                    r2_partial_mode, want_r2_semipartial, r2_semipartial_mode)
@@ -1071,7 +1071,7 @@ def fit_glm_arma11(data, design, tr, a_grid=None, b_grid=None, estimate_per_voxe
                         continue
                     if getattr(res, name) is None:
                         setattr(res, name, torch.zeros((V,) + tuple(val.shape[1:]), dtype=val.dtype))
-                    getattr(res, name)[b0:b1] = val.cpu()
+                    getattr(res, name)[b0:b1].copy_(val)
     if kwargs.get("want_ols"):
         # glm/arma.py:4288-4323: the OLS baseline on the same data and design
         from .core import fit_glm
