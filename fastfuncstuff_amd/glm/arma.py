@@ -1213,11 +1213,11 @@ def _fit_global(data, design, tr, a_grid, b_grid, dev, *, original_shape, task_i
         ss_tot = (Y - Y.mean(1, keepdim=True)).pow(2).sum(1)
         r2 = torch.where(ss_tot <= 1e-10, torch.zeros_like(ss_tot),
                          (1.0 - (Y - pred).pow(2).sum(1) / ss_tot.clamp(min=1e-10)).clamp(max=1.0))
-        res.betas[b0:b1] = (beta * sc).cpu().to(out_dtype)
-        res.tstats[b0:b1] = t.cpu().to(out_dtype)
-        res.sigma2[b0:b1] = (sig * sc[:, 0] ** 2).cpu().to(out_dtype)
-        res.fstats[b0:b1] = F.cpu().to(out_dtype)
-        res.r2[b0:b1] = r2.cpu().to(torch.float32)
+        res.betas[b0:b1].copy_(beta * sc)
+        res.tstats[b0:b1].copy_(t)
+        res.sigma2[b0:b1].copy_(sig * sc[:, 0] ** 2)
+        res.fstats[b0:b1].copy_(F)
+        res.r2[b0:b1].copy_(r2)
         if set_mats is not None:
             for gi, m in enumerate(set_mats):
                 if m is None:
@@ -1229,7 +1229,7 @@ def _fit_global(data, design, tr, a_grid, b_grid, dev, *, original_shape, task_i
                 res.set_sd[b0:b1, gi] = (torch.einsum("bi,ij,bj->b", bs, Cs, bs).clamp(min=0).sqrt()
                                          * sc[:, 0]).cpu().to(out_dtype)
                 if ix.numel() == 1:
-                    res.set_coef[b0:b1, gi] = (bs[:, 0] * sc[:, 0]).cpu().to(out_dtype)
+                    res.set_coef[b0:b1, gi].copy_(bs[:, 0] * sc[:, 0])
         if want_r2_partial:
             part = _partial_r2(t.to(out_dtype), r2.float(), dof, p, task_indices, True, r2_partial_mode, False,
                                "full")
@@ -1238,14 +1238,14 @@ def _fit_global(data, design, tr, a_grid, b_grid, dev, *, original_shape, task_i
                     continue
                 if getattr(res, name) is None:
                     setattr(res, name, torch.zeros((V,) + tuple(val.shape[1:]), dtype=val.dtype))
-                getattr(res, name)[b0:b1] = val.cpu()
+                getattr(res, name)[b0:b1].copy_(val)
         if lj is not None:
-            res.ljung_box[b0:b1] = ljung_box_device(rw, lj[0], lj[1]).cpu()
+            res.ljung_box[b0:b1].copy_(ljung_box_device(rw, lj[0], lj[1]))
         if want_residuals:
-            res.residuals[b0:b1] = ((Y - pred) * sc).cpu().to(out_dtype)
-            res.residuals_whitened[b0:b1] = (rw * sc).cpu().to(out_dtype)
+            res.residuals[b0:b1].copy_(((Y - pred) * sc))
+            res.residuals_whitened[b0:b1].copy_(rw * sc)
         if want_predicted:
-            res.predicted[b0:b1] = (pred * sc).cpu().to(out_dtype)
+            res.predicted[b0:b1].copy_(pred * sc)
     return res
 
 
@@ -1348,13 +1348,13 @@ def _dsort_pass(grid: RemlGrid, data, design, dsort, res, scale_all, dev, out_dt
         Fv = quad / (sig * len(fci) + 1e-30)
         sc = s[:, None]
         bb, tb = beta[:, :p], tst[:, :p]
-        res.betas[b0:b1] = ((bb[:, task_indices] if task_indices else bb) * sc).cpu().to(out_dtype)
-        res.tstats[b0:b1] = (tb[:, task_indices] if task_indices else tb).cpu().to(out_dtype)
-        res.dsort_betas[b0:b1] = (beta[:, p:] * sc).cpu().to(out_dtype)
-        res.dsort_tstats[b0:b1] = tst[:, p:].cpu().to(out_dtype)
-        res.sigma2[b0:b1] = (sig * s**2).cpu().to(out_dtype)
-        res.fstats[b0:b1] = Fv.cpu().to(out_dtype)
-        res.r2[b0:b1] = r2.cpu().to(res.r2.dtype)
+        res.betas[b0:b1].copy_(((bb[:, task_indices] if task_indices else bb) * sc))
+        res.tstats[b0:b1].copy_(tb[:, task_indices] if task_indices else tb)
+        res.dsort_betas[b0:b1].copy_(beta[:, p:] * sc)
+        res.dsort_tstats[b0:b1].copy_(tst[:, p:])
+        res.sigma2[b0:b1].copy_(sig * s**2)
+        res.fstats[b0:b1].copy_(Fv)
+        res.r2[b0:b1].copy_(r2)
         if column_sets:
             # per voxel: the extended design's (X'X)^-1 block (glm/arma.py:3461-3480)
             Xo = torch.cat([Xd.unsqueeze(0).expand(B, T, p), D.transpose(1, 2)], 2)
@@ -1365,11 +1365,11 @@ def _dsort_pass(grid: RemlGrid, data, design, dsort, res, scale_all, dev, out_dt
                 bg = beta.index_select(1, ix)
                 eye_g = torch.eye(ix.numel(), dtype=f64, device=dev)
                 qg = torch.einsum("bi,bi->b", bg, torch.linalg.solve(Ag + 1e-8 * eye_g, bg.unsqueeze(2)).squeeze(2))
-                res.set_fstats[b0:b1, gi] = (qg / (ix.numel() * sig + 1e-30)).cpu().to(out_dtype)
+                res.set_fstats[b0:b1, gi].copy_((qg / (ix.numel() * sig + 1e-30)))
                 sigt = torch.einsum("bti,bi->bt", Xo.index_select(2, ix), bg)
-                res.set_sd[b0:b1, gi] = (sigt.std(dim=1, correction=0) * s).cpu().to(out_dtype)
+                res.set_sd[b0:b1, gi].copy_((sigt.std(dim=1, correction=0) * s))
                 if ix.numel() == 1:
-                    res.set_coef[b0:b1, gi] = (bg[:, 0] * s).cpu().to(out_dtype)
+                    res.set_coef[b0:b1, gi].copy_(bg[:, 0] * s)
         if want_r2_partial or want_r2_semipartial:
             part = _partial_r2(tb.to(out_dtype), r2.float(), dof, p, task_indices, want_r2_partial, r2_partial_mode,
                                want_r2_semipartial, r2_semipartial_mode)
@@ -1378,27 +1378,27 @@ def _dsort_pass(grid: RemlGrid, data, design, dsort, res, scale_all, dev, out_dt
                     continue
                 if getattr(res, name) is None:
                     setattr(res, name, torch.zeros((V,) + tuple(val.shape[1:]), dtype=val.dtype))
-                getattr(res, name)[b0:b1] = val.cpu()
+                getattr(res, name)[b0:b1].copy_(val)
         if C is not None:
             # GLTs on the base-design block (dsort betas cannot enter a GLT)
             Cd = C.to(dev, f64)
             cb = bb @ Cd.T
             cv = sig[:, None] * torch.einsum("cr,brs,cs->bc", Cd, Ai[:, :p, :p], Cd)
             ct = cb / (cv.clamp(min=0.0).sqrt() + 1e-10)
-            res.contrast_betas[b0:b1] = (cb * sc).cpu().to(out_dtype)
-            res.contrast_tstats[b0:b1] = ct.cpu().to(out_dtype)
+            res.contrast_betas[b0:b1].copy_(cb * sc)
+            res.contrast_tstats[b0:b1].copy_(ct)
             if want_r2_partial and res.contrast_r2_partial is not None:
-                res.contrast_r2_partial[b0:b1] = (ct**2 / (ct**2 + dof)).cpu().to(out_dtype)
+                res.contrast_r2_partial[b0:b1].copy_((ct**2 / (ct**2 + dof)))
             if want_r2_semipartial and res.contrast_r2_semipartial is not None:
                 rem = (1.0 - r2).clamp(min=0.0)[:, None]
-                res.contrast_r2_semipartial[b0:b1] = (ct**2 / (ct**2 + dof) * rem).cpu().to(out_dtype)
+                res.contrast_r2_semipartial[b0:b1].copy_((ct**2 / (ct**2 + dof) * rem))
         if lj is not None and res.ljung_box is not None:
-            res.ljung_box[b0:b1] = ljung_box_device(rw, lj[0], lj[1]).cpu()
+            res.ljung_box[b0:b1].copy_(ljung_box_device(rw, lj[0], lj[1]))
         if want_residuals:
-            res.residuals[b0:b1] = ((Y - pred) * sc).cpu().to(out_dtype)
-            res.residuals_whitened[b0:b1] = (rw * sc).cpu().to(out_dtype)
+            res.residuals[b0:b1].copy_(((Y - pred) * sc))
+            res.residuals_whitened[b0:b1].copy_(rw * sc)
         if want_predicted:
-            res.predicted[b0:b1] = (pred * sc).cpu().to(out_dtype)
+            res.predicted[b0:b1].copy_(pred * sc)
 
 
 def _partial_r2(t_full, r2, dof, p, task_indices, want_partial, partial_mode, want_semi, semi_mode) -> dict:
