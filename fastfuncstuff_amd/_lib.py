@@ -56,6 +56,7 @@ SIGNATURES: dict[str, list] = {
     "ffs_reml_subset_search": [_vp, _vp, _i64, _i64, _vp, _vp, _i64, _vp, _vp, _vp, _vp, _vp, _i64, _vp],
     "ffs_reml_plan_set_bands": [_vp, ctypes.c_double, ctypes.c_double, _vp],
     "ffs_reml_plan_band_info": [_vp, _vp, _vp, _vp, _vp, _vp],
+    "ffs_reml_plan_window_info": [_vp, _vp, _vp, _vp, _vp, _vp],
     "ffs_reml_band_workspace": [_vp, _i64, _vp],
     "ffs_reml_band_prepare": [_vp, _vp, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _i64, _vp],
     "ffs_reml_band_energy": [_vp, _i64, _vp, _i64, _vp],

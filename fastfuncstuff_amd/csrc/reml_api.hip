@@ -32,6 +32,13 @@ static int fail(int code, const std::string& msg) {
     return code;
 }
 
+// Search entry points need the lattice and search operands a pair-list plan lacks.
+#define FFS_NEED_SEARCH(P, what)                                                                   \
+    do {                                                                                           \
+        if ((P)->gls_only)                                                                         \
+            return fail(-4, std::string(what) + ": pair-list plan (no lattice) supports GLS only"); \
+    } while (0)
+
 #define HIP_TRY(expr)                                                                          \
     do {                                                                                       \
         hipError_t _e = (expr);                                                                \
@@ -53,6 +60,9 @@ struct ffs_reml_plan {
     // Yt buffer whose rows Tpad..2 Tpad-1 the last ffs_reml_prep filled with
     // split halves (projection reads them only for this buffer)
     mutable const float* split_yt = nullptr;
+    // no lattice: a pair-list plan for the GLS / whitening entry points only
+    // (no search operands, windows or selector tables are built)
+    bool gls_only = false;
     // host copies of the pair list and lattice (band tables are built from them)
     std::vector<double> h_ab;
     std::vector<int32_t> h_lut;
@@ -139,6 +149,10 @@ static int plan_create_impl(ffs_reml_plan** out, int32_t T, int32_t p, const dou
     if (T < 2 || p < 1 || K < 1) return fail(-3, "plan_create: need T >= 2, p >= 1, K >= 1");
     if (p > 96) return fail(-3, "plan_create: p > 96 is not supported");
     if (gap[T - 1] != 0) return fail(-3, "plan_create: gap[T-1] must be 0 (end of the last run)");
+    if ((n_a <= 0) != (n_b <= 0)) return fail(-3, "plan_create: n_a and n_b must both be > 0 or both 0");
+    const bool gls_only = n_a <= 0;
+    if (gls_only && ridge > 0.0) return fail(-3, "plan_create_ridge: a ridge plan needs a lattice (it is search-only)");
+    if (gls_only) n_a = n_b = 0;
     for (int t = 0; t < T; ++t)
         if (gap[t] < 0) return fail(-3, "plan_create: negative time gap");
     hipStream_t s = (hipStream_t)stream;
@@ -146,8 +160,9 @@ static int plan_create_impl(ffs_reml_plan** out, int32_t T, int32_t p, const dou
     std::unique_ptr<ffs_reml_plan> guard(P);
     HIP_TRY(hipGetDevice(&P->device));
     PlanDev& d = P->d;
+    P->gls_only = gls_only;
     P->h_ab.assign(ab, ab + (size_t)2 * K);
-    P->h_lut.assign(lut, lut + (size_t)n_a * n_b);
+    if (!gls_only) P->h_lut.assign(lut, lut + (size_t)n_a * n_b);
     d.T = T;
     d.Tpad = (T + kProjBK - 1) / kProjBK * kProjBK;
     d.p = p;
@@ -214,12 +229,16 @@ static int plan_create_impl(ffs_reml_plan** out, int32_t T, int32_t p, const dou
         (rc = P->alloc(&Xw64, (size_t)K * T * p)) || (rc = P->alloc(&G64, (size_t)K * T * p)) ||
         (rc = P->alloc(&Lx, (size_t)K * p * p)) || (rc = P->alloc(&Ainv, (size_t)K * p * p)) ||
         (rc = P->alloc(&Fm, (size_t)K * p * p)) || (rc = P->alloc(&work, (size_t)K * 2 * p * p)) ||
-        (rc = P->alloc(&ld, K)) || (rc = P->alloc(&ldR, K)) || (rc = P->alloc(&ldX, K)) ||
-        (rc = P->alloc(&G32, (size_t)d.n_tiles * d.Tpad * BN)) ||
-        (rc = P->alloc(&ecoef, (size_t)d.n_chunks * T * 2 * kEnergyNB)) ||
-        (rc = P->alloc(&ec, (size_t)d.n_chunks * T)) || (rc = P->alloc(&es0, (size_t)d.n_chunks * kEnergyNB)))
+        (rc = P->alloc(&ld, K)) || (rc = P->alloc(&ldR, K)) || (rc = P->alloc(&ldX, K)))
         return rc;
-    HIP_TRY(hipMemsetAsync(G32, 0, sizeof(float) * d.n_tiles * d.Tpad * BN, s));
+    G32 = ecoef = ec = es0 = nullptr;
+    if (!gls_only) {
+        if ((rc = P->alloc(&G32, (size_t)d.n_tiles * d.Tpad * BN)) ||
+            (rc = P->alloc(&ecoef, (size_t)d.n_chunks * T * 2 * kEnergyNB)) ||
+            (rc = P->alloc(&ec, (size_t)d.n_chunks * T)) || (rc = P->alloc(&es0, (size_t)d.n_chunks * kEnergyNB)))
+            return rc;
+        HIP_TRY(hipMemsetAsync(G32, 0, sizeof(float) * d.n_tiles * d.Tpad * BN, s));
+    }
     // Local-search tables: lattice axes, per-pair fp32 coefficients, pair-padded operand.
     std::vector<float> a_axis(n_a, 0.f), b_axis(n_b, 0.f);
     for (int ai = 0; ai < n_a; ++ai)
@@ -230,22 +249,27 @@ static int plan_create_impl(ffs_reml_plan** out, int32_t T, int32_t p, const dou
                 b_axis[bi] = (float)ab[2 * q + 1];
             }
         }
-    float *da_axis, *db_axis, *lc, *ls0;
-    float2* lp;
+    float *da_axis, *db_axis, *lc = nullptr, *ls0 = nullptr;
+    float2* lp = nullptr;
     if ((rc = upload(P, &da_axis, a_axis.data(), a_axis.size(), s)) ||
-        (rc = upload(P, &db_axis, b_axis.data(), b_axis.size(), s)) || (rc = P->alloc(&lp, (size_t)K * T)) ||
-        (rc = P->alloc(&lc, (size_t)K * T)) || (rc = P->alloc(&ls0, K)))
+        (rc = upload(P, &db_axis, b_axis.data(), b_axis.size(), s)))
         return rc;
-    uint16_t *Ghf, *Glf, *Ghb, *Glb;
+    if (!gls_only && ((rc = P->alloc(&lp, (size_t)K * T)) || (rc = P->alloc(&lc, (size_t)K * T)) ||
+                      (rc = P->alloc(&ls0, K))))
+        return rc;
+    uint16_t *Ghf = nullptr, *Glf = nullptr, *Ghb = nullptr, *Glb = nullptr;
     float* g16s;
-    const size_t n16 = (size_t)d.n_tiles * d.Tpad * BN;
-    if ((rc = P->alloc(&Ghf, n16)) || (rc = P->alloc(&Glf, n16)) || (rc = P->alloc(&Ghb, n16)) ||
-        (rc = P->alloc(&Glb, n16)) || (rc = P->alloc(&g16s, K)))
-        return rc;
-    HIP_TRY(hipMemsetAsync(Ghf, 0, 2 * n16, s));
-    HIP_TRY(hipMemsetAsync(Glf, 0, 2 * n16, s));
-    HIP_TRY(hipMemsetAsync(Ghb, 0, 2 * n16, s));
-    HIP_TRY(hipMemsetAsync(Glb, 0, 2 * n16, s));
+    if ((rc = P->alloc(&g16s, K))) return rc;
+    if (!gls_only) {
+        const size_t n16 = (size_t)d.n_tiles * d.Tpad * BN;
+        if ((rc = P->alloc(&Ghf, n16)) || (rc = P->alloc(&Glf, n16)) || (rc = P->alloc(&Ghb, n16)) ||
+            (rc = P->alloc(&Glb, n16)))
+            return rc;
+        HIP_TRY(hipMemsetAsync(Ghf, 0, 2 * n16, s));
+        HIP_TRY(hipMemsetAsync(Glf, 0, 2 * n16, s));
+        HIP_TRY(hipMemsetAsync(Ghb, 0, 2 * n16, s));
+        HIP_TRY(hipMemsetAsync(Glb, 0, 2 * n16, s));
+    }
     // f64 MFMA GLS tables
     double *G64p = nullptr, *coefR = nullptr, *dXp = nullptr, *dXtX = nullptr, *dPp = nullptr;
     d.PG = p <= kGlsMfmaMaxP ? 16 * ((p + 15) / 16) : 0;
@@ -275,11 +299,13 @@ static int plan_create_impl(ffs_reml_plan** out, int32_t T, int32_t p, const dou
                        full_f, BN, npt, soa, coef64, Xw64, G64, Lx, Ainv, Fm, work, ld, ldR, ldX, G32, Ghf, Glf, Ghb, Glb, g16s,
                        d.PG, G64p, coefR, ridge, dstatus);
     HIP_TRY(hipGetLastError());
-    hipLaunchKernelGGL(k_local_tables, dim3(K), dim3(128), 0, s, T, K, soa, lp, lc, ls0);
-    HIP_TRY(hipGetLastError());
-    hipLaunchKernelGGL(k_energy_tables, dim3(d.n_chunks), dim3(128), 0, s, T, d.n_chunks, dchunk, soa, ecoef, ec,
-                       es0);
-    HIP_TRY(hipGetLastError());
+    if (!gls_only) {
+        hipLaunchKernelGGL(k_local_tables, dim3(K), dim3(128), 0, s, T, K, soa, lp, lc, ls0);
+        HIP_TRY(hipGetLastError());
+        hipLaunchKernelGGL(k_energy_tables, dim3(d.n_chunks), dim3(128), 0, s, T, d.n_chunks, dchunk, soa, ecoef,
+                           ec, es0);
+        HIP_TRY(hipGetLastError());
+    }
     int32_t status = 0;
     HIP_TRY(hipMemcpyAsync(&status, dstatus, sizeof(int32_t), hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
@@ -320,7 +346,8 @@ static int plan_create_impl(ffs_reml_plan** out, int32_t T, int32_t p, const dou
     d.XtX = dXtX;
     d.pinvTp = dPp;
     d.proj_mode = 0;
-    if (int rc2 = build_windows(P, s)) return rc2;
+    if (!gls_only)
+        if (int rc2 = build_windows(P, s)) return rc2;
     P->pinvT = dpinv;
     P->ridge = ridge;
     P->ldR = ldR;
@@ -482,6 +509,7 @@ static int energy_impl(const ffs_reml_plan* P, const float* Yt, int64_t ldYt, in
 extern "C" int ffs_reml_energy(const ffs_reml_plan* P, const float* Yt, int64_t ldYt, int64_t V, float* E,
                                int64_t ldE, void* stream) {
     if (!P) return fail(-3, "energy: null plan");
+    FFS_NEED_SEARCH(P, "energy");
     if (V <= 0) return 0;
     return energy_impl(P, Yt, ldYt, V, E, ldE, nullptr, (hipStream_t)stream);
 }
@@ -531,6 +559,7 @@ static int project_impl(const ffs_reml_plan* P, const float* Yt, int64_t ldYt, i
 extern "C" int ffs_reml_project(const ffs_reml_plan* P, const float* Yt, int64_t ldYt, int64_t V, float* S,
                                 int64_t ldS, void* stream) {
     if (!P) return fail(-3, "project: null plan");
+    FFS_NEED_SEARCH(P, "project");
     if (V <= 0) return 0;
     // split modes read the halves that prep wrote when p <= 32 (f64 MFMA
     // prep); otherwise they split the residual in the kernel
@@ -545,6 +574,7 @@ extern "C" int ffs_reml_project(const ffs_reml_plan* P, const float* Yt, int64_t
 extern "C" int ffs_reml_surface(const ffs_reml_plan* P, const float* Yt, int64_t ldYt, int64_t V, float* S,
                                 int64_t ldS, int32_t n_chunks, void* stream) {
     if (!P) return fail(-3, "surface: null plan");
+    FFS_NEED_SEARCH(P, "surface");
     if (V <= 0) return 0;
     if (n_chunks < 1 || n_chunks > ffs_reml_plan::kMaxSurfaceChunks)
         return fail(-3, "surface: n_chunks must be in [1, 16]");
@@ -579,6 +609,7 @@ extern "C" int ffs_reml_surface(const ffs_reml_plan* P, const float* Yt, int64_t
 extern "C" int ffs_reml_select(const ffs_reml_plan* P, const float* S, int64_t ldS, int64_t V, int32_t mode,
                                int32_t* best, float* best_score, void* stream) {
     if (!P) return fail(-3, "select: null plan");
+    FFS_NEED_SEARCH(P, "select");
     if (mode < 0 || mode > 2)
         return fail(-3, "select: mode must be 0 (exhaustive), 1 (hierarchical) or 2 (subset descent)");
     if (V <= 0) return 0;
@@ -763,9 +794,11 @@ int build_set_table(ffs_reml_plan* P, const std::vector<std::vector<int32_t>>& s
     return 0;
 }
 
-// Windows of every lattice point: slots in (a, b) order (= pair order), edge =
-// the slot lies on a window side that the lattice continues past, next = the
-// window around that slot's pair.
+// Windows of every lattice cell: the kWin x kWin lattice cells around it.
+// Slots are the valid pairs among them in pair order (a-major, b-minor);
+// edge = the slot lies on a window side that the lattice continues past (a
+// better pair may lie beyond it, also past an invalid neighbour); next = the
+// window centred on that slot (kept inside the lattice).
 int build_windows(ffs_reml_plan* P, hipStream_t s) {
     const int n_a = P->d.n_a, n_b = P->d.n_b;
     std::vector<std::vector<int32_t>> sets((size_t)n_a * n_b), edges(sets.size()), nexts(sets.size());
@@ -840,6 +873,7 @@ SetPtrs set_ptrs(const SetLayout& L, void* workspace) {
 int set_check(const ffs_reml_plan* P, const SetTable& ST, const char* what, int64_t V, void* workspace,
               int64_t ws_bytes, SetLayout* L) {
     if (!P) return fail(-3, std::string(what) + ": null plan");
+    FFS_NEED_SEARCH(P, what);
     if (ST.n <= 0) return fail(-4, std::string(what) + ": no sets on this plan");
     if (P->d.proj_mode != 0 && !P->d.PG)
         return fail(-4, std::string(what) + ": the split projection modes need p <= 32 here (use fp32)");
@@ -975,6 +1009,7 @@ extern "C" int ffs_reml_scatter_best(const int32_t* idx, int64_t n, const int32_
 // ---------------------------------------------------------------------------
 extern "C" int ffs_reml_plan_set_bands(ffs_reml_plan* P, double half_width, double bin_width, void* stream) {
     if (!P) return fail(-3, "plan_set_bands: null plan");
+    FFS_NEED_SEARCH(P, "plan_set_bands");
     PlanDev& d = P->d;
     if (!(half_width > 0.0) || !(bin_width > 0.0)) {
         d.band = SetTable{};
@@ -1034,6 +1069,21 @@ extern "C" int ffs_reml_plan_band_info(const ffs_reml_plan* P, int32_t* n_bands,
     if (!P) return fail(-3, "plan_band_info: null plan");
     const SetTable& t = P->d.band;
     if (n_bands) *n_bands = t.n;
+    if (slots) *slots = t.sb;
+    const size_t n = (size_t)t.n * t.sb;
+    if (t.n > 0) {
+        if (pairs) HIP_TRY(hipMemcpy(pairs, t.pairs, 4 * n, hipMemcpyDeviceToHost));
+        if (edge) HIP_TRY(hipMemcpy(edge, t.edge, 4 * n, hipMemcpyDeviceToHost));
+        if (counts) HIP_TRY(hipMemcpy(counts, t.cnt, 4 * (size_t)t.n, hipMemcpyDeviceToHost));
+    }
+    return 0;
+}
+
+extern "C" int ffs_reml_plan_window_info(const ffs_reml_plan* P, int32_t* n_windows, int32_t* slots,
+                                         int32_t* pairs, int32_t* counts, int32_t* edge) {
+    if (!P) return fail(-3, "plan_window_info: null plan");
+    const SetTable& t = P->d.win;
+    if (n_windows) *n_windows = t.n;
     if (slots) *slots = t.sb;
     const size_t n = (size_t)t.n * t.sb;
     if (t.n > 0) {
@@ -1128,6 +1178,7 @@ extern "C" int ffs_reml_subset_search(const ffs_reml_plan* P, const float* Y, in
                                       float* best_score, float* scale, const double* coef, void* workspace,
                                       int64_t ws_bytes, void* stream) {
     if (!P) return fail(-3, "subset_search: null plan");
+    FFS_NEED_SEARCH(P, "subset_search");
     if (V <= 0 || cap <= 0) return 0;
     if (!idx || !n_live) return fail(-3, "subset_search: idx and n_live are required");
     if (cap > V) cap = V;

@@ -284,10 +284,12 @@ __global__ void __launch_bounds__(64) k_tables(int T, int Tpad, int p, int K, co
     __syncthreads();
     // fp32 projection operand, tile-major: G32[tile][t][slot*p + j]
     const int tile = k / npt, slot = k % npt;
-    float* Gt = G32 + (size_t)tile * Tpad * BN + slot * p;
-    for (int idx = tid; idx < T * p; idx += 64) {
-        const int t = idx / p, j = idx % p;
-        Gt[(size_t)t * BN + j] = (float)g[idx];
+    if (G32) {  // null for a pair-list (GLS-only) plan
+        float* Gt = G32 + (size_t)tile * Tpad * BN + slot * p;
+        for (int idx = tid; idx < T * p; idx += 64) {
+            const int t = idx / p, j = idx % p;
+            Gt[(size_t)t * BN + j] = (float)g[idx];
+        }
     }
     // split 16-bit operands, layout [tile][t/8][BN][8].  The fp16 halves hold
     // G * 2^e with 2^e * max|G| in [2^13, 2^14): the high half keeps headroom

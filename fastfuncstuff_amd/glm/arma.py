@@ -237,7 +237,7 @@ class RemlGrid(Mapping):
     live on the device inside the plan."""
 
     def __init__(self, X, a_grid, b_grid, device, run_starts=None, tau=None, task_indices=None, pairs=None,
-                 ridge: float = 0.0, ensure_zero: bool = True):
+                 ridge: float = 0.0, ensure_zero: bool = True, lattice: bool | None = None):
         self.device = _lib.require_device(device)
         lib = _lib.load()
         X = torch.as_tensor(X)
@@ -245,6 +245,10 @@ class RemlGrid(Mapping):
             raise ValueError("design must be 2-D (n_timepoints, n_regressors)")
         self.X64 = np.ascontiguousarray(X.detach().cpu().to(torch.float64).numpy())
         self.T, self.p = self.X64.shape
+        # ``lattice`` False: a pair list (e.g. per-voxel parameters loaded from
+        # -Rvar) for the GLS only; no (a, b) lattice, window or search tables,
+        # so the plan grows linearly with the number of pairs
+        self.lattice = (pairs is None) if lattice is None else bool(lattice)
         if pairs is None:
             a_grid, b_grid = _grid_tensor(a_grid, "a_grid"), _grid_tensor(b_grid, "b_grid")
             if ensure_zero:
@@ -258,14 +262,20 @@ class RemlGrid(Mapping):
         b_np = self.params[:, 1].double().numpy()
         lam32 = _compute_arma11_lambda(self.params[:, 0], self.params[:, 1])
         white = (lam32 == 0).to(torch.int32).numpy()
-        self.a_axis = sorted(set(a_np.tolist()))
-        self.b_axis = sorted(set(b_np.tolist()))
+        if self.lattice:
+            self.a_axis = sorted(set(a_np.tolist()))
+            self.b_axis = sorted(set(b_np.tolist()))
+        else:
+            if ridge > 0:
+                raise ValueError("a ridge plan needs an (a, b) lattice")
+            self.a_axis, self.b_axis = [], []
         n_a, n_b = len(self.a_axis), len(self.b_axis)
         ai = {v: i for i, v in enumerate(self.a_axis)}
         bi = {v: i for i, v in enumerate(self.b_axis)}
         lut = -np.ones((n_a, n_b), dtype=np.int32)
-        for k in range(self.K):
-            lut[ai[a_np[k]], bi[b_np[k]]] = k
+        if self.lattice:
+            for k in range(self.K):
+                lut[ai[a_np[k]], bi[b_np[k]]] = k
         self.lut = lut
         self.ridge = float(ridge)
         if self.ridge > 0 and task_indices is not None:
@@ -425,6 +435,21 @@ class RemlGrid(Mapping):
         cnt = torch.empty(nb.value, dtype=torch.int32)
         _lib.check(lib.ffs_reml_plan_band_info(self._handle, None, None, _lib.ptr(pairs), _lib.ptr(cnt),
                                                _lib.ptr(edge)), "band_info")
+        return pairs, cnt, edge
+
+    def window_info(self):
+        """(pairs, counts, edge) of the learned search's windows, one per
+        lattice cell ai * n_b + bi, as CPU int32 tensors (n_cells x slots,
+        n_cells, n_cells x slots)."""
+        lib = _lib.load()
+        nw, sw = ctypes.c_int32(), ctypes.c_int32()
+        _lib.check(lib.ffs_reml_plan_window_info(self._handle, ctypes.byref(nw), ctypes.byref(sw), None, None, None),
+                   "window_info")
+        pairs = torch.empty(nw.value, sw.value, dtype=torch.int32)
+        edge = torch.empty(nw.value, sw.value, dtype=torch.int32)
+        cnt = torch.empty(nw.value, dtype=torch.int32)
+        _lib.check(lib.ffs_reml_plan_window_info(self._handle, None, None, _lib.ptr(pairs), _lib.ptr(cnt),
+                                                 _lib.ptr(edge)), "window_info")
         return pairs, cnt, edge
 
     def ainv(self) -> torch.Tensor:
@@ -615,7 +640,7 @@ def reml_grid_search_batched(data, design, a_grid, b_grid, device=None, verbose=
     best_lik = torch.full((V,), float("inf"), dtype=dtype)
     if pairs.shape[0] == 0:
         return best_params, best_lik
-    grid = RemlGrid(X, None, None, dev, run_starts=run_starts, tau=tau, pairs=pairs, ridge=1e-6)
+    grid = RemlGrid(X, None, None, dev, run_starts=run_starts, tau=tau, pairs=pairs, ridge=1e-6, lattice=True)
     Yd = _to_device_f32(Y, dev)
     # The score is computed as |y_w|^2 - |projection|^2 in fp32; on raw data
     # with a large mean that difference loses most of its digits.  Removing the
@@ -828,8 +853,8 @@ def fit_glm_arma11(data, design, tr, a_grid=None, b_grid=None, estimate_per_voxe
                    r2_partial_mode="full", want_r2_semipartial=False, r2_semipartial_mode="full",
                    precomputed_arma_params=None, device=None, verbose=False, use_double=False,
                    force_exhaustive_search=False, task_indices=None, save_profile_likelihoods=False,
-                   run_starts=None, tau=None, grid: RemlGrid | None = None, projection: str = "fp32",
-                   selector: str | None = None, **kwargs) -> ARMA11Results:
+                   run_starts=None, tau=None, grid: RemlGrid | None = None, projection: str = "fp16x3",
+                   selector: str | None = None, proposal=None, **kwargs) -> ARMA11Results:
     """glm/arma.py:3650-5878 ``fit_glm_arma11`` on the MI355X engine.
 
     Per-voxel REML search over the (a, b) grid (hierarchical descent by default,
@@ -837,13 +862,30 @@ def fit_glm_arma11(data, design, tr, a_grid=None, b_grid=None, estimate_per_voxe
     the selected pair.  float32 mode divides each voxel by its std before the
     fit and unscales betas / sigma2 / residuals afterwards (glm/arma.py:3947-3954,
     5827-5862); ``use_double`` skips the normalisation.  Outputs are CPU tensors.
-    ``projection`` ("fp32", "fp16x3", "bf16x3") is the arithmetic of the surface
-    GEMM (an engine option; the reference computes it in fp32).  ``selector``
-    picks the search rule: "hierarchical" (default, the reference's GPU path),
-    "subset" (the reference's CPU default, glm/arma.py:2021-2221) or
-    "exhaustive" (same as ``force_exhaustive_search``).
+    ``projection`` ("fp16x3" default, "fp32", "bf16x3") is the arithmetic of
+    the surface GEMM (an engine option; the reference computes it in fp32;
+    split fp16 keeps ~22-bit operands with fp32 accumulation and meets the
+    same score / argmin bars).  ``selector`` picks the search rule:
+    "hierarchical" (default, the reference's GPU path), "subset" (the
+    reference's CPU default, glm/arma.py:2021-2221), "exhaustive" (same as
+    ``force_exhaustive_search``) or "learned" (no counterpart in the
+    reference: the proposal network's 5 x 5 window, then a lambda band, then
+    the full grid for what is left; its target is the exhaustive global
+    argmin).  ``proposal`` (learned only): a ``ProposalMLP``, a path written by
+    ``ProposalMLP.save``, or None to train one for this design first
+    (``learned.train_proposal``, a few seconds).
     """
     dev = _lib.require_device(device)
+    if selector == "learned":
+        if force_exhaustive_search:
+            raise ValueError("selector='learned' and force_exhaustive_search are exclusive")
+        if use_double:
+            raise NotImplementedError("the learned search scores normalised float32 data; use_double is not supported")
+        if save_profile_likelihoods:
+            raise ValueError("selector='learned' evaluates no full surface; save_profile_likelihoods needs a "
+                             "full-grid selector")
+    elif selector not in (None, "hierarchical", "subset", "exhaustive"):
+        raise ValueError(f"selector must be 'hierarchical', 'subset', 'exhaustive' or 'learned', got {selector!r}")
     out_dtype = torch.float64 if use_double else torch.float32
     data = torch.as_tensor(data)
     design = torch.as_tensor(design)
@@ -889,7 +931,7 @@ def fit_glm_arma11(data, design, tr, a_grid=None, b_grid=None, estimate_per_voxe
             a, b = uniq[bad][0].tolist()
             raise ValueError(f"Invalid ARMA(1,1) parameters: a={a}, b={b}")
         grid = RemlGrid(design, None, None, dev, run_starts=run_starts, tau=tau, task_indices=task_indices,
-                        pairs=uniq.to(pp.dtype if pp.dtype.is_floating_point else torch.float64))
+                        pairs=uniq.to(pp.dtype if pp.dtype.is_floating_point else torch.float64), lattice=False)
         fixed_best = inv.to(torch.int32)
     else:
         if a_grid is None or b_grid is None:
@@ -898,8 +940,33 @@ def fit_glm_arma11(data, design, tr, a_grid=None, b_grid=None, estimate_per_voxe
             b_grid = db if b_grid is None else b_grid
         if grid is None:
             grid = RemlGrid(design, a_grid, b_grid, dev, run_starts=run_starts, tau=tau, task_indices=task_indices)
-        grid.set_projection(projection)
+        if grid.projection != projection:
+            grid.set_projection(projection)
         fixed_best = None
+
+    searcher = None
+    if selector == "learned" and fixed_best is None:
+        from ..learned import LocalSearcher, ProposalMLP, train_proposal
+
+        if grid.p > 32 and grid.projection != "fp32":
+            grid.set_projection("fp32")  # the pair-set kernels take split operands for p <= 32 only
+        if grid.bands is None:
+            grid.set_bands(0.10, 0.02)
+        if proposal is None:
+            mlp = train_proposal(grid)
+        elif isinstance(proposal, ProposalMLP):
+            mlp = proposal
+        else:
+            mlp = ProposalMLP.load(proposal, dev)
+        res_searchers: dict = {}
+
+        def searcher(Yb):
+            ls = res_searchers.get(Yb.shape[0])
+            if ls is None:
+                res_searchers.clear()
+                ls = res_searchers[Yb.shape[0]] = LocalSearcher(grid, mlp, Yb.shape[0])
+            r = ls.run(Yb)
+            return r.best, r.score, r.scale
 
     K = grid.K
     nb = batch_size or min(V, 1 << 20)
@@ -995,7 +1062,9 @@ def fit_glm_arma11(data, design, tr, a_grid=None, b_grid=None, estimate_per_voxe
             b1 = min(V, b0 + nb)
             Yb = _to_device_f32(data[b0:b1], dev)
             Vb = b1 - b0
-            if fixed_best is None:
+            if searcher is not None:
+                best, score, scale = searcher(Yb)
+            elif fixed_best is None:
                 Yt, scale, _ = grid.prep(Yb, normalize=not use_double)
                 S = grid.surface(Yt, Vb)
                 del Yt

@@ -6,8 +6,9 @@ trained on the GPU by the project's own forward / backward / Adam kernels
 (libffsreml.so, include/ffs_mlp.h) on labels produced by the full-grid GPU
 search itself (the global argmin).  At inference the REML score is evaluated
 only on the 5 x 5 lattice window around the snapped proposal; voxels whose
-local optimum sits on a window edge (away from the lattice edge) or whose
-confidence is below ``conf_min`` are searched on the full grid.  Window scores
+local optimum sits on a window side that the lattice continues past, or whose
+confidence is below ``conf_min``, go on to the lambda band and then the full
+grid.  Window scores
 are bitwise equal to the full-grid surface, so an accepted local optimum is
 exactly the global argmin whenever the latter lies inside the window.
 """
@@ -136,11 +137,101 @@ class ProposalMLP:
     def state(self) -> np.ndarray:
         return self.params.detach().cpu().numpy().copy()
 
+    def save(self, path, meta: dict | None = None) -> None:
+        """Write the network as plain arrays (``numpy.savez``): the float32
+        parameters in the flat layout of ``init_params``, the confidence
+        threshold and a JSON metadata string stored as bytes, so loading needs
+        no pickle."""
+        import json
+
+        info = dict(meta or {})
+        info.setdefault("layout", "W1[10][64] b1[64] W2[64][64] b2[64] W3[64][3] b3[3], raw lag-1..10 residual ACF in")
+        blob = np.frombuffer(json.dumps(info, sort_keys=True).encode(), dtype=np.uint8)
+        with open(path, "wb") as fh:
+            np.savez(fh, params=self.state().astype(np.float32), conf_min=np.float64(self.conf_min), meta=blob)
+
+    @classmethod
+    def load(cls, path, device: torch.device | None = None) -> "ProposalMLP":
+        """Read a network written by ``save`` (no pickle is loaded)."""
+        import json
+
+        with np.load(path, allow_pickle=False) as z:
+            params = np.asarray(z["params"], dtype=np.float32)
+            conf_min = float(z["conf_min"])
+            meta = json.loads(bytes(np.asarray(z["meta"], dtype=np.uint8)).decode()) if "meta" in z else {}
+        net = cls(device, params=params)
+        net.conf_min = conf_min
+        net.meta = meta
+        return net
+
 
 def features(grid: RemlGrid, Y: torch.Tensor) -> torch.Tensor:
     """Lag-1..10 autocorrelation of the OLS residual of every voxel (device)."""
     _, _, feats = grid.prep_features(Y)
     return feats
+
+
+def synth_training_voxels(grid: RemlGrid, V: int, seed: int = 0, a_range=(0.1, 0.8), b_range=(-0.5, 0.3),
+                          chunk: int = 250_000) -> torch.Tensor:
+    """Training series for the proposal network on the grid's own timeline
+    (device, (V, T) float32): stationary unit-variance ARMA(1,1) noise with a,
+    b uniform in the given ranges, restarted at every run and sampled at the
+    retained times only (censored samples are dropped), plus N(0, 1)
+    coefficients on the grid's design."""
+    from .glm.arma import run_bounds
+    from .synth import arma11_noise_torch
+
+    dev = grid.device
+    g = torch.Generator(device=dev)
+    g.manual_seed(int(seed))
+    T = grid.T
+    tau = None if grid.tau is None else np.asarray(torch.as_tensor(grid.tau).flatten().tolist(), dtype=np.int64)
+    runs = []
+    for s, e in run_bounds(T, grid.run_starts):
+        t_run = np.arange(e - s) if tau is None else tau[s:e] - int(tau[s])
+        runs.append((s, e, torch.from_numpy(t_run).to(dev), int(t_run[-1]) + 1))
+    Xd = torch.from_numpy(grid.X64).to(dev, torch.float32)
+    Y = torch.empty(V, T, dtype=torch.float32, device=dev)
+    for c0 in range(0, V, chunk):
+        c1 = min(V, c0 + chunk)
+        n = c1 - c0
+        a = torch.rand(n, generator=g, device=dev) * (a_range[1] - a_range[0]) + a_range[0]
+        b = torch.rand(n, generator=g, device=dev) * (b_range[1] - b_range[0]) + b_range[0]
+        beta = torch.randn(n, grid.p, generator=g, device=dev)
+        Yc = beta @ Xd.T
+        for s, e, t_run, length in runs:
+            Yc[:, s:e] += arma11_noise_torch(g, a, b, length)[:, t_run]
+        Y[c0:c1] = Yc
+    return Y
+
+
+def train_proposal(grid: RemlGrid, n_train: int = 1_000_000, epochs: int = 10, n_val: int = 100_000,
+                   target: float = 0.9998, seed: int = 0, batch: int = 4096, lr: float = 3e-3,
+                   lr_final: float = 1e-4, log_every: int = 200) -> ProposalMLP:
+    """Train the proposal network for ``grid``'s design and timeline on
+    synthetic voxels (``synth_training_voxels``) labelled by the full-grid GPU
+    search (global argmin), then calibrate its confidence threshold on
+    ``n_val`` fresh voxels for ``target`` agreement with the full grid (with
+    the grid's current lambda bands).  ``train_log`` and ``meta`` on the
+    returned network record what was done."""
+    t0 = time.perf_counter()
+    Ytr = synth_training_voxels(grid, n_train, seed=seed)
+    labels, _ = full_grid_labels(grid, Ytr)
+    feats = features(grid, Ytr)
+    del Ytr
+    mlp = ProposalMLP(grid.device, seed=seed)
+    log = mlp.train(feats, labels, epochs=epochs, batch=batch, lr=lr, lr_final=lr_final, seed=seed,
+                    log_every=log_every)
+    del feats, labels
+    Yv = synth_training_voxels(grid, n_val, seed=seed + 7919)
+    _, best_v = full_grid_labels(grid, Yv)
+    calibrate_conf_min(mlp, grid, Yv, best_v, target=target)
+    torch.cuda.synchronize(grid.device)
+    mlp.train_log = log
+    mlp.meta = {"T": grid.T, "p": grid.p, "K": grid.K, "n_train": n_train, "epochs": epochs, "n_val": n_val,
+                "target": target, "seed": seed, "bands": list(grid.bands) if grid.bands else None,
+                "train_s": time.perf_counter() - t0, "train_steps": log.steps}
+    return mlp
 
 
 def full_grid_labels(grid: RemlGrid, Y: torch.Tensor, batch: int = 500_000):
@@ -320,15 +411,12 @@ class LocalSearcher:
 
 def window_load(grid: RemlGrid, props: torch.Tensor, conf_min: float) -> tuple[int, int]:
     """(voxels, total pair evaluations) of the first window pass for these
-    proposals: the library's window choice (nearest lattice point, window kept
-    inside the lattice; confidence >= conf_min and finite proposal), restated."""
-    lut = torch.as_tensor(np.asarray(grid.lut))
-    n_a, n_b = lut.shape
-    valid = (lut >= 0).to(torch.int64)
+    proposals: the library's window choice (nearest lattice cell, centre kept
+    two cells inside the lattice; confidence >= conf_min and finite proposal)
+    restated, with the window sizes read from the plan's window table."""
+    _, counts, _ = grid.window_info()
+    n_a, n_b = grid.lut.shape
     half = 2
-    pad = torch.nn.functional.pad(valid[None, None].double(), (half, half, half, half))
-    counts = torch.nn.functional.avg_pool2d(pad, 2 * half + 1, stride=1)[0, 0] * (2 * half + 1) ** 2
-    counts = counts.round().long().to(props.device)  # valid pairs of the window centred at (ai, bi)
     a_ax = torch.tensor(grid.a_axis, dtype=torch.float32, device=props.device)
     b_ax = torch.tensor(grid.b_axis, dtype=torch.float32, device=props.device)
     ok = torch.isfinite(props).all(dim=1) & (props[:, 2] >= conf_min)
@@ -338,7 +426,7 @@ def window_load(grid: RemlGrid, props: torch.Tensor, conf_min: float) -> tuple[i
         ai = ai.clamp(half, n_a - 1 - half)
     if n_b >= 2 * half + 1:
         bi = bi.clamp(half, n_b - 1 - half)
-    n = counts[ai, bi]
+    n = counts.to(props.device).long()[ai * n_b + bi]
     return int(ok.sum()), int(n[ok].sum())
 
 

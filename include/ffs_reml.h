@@ -48,7 +48,12 @@ const char* ffs_reml_last_error(void);
  *   gap         host, T int32: tau[t+1]-tau[t] inside a run, 0 at the last point of a run
  *   K, ab       host, K x 2 float64 (a, b) pairs in search order
  *   white       host, K int32: 1 where lambda == 0 (R is exactly the identity)
- *   n_a, n_b, lut   host lattice of the hierarchical selector: lut[ai*n_b+bi] = pair or -1
+ *   n_a, n_b, lut   host lattice of the hierarchical selector: lut[ai*n_b+bi] = pair or -1;
+ *               n_a = n_b = 0 (lut NULL) makes a pair-list plan for GLS / whitening only
+ *               (the -load_Rvar re-use path, glm/arma.py:4388-4447, which builds one
+ *               whitener per unique pair): no search operands, windows or selector
+ *               tables are built, so memory is linear in K, and the search entry
+ *               points return -4 on it
  *   zero_a, zero_b  lattice coordinates of (0, 0)
  *   ltop        top level of the hierarchical descent
  *   n_task, task_idx  host; F statistic over these columns (n_task = 0: all columns, F = b'Ab)
@@ -180,10 +185,11 @@ int ffs_reml_ljung_box(const double* resid_w, int64_t ldR, int64_t V, int32_t n,
  * searches the whole grid, glm/arma.py:1886-2322).  props (V x 3, device) =
  * (a_hat, b_hat, log-confidence) from the proposal network.  Each voxel is
  * scored on the 5 x 5 lattice window around its snapped proposal (kept inside
- * the lattice) with the same arithmetic as the full grid, so the scores equal
- * the full-grid surface entries bitwise (in every projection mode).  Outputs per
- * voxel: best pair (-1 if no valid pair), its score, need_full = 1 when the
- * optimum is on a window edge that is not a lattice edge, 2 when the confidence
+ * the lattice; the table is ffs_reml_plan_window_info) with the same
+ * arithmetic as the full grid, so the scores equal the full-grid surface
+ * entries bitwise (in every projection mode).  Outputs per voxel: best pair
+ * (-1 if no valid pair), its score, need_full = 1 when the optimum is on a
+ * window side that the lattice continues past, 2 when the confidence
  * is below conf_min or the proposal is not finite (no window is searched; best
  * and score are left as they were); scale = per-voxel std, or, when coef
  * (nullable) is given, the scale written by ffs_reml_prep_features with that
@@ -234,6 +240,12 @@ int ffs_reml_plan_set_bands(ffs_reml_plan* plan, double half_width, double bin_w
 int ffs_reml_plan_band_info(const ffs_reml_plan* plan, int32_t* n_bands, int32_t* slots, int32_t* pairs,
                             int32_t* counts, int32_t* edge);
 int ffs_reml_band_workspace(const ffs_reml_plan* plan, int64_t V, int64_t* bytes);
+/* Host copies (synchronous; each pointer nullable) of the window table of the
+ * learned search, one window per lattice cell c = ai * n_b + bi: window count,
+ * slots per window, pairs [n_windows][slots] (-1 = empty), counts
+ * [n_windows], edge flags [n_windows][slots]. */
+int ffs_reml_plan_window_info(const ffs_reml_plan* plan, int32_t* n_windows, int32_t* slots, int32_t* pairs,
+                              int32_t* counts, int32_t* edge);
 /* Stages of ffs_reml_band_search (same workspace and stream): band choice,
  * grouping and grouped residualisation; band energies; band projection, score
  * and band argmin. */

@@ -189,11 +189,15 @@ class Grid:
         return [(float(a), float(b)) for a, b in self.params.tolist()]
 
 
-def precompute_grid(X, a_grid, b_grid, run_starts=None, tau=None, dtype=torch.float64) -> Grid:
+def precompute_grid(X, a_grid, b_grid, run_starts=None, tau=None, dtype=torch.float64,
+                    autocorr: Grid | None = None) -> Grid:
     """glm/arma.py:1306-1531 ``precompute_reml_grid`` (Cholesky-QR branch).
 
     X_w = L^-1 X;  L_xtx = chol(X_w'X_w);  Q = X_w L_xtx^-T;
     logdet(X'R^-1X) = 2 sum log(|diag L_xtx| + 1e-10)  (glm/arma.py:1485-1506).
+    ``autocorr``: a grid of the same pairs, timeline and dtype whose whiteners
+    and log det R are reused (the reference's ``autocorr_cache``,
+    glm/arma.py:1306-1368, shared by the groups of a grouped fit).
     """
     X = torch.as_tensor(X).to(dtype)
     T, p = X.shape
@@ -201,11 +205,17 @@ def precompute_grid(X, a_grid, b_grid, run_starts=None, tau=None, dtype=torch.fl
     params = valid_pairs(a_grid.to(dtype) if dtype == torch.float64 else a_grid,
                          b_grid.to(dtype) if dtype == torch.float64 else b_grid)
     K = params.shape[0]
-    Linv = torch.empty(K, T, T, dtype=dtype)
-    ldR = torch.empty(K, dtype=dtype)
-    for k in range(K):
-        a, b = float(params[k, 0]), float(params[k, 1])
-        Linv[k], ldR[k] = whitener(a, b, T, run_starts, tau, dtype)
+    if autocorr is not None:
+        if (autocorr.dtype != dtype or autocorr.Linv.shape[1] != T
+                or not torch.equal(autocorr.params.to(params.dtype), params)):
+            raise ValueError("autocorr grid does not match these pairs, timeline length or dtype")
+        Linv, ldR = autocorr.Linv, autocorr.logdet_R
+    else:
+        Linv = torch.empty(K, T, T, dtype=dtype)
+        ldR = torch.empty(K, dtype=dtype)
+        for k in range(K):
+            a, b = float(params[k, 0]), float(params[k, 1])
+            Linv[k], ldR[k] = whitener(a, b, T, run_starts, tau, dtype)
     Xw = Linv @ X
     try:
         Lx = torch.linalg.cholesky(Xw.transpose(1, 2) @ Xw)
@@ -286,9 +296,13 @@ def hier_ltop(n_a: int, n_b: int, afni_faithful: bool = False) -> int:
     return max(0, min(n_a, n_b).bit_length() - 2)
 
 
-def select_hierarchical(surface: torch.Tensor, params: torch.Tensor, afni_faithful=False):
+def select_hierarchical(surface: torch.Tensor, params: torch.Tensor, afni_faithful=False, trace=False):
     """glm/arma.py:2224-2322 ``select_arma_params_hierarchical_from_surface``,
-    restated per voxel with plain loops.  Returns (pair index, score)."""
+    restated per voxel with plain loops.  Returns (pair index, score), and with
+    ``trace`` also, per voxel, the pair index held after each level (ltop
+    first).  A level whose candidates include a NaN takes NaN as its minimum
+    (``Tensor.min`` propagates NaN, glm/arma.py:2299, 2311), so it never
+    improves the running best."""
     a_sorted, b_sorted, ai, bi = _lattice(params)
     n_a, n_b = len(a_sorted), len(b_sorted)
     lut = -np.ones((n_a, n_b), dtype=np.int64)
@@ -307,9 +321,11 @@ def select_hierarchical(surface: torch.Tensor, params: torch.Tensor, afni_faithf
 
     out_k = np.zeros(V, dtype=np.int64)
     out_s = np.zeros(V, dtype=S.dtype)
+    levels = []
     for v in range(V):
         ba, bb = za, zb
         best = val(v, ba, bb)
+        path = []
         for lev in range(ltop, -1, -1):
             step = 1 << lev
             if lev == ltop:
@@ -318,18 +334,28 @@ def select_hierarchical(surface: torch.Tensor, params: torch.Tensor, afni_faithf
             else:
                 ac = [min(max(ba + o, 0), n_a - 1) for o in (-step, 0, step)]
                 bc = [min(max(bb + o, 0), n_b - 1) for o in (-step, 0, step)]
-            # candidates flattened b-outer / a-inner, first minimum wins
+            # candidates flattened b-outer / a-inner, first minimum wins; a NaN
+            # candidate makes the level's minimum NaN
             mn, sel = None, None
             for y in bc:
                 for x in ac:
                     s = val(v, x, y)
+                    if np.isnan(s):
+                        mn, sel = s, (x, y)
+                        break
                     if mn is None or s < mn:
                         mn, sel = s, (x, y)
+                if mn is not None and np.isnan(mn):
+                    break
             if mn < best:
                 best = mn
                 ba, bb = sel
+            path.append(int(lut[ba, bb]))
         out_k[v] = lut[ba, bb]
         out_s[v] = best
+        levels.append(path)
+    if trace:
+        return torch.from_numpy(out_k), torch.from_numpy(out_s), levels
     return torch.from_numpy(out_k), torch.from_numpy(out_s)
 
 

@@ -429,3 +429,36 @@ def test_global_estimation_matches_oracle(dev):
     assert rel(res.set_fstats[:, 1], g["tstats"][:, 3] ** 2) <= 1e-4
     with pytest.raises(ValueError):
         fit_glm_arma11(Y, X, 2.0, a_grid=ag, b_grid=bg, device=dev, estimate_per_voxel=False, task_indices=[0, 1])
+
+
+def test_pair_list_plan_continuous_params(dev):
+    """Per-voxel (a, b) that are all distinct (as loaded from a smoothed -Rvar):
+    the plan is a pair list whose memory is linear in the pairs, the GLS at
+    each voxel's own whitener matches the FP64 oracle, and the search entry
+    points refuse the plan."""
+    from fastfuncstuff_amd._lib import RemlLibraryError
+    from fastfuncstuff_amd.glm.arma import RemlGrid, fit_glm_arma11
+
+    print("[fit] pair-list plan, 5000 distinct pairs", flush=True)
+    rng = np.random.default_rng(77)
+    T, p, V = 120, 5, 5000
+    X = np.column_stack([np.ones(T), np.linspace(-1, 1, T), rng.standard_normal((T, p - 2))])
+    a = rng.uniform(0.1, 0.8, V)
+    b = rng.uniform(-0.5, 0.3, V)
+    Y = (rng.standard_normal((V, p)) @ X.T + rng.standard_normal((V, T))).astype(np.float32)
+    pp = torch.from_numpy(np.stack([a, b], 1))
+    res = fit_glm_arma11(Y, X, 2.0, precomputed_arma_params=pp, device=dev)
+    sel = list(range(0, V, 125))
+    Linv = torch.stack([O.whitener(float(a[v]), float(b[v]), T)[0] for v in sel])
+    g = O.Grid(pp[sel], Linv, None, None, None, None)
+    Yd = torch.from_numpy(Y).double()[sel]
+    ref = O.gls_refit(torch.from_numpy(X), Yd, g, torch.arange(len(sel)), dtype=F64)
+    s = Yd.std(dim=1)
+    denom = torch.maximum(ref[0].abs(), 1e-3 * s[:, None])
+    assert float(((res.betas[sel].double() - ref[0]).abs() / denom).max()) <= 1e-5
+    assert float(((res.tstats[sel].double() - ref[1]).abs() / ref[1].abs().clamp(min=1)).max()) <= 1e-5
+    assert float(((res.sigma2[sel].double() - ref[3]).abs() / ref[3]).max()) <= 1e-5
+    grid = RemlGrid(torch.from_numpy(X), None, None, dev, pairs=pp[:64], lattice=False)
+    Yt, _, _ = grid.prep(torch.from_numpy(Y[:64]).to(dev))
+    with pytest.raises(RemlLibraryError):
+        grid.surface(Yt, 64)

@@ -140,12 +140,13 @@ def test_window_scores_bitwise_equal_full_grid(dev, mode, use_coef):
     best, score = best.cpu().long(), score.cpu()
     lut = grid.lut
     na, nb = lut.shape
+    wp, wc, _ = grid.window_info()
     white = [k for k, (a, b) in enumerate(grid.param_list) if (np.float32(a) + np.float32(b)) == 0]
     for v in range(V):
         ca = min(max(ai[v], 2), na - 3)
         cb = min(max(bi[v], 2), nb - 3)
-        ks = [lut[ca - 2 + r, cb - 2 + c] for r in range(5) for c in range(5)]
-        ks = [k for k in ks if k >= 0]
+        c = ca * nb + cb
+        ks = wp[c, :int(wc[c])].tolist()
         if not ks:  # window entirely in the lambda < 0 region: no local optimum
             assert int(best[v]) == -1 and score[v].item() == float("inf")
             continue
@@ -156,6 +157,45 @@ def test_window_scores_bitwise_equal_full_grid(dev, mode, use_coef):
             k = white[0]
         assert int(best[v]) == k, v
         assert score[v].item() == S[v, ks[j]].item()  # bitwise
+
+
+def _window_restatement(grid):
+    """Windows of the learned search restated: the valid pairs of the 5 x 5
+    lattice cells around each cell, in pair order; edge = the slot is on a
+    window side that the lattice continues past."""
+    lut = np.asarray(grid.lut)
+    na, nb = lut.shape
+    out = []
+    for ca in range(na):
+        for cb in range(nb):
+            ks, edge = [], []
+            for x in range(ca - 2, ca + 3):
+                for y in range(cb - 2, cb + 3):
+                    if 0 <= x < na and 0 <= y < nb and lut[x, y] >= 0:
+                        ks.append(int(lut[x, y]))
+                        edge.append(int((x == ca - 2 and x > 0) or (x == ca + 2 and x < na - 1)
+                                        or (y == cb - 2 and y > 0) or (y == cb + 2 and y < nb - 1)))
+            out.append((ks, edge))
+    return out
+
+
+def test_window_tables_match_restatement(dev):
+    """The plan's window table equals the restated 5 x 5 windows; every
+    window holds at most 25 pairs, in pair order, and contains its centre
+    when the centre is a valid pair."""
+    grid, _, _ = _grid_and_data(dev, 64, seed=11)
+    wp, wc, we = grid.window_info()
+    ref = _window_restatement(grid)
+    lut = np.asarray(grid.lut)
+    nb = lut.shape[1]
+    assert wp.shape[0] == len(ref)
+    for c, (ks, edge) in enumerate(ref):
+        n = int(wc[c])
+        assert wp[c, :n].tolist() == ks, c
+        assert we[c, :n].tolist() == edge, c
+        assert n <= 25 and ks == sorted(ks)
+        if lut[c // nb, c % nb] >= 0:
+            assert int(lut[c // nb, c % nb]) in ks
 
 
 def test_trained_network_gains_over_untrained(dev):

@@ -508,3 +508,60 @@ def test_global_fit_oracle_against_ols_and_whitened_ols():
     # one task column: F = t^2 up to the 1e-8 regulariser
     g1 = O.global_fit(X, Y, 0.6, -0.2, run_starts=[0, 50], task_indices=[1])
     assert rel(g1["fstats"], g1["tstats"][:, 1] ** 2) < 1e-6
+
+
+def test_hierarchical_nan_candidate_blocks_the_level():
+    """glm/arma.py:2299, 2311 take each level's minimum with Tensor.min, which
+    returns NaN when any candidate is NaN, and `NaN < best` is False: a level
+    with a NaN candidate keeps the running best.  Lattice 5 x 5 (ltop 1:
+    coarse lattice {0, 2, 4} per axis, then the 3 x 3 window at step 1)."""
+    g = torch.tensor([-0.5, -0.25, 0.0, 0.25, 0.5])
+    params = torch.stack(torch.meshgrid(g, g, indexing="ij"), -1).reshape(-1, 2)
+    base = torch.full((1, 25), 10.0)
+    base[0, 12] = 5.0  # (0, 0) start
+    base[0, 0] = 1.0   # coarse lattice point (a=-0.5, b=-0.5): the coarse level's minimum
+    k, s = O.select_hierarchical(base, params)
+    assert int(k) == 0 and float(s) == 1.0
+    nan_late = base.clone()
+    nan_late[0, 24] = float("nan")  # last coarse candidate (b-outer / a-inner order)
+    k, s = O.select_hierarchical(nan_late, params)
+    # coarse level blocked; the step-1 window around (0, 0) holds no NaN and no
+    # value below 5, so the start stays
+    assert int(k) == 12 and float(s) == 5.0
+    nan_first = base.clone()
+    nan_first[0, 0] = float("nan")
+    nan_first[0, 2] = 2.0
+    k, s = O.select_hierarchical(nan_first, params)
+    assert int(k) == 12 and float(s) == 5.0
+    # a NaN outside every visited candidate set changes nothing
+    nan_far = base.clone()
+    nan_far[0, 1] = float("nan")  # (a=-0.5, b=-0.25): not on the coarse lattice, not near the picks
+    k, s = O.select_hierarchical(nan_far, params)
+    assert int(k) == 0 and float(s) == 1.0
+
+
+def test_ljung_box_censored_matches_zero_filled_autocovariance():
+    """Censored tau: the lag-pair binning equals the autocovariance of the
+    series placed at its true times on a zero-filled axis (runs 66666 apart),
+    with n_d = pairs of retained samples d apart; LB = n (n + 2) sum over
+    bins with n_d > 1 of c_d^2 / n_d, c_d = sum z_t z_{t+d} / sum r^2."""
+    rng = np.random.default_rng(31)
+    lens = [60, 50]
+    full = sum(lens)
+    good = np.sort(rng.choice(full, full - 14, replace=False))
+    run_starts, tau = O.build_censor_run_info([0, lens[0]], full, good.tolist())
+    n = len(good)
+    r = rng.standard_normal((3, n))
+    ltau = O.ljung_box_tau(n, run_starts, tau).numpy()
+    hh = O.resolve_ljung_box_lag(n, 9)
+    got = O.ljung_box(torch.from_numpy(r), 9, ltau).numpy()
+    span = int(ltau.max()) + 1
+    mask = np.zeros(span)
+    mask[ltau] = 1.0
+    nd = np.array([np.dot(mask[: span - d], mask[d:]) for d in range(hh + 1)])
+    for v in range(3):
+        z = np.zeros(span)
+        z[ltau] = r[v]
+        s0 = np.sum(r[v] ** 2)
+        q = sum((np.dot(z[: span - d], z[d:]) / s0) ** 2 / nd[d] for d in range(1, hh + 1) if nd[d] > 1)
+        assert abs(got[v] - n * (n + 2) * q) < 1e-9 * max(1.0, got[v])
