@@ -41,6 +41,7 @@ SIGNATURES: dict[str, list] = {
     "ffs_reml_project": [_vp, _vp, _i64, _i64, _vp, _i64, _vp],
     "ffs_reml_select": [_vp, _vp, _i64, _i64, _c_int, _vp, _vp, _vp],
     "ffs_reml_gls": [_vp, _vp, _i64, _vp, _vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _vp, _vp, _vp, _vp],
+    "ffs_reml_gls_noalloc": [_vp, _vp, _i64, _vp, _vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _vp, _vp, _vp],
     "ffs_reml_ljung_box": [_vp, _i64, _i64, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp],
     "ffs_reml_local_workspace": [_vp, _i64, _vp],
     "ffs_reml_local_recenter": [_vp, _vp, _i64, _i64, _vp, _vp, _vp, _i64, _vp],

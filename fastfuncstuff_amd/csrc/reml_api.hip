@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -724,6 +725,55 @@ extern "C" int ffs_reml_gls(const ffs_reml_plan* P, const float* Y, int64_t ldY,
         launch_gls<96>(P, Y, ldY, scale, perm, best, (int)V, betas, tstats, fstats, sigma2, r2, resid_w, ldR, resid, pred, s);
     HIP_TRY(hipGetLastError());
     return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free GLS: the voxel grouping lives in a file-scope device array
+// ---------------------------------------------------------------------------
+namespace {
+// int32 capacity of the grouping scratch of ffs_reml_gls_noalloc (8 MiB per GPU).
+constexpr int64_t kNoAllocInts = FFS_REML_GLS_NOALLOC_MAX_INT32;
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+int32_t* g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+// The array's name is a host-side stand-in; kernels get the device address
+// from hipGetSymbolAddress (one copy per GPU, each at its own address).
+static __device__ __align__(256) int32_t scratch[kNoAllocInts];
+
+static int scratch_for_device(int dev, int32_t** out) {
+    if (dev < 0 || dev >= kMaxDevices) return fail(-3, "gls_noalloc: device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* ptr = nullptr;
+        hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return fail(-1, std::string("gls_noalloc: hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_ptr[dev] = static_cast<int32_t*>(ptr);
+    }
+    *out = g_scratch_ptr[dev];
+    return 0;
+}
+
+extern "C" int ffs_reml_gls_noalloc(const ffs_reml_plan* P, const float* Y, int64_t ldY, const float* scale,
+                                    const int32_t* best, int64_t V, float* betas, float* tstats, float* fstats,
+                                    float* sigma2, float* r2, double* resid_w, int64_t ldR, float* resid, float* pred,
+                                    void* stream) {
+    if (!P) return fail(-3, "gls_noalloc: null plan");
+    if (V <= 0) return 0;
+    const int64_t need = gls_work_ints(V, P->d.K);
+    if (need > kNoAllocInts)
+        return fail(-3, "gls_noalloc: V = " + std::to_string(V) + " with K = " + std::to_string(P->d.K) +
+                            " needs " + std::to_string(need) + " int32 of grouping scratch; the maximum is " +
+                            std::to_string(kNoAllocInts) + " (split the batch or use ffs_reml_gls)");
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    if (dev != P->device) return fail(-3, "gls_noalloc: the plan belongs to another device");
+    int32_t* work = nullptr;
+    if (int rc = scratch_for_device(dev, &work)) return rc;
+    return ffs_reml_gls(P, Y, ldY, scale, best, V, betas, tstats, fstats, sigma2, r2, resid_w, ldR, resid, pred, work,
+                        stream);
 }
 
 extern "C" int ffs_reml_ljung_box(const double* resid_w, int64_t ldR, int64_t V, int32_t n, int32_t nbins,

@@ -172,6 +172,23 @@ int ffs_reml_gls(const ffs_reml_plan* plan, const float* Y, int64_t ldY, const f
                  int64_t ldR, float* resid, float* pred, int32_t* work, void* stream);
 
 /*
+ * Allocation-free variant of ffs_reml_gls.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments and results as ffs_reml_gls without `work`: the voxel grouping is
+ * kept in a library-owned device array of FFS_REML_GLS_NOALLOC_MAX_INT32 int32
+ * (8 MiB, one copy per GPU, held until the process ends), so nothing is
+ * allocated or freed.  A call needs 3 K + 1 + 64 ceil((V + 63 min(V, K)) / 64)
+ * int32 of it (K = plan pairs); larger V returns -3 with the limit in the
+ * message (about 2.0 M voxels at K = 561).  Every call on one GPU shares the
+ * array: calls that overlap on one GPU (different streams or threads) race.
+ */
+#define FFS_REML_GLS_NOALLOC_MAX_INT32 (1 << 21)
+int ffs_reml_gls_noalloc(const ffs_reml_plan* plan, const float* Y, int64_t ldY, const float* scale,
+                         const int32_t* best, int64_t V, float* betas, float* tstats, float* fstats, float* sigma2,
+                         float* r2, double* resid_w, int64_t ldR, float* resid, float* pred, void* stream);
+
+/*
  * Ljung-Box of whitened residuals (V x ldR float64).  Lag pairs are pre-binned
  * by time lag: pairs[2q], pairs[2q+1] for q in [bin_off[d], bin_off[d+1]) have
  * lag d (1 <= d <= nbins); wbin[d] = 1/n_d for usable bins, else 0.  All three
