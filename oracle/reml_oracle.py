@@ -405,6 +405,61 @@ def search_hierarchical_subset(surface: torch.Tensor, params: torch.Tensor, afni
     return torch.from_numpy(out_k), torch.from_numpy(out_s)
 
 
+def search_subset_gemm(X, Y, grid: Grid, afni_faithful=False):
+    """glm/arma.py:2021-2221 ``search_voxels_precomputed_grid_hierarchical``
+    (the reference's CPU default) with its own work: per level, the lattice
+    points that are multiples of the step, each scored (dense whitening GEMM,
+    float64 sums) only for the voxels whose window (fixed at the level's
+    start) holds it, with the sequential strict '<' update in b-outer / a-inner
+    order, after the (0, 0) evaluation of every voxel.  Returns (pair index,
+    score).  Same rule as ``search_hierarchical_subset`` on a full surface."""
+    T, p = X.shape
+    Yr = residualize(X, Y)
+    a_sorted, b_sorted, ai_of, bi_of = _lattice(grid.params)
+    n_a, n_b = len(a_sorted), len(b_sorted)
+    lut = -np.ones((n_a, n_b), dtype=np.int64)
+    for k, (x, y) in enumerate(zip(ai_of, bi_of)):
+        lut[x, y] = k
+    za = next(i for i, v in enumerate(a_sorted) if abs(v) < 1e-6)
+    zb = next(i for i, v in enumerate(b_sorted) if abs(v) < 1e-6)
+    V = Y.shape[0]
+    acc = torch.float64
+    ba = torch.full((V,), za, dtype=torch.long)
+    bb = torch.full((V,), zb, dtype=torch.long)
+    best = torch.full((V,), float("inf"), dtype=Y.dtype)
+
+    def evaluate(x, y, idx):
+        k = int(lut[x, y])
+        if k < 0 or idx.numel() == 0:
+            return
+        Yw = Yr.index_select(0, idx) @ grid.Linv[k].to(Y.dtype).T
+        QtY = Yw @ grid.Q[k].to(Y.dtype)
+        rss = Yw.pow(2).sum(1, dtype=acc) - QtY.pow(2).sum(1, dtype=acc)
+        lik = (grid.logdet_R[k].to(acc) + grid.logdet_XRX[k].to(acc) + (T - p) * torch.log(rss + 1e-10)).to(Y.dtype)
+        up = lik < best.index_select(0, idx)
+        sel = idx[up]
+        best[sel] = lik[up]
+        ba[sel] = x
+        bb[sel] = y
+
+    evaluate(za, zb, torch.arange(V))
+    for lev in range(hier_ltop(n_a, n_b, afni_faithful), -1, -1):
+        step = 1 << lev
+        if lev == hier_ltop(n_a, n_b, afni_faithful):
+            alo = torch.zeros(V, dtype=torch.long)
+            ahi = torch.full((V,), n_a - 1, dtype=torch.long)
+            blo = torch.zeros(V, dtype=torch.long)
+            bhi = torch.full((V,), n_b - 1, dtype=torch.long)
+        else:
+            alo, ahi = (ba - step).clamp(0, n_a - 1), (ba + step).clamp(0, n_a - 1)
+            blo, bhi = (bb - step).clamp(0, n_b - 1), (bb + step).clamp(0, n_b - 1)
+        for y in range(0, n_b, step):
+            in_b = (blo <= y) & (y <= bhi)
+            for x in range(0, n_a, step):
+                evaluate(x, y, (in_b & (alo <= x) & (x <= ahi)).nonzero().flatten())
+    return torch.from_numpy(lut[ba.numpy(), bb.numpy()]), best
+
+
 # ---------------------------------------------------------------------------
 # GLS refit and statistics
 # ---------------------------------------------------------------------------

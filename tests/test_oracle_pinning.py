@@ -565,3 +565,26 @@ def test_ljung_box_censored_matches_zero_filled_autocovariance():
         s0 = np.sum(r[v] ** 2)
         q = sum((np.dot(z[: span - d], z[d:]) / s0) ** 2 / nd[d] for d in range(1, hh + 1) if nd[d] > 1)
         assert abs(got[v] - n * (n + 2) * q) < 1e-9 * max(1.0, got[v])
+
+
+def test_subset_gemm_descent_equals_subset_rule_on_surface():
+    """The reference's CPU default computed with its own subset GEMMs picks,
+    for every voxel, the pair that the subset rule picks on the full surface
+    (same scores up to GEMM blocking: ties within 1e-4 accepted)."""
+    rng = np.random.default_rng(12)
+    T, V = 90, 300
+    X = torch.from_numpy(np.column_stack([np.ones(T), np.linspace(-1, 1, T), rng.standard_normal(T)])).float()
+    e = np.zeros((V, T))
+    a = rng.uniform(0.1, 0.8, V)
+    for t in range(T):
+        e[:, t] = (a * e[:, t - 1] if t else 0) + rng.standard_normal(V)
+    Y = torch.from_numpy(e).float()
+    grid = O.precompute_grid(X, torch.linspace(-0.8, 0.8, 17), torch.linspace(-0.8, 0.8, 17), dtype=torch.float32)
+    S = O.score_surface(X, Y, grid)
+    k_s, s_s = O.search_hierarchical_subset(S, grid.params)
+    k_g, s_g = O.search_subset_gemm(X, Y, grid)
+    diff = k_s != k_g
+    assert int(diff.sum()) <= 2
+    ar = torch.arange(V)
+    assert bool(((S[ar, k_s] - S[ar, k_g]).abs()[diff] <= 1e-4).all())
+    assert bool(((s_s - s_g).abs() <= 1e-4 * s_s.abs().clamp(min=1)).all())
