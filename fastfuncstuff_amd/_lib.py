@@ -40,6 +40,8 @@ SIGNATURES: dict[str, list] = {
     "ffs_reml_energy": [_vp, _vp, _i64, _i64, _vp, _i64, _vp],
     "ffs_reml_project": [_vp, _vp, _i64, _i64, _vp, _i64, _vp],
     "ffs_reml_select": [_vp, _vp, _i64, _i64, _c_int, _vp, _vp, _vp],
+    "ffs_reml_surface_f64": [_vp, _vp, _c_int, _i64, _i64, _vp, _vp, _i64, _vp],
+    "ffs_reml_select_f64": [_vp, _vp, _i64, _i64, _c_int, _vp, _vp, _vp],
     "ffs_reml_gls": [_vp, _vp, _i64, _vp, _vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _vp, _vp, _vp, _vp],
     "ffs_reml_gls_noalloc": [_vp, _vp, _i64, _vp, _vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _vp, _vp, _vp],
     "ffs_reml_ljung_box": [_vp, _i64, _i64, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp],

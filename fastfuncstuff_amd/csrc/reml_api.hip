@@ -23,6 +23,7 @@
 #include "reml_sets.hip"
 #include "reml_dense.hip"
 #include "reml_dsort.hip"
+#include "reml_f64.hip"
 
 using namespace remlk;
 
@@ -616,10 +617,10 @@ extern "C" int ffs_reml_select(const ffs_reml_plan* P, const float* S, int64_t l
     if (V <= 0) return 0;
     hipStream_t s = (hipStream_t)stream;
     if (mode == 0)
-        hipLaunchKernelGGL(k_select_first, dim3((unsigned)((V + 3) / 4)), dim3(256), 0, s, P->d, S, ldS, (int)V,
+        hipLaunchKernelGGL(k_select_first<float>, dim3((unsigned)((V + 3) / 4)), dim3(256), 0, s, P->d, S, ldS, (int)V,
                            nullptr, best, best_score);
     else
-        hipLaunchKernelGGL(k_select, dim3((unsigned)((V + 255) / 256)), dim3(256), 0, s, P->d, S, ldS, (int)V, mode,
+        hipLaunchKernelGGL(k_select<float>, dim3((unsigned)((V + 255) / 256)), dim3(256), 0, s, P->d, S, ldS, (int)V, mode,
                            best, best_score);
     HIP_TRY(hipGetLastError());
     return 0;
@@ -774,6 +775,61 @@ extern "C" int ffs_reml_gls_noalloc(const ffs_reml_plan* P, const float* Y, int6
     if (int rc = scratch_for_device(dev, &work)) return rc;
     return ffs_reml_gls(P, Y, ldY, scale, best, V, betas, tstats, fstats, sigma2, r2, resid_w, ldR, resid, pred, work,
                         stream);
+}
+
+// ---------------------------------------------------------------------------
+// FP64 search (the reference's use_double search)
+// ---------------------------------------------------------------------------
+template <typename TY>
+static void launch_surface_f64(const ffs_reml_plan* P, const TY* Y, int64_t ldY, const float* scale, int V, double* S,
+                               int64_t ldS, hipStream_t s) {
+    const dim3 g((unsigned)((int64_t)((V + 63) / 64) * P->d.K)), b(64);
+    const int p = P->d.p;
+    if (p <= 8)
+        hipLaunchKernelGGL((k_surface_f64<8, TY>), g, b, 0, s, P->d, Y, ldY, scale, V, S, ldS);
+    else if (p <= 16)
+        hipLaunchKernelGGL((k_surface_f64<16, TY>), g, b, 0, s, P->d, Y, ldY, scale, V, S, ldS);
+    else if (p <= 32)
+        hipLaunchKernelGGL((k_surface_f64<32, TY>), g, b, 0, s, P->d, Y, ldY, scale, V, S, ldS);
+    else
+        hipLaunchKernelGGL((k_surface_f64<96, TY>), g, b, 0, s, P->d, Y, ldY, scale, V, S, ldS);
+}
+
+extern "C" int ffs_reml_surface_f64(const ffs_reml_plan* P, const void* Y, int32_t y_is_f64, int64_t ldY, int64_t V,
+                                    const float* scale, double* S, int64_t ldS, void* stream) {
+    if (!P) return fail(-3, "surface_f64: null plan");
+    FFS_NEED_SEARCH(P, "surface_f64");
+    if (V <= 0) return 0;
+    if (!Y || !S) return fail(-3, "surface_f64: null argument");
+    if (ldY < P->d.T) return fail(-3, "surface_f64: ldY must be >= T");
+    if (ldS < P->d.K) return fail(-3, "surface_f64: ldS must be >= K");
+    if ((int64_t)((V + 63) / 64) * P->d.K > 0x7fffffffLL) return fail(-3, "surface_f64: batch too large; split it");
+    hipStream_t s = (hipStream_t)stream;
+    if (y_is_f64)
+        launch_surface_f64(P, static_cast<const double*>(Y), ldY, scale, (int)V, S, ldS, s);
+    else
+        launch_surface_f64(P, static_cast<const float*>(Y), ldY, scale, (int)V, S, ldS, s);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+extern "C" int ffs_reml_select_f64(const ffs_reml_plan* P, const double* S, int64_t ldS, int64_t V, int32_t mode,
+                                   int32_t* best, double* best_score, void* stream) {
+    if (!P) return fail(-3, "select_f64: null plan");
+    FFS_NEED_SEARCH(P, "select_f64");
+    if (mode < 0 || mode > 2)
+        return fail(-3, "select_f64: mode must be 0 (exhaustive), 1 (hierarchical) or 2 (subset descent)");
+    if (V <= 0) return 0;
+    if (ldS < P->d.K) return fail(-3, "select_f64: ldS must be >= K");
+    hipStream_t s = (hipStream_t)stream;
+    if (mode == 0)
+        hipLaunchKernelGGL(k_select_first<double>, dim3((unsigned)((V + 3) / 4)), dim3(256), 0, s, P->d, S, ldS,
+                           (int)V, nullptr, best, best_score);
+    else
+        hipLaunchKernelGGL(k_select<double>, dim3((unsigned)((V + 255) / 256)), dim3(256), 0, s, P->d, S, ldS,
+                           (int)V, mode, best, best_score);
+    HIP_TRY(hipGetLastError());
+    return 0;
 }
 
 extern "C" int ffs_reml_ljung_box(const double* resid_w, int64_t ldR, int64_t V, int32_t n, int32_t nbins,
@@ -1247,7 +1303,7 @@ extern "C" int ffs_reml_subset_search(const ffs_reml_plan* P, const float* Y, in
         return rc;
     if (int rc = energy_impl(P, Yt, L.cap_pad, cap, S, K, n_live, s)) return rc;
     if (int rc = project_impl(P, Yt, L.cap_pad, cap, S, K, n_live, split, s)) return rc;
-    hipLaunchKernelGGL(k_select_first, dim3((unsigned)((cap + 3) / 4)), dim3(256), 0, s, P->d, S, (int64_t)K,
+    hipLaunchKernelGGL(k_select_first<float>, dim3((unsigned)((cap + 3) / 4)), dim3(256), 0, s, P->d, S, (int64_t)K,
                        (int)cap, n_live, bsub, ssub);
     HIP_TRY(hipGetLastError());
     hipLaunchKernelGGL(k_scatter_best, dim3((unsigned)((cap + 255) / 256)), dim3(256), 0, s, idx, (int)cap, n_live,

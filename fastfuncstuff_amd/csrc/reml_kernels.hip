@@ -992,9 +992,10 @@ __global__ void __launch_bounds__(256) k_project16(PlanDev P, const float* __res
 //           candidates b-outer / a-inner, first minimum; a NaN candidate
 //           makes the level's minimum NaN (torch.min), which never improves.
 // ---------------------------------------------------------------------------
-__device__ inline float lat_val(const PlanDev& P, const float* row, int ai, int bi) {
+template <typename ST>
+__device__ inline ST lat_val(const PlanDev& P, const ST* row, int ai, int bi) {
     const int k = P.lut[ai * P.n_b + bi];
-    return k >= 0 ? row[k] : INFINITY;
+    return k >= 0 ? row[k] : (ST)INFINITY;
 }
 
 // (iii) mode 0, one wave per voxel: lane l takes pairs l, l + 64, ... with a
@@ -1003,18 +1004,19 @@ __device__ inline float lat_val(const PlanDev& P, const float* row, int ai, int 
 // pairs, as the sequential scan (glm/arma.py:2008-2018).  Pairs whose score
 // is never below +inf (NaN, inf) are never chosen; a voxel with none takes
 // the (0, 0) pair.  n_live (nullable, device) bounds the voxels processed.
-__global__ void __launch_bounds__(256) k_select_first(PlanDev P, const float* __restrict__ S, int64_t ldS, int V,
+template <typename ST>
+__global__ void __launch_bounds__(256) k_select_first(PlanDev P, const ST* __restrict__ S, int64_t ldS, int V,
                                                       const int32_t* __restrict__ n_live, int32_t* __restrict__ best,
-                                                      float* __restrict__ best_score) {
+                                                      ST* __restrict__ best_score) {
     const int lane = threadIdx.x & 63;
     const int v = blockIdx.x * 4 + (threadIdx.x >> 6);
     const int Vl = n_live ? min(*n_live, V) : V;
     if (v >= Vl) return;
-    const float* row = S + (size_t)v * ldS;
-    float bs = INFINITY;
+    const ST* row = S + (size_t)v * ldS;
+    ST bs = (ST)INFINITY;
     int bk = -1;
     for (int k = lane; k < P.K; k += 64) {
-        const float s = row[k];
+        const ST s = row[k];
         if (s < bs) {
             bs = s;
             bk = k;
@@ -1022,7 +1024,7 @@ __global__ void __launch_bounds__(256) k_select_first(PlanDev P, const float* __
     }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
-        const float os = __shfl_xor(bs, o);
+        const ST os = __shfl_xor(bs, o);
         const int ok = __shfl_xor(bk, o);
         if (ok >= 0 && (os < bs || (os == bs && (bk < 0 || ok < bk)))) {
             bs = os;
@@ -1035,17 +1037,18 @@ __global__ void __launch_bounds__(256) k_select_first(PlanDev P, const float* __
     }
 }
 
-__global__ void __launch_bounds__(256) k_select(PlanDev P, const float* __restrict__ S, int64_t ldS, int V, int mode,
-                                                int32_t* __restrict__ best, float* __restrict__ best_score) {
+template <typename ST>
+__global__ void __launch_bounds__(256) k_select(PlanDev P, const ST* __restrict__ S, int64_t ldS, int V, int mode,
+                                                int32_t* __restrict__ best, ST* __restrict__ best_score) {
     const int v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= V) return;
-    const float* row = S + (size_t)v * ldS;
+    const ST* row = S + (size_t)v * ldS;
     const int k00 = P.lut[P.zero_a * P.n_b + P.zero_b];
     if (mode == 0) {
-        float bs = INFINITY;
+        ST bs = (ST)INFINITY;
         int bk = -1;
         for (int k = 0; k < P.K; ++k) {
-            const float s = row[k];
+            const ST s = row[k];
             if (s < bs) {
                 bs = s;
                 bk = k;
@@ -1062,9 +1065,9 @@ __global__ void __launch_bounds__(256) k_select(PlanDev P, const float* __restri
         // / a-inner, each compared with a sequential strict '<' (the best point
         // moves during the sweep); invalid pairs are skipped.
         int ba = P.zero_a, bb = P.zero_b;
-        float bst = INFINITY;
+        ST bst = (ST)INFINITY;
         {
-            const float s0 = lat_val(P, row, ba, bb);
+            const ST s0 = lat_val(P, row, ba, bb);
             if (s0 < bst) bst = s0;
         }
         for (int lev = P.ltop; lev >= 0; --lev) {
@@ -1080,7 +1083,7 @@ __global__ void __launch_bounds__(256) k_select(PlanDev P, const float* __restri
                 for (int x = (alo + step - 1) / step * step; x <= ahi; x += step) {
                     const int k = P.lut[x * P.n_b + y];
                     if (k < 0) continue;
-                    const float sv = row[k];
+                    const ST sv = row[k];
                     if (sv < bst) {
                         bst = sv;
                         ba = x;
@@ -1093,16 +1096,16 @@ __global__ void __launch_bounds__(256) k_select(PlanDev P, const float* __restri
         return;
     }
     int ba = P.zero_a, bb = P.zero_b;
-    float cur = lat_val(P, row, ba, bb);
+    ST cur = lat_val(P, row, ba, bb);
     for (int lev = P.ltop; lev >= 0; --lev) {
         const int step = 1 << lev;
-        float mn = 0.f;
+        ST mn = 0;
         int sa = 0, sb = 0;
         bool first = true, nan = false;
         if (lev == P.ltop) {
             for (int y = 0; y < P.n_b; y += step)
                 for (int x = 0; x < P.n_a; x += step) {
-                    const float s = lat_val(P, row, x, y);
+                    const ST s = lat_val(P, row, x, y);
                     if (s != s) nan = true;
                     if (first || s < mn) {
                         mn = s;
@@ -1116,7 +1119,7 @@ __global__ void __launch_bounds__(256) k_select(PlanDev P, const float* __restri
                 const int y = min(max(bb + oy * step, 0), P.n_b - 1);
                 for (int ox = -1; ox <= 1; ++ox) {
                     const int x = min(max(ba + ox * step, 0), P.n_a - 1);
-                    const float s = lat_val(P, row, x, y);
+                    const ST s = lat_val(P, row, x, y);
                     if (s != s) nan = true;
                     if (first || s < mn) {
                         mn = s;

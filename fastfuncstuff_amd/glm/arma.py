@@ -401,6 +401,32 @@ class RemlGrid(Mapping):
                                                 int(n_chunks), _lib.stream_handle(self.device)), "surface")
         return S
 
+    def surface_f64(self, Y: torch.Tensor, scale: torch.Tensor | None = None,
+                    out: torch.Tensor | None = None) -> torch.Tensor:
+        """(V, K) float64 REML surface of Y (V, T; float32 or float64 on the
+        device) divided by ``scale`` (optional): float64 recurrence energy and
+        projection, no residualisation (the reference's float64 search)."""
+        if Y.dtype not in (torch.float32, torch.float64) or Y.ndim != 2 or Y.stride(1) != 1:
+            raise ValueError("surface_f64 needs a row-contiguous (V, T) float32 or float64 tensor")
+        V = Y.shape[0]
+        S = out if out is not None else torch.empty(V, self.K, dtype=torch.float64, device=self.device)
+        _lib.check(_lib.load().ffs_reml_surface_f64(self._handle, _lib.ptr(Y), int(Y.dtype == torch.float64),
+                                                    Y.stride(0), V, _lib.ptr(scale), _lib.ptr(S), S.stride(0),
+                                                    _lib.stream_handle(self.device)), "surface_f64")
+        return S
+
+    def select_f64(self, S: torch.Tensor, selector: str):
+        """``select`` on a float64 surface: (pair index int32, score float64)."""
+        if selector not in self.SELECTORS:
+            raise ValueError(f"selector must be one of {sorted(self.SELECTORS)}, got {selector!r}")
+        V = S.shape[0]
+        best = torch.empty(V, dtype=torch.int32, device=self.device)
+        score = torch.empty(V, dtype=torch.float64, device=self.device)
+        _lib.check(_lib.load().ffs_reml_select_f64(self._handle, _lib.ptr(S), S.stride(0), V,
+                                                   self.SELECTORS[selector], _lib.ptr(best), _lib.ptr(score),
+                                                   _lib.stream_handle(self.device)), "select_f64")
+        return best, score
+
     PROJECTIONS = {"fp32": 0, "bf16x3": 1, "fp16x3": 2}
 
     def set_projection(self, mode: str):
@@ -861,7 +887,10 @@ def fit_glm_arma11(data, design, tr, a_grid=None, b_grid=None, estimate_per_voxe
     global argmin with ``force_exhaustive_search``), then an FP64 GLS refit at
     the selected pair.  float32 mode divides each voxel by its std before the
     fit and unscales betas / sigma2 / residuals afterwards (glm/arma.py:3947-3954,
-    5827-5862); ``use_double`` skips the normalisation.  Outputs are CPU tensors.
+    5827-5862); ``use_double`` runs the reference's float64 search instead: raw
+    series (no normalisation, no residualisation), float64 surface
+    (``RemlGrid.surface_f64``) and selection, float64 outputs; the GLS reads
+    the float32 samples (exact for float32 data).  Outputs are CPU tensors.
     ``projection`` ("fp16x3" default, "fp32", "bf16x3") is the arithmetic of
     the surface GEMM (an engine option; the reference computes it in fp32;
     split fp16 keeps ~22-bit operands with fp32 accumulation and meets the
@@ -985,7 +1014,8 @@ def fit_glm_arma11(data, design, tr, a_grid=None, b_grid=None, estimate_per_voxe
         res.residuals_whitened = torch.zeros(V, T, dtype=out_dtype)
     if want_predicted:
         res.predicted = torch.zeros(V, T, dtype=out_dtype)
-    surface_acc = torch.empty(V, K, dtype=f) if save_profile_likelihoods and fixed_best is None else None
+    surface_acc = (torch.empty(V, K, dtype=out_dtype) if save_profile_likelihoods and fixed_best is None
+                   else None)
     lj_lag = lj_tau = None
     if want_ljung_box:
         starts = sorted({int(s) for s in (run_starts.tolist() if isinstance(run_starts, torch.Tensor)
@@ -1064,6 +1094,17 @@ def fit_glm_arma11(data, design, tr, a_grid=None, b_grid=None, estimate_per_voxe
             Vb = b1 - b0
             if searcher is not None:
                 best, score, scale = searcher(Yb)
+            elif fixed_best is None and use_double:
+                # the reference's float64 search: raw series, float64 surface
+                # and selection (glm/arma.py:3869)
+                src = data[b0:b1]
+                Yd = (src.to(dev, torch.float64) if src.dtype == torch.float64 else Yb).contiguous()
+                S = grid.surface_f64(Yd)
+                best, score = grid.select_f64(S, "exhaustive" if force_exhaustive_search else (selector or "hierarchical"))
+                scale = None
+                if surface_acc is not None:
+                    surface_acc[b0:b1].copy_(S)
+                del S, Yd
             elif fixed_best is None:
                 Yt, scale, _ = grid.prep(Yb, normalize=not use_double)
                 S = grid.surface(Yt, Vb)

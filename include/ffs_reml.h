@@ -154,6 +154,22 @@ int ffs_reml_select(const ffs_reml_plan* plan, const float* S, int64_t ldS, int6
                     float* best_score, void* stream);
 
 /*
+ * FP64 search: the reference's float64 search (use_double, glm/arma.py:3869;
+ * no residualisation, float64 whitening, projection and sums,
+ * glm/arma.py:1886-2018).  S[v*ldS + k] = logdet R_k + logdet X'R_k^-1X +
+ * (T-p) log(||L_k^-1 y||^2 - ||G_k' y||^2 + 1e-10) in float64 for y = row v of
+ * Y (V x T, row stride ldY; float64 when y_is_f64, else float32 samples read
+ * exactly into float64) divided by scale[v] (nullable = 1).  The energy runs
+ * the innovations recurrence of pair k in float64.  Lattice plans only (-4 on
+ * a pair-list plan); on a ridge plan the score is that plan's ridge score.
+ */
+int ffs_reml_surface_f64(const ffs_reml_plan* plan, const void* Y, int32_t y_is_f64, int64_t ldY, int64_t V,
+                         const float* scale, double* S, int64_t ldS, void* stream);
+/* ffs_reml_select on a float64 surface (same modes and tie rules), float64 scores out. */
+int ffs_reml_select_f64(const ffs_reml_plan* plan, const double* S, int64_t ldS, int64_t V, int32_t mode,
+                        int32_t* best, double* best_score, void* stream);
+
+/*
  * (iv) FP64 GLS refit of Y (V x T float32, original units) at pair best[v].
  * scale (nullable) = per-voxel std used to normalise before the fit.
  * Outputs: betas, tstats (V x p), fstats, sigma2, r2 (V).  Optional (nullable,
