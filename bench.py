@@ -169,35 +169,70 @@ def learned_eval(args):
 
 def grouped(args):
     """Batched uneven designs: V voxels in 20 HRF groups (Dirichlet(0.5) sizes),
-    single-trial design of 64 events + Legendre 0-3 (p=68), T=400, 33x33 grid;
-    one step = fit_glm_arma11_grouped over all voxels (one plan per group, full
-    grid, hierarchical selection, GLS).  Data start on the host (the grouped
-    API takes host arrays), so the step includes each group's upload and plan."""
+    single-trial design of 64 events + Legendre 0-3 (p=68), T=400, 33x33 grid.
+    One step = one grouped fit on the grouped engine (prep, energy, projection,
+    hierarchical selection, FP64 GLS over all groups, one launch per stage,
+    replayed from one HIP graph) with the data resident in HBM; the plans (the
+    tables of all 20 designs in one launch) and the autotuned launch
+    configuration are built first and timed apart.  The drop-in call
+    fit_glm_arma11_grouped from host data to CPU results is timed beside it,
+    and so is the per-group loop it replaces."""
     from fastfuncstuff_amd import _lib, synth
-    from fastfuncstuff_amd.glm.arma import fit_glm_arma11_grouped
+    from fastfuncstuff_amd.glm.arma import RemlGrid, RemlGroup, fit_glm_arma11_grouped
 
     dev = _lib.require_device(None)
     t0 = time.time()
     Y, designs, groups, ag, bg = synth.grouped_config(V=args.voxels, T=args.T)
     gen_s = time.time() - t0
     print(f"[bench] grouped workload generated in {gen_s:.1f} s", file=sys.stderr, flush=True)
+    ids = sorted(designs)
+    agt, bgt = torch.from_numpy(ag), torch.from_numpy(bg)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    grids = RemlGrid.create_many([torch.from_numpy(designs[h]) for h in ids], agt, bgt, dev)
+    for g in grids:
+        g.set_projection(args.projection)
+    torch.cuda.synchronize()
+    plan_s = time.perf_counter() - t0
+    engine = RemlGroup(grids, np.searchsorted(ids, groups).astype(np.int32))
+    Yd = torch.from_numpy(Y).to(dev)
+    t0 = time.perf_counter()
+    tuning = engine.tune(Yd, "hierarchical", reps=3)
+    tune_s = time.perf_counter() - t0
+    stream = torch.cuda.current_stream(dev)
 
     def step(tm):
-        return fit_glm_arma11_grouped(Y, designs, groups, 2.0, a_grid=torch.from_numpy(ag),
-                                      b_grid=torch.from_numpy(bg), device=dev)
+        return engine.fit(Yd, "hierarchical")
 
     ms, _, _, res = timed(step, args.steps, args.warmup, dev)
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ev0.record(stream)
+    for _ in range(args.steps):
+        engine.fit(Yd, "hierarchical")
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    dev_ms = ev0.elapsed_time(ev1) / args.steps
+    api = {}
+    for name, kw in (("engine", {}), ("per_group_loop", {"grouped_engine": False})):
+        t0 = time.perf_counter()
+        fit_glm_arma11_grouped(Y, designs, groups, 2.0, a_grid=agt, b_grid=bgt, device=dev, **kw)
+        torch.cuda.synchronize()
+        api[name] = {"ms": (time.perf_counter() - t0) * 1e3}
+        api[name]["voxels_per_s"] = args.voxels / (api[name]["ms"] / 1e3)
     sizes = np.bincount(groups, minlength=len(designs))
     line = {
         "metric": "REML voxels/s, batched uneven designs (20 HRF groups, p=68, T=400, V=200k)",
         "value": args.voxels / (ms / 1e3), "unit": "voxels/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "fp32", "data": "synthetic",
+        "warmup": args.warmup, "ms_per_step": ms, "device_ms_per_fit": dev_ms, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": args.projection, "data": "synthetic",
         "config": {"workload": f"grouped20_dirichlet0.5_T{args.T}_p68_grid33x33", "V": args.voxels, "T": args.T,
                    "p": 68, "groups": len(designs), "group_sizes_min_max": [int(sizes.min()), int(sizes.max())],
-                   "path": "one plan per group, full grid, hierarchical", "projection": "fp32"},
-        "host_resident_input": True, "datagen_s": gen_s,
+                   "path": "grouped engine: one launch per stage over all groups, HIP graph replay, hierarchical",
+                   "projection": args.projection},
+        "plan_build_s": plan_s, "autotune_s": tune_s, "autotune": tuning,
+        "api_end_to_end": api, "datagen_s": gen_s,
     }
+    engine.close()
     print(json.dumps(line), flush=True)
 
 

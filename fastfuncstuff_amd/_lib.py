@@ -28,6 +28,12 @@ SIGNATURES: dict[str, list] = {
     "ffs_reml_plan_create_ridge": [_pp, _c_int, _c_int, _vp, _vp, _vp, _c_int, _vp, _vp, _c_int, _c_int, _vp, _c_int,
                                    _c_int, _c_int, ctypes.c_double, _vp],
     "ffs_reml_plan_destroy": [_vp],
+    "ffs_reml_plan_create_many": [_vp, _c_int, _c_int, _c_int, _vp, _vp, _vp, _c_int, _vp, _vp, _c_int, _c_int, _vp,
+                                  _c_int, _c_int, _c_int, _c_int, _vp, _vp],
+    "ffs_reml_group_create": [_pp, _c_int, _vp, _i64, _vp, _vp],
+    "ffs_reml_group_destroy": [_vp],
+    "ffs_reml_group_fit": [_vp, _vp, _i64, _c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _vp],
+    "ffs_reml_group_tune": [_vp, _vp, _i64, _c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _c_int, _vp, _i64, _vp],
     "ffs_reml_dense_likelihood": [_vp, _vp, _c_int, _vp, _c_int, _c_int, _vp, _i64, _vp, _vp],
     "ffs_reml_plan_logdets": [_vp, _vp, _vp],
     "ffs_reml_plan_ainv": [_vp, _vp],

@@ -24,6 +24,8 @@
 #include "reml_dense.hip"
 #include "reml_dsort.hip"
 #include "reml_f64.hip"
+#include "reml_group.hip"
+#include "reml_gls_wide.hip"
 
 using namespace remlk;
 
@@ -142,12 +144,24 @@ extern "C" int ffs_reml_plan_create_ridge(ffs_reml_plan** out, int32_t T, int32_
                             ridge, stream);
 }
 
-static int plan_create_impl(ffs_reml_plan** out, int32_t T, int32_t p, const double* X, const double* pinvT,
-                            const int32_t* gap, int32_t K, const double* ab, const int32_t* white, int32_t n_a,
-                            int32_t n_b, const int32_t* lut, int32_t zero_a, int32_t zero_b, int32_t ltop,
-                            int32_t n_task, const int32_t* task_idx, double ridge, void* stream) {
-    if (!out) return fail(-3, "plan_create: null output handle");
-    *out = nullptr;
+// Work left after the allocation phase of one plan (see plan_alloc).
+struct PlanPending {
+    ffs_reml_plan* P = nullptr;
+    bool gls_only = false;
+    double *soa = nullptr, *ldR = nullptr, *ldX = nullptr, *dpinv = nullptr;
+    float *ecoef = nullptr, *ec = nullptr, *es0 = nullptr, *lc = nullptr, *ls0 = nullptr;
+    float2* lp = nullptr;
+    int32_t* dchunk = nullptr;
+    double ridge = 0.0;
+};
+
+// Validation, host tables, device allocations and uploads of one plan; fills
+// `to` with the k_tables outputs of the plan and `pend` with what the finish
+// step needs.  Launches nothing that reads the tables.
+static int plan_alloc(ffs_reml_plan* P, int32_t T, int32_t p, const double* X, const double* pinvT, const int32_t* gap,
+                      int32_t K, const double* ab, const int32_t* white, int32_t n_a, int32_t n_b, const int32_t* lut,
+                      int32_t zero_a, int32_t zero_b, int32_t ltop, int32_t n_task, const int32_t* task_idx,
+                      double ridge, hipStream_t s, TablesOut* to, KTablesArgs* ka, PlanPending* pend) {
     if (T < 2 || p < 1 || K < 1) return fail(-3, "plan_create: need T >= 2, p >= 1, K >= 1");
     if (p > 96) return fail(-3, "plan_create: p > 96 is not supported");
     if (gap[T - 1] != 0) return fail(-3, "plan_create: gap[T-1] must be 0 (end of the last run)");
@@ -157,9 +171,6 @@ static int plan_create_impl(ffs_reml_plan** out, int32_t T, int32_t p, const dou
     if (gls_only) n_a = n_b = 0;
     for (int t = 0; t < T; ++t)
         if (gap[t] < 0) return fail(-3, "plan_create: negative time gap");
-    hipStream_t s = (hipStream_t)stream;
-    auto* P = new ffs_reml_plan();
-    std::unique_ptr<ffs_reml_plan> guard(P);
     HIP_TRY(hipGetDevice(&P->device));
     PlanDev& d = P->d;
     P->gls_only = gls_only;
@@ -213,7 +224,7 @@ static int plan_create_impl(ffs_reml_plan** out, int32_t T, int32_t p, const dou
     d.n_white = (int)whites.size();
 
     int rc = 0;
-    int32_t *dgap, *dwhiteflag, *dtask, *dwhite, *dchunk, *dlut, *dstatus;
+    int32_t *dgap, *dwhiteflag, *dtask, *dwhite, *dchunk, *dlut;
     double *dab, *dX, *dpinv;
     if ((rc = upload(P, &dgap, gap, T, s)) || (rc = upload(P, &dwhiteflag, white, K, s)) ||
         (rc = upload(P, &dtask, tidx.data(), tidx.size(), s)) ||
@@ -222,11 +233,8 @@ static int plan_create_impl(ffs_reml_plan** out, int32_t T, int32_t p, const dou
         (rc = upload(P, &dlut, lut, (size_t)n_a * n_b, s)) || (rc = upload(P, &dab, ab, (size_t)2 * K, s)) ||
         (rc = upload(P, &dX, X, (size_t)T * p, s)) || (rc = upload(P, &dpinv, pinvT, (size_t)T * p, s)))
         return rc;
-    if ((rc = P->alloc(&dstatus, 1))) return rc;
-    HIP_TRY(hipMemsetAsync(dstatus, 0, sizeof(int32_t), s));
     double *soa, *coef64, *Xw64, *G64, *Lx, *Ainv, *Fm, *work, *ld, *ldR, *ldX;
     float *G32, *ecoef, *ec, *es0;
-    const size_t nt = (size_t)d.n_task;
     if ((rc = P->alloc(&soa, (size_t)K * 3 * T)) || (rc = P->alloc(&coef64, (size_t)K * 3 * T)) ||
         (rc = P->alloc(&Xw64, (size_t)K * T * p)) || (rc = P->alloc(&G64, (size_t)K * T * p)) ||
         (rc = P->alloc(&Lx, (size_t)K * p * p)) || (rc = P->alloc(&Ainv, (size_t)K * p * p)) ||
@@ -274,9 +282,12 @@ static int plan_create_impl(ffs_reml_plan** out, int32_t T, int32_t p, const dou
     }
     // f64 MFMA GLS tables
     double *G64p = nullptr, *coefR = nullptr, *dXp = nullptr, *dXtX = nullptr, *dPp = nullptr;
-    d.PG = p <= kGlsMfmaMaxP ? 16 * ((p + 15) / 16) : 0;
-    if (d.PG) {
-        const int PG = d.PG;
+    // f64 MFMA tables for every width (the GLS runs on them for p <= 96);
+    // the MFMA prep and split operands keep p <= kGlsMfmaMaxP (d.PG)
+    d.PGA = 16 * ((p + 15) / 16);
+    d.PG = p <= kGlsMfmaMaxP ? d.PGA : 0;
+    {
+        const int PG = d.PGA;
         std::vector<double> Xp((size_t)d.Tpad * PG, 0.0), Pp((size_t)d.Tpad * PG, 0.0), XtX((size_t)p * p, 0.0);
         for (int t = 0; t < T; ++t)
             for (int j = 0; j < p; ++j) {
@@ -297,27 +308,14 @@ static int plan_create_impl(ffs_reml_plan** out, int32_t T, int32_t p, const dou
     // recurrence tables for every p: K * Tpad * 32 bytes
     if ((rc = P->alloc(&coefR, (size_t)K * d.Tpad * 4))) return rc;
     HIP_TRY(hipMemsetAsync(coefR, 0, sizeof(double) * K * d.Tpad * 4, s));
-    hipLaunchKernelGGL(k_tables, dim3(K), dim3(64), 0, s, T, d.Tpad, p, K, dab, dwhiteflag, dgap, dX, d.n_task, dtask,
-                       full_f, BN, npt, soa, coef64, Xw64, G64, Lx, Ainv, Fm, work, ld, ldR, ldX, G32, Ghf, Glf, Ghb, Glb, g16s,
-                       d.PG, G64p, coefR, ridge, dstatus);
-    HIP_TRY(hipGetLastError());
-    if (!gls_only) {
-        hipLaunchKernelGGL(k_local_tables, dim3(K), dim3(128), 0, s, T, K, soa, lp, lc, ls0);
-        HIP_TRY(hipGetLastError());
-        hipLaunchKernelGGL(k_energy_tables, dim3(d.n_chunks), dim3(128), 0, s, T, d.n_chunks, dchunk, soa, ecoef,
-                           ec, es0);
-        HIP_TRY(hipGetLastError());
-    }
-    int32_t status = 0;
-    HIP_TRY(hipMemcpyAsync(&status, dstatus, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipStreamSynchronize(s));
-    if (status == 1) return fail(-2, "plan_create: X'R^-1X is not positive definite (rank-deficient design)");
-    if (status == 2) return fail(-2, "plan_create: task block of (X'R^-1X)^-1 is singular");
+    *to = TablesOut{dX, soa, coef64, Xw64, G64, Lx, Ainv, Fm, work, ld, ldR, ldX, G32, Ghf, Glf, Ghb, Glb, g16s, G64p, coefR};
+    *ka = KTablesArgs{T, d.Tpad, p, K, dab, dwhiteflag, dgap, d.n_task, dtask, full_f, BN, npt, d.PGA, ridge};
     d.gap = dgap;
     d.ab = dab;
     d.ld = ld;
     d.coef64 = coef64;
     d.X = dX;
+    d.pinvT = dpinv;
     d.Xw64 = Xw64;
     d.G64 = G64;
     d.Lx = Lx;
@@ -348,13 +346,112 @@ static int plan_create_impl(ffs_reml_plan** out, int32_t T, int32_t p, const dou
     d.XtX = dXtX;
     d.pinvTp = dPp;
     d.proj_mode = 0;
-    if (!gls_only)
-        if (int rc2 = build_windows(P, s)) return rc2;
-    P->pinvT = dpinv;
-    P->ridge = ridge;
-    P->ldR = ldR;
-    P->ldX = ldX;
+    pend->P = P;
+    pend->gls_only = gls_only;
+    pend->soa = soa;
+    pend->ldR = ldR;
+    pend->ldX = ldX;
+    pend->dpinv = dpinv;
+    pend->ecoef = ecoef;
+    pend->ec = ec;
+    pend->es0 = es0;
+    pend->lp = lp;
+    pend->lc = lc;
+    pend->ls0 = ls0;
+    pend->dchunk = dchunk;
+    pend->ridge = ridge;
+    return 0;
+}
+
+// The per-pair tables of G plans of one timeline and pair list (designs may
+// differ) in one k_tables launch over (pair, plan): a single plan's K one-wave
+// blocks leave most of the chip idle.  Then each plan's search tables, one
+// synchronisation, the status of every plan, the windows.
+static int plans_build(std::vector<PlanPending>& pend, const std::vector<TablesOut>& touts, const KTablesArgs& ka,
+                       hipStream_t s) {
+    const int G = (int)pend.size();
+    ffs_reml_plan* P0 = pend[0].P;
+    TablesOut* douts;
+    int32_t* dstatus;
+    if (int rc = P0->alloc(&douts, G)) return rc;
+    if (int rc = P0->alloc(&dstatus, G)) return rc;
+    HIP_TRY(hipMemcpyAsync(douts, touts.data(), sizeof(TablesOut) * G, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemsetAsync(dstatus, 0, sizeof(int32_t) * G, s));
+    hipLaunchKernelGGL(k_tables, dim3(ka.K, G), dim3(64), 0, s, ka, douts, dstatus);
+    HIP_TRY(hipGetLastError());
+    for (PlanPending& q : pend) {
+        if (q.gls_only) continue;
+        const PlanDev& d = q.P->d;
+        hipLaunchKernelGGL(k_local_tables, dim3(d.K), dim3(128), 0, s, d.T, d.K, q.soa, q.lp, q.lc, q.ls0);
+        HIP_TRY(hipGetLastError());
+        hipLaunchKernelGGL(k_energy_tables, dim3(d.n_chunks), dim3(128), 0, s, d.T, d.n_chunks, q.dchunk, q.soa,
+                           q.ecoef, q.ec, q.es0);
+        HIP_TRY(hipGetLastError());
+    }
+    std::vector<int32_t> status(G, 0);
+    HIP_TRY(hipMemcpyAsync(status.data(), dstatus, sizeof(int32_t) * G, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    for (int g = 0; g < G; ++g) {
+        const std::string who = G > 1 ? "plan_create_many (design " + std::to_string(g) + "): " : "plan_create: ";
+        if (status[g] == 1) return fail(-2, who + "X'R^-1X is not positive definite (rank-deficient design)");
+        if (status[g] == 2) return fail(-2, who + "task block of (X'R^-1X)^-1 is singular");
+    }
+    for (PlanPending& q : pend) {
+        if (!q.gls_only)
+            if (int rc = build_windows(q.P, s)) return rc;
+        q.P->pinvT = q.dpinv;
+        q.P->ridge = q.ridge;
+        q.P->ldR = q.ldR;
+        q.P->ldX = q.ldX;
+    }
+    return 0;
+}
+
+static int plan_create_impl(ffs_reml_plan** out, int32_t T, int32_t p, const double* X, const double* pinvT,
+                            const int32_t* gap, int32_t K, const double* ab, const int32_t* white, int32_t n_a,
+                            int32_t n_b, const int32_t* lut, int32_t zero_a, int32_t zero_b, int32_t ltop,
+                            int32_t n_task, const int32_t* task_idx, double ridge, void* stream) {
+    if (!out) return fail(-3, "plan_create: null output handle");
+    *out = nullptr;
+    hipStream_t s = (hipStream_t)stream;
+    auto* P = new ffs_reml_plan();
+    std::unique_ptr<ffs_reml_plan> guard(P);
+    std::vector<PlanPending> pend(1);
+    std::vector<TablesOut> touts(1);
+    KTablesArgs ka{};
+    if (int rc = plan_alloc(P, T, p, X, pinvT, gap, K, ab, white, n_a, n_b, lut, zero_a, zero_b, ltop, n_task,
+                            task_idx, ridge, s, &touts[0], &ka, &pend[0]))
+        return rc;
+    if (int rc = plans_build(pend, touts, ka, s)) return rc;
     *out = guard.release();
+    return 0;
+}
+
+extern "C" int ffs_reml_plan_create_many(ffs_reml_plan** out, int32_t G, int32_t T, int32_t p, const double* X,
+                                         const double* pinvT, const int32_t* gap, int32_t K, const double* ab,
+                                         const int32_t* white, int32_t n_a, int32_t n_b, const int32_t* lut,
+                                         int32_t zero_a, int32_t zero_b, int32_t ltop, int32_t n_task,
+                                         const int32_t* task_idx, void* stream) {
+    if (!out) return fail(-3, "plan_create_many: null output array");
+    if (G < 1) return fail(-3, "plan_create_many: need G >= 1");
+    if (!X || !pinvT) return fail(-3, "plan_create_many: null design");
+    for (int g = 0; g < G; ++g) out[g] = nullptr;
+    hipStream_t s = (hipStream_t)stream;
+    std::vector<std::unique_ptr<ffs_reml_plan>> owned;
+    std::vector<PlanPending> pend(G);
+    std::vector<TablesOut> touts(G);
+    KTablesArgs ka{};
+    for (int g = 0; g < G; ++g) {
+        owned.emplace_back(new ffs_reml_plan());
+        KTablesArgs kg{};
+        if (int rc = plan_alloc(owned.back().get(), T, p, X + (size_t)g * T * p, pinvT + (size_t)g * T * p, gap, K,
+                                ab, white, n_a, n_b, lut, zero_a, zero_b, ltop, n_task, task_idx, 0.0, s, &touts[g],
+                                &kg, &pend[g]))
+            return rc;
+        if (g == 0) ka = kg;
+    }
+    if (int rc = plans_build(pend, touts, ka, s)) return rc;
+    for (int g = 0; g < G; ++g) out[g] = owned[g].release();
     return 0;
 }
 
@@ -396,23 +493,28 @@ extern "C" int ffs_reml_plan_dims(const ffs_reml_plan* P, int32_t* Tpad, int32_t
 
 static int64_t vpad_of(int64_t V) { return (V + kVoxPad - 1) / kVoxPad * kVoxPad; }
 
+// Grouped layouts (ffs_reml_group_*): `plans` (device, one PlanDev per design)
+// and `grp` (the design of each 64-column block, or of each projection voxel
+// tile, or of each voxel for the GLS) select each block's tables; null for a
+// single-design call.
 template <int PM>
 static void launch_prep(const ffs_reml_plan* P, const float* Y, int64_t ldY, int V, const int32_t* rows, int ncols,
                         int ncols_pad, float* Yt, int64_t ldYt, float* scale, int normalize, float* feats,
-                        hipStream_t s) {
+                        hipStream_t s, const PlanDev* plans, const int32_t* grp) {
     const dim3 grid((ncols_pad + 63) / 64), block(64);
     if (feats)
         hipLaunchKernelGGL((k_prep<PM, 10>), grid, block, 0, s, P->d.T, P->d.p, Y, ldY, V, rows, ncols, ncols_pad,
-                           P->pinvT, P->d.X, normalize, Yt, ldYt, scale, feats);
+                           P->pinvT, P->d.X, normalize, Yt, ldYt, scale, feats, plans, grp);
     else
         hipLaunchKernelGGL((k_prep<PM, 0>), grid, block, 0, s, P->d.T, P->d.p, Y, ldY, V, rows, ncols, ncols_pad,
-                           P->pinvT, P->d.X, normalize, Yt, ldYt, scale, feats);
+                           P->pinvT, P->d.X, normalize, Yt, ldYt, scale, feats, plans, grp);
 }
 
 static int prep_impl(const ffs_reml_plan* P, const float* Y, int64_t ldY, int64_t V, const int32_t* rows,
                      int64_t ncols, float* Yt, int64_t ldYt, float* scale, int normalize, float* feats,
                      hipStream_t s, const int32_t* ncols_live = nullptr, bool write_split = true,
-                     bool record_split = true, const double* coef_in = nullptr, double* coef_out = nullptr) {
+                     bool record_split = true, const double* coef_in = nullptr, double* coef_out = nullptr,
+                     const PlanDev* plans = nullptr, const int32_t* grp = nullptr) {
     const int64_t ncp = vpad_of(ncols);
     if (Yt && ldYt < ncp) return fail(-3, "prep: ldYt must be >= the padded column count");
     if (ldY < P->d.T) return fail(-3, "prep: ldY must be >= T");
@@ -434,7 +536,7 @@ static int prep_impl(const ffs_reml_plan* P, const float* Y, int64_t ldY, int64_
         if (coef_in && (feats || !scale)) return fail(-3, "prep: given coefficients need the scales and no features");
 #define FFS_PREP(NT_, NF_, CI_)                                                                                     \
     hipLaunchKernelGGL((k_prep_mfma<NT_, NF_, CI_>), grid, block, 0, s, P->d, Y, ldY, v, rows, nc, ncpi, normalize, \
-                       Yt, ldYt, scale, feats, ncols_live, Ysh, Ysl, sbf, coef_in, coef_out)
+                       Yt, ldYt, scale, feats, ncols_live, Ysh, Ysl, sbf, coef_in, coef_out, plans, grp)
         if (P->d.PG == 16) {
             if (feats) FFS_PREP(1, 10, false);
             else if (coef_in) FFS_PREP(1, 0, true);
@@ -450,13 +552,13 @@ static int prep_impl(const ffs_reml_plan* P, const float* Y, int64_t ldY, int64_
     }
     if (coef_in || coef_out) return fail(-4, "prep: residual coefficients are kept only for p <= 32");
     if (p <= 8)
-        launch_prep<8>(P, Y, ldY, v, rows, nc, ncpi, Yt, ldYt, scale, normalize, feats, s);
+        launch_prep<8>(P, Y, ldY, v, rows, nc, ncpi, Yt, ldYt, scale, normalize, feats, s, plans, grp);
     else if (p <= 16)
-        launch_prep<16>(P, Y, ldY, v, rows, nc, ncpi, Yt, ldYt, scale, normalize, feats, s);
+        launch_prep<16>(P, Y, ldY, v, rows, nc, ncpi, Yt, ldYt, scale, normalize, feats, s, plans, grp);
     else if (p <= 32)
-        launch_prep<32>(P, Y, ldY, v, rows, nc, ncpi, Yt, ldYt, scale, normalize, feats, s);
+        launch_prep<32>(P, Y, ldY, v, rows, nc, ncpi, Yt, ldYt, scale, normalize, feats, s, plans, grp);
     else
-        launch_prep<96>(P, Y, ldY, v, rows, nc, ncpi, Yt, ldYt, scale, normalize, feats, s);
+        launch_prep<96>(P, Y, ldY, v, rows, nc, ncpi, Yt, ldYt, scale, normalize, feats, s, plans, grp);
     HIP_TRY(hipGetLastError());
     // rows T..Tpad-1 of the time-major buffer are zero (GEMM k padding)
     if (Yt && P->d.Tpad > P->d.T) {
@@ -517,7 +619,8 @@ extern "C" int ffs_reml_energy(const ffs_reml_plan* P, const float* Yt, int64_t 
 }
 
 static int project_impl(const ffs_reml_plan* P, const float* Yt, int64_t ldYt, int64_t V, float* S, int64_t ldS,
-                        const int32_t* n_live, bool presplit, hipStream_t s) {
+                        const int32_t* n_live, bool presplit, hipStream_t s, const PlanDev* plans = nullptr,
+                        const int32_t* grp = nullptr) {
     const int64_t Vpad = vpad_of(V);
     if (ldYt < Vpad) return fail(-3, "project: ldYt must be >= the padded voxel count");
     if (ldS < P->d.K) return fail(-3, "project: ldS must be >= K");
@@ -533,26 +636,26 @@ static int project_impl(const ffs_reml_plan* P, const float* Yt, int64_t ldYt, i
     const int v = (int)V;
     if (P->d.BN == 64) {
         if (mode == 1 && presplit)
-            hipLaunchKernelGGL((k_project16b<64, true>), g16, b, 0, s, P->d, Yt, ldYt, v, S, ldS, n_live);
+            hipLaunchKernelGGL((k_project16b<64, true>), g16, b, 0, s, P->d, Yt, ldYt, v, S, ldS, n_live, plans, grp);
         else if (mode == 2 && presplit)
-            hipLaunchKernelGGL((k_project16b<64, false>), g16, b, 0, s, P->d, Yt, ldYt, v, S, ldS, n_live);
+            hipLaunchKernelGGL((k_project16b<64, false>), g16, b, 0, s, P->d, Yt, ldYt, v, S, ldS, n_live, plans, grp);
         else if (mode == 1)
-            hipLaunchKernelGGL((k_project16<64, true>), g, b, 0, s, P->d, Yt, ldYt, v, S, ldS, n_live);
+            hipLaunchKernelGGL((k_project16<64, true>), g, b, 0, s, P->d, Yt, ldYt, v, S, ldS, n_live, plans, grp);
         else if (mode == 2)
-            hipLaunchKernelGGL((k_project16<64, false>), g, b, 0, s, P->d, Yt, ldYt, v, S, ldS, n_live);
+            hipLaunchKernelGGL((k_project16<64, false>), g, b, 0, s, P->d, Yt, ldYt, v, S, ldS, n_live, plans, grp);
         else
-            hipLaunchKernelGGL(k_project<64>, g, b, 0, s, P->d, Yt, ldYt, v, S, ldS, n_live);
+            hipLaunchKernelGGL(k_project<64>, g, b, 0, s, P->d, Yt, ldYt, v, S, ldS, n_live, plans, grp);
     } else {
         if (mode == 1 && presplit)
-            hipLaunchKernelGGL((k_project16b<96, true>), g16, b, 0, s, P->d, Yt, ldYt, v, S, ldS, n_live);
+            hipLaunchKernelGGL((k_project16b<96, true>), g16, b, 0, s, P->d, Yt, ldYt, v, S, ldS, n_live, plans, grp);
         else if (mode == 2 && presplit)
-            hipLaunchKernelGGL((k_project16b<96, false>), g16, b, 0, s, P->d, Yt, ldYt, v, S, ldS, n_live);
+            hipLaunchKernelGGL((k_project16b<96, false>), g16, b, 0, s, P->d, Yt, ldYt, v, S, ldS, n_live, plans, grp);
         else if (mode == 1)
-            hipLaunchKernelGGL((k_project16<96, true>), g, b, 0, s, P->d, Yt, ldYt, v, S, ldS, n_live);
+            hipLaunchKernelGGL((k_project16<96, true>), g, b, 0, s, P->d, Yt, ldYt, v, S, ldS, n_live, plans, grp);
         else if (mode == 2)
-            hipLaunchKernelGGL((k_project16<96, false>), g, b, 0, s, P->d, Yt, ldYt, v, S, ldS, n_live);
+            hipLaunchKernelGGL((k_project16<96, false>), g, b, 0, s, P->d, Yt, ldYt, v, S, ldS, n_live, plans, grp);
         else
-            hipLaunchKernelGGL(k_project<96>, g, b, 0, s, P->d, Yt, ldYt, v, S, ldS, n_live);
+            hipLaunchKernelGGL(k_project<96>, g, b, 0, s, P->d, Yt, ldYt, v, S, ldS, n_live, plans, grp);
     }
     HIP_TRY(hipGetLastError());
     return 0;
@@ -629,9 +732,10 @@ extern "C" int ffs_reml_select(const ffs_reml_plan* P, const float* S, int64_t l
 template <int PM>
 static void launch_gls(const ffs_reml_plan* P, const float* Y, int64_t ldY, const float* scale, const int32_t* perm,
                        const int32_t* best, int V, float* betas, float* tstats, float* fstats, float* sigma2,
-                       float* r2, double* resid_w, int64_t ldR, float* resid, float* pred, hipStream_t s) {
+                       float* r2, double* resid_w, int64_t ldR, float* resid, float* pred, hipStream_t s,
+                       const PlanDev* plans = nullptr, const int32_t* vgroup = nullptr) {
     hipLaunchKernelGGL(k_gls<PM>, dim3((V + 63) / 64), dim3(64), 0, s, P->d, Y, ldY, scale,
-                       perm, best, V, betas, tstats, fstats, sigma2, r2, resid_w, ldR, resid, pred);
+                       perm, best, V, betas, tstats, fstats, sigma2, r2, resid_w, ldR, resid, pred, plans, vgroup);
 }
 
 // Voxels grouped by key (see k_bucket_count): cnt[nb], off[nb + 1], cursor[nb],
@@ -659,6 +763,35 @@ static int group_by_key(const int32_t* keys, int64_t V, int nb, int pad, int32_t
         }
     } else {
         hipLaunchKernelGGL(k_bucket_scan, dim3(1), dim3(1024), 0, s, cnt, nb, pad, off, cursor);
+    }
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+// Wide f64 MFMA GLS (32 < p <= 96) over `cap` slots of 64-slot (pair) groups.
+static int launch_gls_wide(const PlanDev& d, const float* Y, int64_t ldY, const float* scale, const int32_t* perm,
+                           const int32_t* best, int64_t cap, float* betas, float* tstats, float* fstats, float* sigma2,
+                           float* r2, const PlanDev* plans, const int32_t* vgroup, hipStream_t s) {
+    const dim3 g((unsigned)(cap / 64)), b(256);
+    switch (d.PGA) {
+        case 48:
+            hipLaunchKernelGGL(k_gls_wide<3>, g, b, 0, s, d, Y, ldY, scale, perm, best, betas, tstats, fstats, sigma2,
+                               r2, plans, vgroup);
+            break;
+        case 64:
+            hipLaunchKernelGGL(k_gls_wide<4>, g, b, 0, s, d, Y, ldY, scale, perm, best, betas, tstats, fstats, sigma2,
+                               r2, plans, vgroup);
+            break;
+        case 80:
+            hipLaunchKernelGGL(k_gls_wide<5>, g, b, 0, s, d, Y, ldY, scale, perm, best, betas, tstats, fstats, sigma2,
+                               r2, plans, vgroup);
+            break;
+        case 96:
+            hipLaunchKernelGGL(k_gls_wide<6>, g, b, 0, s, d, Y, ldY, scale, perm, best, betas, tstats, fstats, sigma2,
+                               r2, plans, vgroup);
+            break;
+        default:
+            return fail(-3, "gls: no wide GLS kernel for this design width");
     }
     HIP_TRY(hipGetLastError());
     return 0;
@@ -700,11 +833,24 @@ extern "C" int ffs_reml_gls(const ffs_reml_plan* P, const float* Y, int64_t ldY,
         const dim3 g((unsigned)(cap / 64)), b(64);
         if (P->d.PG == 16)
             hipLaunchKernelGGL(k_gls_mfma<1>, g, b, 0, s, P->d, Y, ldY, scale, perm, best, betas, tstats, fstats,
-                               sigma2, r2);
+                               sigma2, r2, nullptr, nullptr);
         else
             hipLaunchKernelGGL(k_gls_mfma<2>, g, b, 0, s, P->d, Y, ldY, scale, perm, best, betas, tstats, fstats,
-                               sigma2, r2);
+                               sigma2, r2, nullptr, nullptr);
         HIP_TRY(hipGetLastError());
+        return 0;
+    }
+    if (work && !series_out) {
+        // 32 < p <= 96: wide f64 MFMA GLS, four waves per 64-slot pair group
+        int32_t* cnt = work;
+        int32_t* off = cnt + K;
+        int32_t* cursor = off + K + 1;
+        int32_t* perm = cursor + K;
+        const int64_t cap = gls_perm_cap(V, K);
+        if (int rc = group_by_key(best, V, K, 64, cnt, off, cursor, perm, cap, s)) return rc;
+        if (int rc = launch_gls_wide(P->d, Y, ldY, scale, perm, best, cap, betas, tstats, fstats, sigma2, r2, nullptr,
+                                     nullptr, s))
+            return rc;
         return 0;
     }
     int32_t* perm = nullptr;
@@ -1345,3 +1491,342 @@ extern "C" int ffs_reml_dense_likelihood(const double* X, const double* Y, int32
     return 0;
 }
 
+
+// ---------------------------------------------------------------------------
+// Grouped fits: one launch per stage over the voxels of every group
+// ---------------------------------------------------------------------------
+struct ffs_reml_group {
+    int device = 0;
+    int G = 0, K = 0, T = 0, p = 0;
+    int64_t V = 0, ncols = 0;  // voxels; padded columns (each group padded to kVoxPad)
+    std::vector<ffs_reml_plan*> plans;  // borrowed
+    PlanDev* dplans = nullptr;
+    int32_t *rows = nullptr, *blk_grp = nullptr, *tile_grp = nullptr, *vgroup = nullptr;
+    float *Yt = nullptr, *S = nullptr, *score_col = nullptr;
+    int32_t *best_col = nullptr, *key = nullptr, *work = nullptr;
+    int64_t yt_rows = 0, work_ints = 0;
+    std::vector<int64_t> group_cols;  // padded columns of each group
+    // launch configuration (autotuned): surface in n_chunks column chunks on
+    // two graph branches; split projection on pre-split halves (p <= 32) or not
+    int n_chunks = 1;
+    bool presplit = true;
+    std::string tuning;
+    hipStream_t gs = nullptr, gs2 = nullptr;
+    hipEvent_t ev_in = nullptr, ev_out = nullptr;
+    std::vector<hipEvent_t> ev_chunk;
+    hipGraphExec_t exec = nullptr;
+    std::vector<const void*> graph_key;
+    std::vector<void*> allocs;
+    template <class Tp>
+    int alloc(Tp** out, size_t n) {
+        void* q = nullptr;
+        hipError_t e = hipMalloc(&q, (n ? n : 1) * sizeof(Tp));
+        if (e != hipSuccess) return fail(-1, std::string("hipMalloc: ") + hipGetErrorString(e));
+        allocs.push_back(q);
+        *out = static_cast<Tp*>(q);
+        return 0;
+    }
+    ~ffs_reml_group() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        for (hipEvent_t e : ev_chunk) (void)hipEventDestroy(e);
+        if (ev_in) (void)hipEventDestroy(ev_in);
+        if (ev_out) (void)hipEventDestroy(ev_out);
+        if (gs) (void)hipStreamDestroy(gs);
+        if (gs2) (void)hipStreamDestroy(gs2);
+        for (void* q : allocs) (void)hipFree(q);
+    }
+};
+
+extern "C" int ffs_reml_group_create(ffs_reml_group** out, int32_t G, ffs_reml_plan* const* plans, int64_t V,
+                                     const int32_t* group_of_voxel, void* stream) {
+    if (!out) return fail(-3, "group_create: null output handle");
+    *out = nullptr;
+    if (G < 1 || !plans || V < 1 || !group_of_voxel) return fail(-3, "group_create: need G >= 1 plans and V >= 1");
+    const ffs_reml_plan* P0 = plans[0];
+    for (int g = 0; g < G; ++g) {
+        const ffs_reml_plan* P = plans[g];
+        if (!P) return fail(-3, "group_create: null plan");
+        if (P->gls_only || P->ridge > 0.0) return fail(-3, "group_create: plans must be lattice (search) plans");
+        if (P->d.T != P0->d.T || P->d.p != P0->d.p || P->d.K != P0->d.K || P->d.BN != P0->d.BN ||
+            P->d.proj_mode != P0->d.proj_mode || P->d.n_task != P0->d.n_task || P->h_ab != P0->h_ab ||
+            P->h_lut != P0->h_lut || P->device != P0->device)
+            return fail(-3, "group_create: plans must share the timeline, pair list, lattice, width, projection "
+                            "mode, F columns and device");
+    }
+    if (V > 0x7fffffff / 2) return fail(-3, "group_create: too many voxels; split them");
+    hipStream_t s = (hipStream_t)stream;
+    auto* R = new ffs_reml_group();
+    std::unique_ptr<ffs_reml_group> guard(R);
+    R->device = P0->device;
+    R->G = G;
+    R->K = P0->d.K;
+    R->T = P0->d.T;
+    R->p = P0->d.p;
+    R->V = V;
+    R->plans.assign(plans, plans + G);
+    // padded layout: groups in id order, each padded to kVoxPad columns
+    std::vector<std::vector<int32_t>> members(G);
+    for (int64_t v = 0; v < V; ++v) {
+        const int g = group_of_voxel[v];
+        if (g < 0 || g >= G) return fail(-3, "group_create: group index out of range");
+        members[g].push_back((int32_t)v);
+    }
+    std::vector<int32_t> rows, blk, tile, vg(group_of_voxel, group_of_voxel + V);
+    for (int g = 0; g < G; ++g) {
+        const int64_t n = (int64_t)members[g].size();
+        const int64_t npad = (n + kVoxPad - 1) / kVoxPad * kVoxPad;
+        R->group_cols.push_back(npad);
+        for (int64_t i = 0; i < npad; ++i) rows.push_back(i < n ? members[g][i] : -1);
+        for (int64_t i = 0; i < npad / 64; ++i) blk.push_back(g);
+        for (int64_t i = 0; i < npad / kProjBM; ++i) tile.push_back(g);
+    }
+    R->ncols = (int64_t)rows.size();
+    if (R->ncols > 0x7fffffff) return fail(-3, "group_create: padded layout too large");
+    std::vector<PlanDev> hp(G);
+    for (int g = 0; g < G; ++g) hp[g] = plans[g]->d;
+    R->yt_rows = yt_rows(P0);
+    R->work_ints = 3 * (int64_t)G * R->K + 1 + gls_perm_cap(V, G * R->K);
+    int rc = 0;
+    if ((rc = R->alloc(&R->dplans, G)) || (rc = R->alloc(&R->rows, rows.size())) ||
+        (rc = R->alloc(&R->blk_grp, blk.size())) || (rc = R->alloc(&R->tile_grp, tile.size())) ||
+        (rc = R->alloc(&R->vgroup, vg.size())) || (rc = R->alloc(&R->Yt, (size_t)R->yt_rows * R->ncols)) ||
+        (rc = R->alloc(&R->S, (size_t)R->ncols * R->K)) || (rc = R->alloc(&R->score_col, R->ncols)) ||
+        (rc = R->alloc(&R->best_col, R->ncols)) || (rc = R->alloc(&R->key, V)) ||
+        (rc = R->alloc(&R->work, R->work_ints)))
+        return rc;
+    HIP_TRY(hipMemcpyAsync(R->dplans, hp.data(), sizeof(PlanDev) * G, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(R->rows, rows.data(), 4 * rows.size(), hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(R->blk_grp, blk.data(), 4 * blk.size(), hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(R->tile_grp, tile.data(), 4 * tile.size(), hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(R->vgroup, vg.data(), 4 * vg.size(), hipMemcpyHostToDevice, s));
+    HIP_TRY(hipStreamSynchronize(s));  // the host vectors above are uploaded
+    HIP_TRY(hipStreamCreateWithFlags(&R->gs, hipStreamNonBlocking));
+    HIP_TRY(hipStreamCreateWithFlags(&R->gs2, hipStreamNonBlocking));
+    HIP_TRY(hipEventCreateWithFlags(&R->ev_in, hipEventDisableTiming));
+    HIP_TRY(hipEventCreateWithFlags(&R->ev_out, hipEventDisableTiming));
+    R->presplit = P0->d.PG != 0 && P0->d.proj_mode != 0;
+    *out = guard.release();
+    return 0;
+}
+
+extern "C" int ffs_reml_group_destroy(ffs_reml_group* grp) {
+    delete grp;
+    return 0;
+}
+
+// The fit stages of a grouped layout, enqueued on s (and on the group's second
+// stream for the odd surface chunks): prep, energy + projection per column
+// chunk, selection, scatter to voxel order, GLS grouped by (design, pair).
+struct GroupIO {
+    const float* Y;
+    int64_t ldY;
+    int32_t mode;
+    int32_t *best;
+    float *score, *scale, *betas, *tstats, *fstats, *sigma2, *r2, *S_out;
+    int64_t ldS_out;
+};
+
+static int group_enqueue(ffs_reml_group* R, const GroupIO& io, hipStream_t s) {
+    const ffs_reml_plan* P0 = R->plans[0];
+    const int K = R->K;
+    const int64_t nc = R->ncols;
+    const bool split_prep = P0->d.PG && P0->d.proj_mode != 0 && R->presplit;
+    if (int rc = prep_impl(P0, io.Y, io.ldY, R->V, R->rows, nc, R->Yt, nc, io.scale, 1, nullptr, s, nullptr,
+                           split_prep, false, nullptr, nullptr, R->dplans, R->blk_grp))
+        return rc;
+    // surface: energy of a chunk, then its projection; chunks alternate
+    // between the two streams so that one chunk's VALU energy can overlap the
+    // previous chunk's MFMA projection
+    // the pre-split halves are addressed by absolute column: one chunk only
+    const int nch = split_prep ? 1 : std::max(1, std::min<int>(R->n_chunks, (int)(nc / kVoxPad)));
+    const int64_t step = (nc / kVoxPad + nch - 1) / nch * kVoxPad;
+    while ((int)R->ev_chunk.size() < 2 * nch + 2) {
+        hipEvent_t e;
+        HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        R->ev_chunk.push_back(e);
+    }
+    HIP_TRY(hipEventRecord(R->ev_chunk[0], s));
+    HIP_TRY(hipStreamWaitEvent(R->gs2, R->ev_chunk[0], 0));
+    int i = 0;
+    for (int64_t c0 = 0; c0 < nc; c0 += step, ++i) {
+        const int64_t n = std::min(step, nc - c0);
+        hipStream_t cs = (i & 1) ? R->gs2 : s;
+        if (int rc = energy_impl(P0, R->Yt + c0, nc, n, R->S + c0 * K, K, nullptr, cs)) return rc;
+        if (int rc = project_impl(P0, R->Yt + c0, nc, n, R->S + c0 * K, K, nullptr, split_prep, cs, R->dplans,
+                                  R->tile_grp + c0 / kProjBM))
+            return rc;
+    }
+    HIP_TRY(hipEventRecord(R->ev_chunk[1], R->gs2));
+    HIP_TRY(hipStreamWaitEvent(s, R->ev_chunk[1], 0));
+    if (io.mode == 0)
+        hipLaunchKernelGGL(k_select_first<float>, dim3((unsigned)((nc + 3) / 4)), dim3(256), 0, s, P0->d, R->S,
+                           (int64_t)K, (int)nc, nullptr, R->best_col, R->score_col);
+    else
+        hipLaunchKernelGGL(k_select<float>, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, s, P0->d, R->S,
+                           (int64_t)K, (int)nc, (int)io.mode, R->best_col, R->score_col);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(k_group_scatter, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, s, R->rows, (int)nc,
+                       R->best_col, R->score_col, R->vgroup, K, io.best, io.score, R->key);
+    HIP_TRY(hipGetLastError());
+    if (io.S_out) {
+        hipLaunchKernelGGL(k_group_gather_rows, dim3((unsigned)nc), dim3(256), 0, s, R->rows, (int)nc, R->S,
+                           (int64_t)K, K, io.S_out, io.ldS_out);
+        HIP_TRY(hipGetLastError());
+    }
+    // GLS: voxels grouped by (design, pair) in 64-slot groups, so every wave
+    // (or block) holds one design's pair
+    const int nb = R->G * K;
+    int32_t* cnt = R->work;
+    int32_t* off = cnt + nb;
+    int32_t* cursor = off + nb + 1;
+    int32_t* perm = cursor + nb;
+    const int64_t cap = gls_perm_cap(R->V, nb);
+    hipLaunchKernelGGL(k_gls_invalid, dim3((unsigned)((R->V + 255) / 256)), dim3(256), 0, s, io.best, (int)R->V, K,
+                       R->p, io.betas, io.tstats, io.fstats, io.sigma2, io.r2);
+    if (int rc = group_by_key(R->key, R->V, nb, 64, cnt, off, cursor, perm, cap, s)) return rc;
+    if (!P0->d.PG) {
+        if (int rc = launch_gls_wide(P0->d, io.Y, io.ldY, io.scale, perm, io.best, cap, io.betas, io.tstats,
+                                     io.fstats, io.sigma2, io.r2, R->dplans, R->vgroup, s))
+            return rc;
+    } else {
+        const dim3 g((unsigned)(cap / 64)), b(64);
+        if (P0->d.PG == 16)
+            hipLaunchKernelGGL(k_gls_mfma<1>, g, b, 0, s, P0->d, io.Y, io.ldY, io.scale, perm, io.best, io.betas,
+                               io.tstats, io.fstats, io.sigma2, io.r2, R->dplans, R->vgroup);
+        else
+            hipLaunchKernelGGL(k_gls_mfma<2>, g, b, 0, s, P0->d, io.Y, io.ldY, io.scale, perm, io.best, io.betas,
+                               io.tstats, io.fstats, io.sigma2, io.r2, R->dplans, R->vgroup);
+    }
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+// Captures the stages into a graph on the group's own streams (pointers of
+// the call are baked in: a call with other pointers or mode captures again).
+static int group_capture(ffs_reml_group* R, const GroupIO& io) {
+    if (R->exec) {
+        (void)hipGraphExecDestroy(R->exec);
+        R->exec = nullptr;
+    }
+    HIP_TRY(hipStreamBeginCapture(R->gs, hipStreamCaptureModeThreadLocal));
+    const int rc = group_enqueue(R, io, R->gs);
+    hipGraph_t graph = nullptr;
+    const hipError_t e = hipStreamEndCapture(R->gs, &graph);
+    if (rc) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return rc;
+    }
+    if (e != hipSuccess) return fail(-1, std::string("group_fit: hipStreamEndCapture: ") + hipGetErrorString(e));
+    const hipError_t e2 = hipGraphInstantiate(&R->exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e2 != hipSuccess) return fail(-1, std::string("group_fit: hipGraphInstantiate: ") + hipGetErrorString(e2));
+    return 0;
+}
+
+static int group_launch(ffs_reml_group* R, const GroupIO& io, hipStream_t s) {
+    const std::vector<const void*> gk = {io.Y, (const void*)(intptr_t)io.ldY, (const void*)(intptr_t)io.mode,
+                                         io.best, io.score, io.scale, io.betas, io.tstats, io.fstats, io.sigma2,
+                                         io.r2, io.S_out, (const void*)(intptr_t)io.ldS_out,
+                                         (const void*)(intptr_t)R->n_chunks, (const void*)(intptr_t)R->presplit};
+    if (!R->exec || gk != R->graph_key) {
+        R->graph_key.clear();
+        if (int rc = group_capture(R, io)) return rc;
+        R->graph_key = gk;
+    }
+    HIP_TRY(hipEventRecord(R->ev_in, s));
+    HIP_TRY(hipStreamWaitEvent(R->gs, R->ev_in, 0));
+    HIP_TRY(hipGraphLaunch(R->exec, R->gs));
+    HIP_TRY(hipEventRecord(R->ev_out, R->gs));
+    HIP_TRY(hipStreamWaitEvent(s, R->ev_out, 0));
+    return 0;
+}
+
+extern "C" int ffs_reml_group_fit(ffs_reml_group* R, const float* Y, int64_t ldY, int32_t mode, int32_t* best,
+                                  float* score, float* scale, float* betas, float* tstats, float* fstats,
+                                  float* sigma2, float* r2, float* S_out, int64_t ldS_out, void* stream) {
+    if (!R) return fail(-3, "group_fit: null group");
+    if (mode < 0 || mode > 2) return fail(-3, "group_fit: mode must be 0, 1 or 2");
+    if (!Y || !best || !score || !scale || !betas || !tstats || !fstats || !sigma2 || !r2)
+        return fail(-3, "group_fit: null argument");
+    if (ldY < R->T) return fail(-3, "group_fit: ldY must be >= T");
+    if (S_out && ldS_out < R->K) return fail(-3, "group_fit: ldS_out must be >= K");
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    if (dev != R->device) return fail(-3, "group_fit: the group belongs to another device");
+    for (const ffs_reml_plan* P : R->plans)
+        if (P->d.proj_mode != R->plans[0]->d.proj_mode)
+            return fail(-3, "group_fit: the plans' projection modes changed after group_create");
+    GroupIO io{Y, ldY, mode, best, score, scale, betas, tstats, fstats, sigma2, r2, S_out, ldS_out};
+    return group_launch(R, io, (hipStream_t)stream);
+}
+
+// Autotuner: times `reps` fits of each candidate launch configuration on the
+// caller's data (graph replays timed by events on the caller's stream, after
+// one untimed capture + replay) and keeps the fastest.  Candidates: surface
+// column chunks n in {1, 2, 4} (chunks alternate between two graph branches,
+// so the VALU energy of one overlaps the MFMA projection of the other) and,
+// for p <= 32 in a split projection mode, the projection voxel tile (256
+// voxels on pre-split halves, one chunk; or 128 voxels split in the kernel).
+extern "C" int ffs_reml_group_tune(ffs_reml_group* R, const float* Y, int64_t ldY, int32_t mode, int32_t* best,
+                                   float* score, float* scale, float* betas, float* tstats, float* fstats,
+                                   float* sigma2, float* r2, int32_t reps, char* report, int64_t report_len,
+                                   void* stream) {
+    if (!R) return fail(-3, "group_tune: null group");
+    if (reps < 1) reps = 1;
+    hipStream_t s = (hipStream_t)stream;
+    const ffs_reml_plan* P0 = R->plans[0];
+    std::vector<std::pair<int, bool>> cands;
+    const bool can_split = P0->d.PG && P0->d.proj_mode != 0;
+    for (int n : {1, 2, 4})
+        for (bool ps : {true, false}) {
+            if (!can_split && !ps) continue;  // the tile choice exists only for pre-split operands
+            if (can_split && ps && n > 1) continue;  // pre-split halves: one surface chunk
+            cands.push_back({n, can_split && ps});
+        }
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    std::string rep = "{\"candidates\": [";
+    double best_ms = 1e300;
+    std::pair<int, bool> best_c = cands[0];
+    int rc = 0;
+    for (size_t ci = 0; ci < cands.size() && !rc; ++ci) {
+        R->n_chunks = cands[ci].first;
+        R->presplit = cands[ci].second;
+        GroupIO io{Y, ldY, mode, best, score, scale, betas, tstats, fstats, sigma2, r2, nullptr, 0};
+        if ((rc = group_launch(R, io, s))) break;  // capture + first replay (discarded)
+        hipError_t e = hipEventRecord(e0, s);
+        for (int r = 0; r < reps && !rc && e == hipSuccess; ++r) rc = group_launch(R, io, s);
+        if (!rc && e == hipSuccess) e = hipEventRecord(e1, s);
+        if (!rc && e == hipSuccess) e = hipEventSynchronize(e1);
+        float ms = 0.f;
+        if (!rc && e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+        if (!rc && e != hipSuccess) rc = fail(-1, std::string("group_tune: ") + hipGetErrorString(e));
+        if (rc) break;
+        const double per = ms / reps;
+        char buf[160];
+        snprintf(buf, sizeof buf, "%s{\"surface_chunks\": %d, \"projection_voxel_tile\": %d, \"ms\": %.4f}",
+                 ci ? ", " : "", cands[ci].first, (can_split && cands[ci].second) ? 256 : 128, per);
+        rep += buf;
+        if (per < best_ms) {
+            best_ms = per;
+            best_c = cands[ci];
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return rc;
+    R->n_chunks = best_c.first;
+    R->presplit = best_c.second;
+    char buf[160];
+    snprintf(buf, sizeof buf, "], \"chosen\": {\"surface_chunks\": %d, \"projection_voxel_tile\": %d, \"ms\": %.4f}}",
+             best_c.first, (can_split && best_c.second) ? 256 : 128, best_ms);
+    rep += buf;
+    R->tuning = rep;
+    if (report && report_len > 0) {
+        const size_t n = std::min<size_t>(rep.size(), (size_t)report_len - 1);
+        memcpy(report, rep.data(), n);
+        report[n] = '\0';
+    }
+    return 0;
+}

@@ -65,6 +65,30 @@ struct SetTable {
     const float* s0;       // [n][sb]
 };
 
+// Scalar arguments of k_tables shared by every design of one launch (one
+// timeline, one pair list).
+struct KTablesArgs {
+    int T, Tpad, p, K;
+    const double* ab;          // [K][2]
+    const int32_t* white_flag; // [K]
+    const int32_t* gap;        // [T]
+    int n_task;
+    const int32_t* task_idx;   // [n_task]
+    int full_f, BN, npt, PG;
+    double ridge;
+};
+
+// Output tables of k_tables for one design (device pointers; null where the
+// plan keeps no such table).
+struct TablesOut {
+    const double* X;
+    double *soa, *coef64, *Xw64, *G64, *Lx, *Ainv, *Fm, *work, *ld, *ldR, *ldX;
+    float* G32;
+    uint16_t *Ghf, *Glf, *Ghb, *Glb;
+    float* g16s;
+    double *G64p, *coefR;
+};
+
 // Device-side description of the pair tables of one plan.
 struct PlanDev {
     int T;        // retained time points
@@ -86,6 +110,7 @@ struct PlanDev {
     const double* ld;          // [K] logdet R + logdet X'R^-1X
     const double* coef64;      // [K][T][3]  (sqrt(1/v_t), c_t, alpha_t)
     const double* X;           // [T][p] design
+    const double* pinvT;       // [T][p] pinv(X)' (row-major, every p)
     const double* Xw64;        // [K][T][p]  L^-1 X
     const double* G64;         // [K][T][p]  R^-1 X Lx^-T
     const double* Lx;          // [K][p][p]  chol(X'R^-1X)
@@ -101,6 +126,7 @@ struct PlanDev {
     const float* g16_unscale;  // [K] 2^-2e_k: the fp16 operand of pair k holds G * 2^e_k
     // GLS on f64 MFMA (p <= kGlsMfmaMaxP): tables padded to PG = 16 * ceil(p / 16) columns.
     int PG;                    // 0 when p > kGlsMfmaMaxP
+    int PGA;                   // width of G64p / Xp / pinvTp / XtX: 16 * ceil(p / 16) for every p <= 96
     const double* G64p;        // [K][Tpad][PG]  G, zero outside t < T, j < p
     const double* Xp;          // [Tpad][PG]     X, zero padded
     const double* pinvTp;      // [Tpad][PG]     pinv(X)', zero padded

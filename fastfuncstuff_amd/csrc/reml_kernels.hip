@@ -140,21 +140,38 @@ __device__ int block_gauss_jordan_inverse(int p, const double* A, double* inv, d
     return 0;
 }
 
-__global__ void __launch_bounds__(64) k_tables(int T, int Tpad, int p, int K, const double* __restrict__ ab,
-                                               const int32_t* __restrict__ white_flag,
-                                               const int32_t* __restrict__ gap, const double* __restrict__ X,
-                                               int n_task, const int32_t* __restrict__ task_idx, int full_f,
-                                               int BN, int npt, double* __restrict__ soa, double* __restrict__ coef64,
-                                               double* __restrict__ Xw64, double* __restrict__ G64,
-                                               double* __restrict__ Lx, double* __restrict__ Ainv,
-                                               double* __restrict__ Fm, double* __restrict__ work,
-                                               double* __restrict__ ld, double* __restrict__ ldR_out,
-                                               double* __restrict__ ldX_out, float* __restrict__ G32,
-                                               uint16_t* __restrict__ Ghf,
-                                               uint16_t* __restrict__ Glf, uint16_t* __restrict__ Ghb,
-                                               uint16_t* __restrict__ Glb, float* __restrict__ g16_unscale,
-                                               int PG, double* __restrict__ G64p, double* __restrict__ coefR,
-                                               double ridge, int* __restrict__ status) {
+__global__ void __launch_bounds__(64) k_tables(KTablesArgs A, const TablesOut* __restrict__ outs,
+                                               int* __restrict__ status_all) {
+    // one block per (pair, design): blockIdx.x = pair, blockIdx.y = design
+    const TablesOut o = outs[blockIdx.y];
+    int* status = status_all + blockIdx.y;
+    const int T = A.T, Tpad = A.Tpad, p = A.p, K = A.K, n_task = A.n_task, full_f = A.full_f, BN = A.BN,
+              npt = A.npt, PG = A.PG;
+    const double ridge = A.ridge;
+    const double* __restrict__ ab = A.ab;
+    const int32_t* __restrict__ white_flag = A.white_flag;
+    const int32_t* __restrict__ gap = A.gap;
+    const int32_t* __restrict__ task_idx = A.task_idx;
+    const double* __restrict__ X = o.X;
+    double* __restrict__ soa = o.soa;
+    double* __restrict__ coef64 = o.coef64;
+    double* __restrict__ Xw64 = o.Xw64;
+    double* __restrict__ G64 = o.G64;
+    double* __restrict__ Lx = o.Lx;
+    double* __restrict__ Ainv = o.Ainv;
+    double* __restrict__ Fm = o.Fm;
+    double* __restrict__ work = o.work;
+    double* __restrict__ ld = o.ld;
+    double* __restrict__ ldR_out = o.ldR;
+    double* __restrict__ ldX_out = o.ldX;
+    float* __restrict__ G32 = o.G32;
+    uint16_t* __restrict__ Ghf = o.Ghf;
+    uint16_t* __restrict__ Glf = o.Glf;
+    uint16_t* __restrict__ Ghb = o.Ghb;
+    uint16_t* __restrict__ Glb = o.Glb;
+    float* __restrict__ g16_unscale = o.g16s;
+    double* __restrict__ G64p = o.G64p;
+    double* __restrict__ coefR = o.coefR;
     const int k = blockIdx.x;
     if (k >= K) return;
     const int tid = threadIdx.x;
@@ -458,11 +475,17 @@ __global__ void __launch_bounds__(64) k_prep(int T, int p, const float* __restri
                                              const int32_t* __restrict__ rows, int ncols, int ncols_pad,
                                              const double* __restrict__ pinvT, const double* __restrict__ X,
                                              int normalize, float* __restrict__ Yt, int64_t ldYt,
-                                             float* __restrict__ scale, float* __restrict__ feats) {
+                                             float* __restrict__ scale, float* __restrict__ feats,
+                                             const PlanDev* __restrict__ plans, const int32_t* __restrict__ grp) {
     __shared__ double tab[kStageChunk * PM];
     const int lane = threadIdx.x & 63;
     const int col = blockIdx.x * 64 + lane;
     if (blockIdx.x * 64 >= ncols_pad) return;
+    if (plans) {  // grouped layout: the design of this 64-column block
+        const int g = max(grp[blockIdx.x], 0);
+        pinvT = plans[g].pinvT;
+        X = plans[g].X;
+    }
     int v = (col < ncols) ? (rows ? rows[col] : col) : -1;
     if (v >= V) v = -1;
     const float* __restrict__ row = Y + (size_t)(v >= 0 ? v : 0) * ldY;
@@ -716,7 +739,8 @@ __device__ __forceinline__ void proj_sstore(const f32x4 (&ra)[A_V4], const f32x4
 // ---------------------------------------------------------------------------
 template <int BN>
 __global__ void __launch_bounds__(256) k_project(PlanDev P, const float* __restrict__ Yt, int64_t ldY, int V,
-                                                 float* __restrict__ S, int64_t ldS, const int32_t* __restrict__ n_live) {
+                                                 float* __restrict__ S, int64_t ldS, const int32_t* __restrict__ n_live,
+                                                 const PlanDev* __restrict__ plans, const int32_t* __restrict__ grp) {
     constexpr int BM = kProjBM, BK = kProjBK, NBLK = BN / 32;
     constexpr int A_ELEMS = BK * BM, B_ELEMS = BK * BN;
     constexpr int STAGE = A_ELEMS + B_ELEMS;
@@ -730,6 +754,11 @@ __global__ void __launch_bounds__(256) k_project(PlanDev P, const float* __restr
     int vt, tile;
     if (!xcd_tile_map(blockIdx.x, gridDim.x, P.n_tiles, &vt, &tile)) return;
     if (n_live && vt * BM >= *n_live) return;  // past the live voxels
+    if (plans) {  // grouped layout: the design of this voxel tile
+        const int g = grp[vt];
+        if (g < 0) return;
+        P = plans[g];
+    }
     const int v0 = vt * BM;
     const float* __restrict__ Ag = Yt + v0;
     const float* __restrict__ Bg = P.G32 + (size_t)tile * P.Tpad * BN;
@@ -896,7 +925,9 @@ __device__ __forceinline__ void proj16_sstore(const float (&ra)[A_U][8], const u
 template <int BN, bool BF16>
 __global__ void __launch_bounds__(256) k_project16(PlanDev P, const float* __restrict__ Yt, int64_t ldY, int V,
                                                    float* __restrict__ S, int64_t ldS,
-                                                   const int32_t* __restrict__ n_live) {
+                                                   const int32_t* __restrict__ n_live,
+                                                   const PlanDev* __restrict__ plans,
+                                                   const int32_t* __restrict__ grp) {
     constexpr int BM = kProjBM, BK = kProjBK, NBLK = BN / 32, NTB = BK / 8;
     constexpr int RP = NTB * 8 + 8;             // row pitch in halves (padded 16 B)
     constexpr int A_H = BM * RP, B_H = BN * RP;  // halves per hi (or lo) plane
@@ -913,6 +944,11 @@ __global__ void __launch_bounds__(256) k_project16(PlanDev P, const float* __res
     int vt, tile;
     if (!xcd_tile_map(blockIdx.x, gridDim.x, P.n_tiles, &vt, &tile)) return;
     if (n_live && vt * BM >= *n_live) return;  // past the live voxels
+    if (plans) {  // grouped layout: the design of this voxel tile
+        const int g = grp[vt];
+        if (g < 0) return;
+        P = plans[g];
+    }
     const int v0 = vt * BM;
     const uint16_t* __restrict__ gh = (BF16 ? P.Gh_bf16 : P.Gh_f16) + (size_t)tile * P.Tpad * BN;
     const uint16_t* __restrict__ gl = (BF16 ? P.Gl_bf16 : P.Gl_f16) + (size_t)tile * P.Tpad * BN;
@@ -1290,11 +1326,17 @@ __global__ void __launch_bounds__(64) k_gls(PlanDev P, const float* __restrict__
                                             float* __restrict__ tstats, float* __restrict__ fstats,
                                             float* __restrict__ sigma2, float* __restrict__ r2,
                                             double* __restrict__ resid_w, int64_t ldR, float* __restrict__ resid,
-                                            float* __restrict__ pred_out) {
+                                            float* __restrict__ pred_out, const PlanDev* __restrict__ plans,
+                                            const int32_t* __restrict__ vgroup) {
     __shared__ double tab[kGlsChunk * (2 * PM + 3)];
     const int lane = threadIdx.x & 63;
     const int i = blockIdx.x * 64 + lane;
     int v = i < V ? (perm ? perm[i] : i) : -1;
+    if (plans) {  // grouped: the 64 slots of a block hold one (design, pair) group
+        const int v0 = __builtin_amdgcn_readfirstlane(perm[blockIdx.x * 64]);
+        if (v0 < 0) return;
+        P = plans[vgroup[v0]];
+    }
     int k = v >= 0 ? best[v] : -1;
     if (v >= 0 && (k < 0 || k >= P.K)) {  // no valid pair: NaN statistics
         for (int j = 0; j < P.p; ++j) {

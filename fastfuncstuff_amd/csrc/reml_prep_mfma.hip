@@ -47,7 +47,10 @@ __global__ void __launch_bounds__(64 * kPrepWaves) k_prep_mfma(PlanDev P, const 
                                                                const int32_t* __restrict__ ncols_live,
                                                                uint16_t* __restrict__ Ysh, uint16_t* __restrict__ Ysl,
                                                                int split_bf16, const double* __restrict__ coef_in,
-                                                               double* __restrict__ coef_out) {
+                                                               double* __restrict__ coef_out,
+                                                               const PlanDev* __restrict__ plans,
+                                                               const int32_t* __restrict__ grp) {
+    if (plans) P = plans[max(grp[blockIdx.x], 0)];  // grouped layout: the design of this 64-column block
     constexpr int PG = 16 * NT;
     constexpr int RB = 65;  // feature tile pitch (floats)
     constexpr int W = kPrepWaves;

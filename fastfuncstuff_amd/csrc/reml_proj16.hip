@@ -190,11 +190,18 @@ __device__ __forceinline__ void proj16_item(const PlanDev& P, const float* __res
 template <int BN, bool BF16>
 __global__ void __launch_bounds__(256) k_project16b(PlanDev P, const float* __restrict__ Yt, int64_t ldY, int V,
                                                     float* __restrict__ S, int64_t ldS,
-                                                    const int32_t* __restrict__ n_live) {
+                                                    const int32_t* __restrict__ n_live,
+                                                    const PlanDev* __restrict__ plans,
+                                                    const int32_t* __restrict__ grp) {
     __shared__ __attribute__((aligned(16))) unsigned char lds_raw[Proj16Lds<BN>::BYTES];
     int vt, tile;
     if (!xcd_tile_map(blockIdx.x, gridDim.x, P.n_tiles, &vt, &tile)) return;
     if (n_live && vt * kProj16BM >= *n_live) return;  // past the live voxels
+    if (plans) {  // grouped layout: the design of this voxel tile (grp per kProjBM-voxel tile)
+        const int g = grp[vt * (kProj16BM / kProjBM)];
+        if (g < 0) return;
+        P = plans[g];
+    }
     proj16_item<BN, BF16>(P, Yt, ldY, V, S, ldS, vt, tile, lds_raw);
 }
 

@@ -255,7 +255,9 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) k_
                                                  const float* __restrict__ scale, const int32_t* __restrict__ perm,
                                                  const int32_t* __restrict__ best, float* __restrict__ betas,
                                                  float* __restrict__ tstats, float* __restrict__ fstats,
-                                                 float* __restrict__ sigma2, float* __restrict__ r2) {
+                                                 float* __restrict__ sigma2, float* __restrict__ r2,
+                                                 const PlanDev* __restrict__ plans,
+                                                 const int32_t* __restrict__ vgroup) {
     constexpr int PG = 16 * NT;
     constexpr int TP = PG + 1;  // transpose pitch (doubles)
     __shared__ double cbuf[32 * 4];
@@ -265,6 +267,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) k_
     const int64_t slot0 = (int64_t)blockIdx.x * 64;
     const int v0 = __builtin_amdgcn_readfirstlane(perm[slot0]);
     if (v0 < 0) return;  // padding or beyond the last group
+    if (plans) P = plans[vgroup[v0]];  // grouped: one (design, pair) group per 64 slots
     const int k = __builtin_amdgcn_readfirstlane(best[v0]);
     const int v = perm[slot0 + lane];
     const int T = P.T, Tpad = P.Tpad, p = P.p;

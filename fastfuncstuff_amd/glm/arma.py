@@ -238,8 +238,61 @@ class RemlGrid(Mapping):
 
     def __init__(self, X, a_grid, b_grid, device, run_starts=None, tau=None, task_indices=None, pairs=None,
                  ridge: float = 0.0, ensure_zero: bool = True, lattice: bool | None = None):
-        self.device = _lib.require_device(device)
+        args = self._setup_host(X, a_grid, b_grid, device, run_starts, tau, task_indices, pairs, ridge, ensure_zero,
+                                lattice)
         lib = _lib.load()
+        handle = ctypes.c_void_p()
+        with torch.cuda.device(self.device):
+            if self.ridge > 0:
+                rc = lib.ffs_reml_plan_create_ridge(
+                    ctypes.byref(handle), self.T, self.p, self.X64.ctypes.data, args["pinvT"].ctypes.data,
+                    self.gap.ctypes.data, self.K, args["ab"].ctypes.data, args["white"].ctypes.data,
+                    args["n_a"], args["n_b"], self.lut.ctypes.data, args["za"], args["zb"], self.ltop, self.ridge,
+                    _lib.stream_handle(self.device))
+            else:
+                task = args["task"]
+                rc = lib.ffs_reml_plan_create(
+                    ctypes.byref(handle), self.T, self.p, self.X64.ctypes.data, args["pinvT"].ctypes.data,
+                    self.gap.ctypes.data, self.K, args["ab"].ctypes.data, args["white"].ctypes.data,
+                    args["n_a"], args["n_b"], self.lut.ctypes.data, args["za"], args["zb"], self.ltop,
+                    int(task.size), task.ctypes.data if task.size else None, _lib.stream_handle(self.device))
+        _lib.check(rc, "precompute_reml_grid")
+        self._attach(handle)
+
+    @classmethod
+    def create_many(cls, designs, a_grid, b_grid, device, run_starts=None, tau=None, task_indices=None):
+        """One plan per design (a list of (T, p) arrays of one shape) on one
+        timeline and (a, b) grid, the per-pair tables of all designs built in
+        one device launch (ffs_reml_plan_create_many).  Returns the list of
+        RemlGrid objects."""
+        grids = [cls.__new__(cls) for _ in designs]
+        setups = [g._setup_host(X, a_grid, b_grid, device, run_starts, tau, task_indices, None, 0.0, True, None)
+                  for g, X in zip(grids, designs)]
+        g0, a0 = grids[0], setups[0]
+        if any(g.X64.shape != g0.X64.shape for g in grids):
+            raise ValueError("create_many: every design must have the same shape")
+        Xall = np.ascontiguousarray(np.stack([g.X64 for g in grids]))
+        Pall = np.ascontiguousarray(np.stack([a["pinvT"] for a in setups]))
+        G = len(grids)
+        handles = (ctypes.c_void_p * G)()
+        task = a0["task"]
+        lib = _lib.load()
+        with torch.cuda.device(g0.device):
+            rc = lib.ffs_reml_plan_create_many(
+                ctypes.cast(handles, ctypes.c_void_p), G, g0.T, g0.p, Xall.ctypes.data, Pall.ctypes.data,
+                g0.gap.ctypes.data, g0.K, a0["ab"].ctypes.data, a0["white"].ctypes.data, a0["n_a"], a0["n_b"],
+                g0.lut.ctypes.data, a0["za"], a0["zb"], g0.ltop, int(task.size),
+                task.ctypes.data if task.size else None, _lib.stream_handle(g0.device))
+        _lib.check(rc, "precompute_reml_grid (grouped designs)")
+        for g, h in zip(grids, handles):
+            g._attach(ctypes.c_void_p(h))
+        return grids
+
+    def _setup_host(self, X, a_grid, b_grid, device, run_starts, tau, task_indices, pairs, ridge, ensure_zero,
+                    lattice) -> dict:
+        """Host-side tables of the plan (pairs, lattice, timeline, pinv);
+        returns the remaining plan-creation arguments."""
+        self.device = _lib.require_device(device)
         X = torch.as_tensor(X)
         if X.ndim != 2:
             raise ValueError("design must be 2-D (n_timepoints, n_regressors)")
@@ -294,20 +347,12 @@ class RemlGrid(Mapping):
         ab = np.ascontiguousarray(np.stack([a_np, b_np], 1))
         task = np.asarray(task_indices if task_indices is not None else [], dtype=np.int32)
         self.task_indices = None if task_indices is None else list(task_indices)
-        handle = ctypes.c_void_p()
-        with torch.cuda.device(self.device):
-            if self.ridge > 0:
-                rc = lib.ffs_reml_plan_create_ridge(
-                    ctypes.byref(handle), self.T, self.p, self.X64.ctypes.data, pinvT.ctypes.data,
-                    self.gap.ctypes.data, self.K, ab.ctypes.data, np.ascontiguousarray(white).ctypes.data,
-                    n_a, n_b, lut.ctypes.data, za, zb, self.ltop, self.ridge, _lib.stream_handle(self.device))
-            else:
-                rc = lib.ffs_reml_plan_create(
-                    ctypes.byref(handle), self.T, self.p, self.X64.ctypes.data, pinvT.ctypes.data,
-                    self.gap.ctypes.data, self.K, ab.ctypes.data, np.ascontiguousarray(white).ctypes.data,
-                    n_a, n_b, lut.ctypes.data, za, zb, self.ltop, int(task.size),
-                    task.ctypes.data if task.size else None, _lib.stream_handle(self.device))
-        _lib.check(rc, "precompute_reml_grid")
+        return {"pinvT": pinvT, "ab": ab, "white": np.ascontiguousarray(white), "task": task, "n_a": n_a, "n_b": n_b,
+                "za": za, "zb": zb}
+
+    def _attach(self, handle) -> None:
+        """Take ownership of a created plan and read its dimensions and log-dets."""
+        lib = _lib.load()
         self._handle = handle
         tp, vp, _ = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
         _lib.check(lib.ffs_reml_plan_dims(handle, ctypes.byref(tp), ctypes.byref(vp), None), "plan_dims")
@@ -1543,6 +1588,139 @@ def _partial_r2(t_full, r2, dof, p, task_indices, want_partial, partial_mode, wa
     return out
 
 
+class RemlGroup:
+    """Voxel groups with their own designs on one timeline and (a, b) grid,
+    fitted with one launch per stage over all groups (ffs_reml_group_*): voxels
+    laid out group by group (each padded to 256 columns), per-design tables
+    picked per block, the stages captured into one HIP graph and replayed.
+    ``grids`` (from ``RemlGrid.create_many``) must outlive the group."""
+
+    def __init__(self, grids: list, group_of_voxel):
+        self.grids = list(grids)
+        g0 = self.grids[0]
+        self.device, self.K, self.T, self.p = g0.device, g0.K, g0.T, g0.p
+        gv = np.ascontiguousarray(np.asarray(group_of_voxel, dtype=np.int32))
+        self.V = int(gv.shape[0])
+        handles = (ctypes.c_void_p * len(self.grids))(*[g.handle.value for g in self.grids])
+        h = ctypes.c_void_p()
+        with torch.cuda.device(self.device):
+            _lib.check(_lib.load().ffs_reml_group_create(ctypes.byref(h), len(self.grids),
+                                                         ctypes.cast(handles, ctypes.c_void_p), self.V,
+                                                         gv.ctypes.data, _lib.stream_handle(self.device)),
+                       "group_create")
+        self._handle = h
+        self.tuning = None
+        self._out = None
+
+    def _outputs(self):
+        if self._out is None:
+            f32 = dict(dtype=torch.float32, device=self.device)
+            V, p = self.V, self.p
+            self._out = {"best": torch.empty(V, dtype=torch.int32, device=self.device),
+                         "score": torch.empty(V, **f32), "scale": torch.empty(V, **f32),
+                         "betas": torch.empty(V, p, **f32), "tstats": torch.empty(V, p, **f32),
+                         "fstats": torch.empty(V, **f32), "sigma2": torch.empty(V, **f32), "r2": torch.empty(V, **f32)}
+        return self._out
+
+    def fit(self, Y: torch.Tensor, selector: str = "hierarchical", surface: torch.Tensor | None = None) -> dict:
+        """All stages for Y (V, T) float32 on the device; returns device tensors
+        (reused across calls) best, score, scale, betas, tstats, fstats,
+        sigma2, r2; ``surface`` (optional (V, K) float32) receives the scores."""
+        if Y.shape != (self.V, self.T) or Y.dtype != torch.float32 or Y.stride(1) != 1:
+            raise ValueError(f"Y must be a row-contiguous ({self.V}, {self.T}) float32 device tensor")
+        o = self._outputs()
+        mode = RemlGrid.SELECTORS[selector]
+        _lib.check(_lib.load().ffs_reml_group_fit(
+            self._handle, _lib.ptr(Y), Y.stride(0), mode, _lib.ptr(o["best"]), _lib.ptr(o["score"]),
+            _lib.ptr(o["scale"]), _lib.ptr(o["betas"]), _lib.ptr(o["tstats"]), _lib.ptr(o["fstats"]),
+            _lib.ptr(o["sigma2"]), _lib.ptr(o["r2"]), _lib.ptr(surface), 0 if surface is None else surface.stride(0),
+            _lib.stream_handle(self.device)), "group_fit")
+        return o
+
+    def tune(self, Y: torch.Tensor, selector: str = "hierarchical", reps: int = 3) -> dict:
+        """Autotune the launch configuration on Y (see ffs_reml_group_tune);
+        returns and keeps the report (candidates with ms per fit, the choice)."""
+        import json
+
+        o = self._outputs()
+        buf = ctypes.create_string_buffer(4096)
+        _lib.check(_lib.load().ffs_reml_group_tune(
+            self._handle, _lib.ptr(Y), Y.stride(0), RemlGrid.SELECTORS[selector], _lib.ptr(o["best"]),
+            _lib.ptr(o["score"]), _lib.ptr(o["scale"]), _lib.ptr(o["betas"]), _lib.ptr(o["tstats"]),
+            _lib.ptr(o["fstats"]), _lib.ptr(o["sigma2"]), _lib.ptr(o["r2"]), int(reps), buf, len(buf),
+            _lib.stream_handle(self.device)), "group_tune")
+        self.tuning = json.loads(buf.value.decode())
+        return self.tuning
+
+    def close(self):
+        h = getattr(self, "_handle", None)
+        if h is not None and h.value:
+            _lib.load().ffs_reml_group_destroy(h)
+            self._handle = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# fit_glm_arma11 options the grouped engine computes (others take the per-group loop)
+_GROUPED_ENGINE_KW = {"a_grid", "b_grid", "device", "run_starts", "tau", "task_indices", "projection",
+                      "force_exhaustive_search", "selector", "save_profile_likelihoods", "batch_size", "verbose",
+                      "estimate_per_voxel", "autotune"}
+
+
+def _grouped_engine_fit(data_t, designs_by_group, groups, group_indices, kw) -> ARMA11Results:
+    """fit_glm_arma11_grouped on the grouped engine: one plan per design (tables
+    of all designs in one launch), one graph of single launches over all
+    groups, outputs in voxel order."""
+    dev = _lib.require_device(kw.get("device"))
+    V = data_t.shape[0]
+    X0 = torch.as_tensor(designs_by_group[groups[0]])
+    T, p = X0.shape
+    if data_t.shape[1] != T:
+        raise ValueError(f"Timepoints mismatch: Data has {data_t.shape[1]}, Design has {T}")
+    ti = kw.get("task_indices")
+    grids = RemlGrid.create_many([torch.as_tensor(designs_by_group[h]) for h in groups], kw["a_grid"], kw["b_grid"],
+                                 dev, run_starts=kw.get("run_starts"), tau=kw.get("tau"), task_indices=ti)
+    proj = kw.get("projection", "fp16x3")
+    for g in grids:
+        g.set_projection(proj)
+    gid = {h: i for i, h in enumerate(groups)}
+    gv = np.asarray([gid[int(h)] for h in group_indices.tolist()], dtype=np.int32)
+    sel = "exhaustive" if kw.get("force_exhaustive_search") else (kw.get("selector") or "hierarchical")
+    if sel not in RemlGrid.SELECTORS:
+        raise ValueError(f"selector must be one of {sorted(RemlGrid.SELECTORS)}, got {sel!r}")
+    engine = RemlGroup(grids, gv)
+    Y = _to_device_f32(data_t, dev)
+    tuning = engine.tune(Y, sel) if kw.get("autotune") else None
+    S = torch.empty(V, grids[0].K, dtype=torch.float32, device=dev) if kw.get("save_profile_likelihoods") else None
+    o = engine.fit(Y, sel, surface=S)
+    res = ARMA11Results()
+    res.dof = max(1, T - p)
+    res.tr = kw.get("tr")
+    res.fitted_column_indices = ti
+    res.n_regressors_full = p
+    ix = torch.as_tensor(ti, dtype=torch.long, device=dev) if ti else None
+    res.betas = (o["betas"].index_select(1, ix) if ix is not None else o["betas"]).cpu()
+    res.tstats = (o["tstats"].index_select(1, ix) if ix is not None else o["tstats"]).cpu()
+    res.fstats, res.sigma2, res.r2 = o["fstats"].cpu(), o["sigma2"].cpu(), o["r2"].cpu()
+    best = o["best"].long()
+    params = grids[0].params.to(dev, torch.float32)
+    res.arma_params = params[best.clamp(min=0)].cpu()
+    res.reml_likelihood = o["score"].cpu()
+    res.pair_index = best.cpu()
+    a, b = res.arma_params[:, 0], res.arma_params[:, 1]
+    res.arma_lambda = (b + a) * (1 + a * b) / (1 + 2 * a * b + b**2 + 1e-10)  # glm/arma.py:4769-4777
+    if S is not None:
+        res.reml_lklhd_surface = S.cpu()
+        res.reml_surface_params = list(grids[0].param_list)
+    res.grouped_engine = {"groups": len(groups), "tuning": tuning}
+    engine.close()
+    return res
+
+
 def fit_glm_arma11_grouped(data, designs_by_group: dict, group_indices, tr: float, group_label: str = "group",
                            **fit_kwargs) -> ARMA11Results:
     """glm/arma.py:2836-3183: REML with one design per voxel group (per-voxel
@@ -1550,7 +1728,16 @@ def fit_glm_arma11_grouped(data, designs_by_group: dict, group_indices, tr: floa
     with its own plan (the design-dependent tables), on the same (a, b) grid,
     and the per-voxel outputs are scattered back to the full voxel axis
     (``VOXEL_SCATTER_ATTRS``, zeros elsewhere), as the reference merges them.
-    ``precomputed_arma_params`` is sliced per group."""
+    ``precomputed_arma_params`` is sliced per group.
+
+    Without options that need the per-group fit (series outputs, Ljung-Box,
+    partial R^2, GLTs, column sets, -dsort, the OLS baseline, precomputed
+    parameters, use_double, the learned selector) the fit runs on the grouped
+    engine (``RemlGroup``): the tables of every design in one launch, then one
+    launch per stage over all groups, captured in a HIP graph;
+    ``autotune=True`` times its launch configurations first and reports the
+    choice in ``result.grouped_engine``.  ``grouped_engine=False`` keeps the
+    per-group fits."""
     group_indices = torch.as_tensor(group_indices).long().cpu()
     data_t = torch.as_tensor(data)
     n_voxels = data_t.shape[0]
@@ -1588,6 +1775,13 @@ def fit_glm_arma11_grouped(data, designs_by_group: dict, group_indices, tr: floa
         kw.setdefault("b_grid", db)
         kw["a_grid"] = da if kw["a_grid"] is None else kw["a_grid"]
         kw["b_grid"] = db if kw["b_grid"] is None else kw["b_grid"]
+    # one launch per stage over all groups unless an option needs the per-group fit
+    use_engine = kw.pop("grouped_engine", True)
+    if (use_engine and pre is None and dsort_full is None and set(kw) <= _GROUPED_ENGINE_KW
+            and kw.get("estimate_per_voxel", True) and shape[1] <= 96 and n_voxels > 0
+            and kw.get("selector") != "learned"):
+        return _grouped_engine_fit(data_t, designs_by_group, groups, group_indices, dict(kw, tr=tr))
+    kw.pop("autotune", None)
     per, masks = {}, {}
     order = torch.argsort(group_indices, stable=True)
     counts = torch.bincount(group_indices - group_indices.min()) if n_voxels else torch.zeros(0, dtype=torch.long)

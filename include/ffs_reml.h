@@ -68,6 +68,18 @@ int ffs_reml_plan_create(ffs_reml_plan** out, int32_t T, int32_t p, const double
 
 int ffs_reml_plan_destroy(ffs_reml_plan* plan);
 
+/* G plans of one timeline and pair list with G designs (X and pinvT: G x T x p,
+ * host, row-major per design), for grouped fits (glm/arma.py:2836-3183 fits
+ * one design per voxel group on one shared autocorrelation grid,
+ * glm/arma.py:2971-2989).  The per-pair tables of all designs are built in one
+ * launch over (pair, design).  Same other arguments as ffs_reml_plan_create;
+ * out[0..G) receive the plans (each destroyed with ffs_reml_plan_destroy).
+ * Returns -2 naming the first rank-deficient design. */
+int ffs_reml_plan_create_many(ffs_reml_plan** out, int32_t G, int32_t T, int32_t p, const double* X,
+                              const double* pinvT, const int32_t* gap, int32_t K, const double* ab,
+                              const int32_t* white, int32_t n_a, int32_t n_b, const int32_t* lut, int32_t zero_a,
+                              int32_t zero_b, int32_t ltop, int32_t n_task, const int32_t* task_idx, void* stream);
+
 /* Search-only plan whose score uses A + ridge I (A = X'R^-1X) in the
  * log-determinant and in the solve for beta, with the whitened residual of
  * that beta: the score of compute_reml_likelihood / reml_grid_search_batched
@@ -186,6 +198,39 @@ int ffs_reml_gls_workspace(const ffs_reml_plan* plan, int64_t V, int64_t* n_int3
 int ffs_reml_gls(const ffs_reml_plan* plan, const float* Y, int64_t ldY, const float* scale, const int32_t* best,
                  int64_t V, float* betas, float* tstats, float* fstats, float* sigma2, float* r2, double* resid_w,
                  int64_t ldR, float* resid, float* pred, int32_t* work, void* stream);
+
+/*
+ * Grouped fits (fit_glm_arma11_grouped, glm/arma.py:2836-3183, which loops
+ * over the groups one fit at a time): one launch per stage over the voxels of
+ * all groups.  ffs_reml_group_create takes G plans of one timeline, pair
+ * list, lattice, width and projection mode (ffs_reml_plan_create_many) and
+ * the group of every voxel (host, V int32 in [0, G)); it lays the voxels out
+ * group by group, each group padded to 256 columns, so every prep block and
+ * projection voxel tile holds one design.  The group borrows the plans: keep
+ * them alive while it is used.
+ * ffs_reml_group_fit: prep (normalise + residualise on each voxel's design),
+ * energy, projection with the score epilogue, selection (mode as
+ * ffs_reml_select), then the FP64 GLS of every voxel at its pair with its
+ * design (voxels grouped by (design, pair)).  Outputs in voxel order: best
+ * pair, score, scale (std used to normalise), betas, tstats (V x p), fstats,
+ * sigma2, r2 (V), and, when S_out is given, the surface rows (V x K, stride
+ * ldS_out).  The stages are captured once into a HIP graph on the group's own
+ * streams and replayed on later calls with the same pointers (the caller's
+ * stream waits for the replay); other pointers capture again.
+ * ffs_reml_group_tune: the autotuner; times `reps` replays of each candidate
+ * launch configuration on the given data and keeps the fastest; `report`
+ * (nullable) receives a JSON description of the candidates and the choice.
+ */
+typedef struct ffs_reml_group ffs_reml_group;
+int ffs_reml_group_create(ffs_reml_group** out, int32_t G, ffs_reml_plan* const* plans, int64_t V,
+                          const int32_t* group_of_voxel, void* stream);
+int ffs_reml_group_destroy(ffs_reml_group* grp);
+int ffs_reml_group_fit(ffs_reml_group* grp, const float* Y, int64_t ldY, int32_t mode, int32_t* best, float* score,
+                       float* scale, float* betas, float* tstats, float* fstats, float* sigma2, float* r2,
+                       float* S_out, int64_t ldS_out, void* stream);
+int ffs_reml_group_tune(ffs_reml_group* grp, const float* Y, int64_t ldY, int32_t mode, int32_t* best, float* score,
+                        float* scale, float* betas, float* tstats, float* fstats, float* sigma2, float* r2,
+                        int32_t reps, char* report, int64_t report_len, void* stream);
 
 /*
  * Allocation-free variant of ffs_reml_gls.  No counterpart in the reference:

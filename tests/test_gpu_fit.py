@@ -462,3 +462,59 @@ def test_pair_list_plan_continuous_params(dev):
     Yt, _, _ = grid.prep(torch.from_numpy(Y[:64]).to(dev))
     with pytest.raises(RemlLibraryError):
         grid.surface(Yt, 64)
+
+
+@pytest.mark.parametrize("p", [6, 40, 68])
+def test_grouped_engine_matches_per_group_fits(dev, p):
+    """The grouped engine (one launch per stage over all groups, HIP graph)
+    gives the per-group fits' results: scores within 2e-4, pairs on >= 99.9%
+    of voxels (float32 ties aside), betas / t / F / sigma2 / R^2 within 1e-6
+    relative where the pairs agree; the autotuner reports its candidates and
+    choice; graph replays with new data in the same buffers reproduce a fresh
+    fit of that data."""
+    from fastfuncstuff_amd.glm.arma import RemlGrid, RemlGroup, fit_glm_arma11_grouped
+
+    print(f"[fit] grouped engine p={p}", flush=True)
+    rng = np.random.default_rng(100 + p)
+    T, V, G = 150, 1200, 4
+    ag = bg = torch.linspace(-0.8, 0.8, 17)
+    designs = {}
+    for h in range(G):
+        designs[h + 10] = np.column_stack([np.ones(T), np.linspace(-1, 1, T), rng.standard_normal((T, p - 2))])
+    groups = rng.choice(np.arange(10, 10 + G), V, p=[0.5, 0.3, 0.15, 0.05])
+    e = np.zeros((V, T))
+    a = rng.uniform(0.1, 0.8, V)
+    for t in range(T):
+        e[:, t] = (a * e[:, t - 1] if t else 0) + rng.standard_normal(V)
+    sig = np.stack([rng.standard_normal(p) @ designs[h].T for h in groups]) * 0.3
+    Y = torch.from_numpy((sig + 2.0 + e).astype(np.float32))
+    eng = fit_glm_arma11_grouped(Y, designs, torch.from_numpy(groups), 2.0, a_grid=ag, b_grid=bg, device=dev,
+                                 autotune=True, save_profile_likelihoods=True)
+    loop = fit_glm_arma11_grouped(Y, designs, torch.from_numpy(groups), 2.0, a_grid=ag, b_grid=bg, device=dev,
+                                  grouped_engine=False, save_profile_likelihoods=True)
+    tun = eng.grouped_engine["tuning"]
+    print(f"[fit] tuning {tun}", flush=True)
+    assert tun["chosen"]["ms"] > 0 and len(tun["candidates"]) >= 3
+    # the tuner may pick the in-kernel split projection: scores agree to fp32
+    # rounding, pairs except float32 ties
+    dS = (eng.reml_lklhd_surface.double() - loop.reml_lklhd_surface.double()).abs()
+    assert float(dS.max()) <= 2e-4
+    same = eng.pair_index == loop.pair_index
+    assert float(same.double().mean()) >= 0.999
+    for name in ("betas", "tstats", "fstats", "sigma2", "r2"):
+        x, y = getattr(eng, name).double()[same], getattr(loop, name).double()[same]
+        assert float(((x - y).abs() / y.abs().clamp(min=1e-3)).max()) <= 1e-6, name
+    # replays on new data in the same buffer equal a fresh engine fit
+    grids = RemlGrid.create_many([torch.from_numpy(designs[h]) for h in sorted(designs)], ag, bg, dev)
+    for g in grids:
+        g.set_projection("fp16x3")
+    gv = np.searchsorted(sorted(designs), groups).astype(np.int32)
+    engine = RemlGroup(grids, gv)
+    Yd = Y.to(dev)
+    first = {k: v.clone() for k, v in engine.fit(Yd).items()}
+    Yd.mul_(1.5).add_(0.25)
+    second = {k: v.clone() for k, v in engine.fit(Yd).items()}
+    fresh = RemlGroup(grids, gv).fit(Yd)
+    assert bool((second["best"] == fresh["best"]).all()) and bool((second["betas"] == fresh["betas"]).all())
+    assert not bool((second["betas"] == first["betas"]).all())
+    engine.close()
