@@ -28,6 +28,10 @@ SIGNATURES: dict[str, list] = {
     "ffs_reml_plan_create_ridge": [_pp, _c_int, _c_int, _vp, _vp, _vp, _c_int, _vp, _vp, _c_int, _c_int, _vp, _c_int,
                                    _c_int, _c_int, ctypes.c_double, _vp],
     "ffs_reml_plan_destroy": [_vp],
+    "ffs_reml_plan_rank": [_vp],
+    "ffs_reml_dsort_workspace": [_vp, _i64, _c_int, _c_int, _vp],
+    "ffs_reml_dsort_gls": [_vp, _vp, _i64, _c_int, _vp, _vp, _vp, _c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+                           _vp, _vp, _vp, _i64, _vp],
     "ffs_reml_plan_create_many": [_vp, _c_int, _c_int, _c_int, _vp, _vp, _vp, _c_int, _vp, _vp, _c_int, _c_int, _vp,
                                   _c_int, _c_int, _c_int, _c_int, _vp, _vp],
     "ffs_reml_group_create": [_pp, _c_int, _vp, _i64, _vp, _vp],

@@ -144,6 +144,79 @@ extern "C" int ffs_reml_plan_create_ridge(ffs_reml_plan** out, int32_t T, int32_
                             ridge, stream);
 }
 
+// Column rank structure of a design (glm/arma.py:1493-1500 falls back to QR
+// when the Cholesky of X_w'X_w fails).  Whitening by the invertible L^-1
+// keeps the column space, so which columns depend on the columns before them
+// is a property of X alone: column j is dependent when its squared residual
+// against columns 0..j-1 (the pivot of a Cholesky of X'X) is at most
+// kRankTol |x_j|^2.  Returns the rank r; for r < p fills dep (1 = dependent)
+// and Pi = M'(MM')^-1 M, the projection onto the row space of X, where
+// X = X_I M with X_I the independent columns (M = (X_I'X_I)^-1 X_I'X).
+// FP64 on the host: p^3 work once per design.
+constexpr double kRankTol = 1e-13;
+static int design_rank(int p, const std::vector<double>& A, std::vector<int32_t>& dep, std::vector<double>& Pi) {
+    std::vector<double> L((size_t)p * p, 0.0);
+    std::vector<int> I;
+    dep.assign(p, 0);
+    for (int j = 0; j < p; ++j) {
+        const double ajj = A[(size_t)j * p + j];
+        double d = ajj;
+        for (int k = 0; k < j; ++k) d -= L[(size_t)j * p + k] * L[(size_t)j * p + k];
+        if (std::isfinite(d) && !(d > kRankTol * ajj)) {
+            dep[j] = 1;
+            continue;
+        }
+        const double l = sqrt(d);  // NaN designs fail later in the device Cholesky
+        L[(size_t)j * p + j] = l;
+        for (int i = j + 1; i < p; ++i) {
+            double s = A[(size_t)i * p + j];
+            for (int k = 0; k < j; ++k) s -= L[(size_t)i * p + k] * L[(size_t)j * p + k];
+            L[(size_t)i * p + j] = s / l;
+        }
+        I.push_back(j);
+    }
+    const int r = (int)I.size();
+    if (r == p) return p;
+    Pi.assign((size_t)p * p, 0.0);
+    if (r == 0) return 0;
+    // M (r x p): L_I z = A_I,c, L_I' m = z per column c (L_I: rows / cols I of L)
+    std::vector<double> M((size_t)r * p), z(r);
+    for (int c = 0; c < p; ++c) {
+        for (int a = 0; a < r; ++a) {
+            double s = A[(size_t)I[a] * p + c];
+            for (int b = 0; b < a; ++b) s -= L[(size_t)I[a] * p + I[b]] * z[b];
+            z[a] = s / L[(size_t)I[a] * p + I[a]];
+        }
+        for (int a = r - 1; a >= 0; --a) {
+            double s = z[a];
+            for (int b = a + 1; b < r; ++b) s -= L[(size_t)I[b] * p + I[a]] * M[(size_t)b * p + c];
+            M[(size_t)a * p + c] = s / L[(size_t)I[a] * p + I[a]];
+        }
+    }
+    // Pi = Z'Z with Z = C^-1 M, C = chol(MM') (MM' >= I: M holds I_r)
+    std::vector<double> C((size_t)r * r, 0.0), Z((size_t)r * p);
+    for (int a = 0; a < r; ++a)
+        for (int b = 0; b <= a; ++b) {
+            double s = 0.0;
+            for (int c = 0; c < p; ++c) s += M[(size_t)a * p + c] * M[(size_t)b * p + c];
+            for (int k = 0; k < b; ++k) s -= C[(size_t)a * r + k] * C[(size_t)b * r + k];
+            C[(size_t)a * r + b] = (a == b) ? sqrt(s) : s / C[(size_t)b * r + b];
+        }
+    for (int c = 0; c < p; ++c)
+        for (int a = 0; a < r; ++a) {
+            double s = M[(size_t)a * p + c];
+            for (int b = 0; b < a; ++b) s -= C[(size_t)a * r + b] * Z[(size_t)b * p + c];
+            Z[(size_t)a * p + c] = s / C[(size_t)a * r + a];
+        }
+    for (int i = 0; i < p; ++i)
+        for (int j = 0; j < p; ++j) {
+            double s = 0.0;
+            for (int a = 0; a < r; ++a) s += Z[(size_t)a * p + i] * Z[(size_t)a * p + j];
+            Pi[(size_t)i * p + j] = s;
+        }
+    return r;
+}
+
 // Work left after the allocation phase of one plan (see plan_alloc).
 struct PlanPending {
     ffs_reml_plan* P = nullptr;
@@ -280,6 +353,26 @@ static int plan_alloc(ffs_reml_plan* P, int32_t T, int32_t p, const double* X, c
         HIP_TRY(hipMemsetAsync(Ghb, 0, 2 * n16, s));
         HIP_TRY(hipMemsetAsync(Glb, 0, 2 * n16, s));
     }
+    // X'X of the design (f64 MFMA GLS table and the rank structure)
+    std::vector<double> XtX((size_t)p * p, 0.0);
+    for (int a = 0; a < p; ++a)
+        for (int b = 0; b < p; ++b) {
+            double acc = 0.0;
+            for (int t = 0; t < T; ++t) acc += X[(size_t)t * p + a] * X[(size_t)t * p + b];
+            XtX[(size_t)a * p + b] = acc;
+        }
+    int32_t* ddep = nullptr;
+    double* dPi = nullptr;
+    d.rank = p;
+    if (ridge <= 0.0) {
+        std::vector<int32_t> dep;
+        std::vector<double> Pi;
+        d.rank = design_rank(p, XtX, dep, Pi);
+        if (d.rank < p &&
+            ((rc = upload(P, &ddep, dep.data(), dep.size(), s)) || (rc = upload(P, &dPi, Pi.data(), Pi.size(), s))))
+            return rc;
+    }
+    d.Pi = dPi;
     // f64 MFMA GLS tables
     double *G64p = nullptr, *coefR = nullptr, *dXp = nullptr, *dXtX = nullptr, *dPp = nullptr;
     // f64 MFMA tables for every width (the GLS runs on them for p <= 96);
@@ -288,17 +381,11 @@ static int plan_alloc(ffs_reml_plan* P, int32_t T, int32_t p, const double* X, c
     d.PG = p <= kGlsMfmaMaxP ? d.PGA : 0;
     {
         const int PG = d.PGA;
-        std::vector<double> Xp((size_t)d.Tpad * PG, 0.0), Pp((size_t)d.Tpad * PG, 0.0), XtX((size_t)p * p, 0.0);
+        std::vector<double> Xp((size_t)d.Tpad * PG, 0.0), Pp((size_t)d.Tpad * PG, 0.0);
         for (int t = 0; t < T; ++t)
             for (int j = 0; j < p; ++j) {
                 Xp[(size_t)t * PG + j] = X[(size_t)t * p + j];
                 Pp[(size_t)t * PG + j] = pinvT[(size_t)t * p + j];
-            }
-        for (int a = 0; a < p; ++a)
-            for (int b = 0; b < p; ++b) {
-                double acc = 0.0;
-                for (int t = 0; t < T; ++t) acc += X[(size_t)t * p + a] * X[(size_t)t * p + b];
-                XtX[(size_t)a * p + b] = acc;
             }
         if ((rc = P->alloc(&G64p, (size_t)K * d.Tpad * PG)) || (rc = upload(P, &dXp, Xp.data(), Xp.size(), s)) ||
             (rc = upload(P, &dPp, Pp.data(), Pp.size(), s)) || (rc = upload(P, &dXtX, XtX.data(), XtX.size(), s)))
@@ -308,7 +395,8 @@ static int plan_alloc(ffs_reml_plan* P, int32_t T, int32_t p, const double* X, c
     // recurrence tables for every p: K * Tpad * 32 bytes
     if ((rc = P->alloc(&coefR, (size_t)K * d.Tpad * 4))) return rc;
     HIP_TRY(hipMemsetAsync(coefR, 0, sizeof(double) * K * d.Tpad * 4, s));
-    *to = TablesOut{dX, soa, coef64, Xw64, G64, Lx, Ainv, Fm, work, ld, ldR, ldX, G32, Ghf, Glf, Ghb, Glb, g16s, G64p, coefR};
+    *to = TablesOut{dX,  soa, coef64, Xw64, G64, Lx,   Ainv,  Fm,  work, ld, ldR,
+                    ldX, G32, Ghf,    Glf,  Ghb, Glb,  g16s,  G64p, coefR, ddep, dPi};
     *ka = KTablesArgs{T, d.Tpad, p, K, dab, dwhiteflag, dgap, d.n_task, dtask, full_f, BN, npt, d.PGA, ridge};
     d.gap = dgap;
     d.ab = dab;
@@ -807,17 +895,38 @@ extern "C" int ffs_reml_gls_workspace(const ffs_reml_plan* P, int64_t V, int64_t
     return 0;
 }
 
+static int gls_impl(const ffs_reml_plan* P, const float* Y, int64_t ldY, const float* scale, const int32_t* best,
+                    int64_t V, float* betas, float* tstats, float* fstats, float* sigma2, float* r2, double* resid_w,
+                    int64_t ldR, float* resid, float* pred, int32_t* work, hipStream_t s);
+
 extern "C" int ffs_reml_gls(const ffs_reml_plan* P, const float* Y, int64_t ldY, const float* scale,
                             const int32_t* best, int64_t V, float* betas, float* tstats, float* fstats,
                             float* sigma2, float* r2, double* resid_w, int64_t ldR, float* resid, float* pred,
                             int32_t* work, void* stream) {
+    hipStream_t s = (hipStream_t)stream;
+    if (int rc = gls_impl(P, Y, ldY, scale, best, V, betas, tstats, fstats, sigma2, r2, resid_w, ldR, resid, pred, work,
+                          s))
+        return rc;
+    if (V > 0 && P->d.Pi) {
+        // rank-deficient design: minimum-norm beta and its statistics
+        hipLaunchKernelGGL(k_gls_rank, dim3((unsigned)V), dim3(64), 0, s, P->d, (int)V, scale, best, betas, tstats,
+                           fstats, sigma2, nullptr, nullptr);
+        HIP_TRY(hipGetLastError());
+    }
+    return 0;
+}
+
+extern "C" int32_t ffs_reml_plan_rank(const ffs_reml_plan* P) { return P ? P->d.rank : -1; }
+
+static int gls_impl(const ffs_reml_plan* P, const float* Y, int64_t ldY, const float* scale, const int32_t* best,
+                    int64_t V, float* betas, float* tstats, float* fstats, float* sigma2, float* r2, double* resid_w,
+                    int64_t ldR, float* resid, float* pred, int32_t* work, hipStream_t s) {
     if (!P) return fail(-3, "gls: null plan");
     if (P->ridge > 0.0) return fail(-3, "gls: a ridge plan is search-only");
     if (V <= 0) return 0;
     if (V > 0x7fffffff - 64 * (int64_t)P->d.K) return fail(-3, "gls: batch too large; split it");
     if (ldY < P->d.T) return fail(-3, "gls: ldY must be >= T");
     if ((resid_w || resid || pred) && ldR < P->d.T) return fail(-3, "gls: ldR must be >= T");
-    hipStream_t s = (hipStream_t)stream;
     const int K = P->d.K, p = P->d.p;
     hipLaunchKernelGGL(k_gls_invalid, dim3((unsigned)((V + 255) / 256)), dim3(256), 0, s, best, (int)V, K, p, betas,
                        tstats, fstats, sigma2, r2);
@@ -1473,6 +1582,61 @@ extern "C" int ffs_reml_whiten_cols(const ffs_reml_plan* P, const int32_t* pair,
     return 0;
 }
 
+extern "C" int ffs_reml_dsort_workspace(const ffs_reml_plan* P, int64_t V, int32_t q, int32_t n_f, int64_t* n_double) {
+    if (!P || !n_double) return fail(-3, "dsort_workspace: null argument");
+    *n_double = std::max<int64_t>(V, 0) * dsort_work_doubles(P->d.p + q, n_f, P->d.T);
+    return 0;
+}
+
+extern "C" int ffs_reml_dsort_gls(const ffs_reml_plan* P, const int32_t* pair, int64_t V, int32_t q, const double* D,
+                                  const double* Y, const int32_t* fci, int32_t n_f, double* beta, double* tstat,
+                                  double* sigma2, double* fstat, double* r2, double* Ainv, double* resid_w,
+                                  double* resid, double* pred, int32_t* info, double* work, int64_t work_doubles,
+                                  void* stream) {
+    if (!P) return fail(-3, "dsort_gls: null plan");
+    if (V <= 0) return 0;
+    if (V > 0x7fffffff) return fail(-3, "dsort_gls: batch too large; split it");
+    if (!P->d.coefR || !P->d.Xw64) return fail(-4, "dsort_gls: plan has no recurrence tables");
+    const int pe = P->d.p + q;
+    if (q < 0 || q > 63 || pe + 1 > kDsortMaxM)
+        return fail(-3, "dsort_gls: need 0 <= q <= 63 and p + q <= " + std::to_string(kDsortMaxM - 1));
+    if (n_f < 1 || n_f > pe || !fci) return fail(-3, "dsort_gls: need 1 <= n_f <= p + q F columns");
+    if (!pair || !Y || (q > 0 && !D) || !beta || !tstat || !sigma2 || !fstat || !r2 || !info || !work)
+        return fail(-3, "dsort_gls: null argument");
+    const int64_t ws = dsort_work_doubles(pe, n_f, P->d.T);
+    if (work_doubles < V * ws) return fail(-3, "dsort_gls: workspace too small (ffs_reml_dsort_workspace)");
+    DsortArgs A{};
+    A.pair = pair;
+    A.V = (int)V;
+    A.q = q;
+    A.D = D;
+    A.Y = Y;
+    A.n_f = n_f;
+    for (int i = 0; i < n_f; ++i) {
+        if (fci[i] < 0 || fci[i] >= pe) return fail(-3, "dsort_gls: F column out of range");
+        A.fci[i] = fci[i];
+    }
+    A.beta = beta;
+    A.tstat = tstat;
+    A.sigma2 = sigma2;
+    A.fstat = fstat;
+    A.r2 = r2;
+    A.Ai_out = Ainv;
+    A.rw = resid_w;
+    A.resid = resid;
+    A.pred = pred;
+    A.info = info;
+    A.work = work;
+    A.ws = ws;
+    const int m = pe + 1;
+    const size_t shm = sizeof(double) * ((size_t)m * (m + 1) / 2 + (size_t)kDsortChunk * m);
+    if (shm > 64 * 1024)
+        HIP_TRY(hipFuncSetAttribute((const void*)k_dsort_gls, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+    hipLaunchKernelGGL(k_dsort_gls, dim3((unsigned)V), dim3(64), shm, (hipStream_t)stream, P->d, A);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
 extern "C" int ffs_reml_plan_set_projection(ffs_reml_plan* P, int32_t mode) {
     if (!P) return fail(-3, "plan_set_projection: null plan");
     if (mode < 0 || mode > 2) return fail(-3, "plan_set_projection: mode must be 0 (fp32), 1 (bf16x3) or 2 (fp16x3)");
@@ -1510,6 +1674,7 @@ struct ffs_reml_group {
     // two graph branches; split projection on pre-split halves (p <= 32) or not
     int n_chunks = 1;
     bool presplit = true;
+    bool rank_deficient = false;  // some design has dependent columns
     std::string tuning;
     hipStream_t gs = nullptr, gs2 = nullptr;
     hipEvent_t ev_in = nullptr, ev_out = nullptr;
@@ -1583,7 +1748,10 @@ extern "C" int ffs_reml_group_create(ffs_reml_group** out, int32_t G, ffs_reml_p
     R->ncols = (int64_t)rows.size();
     if (R->ncols > 0x7fffffff) return fail(-3, "group_create: padded layout too large");
     std::vector<PlanDev> hp(G);
-    for (int g = 0; g < G; ++g) hp[g] = plans[g]->d;
+    for (int g = 0; g < G; ++g) {
+        hp[g] = plans[g]->d;
+        if (hp[g].Pi) R->rank_deficient = true;
+    }
     R->yt_rows = yt_rows(P0);
     R->work_ints = 3 * (int64_t)G * R->K + 1 + gls_perm_cap(V, G * R->K);
     int rc = 0;
@@ -1698,6 +1866,11 @@ static int group_enqueue(ffs_reml_group* R, const GroupIO& io, hipStream_t s) {
                                io.tstats, io.fstats, io.sigma2, io.r2, R->dplans, R->vgroup);
     }
     HIP_TRY(hipGetLastError());
+    if (R->rank_deficient) {
+        hipLaunchKernelGGL(k_gls_rank, dim3((unsigned)R->V), dim3(64), 0, s, P0->d, (int)R->V, io.scale, io.best,
+                           io.betas, io.tstats, io.fstats, io.sigma2, R->dplans, R->vgroup);
+        HIP_TRY(hipGetLastError());
+    }
     return 0;
 }
 

@@ -156,7 +156,8 @@ __global__ void __launch_bounds__(256) k_gls_wide(PlanDev P, const float* __rest
                 if (m > jj && m < p) part = fma(L[m * p + jj], bb[s2], part);
             }
             const double acc = __shfl(hv[s], 16 * q + j) - sum4(part);
-            if (kq == q) bb[s] = acc / L[jj * p + jj];
+            const double ljj = L[jj * p + jj];
+            if (kq == q) bb[s] = ljj != 0.0 ? acc / ljj : 0.0;  // zero pivot: basic solution
         }
     // quadratic forms and cross products
     double ra[NS];

@@ -87,6 +87,11 @@ struct TablesOut {
     uint16_t *Ghf, *Glf, *Ghb, *Glb;
     float* g16s;
     double *G64p, *coefR;
+    // rank-deficient design (null when X has full column rank): columns that
+    // depend on the columns before them, and Pi = projection onto the row
+    // space of X (p x p)
+    const int32_t* dep;
+    const double* Pi;
 };
 
 // Device-side description of the pair tables of one plan.
@@ -132,6 +137,12 @@ struct PlanDev {
     const double* pinvTp;      // [Tpad][PG]     pinv(X)', zero padded
     const double* coefR;       // [K][Tpad][4]   (sqrt(1/v_t), c_{t-1}, alpha_{t-1}, 0); c = alpha = 0 at run starts
     const double* XtX;         // [p][p]
+    // rank-deficient design: Pi = projection onto the row space of X (p x p);
+    // null when X has full column rank.  The tables then hold the rank-r
+    // forms (zero pivots and Q columns for the dependent columns, Ainv =
+    // (X'R^-1X)^+) and the reported beta is Pi times the basic solution.
+    const double* Pi;
+    int rank;
     int proj_mode;             // 0 fp32 MFMA, 1 bf16 x3, 2 fp16 x3
     const float* ecoef;        // [n_chunks][T][2*NB] (alpha_t, 1/v_{t+1}) per chunk slot
     const float* ec;           // [n_chunks][T] c_t of the chunk's AR coefficient

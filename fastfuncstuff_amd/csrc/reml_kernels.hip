@@ -35,22 +35,33 @@ typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 // every lane reads it after the same barrier, so all lanes leave together.
 
 // A (p x p, row-major, symmetric) -> lower factor L (upper part zeroed).
-// Returns -1 (flag = 1) when a pivot is not positive.
-__device__ int block_cholesky(int p, double* A, int* flag) {
+// Returns -1 (flag = 1) when a pivot is not positive.  dep (optional): the
+// columns marked there get a zero pivot and a zero column below it (the
+// rank-r factor of a design whose column j depends on columns 0..j-1).
+__device__ int block_cholesky(int p, double* A, int* flag, const int32_t* dep = nullptr) {
     const int tid = threadIdx.x;
     for (int j = 0; j < p; ++j) {
+        const bool skip = dep && dep[j];
         if (tid == 0) {
-            double d = A[j * p + j];
-            for (int k = 0; k < j; ++k) d -= A[j * p + k] * A[j * p + k];
-            if (!(d > 0.0))
-                *flag = 1;
-            else
-                A[j * p + j] = sqrt(d);
+            if (skip) {
+                A[j * p + j] = 0.0;
+            } else {
+                double d = A[j * p + j];
+                for (int k = 0; k < j; ++k) d -= A[j * p + k] * A[j * p + k];
+                if (!(d > 0.0))
+                    *flag = 1;
+                else
+                    A[j * p + j] = sqrt(d);
+            }
         }
         __syncthreads();
         if (*flag) return -1;
         const double d = A[j * p + j];
         for (int i = j + 1 + tid; i < p; i += 64) {
+            if (skip) {
+                A[i * p + j] = 0.0;
+                continue;
+            }
             double s = A[i * p + j];
             for (int k = 0; k < j; ++k) s -= A[i * p + k] * A[j * p + k];
             A[i * p + j] = s / d;
@@ -62,12 +73,17 @@ __device__ int block_cholesky(int p, double* A, int* flag) {
 }
 
 // Ainv = (L L')^-1 = L^-T L^-1; tmp (p x p) receives L^-1, one column per lane.
+// Zero pivots (dependent columns of a rank-r factor) get zero rows and
+// columns in L^-1, so Ainv is the rank-r inverse embedded at the independent
+// columns.
 __device__ void block_cholesky_inverse(int p, const double* L, double* Ainv, double* tmp) {
     const int tid = threadIdx.x;
     for (int j = tid; j < p; j += 64) {
         for (int i = 0; i < p; ++i) tmp[i * p + j] = 0.0;
+        if (L[j * p + j] == 0.0) continue;
         tmp[j * p + j] = 1.0 / L[j * p + j];
         for (int i = j + 1; i < p; ++i) {
+            if (L[i * p + i] == 0.0) continue;
             double s = 0.0;
             for (int k = j; k < i; ++k) s += L[i * p + k] * tmp[k * p + j];
             tmp[i * p + j] = -s / L[i * p + i];
@@ -81,6 +97,27 @@ __device__ void block_cholesky_inverse(int p, const double* L, double* Ainv, dou
         for (int k = i; k < p; ++k) s += tmp[k * p + i] * tmp[k * p + j];
         Ainv[i * p + j] = s;
         Ainv[j * p + i] = s;
+    }
+    __syncthreads();
+}
+
+// A <- Pi A Pi' (p x p, A symmetric); W (p x p) is workspace.
+__device__ void block_sandwich(int p, const double* Pi, double* A, double* W) {
+    const int tid = threadIdx.x;
+    for (int idx = tid; idx < p * p; idx += 64) {
+        const int i = idx / p, j = idx % p;
+        double s = 0.0;
+        for (int q = 0; q < p; ++q) s = fma(Pi[i * p + q], A[q * p + j], s);
+        W[idx] = s;
+    }
+    __syncthreads();
+    for (int idx = tid; idx < p * p; idx += 64) {
+        const int i = idx / p, j = idx % p;
+        if (j > i) continue;
+        double s = 0.0;
+        for (int q = 0; q < p; ++q) s = fma(W[i * p + q], Pi[j * p + q], s);
+        A[i * p + j] = s;
+        A[j * p + i] = s;
     }
     __syncthreads();
 }
@@ -223,7 +260,10 @@ __global__ void __launch_bounds__(64) k_tables(KTablesArgs A, const TablesOut* _
     if (ridge > 0.0)
         for (int j = tid; j < p; j += 64) L[j * p + j] += ridge;
     __syncthreads();
-    if (block_cholesky(p, L, &sh_fail) == 0) {
+    // rank-deficient designs (never ridge plans): the dependent columns get
+    // zero pivots (glm/arma.py:1493-1500 falls back to QR, whose R has a
+    // rounding-level diagonal there)
+    if (block_cholesky(p, L, &sh_fail, ridge > 0.0 ? nullptr : o.dep) == 0) {
         if (tid == 0) {
             double s = 0.0;
             if (ridge > 0.0)
@@ -233,6 +273,8 @@ __global__ void __launch_bounds__(64) k_tables(KTablesArgs A, const TablesOut* _
             sh_ldX = s;
         }
         block_cholesky_inverse(p, L, Ai, wk);
+        // (X'R^-1X)^+ = Pi (rank-r inverse) Pi'
+        if (o.Pi && ridge <= 0.0) block_sandwich(p, o.Pi, Ai, wk);
         if (ridge > 0.0) {
             // |y_w - X_w beta|^2 = E - c' W c, c = X_w' y_w, M = (A + rI)^-1,
             // W = M (A + 2rI) M = M + r M M; L <- chol(W)
@@ -292,7 +334,7 @@ __global__ void __launch_bounds__(64) k_tables(KTablesArgs A, const TablesOut* _
         for (int i = 0; i < p; ++i) {
             double s = x[i];
             for (int m = 0; m < i; ++m) s -= L[i * p + m] * q[m];
-            q[i] = s / L[i * p + i];
+            q[i] = L[i * p + i] != 0.0 ? s / L[i * p + i] : 0.0;  // zero pivot: dependent column
         }
     }
     __syncthreads();
@@ -1240,7 +1282,8 @@ __device__ inline void gls_body(const PlanDev& P, const float* __restrict__ Y, i
 #pragma unroll
         for (int m = jj + 1; m < PM; ++m)
             if (m < p) acc -= L[m * p + jj] * be[m];
-        be[jj] = acc / L[jj * p + jj];
+        const double ljj = L[jj * p + jj];
+        be[jj] = ljj != 0.0 ? acc / ljj : 0.0;  // zero pivot: basic solution
     }
     const double mean = sy / T;
     double ssw = 0.0, ssres = 0.0, sstot = 0.0;

@@ -346,7 +346,8 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) k_
         double acc = hj;
 #pragma unroll 1
         for (int m = jj + 1; m < p; ++m) acc -= L[m * p + jj] * be[m];
-        be[jj] = acc / L[jj * p + jj];
+        const double ljj = L[jj * p + jj];
+        be[jj] = ljj != 0.0 ? acc / ljj : 0.0;  // zero pivot: basic solution
     }
     __syncthreads();
 #pragma unroll
@@ -391,6 +392,51 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) k_
     fstats[v] = (float)(quad / (sig * P.n_task + 1e-10));
     sigma2[v] = (float)(sig * s * s);
     r2[v] = (float)(sstot <= 1e-10 ? 0.0 : fmin(1.0 - ssres / sstot, 1.0));
+}
+
+// Rank-deficient designs (plan Pi set): the GLS kernels leave the basic
+// solution (beta 0 on the dependent columns); the reported beta is the
+// minimum-norm solution Pi beta_basic, the solution of the reference's pinv
+// fallback (glm/arma.py:3410-3416), with t from diag (X'R^-1X)^+ and F from
+// the plan's quadratic form on that beta.  Fitted values, residuals, sigma^2
+// and R^2 are the same for every least-squares solution and stay as written.
+// One wave per voxel; grouped fits pass plans / vgroup.
+__global__ void __launch_bounds__(64) k_gls_rank(PlanDev P, int V, const float* __restrict__ scale,
+                                                 const int32_t* __restrict__ best, float* __restrict__ betas,
+                                                 float* __restrict__ tstats, float* __restrict__ fstats,
+                                                 const float* __restrict__ sigma2, const PlanDev* __restrict__ plans,
+                                                 const int32_t* __restrict__ vgroup) {
+    const int v = blockIdx.x;
+    if (v >= V) return;
+    if (plans) P = plans[vgroup[v]];
+    if (!P.Pi) return;
+    const int k = best[v];
+    if (k < 0 || k >= P.K) return;
+    const int p = P.p, tid = threadIdx.x;
+    __shared__ double bb[96], bm[96];
+    const double s = scale ? (double)scale[v] : 1.0;
+    for (int j = tid; j < p; j += 64) bb[j] = (double)betas[(size_t)v * p + j] / s;
+    __syncthreads();
+    for (int i = tid; i < p; i += 64) {
+        double acc = 0.0;
+        for (int j = 0; j < p; ++j) acc = fma(P.Pi[i * p + j], bb[j], acc);
+        bm[i] = acc;
+    }
+    __syncthreads();
+    const double sig = (double)sigma2[v] / (s * s);
+    const double* __restrict__ Ai = P.Ainv + (size_t)k * p * p;
+    const double* __restrict__ F = P.Fm + (size_t)k * p * p;
+    double part = 0.0;
+    for (int i = tid; i < p; i += 64) {
+        const double se = sqrt(sig * Ai[i * p + i]);
+        tstats[(size_t)v * p + i] = (float)(bm[i] / (se + 1e-10));
+        betas[(size_t)v * p + i] = (float)(bm[i] * s);
+        double r = 0.0;
+        for (int j = 0; j < p; ++j) r = fma(F[i * p + j], bm[j], r);
+        part = fma(bm[i], r, part);
+    }
+    for (int o = 32; o > 0; o >>= 1) part += __shfl_xor(part, o);
+    if (tid == 0) fstats[v] = (float)(part / (sig * P.n_task + 1e-10));
 }
 
 // NaN statistics for voxels whose pair index is not a valid pair.

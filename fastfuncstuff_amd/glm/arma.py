@@ -196,6 +196,44 @@ def timeline_gaps(n: int, run_starts=None, tau=None) -> np.ndarray:
     return gap
 
 
+class _RemlMode:
+    """Run-wide REML mode of glm/arma.py:59-92 (``set_afni_mode``).
+
+    Only the default mode is built: the exact dense ARMA(1,1) covariance, the
+    default grid filter and hierarchical top level.  The AFNI-faithful mode
+    changes the covariance itself (R truncated to AFNI's bmax off-diagonals,
+    glm/arma.py:918-926, 1075-1089), which the two-term innovations recurrence
+    of every kernel does not represent, so a search in that mode raises rather
+    than returning default-mode results under the AFNI-mode name."""
+
+    afni_faithful: bool = False
+    corcut: float = 1e-4
+
+
+_REML_MODE = _RemlMode()
+
+
+def set_afni_mode(enabled: bool, corcut: float = 1e-4) -> None:
+    """glm/arma.py:85-92.  ``enabled=False`` (the reference's default) is
+    accepted; ``enabled=True`` raises NotImplementedError (banded R is not
+    built)."""
+    if enabled:
+        raise NotImplementedError(
+            "AFNI-faithful REML mode (banded R, corcut grid filter, AFNI ltop) is not built; "
+            "only the default exact-covariance mode is available")
+    _REML_MODE.afni_faithful = False
+    _REML_MODE.corcut = float(corcut)
+
+
+def _check_reml_mode() -> None:
+    """Refuse a search when the mode object was switched to AFNI-faithful by
+    hand (set_afni_mode itself refuses it)."""
+    if _REML_MODE.afni_faithful:
+        raise NotImplementedError(
+            "AFNI-faithful REML mode (banded R, corcut grid filter, AFNI ltop) is not built; "
+            "set _REML_MODE.afni_faithful = False")
+
+
 def hierarchical_ltop(n_a: int, n_b: int) -> int:
     """glm/arma.py:2290-2296 (default, non-AFNI mode)."""
     return max(0, min(n_a, n_b).bit_length() - 2)
@@ -292,6 +330,7 @@ class RemlGrid(Mapping):
                     lattice) -> dict:
         """Host-side tables of the plan (pairs, lattice, timeline, pinv);
         returns the remaining plan-creation arguments."""
+        _check_reml_mode()
         self.device = _lib.require_device(device)
         X = torch.as_tensor(X)
         if X.ndim != 2:
@@ -361,6 +400,10 @@ class RemlGrid(Mapping):
         ldX = np.zeros(self.K)
         _lib.check(lib.ffs_reml_plan_logdets(handle, ldR.ctypes.data, ldX.ctypes.data), "plan_logdets")
         self.logdet_R, self.logdet_XRX = ldR, ldX
+        # column rank of the design: below p the plan holds the rank-r tables
+        # (the reference's QR fallback, glm/arma.py:1493-1508) and the GLS
+        # reports the minimum-norm beta
+        self.rank = int(lib.ffs_reml_plan_rank(handle))
         self.param_list = [(float(a), float(b)) for a, b in self.params.tolist()]
         self.projection = "fp32"
         self.bands = None
@@ -597,7 +640,11 @@ def precompute_reml_grid(X, n_timepoints, a_grid, b_grid, device=None, verbose=F
     X = torch.as_tensor(X)
     if X.shape[0] != n_timepoints:
         raise ValueError(f"Timepoints mismatch: Data has {n_timepoints}, Design has {X.shape[0]}")
-    return RemlGrid(X, a_grid, b_grid, device, run_starts=run_starts, tau=tau, task_indices=task_indices)
+    grid = RemlGrid(X, a_grid, b_grid, device, run_starts=run_starts, tau=tau, task_indices=task_indices)
+    if verbose and grid.rank < grid.p:
+        print(f"  design has rank {grid.rank} < {grid.p} columns: rank-deficient tables (QR-fallback log-det, "
+              "minimum-norm GLS)")
+    return grid
 
 
 def _to_device_f32(Y, device) -> torch.Tensor:
@@ -647,6 +694,7 @@ def select_arma_params_hierarchical_from_surface(surface: torch.Tensor, param_li
                                                  grid: RemlGrid | None = None):
     """glm/arma.py:2224-2322 on a (V, K) surface.  ``grid`` (optional) supplies
     the lattice; otherwise it is rebuilt from ``param_list``."""
+    _check_reml_mode()
     if grid is None or list(grid.param_list) != list(param_list):
         raise ValueError("select_arma_params_hierarchical_from_surface needs the RemlGrid the surface came from")
     S = surface.to(device=grid.device, dtype=torch.float32).contiguous()
@@ -1434,7 +1482,9 @@ def _dsort_pass(grid: RemlGrid, data, design, dsort, res, scale_all, dev, out_dt
     at the (a, b) selected without dsort.  The voxel's columns (design, its
     dsort series, its normalised data) are whitened on the device by the
     plan's recurrence of its pair (ffs_reml_whiten_cols), then solved per voxel
-    in fp64 (A + 1e-8 I by Cholesky).  Overwrites betas / tstats / sigma2 / r2 /
+    in fp64 (A + 1e-8 I by Cholesky) by the batched solve kernel
+    (ffs_reml_dsort_gls, one block per voxel); voxels whose Cholesky fails take
+    the reference's pinv(A) solve (torch on the device).  Overwrites betas / tstats / sigma2 / r2 /
     fstats / dof (and the GLT, partial R^2, column-set, Ljung-Box, residual and
     prediction outputs), fills dsort_betas / dsort_tstats; everything computed
     in the normalised units of the fit and unscaled like betas (:5827-5848).
@@ -1457,10 +1507,14 @@ def _dsort_pass(grid: RemlGrid, data, design, dsort, res, scale_all, dev, out_dt
     res.dsort_tstats = torch.zeros(V, q, dtype=out_dtype)
     fci = list(task_indices) if task_indices else list(range(p))
     fci_d = torch.as_tensor(fci, dtype=torch.long, device=dev)
+    fci_h = np.ascontiguousarray(fci, dtype=np.int32)
     eye_f = torch.eye(len(fci), dtype=f64, device=dev)
-    eye_e = torch.eye(pe, dtype=f64, device=dev)
-    m = pe + 1
-    nb = max(1, min(batch, max(1, (1 << 28) // (m * T * 8 * 3))))
+    want_ai = bool(column_sets) or C is not None
+    want_rw = lj is not None or want_residuals
+    ws1 = ctypes.c_int64()
+    _lib.check(lib.ffs_reml_dsort_workspace(grid.handle, 1, q, len(fci), ctypes.byref(ws1)), "dsort_workspace")
+    per_voxel = 8 * (ws1.value + (q + 3) * T + 2 * pe + (pe * pe if want_ai else 0))
+    nb = max(1, min(batch, max(1, (1 << 28) // per_voxel)))
     for b0 in range(0, V, nb):
         b1 = min(V, b0 + nb)
         B = b1 - b0
@@ -1475,32 +1529,34 @@ def _dsort_pass(grid: RemlGrid, data, design, dsort, res, scale_all, dev, out_dt
                     D[:, si * n_runs + r, t0:t1] = Dc[:, si, t0:t1]
         else:
             D = dsort[b0:b1].to(dev, f64)
-        cols = torch.cat([Xd.T.unsqueeze(0).expand(B, p, T), D, Y.unsqueeze(1)], dim=1).contiguous()
-        W = torch.empty_like(cols)
+        D = D.contiguous()
+        Y = Y.contiguous()
         pair = res.pair_index[b0:b1].to(dev, torch.int32).contiguous()
-        _lib.check(lib.ffs_reml_whiten_cols(grid.handle, _lib.ptr(pair), B, m, _lib.ptr(cols), T, _lib.ptr(W), st),
-                   "whiten_cols")
-        Xw = W[:, :pe, :].transpose(1, 2)  # (B, T, pe)
-        yw = W[:, pe, :]
-        A = Xw.transpose(1, 2) @ Xw + 1e-8 * eye_e
-        Lc, info = torch.linalg.cholesky_ex(A)
-        Ai = torch.cholesky_inverse(Lc)
-        if bool((info != 0).any()):
-            badv = info != 0
-            Ai[badv] = torch.linalg.pinv(A[badv])
-        beta = (Ai @ (Xw.transpose(1, 2) @ yw.unsqueeze(2))).squeeze(2)
-        rw = yw - (Xw @ beta.unsqueeze(2)).squeeze(2)
-        sig = rw.pow(2).sum(1) / dof
-        tst = beta / (torch.sqrt(sig[:, None] * torch.diagonal(Ai, dim1=1, dim2=2)) + 1e-10)
-        pred = beta[:, :p] @ Xd.T + torch.einsum("bq,bqt->bt", beta[:, p:], D)
-        ss_res = (Y - pred).pow(2).sum(1)
-        ss_tot = (Y - Y.mean(1, keepdim=True)).pow(2).sum(1)
-        r2 = torch.where(ss_tot <= 1e-10, torch.zeros_like(ss_tot),
-                         (1.0 - ss_res / ss_tot.clamp(min=1e-10)).clamp(max=1.0))
-        At = Ai.index_select(1, fci_d).index_select(2, fci_d)
-        bt = beta.index_select(1, fci_d)
-        quad = torch.einsum("bi,bij,bj->b", bt, torch.linalg.inv(At + 1e-8 * eye_f), bt)
-        Fv = quad / (sig * len(fci) + 1e-30)
+        beta = torch.empty(B, pe, dtype=f64, device=dev)
+        tst = torch.empty_like(beta)
+        sig, Fv, r2 = (torch.empty(B, dtype=f64, device=dev) for _ in range(3))
+        info = torch.empty(B, dtype=torch.int32, device=dev)
+        Ai = torch.empty(B, pe, pe, dtype=f64, device=dev) if want_ai else None
+        rw = torch.empty(B, T, dtype=f64, device=dev) if want_rw else None
+        pred = torch.empty(B, T, dtype=f64, device=dev) if (want_predicted or want_residuals) else None
+        work = torch.empty(B * ws1.value, dtype=f64, device=dev)
+        _lib.check(lib.ffs_reml_dsort_gls(grid.handle, _lib.ptr(pair), B, q, _lib.ptr(D), _lib.ptr(Y),
+                                          fci_h.ctypes.data, len(fci), _lib.ptr(beta), _lib.ptr(tst), _lib.ptr(sig),
+                                          _lib.ptr(Fv), _lib.ptr(r2), _lib.ptr(Ai), _lib.ptr(rw), None,
+                                          _lib.ptr(pred), _lib.ptr(info), _lib.ptr(work), B * ws1.value, st),
+                   "dsort_gls")
+        del work
+        bad = (info == 1).nonzero().flatten()
+        if bad.numel():
+            # Cholesky of A + 1e-8 I failed: the reference's pinv(A) solve (glm/arma.py:3410-3416)
+            _dsort_pinv(grid, lib, st, Xd, D[bad], Y[bad], pair[bad], fci_d, eye_f, dof,
+                        beta, tst, sig, Fv, r2, Ai, rw, pred, bad)
+        if bool((info == 2).any()):
+            nan = info == 2
+            for t_ in (beta, tst):
+                t_[nan] = float("nan")
+            for t_ in (sig, Fv, r2):
+                t_[nan] = float("nan")
         sc = s[:, None]
         bb, tb = beta[:, :p], tst[:, :p]
         res.betas[b0:b1].copy_(((bb[:, task_indices] if task_indices else bb) * sc))
@@ -1554,6 +1610,47 @@ def _dsort_pass(grid: RemlGrid, data, design, dsort, res, scale_all, dev, out_dt
             res.residuals_whitened[b0:b1].copy_(rw * sc)
         if want_predicted:
             res.predicted[b0:b1].copy_(pred * sc)
+
+
+def _dsort_pinv(grid, lib, st, Xd, D, Y, pair, fci_d, eye_f, dof, beta, tst, sig, Fv, r2, Ai, rw, pred, rows):
+    """-dsort voxels whose Cholesky of A + 1e-8 I failed: the reference's pinv
+    branch (glm/arma.py:3410-3416), A^+ = pinv(A) of A = E_w'E_w without the
+    1e-8 term, beta = A^+ E_w'y_w, the other outputs as the batched kernel
+    defines them.  The reference switches a whole sub-batch to pinv when one
+    voxel fails; per voxel here (the full-rank voxels' solutions differ by the
+    1e-8 ridge only)."""
+    f64 = torch.float64
+    B, q, T = D.shape
+    p = Xd.shape[1]
+    pe = p + q
+    cols = torch.cat([Xd.T.unsqueeze(0).expand(B, p, T), D, Y.unsqueeze(1)], dim=1).contiguous()
+    W = torch.empty_like(cols)
+    _lib.check(lib.ffs_reml_whiten_cols(grid.handle, _lib.ptr(pair.contiguous()), B, pe + 1, _lib.ptr(cols), T,
+                                        _lib.ptr(W), st), "whiten_cols")
+    Xw = W[:, :pe, :].transpose(1, 2)  # (B, T, pe)
+    yw = W[:, pe, :]
+    Ap = torch.linalg.pinv(Xw.transpose(1, 2) @ Xw)
+    b = (Ap @ (Xw.transpose(1, 2) @ yw.unsqueeze(2))).squeeze(2)
+    r_w = yw - (Xw @ b.unsqueeze(2)).squeeze(2)
+    s2 = r_w.pow(2).sum(1) / dof
+    pr = b[:, :p] @ Xd.T + torch.einsum("bq,bqt->bt", b[:, p:], D)
+    ss_res = (Y - pr).pow(2).sum(1)
+    ss_tot = (Y - Y.mean(1, keepdim=True)).pow(2).sum(1)
+    bt = b.index_select(1, fci_d)
+    At = Ap.index_select(1, fci_d).index_select(2, fci_d)
+    quad = torch.einsum("bi,bij,bj->b", bt, torch.linalg.inv(At + 1e-8 * eye_f), bt)
+    beta[rows] = b
+    tst[rows] = b / (torch.sqrt(s2[:, None] * torch.diagonal(Ap, dim1=1, dim2=2)) + 1e-10)
+    sig[rows] = s2
+    Fv[rows] = quad / (s2 * fci_d.numel() + 1e-30)
+    r2[rows] = torch.where(ss_tot <= 1e-10, torch.zeros_like(ss_tot),
+                           (1.0 - ss_res / ss_tot.clamp(min=1e-10)).clamp(max=1.0)).to(f64)
+    if Ai is not None:
+        Ai[rows] = Ap
+    if rw is not None:
+        rw[rows] = r_w
+    if pred is not None:
+        pred[rows] = pr
 
 
 def _partial_r2(t_full, r2, dof, p, task_indices, want_partial, partial_mode, want_semi, semi_mode) -> dict:
@@ -1898,25 +1995,19 @@ def compare_ols_vs_arma11(data, design, tr: float, device=None) -> dict:
     from .core import fit_glm
 
     dev = _lib.require_device(device)
-    print("Running OLS GLM...")
     ols = fit_glm(data, design, tr, device=dev, verbose=False)
-    print("\nRunning ARMA(1,1) GLM...")
     arma = fit_glm_arma11(data, design, tr, device=dev)
-    tstat_ratio = arma.tstats.abs().mean() / (ols.tstats.abs().mean() + 1e-10)
+    t_ols, t_arma = ols.tstats.abs().mean(), arma.tstats.abs().mean()
+    tstat_ratio = t_arma / (t_ols + 1e-10)
     r2_improvement = arma.r2.mean() - ols.r2.mean()
-    summary = f"""
-    OLS vs ARMA(1,1) Comparison:
-    ============================
-    OLS Mean R²:      {ols.r2.mean():.4f}
-    ARMA Mean R²:     {arma.r2.mean():.4f}
-    R² Improvement:   {r2_improvement:.4f}
-    OLS Mean |t|:     {ols.tstats.abs().mean():.3f}
-    ARMA Mean |t|:    {arma.tstats.abs().mean():.3f}
-    |t| Ratio:        {tstat_ratio:.3f}
-    ARMA Parameters:
-    Mean (a, b):      ({arma.arma_params[:, 0].mean():.3f}, {arma.arma_params[:, 1].mean():.3f})
-    Mean λ (lag-1):   {arma.arma_lambda.mean():.3f}
-    """
+    a_mean, b_mean = arma.arma_params[:, 0].mean(), arma.arma_params[:, 1].mean()
+    rows = [
+        ("", "OLS", "ARMA(1,1)", "ARMA vs OLS"),
+        ("mean R^2", f"{ols.r2.mean():.4f}", f"{arma.r2.mean():.4f}", f"{r2_improvement:+.4f}"),
+        ("mean |t|", f"{t_ols:.3f}", f"{t_arma:.3f}", f"x{tstat_ratio:.3f}"),
+    ]
+    summary = "\n".join(f"{r[0]:<10}{r[1]:>10}{r[2]:>12}{r[3]:>14}" for r in rows)
+    summary += f"\nARMA(1,1) mean (a, b) = ({a_mean:.3f}, {b_mean:.3f}), mean lag-1 correlation {arma.arma_lambda.mean():.3f}"
     print(summary)
     return {"ols": ols, "arma11": arma, "tstat_ratio": float(tstat_ratio), "r2_improvement": float(r2_improvement),
             "summary": summary}

@@ -58,8 +58,16 @@ const char* ffs_reml_last_error(void);
  *   ltop        top level of the hierarchical descent
  *   n_task, task_idx  host; F statistic over these columns (n_task = 0: all columns, F = b'Ab)
  *   stream      hipStream_t
- * Returns -2 on a rank-deficient whitened design (the reference's QR / inv
- * fallbacks are not provided).
+ * Rank-deficient designs (glm/arma.py:1493-1500 falls back to QR when the
+ * Cholesky of X_w'X_w fails; the GLS to inv, :4864-4873, or pinv, :3410-3416)
+ * build: a column whose residual against the columns before it has squared
+ * norm <= 1e-13 |x_j|^2 is dependent (whitening keeps the column space, so
+ * this is decided on X).  Its pivot is zero, so logdet X'R^-1X carries
+ * 2 log(1e-10) for it (the QR branch's |R_jj| + 1e-10 with a rounding-level
+ * R_jj), the projection is onto the rank-r column space, (X'R^-1X)^-1 becomes
+ * the pseudo-inverse and ffs_reml_gls reports the minimum-norm beta (the pinv
+ * solution).  Returns -2 only when a pivot of an independent column is not
+ * positive after whitening (or X is not finite).
  */
 int ffs_reml_plan_create(ffs_reml_plan** out, int32_t T, int32_t p, const double* X, const double* pinvT,
                          const int32_t* gap, int32_t K, const double* ab, const int32_t* white, int32_t n_a,
@@ -67,6 +75,11 @@ int ffs_reml_plan_create(ffs_reml_plan** out, int32_t T, int32_t p, const double
                          int32_t n_task, const int32_t* task_idx, void* stream);
 
 int ffs_reml_plan_destroy(ffs_reml_plan* plan);
+
+/* Column rank of the plan's design as decided at creation (p = full rank);
+ * -1 for a null plan.  No reference counterpart: the reference reports the
+ * QR fallback only as a verbose message (glm/arma.py:1507-1508). */
+int32_t ffs_reml_plan_rank(const ffs_reml_plan* plan);
 
 /* G plans of one timeline and pair list with G designs (X and pinvT: G x T x p,
  * host, row-major per design), for grouped fits (glm/arma.py:2836-3183 fits
@@ -190,6 +203,8 @@ int ffs_reml_select_f64(const ffs_reml_plan* plan, const double* S, int64_t ldS,
  * work: caller-provided int32 workspace of ffs_reml_gls_workspace(V) elements
  * (voxels grouped by pair; nullable = no grouping, slower).  Without series
  * outputs and with p <= 32 the fit runs on f64 MFMA over pair-uniform waves.
+ * On a rank-deficient plan beta is the minimum-norm solution (zero
+ * component in the null space of X), t uses diag (X'R^-1X)^+.
  */
 /* int32 elements of the GLS workspace for V voxels (glm/arma.py:4784-4811 groups
  * voxels by unique (a, b); the workspace holds that grouping). */
@@ -361,6 +376,29 @@ int ffs_reml_subset_search(const ffs_reml_plan* plan, const float* Y, int64_t ld
  * width. */
 int ffs_reml_whiten_cols(const ffs_reml_plan* plan, const int32_t* pair, int64_t V, int32_t m, const double* cols,
                          int64_t ld, double* out, void* stream);
+
+/*
+ * -dsort per-voxel GLS (glm/arma.py:3248-3539, the final fit redone on the
+ * extended design [X | D_v] at the pair selected without the dsort columns):
+ * one batched small solve per voxel on the device.  pair (V int32), D (V x q
+ * x T float64: each voxel's q dsort series, run-expanded), Y (V x T float64,
+ * the normalised series) and every output are device arrays; fci (host,
+ * n_f int32 in [0, p + q)) are the F columns.  Per voxel, with E = [X | D_v]
+ * and pe = p + q: A = E_w'E_w + 1e-8 I by Cholesky, beta = A^-1 E_w'y_w,
+ * sigma2 = |y_w - E_w beta|^2 / max(1, T - pe), tstat = beta /
+ * (sqrt(sigma2 diag A^-1) + 1e-10), fstat = b' inv(A^-1[fci, fci] + 1e-8 I) b /
+ * (sigma2 n_f + 1e-30) over b = beta[fci], r2 = CoD(y, E beta) in original
+ * space.  Outputs (float64): beta, tstat (V x pe), sigma2, fstat, r2 (V);
+ * nullable: Ainv (V x pe x pe), resid_w, resid, pred (V x T).  info[v] = 0,
+ * 1 when the Cholesky fails (the reference then solves with pinv(A); the
+ * caller does that for these voxels), 2 for a pair outside [0, K) (no
+ * outputs).  work: ffs_reml_dsort_workspace doubles.  q <= 63, p + q <= 127.
+ */
+int ffs_reml_dsort_workspace(const ffs_reml_plan* plan, int64_t V, int32_t q, int32_t n_f, int64_t* n_double);
+int ffs_reml_dsort_gls(const ffs_reml_plan* plan, const int32_t* pair, int64_t V, int32_t q, const double* D,
+                       const double* Y, const int32_t* fci, int32_t n_f, double* beta, double* tstat, double* sigma2,
+                       double* fstat, double* r2, double* Ainv, double* resid_w, double* resid, double* pred,
+                       int32_t* info, double* work, int64_t work_doubles, void* stream);
 
 /* idx[0..*count) = voxels with flags[v] != 0 (order unspecified); count is device memory. */
 int ffs_reml_compact(const int32_t* flags, int64_t V, int32_t* idx, int32_t* count, void* stream);

@@ -189,8 +189,24 @@ class Grid:
         return [(float(a), float(b)) for a, b in self.params.tolist()]
 
 
+def dependent_columns(X) -> list[int]:
+    """Columns of X that lie in the span of the columns before them: j such
+    that rank(X[:, :j+1]) == rank(X[:, :j]), each rank from the singular
+    values in float64 with the default relative tolerance (max(T, j+1) eps
+    sigma_max).  Pinned by construction in the tests (designs with known
+    duplicated and combined columns)."""
+    Xd = torch.as_tensor(X).to(torch.float64)
+    dep, prev = [], 0
+    for j in range(Xd.shape[1]):
+        r = int(torch.linalg.matrix_rank(Xd[:, :j + 1]))
+        if r == prev:
+            dep.append(j)
+        prev = r
+    return dep
+
+
 def precompute_grid(X, a_grid, b_grid, run_starts=None, tau=None, dtype=torch.float64,
-                    autocorr: Grid | None = None) -> Grid:
+                    autocorr: Grid | None = None, rank_fallback: str = "qr") -> Grid:
     """glm/arma.py:1306-1531 ``precompute_reml_grid`` (Cholesky-QR branch).
 
     X_w = L^-1 X;  L_xtx = chol(X_w'X_w);  Q = X_w L_xtx^-T;
@@ -198,6 +214,15 @@ def precompute_grid(X, a_grid, b_grid, run_starts=None, tau=None, dtype=torch.fl
     ``autocorr``: a grid of the same pairs, timeline and dtype whose whiteners
     and log det R are reused (the reference's ``autocorr_cache``,
     glm/arma.py:1306-1368, shared by the groups of a grouped fit).
+
+    ``rank_fallback`` for a design with dependent columns: "qr" runs the
+    reference's branch as written (Cholesky, and QR of X_w for every pair when
+    any Cholesky fails, glm/arma.py:1493-1500; ``extra["used_qr"]`` says which
+    ran).  "rank" is that QR branch in exact arithmetic: for a column in the
+    span of the columns before it Householder leaves R_jj = 0, so log det
+    X'R^-1X carries 2 log(1e-10) for it, and Q's column j is undetermined
+    (rounding picks some unit vector orthogonal to the columns before it); it
+    is taken as zero here, so Q spans exactly col(X_w) (the rank-r tables).
     """
     X = torch.as_tensor(X).to(dtype)
     T, p = X.shape
@@ -217,15 +242,27 @@ def precompute_grid(X, a_grid, b_grid, run_starts=None, tau=None, dtype=torch.fl
             a, b = float(params[k, 0]), float(params[k, 1])
             Linv[k], ldR[k] = whitener(a, b, T, run_starts, tau, dtype)
     Xw = Linv @ X
+    dep = dependent_columns(X) if rank_fallback == "rank" else []
+    if dep:
+        ind = [j for j in range(p) if j not in dep]
+        Xr = Xw[:, :, ind]
+        Lr = torch.linalg.cholesky(Xr.transpose(1, 2) @ Xr)
+        Q = torch.zeros_like(Xw)
+        Q[:, :, ind] = torch.linalg.solve_triangular(Lr, Xr.transpose(1, 2), upper=False).transpose(1, 2)
+        ldX = (2 * torch.log(torch.abs(torch.diagonal(Lr, dim1=1, dim2=2)) + 1e-10).sum(1)
+               + 2 * len(dep) * math.log(1e-10))
+        return Grid(params, Linv, Xw, Q, ldR, ldX, dtype, extra={"dependent": dep, "used_qr": False})
+    used_qr, Rq = False, None
     try:
         Lx = torch.linalg.cholesky(Xw.transpose(1, 2) @ Xw)
         Q = torch.linalg.solve_triangular(Lx, Xw.transpose(1, 2), upper=False).transpose(1, 2)
         diag = torch.diagonal(Lx, dim1=1, dim2=2)
     except torch.linalg.LinAlgError:  # glm/arma.py:1497-1500
+        used_qr = True
         Q, Rq = torch.linalg.qr(Xw)
         diag = torch.diagonal(Rq, dim1=1, dim2=2)
     ldX = 2 * torch.log(torch.abs(diag) + 1e-10).sum(1)
-    return Grid(params, Linv, Xw, Q, ldR, ldX, dtype)
+    return Grid(params, Linv, Xw, Q, ldR, ldX, dtype, extra={"used_qr": used_qr, "qr_R": Rq})
 
 
 def design_pinv(X: torch.Tensor) -> torch.Tensor:
@@ -555,13 +592,18 @@ class OracleFit:
 
 
 def gls_refit(X, Y, grid: Grid, pair_index, task_indices=None, dtype=torch.float64,
-              want_ljung_box=False, lj_max_lag=None, lj_tau=None):
+              want_ljung_box=False, lj_max_lag=None, lj_tau=None, rank_fallback: str = "inv"):
     """glm/arma.py:4784-5601, X'X branch (use_qr=False), restated per voxel group.
 
     A = X_w'X_w; A^-1 via Cholesky; beta = (y_w X_w) A^-T; sigma2 =
     sum(resid_w^2)/dof (float64 sum); t = beta/(sqrt(sigma2 diag A^-1)+1e-10);
     F = b_task' inv(A^-1_task + 1e-8 I) b_task / (sigma2 p_task + 1e-10), or
-    b' A b / (sigma2 p + 1e-10) with no task subset; R^2 = CoD(y, X beta)."""
+    b' A b / (sigma2 p + 1e-10) with no task subset; R^2 = CoD(y, X beta).
+    ``rank_fallback``: "inv" is the group path's fallback when the Cholesky
+    fails (glm/arma.py:4864-4873); "pinv" takes A^+ (torch.linalg.pinv,
+    hermitian) whenever X has dependent columns, the -dsort fallback
+    (glm/arma.py:3410-3416): beta is then the minimum-norm solution."""
+    rank_def = rank_fallback == "pinv" and bool(dependent_columns(X))
     X = torch.as_tensor(X).to(dtype)
     Y = Y.to(dtype)
     T, p = X.shape
@@ -579,10 +621,13 @@ def gls_refit(X, Y, grid: Grid, pair_index, task_indices=None, dtype=torch.float
         Linv = grid.Linv[k].to(dtype)
         Xw = Linv @ X
         A = Xw.T @ Xw
-        try:
-            Ainv = torch.cholesky_inverse(torch.linalg.cholesky(A))
-        except torch.linalg.LinAlgError:
-            Ainv = torch.linalg.inv(A)
+        if rank_def:
+            Ainv = torch.linalg.pinv(A, hermitian=True)
+        else:
+            try:
+                Ainv = torch.cholesky_inverse(torch.linalg.cholesky(A))
+            except torch.linalg.LinAlgError:
+                Ainv = torch.linalg.inv(A)
         yw = Y[m] @ Linv.T
         b = (yw @ Xw) @ Ainv.T
         res = yw - b @ Xw.T

@@ -261,6 +261,38 @@ def test_dsort_pass_matches_oracle(dev):
     assert rel(res.set_fstats[:, 1], t[:, 5] ** 2) <= 1e-4
 
 
+def test_dsort_series_outputs_match_oracle(dev):
+    """Series outputs of the batched -dsort solve (glm/arma.py:3418-3430,
+    3522-3528): whitened residuals, residuals and prediction of the extended
+    fit in original units vs the FP64 oracle (abs <= 1e-5 of the voxel sd),
+    and the Ljung-Box statistic recomputed on the extended fit's whitened
+    residuals (rel <= 1e-5 against the oracle's on the oracle residuals)."""
+    from fastfuncstuff_amd.glm.arma import fit_glm_arma11
+
+    print("[fit] dsort series outputs", flush=True)
+    rng = np.random.default_rng(18)
+    T, V, q = 120, 100, 3
+    X = _design(rng, T)
+    Y = _data(rng, T, X, V)
+    D = rng.standard_normal((V, q, T)).astype(np.float32)
+    Y = Y + 0.4 * D[:, 1, :]
+    ag = bg = torch.linspace(-0.8, 0.8, 9)
+    res = fit_glm_arma11(Y, X, 2.0, a_grid=ag, b_grid=bg, device=dev, dsort=D, want_residuals=True,
+                         want_predicted=True, want_ljung_box=True)
+    idx = res.pair_index.long()
+    s = torch.from_numpy(Y).double().std(dim=1).clamp(min=1e-8)
+    g64 = O.precompute_grid(torch.from_numpy(X).double(), ag, bg, None, None, F64)
+    Yn = torch.from_numpy(Y).double() / s[:, None]
+    b, t, s2, r2, F, rw, pred, _ = O.dsort_gls(torch.from_numpy(X).double(), torch.from_numpy(D).double(), Yn, g64,
+                                               idx)
+    sc = s[:, None]
+    assert float(((res.residuals_whitened.double() - rw * sc).abs() / sc).max()) <= 1e-5
+    assert float(((res.predicted.double() - pred * sc).abs() / sc).max()) <= 1e-5
+    assert float(((res.residuals.double() - (Yn - pred) * sc).abs() / sc).max()) <= 1e-5
+    lb = O.ljung_box(rw, O.ljung_box_max_lag(T, X.shape[1], T))
+    assert float(((res.ljung_box.double() - lb).abs() / lb.abs().clamp(min=1)).max()) <= 1e-5
+
+
 def test_ols_baseline_matches_oracle(dev):
     """want_ols (glm/arma.py:4288-4323 -> glm/core.py fit_glm): the OLS fit
     that runs alongside, on the engine's GLS kernel at the white pair, vs the
@@ -518,3 +550,84 @@ def test_grouped_engine_matches_per_group_fits(dev, p):
     assert bool((second["best"] == fresh["best"]).all()) and bool((second["betas"] == fresh["betas"]).all())
     assert not bool((second["betas"] == first["betas"]).all())
     engine.close()
+
+
+def _deficient_design(rng, T, p):
+    """Column 2 duplicates column 1 and column p - 2 is column 0 + 2 column 3
+    (two dependent columns, rank p - 2); the others are smooth plus noise."""
+    t = np.linspace(0, 1, T)
+    base = np.column_stack([np.sin((j + 1) * np.pi * t) + 0.3 * rng.standard_normal(T) for j in range(p - 2)])
+    cols = [base[:, 0], base[:, 1], base[:, 1].copy()] + [base[:, j] for j in range(2, p - 3)]
+    cols += [base[:, 0] + 2.0 * base[:, 2], base[:, p - 3]]
+    return np.column_stack(cols)
+
+
+@pytest.mark.parametrize("p", [6, 40])
+def test_rank_deficient_design_matches_oracle(dev, p):
+    """A design with a duplicated and a combined column (rank p - 2) builds
+    instead of raising (glm/arma.py:1493-1500 falls back to QR): rank p - 2;
+    log dets equal the oracle's rank-r tables (the QR branch with its
+    rounding-picked directions removed) to 1e-8; the full-grid surface is
+    within 5e-4 + 4e-6 |S| of the FP64 oracle surface on those tables (the
+    float32 search at the (0.8, 0.8) corner of the grid is 3.2e-3 off at
+    |S| = 844 on this data with the full-rank reduced design too); at the
+    engine's pairs the betas are the minimum-norm solution (the pinv fallback,
+    glm/arma.py:3410-3416) and t, F, sigma^2, R^2 match the FP64 oracle refit
+    (betas / t rel <= 1e-5, F / sigma^2 rel <= 1e-5, R^2 rel <= 1e-6).  The
+    grouped engine with this design and a full-rank one gives the same for
+    the deficient group (p = 6 uses the f64 MFMA GLS, p = 40 the wide GLS)."""
+    from fastfuncstuff_amd.glm.arma import RemlGrid, fit_glm_arma11, fit_glm_arma11_grouped
+
+    print(f"[fit] rank-deficient design p={p}", flush=True)
+    rng = np.random.default_rng(40 + p)
+    T, V = 150, 300
+    X = _deficient_design(rng, T, p)
+    Y = _data(rng, T, X, V)
+    ag = bg = torch.linspace(-0.8, 0.8, 9)
+    X64, Y64 = torch.from_numpy(X), torch.from_numpy(Y).double()
+    gr = O.precompute_grid(X64, ag, bg, rank_fallback="rank")
+    grid = RemlGrid(X, ag, bg, dev)
+    assert grid.rank == p - 2
+    ld_ref = gr.logdet_XRX.numpy()
+    assert np.abs(grid.logdet_XRX - ld_ref).max() <= 1e-8 * max(1.0, np.abs(ld_ref).max())
+    grid.close()
+
+    s = Y64.std(dim=1).clamp(min=1e-8)
+    Yn = Y64 / s[:, None]
+    S64 = O.score_surface(X64, Yn, gr)
+
+    def check(res, rows):
+        S = res.reml_lklhd_surface.double() if res.reml_lklhd_surface is not None else None
+        if S is not None:
+            err = (S - S64[rows]).abs()
+            print(f"[fit] rank-deficient p={p}: surface error vs FP64 max {float(err.max()):.3e}", flush=True)
+            assert bool((err <= 5e-4 + 4e-6 * S64[rows].abs()).all()), float(err.max())
+        idx = res.pair_index.long()
+        b, t, f, s2, r2, _, _ = O.gls_refit(X64, Yn[rows], gr, idx, rank_fallback="pinv")
+        sc = s[rows]
+        b = b * sc[:, None]
+        den = torch.maximum(b.abs(), 1e-3 * sc[:, None])
+        assert float(((res.betas.double() - b).abs() / den).max()) <= 1e-5
+        assert float(((res.tstats.double() - t).abs() / t.abs().clamp(min=1)).max()) <= 1e-5
+        assert float(((res.fstats.double() - f).abs() / f.abs().clamp(min=1)).max()) <= 1e-5
+        assert float(((res.sigma2.double() - s2 * sc**2).abs() / (s2 * sc**2)).max()) <= 1e-5
+        # R^2 is far below 0 here (no intercept column): float32 output, relative bar
+        assert float(((res.r2.double() - r2).abs() / r2.abs().clamp(min=1)).max()) <= 1e-6
+
+    res = fit_glm_arma11(Y, X, 2.0, a_grid=ag, b_grid=bg, device=dev, save_profile_likelihoods=True)
+    check(res, torch.arange(V))
+    # grouped engine: this design for voxels with gid 1, a full-rank one for gid 0
+    gid = rng.integers(0, 2, V)
+    Xf = rng.standard_normal((T, p))
+    res_g = fit_glm_arma11_grouped(Y, {0: Xf, 1: X}, torch.from_numpy(gid), 2.0, a_grid=ag, b_grid=bg, device=dev,
+                                   save_profile_likelihoods=True)
+    m = torch.from_numpy(gid == 1)
+
+    class Sub:
+        pass
+
+    sub = Sub()
+    for name in ("pair_index", "betas", "tstats", "fstats", "sigma2", "r2", "reml_lklhd_surface"):
+        val = getattr(res_g, name)
+        setattr(sub, name, None if val is None else val[m])
+    check(sub, m.nonzero().flatten())

@@ -8,6 +8,8 @@ R^-1 solve, and brute force.
 
 from fractions import Fraction
 
+import math
+
 import numpy as np
 import pytest
 import torch
@@ -588,3 +590,93 @@ def test_subset_gemm_descent_equals_subset_rule_on_surface():
     ar = torch.arange(V)
     assert bool(((S[ar, k_s] - S[ar, k_g]).abs()[diff] <= 1e-4).all())
     assert bool(((s_s - s_g).abs() <= 1e-4 * s_s.abs().clamp(min=1)).all())
+
+
+def _deficient_design(rng, T, p):
+    """T x p design whose column 2 duplicates column 1 and whose column p - 2
+    is column 0 + 2 * column 3 (two dependent columns, rank p - 2)."""
+    base = rng.standard_normal((T, p - 2))
+    cols = [base[:, 0], base[:, 1], base[:, 1].copy()] + [base[:, j] for j in range(2, p - 3)]
+    cols += [base[:, 0] + 2.0 * base[:, 2], base[:, p - 3]]
+    return np.column_stack(cols)
+
+
+def test_dependent_columns_of_constructed_designs():
+    rng = np.random.default_rng(20)
+    assert O.dependent_columns(_deficient_design(rng, 50, 6)) == [2, 4]
+    assert O.dependent_columns(_deficient_design(rng, 80, 12)) == [2, 10]
+    X = rng.standard_normal((40, 4))
+    X[:, 0] = 0.0  # a zero column depends on the empty set
+    assert O.dependent_columns(X) == [0]
+    assert O.dependent_columns(rng.standard_normal((40, 5))) == []
+
+
+def test_rank_tables_match_the_reference_qr_branch():
+    """The rank-r tables against the reference's branch run as written on a
+    design with dependent columns: its Cholesky fails on some pair, so every
+    pair takes the QR branch (glm/arma.py:1493-1500), whose log det carries
+    log(|R_jj| + 1e-10) with a rounding-level R_jj, and Q's column j is a
+    rounding-picked direction orthogonal to col(X_w) that also takes a share
+    of every later column (so the branch's log det is rounding-determined at
+    the 0.1 level here).  Pins: X_w's independent columns are Q R[:, I]
+    exactly, so log det X_I'R^-1X_I = log det R[:, I]' R[:, I], which the
+    rank tables carry plus 2 log(1e-10) per dependent column; the branch's own
+    log det is below that (Cauchy-Binet); Q of the rank tables is an orthonormal basis of col(X_w) (zero columns at
+    the dependent positions), contained in the span of the QR branch's Q, and
+    so its projection of any y is at most the QR branch's."""
+    rng = np.random.default_rng(21)
+    T, p = 60, 6
+    X = torch.from_numpy(_deficient_design(rng, T, p))
+    ag = bg = torch.linspace(-0.6, 0.6, 5)
+    gq = O.precompute_grid(X, ag, bg)
+    gr = O.precompute_grid(X, ag, bg, rank_fallback="rank")
+    assert gq.extra["used_qr"] and gr.extra["dependent"] == [2, 4]
+    ind = [0, 1, 3, 5]
+    R = gq.extra["qr_R"]
+    rdep = torch.diagonal(R, dim1=1, dim2=2)[:, [2, 4]].abs()
+    assert float(rdep.max()) < 1e-10
+    RI = R[:, :, ind]
+    ld_exact = torch.logdet(RI.transpose(1, 2) @ RI) + 2 * 2 * math.log(1e-10)
+    assert float((gr.logdet_XRX - ld_exact).abs().max()) < 1e-8
+    excess = 2 * (torch.log(rdep + 1e-10) - math.log(1e-10)).sum(1)
+    assert bool((gq.logdet_XRX - excess <= gr.logdet_XRX + 1e-9).all())
+    y = torch.from_numpy(rng.standard_normal(T))
+    for k in range(gr.params.shape[0]):
+        Qr, Qq, Xw = gr.Q[k], gq.Q[k], gr.Xw[k]
+        assert float(Qr[:, [2, 4]].abs().max()) == 0.0
+        Qi = Qr[:, ind]
+        assert torch.allclose(Qi.T @ Qi, torch.eye(4, dtype=F64), atol=1e-12)
+        assert torch.allclose(Qi @ (Qi.T @ Xw), Xw, atol=1e-10)
+        assert torch.allclose(Qq @ (Qq.T @ Qi), Qi, atol=1e-8)
+        yw = gr.Linv[k] @ y
+        assert float((Qq.T @ yw).pow(2).sum()) >= float((Qi.T @ yw).pow(2).sum()) - 1e-10
+
+
+def test_pinv_gls_is_the_minimum_norm_least_squares_fit():
+    """gls_refit(rank_fallback="pinv") on a deficient design: beta equals the
+    minimum-norm least-squares solution of X_w beta = y_w (LAPACK gelsd via
+    torch.linalg.lstsq), the fitted values and sigma^2 equal those of the
+    full-rank fit on the independent columns, and the dependent direction
+    of beta is zero (beta is orthogonal to the null space of X)."""
+    rng = np.random.default_rng(22)
+    T, p, V = 70, 6, 9
+    X = torch.from_numpy(_deficient_design(rng, T, p))
+    Y = torch.from_numpy(rng.standard_normal((V, T)))
+    ag = bg = torch.linspace(-0.6, 0.6, 5)
+    gr = O.precompute_grid(X, ag, bg, rank_fallback="rank")
+    idx = torch.from_numpy(rng.integers(0, gr.params.shape[0], V))
+    b, t, f, s2, r2, _, rw = O.gls_refit(X, Y, gr, idx, rank_fallback="pinv")
+    ind = [0, 1, 3, 5]
+    g4 = O.precompute_grid(X[:, ind], ag, bg)
+    b4, _, _, s2_4, r2_4, _, _ = O.gls_refit(X[:, ind], Y, g4, idx)
+    null = torch.tensor([[0, 1, -1, 0, 0, 0], [1, 0, 0, 2, -1, 0]], dtype=F64)  # X null @ = 0
+    assert float((X @ null.T).abs().max()) < 1e-12
+    for v in range(V):
+        k = int(idx[v])
+        Xw, yw = gr.Xw[k], gr.Linv[k] @ Y[v]
+        mn = torch.linalg.lstsq(Xw, yw.unsqueeze(1), driver="gelsd").solution.squeeze(1)
+        assert torch.allclose(b[v], mn, rtol=1e-9, atol=1e-11)
+    assert float((b @ null.T).abs().max()) < 1e-10
+    assert torch.allclose(b @ X.T, b4 @ X[:, ind].T, atol=1e-10)
+    assert torch.allclose(s2 * (T - p), s2_4 * (T - 4), rtol=1e-10)
+    assert torch.allclose(r2, r2_4, atol=1e-12)
