@@ -81,6 +81,8 @@ SIGNATURES: dict[str, list] = {
     "ffs_mlp_last_error": [],
     "ffs_mlp_forward": [_vp, _vp, _vp, _i64, _vp, _vp],
     "ffs_mlp_grad": [_vp, _vp, _vp, _vp, _i64, _vp, _vp, _vp, _vp],
+    "ffs_mlp_train_steps": [_vp, _vp, _vp, _vp, _vp, _vp, _i64, _c_int, _c_int, _c_int, ctypes.c_float,
+                            ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float, _vp, _vp, _vp],
     "ffs_mlp_adam": [_vp, _vp, _vp, _vp, _i64, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float,
                      _c_int, _vp],
 }

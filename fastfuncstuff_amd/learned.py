@@ -97,30 +97,29 @@ class ProposalMLP:
         dev = self.device
         n = feats.shape[0]
         rng = np.random.default_rng(seed)
+        batch = max(1, min(batch, n))
         m = torch.zeros_like(self.params)
         v = torch.zeros_like(self.params)
-        grad = torch.empty_like(self.params)
-        loss = torch.empty(1, dtype=torch.float32, device=dev)
         ws = torch.empty(int(lib.ffs_mlp_workspace_floats(batch)), dtype=torch.float32, device=dev)
         steps_per_epoch = max(1, n // batch)
         total = epochs * steps_per_epoch
         log = TrainLog()
         st = _lib.stream_handle(dev)
+        torch.cuda.synchronize(dev)  # the timing covers the training only, not work queued before it
         t0 = time.perf_counter()
         step = 0
+        losses = torch.empty(steps_per_epoch, dtype=torch.float32, device=dev)
         for ep in range(epochs):
             perm = torch.from_numpy(rng.permutation(n).astype(np.int32)).to(dev)
-            for s in range(steps_per_epoch):
-                idx = perm[s * batch:(s + 1) * batch]
-                step += 1
-                frac = (step - 1) / max(1, total - 1)
-                lr_t = lr * (lr_final / lr) ** frac
-                _lib.check(lib.ffs_mlp_grad(_lib.ptr(self.params), _lib.ptr(feats), _lib.ptr(target), _lib.ptr(idx),
-                                            batch, _lib.ptr(grad), _lib.ptr(loss), _lib.ptr(ws), st), "mlp_grad")
-                _lib.check(lib.ffs_mlp_adam(_lib.ptr(self.params), _lib.ptr(grad), _lib.ptr(m), _lib.ptr(v),
-                                            self.n_params, lr_t, 0.9, 0.999, 1e-8, step, st), "mlp_adam")
-                if log_every and step % log_every == 0:
-                    log.losses.append((step, float(loss.item())))
+            # one epoch of steps launched from the library (no host loop per step)
+            _lib.check(lib.ffs_mlp_train_steps(_lib.ptr(self.params), _lib.ptr(m), _lib.ptr(v), _lib.ptr(feats),
+                                               _lib.ptr(target), _lib.ptr(perm), batch, steps_per_epoch, step, total,
+                                               lr, lr_final, 0.9, 0.999, 1e-8, _lib.ptr(losses), _lib.ptr(ws), st),
+                       "mlp_train_steps")
+            if log_every:
+                lh = losses.cpu().tolist()
+                log.losses += [(step + i + 1, lh[i]) for i in range(steps_per_epoch) if (step + i + 1) % log_every == 0]
+            step += steps_per_epoch
         torch.cuda.synchronize(dev)
         log.seconds = time.perf_counter() - t0
         log.steps = step

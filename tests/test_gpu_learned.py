@@ -85,6 +85,52 @@ def test_mlp_forward_grad_adam_match_oracle(dev):
     np.testing.assert_allclose(mlp.params.cpu().double().numpy(), th_ref, rtol=1e-6, atol=1e-7)
 
 
+def test_train_steps_equal_grad_and_adam_steps(dev):
+    """ffs_mlp_train_steps (an epoch of steps launched from the library, the
+    reduction and Adam fused) is bitwise the sequence ffs_mlp_grad +
+    ffs_mlp_adam at the same learning-rate schedule, losses included."""
+    from fastfuncstuff_amd import _lib
+
+    lib = _lib.load()
+    st = _lib.stream_handle(dev)
+    rng = np.random.default_rng(5)
+    n, B, steps, total = 3000, 500, 5, 9
+    theta = (M.init_params(6) + 0.05 * rng.standard_normal(M.N_PARAMS)).astype(np.float32)
+    X = torch.from_numpy(rng.uniform(-0.6, 0.95, (n, M.N_IN)).astype(np.float32)).to(dev)
+    Tg = torch.from_numpy(rng.uniform(-0.8, 0.8, (n, 2)).astype(np.float32)).to(dev)
+    perm = torch.from_numpy(rng.permutation(n).astype(np.int32)).to(dev)
+    ws = torch.empty(int(lib.ffs_mlp_workspace_floats(B)), dtype=torch.float32, device=dev)
+    lr, lr_final, step0 = 3e-3, 1e-4, 2
+
+    def fresh():
+        p = torch.from_numpy(theta.copy()).to(dev)
+        return p, torch.zeros_like(p), torch.zeros_like(p)
+
+    p1, m1, v1 = fresh()
+    losses = torch.empty(steps, dtype=torch.float32, device=dev)
+    _lib.check(lib.ffs_mlp_train_steps(_lib.ptr(p1), _lib.ptr(m1), _lib.ptr(v1), _lib.ptr(X), _lib.ptr(Tg),
+                                       _lib.ptr(perm), B, steps, step0, total, lr, lr_final, 0.9, 0.999, 1e-8,
+                                       _lib.ptr(losses), _lib.ptr(ws), st), "train_steps")
+    p2, m2, v2 = fresh()
+    g = torch.empty_like(p2)
+    loss = torch.empty(1, dtype=torch.float32, device=dev)
+    ref_losses = []
+    for i in range(steps):
+        step = step0 + i + 1
+        lr_t = float(np.float32(lr) * np.float32(np.float32(lr_final) / np.float32(lr))
+                     ** np.float32((step - 1) / (total - 1)))
+        _lib.check(lib.ffs_mlp_grad(_lib.ptr(p2), _lib.ptr(X), _lib.ptr(Tg), _lib.ptr(perm[i * B:(i + 1) * B]), B,
+                                    _lib.ptr(g), _lib.ptr(loss), _lib.ptr(ws), st), "grad")
+        ref_losses.append(float(loss.item()))
+        _lib.check(lib.ffs_mlp_adam(_lib.ptr(p2), _lib.ptr(g), _lib.ptr(m2), _lib.ptr(v2), M.N_PARAMS, lr_t, 0.9,
+                                    0.999, 1e-8, step, st), "adam")
+    assert losses.cpu().tolist() == ref_losses
+    # the learning rate is formed in float32 on both sides (powf vs numpy
+    # float32 power may differ in the last bit): parameters to 1 ulp-level
+    torch.testing.assert_close(p1, p2, rtol=2e-6, atol=1e-8)
+    torch.testing.assert_close(m1, m2, rtol=0, atol=0)
+
+
 def _grid_and_data(dev, V, T=120, seed=0, noise_seed=None):
     from fastfuncstuff_amd import synth
     from fastfuncstuff_amd.glm.arma import RemlGrid
