@@ -5,7 +5,6 @@
 #include <cmath>
 #include <numeric>
 #include <cstdio>
-#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <mutex>
@@ -58,12 +57,10 @@ struct ffs_reml_plan {
     const double* ldR = nullptr;
     const double* ldX = nullptr;
     double ridge = 0.0;  // > 0: search-only plan with the ridge-regularised score
-    // second stream + events of ffs_reml_surface and of the pair-set passes
-    // (created on first use): ev[0..16] surface chunks, ev[17..18] set passes
+    // second stream + events of ffs_reml_surface (created on first use)
     static constexpr int kMaxSurfaceChunks = 16;
-    static constexpr int kSetEv = kMaxSurfaceChunks + 1;
     mutable hipStream_t aux = nullptr;
-    mutable hipEvent_t ev[kMaxSurfaceChunks + 3] = {};
+    mutable hipEvent_t ev[kMaxSurfaceChunks + 1] = {};
     // Yt buffer whose rows Tpad..2 Tpad-1 the last ffs_reml_prep filled with
     // split halves (projection reads them only for this buffer)
     mutable const float* split_yt = nullptr;
@@ -125,34 +122,6 @@ static int plan_create_impl(ffs_reml_plan** out, int32_t T, int32_t p, const dou
                             const int32_t* gap, int32_t K, const double* ab, const int32_t* white, int32_t n_a,
                             int32_t n_b, const int32_t* lut, int32_t zero_a, int32_t zero_b, int32_t ltop,
                             int32_t n_task, const int32_t* task_idx, double ridge, void* stream);
-
-// Block caps of the concurrent set-pass launches (energy on the VALU,
-// projection on the matrix cores); FFS_SET_ENERGY_BLOCKS / FFS_SET_PROJECT_BLOCKS
-// override them for launch-shape sweeps.
-static int64_t env_blocks(const char* name, int64_t dflt) {
-    const char* e = std::getenv(name);
-    const long v = e ? std::strtol(e, nullptr, 10) : 0;
-    return v >= 8 && v <= 65536 ? (int64_t)v : dflt;
-}
-static int64_t set_energy_blocks() {
-    static const int64_t n = env_blocks("FFS_SET_ENERGY_BLOCKS", 2048);
-    return n;
-}
-static int64_t set_project_blocks() {
-    static const int64_t n = env_blocks("FFS_SET_PROJECT_BLOCKS", 1024);
-    return n;
-}
-
-// The plan's auxiliary non-blocking stream and its events, created on first use.
-static int ensure_aux(const ffs_reml_plan* P, const char* what) {
-    if (P->aux) return 0;
-    int dev = 0;
-    HIP_TRY(hipGetDevice(&dev));
-    if (dev != P->device) return fail(-3, std::string(what) + ": the plan belongs to another device");
-    HIP_TRY(hipStreamCreateWithFlags(&P->aux, hipStreamNonBlocking));
-    for (hipEvent_t& e : P->ev) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-    return 0;
-}
 namespace {
 int build_windows(ffs_reml_plan* P, hipStream_t s);
 size_t align256(size_t x) { return (x + 255) / 256 * 256; }
@@ -809,7 +778,13 @@ extern "C" int ffs_reml_surface(const ffs_reml_plan* P, const float* Yt, int64_t
         return ffs_reml_project(P, Yt, ldYt, V, S, ldS, stream);
     }
     hipStream_t s = (hipStream_t)stream;
-    if (int rc = ensure_aux(P, "surface")) return rc;
+    if (!P->aux) {
+        int dev = 0;
+        HIP_TRY(hipGetDevice(&dev));
+        if (dev != P->device) return fail(-3, "surface: the plan belongs to another device");
+        HIP_TRY(hipStreamCreateWithFlags(&P->aux, hipStreamNonBlocking));
+        for (hipEvent_t& e : P->ev) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    }
     // the auxiliary stream starts after the caller's preceding work (e.g. prep)
     HIP_TRY(hipEventRecord(P->ev[ffs_reml_plan::kMaxSurfaceChunks], s));
     HIP_TRY(hipStreamWaitEvent(P->aux, P->ev[ffs_reml_plan::kMaxSurfaceChunks], 0));
@@ -1209,7 +1184,7 @@ int build_windows(ffs_reml_plan* P, hipStream_t s) {
 
 struct SetLayout {
     int64_t n, max_cols, max_tiles, yt_rows;
-    size_t off_key, off_cnt, off_fill, off_cursor, off_tset, off_tcol, off_nt, off_rows, off_yt, off_s, off_pd, total;
+    size_t off_key, off_cnt, off_fill, off_cursor, off_tset, off_tcol, off_nt, off_rows, off_yt, off_s, total;
 };
 SetLayout set_layout(const ffs_reml_plan* P, const SetTable& ST, int64_t V) {
     SetLayout L{};
@@ -1234,14 +1209,12 @@ SetLayout set_layout(const ffs_reml_plan* P, const SetTable& ST, int64_t V) {
     L.off_rows = take(4 * (size_t)L.max_cols);
     L.off_yt = take(4 * (size_t)L.yt_rows * L.max_cols);
     L.off_s = take(4 * (size_t)L.max_cols * std::max(ST.sb, 1));
-    L.off_pd = take(8 * (size_t)L.max_cols * std::max(ST.sb, 1));
     L.total = o;
     return L;
 }
 struct SetPtrs {
     int32_t *key, *cnt, *fill, *cursor, *tset, *tcol, *nt, *rows;
-    float *Yt, *S;  // residuals; energies per (column, slot)
-    double* Pd;     // projections per (column, slot)
+    float *Yt, *S;
 };
 SetPtrs set_ptrs(const SetLayout& L, void* workspace) {
     char* ws = static_cast<char*>(workspace);
@@ -1256,7 +1229,6 @@ SetPtrs set_ptrs(const SetLayout& L, void* workspace) {
     q.rows = reinterpret_cast<int32_t*>(ws + L.off_rows);
     q.Yt = reinterpret_cast<float*>(ws + L.off_yt);
     q.S = reinterpret_cast<float*>(ws + L.off_s);
-    q.Pd = reinterpret_cast<double*>(ws + L.off_pd);
     return q;
 }
 int set_check(const ffs_reml_plan* P, const SetTable& ST, const char* what, int64_t V, void* workspace,
@@ -1283,22 +1255,23 @@ int set_group_prep(const ffs_reml_plan* P, const SetTable& ST, const float* Y, i
     return prep_impl(P, Y, ldY, V, q.rows, L.max_cols, q.Yt, L.max_cols, scale, 1, nullptr, s, q.fill + L.n, split,
                      false, coef);
 }
-// Energies (VALU, caller's stream) and projections (MFMA, the plan's auxiliary
-// stream) of a set pass run concurrently; set_select joins them.
-int set_energy_project(const ffs_reml_plan* P, const SetTable& ST, const SetLayout& L, const SetPtrs& q,
-                       hipStream_t s) {
-    if (int rc = ensure_aux(P, "set search")) return rc;
-    hipEvent_t ev_in = P->ev[ffs_reml_plan::kSetEv], ev_proj = P->ev[ffs_reml_plan::kSetEv + 1];
-    HIP_TRY(hipEventRecord(ev_in, s));
-    HIP_TRY(hipStreamWaitEvent(P->aux, ev_in, 0));
-    const int64_t pitems = L.max_tiles * (ST.sb / (kSetWaves * P->d.npt));
-    const dim3 g((unsigned)std::min<int64_t>(set_project_blocks(), std::max<int64_t>(8, (pitems + 7) / 8 * 8))),
-        b(64 * kSetWaves);
+int set_energy(const ffs_reml_plan* P, const SetTable& ST, const SetLayout& L, const SetPtrs& q, hipStream_t s) {
+    const int64_t n_cg = (ST.sb / kEnergyNB + kSetWaves - 1) / kSetWaves;
+    const int64_t items = L.max_tiles * n_cg;
+    const unsigned nblk = (unsigned)std::min<int64_t>(2048, std::max<int64_t>(8, (items + 7) / 8 * 8));
+    hipLaunchKernelGGL(k_set_energy, dim3(nblk), dim3(64 * kSetWaves), 0, s, P->d, ST, q.Yt, (int64_t)L.max_cols,
+                       q.tset, q.nt, q.S);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+int set_project(const ffs_reml_plan* P, const SetTable& ST, const SetLayout& L, const SetPtrs& q, int32_t* best,
+                float* best_score, int32_t* need, int32_t* next_key, hipStream_t s) {
+    const int64_t items = L.max_tiles * (ST.sb / (kSetWaves * P->d.npt));
+    const dim3 g((unsigned)std::min<int64_t>(1024, std::max<int64_t>(8, (items + 7) / 8 * 8))), b(64 * kSetWaves);
     const int64_t ld = L.max_cols;
     const int mode = P->d.proj_mode;
-    hipStream_t a = P->aux;
 #define FFS_SET_PROJ(BN_, MODE_) \
-    hipLaunchKernelGGL((k_set_project<BN_, MODE_>), g, b, 0, a, P->d, ST, q.Yt, ld, q.tset, q.nt, q.Pd)
+    hipLaunchKernelGGL((k_set_project<BN_, MODE_>), g, b, 0, s, P->d, ST, q.Yt, ld, q.tset, q.nt, q.S)
     if (P->d.BN == 64) {
         if (mode == 1) FFS_SET_PROJ(64, 1);
         else if (mode == 2) FFS_SET_PROJ(64, 2);
@@ -1310,20 +1283,8 @@ int set_energy_project(const ffs_reml_plan* P, const SetTable& ST, const SetLayo
     }
 #undef FFS_SET_PROJ
     HIP_TRY(hipGetLastError());
-    HIP_TRY(hipEventRecord(ev_proj, a));
-    const int64_t n_cg = (ST.sb / kEnergyNB + kSetWaves - 1) / kSetWaves;
-    const int64_t items = L.max_tiles * n_cg;
-    const unsigned nblk = (unsigned)std::min<int64_t>(set_energy_blocks(), std::max<int64_t>(8, (items + 7) / 8 * 8));
-    hipLaunchKernelGGL(k_set_energy, dim3(nblk), dim3(64 * kSetWaves), 0, s, P->d, ST, q.Yt, (int64_t)L.max_cols,
-                       q.tset, q.nt, q.S);
-    HIP_TRY(hipGetLastError());
-    return 0;
-}
-int set_select(const ffs_reml_plan* P, const SetTable& ST, const SetLayout& L, const SetPtrs& q, int32_t* best,
-               float* best_score, int32_t* need, int32_t* next_key, hipStream_t s) {
-    HIP_TRY(hipStreamWaitEvent(s, P->ev[ffs_reml_plan::kSetEv + 1], 0));
     hipLaunchKernelGGL(k_set_select, dim3((unsigned)(L.max_cols / 256)), dim3(256), 0, s, P->d, ST, q.rows, q.tset,
-                       q.nt, q.S, q.Pd, best, best_score, need, next_key);
+                       q.nt, q.S, best, best_score, need, next_key);
     HIP_TRY(hipGetLastError());
     return 0;
 }
@@ -1362,7 +1323,7 @@ extern "C" int ffs_reml_local_energy(const ffs_reml_plan* P, int64_t V, void* wo
     if (V <= 0) return 0;
     SetLayout L;
     if (int rc = set_check(P, P->d.win, "local search", V, workspace, ws_bytes, &L)) return rc;
-    return set_energy_project(P, P->d.win, L, set_ptrs(L, workspace), (hipStream_t)stream);
+    return set_energy(P, P->d.win, L, set_ptrs(L, workspace), (hipStream_t)stream);
 }
 
 extern "C" int ffs_reml_local_project(const ffs_reml_plan* P, int64_t V, int32_t* best, float* best_score,
@@ -1372,7 +1333,7 @@ extern "C" int ffs_reml_local_project(const ffs_reml_plan* P, int64_t V, int32_t
     if (int rc = set_check(P, P->d.win, "local search", V, workspace, ws_bytes, &L)) return rc;
     SetPtrs q = set_ptrs(L, workspace);
     // the window to continue with (walk) replaces the voxel's key
-    return set_select(P, P->d.win, L, q, best, best_score, need_full, q.key, (hipStream_t)stream);
+    return set_project(P, P->d.win, L, q, best, best_score, need_full, q.key, (hipStream_t)stream);
 }
 
 extern "C" int ffs_reml_local_search(const ffs_reml_plan* P, const float* Y, int64_t ldY, int64_t V,
@@ -1519,7 +1480,7 @@ extern "C" int ffs_reml_band_energy(const ffs_reml_plan* P, int64_t V, void* wor
     if (V <= 0) return 0;
     SetLayout L;
     if (int rc = set_check(P, P->d.band, "band search", V, workspace, ws_bytes, &L)) return rc;
-    return set_energy_project(P, P->d.band, L, set_ptrs(L, workspace), (hipStream_t)stream);
+    return set_energy(P, P->d.band, L, set_ptrs(L, workspace), (hipStream_t)stream);
 }
 
 extern "C" int ffs_reml_band_project(const ffs_reml_plan* P, int64_t V, int32_t* best, float* best_score,
@@ -1527,8 +1488,8 @@ extern "C" int ffs_reml_band_project(const ffs_reml_plan* P, int64_t V, int32_t*
     if (V <= 0) return 0;
     SetLayout L;
     if (int rc = set_check(P, P->d.band, "band search", V, workspace, ws_bytes, &L)) return rc;
-    return set_select(P, P->d.band, L, set_ptrs(L, workspace), best, best_score, need_full, nullptr,
-                      (hipStream_t)stream);
+    return set_project(P, P->d.band, L, set_ptrs(L, workspace), best, best_score, need_full, nullptr,
+                       (hipStream_t)stream);
 }
 
 extern "C" int ffs_reml_band_search(const ffs_reml_plan* P, const float* Y, int64_t ldY, int64_t V,

@@ -227,19 +227,17 @@ __global__ void __launch_bounds__(64 * kSetWaves) k_set_energy(PlanDev P, SetTab
     }
 }
 
-// Projection of every live 64-column tile over its set: items (tile, quad of
-// 4 set tiles); wave w takes set tile 4 quad + w (npt pairs, BN columns) for
-// all 64 columns.  MODE 0: fp32 MFMA 32x32x2 over the fp32 residual rows and
-// the fp32 operand (k_project's k-order); MODE 1 / 2: split bf16 / fp16 halves
-// on 32x32x16 (k_project16b's order: per 16-sample k-step al*bh, ah*bl,
-// ah*bh).  Writes P = sum_j C[col][slot p + j]^2 (fp64, j ascending, unscaled
-// as k_project16b) to Pd[col][slot]; k_set_select forms the score, so this
-// kernel does not wait for the energies and runs beside k_set_energy.
+// Projection + score of every live 64-column tile over its set: items
+// (tile, quad of 4 set tiles); wave w takes set tile 4 quad + w (npt pairs,
+// BN columns) for all 64 columns.  MODE 0: fp32 MFMA 32x32x2 over the fp32
+// residual rows and the fp32 operand (k_project's k-order); MODE 1 / 2: split
+// bf16 / fp16 halves on 32x32x16 (k_project16b's order: per 16-sample k-step
+// al*bh, ah*bl, ah*bh).  Scores go over the energies in Sb[col][slot].
 template <int BN, int MODE>
 __global__ void __launch_bounds__(64 * kSetWaves) k_set_project(PlanDev P, SetTable ST, const float* __restrict__ Yt,
                                                                 int64_t ldY, const int32_t* __restrict__ tset,
                                                                 const int32_t* __restrict__ n_tiles,
-                                                                double* __restrict__ Pd) {
+                                                                float* __restrict__ Sb) {
     constexpr int NBLK = BN / 32;
     constexpr int C_LD = BN + 1;
     __shared__ float patch[kSetWaves][32 * C_LD];
@@ -248,6 +246,7 @@ __global__ void __launch_bounds__(64 * kSetWaves) k_set_project(PlanDev P, SetTa
     const int SB = ST.sb, npt = P.npt, p = P.p;
     const int QT = SB / (kSetWaves * npt);  // quads per set
     const SetItems I = set_items(*n_tiles, QT);
+    const double tp = (double)(P.T - p);
     float* Cw = patch[w];
     for (int it = I.lb; it < I.n; it += I.nlb) {
         const int tt = I.lo + it / QT, quad = it % QT;
@@ -418,39 +417,34 @@ __global__ void __launch_bounds__(64 * kSetWaves) k_set_project(PlanDev P, SetTa
                     pp = fma(cv, cv, pp);
                 }
                 if (MODE == 2) pp *= (double)P.g16_unscale[k];
-                Pd[(size_t)(col0 + 32 * r + row) * SB + stile * npt + qq] = pp;
+                float* sp = Sb + (size_t)(col0 + 32 * r + row) * SB + stile * npt + qq;
+                const double rss = (double)(*sp) - pp;
+                *sp = (float)fma(tp, log(rss + 1e-10), P.ld[k]);
             }
         }
         __syncthreads();
     }
 }
 
-// Scores of the set's slots, score = ld_k + (T - p) log(E - P + 1e-10) from the
-// energies Eb and projections Pd (the arithmetic of the full-grid epilogue),
-// then the first minimum over the slots (pair order), lambda == 0 pairs mapped
+// First minimum over the set's slots (pair order), lambda == 0 pairs mapped
 // to the first of them; need = 1 when that pair's edge flag is set or no slot
 // has a finite score, else 0; next_key (nullable) = the window to continue
 // with (edge) or -1.
 __global__ void k_set_select(PlanDev P, SetTable ST, const int32_t* __restrict__ rows,
                              const int32_t* __restrict__ tset, const int32_t* __restrict__ n_tiles,
-                             const float* __restrict__ Eb, const double* __restrict__ Pd,
-                             int32_t* __restrict__ best, float* __restrict__ best_score, int32_t* __restrict__ need,
-                             int32_t* __restrict__ next_key) {
+                             const float* __restrict__ Sb, int32_t* __restrict__ best, float* __restrict__ best_score,
+                             int32_t* __restrict__ need, int32_t* __restrict__ next_key) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= *n_tiles * kSetTile) return;
     const int v = rows[c];
     if (v < 0) return;
     const int set = tset[c / kSetTile];
     const int SB = ST.sb, nb = ST.cnt[set];
-    const float* er = Eb + (size_t)c * SB;
-    const double* pr = Pd + (size_t)c * SB;
-    const int32_t* kp = ST.pairs + (size_t)set * SB;
-    const double tp = (double)(P.T - P.p);
+    const float* sr = Sb + (size_t)c * SB;
     float bs = INFINITY;
     int bslot = -1;
     for (int s = 0; s < nb; ++s) {
-        const double rss = (double)er[s] - pr[s];
-        const float sc = (float)fma(tp, log(rss + 1e-10), P.ld[kp[s]]);
+        const float sc = sr[s];
         if (sc < bs) {
             bs = sc;
             bslot = s;
