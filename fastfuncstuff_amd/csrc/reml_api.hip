@@ -277,7 +277,9 @@ static int plan_alloc(ffs_reml_plan* P, int32_t T, int32_t p, const double* X, c
     d.n_task = (int)tidx.size();
 
     // Energy chunks: lambda == 0 pairs form chunk 0; the rest are grouped by
-    // identical a (consecutive in the a-major pair order), kEnergyNB per chunk.
+    // identical a (consecutive in the a-major pair order) into halves of
+    // kEnergyHalf slots, two halves (of one or two values of a) per chunk, so a
+    // row of the lattice leaves at most kEnergyHalf - 1 empty slots.
     std::vector<int32_t> whites, chunks;
     for (int k = 0; k < K; ++k)
         if (white[k]) whites.push_back(k);
@@ -289,10 +291,11 @@ static int plan_alloc(ffs_reml_plan* P, int32_t T, int32_t p, const double* X, c
         std::vector<int32_t> grp;
         for (int q = k; q < e; ++q)
             if (!white[q]) grp.push_back(q);
-        for (size_t q = 0; q < grp.size(); q += kEnergyNB)
-            for (int j = 0; j < kEnergyNB; ++j) chunks.push_back(q + j < grp.size() ? grp[q + j] : -1);
+        for (size_t q = 0; q < grp.size(); q += kEnergyHalf)
+            for (int j = 0; j < kEnergyHalf; ++j) chunks.push_back(q + j < grp.size() ? grp[q + j] : -1);
         k = e;
     }
+    if (chunks.size() % kEnergyNB) chunks.insert(chunks.end(), kEnergyHalf, -1);
     d.n_chunks = (int)chunks.size() / kEnergyNB;
     d.n_white = (int)whites.size();
 
@@ -318,7 +321,7 @@ static int plan_alloc(ffs_reml_plan* P, int32_t T, int32_t p, const double* X, c
     if (!gls_only) {
         if ((rc = P->alloc(&G32, (size_t)d.n_tiles * d.Tpad * BN)) ||
             (rc = P->alloc(&ecoef, (size_t)d.n_chunks * T * 2 * kEnergyNB)) ||
-            (rc = P->alloc(&ec, (size_t)d.n_chunks * T)) || (rc = P->alloc(&es0, (size_t)d.n_chunks * kEnergyNB)))
+            (rc = P->alloc(&ec, (size_t)d.n_chunks * T * 2)) || (rc = P->alloc(&es0, (size_t)d.n_chunks * kEnergyNB)))
             return rc;
         HIP_TRY(hipMemsetAsync(G32, 0, sizeof(float) * d.n_tiles * d.Tpad * BN, s));
     }
@@ -689,8 +692,8 @@ static int energy_impl(const ffs_reml_plan* P, const float* Yt, int64_t ldYt, in
     if (ldYt < Vpad) return fail(-3, "energy: ldYt must be >= the padded voxel count");
     if (ldE < P->d.K) return fail(-3, "energy: ldE must be >= K");
     const int64_t n_xt = (V + kEnergyVox - 1) / kEnergyVox;
-    const int64_t n_yg = (P->d.n_chunks + kEnergyWaves - 1) / kEnergyWaves;
-    const int64_t nblk = 8 * ((n_xt + 7) / 8) * n_yg;  // see k_energy's block map
+    const int64_t n_xo = (n_xt + kEnergyWaves - 1) / kEnergyWaves;
+    const int64_t nblk = 8 * ((n_xo * P->d.n_chunks + 7) / 8);  // see k_energy's item map
     if (nblk > 0x7fffffff) return fail(-3, "energy: grid too large; split the voxel batch");
     hipLaunchKernelGGL(k_energy, dim3((unsigned)nblk), dim3(kEnergyVox * kEnergyWaves), 0, s, P->d, Yt, ldYt, (int)V,
                        E, ldE, n_live);

@@ -5,10 +5,14 @@
 
 namespace remlk {
 
-// Pairs handled by one wave of the energy kernel: all share the AR coefficient
-// a (the pair list is a-major), so the AR-filtered sample z_t = y_{t+1} - c_t y_t
-// is computed once per time step for all of them.
+// Pairs handled by one wave of the energy kernel, in two halves; the pairs of
+// a half share the AR coefficient a (the pair list is a-major), so the
+// AR-filtered sample z_t = y_{t+1} - c_t y_t is computed once per time step
+// for each half.
 constexpr int kEnergyNB = 8;
+// Slots per half of an energy chunk: the pairs of a half share a (the two
+// halves of a chunk may differ), so the filtered sample is formed once per half.
+constexpr int kEnergyHalf = 4;
 // Voxels per wave in the energy kernel (one per lane).
 constexpr int kEnergyVox = 64;
 // Waves per energy-kernel block: they share the block's 64-voxel slab via L1.
@@ -145,7 +149,7 @@ struct PlanDev {
     int rank;
     int proj_mode;             // 0 fp32 MFMA, 1 bf16 x3, 2 fp16 x3
     const float* ecoef;        // [n_chunks][T][2*NB] (alpha_t, 1/v_{t+1}) per chunk slot
-    const float* ec;           // [n_chunks][T] c_t of the chunk's AR coefficient
+    const float* ec;           // [n_chunks][T][2] c_t of the AR coefficient of each half of the chunk
     const float* es0;          // [n_chunks][NB] 1/v_0
     const int32_t* echunk;     // [n_chunks][NB] pair index (-1 = empty slot)
     const int32_t* white;      // [n_white] lambda==0 pairs

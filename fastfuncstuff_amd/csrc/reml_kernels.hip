@@ -419,14 +419,13 @@ __global__ void k_energy_tables(int T, int n_chunks, const int32_t* __restrict__
     const int chunk = blockIdx.x;
     if (chunk >= n_chunks) return;
     const int32_t* slots = echunk + chunk * kEnergyNB;
-    int first = -1;
-    for (int j = 0; j < kEnergyNB; ++j)
-        if (slots[j] >= 0) {
-            first = slots[j];
-            break;
-        }
+    int first[2] = {-1, -1};  // first pair of each half (every pair of a half has the same a)
+    for (int h = 0; h < 2; ++h)
+        for (int j = 0; j < kEnergyHalf; ++j)
+            if (first[h] < 0 && slots[h * kEnergyHalf + j] >= 0) first[h] = slots[h * kEnergyHalf + j];
     for (int t = threadIdx.x; t < T; t += blockDim.x) {
-        ec[(size_t)chunk * T + t] = first >= 0 ? (float)soa[(size_t)first * 3 * T + T + t] : 0.f;
+        for (int h = 0; h < 2; ++h)
+            ec[((size_t)chunk * T + t) * 2 + h] = first[h] >= 0 ? (float)soa[(size_t)first[h] * 3 * T + T + t] : 0.f;
         for (int j = 0; j < kEnergyNB; ++j) {
             const int k = slots[j];
             float alpha = 0.f, s2n = 0.f;
@@ -631,30 +630,37 @@ __device__ inline float white_energy(const float* __restrict__ y, int64_t ldY, i
 
 // ---------------------------------------------------------------------------
 // (i) Energy ||L^-1 y||^2 for every (voxel, pair).
-// Block: kEnergyWaves waves over one 64-voxel slab; wave w takes chunk
-// blockIdx.y * kEnergyWaves + w (kEnergyNB pairs sharing the AR coefficient).
-// Per step and pair: e <- alpha e + z ; E += e^2 / v   (3 VALU ops), fp32 partial
-// sums flushed into fp64 every 16 steps.
+// Block: kEnergyWaves waves on one chunk (kEnergyNB pairs, each half of
+// kEnergyHalf sharing the AR coefficient); wave w takes the block's w-th
+// 64-voxel tile.
+// Per step and half: z = y_{t+1} - c_t y_t; per pair: e <- alpha e + z ;
+// E += e^2 / v   (3 VALU ops), fp32 partial sums flushed into fp64 every 8 steps.
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(kEnergyVox* kEnergyWaves) k_energy(PlanDev P, const float* __restrict__ Yt,
                                                                      int64_t ldY, int V, float* __restrict__ E,
                                                                      int64_t ldE, const int32_t* __restrict__ n_live) {
     const int lane = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    // (voxel tile, chunk group) of this block: the chunk groups of one voxel
-    // tile run together on one XCD so its residual columns are fetched once
-    // into that XCD's L2 (same map as the projection, chunk group fastest).
-    // With a live count the map covers the live voxels only (the grid is
-    // sized for V), so a small count still spreads over every XCD.
+    // Item (voxel octet, chunk) of this block: its kEnergyWaves waves take the
+    // chunk on kEnergyWaves consecutive 64-voxel tiles, so they read one
+    // coefficient stream (scalar-cache hits for all but one wave).  Items are
+    // numbered octet-major, chunk fastest, and each XCD (block id % 8) takes a
+    // contiguous eighth of them: the chunks of an octet run together on one
+    // XCD (at most two at a range boundary), so its residual columns are
+    // fetched once into that XCD's L2, and any voxel count spreads evenly.
+    // With a live count the items cover the live voxels only (the grid is
+    // sized for V).
     if (n_live) V = min(V, *n_live);
     const int n_xt = (V + kEnergyVox - 1) / kEnergyVox;
-    const int n_yg = (P.n_chunks + kEnergyWaves - 1) / kEnergyWaves;
-    const int bid = blockIdx.x, xcd = bid & 7, local = bid >> 3;
-    const int per = (n_xt + 7) >> 3;  // voxel tiles per XCD
-    const int xt = xcd * per + local / n_yg;
-    if (local / n_yg >= per || xt >= n_xt) return;
-    const int chunk = (local % n_yg) * kEnergyWaves + w;
-    if (chunk >= P.n_chunks) return;
+    const int n_xo = (n_xt + kEnergyWaves - 1) / kEnergyWaves;
+    const int n_items = n_xo * P.n_chunks;
+    const int per = (n_items + 7) >> 3;  // items per XCD
+    const int bid = blockIdx.x, local = bid >> 3;
+    const int item = (bid & 7) * per + local;
+    if (local >= per || item >= n_items) return;
+    const int xo = item / P.n_chunks, chunk = item % P.n_chunks;
+    const int xt = xo * kEnergyWaves + w;
+    if (xt >= n_xt) return;
     const int v = xt * kEnergyVox + lane;
     const int T = P.T;
     const float* __restrict__ y = Yt + v;
@@ -668,7 +674,7 @@ __global__ void __launch_bounds__(kEnergyVox* kEnergyWaves) k_energy(PlanDev P, 
             for (int i = 0; i < P.n_white; ++i) E[(size_t)v * ldE + P.white[i]] = acc;
         return;
     }
-    const float* __restrict__ cz = P.ec + (size_t)chunk * T;
+    const float* __restrict__ cz = P.ec + (size_t)chunk * T * 2;
     const float* __restrict__ co = P.ecoef + (size_t)chunk * T * 2 * kEnergyNB;
     const float* __restrict__ s0 = P.es0 + chunk * kEnergyNB;
     float e[kEnergyNB], part[kEnergyNB];
@@ -681,12 +687,14 @@ __global__ void __launch_bounds__(kEnergyVox* kEnergyWaves) k_energy(PlanDev P, 
         acc[j] = 0.0;
     }
     float yc = y0;
-    // One step t -> t+1 for every pair of the chunk (coefficients wave-uniform).
-    auto step = [&](float yn, float czt, const float* c) __attribute__((always_inline)) {
-        const float z = fmaf(-czt, yc, yn);
+    // One step t -> t+1 for every pair of the chunk (coefficients wave-uniform):
+    // the AR-filtered sample of each half, shared by its pairs.
+    auto step = [&](float yn, float cz0, float cz1, const float* c) __attribute__((always_inline)) {
+        const float z0 = fmaf(-cz0, yc, yn);
+        const float z1 = fmaf(-cz1, yc, yn);
 #pragma unroll
         for (int j = 0; j < kEnergyNB; ++j) {
-            e[j] = fmaf(c[2 * j], e[j], z);
+            e[j] = fmaf(c[2 * j], e[j], j < kEnergyHalf ? z0 : z1);
             part[j] = fmaf(e[j] * c[2 * j + 1], e[j], part[j]);
         }
         yc = yn;
@@ -708,15 +716,16 @@ __global__ void __launch_bounds__(kEnergyVox* kEnergyWaves) k_energy(PlanDev P, 
         for (int i = 0; i < U; ++i) ybuf[i] = Yt[(size_t)min(t0 + U + 1 + i, T - 1) * ldY + v];
 #pragma unroll
         for (int g0 = 0; g0 < U; g0 += G) {
-            float cg[G][2 * kEnergyNB], zg[G];
+            float cg[G][2 * kEnergyNB], zg[G][2];
 #pragma unroll
             for (int i = 0; i < G; ++i) {
-                zg[i] = cz[t0 + g0 + i];
+                zg[i][0] = cz[2 * (t0 + g0 + i)];
+                zg[i][1] = cz[2 * (t0 + g0 + i) + 1];
 #pragma unroll
                 for (int q = 0; q < 2 * kEnergyNB; ++q) cg[i][q] = co[(size_t)(t0 + g0 + i) * 2 * kEnergyNB + q];
             }
 #pragma unroll
-            for (int i = 0; i < G; ++i) step(ycur[g0 + i], zg[i], cg[i]);
+            for (int i = 0; i < G; ++i) step(ycur[g0 + i], zg[i][0], zg[i][1], cg[i]);
         }
 #pragma unroll
         for (int j = 0; j < kEnergyNB; ++j) {
@@ -725,7 +734,8 @@ __global__ void __launch_bounds__(kEnergyVox* kEnergyWaves) k_energy(PlanDev P, 
         }
     }
     // tail (fewer than U steps); the flush matches the blocked schedule
-    for (int t = nfull; t < nsteps; ++t) step(Yt[(size_t)(t + 1) * ldY + v], cz[t], co + (size_t)t * 2 * kEnergyNB);
+    for (int t = nfull; t < nsteps; ++t)
+        step(Yt[(size_t)(t + 1) * ldY + v], cz[2 * t], cz[2 * t + 1], co + (size_t)t * 2 * kEnergyNB);
     if (v < V) {
         const int32_t* slots = P.echunk + chunk * kEnergyNB;
 #pragma unroll
