@@ -60,6 +60,9 @@ SIGNATURES: dict[str, list] = {
     "ffs_reml_gls_workspace": [_vp, _i64, _vp],
     "ffs_reml_plan_yt_rows": [_vp, _vp],
     "ffs_reml_surface": [_vp, _vp, _i64, _i64, _vp, _i64, _c_int, _vp],
+    "ffs_reml_plan_build_hier": [_vp, _vp],
+    "ffs_reml_hier_workspace": [_vp, _i64, _vp],
+    "ffs_reml_hier_search": [_vp, _vp, _i64, _i64, _vp, _vp, _vp, _vp, _i64, _vp],
     "ffs_reml_local_prepare": [_vp, _vp, _i64, _i64, _vp, ctypes.c_float, _vp, _vp, _vp, _vp, _i64, _vp],
     "ffs_reml_local_energy": [_vp, _i64, _vp, _i64, _vp],
     "ffs_reml_local_project": [_vp, _i64, _vp, _vp, _vp, _vp, _i64, _vp],

@@ -70,6 +70,8 @@ struct ffs_reml_plan {
     // host copies of the pair list and lattice (band tables are built from them)
     std::vector<double> h_ab;
     std::vector<int32_t> h_lut;
+    // passes of the hierarchical descent on pair sets (ffs_reml_plan_build_hier)
+    std::vector<HierPass> hier;
     template <class Tp>
     int alloc(Tp** out, size_t n) {
         void* p = nullptr;
@@ -1249,14 +1251,15 @@ int set_check(const ffs_reml_plan* P, const SetTable& ST, const char* what, int6
 // voxels grouped by q.key, residualised (with the split halves in the split
 // modes) into the set layout
 int set_group_prep(const ffs_reml_plan* P, const SetTable& ST, const float* Y, int64_t ldY, int64_t V, float* scale,
-                   const double* coef, const SetLayout& L, const SetPtrs& q, hipStream_t s) {
+                   const double* coef, const SetLayout& L, const SetPtrs& q, hipStream_t s,
+                   double* coef_out = nullptr) {
     if (int rc = group_by_key(q.key, V, (int)L.n, kSetTile, q.cnt, q.fill, q.cursor, q.rows, L.max_cols, s)) return rc;
     hipLaunchKernelGGL(k_group_tiles, dim3((unsigned)((L.max_tiles + 255) / 256)), dim3(256), 0, s, q.fill, (int)L.n,
                        (int)L.max_tiles, kSetTile, q.tset, q.tcol, q.nt);
     HIP_TRY(hipGetLastError());
     const bool split = P->d.PG && P->d.proj_mode != 0;
     return prep_impl(P, Y, ldY, V, q.rows, L.max_cols, q.Yt, L.max_cols, scale, 1, nullptr, s, q.fill + L.n, split,
-                     false, coef);
+                     false, coef, coef_out);
 }
 int set_energy(const ffs_reml_plan* P, const SetTable& ST, const SetLayout& L, const SetPtrs& q, hipStream_t s) {
     const int64_t n_cg = (ST.sb / kEnergyNB + kSetWaves - 1) / kSetWaves;
@@ -1267,8 +1270,9 @@ int set_energy(const ffs_reml_plan* P, const SetTable& ST, const SetLayout& L, c
     HIP_TRY(hipGetLastError());
     return 0;
 }
-int set_project(const ffs_reml_plan* P, const SetTable& ST, const SetLayout& L, const SetPtrs& q, int32_t* best,
-                float* best_score, int32_t* need, int32_t* next_key, hipStream_t s) {
+// scores of every column's set pairs into q.S (the full-grid projection
+// arithmetic over the residual tile, score epilogue over the set energies)
+int set_scores(const ffs_reml_plan* P, const SetTable& ST, const SetLayout& L, const SetPtrs& q, hipStream_t s) {
     const int64_t items = L.max_tiles * (ST.sb / (kSetWaves * P->d.npt));
     const dim3 g((unsigned)std::min<int64_t>(1024, std::max<int64_t>(8, (items + 7) / 8 * 8))), b(64 * kSetWaves);
     const int64_t ld = L.max_cols;
@@ -1286,6 +1290,11 @@ int set_project(const ffs_reml_plan* P, const SetTable& ST, const SetLayout& L, 
     }
 #undef FFS_SET_PROJ
     HIP_TRY(hipGetLastError());
+    return 0;
+}
+int set_project(const ffs_reml_plan* P, const SetTable& ST, const SetLayout& L, const SetPtrs& q, int32_t* best,
+                float* best_score, int32_t* need, int32_t* next_key, hipStream_t s) {
+    if (int rc = set_scores(P, ST, L, q, s)) return rc;
     hipLaunchKernelGGL(k_set_select, dim3((unsigned)(L.max_cols / 256)), dim3(256), 0, s, P->d, ST, q.rows, q.tset,
                        q.nt, q.S, best, best_score, need, next_key);
     HIP_TRY(hipGetLastError());
@@ -1365,6 +1374,195 @@ extern "C" int ffs_reml_scatter_best(const int32_t* idx, int64_t n, const int32_
     hipLaunchKernelGGL(k_scatter_best, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, idx,
                        (int)n, nullptr, best_sub, score_sub, best, score);
     HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Hierarchical descent on pair sets
+// ---------------------------------------------------------------------------
+// The lattice points each level of the descent can visit, enumerated from the
+// start point: pass 0 is one set holding the top level's points, the start
+// point and every point of the level below's neighbourhoods around the points
+// the top level can end on; each later level has one set per centre the
+// previous levels can end on (its clamped 3 x 3 neighbourhood at the level's
+// step, distinct valid pairs in visit order) and a map from the 9 visits to
+// the set's slots.  Visit orders are those of k_select mode 1.
+extern "C" int ffs_reml_plan_build_hier(ffs_reml_plan* P, void* stream) {
+    if (!P) return fail(-3, "plan_build_hier: null plan");
+    FFS_NEED_SEARCH(P, "plan_build_hier");
+    if (!P->hier.empty()) return 0;
+    hipStream_t s = (hipStream_t)stream;
+    const int n_a = P->d.n_a, n_b = P->d.n_b, ltop = P->d.ltop;
+    const int za = P->d.zero_a, zb = P->d.zero_b;
+    auto kof = [&](int x, int y) { return P->h_lut[(size_t)x * n_b + y]; };
+    auto nbh = [&](int cx, int cy, int lev) {
+        const int step = 1 << lev;
+        std::vector<std::pair<int, int>> out;
+        for (int oy = -1; oy <= 1; ++oy) {
+            const int y = std::min(std::max(cy + oy * step, 0), n_b - 1);
+            for (int ox = -1; ox <= 1; ++ox) out.push_back({std::min(std::max(cx + ox * step, 0), n_a - 1), y});
+        }
+        return out;
+    };
+    const size_t NL = (size_t)n_a * n_b;
+    std::vector<char> W(NL, 0), U(NL, 0);  // points a voxel can be at; points of pass 0's set
+    W[(size_t)za * n_b + zb] = 1;
+    U[(size_t)za * n_b + zb] = 1;
+    const int st = 1 << ltop;
+    for (int y = 0; y < n_b; y += st)
+        for (int x = 0; x < n_a; x += st) {
+            U[(size_t)x * n_b + y] = 1;
+            if (kof(x, y) >= 0) W[(size_t)x * n_b + y] = 1;
+        }
+    if (ltop >= 1) {
+        std::vector<char> W2(NL, 0);
+        for (size_t c = 0; c < NL; ++c) {
+            if (!W[c]) continue;
+            W2[c] = 1;
+            for (auto [x, y] : nbh((int)(c / n_b), (int)(c % n_b), ltop - 1)) {
+                U[(size_t)x * n_b + y] = 1;
+                if (kof(x, y) >= 0) W2[(size_t)x * n_b + y] = 1;
+            }
+        }
+        W.swap(W2);
+    }
+    std::vector<HierPass> passes;
+    {
+        std::vector<std::vector<int32_t>> sets(1), edges(1);
+        std::vector<int32_t> pslot(NL, -1);
+        for (size_t c = 0; c < NL; ++c) {
+            const int k = U[c] ? kof((int)(c / n_b), (int)(c % n_b)) : -1;
+            if (k < 0) continue;
+            pslot[c] = (int32_t)sets[0].size();
+            sets[0].push_back(k);
+            edges[0].push_back(0);
+        }
+        HierPass h{};
+        if (int rc = build_set_table(P, sets, edges, nullptr, &h.ST, s)) return rc;
+        int32_t* d;
+        if (int rc = upload(P, &d, pslot.data(), pslot.size(), s)) return rc;
+        h.point_slot = d;
+        h.level = ltop;
+        h.first = 1;
+        passes.push_back(h);
+    }
+    for (int lev = ltop - 2; lev >= 0; --lev) {
+        std::vector<std::vector<int32_t>> sets, edges;
+        std::vector<int32_t> cset(NL, -1), vmap;
+        std::vector<char> W2(NL, 0);
+        for (size_t c = 0; c < NL; ++c) {
+            if (!W[c]) continue;
+            W2[c] = 1;
+            cset[c] = (int32_t)sets.size();
+            std::vector<int32_t> pairs, edge;
+            for (auto [x, y] : nbh((int)(c / n_b), (int)(c % n_b), lev)) {
+                const int k = kof(x, y);
+                int slot = -1;
+                if (k >= 0) {
+                    W2[(size_t)x * n_b + y] = 1;
+                    for (size_t q = 0; q < pairs.size(); ++q)
+                        if (pairs[q] == k) slot = (int)q;
+                    if (slot < 0) {
+                        slot = (int)pairs.size();
+                        pairs.push_back(k);
+                        edge.push_back(0);
+                    }
+                }
+                vmap.push_back(slot);
+            }
+            sets.push_back(pairs);
+            edges.push_back(edge);
+        }
+        HierPass h{};
+        if (int rc = build_set_table(P, sets, edges, nullptr, &h.ST, s)) return rc;
+        int32_t *dc, *dv;
+        if (int rc = upload(P, &dc, cset.data(), cset.size(), s)) return rc;
+        if (int rc = upload(P, &dv, vmap.data(), vmap.size(), s)) return rc;
+        HIP_TRY(hipStreamSynchronize(s));  // the host vectors above are uploaded
+        h.center_set = dc;
+        h.vmap = dv;
+        h.level = lev;
+        passes.push_back(h);
+        W.swap(W2);
+    }
+    HIP_TRY(hipStreamSynchronize(s));
+    P->hier = passes;
+    return 0;
+}
+
+namespace {
+struct HierLayout {
+    size_t off_ba, off_bb, off_cur, off_coef, off_sets, total;
+    int stride;
+};
+HierLayout hier_layout(const ffs_reml_plan* P, int64_t V) {
+    HierLayout H{};
+    H.stride = P->d.PG;  // residualisation coefficients per voxel (0: no f64 MFMA prep, p > 32)
+    size_t o = 0;
+    auto take = [&](size_t bytes) {
+        const size_t at = o;
+        o = align256(o + bytes);
+        return at;
+    };
+    H.off_ba = take(4 * (size_t)V);
+    H.off_bb = take(4 * (size_t)V);
+    H.off_cur = take(4 * (size_t)V);
+    H.off_coef = take(8 * (size_t)V * H.stride);
+    H.off_sets = o;
+    size_t mx = 0;
+    for (const HierPass& h : P->hier) mx = std::max(mx, set_layout(P, h.ST, V).total);
+    H.total = o + mx;
+    return H;
+}
+}  // namespace
+
+extern "C" int ffs_reml_hier_workspace(const ffs_reml_plan* P, int64_t V, int64_t* bytes) {
+    if (!P || !bytes) return fail(-3, "hier_workspace: null argument");
+    if (P->hier.empty()) return fail(-4, "hier_workspace: call ffs_reml_plan_build_hier first");
+    *bytes = (int64_t)hier_layout(P, std::max<int64_t>(V, 0)).total;
+    return 0;
+}
+
+extern "C" int ffs_reml_hier_search(const ffs_reml_plan* P, const float* Y, int64_t ldY, int64_t V, int32_t* best,
+                                    float* best_score, float* scale, void* workspace, int64_t ws_bytes, void* stream) {
+    if (!P) return fail(-3, "hier_search: null plan");
+    FFS_NEED_SEARCH(P, "hier_search");
+    if (V <= 0) return 0;
+    if (P->hier.empty()) return fail(-4, "hier_search: call ffs_reml_plan_build_hier first");
+    if (!Y || !best || !best_score || !scale) return fail(-3, "hier_search: null argument");
+    if (P->d.proj_mode != 0 && !P->d.PG)
+        return fail(-4, "hier_search: the split projection modes need p <= 32 here (use fp32)");
+    const HierLayout H = hier_layout(P, V);
+    if (!workspace || ws_bytes < (int64_t)H.total) return fail(-3, "hier_search: workspace too small");
+    char* ws = static_cast<char*>(workspace);
+    int32_t* hba = reinterpret_cast<int32_t*>(ws + H.off_ba);
+    int32_t* hbb = reinterpret_cast<int32_t*>(ws + H.off_bb);
+    float* hcur = reinterpret_cast<float*>(ws + H.off_cur);
+    double* coef = H.stride ? reinterpret_cast<double*>(ws + H.off_coef) : nullptr;
+    hipStream_t s = (hipStream_t)stream;
+    const int np = (int)P->hier.size();
+    for (int i = 0; i < np; ++i) {
+        const HierPass& h = P->hier[i];
+        const SetLayout L = set_layout(P, h.ST, V);
+        if (L.max_cols > 0x7fffffff) return fail(-3, "hier_search: batch too large; split it");
+        const SetPtrs q = set_ptrs(L, ws + H.off_sets);
+        if (i == 0)
+            hipLaunchKernelGGL(k_fill_i32, dim3((unsigned)((V + 255) / 256)), dim3(256), 0, s, q.key, V, 0);
+        else
+            hipLaunchKernelGGL(k_hier_key, dim3((unsigned)((V + 255) / 256)), dim3(256), 0, s, P->d, h.center_set,
+                               hba, hbb, (int)V, q.key);
+        HIP_TRY(hipGetLastError());
+        // pass 0 residualises from Y and keeps each voxel's coefficients and
+        // scale; the later passes run the second half of the prep from them
+        if (int rc = set_group_prep(P, h.ST, Y, ldY, V, scale, i == 0 ? nullptr : coef, L, q, s,
+                                    i == 0 ? coef : nullptr))
+            return rc;
+        if (int rc = set_energy(P, h.ST, L, q, s)) return rc;
+        if (int rc = set_scores(P, h.ST, L, q, s)) return rc;
+        hipLaunchKernelGGL(k_hier_select, dim3((unsigned)(L.max_cols / 256)), dim3(256), 0, s, P->d, h, q.rows,
+                           q.tset, q.nt, q.S, hba, hbb, hcur, (int)(i == np - 1), best, best_score);
+        HIP_TRY(hipGetLastError());
+    }
     return 0;
 }
 

@@ -168,6 +168,20 @@ struct PlanDev {
     SetTable win;              // set of window centre c = ai * n_b + bi
 };
 
+// One pass of the hierarchical descent evaluated on pair sets (k_hier_select):
+// pass 0 scores the lattice points that the top level and the level below it
+// can visit (one set for every voxel); each later pass scores, per voxel, the
+// 3 x 3 neighbourhood of its current point at the pass's level (sets keyed by
+// the centre, built only for the centres the descent can reach).
+struct HierPass {
+    SetTable ST;
+    int level;                  // the pass's level (pass 0: ltop, and ltop - 1 when ltop >= 1)
+    int first;                  // 1 for pass 0
+    const int32_t* point_slot;  // pass 0: [n_a * n_b] slot of each lattice point in the set (-1: absent)
+    const int32_t* center_set;  // passes >= 1: [n_a * n_b] set of each centre (-1: unreachable)
+    const int32_t* vmap;        // passes >= 1: [ST.n][9] slot of each visit (-1: invalid pair)
+};
+
 // Lattice window of the learned search: kWin x kWin pairs around the proposal.
 constexpr int kWin = 5;
 constexpr int kWinHalf = 2;

@@ -467,4 +467,97 @@ __global__ void k_set_select(PlanDev P, SetTable ST, const int32_t* __restrict__
     if (next_key) next_key[v] = edge && ST.next ? ST.next[o] : -1;
 }
 
+// ---------------------------------------------------------------------------
+// Hierarchical descent (glm/arma.py:2224-2322) on demand: the levels of the
+// descent that k_select mode 1 runs on the full surface, run on pair sets
+// whose scores equal the surface bitwise, so the selected pair and score are
+// the same.  The state of each voxel between passes is its current lattice
+// point (ba, bb) and score cur.
+// ---------------------------------------------------------------------------
+// Set of each voxel for a pass >= 1: the one keyed by its current point.
+__global__ void k_hier_key(PlanDev P, const int32_t* __restrict__ center_set, const int32_t* __restrict__ hba,
+                           const int32_t* __restrict__ hbb, int V, int32_t* __restrict__ key) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= V) return;
+    key[v] = center_set[hba[v] * P.n_b + hbb[v]];
+}
+
+// One pass: per voxel column, the level(s) of the pass with k_select mode 1's
+// visit order (top level: b-outer / a-inner over the multiples of the step;
+// below: the clamped 3 x 3 neighbourhood, b-offset outer), first strict
+// minimum, a level with a NaN candidate leaving the point unchanged, and the
+// move only to a score below the current one.  last: write the voxel's pair
+// and score.
+__global__ void k_hier_select(PlanDev P, HierPass H, const int32_t* __restrict__ rows,
+                              const int32_t* __restrict__ tset, const int32_t* __restrict__ n_tiles,
+                              const float* __restrict__ Sb, int32_t* __restrict__ hba, int32_t* __restrict__ hbb,
+                              float* __restrict__ hcur, int last, int32_t* __restrict__ best,
+                              float* __restrict__ best_score) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= *n_tiles * kSetTile) return;
+    const int v = rows[c];
+    if (v < 0) return;
+    const int set = tset[c / kSetTile];
+    const float* sr = Sb + (size_t)c * H.ST.sb;
+    int ba, bb;
+    float cur;
+    auto run_level = [&](int lev, auto score_of) {
+        const int step = 1 << lev;
+        float mn = 0.f;
+        int sa = 0, sb = 0, i = 0;
+        bool first = true, nan = false;
+        auto visit = [&](int x, int y) {
+            const float s = score_of(i++, x, y);
+            if (s != s) nan = true;
+            if (first || s < mn) {
+                mn = s;
+                sa = x;
+                sb = y;
+                first = false;
+            }
+        };
+        if (lev == P.ltop) {
+            for (int y = 0; y < P.n_b; y += step)
+                for (int x = 0; x < P.n_a; x += step) visit(x, y);
+        } else {
+            for (int oy = -1; oy <= 1; ++oy) {
+                const int y = min(max(bb + oy * step, 0), P.n_b - 1);
+                for (int ox = -1; ox <= 1; ++ox) visit(min(max(ba + ox * step, 0), P.n_a - 1), y);
+            }
+        }
+        if (!nan && mn < cur) {
+            cur = mn;
+            ba = sa;
+            bb = sb;
+        }
+    };
+    if (H.first) {
+        auto by_point = [&](int, int x, int y) {
+            const int sl = H.point_slot[x * P.n_b + y];
+            return sl >= 0 ? sr[sl] : INFINITY;
+        };
+        ba = P.zero_a;
+        bb = P.zero_b;
+        cur = by_point(0, ba, bb);
+        run_level(P.ltop, by_point);
+        if (P.ltop >= 1) run_level(P.ltop - 1, by_point);
+    } else {
+        ba = hba[v];
+        bb = hbb[v];
+        cur = hcur[v];
+        const int32_t* vm = H.vmap + (size_t)set * 9;
+        run_level(H.level, [&](int i, int, int) {
+            const int sl = vm[i];
+            return sl >= 0 ? sr[sl] : INFINITY;
+        });
+    }
+    hba[v] = ba;
+    hbb[v] = bb;
+    hcur[v] = cur;
+    if (last) {
+        best[v] = P.lut[ba * P.n_b + bb];
+        best_score[v] = cur;
+    }
+}
+
 }  // namespace remlk

@@ -593,6 +593,29 @@ class RemlGrid(Mapping):
                    "select")
         return best, score
 
+    def hier_search(self, Y: torch.Tensor):
+        """The hierarchical descent (glm/arma.py:2224-2322) on demand: the pairs
+        and scores that ``select(surface(prep(Y)), hierarchical=True)`` gives,
+        evaluated on the lattice points the descent visits only (about 45 per
+        voxel; same scores bitwise).  Y: (V, T) float32 on the device.  Returns
+        (best, score, scale)."""
+        lib = _lib.load()
+        st = _lib.stream_handle(self.device)
+        if not getattr(self, "_hier_built", False):
+            _lib.check(lib.ffs_reml_plan_build_hier(self._handle, st), "plan_build_hier")
+            self._hier_built = True
+        V = Y.shape[0]
+        nb = ctypes.c_int64()
+        _lib.check(lib.ffs_reml_hier_workspace(self._handle, V, ctypes.byref(nb)), "hier_workspace")
+        ws = torch.empty(max(int(nb.value), 1), dtype=torch.uint8, device=self.device)
+        best = torch.empty(V, dtype=torch.int32, device=self.device)
+        score = torch.empty(V, dtype=torch.float32, device=self.device)
+        scale = torch.empty(V, dtype=torch.float32, device=self.device)
+        _lib.check(lib.ffs_reml_hier_search(self._handle, _lib.ptr(Y), Y.stride(0), V, _lib.ptr(best),
+                                            _lib.ptr(score), _lib.ptr(scale), _lib.ptr(ws), ws.numel(), st),
+                   "hier_search")
+        return best, score, scale
+
     def gls(self, Y: torch.Tensor, scale: torch.Tensor | None, best: torch.Tensor, want_resid_w: bool = False,
             want_residuals: bool = False, want_predicted: bool = False):
         V = Y.shape[0]
@@ -973,7 +996,8 @@ def fit_glm_arma11(data, design, tr, a_grid=None, b_grid=None, estimate_per_voxe
                    precomputed_arma_params=None, device=None, verbose=False, use_double=False,
                    force_exhaustive_search=False, task_indices=None, save_profile_likelihoods=False,
                    run_starts=None, tau=None, grid: RemlGrid | None = None, projection: str = "fp16x3",
-                   selector: str | None = None, proposal=None, **kwargs) -> ARMA11Results:
+                   selector: str | None = None, proposal=None, hierarchical_search: str = "descent",
+                   **kwargs) -> ARMA11Results:
     """glm/arma.py:3650-5878 ``fit_glm_arma11`` on the MI355X engine.
 
     Per-voxel REML search over the (a, b) grid (hierarchical descent by default,
@@ -995,8 +1019,15 @@ def fit_glm_arma11(data, design, tr, a_grid=None, b_grid=None, estimate_per_voxe
     the full grid for what is left; its target is the exhaustive global
     argmin).  ``proposal`` (learned only): a ``ProposalMLP``, a path written by
     ``ProposalMLP.save``, or None to train one for this design first
-    (``learned.train_proposal``, a few seconds).
+    (``learned.train_proposal``, a few seconds).  ``hierarchical_search``
+    (engine option for the hierarchical selector): "descent" (default) scores
+    only the lattice points the descent visits (``RemlGrid.hier_search``, the
+    same pairs and scores as the surface descent), "surface" computes the
+    full surface first; the surface is computed whenever it is an output
+    (``save_profile_likelihoods``).
     """
+    if hierarchical_search not in ("descent", "surface"):
+        raise ValueError(f"hierarchical_search must be 'descent' or 'surface', got {hierarchical_search!r}")
     dev = _lib.require_device(device)
     if selector == "learned":
         if force_exhaustive_search:
@@ -1180,6 +1211,12 @@ def fit_glm_arma11(data, design, tr, a_grid=None, b_grid=None, estimate_per_voxe
         dsort = _dsort_constant_guard(dsort)
         scale_all = torch.ones(V, dtype=torch.float64)
     params_dev = grid.params.to(dev)
+    # the hierarchical descent on demand: same pairs and scores as the descent
+    # on the full surface, when no surface is kept (the pair-set kernels take
+    # split operands for p <= 32 only)
+    use_descent = (fixed_best is None and searcher is None and not use_double and surface_acc is None
+                   and not force_exhaustive_search and selector in (None, "hierarchical")
+                   and hierarchical_search == "descent" and (grid.p <= 32 or grid.projection == "fp32"))
     with torch.cuda.device(dev):
         for b0 in range(0, V, nb):
             b1 = min(V, b0 + nb)
@@ -1198,6 +1235,8 @@ def fit_glm_arma11(data, design, tr, a_grid=None, b_grid=None, estimate_per_voxe
                 if surface_acc is not None:
                     surface_acc[b0:b1].copy_(S)
                 del S, Yd
+            elif fixed_best is None and use_descent:
+                best, score, scale = grid.hier_search(Yb)
             elif fixed_best is None:
                 Yt, scale, _ = grid.prep(Yb, normalize=not use_double)
                 S = grid.surface(Yt, Vb)

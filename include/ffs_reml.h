@@ -171,6 +171,24 @@ int ffs_reml_plan_set_projection(ffs_reml_plan* plan, int32_t mode);
 int ffs_reml_surface(const ffs_reml_plan* plan, const float* Yt, int64_t ldYt, int64_t V, float* S, int64_t ldS,
                      int32_t n_chunks, void* stream);
 
+/*
+ * Hierarchical descent on demand (glm/arma.py:2224-2322, the reference's GPU
+ * default selector): the levels that ffs_reml_select mode 1 runs on the full
+ * surface, run on pair sets holding only the lattice points the descent can
+ * visit (pass 0: the top level and the level below it, the same points for
+ * every voxel; then one pass per level, each voxel scored on the 3 x 3
+ * neighbourhood of its current point).  The set scores equal the surface
+ * bitwise, so best / best_score are those of ffs_reml_prep + ffs_reml_surface
+ * + ffs_reml_select(mode 1) for the same Y (normalize = 1), at about 45 pair
+ * evaluations per voxel instead of K.  scale (out): each voxel's std, as
+ * ffs_reml_prep writes it.  ffs_reml_plan_build_hier builds the pass tables
+ * once per plan (about 60 MB at K = 561 on a 33 x 33 lattice, T = 380).
+ */
+int ffs_reml_plan_build_hier(ffs_reml_plan* plan, void* stream);
+int ffs_reml_hier_workspace(const ffs_reml_plan* plan, int64_t V, int64_t* bytes);
+int ffs_reml_hier_search(const ffs_reml_plan* plan, const float* Y, int64_t ldY, int64_t V, int32_t* best,
+                         float* best_score, float* scale, void* workspace, int64_t ws_bytes, void* stream);
+
 /* (iii) mode 0: global argmin (first minimum), glm/arma.py:1886-2018;
  * mode 1: hierarchical descent on the surface, glm/arma.py:2224-2322 (the
  * reference's GPU default); mode 2: subset descent, glm/arma.py:2021-2221 (the
