@@ -70,6 +70,7 @@ struct ffs_reml_plan {
     // host copies of the pair list and lattice (band tables are built from them)
     std::vector<double> h_ab;
     std::vector<int32_t> h_lut;
+    std::vector<int32_t> h_white;
     // passes of the hierarchical descent on pair sets (ffs_reml_plan_build_hier)
     std::vector<HierPass> hier;
     template <class Tp>
@@ -250,6 +251,7 @@ static int plan_alloc(ffs_reml_plan* P, int32_t T, int32_t p, const double* X, c
     PlanDev& d = P->d;
     P->gls_only = gls_only;
     P->h_ab.assign(ab, ab + (size_t)2 * K);
+    P->h_white.assign(white, white + K);
     if (!gls_only) P->h_lut.assign(lut, lut + (size_t)n_a * n_b);
     d.T = T;
     d.Tpad = (T + kProjBK - 1) / kProjBK * kProjBK;
@@ -397,6 +399,19 @@ static int plan_alloc(ffs_reml_plan* P, int32_t T, int32_t p, const double* X, c
             return rc;
         HIP_TRY(hipMemsetAsync(G64p, 0, sizeof(double) * K * d.Tpad * PG, s));
     }
+    // GLS epilogue tables (f64 MFMA GLS widths): written after k_tables
+    double *gls_epi = nullptr, *dXtXp = nullptr;
+    if (d.PG) {
+        const int PG = d.PG;
+        std::vector<double> XtXp((size_t)PG * PG, 0.0);
+        for (int a = 0; a < p; ++a)
+            for (int b = 0; b < p; ++b) XtXp[(size_t)a * PG + b] = XtX[(size_t)a * p + b];
+        if ((rc = P->alloc(&gls_epi, (size_t)K * (2 * PG * PG + PG))) ||
+            (rc = upload(P, &dXtXp, XtXp.data(), XtXp.size(), s)))
+            return rc;
+    }
+    d.gls_epi = gls_epi;
+    d.XtXp = dXtXp;
     // recurrence tables for every p: K * Tpad * 32 bytes
     if ((rc = P->alloc(&coefR, (size_t)K * d.Tpad * 4))) return rc;
     HIP_TRY(hipMemsetAsync(coefR, 0, sizeof(double) * K * d.Tpad * 4, s));
@@ -479,6 +494,12 @@ static int plans_build(std::vector<PlanPending>& pend, const std::vector<TablesO
         HIP_TRY(hipGetLastError());
         hipLaunchKernelGGL(k_energy_tables, dim3(d.n_chunks), dim3(128), 0, s, d.T, d.n_chunks, q.dchunk, q.soa,
                            q.ecoef, q.ec, q.es0);
+        HIP_TRY(hipGetLastError());
+    }
+    for (PlanPending& q : pend) {
+        const PlanDev& d = q.P->d;
+        if (!d.gls_epi) continue;
+        hipLaunchKernelGGL(k_gls_epi_tables, dim3(d.K), dim3(64), 0, s, d, const_cast<double*>(d.gls_epi));
         HIP_TRY(hipGetLastError());
     }
     std::vector<int32_t> status(G, 0);
@@ -1115,10 +1136,13 @@ int host_clamp_center(int n_a, int n_b, int ai, int bi) {
 }
 
 // Uploads a family of pair sets (pairs in pair order) with their edge / next
-// flags and builds the energy chunk tables.
+// flags and builds the energy chunk tables.  Energy chunks: the slots of a set
+// are grouped by AR coefficient sequence (same a and lambda == 0 flag; groups
+// in order of first slot), each group cut into pieces of gsize positions, and
+// ngroups pieces make a chunk.
 int build_set_table(ffs_reml_plan* P, const std::vector<std::vector<int32_t>>& sets,
                     const std::vector<std::vector<int32_t>>& edges, const std::vector<std::vector<int32_t>>* nexts,
-                    SetTable* out, hipStream_t s) {
+                    SetTable* out, hipStream_t s, int gsize, int ngroups) {
     PlanDev& d = P->d;
     size_t maxn = 0;
     for (const auto& v : sets) maxn = std::max(maxn, v.size());
@@ -1127,6 +1151,9 @@ int build_set_table(ffs_reml_plan* P, const std::vector<std::vector<int32_t>>& s
     const int n = (int)sets.size();
     std::vector<int32_t> pairs((size_t)n * SB, -1), edge((size_t)n * SB, 0), next, cnt(n, 0);
     if (nexts) next.assign((size_t)n * SB, -1);
+    const int NB = gsize * ngroups;
+    std::vector<std::vector<int32_t>> chunks(n);  // positions of each set's chunks
+    int MC = 1;
     for (int i = 0; i < n; ++i) {
         cnt[i] = (int32_t)sets[i].size();
         for (size_t q = 0; q < sets[i].size(); ++q) {
@@ -1134,15 +1161,39 @@ int build_set_table(ffs_reml_plan* P, const std::vector<std::vector<int32_t>>& s
             edge[(size_t)i * SB + q] = edges[i][q];
             if (nexts) next[(size_t)i * SB + q] = (*nexts)[i][q];
         }
+        std::vector<std::vector<int32_t>> groups;
+        for (size_t q = 0; q < sets[i].size(); ++q) {
+            const int k = sets[i][q];
+            size_t g = 0;
+            for (; g < groups.size(); ++g) {
+                const int k0 = sets[i][groups[g][0]];
+                if (P->h_ab[2 * k0] == P->h_ab[2 * k] && P->h_white[k0] == P->h_white[k]) break;
+            }
+            if (g == groups.size()) groups.emplace_back();
+            groups[g].push_back((int32_t)q);
+        }
+        std::vector<int32_t>& c = chunks[i];
+        for (const auto& g : groups)
+            for (size_t q = 0; q < g.size(); q += gsize)
+                for (int j = 0; j < gsize; ++j) c.push_back(q + j < g.size() ? g[q + j] : -1);
+        while (c.size() % NB) c.push_back(-1);
+        MC = std::max(MC, (int)(c.size() / NB));
     }
-    int32_t *dpairs, *dedge, *dcnt, *dnext = nullptr;
+    std::vector<int32_t> eslot((size_t)n * MC * NB, -1), ecnt(n);
+    for (int i = 0; i < n; ++i) {
+        ecnt[i] = (int32_t)(chunks[i].size() / NB);
+        std::copy(chunks[i].begin(), chunks[i].end(), eslot.begin() + (size_t)i * MC * NB);
+    }
+    int32_t *dpairs, *dedge, *dcnt, *dnext = nullptr, *deslot, *decnt;
     float *coef, *s0;
-    const size_t nchunks = (size_t)n * (SB / kEnergyNB);
+    const size_t nchunks = (size_t)n * MC;
     int rc = 0;
     if ((rc = upload(P, &dpairs, pairs.data(), pairs.size(), s)) ||
         (rc = upload(P, &dedge, edge.data(), edge.size(), s)) || (rc = upload(P, &dcnt, cnt.data(), cnt.size(), s)) ||
         (nexts && (rc = upload(P, &dnext, next.data(), next.size(), s))) ||
-        (rc = P->alloc(&coef, nchunks * d.T * 3 * kEnergyNB)) || (rc = P->alloc(&s0, (size_t)n * SB)))
+        (rc = upload(P, &deslot, eslot.data(), eslot.size(), s)) ||
+        (rc = upload(P, &decnt, ecnt.data(), ecnt.size(), s)) ||
+        (rc = P->alloc(&coef, nchunks * d.T * (ngroups + 2 * NB))) || (rc = P->alloc(&s0, nchunks * NB)))
         return rc;
     SetTable t{};
     t.n = n;
@@ -1151,6 +1202,11 @@ int build_set_table(ffs_reml_plan* P, const std::vector<std::vector<int32_t>>& s
     t.pairs = dpairs;
     t.edge = dedge;
     t.next = dnext;
+    t.enb = NB;
+    t.eng = ngroups;
+    t.mc = MC;
+    t.ecnt = decnt;
+    t.eslot = deslot;
     t.coef = coef;
     t.s0 = s0;
     hipLaunchKernelGGL(k_set_tables, dim3((unsigned)nchunks), dim3(128), 0, s, d, t, coef, s0);
@@ -1184,7 +1240,9 @@ int build_windows(ffs_reml_plan* P, hipStream_t s) {
                     nexts[w].push_back(e ? host_clamp_center(n_a, n_b, ai, bi) : -1);
                 }
         }
-    return build_set_table(P, sets, edges, &nexts, &P->d.win, s);
+    // a window row (one a) forms one chunk of kWin = 5 positions
+    static_assert(kWin == 5, "window rows use the 5-position energy chunk");
+    return build_set_table(P, sets, edges, &nexts, &P->d.win, s, kWin, 1);
 }
 
 struct SetLayout {
@@ -1262,11 +1320,19 @@ int set_group_prep(const ffs_reml_plan* P, const SetTable& ST, const float* Y, i
                      false, coef, coef_out);
 }
 int set_energy(const ffs_reml_plan* P, const SetTable& ST, const SetLayout& L, const SetPtrs& q, hipStream_t s) {
-    const int64_t n_cg = (ST.sb / kEnergyNB + kSetWaves - 1) / kSetWaves;
-    const int64_t items = L.max_tiles * n_cg;
-    const unsigned nblk = (unsigned)std::min<int64_t>(2048, std::max<int64_t>(8, (items + 7) / 8 * 8));
-    hipLaunchKernelGGL(k_set_energy, dim3(nblk), dim3(64 * kSetWaves), 0, s, P->d, ST, q.Yt, (int64_t)L.max_cols,
-                       q.tset, q.nt, q.S);
+    // one block per (tile octet, chunk) of the largest tile count, 8 * ceil(items / 8)
+    const int64_t items = (L.max_tiles + kSetEnergyWaves - 1) / kSetEnergyWaves * ST.mc;
+    const dim3 g((unsigned)(8 * ((items + 7) / 8))), b(64 * kSetEnergyWaves);
+    const int64_t ld = L.max_cols;
+    if (ST.enb == 5 && ST.eng == 1)
+        hipLaunchKernelGGL((k_set_energy<5, 1>), g, b, 0, s, P->d, ST, q.Yt, ld, q.tset, q.nt, q.S);
+    else if (ST.enb == 3 && ST.eng == 1)
+        hipLaunchKernelGGL((k_set_energy<3, 1>), g, b, 0, s, P->d, ST, q.Yt, ld, q.tset, q.nt, q.S);
+    else if (ST.enb == kEnergyNB && ST.eng == kEnergyNB)
+        hipLaunchKernelGGL((k_set_energy<kEnergyNB, kEnergyNB>), g, b, 0, s, P->d, ST, q.Yt, ld, q.tset, q.nt,
+                           q.S);
+    else
+        return fail(-3, "set_energy: no kernel for this chunk shape");
     HIP_TRY(hipGetLastError());
     return 0;
 }
@@ -1438,7 +1504,8 @@ extern "C" int ffs_reml_plan_build_hier(ffs_reml_plan* P, void* stream) {
             edges[0].push_back(0);
         }
         HierPass h{};
-        if (int rc = build_set_table(P, sets, edges, nullptr, &h.ST, s)) return rc;
+        // lattice rows of the top levels, 5 points to a chunk
+        if (int rc = build_set_table(P, sets, edges, nullptr, &h.ST, s, 5, 1)) return rc;
         int32_t* d;
         if (int rc = upload(P, &d, pslot.data(), pslot.size(), s)) return rc;
         h.point_slot = d;
@@ -1474,7 +1541,8 @@ extern "C" int ffs_reml_plan_build_hier(ffs_reml_plan* P, void* stream) {
             edges.push_back(edge);
         }
         HierPass h{};
-        if (int rc = build_set_table(P, sets, edges, nullptr, &h.ST, s)) return rc;
+        // a 3 x 3 neighbourhood: each of its rows (one a) forms a chunk
+        if (int rc = build_set_table(P, sets, edges, nullptr, &h.ST, s, 3, 1)) return rc;
         int32_t *dc, *dv;
         if (int rc = upload(P, &dc, cset.data(), cset.size(), s)) return rc;
         if (int rc = upload(P, &dv, vmap.data(), vmap.size(), s)) return rc;
@@ -1619,7 +1687,8 @@ extern "C" int ffs_reml_plan_set_bands(ffs_reml_plan* P, double half_width, doub
     }
     if (maxn == 0) return fail(-3, "plan_set_bands: no pair lies in any band");
     SetTable t{};
-    if (int rc = build_set_table(P, bands, edges, nullptr, &t, (hipStream_t)stream)) return rc;
+    // a band holds few pairs of each a: one position per group, 8 to a chunk
+    if (int rc = build_set_table(P, bands, edges, nullptr, &t, (hipStream_t)stream, 1, kEnergyNB)) return rc;
     d.band = t;
     d.band_lo = -1.f;
     d.band_h = (float)bin_width;

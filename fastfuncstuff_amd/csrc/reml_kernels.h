@@ -50,14 +50,21 @@ constexpr int kGlsMfmaMaxP = 32;
 // Pair-set search: voxel groups padded to 64-column tiles; blocks of 4 waves.
 constexpr int kSetTile = 64;
 constexpr int kSetWaves = 4;
+// Waves per block of the set energy kernel (one chunk on 8 consecutive tiles).
+constexpr int kSetEnergyWaves = 8;
 
 // A family of pair sets (lambda bands, lattice windows).  Set s holds cnt[s]
 // pairs, in pair order, in slots [s*sb, s*sb + cnt[s]) (-1 past them); sb is a
 // multiple of kEnergyNB and of 4 projection tiles.  Per slot: edge = 1 when
 // the pair has a lattice neighbour that the set misses (the set's optimum is
 // then not trusted), next = the window to continue with from that pair
-// (windows only; -1 = none).  coef / s0: interleaved energy coefficients of
-// each 8-slot chunk (c_t, alpha_t, 1/v_{t+1}) and 1/v_0.
+// (windows only; -1 = none).
+// Energy chunks: chunk m of set s has enb positions in eng groups of enb / eng
+// consecutive positions; the pairs of a group share the AR coefficient
+// sequence c_t (same a, same lambda == 0 flag), so its filtered sample is
+// formed once.  eslot maps each position to a slot of the set (-1: empty);
+// coef holds per step the c_t of each group, then (alpha_t, 1/v_{t+1}) of each
+// position; s0 = 1/v_0 per position.
 struct SetTable {
     int n;
     int sb;
@@ -65,8 +72,11 @@ struct SetTable {
     const int32_t* pairs;  // [n][sb]
     const int32_t* edge;   // [n][sb]
     const int32_t* next;   // [n][sb] or null
-    const float* coef;     // [n * sb / kEnergyNB][T][3 * kEnergyNB]
-    const float* s0;       // [n][sb]
+    int enb, eng, mc;      // positions and groups per energy chunk; chunks per set (stride)
+    const int32_t* ecnt;   // [n] energy chunks in use
+    const int32_t* eslot;  // [n][mc][enb]
+    const float* coef;     // [n][mc][T][eng + 2 enb]
+    const float* s0;       // [n][mc][enb]
 };
 
 // Scalar arguments of k_tables shared by every design of one launch (one
@@ -141,6 +151,10 @@ struct PlanDev {
     const double* pinvTp;      // [Tpad][PG]     pinv(X)', zero padded
     const double* coefR;       // [K][Tpad][4]   (sqrt(1/v_t), c_{t-1}, alpha_{t-1}, 0); c = alpha = 0 at run starts
     const double* XtX;         // [p][p]
+    // GLS epilogue tables padded to PG (zero past p; PG > 0 only): per pair
+    // Lx transposed, the F form and diag (X'R^-1X)^-1, and X'X
+    const double* gls_epi;     // [K][2 PG^2 + PG]
+    const double* XtXp;        // [PG][PG]
     // rank-deficient design: Pi = projection onto the row space of X (p x p);
     // null when X has full column rank.  The tables then hold the rank-r
     // forms (zero pivots and Q columns for the dependent columns, Ainv =

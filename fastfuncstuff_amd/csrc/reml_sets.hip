@@ -27,36 +27,43 @@
 
 namespace remlk {
 
-// Interleaved per-chunk energy coefficients of a set table:
-// coef[chunk][t][3 j + (0, 1, 2)] = (c_t, alpha_t, 1/v_{t+1}) of slot j, the
-// same fp32 roundings as the full-grid chunk tables; s0 = 1/v_0.
+// Energy chunk coefficients of a set table, one block per (set, chunk): per
+// step the c_t of each group (taken from its first pair: the pairs of a group
+// share it), then (alpha_t, 1/v_{t+1}) of each position, zero for empty
+// positions; s0 = 1/v_0.  The same fp32 values as the full-grid chunk tables.
 __global__ void k_set_tables(PlanDev P, SetTable ST, float* __restrict__ coef, float* __restrict__ s0) {
-    const int MC = ST.sb / kEnergyNB;
     const int cb = blockIdx.x;
-    if (cb >= ST.n * MC) return;
-    const int set = cb / MC, chunk = cb % MC;
-    const int T = P.T;
-    const int32_t* slots = ST.pairs + (size_t)set * ST.sb + chunk * kEnergyNB;
+    if (cb >= ST.n * ST.mc) return;
+    const int set = cb / ST.mc, m = cb % ST.mc;
+    if (m >= ST.ecnt[set]) return;
+    const int T = P.T, NB = ST.enb, NG = ST.eng, GS = NB / NG, W = NG + 2 * NB;
+    const int32_t* pos = ST.eslot + (size_t)cb * NB;
+    const int32_t* slots = ST.pairs + (size_t)set * ST.sb;
     for (int t = threadIdx.x; t < T; t += blockDim.x) {
-        float* o = coef + ((size_t)cb * T + t) * 3 * kEnergyNB;
-#pragma unroll
-        for (int j = 0; j < kEnergyNB; ++j) {
-            const int k = slots[j];
-            float c = 0.f, al = 0.f, sn = 0.f;
-            if (k >= 0) {
-                const float2 q = P.lp[(size_t)k * T + t];
-                c = P.lc[(size_t)k * T + t];
-                al = q.x;
-                sn = q.y;
+        float* o = coef + ((size_t)cb * T + t) * W;
+        for (int g = 0; g < NG; ++g) {
+            float c = 0.f;
+            for (int i = GS - 1; i >= 0; --i) {
+                const int q = pos[g * GS + i];
+                if (q >= 0) c = P.lc[(size_t)slots[q] * T + t];
             }
-            o[3 * j] = c;
-            o[3 * j + 1] = al;
-            o[3 * j + 2] = sn;
+            o[g] = c;
+        }
+        for (int j = 0; j < NB; ++j) {
+            const int q = pos[j];
+            float al = 0.f, sn = 0.f;
+            if (q >= 0) {
+                const float2 v = P.lp[(size_t)slots[q] * T + t];
+                al = v.x;
+                sn = v.y;
+            }
+            o[NG + 2 * j] = al;
+            o[NG + 2 * j + 1] = sn;
         }
     }
-    if (threadIdx.x < kEnergyNB) {
-        const int k = slots[threadIdx.x];
-        s0[(size_t)set * ST.sb + chunk * kEnergyNB + threadIdx.x] = k >= 0 ? P.ls0[k] : 0.f;
+    for (int j = threadIdx.x; j < NB; j += blockDim.x) {
+        const int q = pos[j];
+        s0[(size_t)cb * NB + j] = q >= 0 ? P.ls0[slots[q]] : 0.f;
     }
 }
 
@@ -148,82 +155,104 @@ __device__ inline SetItems set_items(int n_tiles, int per_tile) {
     return r;
 }
 
-// Energies E[col][slot] of every live 64-column tile and every 8-slot chunk of
-// its set.  An item is (chunk, quad of 4 consecutive tiles of the XCD's range),
-// wave w taking tile 4 quad + w: the waves of a block, which mostly share a set,
-// stream the same chunk coefficients through the scalar cache, and the items
-// run chunk-major, so the blocks of an XCD share them in L2.
-__global__ void __launch_bounds__(64 * kSetWaves) k_set_energy(PlanDev P, SetTable ST, const float* __restrict__ Yt,
-                                                               int64_t ldY, const int32_t* __restrict__ tset,
-                                                               const int32_t* __restrict__ n_tiles,
-                                                               float* __restrict__ Eb) {
+// Energies E[col][slot] of every live 64-column tile and every energy chunk of
+// its set: the k_energy recurrence and fp64 flush schedule, so each energy
+// equals the full-grid one bitwise.  NB positions per chunk in NG groups; per
+// step the filtered sample z = y_{t+1} - c_t y_t of each group, then per
+// position e <- alpha e + z, E += e^2 / v.  Items are (octet of consecutive
+// tiles, chunk), chunk fastest, each XCD (block id % 8) taking a contiguous
+// eighth: the kSetEnergyWaves waves of a block run one chunk on the octet's
+// tiles, which mostly share a set and so its coefficient stream (scalar-cache
+// hits), and the chunks of an octet run together on one XCD, so its residual
+// columns come from that XCD's L2 after the first.  The grid is sized for the
+// largest tile count; blocks past the live items return at once.
+template <int NB, int NG>
+__global__ void __launch_bounds__(64 * kSetEnergyWaves) k_set_energy(PlanDev P, SetTable ST,
+                                                                     const float* __restrict__ Yt, int64_t ldY,
+                                                                     const int32_t* __restrict__ tset,
+                                                                     const int32_t* __restrict__ n_tiles,
+                                                                     float* __restrict__ Eb) {
+    constexpr int GS = NB / NG, W = NG + 2 * NB;
+    static_assert(GS * NG == NB, "groups must split the chunk evenly");
     const int lane = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int SB = ST.sb, MC = SB / kEnergyNB;
-    const int nt = *n_tiles, per = (nt + 7) >> 3;
-    const int lo = (blockIdx.x & 7) * per;
-    const int n_xt = max(0, min(nt, lo + per) - lo);  // tiles of this XCD
-    const int nq = (n_xt + kSetWaves - 1) / kSetWaves;
-    const int n_items = nq * MC;
+    const int nt = *n_tiles, MC = ST.mc;
+    const int n_xo = (nt + kSetEnergyWaves - 1) / kSetEnergyWaves;
+    const int n_items = n_xo * MC;
+    const int per = (n_items + 7) >> 3;
+    const int local = blockIdx.x >> 3;
+    const int item = (blockIdx.x & 7) * per + local;
+    if (local >= per || item >= n_items) return;
+    const int xo = item / MC, chunk = item % MC;
+    const int tt = xo * kSetEnergyWaves + w;
+    if (tt >= nt) return;
+    const int set = tset[tt];
+    if (chunk >= ST.ecnt[set]) return;
     const int T = P.T;
+    const size_t cb = (size_t)set * MC + chunk;
+    const int col = tt * kSetTile + lane;
+    const float* __restrict__ y = Yt + col;
+    const float* __restrict__ co = ST.coef + cb * T * W;
+    const float* __restrict__ s0 = ST.s0 + cb * NB;
+    float e[NB], part[NB];
+    double acc[NB];
+    const float y0 = y[0];
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+        e[j] = y0;
+        part[j] = y0 * y0 * s0[j];
+        acc[j] = 0.0;
+    }
+    float yc = y0;
+    auto step = [&](float yn, const float* c) __attribute__((always_inline)) {
+        float z[NG];
+#pragma unroll
+        for (int g = 0; g < NG; ++g) z[g] = fmaf(-c[g], yc, yn);
+#pragma unroll
+        for (int j = 0; j < NB; ++j) {
+            e[j] = fmaf(c[NG + 2 * j], e[j], z[j / GS]);
+            part[j] = fmaf(e[j] * c[NG + 2 * j + 1], e[j], part[j]);
+        }
+        yc = yn;
+    };
     constexpr int U = 8;  // steps per fp64 flush
+    constexpr int G = 2;  // steps whose coefficients are fetched together
     const int nsteps = T - 1;
     const int nfull = nsteps / U * U;
-    for (int it = blockIdx.x >> 3; it < n_items; it += gridDim.x >> 3) {
-        const int chunk = it / nq, tl = (it % nq) * kSetWaves + w;
-        if (tl >= n_xt) continue;
-        const int tt = lo + tl;
-        const int set = tset[tt];
-        if (chunk * kEnergyNB >= ST.cnt[set]) continue;  // past the set
-        const int col = tt * kSetTile + lane;
-        const float* __restrict__ y = Yt + col;
-        const float* __restrict__ co = ST.coef + ((size_t)set * MC + chunk) * T * 3 * kEnergyNB;
-        const float* __restrict__ s0 = ST.s0 + (size_t)set * SB + chunk * kEnergyNB;
-        float e[kEnergyNB], part[kEnergyNB];
-        double acc[kEnergyNB];
-        const float y0 = y[0];
+    // samples of the next block, fetched one block ahead (indices clamped: the
+    // values past T are never used)
+    float ybuf[U];
 #pragma unroll
-        for (int j = 0; j < kEnergyNB; ++j) {
-            e[j] = y0;
-            part[j] = y0 * y0 * s0[j];
-            acc[j] = 0.0;
+    for (int i = 0; i < U; ++i) ybuf[i] = y[(size_t)min(1 + i, T - 1) * ldY];
+    for (int t0 = 0; t0 < nfull; t0 += U) {
+        float ycur[U];
+#pragma unroll
+        for (int i = 0; i < U; ++i) ycur[i] = ybuf[i];
+#pragma unroll
+        for (int i = 0; i < U; ++i) ybuf[i] = y[(size_t)min(t0 + U + 1 + i, T - 1) * ldY];
+#pragma unroll
+        for (int g0 = 0; g0 < U; g0 += G) {
+            float cg[G][W];
+#pragma unroll
+            for (int i = 0; i < G; ++i)
+#pragma unroll
+                for (int q = 0; q < W; ++q) cg[i][q] = co[(size_t)(t0 + g0 + i) * W + q];
+#pragma unroll
+            for (int i = 0; i < G; ++i) step(ycur[g0 + i], cg[i]);
         }
-        float yc = y0;
-        auto step = [&](float yn, const float* c) __attribute__((always_inline)) {
 #pragma unroll
-            for (int j = 0; j < kEnergyNB; ++j) {
-                const float z = fmaf(-c[3 * j], yc, yn);
-                e[j] = fmaf(c[3 * j + 1], e[j], z);
-                part[j] = fmaf(e[j] * c[3 * j + 2], e[j], part[j]);
-            }
-            yc = yn;
-        };
-        float ybuf[U];
-#pragma unroll
-        for (int i = 0; i < U; ++i) ybuf[i] = y[(size_t)min(1 + i, T - 1) * ldY];
-        for (int t0 = 0; t0 < nfull; t0 += U) {
-            float ycur[U];
-#pragma unroll
-            for (int i = 0; i < U; ++i) ycur[i] = ybuf[i];
-#pragma unroll
-            for (int i = 0; i < U; ++i) ybuf[i] = y[(size_t)min(t0 + U + 1 + i, T - 1) * ldY];
-#pragma unroll
-            for (int i = 0; i < U; ++i) {
-                float cg[3 * kEnergyNB];
-#pragma unroll
-                for (int q = 0; q < 3 * kEnergyNB; ++q) cg[q] = co[(size_t)(t0 + i) * 3 * kEnergyNB + q];
-                step(ycur[i], cg);
-            }
-#pragma unroll
-            for (int j = 0; j < kEnergyNB; ++j) {
-                acc[j] += part[j];
-                part[j] = 0.f;
-            }
+        for (int j = 0; j < NB; ++j) {
+            acc[j] += part[j];
+            part[j] = 0.f;
         }
-        for (int t = nfull; t < nsteps; ++t) step(y[(size_t)(t + 1) * ldY], co + (size_t)t * 3 * kEnergyNB);
-        float* out = Eb + (size_t)col * SB + chunk * kEnergyNB;
+    }
+    for (int t = nfull; t < nsteps; ++t) step(y[(size_t)(t + 1) * ldY], co + (size_t)t * W);
+    float* out = Eb + (size_t)col * ST.sb;
+    const int32_t* pos = ST.eslot + cb * NB;
 #pragma unroll
-        for (int j = 0; j < kEnergyNB; ++j) out[j] = (float)(acc[j] + part[j]);
+    for (int j = 0; j < NB; ++j) {
+        const int q = pos[j];
+        if (q >= 0) out[q] = (float)(acc[j] + part[j]);
     }
 }
 

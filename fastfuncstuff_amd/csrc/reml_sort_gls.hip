@@ -250,6 +250,27 @@ __device__ __forceinline__ void gls_half(int t0, int hf, int j, int kq, int T, b
         }
 }
 
+// Epilogue tables of k_gls_mfma for pair k (one wave): Lx transposed
+// (LT[j][m] = Lx[m][j] for m >= j, zero above), the F quadratic form and
+// diag (X'R^-1X)^-1, padded to PG x PG with zeros, so the epilogue runs
+// fixed-size inner loops whose padded terms add exact zeros.
+__global__ void __launch_bounds__(64) k_gls_epi_tables(PlanDev P, double* __restrict__ out) {
+    const int k = blockIdx.x;
+    if (k >= P.K) return;
+    const int p = P.p, PG = P.PG;
+    const double* __restrict__ L = P.Lx + (size_t)k * p * p;
+    const double* __restrict__ F = P.Fm + (size_t)k * p * p;
+    const double* __restrict__ A = P.Ainv + (size_t)k * p * p;
+    double* o = out + (size_t)k * (2 * PG * PG + PG);
+    for (int i = threadIdx.x; i < PG * PG; i += 64) {
+        const int r = i / PG, c = i % PG;
+        const bool in = r < p && c < p;
+        o[i] = in && c >= r ? L[(size_t)c * p + r] : 0.0;  // Lx lower triangle only
+        o[PG * PG + i] = in ? F[(size_t)r * p + c] : 0.0;
+    }
+    for (int j = threadIdx.x; j < PG; j += 64) o[2 * PG * PG + j] = j < p ? A[(size_t)j * p + j] : 0.0;
+}
+
 template <int NT>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) k_gls_mfma(PlanDev P, const float* __restrict__ Y, int64_t ldY,
                                                  const float* __restrict__ scale, const int32_t* __restrict__ perm,
@@ -333,20 +354,29 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) k_
     E *= inv2;
     sy *= inv;
     syy *= inv2;
-    // beta = Lx^-T h (back substitution on the upper factor Lx'), h = G'y / s;
-    // beta lives in the lane's LDS row (runtime loops keep the register count
-    // of the epilogue below the main loop's)
-    const double* __restrict__ L = P.Lx + (size_t)k * p * p;
+    // beta = Lx^-T h (back substitution on the upper factor Lx'), h = G'y / s.
+    // beta lives in the lane's LDS row, zero-filled to PG; the inner loops run
+    // over the padded width PG with the epilogue tables (Lx' zero below its
+    // diagonal and past p, X'X and F zero past p), so every padded term adds
+    // an exact zero and the sums equal those over p columns, while the loads of
+    // a row are issued together.  The outer loops stay runtime loops (their
+    // index addresses the LDS row).
+    const double* __restrict__ ep = P.gls_epi + (size_t)k * (2 * PG * PG + PG);
+    const double* __restrict__ LT = ep;
+    const double* __restrict__ Fq = ep + PG * PG;
+    const double* __restrict__ Ad = ep + 2 * PG * PG;
     double* be = bes + lane * TP;
+#pragma unroll
+    for (int m = 0; m < PG; ++m) be[m] = 0.0;
     double hh = 0.0;
 #pragma unroll 1
     for (int jj = p - 1; jj >= 0; --jj) {
         const double hj = tbuf[lane * TP + jj] * inv;
         hh = fma(hj, hj, hh);
         double acc = hj;
-#pragma unroll 1
-        for (int m = jj + 1; m < p; ++m) acc -= L[m * p + jj] * be[m];
-        const double ljj = L[jj * p + jj];
+#pragma unroll
+        for (int m = 0; m < PG; ++m) acc -= LT[jj * PG + m] * be[m];  // m <= jj: zero terms
+        const double ljj = LT[jj * PG + jj];
         be[jj] = ljj != 0.0 ? acc / ljj : 0.0;  // zero pivot: basic solution
     }
     __syncthreads();
@@ -359,13 +389,13 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) k_
     __syncthreads();
     if (v < 0) return;
     double bq = 0.0, bxb = 0.0;
-    const double* __restrict__ XtX = P.XtX;
+    const double* __restrict__ XtX = P.XtXp;
 #pragma unroll 1
     for (int a = 0; a < p; ++a) {
         bq = fma(be[a], tbuf[lane * TP + a] * inv, bq);
         double ra = 0.0;
-#pragma unroll 1
-        for (int b2 = 0; b2 < p; ++b2) ra = fma(XtX[a * p + b2], be[b2], ra);
+#pragma unroll
+        for (int b2 = 0; b2 < PG; ++b2) ra = fma(XtX[a * PG + b2], be[b2], ra);
         bxb = fma(be[a], ra, bxb);
     }
     const double ssw = E - hh;
@@ -373,20 +403,18 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) k_
     const double sstot = syy - sy * (sy / T);
     const int dof = max(1, T - p);
     const double sig = ssw / dof;
-    const double* __restrict__ Ai = P.Ainv + (size_t)k * p * p;
 #pragma unroll 1
     for (int jj = 0; jj < p; ++jj) {
-        const double se = sqrt(sig * Ai[jj * p + jj]);
+        const double se = sqrt(sig * Ad[jj]);
         tstats[(size_t)v * p + jj] = (float)(be[jj] / (se + 1e-10));
         betas[(size_t)v * p + jj] = (float)(be[jj] * s);
     }
-    const double* __restrict__ F = P.Fm + (size_t)k * p * p;
     double quad = 0.0;
 #pragma unroll 1
     for (int a = 0; a < p; ++a) {
         double ra = 0.0;
-#pragma unroll 1
-        for (int b2 = 0; b2 < p; ++b2) ra += F[a * p + b2] * be[b2];
+#pragma unroll
+        for (int b2 = 0; b2 < PG; ++b2) ra += Fq[a * PG + b2] * be[b2];
         quad += be[a] * ra;
     }
     fstats[v] = (float)(quad / (sig * P.n_task + 1e-10));
