@@ -8,6 +8,7 @@ from __future__ import annotations
 
 import ctypes
 import os
+import threading
 from pathlib import Path
 
 import torch
@@ -149,3 +150,48 @@ def ptr(t: torch.Tensor | None):
 
 def stream_handle(device: torch.device):
     return ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
+
+
+# Host -> device upload of large host arrays through two pinned staging
+# buffers: the host copies chunk i + 1 into one buffer while the DMA of chunk i
+# from the other runs, instead of the runtime's own staging of a pageable
+# source (about 2x the rate for a few hundred MB).  The buffers are allocated
+# pinned once per device and reused; a lock serialises their use.
+_STAGE_BYTES = 32 << 20
+_STAGE_MIN_BYTES = 16 << 20
+_stage: dict = {}
+_stage_lock = threading.Lock()
+
+
+def upload_f32(src, device: torch.device) -> torch.Tensor:
+    """``src`` (array-like, any real dtype) as a contiguous float32 tensor on
+    ``device``, in stream order on the current stream (the values are those
+    of ``torch.as_tensor(src).to(device, torch.float32)``)."""
+    src = torch.as_tensor(src)
+    if (src.device.type != "cpu" or src.numel() * 4 < _STAGE_MIN_BYTES or src.ndim == 0 or src.is_pinned()
+            or not src.is_contiguous()):
+        return src.to(device=device, dtype=torch.float32).contiguous()
+    out = torch.empty(src.shape, dtype=torch.float32, device=device)
+    flat_src = src.reshape(-1)
+    flat_out = out.reshape(-1)
+    n = flat_src.numel()
+    chunk = _STAGE_BYTES // 4
+    stream = torch.cuda.current_stream(device)
+    with _stage_lock:
+        key = device.index if device.index is not None else torch.cuda.current_device()
+        st = _stage.get(key)
+        if st is None:
+            st = _stage[key] = {"buf": [torch.empty(chunk, dtype=torch.float32).pin_memory() for _ in range(2)],
+                                "ev": [None, None]}
+        for i, o in enumerate(range(0, n, chunk)):
+            m = min(chunk, n - o)
+            slot = i & 1
+            if st["ev"][slot] is not None:
+                st["ev"][slot].synchronize()  # the DMA that last read this buffer is done
+            buf = st["buf"][slot][:m]
+            buf.copy_(flat_src[o:o + m])
+            flat_out[o:o + m].copy_(buf, non_blocking=True)
+            ev = torch.cuda.Event()
+            ev.record(stream)
+            st["ev"][slot] = ev
+    return out

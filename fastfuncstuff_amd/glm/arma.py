@@ -671,8 +671,7 @@ def precompute_reml_grid(X, n_timepoints, a_grid, b_grid, device=None, verbose=F
 
 
 def _to_device_f32(Y, device) -> torch.Tensor:
-    Y = torch.as_tensor(Y)
-    return Y.to(device=device, dtype=torch.float32).contiguous()
+    return _lib.upload_f32(Y, device)
 
 
 def search_voxels_precomputed_grid(X, Y_batch, precomputed_grid: RemlGrid, device=None, verbose=False,

@@ -226,3 +226,24 @@ def test_prewhiten_ljung_box_and_comparison_helpers(dev):
     assert fit.ljung_box is not None and bool(torch.isfinite(fit.ljung_box).all())
     out = compare_ols_vs_arma11(Yv, X.numpy(), 2.0, device=dev)
     assert set(out) >= {"ols", "arma11", "tstat_ratio", "r2_improvement", "summary"}
+
+
+def test_staged_upload_equals_plain_copy(dev):
+    """Host arrays of 16 MB and more go to the device through two pinned
+    staging buffers (chunks of 32 MB); the device tensor equals torch's own
+    copy for float32 and float64 sources, a partial last chunk, a second call
+    reusing the buffers, a non-contiguous source and a small array."""
+    from fastfuncstuff_amd import _lib
+
+    print("[upload] staged host -> device copies", flush=True)
+    rng = np.random.default_rng(5)
+    cases = [rng.standard_normal((70_001, 300)).astype(np.float32),  # 3 chunks, the last partial
+             rng.standard_normal((20_000, 400)),  # float64 source: converted on the host
+             rng.standard_normal((30_000, 400)).astype(np.float32)[:, ::2],  # non-contiguous
+             rng.standard_normal((10, 10)).astype(np.float32)]
+    for a in cases + cases[:1]:
+        got = _lib.upload_f32(a, dev)
+        ref = torch.from_numpy(np.ascontiguousarray(a)).to(dev, torch.float32)
+        torch.cuda.synchronize()
+        assert got.dtype == torch.float32 and got.is_contiguous() and got.shape == ref.shape
+        assert torch.equal(got, ref)
