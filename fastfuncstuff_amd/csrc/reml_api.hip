@@ -665,6 +665,22 @@ static int prep_impl(const ffs_reml_plan* P, const float* Y, int64_t ldY, int64_
         return 0;
     }
     if (coef_in || coef_out) return fail(-4, "prep: residual coefficients are kept only for p <= 32");
+    if (!feats && P->d.PGA >= 48 && P->d.PGA <= 96) {
+        // 32 < p <= 96 on f64 MFMA (writes the zero rows T..Tpad-1 too)
+        const dim3 grid((unsigned)(ncp / 64)), block(64 * kPrepWaves);
+#define FFS_PREPW(NT_)                                                                                             \
+    hipLaunchKernelGGL((k_prep_mfma_w<NT_>), grid, block, 0, s, P->d, Y, ldY, v, rows, nc, ncpi, normalize, Yt, \
+                       ldYt, scale, ncols_live, plans, grp)
+        switch (P->d.PGA / 16) {
+            case 3: FFS_PREPW(3); break;
+            case 4: FFS_PREPW(4); break;
+            case 5: FFS_PREPW(5); break;
+            default: FFS_PREPW(6); break;
+        }
+#undef FFS_PREPW
+        HIP_TRY(hipGetLastError());
+        return 0;
+    }
     if (p <= 8)
         launch_prep<8>(P, Y, ldY, v, rows, nc, ncpi, Yt, ldYt, scale, normalize, feats, s, plans, grp);
     else if (p <= 16)

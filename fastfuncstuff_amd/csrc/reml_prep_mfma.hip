@@ -374,4 +374,176 @@ __global__ void __launch_bounds__(64 * kPrepWaves) k_prep_mfma(PlanDev P, const 
     }
 }
 
+// ---------------------------------------------------------------------------
+// Normalise + residualise on f64 MFMA for 32 < p <= 96 (PGA = 16 NT columns,
+// NT = 3..6): k_prep_mfma's two passes organised for many design columns.
+// Pass 1: wave w owns the column tiles w, w + 4 of c = pinv(X) y over the
+// whole time range (no cross-wave sum), with the block's 64 columns staged
+// through LDS 16 samples at a time (double-buffered, the next step's samples
+// in flight during the MFMAs); wave 0 also forms the sums of y and y^2.  c and
+// 1/s go to LDS.  Pass 2: each wave computes r = y / s - X c / s for a quarter
+// of the time steps, c read from LDS as the B operand, and writes Yt[t][column]
+// (rows T..Tpad-1 come out zero).  No features, no stored coefficients.
+// ---------------------------------------------------------------------------
+template <int NT>
+__global__ void __launch_bounds__(64 * kPrepWaves) k_prep_mfma_w(PlanDev P, const float* __restrict__ Y, int64_t ldY,
+                                                                 int V, const int32_t* __restrict__ rows, int ncols,
+                                                                 int ncols_pad, int normalize, float* __restrict__ Yt,
+                                                                 int64_t ldYt, float* __restrict__ scale,
+                                                                 const int32_t* __restrict__ ncols_live,
+                                                                 const PlanDev* __restrict__ plans,
+                                                                 const int32_t* __restrict__ grp) {
+    if (plans) P = plans[max(grp[blockIdx.x], 0)];  // grouped layout: the design of this 64-column block
+    constexpr int PG = 16 * NT;
+    constexpr int W = kPrepWaves;
+    constexpr int RW = (NT + W - 1) / W;  // column tiles per wave
+    constexpr int CP = PG + 1;            // LDS row pitch of c (doubles)
+    __shared__ float ys[2][16][64];       // one step's samples, [t][column]
+    __shared__ double cL[64 * CP];        // c of each column
+    __shared__ double invL[64];
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int col0 = blockIdx.x * 64;
+    if (col0 >= ncols_pad) return;
+    if (ncols_live && col0 >= *ncols_live) return;  // beyond the columns in use (device-side count)
+    const int T = P.T, Tpad = P.Tpad;
+    const int nst = Tpad / 16;
+    const int j = lane & 15, kq = lane >> 4;
+    const bool vec4 = (ldY & 3) == 0 && (reinterpret_cast<uintptr_t>(Y) & 15) == 0;
+    const double* __restrict__ Pp = P.pinvTp;
+    const double* __restrict__ Xp = P.Xp;
+
+    // staging: thread tid fetches samples t0 + 4 (tid & 3) .. + 4 of column tid / 4
+    const int sv = prep_voxel(col0 + (tid >> 2), ncols, rows, V);
+    const float* srow = Y + (size_t)(sv >= 0 ? sv : 0) * ldY;
+    auto fetch = [&](int t0, float (&q)[4]) __attribute__((always_inline)) {
+        load_span<4>(srow, T, t0 + 4 * (tid & 3), vec4 && t0 + 16 <= T, q);
+#pragma unroll
+        for (int m = 0; m < 4; ++m) q[m] = sv >= 0 ? q[m] : 0.f;
+    };
+    auto store = [&](int buf, const float (&q)[4]) __attribute__((always_inline)) {
+#pragma unroll
+        for (int m = 0; m < 4; ++m) ys[buf][4 * (tid & 3) + m][tid >> 2] = q[m];
+    };
+
+    // ---- pass 1 --------------------------------------------------------
+    f64x4 acc[4][RW];
+#pragma unroll
+    for (int g = 0; g < 4; ++g)
+#pragma unroll
+        for (int u = 0; u < RW; ++u) acc[g][u] = f64x4{0.0, 0.0, 0.0, 0.0};
+    double s1[4] = {0.0, 0.0, 0.0, 0.0}, s2[4] = {0.0, 0.0, 0.0, 0.0};
+    const bool resid = !(normalize & 2);
+    float q[4];
+    fetch(0, q);
+    store(0, q);
+    __syncthreads();
+#pragma unroll 1
+    for (int st = 0; st < nst; ++st) {
+        const int buf = st & 1, t0 = 16 * st;
+        if (st + 1 < nst) fetch(t0 + 16, q);
+        // MFMA k-slot kq of step m covers time t0 + 4 kq + m; a wave past the
+        // last tile repeats it (its sums are not stored), so no MFMA is branched
+        double a[4][RW];
+#pragma unroll
+        for (int m = 0; m < 4; ++m)
+#pragma unroll
+            for (int u = 0; u < RW; ++u) {
+                const int rt = min(w + W * u, NT - 1);
+                const double x = Pp[(size_t)(t0 + 4 * kq + m) * PG + 16 * rt + j];
+                a[m][u] = resid ? x : 0.0;
+            }
+#pragma unroll
+        for (int m = 0; m < 4; ++m)
+#pragma unroll
+            for (int g = 0; g < 4; ++g) {
+                const double b = (double)ys[buf][4 * kq + m][16 * g + j];
+                s1[g] += b;
+                s2[g] = fma(b, b, s2[g]);
+#pragma unroll
+                for (int u = 0; u < RW; ++u)
+                    acc[g][u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[m][u], b, acc[g][u], 0, 0, 0);
+            }
+        if (st + 1 < nst) store(buf ^ 1, q);
+        __syncthreads();
+    }
+    int vB[4];
+    const float* rowB[4];
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
+        vB[g] = prep_voxel(col0 + 16 * g + j, ncols, rows, V);
+        rowB[g] = Y + (size_t)(vB[g] >= 0 ? vB[g] : 0) * ldY;
+    }
+    if (w == 0) {
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+            s1[g] += __shfl_xor(s1[g], 16);
+            s1[g] += __shfl_xor(s1[g], 32);
+            s2[g] += __shfl_xor(s2[g], 16);
+            s2[g] += __shfl_xor(s2[g], 32);
+            double s = 1.0;
+            if (normalize & 1) {
+                const double mean = s1[g] / T;
+                const double var = T > 1 ? (s2[g] - T * mean * mean) / (T - 1) : 0.0;
+                const double sd = var > 0.0 ? sqrt(var) : 0.0;
+                s = (double)fmaxf((float)sd, 1e-8f);
+            }
+            if (kq == 0) {
+                invL[16 * g + j] = 1.0 / s;
+                if (vB[g] >= 0 && scale) scale[vB[g]] = (float)s;
+            }
+        }
+    }
+    // c of column 16 g + j, design column 16 rt + kq + 4 r (the f64 MFMA D layout)
+#pragma unroll
+    for (int g = 0; g < 4; ++g)
+#pragma unroll
+        for (int u = 0; u < RW; ++u) {
+            const int rt = w + W * u;
+            if (rt < NT)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) cL[(16 * g + j) * CP + 16 * rt + kq + 4 * r] = acc[g][u][r];
+        }
+    __syncthreads();
+    if (!Yt) return;
+
+    // ---- pass 2 --------------------------------------------------------
+    // A row i of a tile holds time tt + 4 (i % 4) + i / 4, so output element r
+    // of lane (j, kq) is time tt + 4 kq + r.
+    const int arow = 4 * (j & 3) + (j >> 2);
+    double inv[4];
+#pragma unroll
+    for (int g = 0; g < 4; ++g) inv[g] = invL[16 * g + j];
+    const int tb = 16 * (w * nst / W), te = 16 * ((w + 1) * nst / W);  // this wave's time range
+#pragma unroll 1
+    for (int tt = tb; tt < te; tt += 16) {
+        float y4[4][4];
+        const bool full = vec4 && tt + 16 <= T;
+#pragma unroll
+        for (int g = 0; g < 4; ++g) load_span<4>(rowB[g], T, tt + 4 * kq, full, y4[g]);
+        f64x4 d[4];
+#pragma unroll
+        for (int g = 0; g < 4; ++g) d[g] = f64x4{0.0, 0.0, 0.0, 0.0};
+        if (resid) {
+#pragma unroll 1
+            for (int rt = 0; rt < NT; ++rt)
+#pragma unroll
+                for (int kk = 0; kk < 4; ++kk) {
+                    const double a = Xp[(size_t)(tt + arow) * PG + 16 * rt + kq + 4 * kk];
+#pragma unroll
+                    for (int g = 0; g < 4; ++g)
+                        d[g] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, cL[(16 * g + j) * CP + 16 * rt + kq + 4 * kk],
+                                                                    d[g], 0, 0, 0);
+                }
+        }
+#pragma unroll
+        for (int g = 0; g < 4; ++g)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const double y = vB[g] >= 0 ? (double)y4[g][r] : 0.0;
+                Yt[(size_t)(tt + 4 * kq + r) * ldYt + col0 + 16 * g + j] = (float)((y - d[g][r]) * inv[g]);
+            }
+    }
+}
+
 }  // namespace remlk

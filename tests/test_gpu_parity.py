@@ -21,9 +21,13 @@ F64 = torch.float64
 
 
 def _case(name):
-    rng = np.random.default_rng({"plain": 1, "censored": 2, "afni": 3, "ragged": 4, "wide": 5}[name])
-    if name in ("plain", "ragged", "wide"):
-        T, p, V = 100, {"plain": 6, "ragged": 6, "wide": 40}[name], {"plain": 256, "ragged": 333, "wide": 130}[name]
+    rng = np.random.default_rng({"plain": 1, "censored": 2, "afni": 3, "ragged": 4, "wide": 5, "wide60": 6,
+                                 "wide68": 7, "wide90": 8}[name])
+    if name in ("plain", "ragged", "wide", "wide60", "wide68", "wide90"):
+        # design widths 40 / 60 / 68 / 90 cover the f64-MFMA prep's 3 to 6 column tiles
+        T = {"wide60": 140, "wide68": 150, "wide90": 180}.get(name, 100)
+        p = {"plain": 6, "ragged": 6, "wide": 40, "wide60": 60, "wide68": 68, "wide90": 90}[name]
+        V = {"plain": 256, "ragged": 333, "wide": 130}.get(name, 100)
         X = np.column_stack([np.ones(T), np.linspace(-1, 1, T), rng.standard_normal((T, p - 2))])
         run_starts, tau = None, None
         ag = bg = torch.linspace(-0.8, 0.8, 33)
@@ -66,7 +70,7 @@ def dev():
     return torch.device("cuda", 0)
 
 
-@pytest.mark.parametrize("name", ["plain", "censored", "afni", "ragged", "wide"])
+@pytest.mark.parametrize("name", ["plain", "censored", "afni", "ragged", "wide", "wide60", "wide68", "wide90"])
 def test_surface_selection_and_gls_match_oracle(dev, name):
     from fastfuncstuff_amd.glm.arma import RemlGrid
 
