@@ -589,7 +589,10 @@ extern "C" int ffs_reml_plan_ainv(const ffs_reml_plan* P, double* ainv) {
 
 // Rows of the time-major buffer that ffs_reml_prep writes and ffs_reml_project
 // reads: Tpad, or 2 Tpad when the projection runs on pre-split halves.
-static int yt_rows(const ffs_reml_plan* P) { return (P->d.PG && P->d.proj_mode != 0) ? 2 * P->d.Tpad : P->d.Tpad; }
+// prep writes the split halves of the residual (rows Tpad..2 Tpad-1 of Yt) in
+// the split projection modes for every p <= 96 (f64 MFMA prep)
+static bool split_halves(const ffs_reml_plan* P) { return P->d.PGA != 0 && P->d.proj_mode != 0; }
+static int yt_rows(const ffs_reml_plan* P) { return split_halves(P) ? 2 * P->d.Tpad : P->d.Tpad; }
 
 extern "C" int ffs_reml_plan_yt_rows(const ffs_reml_plan* P, int32_t* rows) {
     if (!P || !rows) return fail(-3, "plan_yt_rows: null argument");
@@ -666,11 +669,19 @@ static int prep_impl(const ffs_reml_plan* P, const float* Y, int64_t ldY, int64_
     }
     if (coef_in || coef_out) return fail(-4, "prep: residual coefficients are kept only for p <= 32");
     if (!feats && P->d.PGA >= 48 && P->d.PGA <= 96) {
-        // 32 < p <= 96 on f64 MFMA (writes the zero rows T..Tpad-1 too)
+        // 32 < p <= 96 on f64 MFMA (writes the zero rows T..Tpad-1 too, and
+        // the split halves in the split projection modes)
+        uint16_t *Ysh = nullptr, *Ysl = nullptr;
+        if (write_split && Yt && P->d.proj_mode != 0) {
+            Ysh = reinterpret_cast<uint16_t*>(Yt + (size_t)P->d.Tpad * ldYt);
+            Ysl = Ysh + (size_t)P->d.Tpad * ldYt;
+        }
+        if (write_split && record_split) P->split_yt = Ysh ? Yt : nullptr;
+        const int sbf = P->d.proj_mode == 1;
         const dim3 grid((unsigned)(ncp / 64)), block(64 * kPrepWaves);
 #define FFS_PREPW(NT_)                                                                                             \
     hipLaunchKernelGGL((k_prep_mfma_w<NT_>), grid, block, 0, s, P->d, Y, ldY, v, rows, nc, ncpi, normalize, Yt, \
-                       ldYt, scale, ncols_live, plans, grp)
+                       ldYt, scale, ncols_live, Ysh, Ysl, sbf, plans, grp)
         switch (P->d.PGA / 16) {
             case 3: FFS_PREPW(3); break;
             case 4: FFS_PREPW(4); break;
@@ -798,7 +809,7 @@ extern "C" int ffs_reml_project(const ffs_reml_plan* P, const float* Yt, int64_t
     if (V <= 0) return 0;
     // split modes read the halves that prep wrote when p <= 32 (f64 MFMA
     // prep); otherwise they split the residual in the kernel
-    const bool presplit = P->d.PG != 0 && Yt == P->split_yt;
+    const bool presplit = P->d.PGA != 0 && Yt == P->split_yt;
     return project_impl(P, Yt, ldYt, V, S, ldS, nullptr, presplit, (hipStream_t)stream);
 }
 
@@ -1799,7 +1810,7 @@ struct SubsetLayout {
 SubsetLayout subset_layout(const ffs_reml_plan* P, int64_t cap) {
     SubsetLayout L{};
     L.cap_pad = vpad_of(std::max<int64_t>(cap, 1));
-    L.rows = (P->d.PG && P->d.proj_mode != 0) ? 2 * (int64_t)P->d.Tpad : P->d.Tpad;
+    L.rows = yt_rows(P);
     size_t o = 0;
     L.off_yt = o;
     o = align256(o + 4 * (size_t)L.rows * L.cap_pad);
@@ -1839,7 +1850,7 @@ extern "C" int ffs_reml_subset_search(const ffs_reml_plan* P, const float* Y, in
     float* ssub = reinterpret_cast<float*>(ws + L.off_sc);
     hipStream_t s = (hipStream_t)stream;
     const int K = P->d.K;
-    const bool split = P->d.PG && P->d.proj_mode != 0;
+    const bool split = split_halves(P);
     if (int rc = prep_impl(P, Y, ldY, V, idx, cap, Yt, L.cap_pad, scale, 1, nullptr, s, n_live, split, false, coef))
         return rc;
     if (int rc = energy_impl(P, Yt, L.cap_pad, cap, S, K, n_live, s)) return rc;
@@ -2058,7 +2069,7 @@ extern "C" int ffs_reml_group_create(ffs_reml_group** out, int32_t G, ffs_reml_p
     HIP_TRY(hipStreamCreateWithFlags(&R->gs2, hipStreamNonBlocking));
     HIP_TRY(hipEventCreateWithFlags(&R->ev_in, hipEventDisableTiming));
     HIP_TRY(hipEventCreateWithFlags(&R->ev_out, hipEventDisableTiming));
-    R->presplit = P0->d.PG != 0 && P0->d.proj_mode != 0;
+    R->presplit = split_halves(P0);
     *out = guard.release();
     return 0;
 }
@@ -2084,7 +2095,7 @@ static int group_enqueue(ffs_reml_group* R, const GroupIO& io, hipStream_t s) {
     const ffs_reml_plan* P0 = R->plans[0];
     const int K = R->K;
     const int64_t nc = R->ncols;
-    const bool split_prep = P0->d.PG && P0->d.proj_mode != 0 && R->presplit;
+    const bool split_prep = split_halves(P0) && R->presplit;
     if (int rc = prep_impl(P0, io.Y, io.ldY, R->V, R->rows, nc, R->Yt, nc, io.scale, 1, nullptr, s, nullptr,
                            split_prep, false, nullptr, nullptr, R->dplans, R->blk_grp))
         return rc;
@@ -2235,7 +2246,7 @@ extern "C" int ffs_reml_group_tune(ffs_reml_group* R, const float* Y, int64_t ld
     hipStream_t s = (hipStream_t)stream;
     const ffs_reml_plan* P0 = R->plans[0];
     std::vector<std::pair<int, bool>> cands;
-    const bool can_split = P0->d.PG && P0->d.proj_mode != 0;
+    const bool can_split = split_halves(P0);
     for (int n : {1, 2, 4})
         for (bool ps : {true, false}) {
             if (!can_split && !ps) continue;  // the tile choice exists only for pre-split operands

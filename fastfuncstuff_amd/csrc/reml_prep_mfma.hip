@@ -383,7 +383,8 @@ __global__ void __launch_bounds__(64 * kPrepWaves) k_prep_mfma(PlanDev P, const 
 // in flight during the MFMAs); wave 0 also forms the sums of y and y^2.  c and
 // 1/s go to LDS.  Pass 2: each wave computes r = y / s - X c / s for a quarter
 // of the time steps, c read from LDS as the B operand, and writes Yt[t][column]
-// (rows T..Tpad-1 come out zero).  No features, no stored coefficients.
+// (rows T..Tpad-1 come out zero) and, when asked, the split halves as
+// k_prep_mfma does.  No features, no stored coefficients.
 // ---------------------------------------------------------------------------
 template <int NT>
 __global__ void __launch_bounds__(64 * kPrepWaves) k_prep_mfma_w(PlanDev P, const float* __restrict__ Y, int64_t ldY,
@@ -391,7 +392,8 @@ __global__ void __launch_bounds__(64 * kPrepWaves) k_prep_mfma_w(PlanDev P, cons
                                                                  int ncols_pad, int normalize, float* __restrict__ Yt,
                                                                  int64_t ldYt, float* __restrict__ scale,
                                                                  const int32_t* __restrict__ ncols_live,
-                                                                 const PlanDev* __restrict__ plans,
+                                                                 uint16_t* __restrict__ Ysh, uint16_t* __restrict__ Ysl,
+                                                                 int split_bf16, const PlanDev* __restrict__ plans,
                                                                  const int32_t* __restrict__ grp) {
     if (plans) P = plans[max(grp[blockIdx.x], 0)];  // grouped layout: the design of this 64-column block
     constexpr int PG = 16 * NT;
@@ -537,12 +539,30 @@ __global__ void __launch_bounds__(64 * kPrepWaves) k_prep_mfma_w(PlanDev P, cons
                 }
         }
 #pragma unroll
-        for (int g = 0; g < 4; ++g)
+        for (int g = 0; g < 4; ++g) {
+            float r4[4];
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const double y = vB[g] >= 0 ? (double)y4[g][r] : 0.0;
-                Yt[(size_t)(tt + 4 * kq + r) * ldYt + col0 + 16 * g + j] = (float)((y - d[g][r]) * inv[g]);
+                r4[r] = (float)((y - d[g][r]) * inv[g]);
+                Yt[(size_t)(tt + 4 * kq + r) * ldYt + col0 + 16 * g + j] = r4[r];
             }
+            if (Ysh) {
+                // split halves, [t/8][column][8] (this lane: one 8-byte run of 4 samples)
+                uint16_t h[4], l[4];
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    if (split_bf16)
+                        split1<true>(r4[r], h[r], l[r]);
+                    else
+                        split1<false>(r4[r], h[r], l[r]);
+                }
+                const int t4 = tt + 4 * kq;
+                const size_t o = ((size_t)(t4 >> 3) * ldYt + col0 + 16 * g + j) * 8 + (t4 & 7);
+                *reinterpret_cast<uint2*>(Ysh + o) = make_uint2(h[0] | (uint32_t)h[1] << 16, h[2] | (uint32_t)h[3] << 16);
+                *reinterpret_cast<uint2*>(Ysl + o) = make_uint2(l[0] | (uint32_t)l[1] << 16, l[2] | (uint32_t)l[3] << 16);
+            }
+        }
     }
 }
 
