@@ -807,8 +807,8 @@ extern "C" int ffs_reml_project(const ffs_reml_plan* P, const float* Yt, int64_t
     if (!P) return fail(-3, "project: null plan");
     FFS_NEED_SEARCH(P, "project");
     if (V <= 0) return 0;
-    // split modes read the halves that prep wrote when p <= 32 (f64 MFMA
-    // prep); otherwise they split the residual in the kernel
+    // split modes read the halves that prep wrote into this buffer (f64 MFMA
+    // prep, p <= 96); otherwise they split the residual in the kernel
     const bool presplit = P->d.PGA != 0 && Yt == P->split_yt;
     return project_impl(P, Yt, ldYt, V, S, ldS, nullptr, presplit, (hipStream_t)stream);
 }
@@ -1282,7 +1282,7 @@ SetLayout set_layout(const ffs_reml_plan* P, const SetTable& ST, int64_t V) {
     const int64_t groups = std::min<int64_t>(V, L.n);
     L.max_cols = vpad_of(V + (kSetTile - 1) * groups);
     L.max_tiles = L.max_cols / kSetTile;
-    L.yt_rows = (P->d.PG && P->d.proj_mode != 0) ? 2 * (int64_t)P->d.Tpad : P->d.Tpad;
+    L.yt_rows = yt_rows(P);
     size_t o = 0;
     auto take = [&](size_t bytes) {
         const size_t at = o;
@@ -1326,8 +1326,8 @@ int set_check(const ffs_reml_plan* P, const SetTable& ST, const char* what, int6
     if (!P) return fail(-3, std::string(what) + ": null plan");
     FFS_NEED_SEARCH(P, what);
     if (ST.n <= 0) return fail(-4, std::string(what) + ": no sets on this plan");
-    if (P->d.proj_mode != 0 && !P->d.PG)
-        return fail(-4, std::string(what) + ": the split projection modes need p <= 32 here (use fp32)");
+    if (P->d.proj_mode != 0 && !split_halves(P))
+        return fail(-4, std::string(what) + ": the split projection modes need p <= 96 here (use fp32)");
     *L = set_layout(P, ST, V);
     if (!workspace || ws_bytes < (int64_t)L->total) return fail(-3, std::string(what) + ": workspace too small");
     if (L->max_cols > 0x7fffffff) return fail(-3, std::string(what) + ": batch too large; split it");
@@ -1342,7 +1342,7 @@ int set_group_prep(const ffs_reml_plan* P, const SetTable& ST, const float* Y, i
     hipLaunchKernelGGL(k_group_tiles, dim3((unsigned)((L.max_tiles + 255) / 256)), dim3(256), 0, s, q.fill, (int)L.n,
                        (int)L.max_tiles, kSetTile, q.tset, q.tcol, q.nt);
     HIP_TRY(hipGetLastError());
-    const bool split = P->d.PG && P->d.proj_mode != 0;
+    const bool split = split_halves(P);
     return prep_impl(P, Y, ldY, V, q.rows, L.max_cols, q.Yt, L.max_cols, scale, 1, nullptr, s, q.fill + L.n, split,
                      false, coef, coef_out);
 }
@@ -1625,8 +1625,8 @@ extern "C" int ffs_reml_hier_search(const ffs_reml_plan* P, const float* Y, int6
     if (V <= 0) return 0;
     if (P->hier.empty()) return fail(-4, "hier_search: call ffs_reml_plan_build_hier first");
     if (!Y || !best || !best_score || !scale) return fail(-3, "hier_search: null argument");
-    if (P->d.proj_mode != 0 && !P->d.PG)
-        return fail(-4, "hier_search: the split projection modes need p <= 32 here (use fp32)");
+    if (P->d.proj_mode != 0 && !split_halves(P))
+        return fail(-4, "hier_search: the split projection modes need p <= 96 here (use fp32)");
     const HierLayout H = hier_layout(P, V);
     if (!workspace || ws_bytes < (int64_t)H.total) return fail(-3, "hier_search: workspace too small");
     char* ws = static_cast<char*>(workspace);
@@ -1968,7 +1968,7 @@ struct ffs_reml_group {
     int64_t yt_rows = 0, work_ints = 0;
     std::vector<int64_t> group_cols;  // padded columns of each group
     // launch configuration (autotuned): surface in n_chunks column chunks on
-    // two graph branches; split projection on pre-split halves (p <= 32) or not
+    // two graph branches; split projection on pre-split halves (p <= 96) or not
     int n_chunks = 1;
     bool presplit = true;
     bool rank_deficient = false;  // some design has dependent columns
@@ -2235,7 +2235,7 @@ extern "C" int ffs_reml_group_fit(ffs_reml_group* R, const float* Y, int64_t ldY
 // one untimed capture + replay) and keeps the fastest.  Candidates: surface
 // column chunks n in {1, 2, 4} (chunks alternate between two graph branches,
 // so the VALU energy of one overlaps the MFMA projection of the other) and,
-// for p <= 32 in a split projection mode, the projection voxel tile (256
+// in a split projection mode, the projection voxel tile (256
 // voxels on pre-split halves, one chunk; or 128 voxels split in the kernel).
 extern "C" int ffs_reml_group_tune(ffs_reml_group* R, const float* Y, int64_t ldY, int32_t mode, int32_t* best,
                                    float* score, float* scale, float* betas, float* tstats, float* fstats,

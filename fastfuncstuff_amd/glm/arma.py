@@ -1100,8 +1100,6 @@ def fit_glm_arma11(data, design, tr, a_grid=None, b_grid=None, estimate_per_voxe
     if selector == "learned" and fixed_best is None:
         from ..learned import LocalSearcher, ProposalMLP, train_proposal
 
-        if grid.p > 32 and grid.projection != "fp32":
-            grid.set_projection("fp32")  # the pair-set kernels take split operands for p <= 32 only
         if grid.bands is None:
             grid.set_bands(0.10, 0.02)
         if proposal is None:
@@ -1211,11 +1209,10 @@ def fit_glm_arma11(data, design, tr, a_grid=None, b_grid=None, estimate_per_voxe
         scale_all = torch.ones(V, dtype=torch.float64)
     params_dev = grid.params.to(dev)
     # the hierarchical descent on demand: same pairs and scores as the descent
-    # on the full surface, when no surface is kept (the pair-set kernels take
-    # split operands for p <= 32 only)
+    # on the full surface, when no surface is kept
     use_descent = (fixed_best is None and searcher is None and not use_double and surface_acc is None
                    and not force_exhaustive_search and selector in (None, "hierarchical")
-                   and hierarchical_search == "descent" and (grid.p <= 32 or grid.projection == "fp32"))
+                   and hierarchical_search == "descent")
     with torch.cuda.device(dev):
         for b0 in range(0, V, nb):
             b1 = min(V, b0 + nb)
