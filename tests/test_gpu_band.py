@@ -37,12 +37,15 @@ def _lam(a, b):
     return (1.0 + a * b) * (a + b) / (1.0 + 2.0 * a * b + b * b)
 
 
-def _grid(dev, V, T=120, seed=11, noise_seed=None):
+def _grid(dev, V, T=120, seed=11, noise_seed=None, extra=0):
     from fastfuncstuff_amd import synth
     from fastfuncstuff_amd.glm.arma import RemlGrid
 
     wl = synth.plain_config(V=V, T=T, seed=seed, noise_seed=noise_seed)
-    grid = RemlGrid(torch.from_numpy(wl.X), torch.from_numpy(wl.a_grid), torch.from_numpy(wl.b_grid), dev)
+    X = wl.X
+    if extra:  # a wider design: random columns beside the plain one
+        X = np.concatenate([X, np.random.default_rng(99).standard_normal((X.shape[0], extra))], 1)
+    grid = RemlGrid(torch.from_numpy(X), torch.from_numpy(wl.a_grid), torch.from_numpy(wl.b_grid), dev)
     return grid, torch.from_numpy(wl.Y).to(dev)
 
 
@@ -88,13 +91,15 @@ def test_band_tables_match_restatement(dev):
     assert grid.band_info() is None
 
 
-@pytest.mark.parametrize("mode", ["fp32", "fp16x3", "bf16x3"])
-def test_band_scores_bitwise_equal_full_grid(dev, mode):
+@pytest.mark.parametrize("mode,extra", [("fp32", 0), ("fp16x3", 0), ("bf16x3", 0), ("fp16x3", 34), ("bf16x3", 62)])
+def test_band_scores_bitwise_equal_full_grid(dev, mode, extra):
+    """Band scores equal the full-grid surface bitwise; also for p = 40 and 68
+    (split halves from the f64-MFMA prep for 32 < p <= 96)."""
     from fastfuncstuff_amd import _lib
 
-    print(f"[band] bitwise mode={mode}", flush=True)
+    print(f"[band] bitwise mode={mode} extra={extra}", flush=True)
     lib = _lib.load()
-    grid, Y = _grid(dev, 900)
+    grid, Y = _grid(dev, 900, extra=extra)
     grid.set_projection(mode)
     grid.set_bands(W, H)
     V, K = Y.shape[0], grid.K
