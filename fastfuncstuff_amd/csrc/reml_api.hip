@@ -399,10 +399,10 @@ static int plan_alloc(ffs_reml_plan* P, int32_t T, int32_t p, const double* X, c
             return rc;
         HIP_TRY(hipMemsetAsync(G64p, 0, sizeof(double) * K * d.Tpad * PG, s));
     }
-    // GLS epilogue tables (f64 MFMA GLS widths): written after k_tables
+    // GLS epilogue tables (f64 MFMA GLS widths, p <= 96): written after k_tables
     double *gls_epi = nullptr, *dXtXp = nullptr;
-    if (d.PG) {
-        const int PG = d.PG;
+    if (d.PGA) {
+        const int PG = d.PGA;
         std::vector<double> XtXp((size_t)PG * PG, 0.0);
         for (int a = 0; a < p; ++a)
             for (int b = 0; b < p; ++b) XtXp[(size_t)a * PG + b] = XtX[(size_t)a * p + b];

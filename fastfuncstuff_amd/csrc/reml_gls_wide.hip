@@ -13,10 +13,10 @@
 // every fourth coefficient: after h = G'y, lane kq holds h[4 s + kq] in
 // accG[s / 4][s % 4] (s = 0 .. PG/4 - 1), and that is also the B operand
 // layout of a product M * beta (k-step s, k-slot kq).  So beta is kept in the
-// same distributed form (bb[s] = beta[4 s + kq]): the back-substitution sums
-// each step's dot product over the four lanes (two shuffles), and the two
-// quadratic forms beta'X'X beta and beta'F beta are f64 MFMAs whose output
-// lands in that layout again.  In the main loop each 16-sample step feeds the
+// same distributed form (bb[s] = beta[4 s + kq]): beta = M h with the plan's
+// M = Lx^-T and the two quadratic forms beta'X'X beta and beta'F beta are f64
+// MFMA products over zero-padded tables whose output lands in that layout
+// again.  In the main loop each 16-sample step feeds the
 // MFMAs (lane samples t0 + 4 kq + m, m = 0..3) and, after a shuffle gather of
 // the voxel's 16 samples, the recurrence (computed alike on the voxel's four
 // lanes).  Included by reml_api.hip.
@@ -31,11 +31,13 @@ __device__ __forceinline__ double sum4(double x) {
     return x;
 }
 
-// M (p x p, row-major, zero outside) times the distributed beta of 16 voxels:
-// out[s] = (M beta)[4 s + kq] for the lane's voxel.
+// M (PG x PG, row-major, zero past p) times a vector of 16 voxels held in the
+// distributed layout: out[s] = (M x)[4 s + kq] for the lane's voxel.  The
+// padded table needs no bounds tests, so a row's loads issue together.
 template <int NT>
-__device__ __forceinline__ void mat_beta(const double* __restrict__ M, int p, int j, int kq, const double (&bb)[4 * NT],
-                                         double (&out)[4 * NT]) {
+__device__ __forceinline__ void mat_vec(const double* __restrict__ M, int j, int kq, const double (&x)[4 * NT],
+                                        double (&out)[4 * NT]) {
+    constexpr int PG = 16 * NT;
     f64x4 acc[NT];
 #pragma unroll
     for (int rt = 0; rt < NT; ++rt) acc[rt] = f64x4{0.0, 0.0, 0.0, 0.0};
@@ -43,11 +45,8 @@ __device__ __forceinline__ void mat_beta(const double* __restrict__ M, int p, in
     for (int s = 0; s < 4 * NT; ++s) {
         const int col = 4 * s + kq;
 #pragma unroll
-        for (int rt = 0; rt < NT; ++rt) {
-            const int row = 16 * rt + j;
-            const double a = (row < p && col < p) ? M[row * p + col] : 0.0;
-            acc[rt] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bb[s], acc[rt], 0, 0, 0);
-        }
+        for (int rt = 0; rt < NT; ++rt)
+            acc[rt] = __builtin_amdgcn_mfma_f64_16x16x4f64(M[(16 * rt + j) * PG + col], x[s], acc[rt], 0, 0, 0);
     }
 #pragma unroll
     for (int s = 0; s < 4 * NT; ++s) out[s] = acc[s / 4][s % 4];
@@ -142,26 +141,12 @@ __global__ void __launch_bounds__(256) k_gls_wide(PlanDev P, const float* __rest
 #pragma unroll
     for (int q = 0; q < NS; ++q) hh = fma(hv[q], hv[q], hh);
     hh = sum4(hh);
-    const double* __restrict__ L = P.Lx + (size_t)k * p * p;
-#pragma unroll
-    for (int s = NS - 1; s >= 0; --s)
-#pragma unroll
-        for (int q = 3; q >= 0; --q) {
-            const int jj = 4 * s + q;  // wave-uniform
-            if (jj >= p) continue;
-            double part = 0.0;  // sum over m > jj held by this lane: m = 4 s2 + kq
-#pragma unroll
-            for (int s2 = s; s2 < NS; ++s2) {
-                const int m = 4 * s2 + kq;
-                if (m > jj && m < p) part = fma(L[m * p + jj], bb[s2], part);
-            }
-            const double acc = __shfl(hv[s], 16 * q + j) - sum4(part);
-            const double ljj = L[jj * p + jj];
-            if (kq == q) bb[s] = ljj != 0.0 ? acc / ljj : 0.0;  // zero pivot: basic solution
-        }
-    // quadratic forms and cross products
+    // beta = M h with M = Lx^-T from the plan's epilogue tables (the back
+    // substitution's zero-pivot rule included); then the quadratic forms
+    const double* __restrict__ ep = P.gls_epi + (size_t)k * (2 * PG * PG + PG);
+    mat_vec<NT>(ep, j, kq, hv, bb);
     double ra[NS];
-    mat_beta<NT>(P.XtX, p, j, kq, bb, ra);
+    mat_vec<NT>(P.XtXp, j, kq, bb, ra);
     double bq = 0.0, bxb = 0.0;
 #pragma unroll
     for (int q = 0; q < NS; ++q) {
@@ -170,7 +155,7 @@ __global__ void __launch_bounds__(256) k_gls_wide(PlanDev P, const float* __rest
     }
     bq = sum4(bq);
     bxb = sum4(bxb);
-    mat_beta<NT>(P.Fm + (size_t)k * p * p, p, j, kq, bb, ra);
+    mat_vec<NT>(ep + PG * PG, j, kq, bb, ra);
     double quad = 0.0;
 #pragma unroll
     for (int q = 0; q < NS; ++q) quad = fma(bb[q], ra[q], quad);
@@ -181,12 +166,12 @@ __global__ void __launch_bounds__(256) k_gls_wide(PlanDev P, const float* __rest
     const double sstot = syy - sy * (sy / T);
     const int dof = max(1, T - p);
     const double sig = ssw / dof;
-    const double* __restrict__ Ai = P.Ainv + (size_t)k * p * p;
+    const double* __restrict__ Ad = ep + 2 * PG * PG;
 #pragma unroll
     for (int q = 0; q < NS; ++q) {
         const int jj = 4 * q + kq;
         if (jj < p) {
-            const double se = sqrt(sig * Ai[jj * p + jj]);
+            const double se = sqrt(sig * Ad[jj]);
             tstats[(size_t)v * p + jj] = (float)(bb[q] / (se + 1e-10));
             betas[(size_t)v * p + jj] = (float)(bb[q] * s);
         }

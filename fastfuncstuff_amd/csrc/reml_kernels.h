@@ -151,10 +151,12 @@ struct PlanDev {
     const double* pinvTp;      // [Tpad][PG]     pinv(X)', zero padded
     const double* coefR;       // [K][Tpad][4]   (sqrt(1/v_t), c_{t-1}, alpha_{t-1}, 0); c = alpha = 0 at run starts
     const double* XtX;         // [p][p]
-    // GLS epilogue tables padded to PG (zero past p; PG > 0 only): per pair
-    // Lx transposed, the F form and diag (X'R^-1X)^-1, and X'X
-    const double* gls_epi;     // [K][2 PG^2 + PG]
-    const double* XtXp;        // [PG][PG]
+    // GLS epilogue tables padded to PGA (zero past p; every p <= 96): per
+    // pair, for p <= 32 Lx transposed, for 32 < p <= 96 M = Lx^-T (beta = M h,
+    // the back substitution with its zero-pivot rule applied to the unit
+    // vectors); then the F form and diag (X'R^-1X)^-1; and X'X
+    const double* gls_epi;     // [K][2 PGA^2 + PGA]
+    const double* XtXp;        // [PGA][PGA]
     // rank-deficient design: Pi = projection onto the row space of X (p x p);
     // null when X has full column rank.  The tables then hold the rank-r
     // forms (zero pivots and Q columns for the dependent columns, Ainv =

@@ -250,14 +250,17 @@ __device__ __forceinline__ void gls_half(int t0, int hf, int j, int kq, int T, b
         }
 }
 
-// Epilogue tables of k_gls_mfma for pair k (one wave): Lx transposed
-// (LT[j][m] = Lx[m][j] for m >= j, zero above), the F quadratic form and
-// diag (X'R^-1X)^-1, padded to PG x PG with zeros, so the epilogue runs
-// fixed-size inner loops whose padded terms add exact zeros.
+// Epilogue tables of the f64 MFMA GLS kernels for pair k (one wave), padded
+// to PG x PG (PG = PGA) with zeros, so the epilogues run fixed-size loops whose
+// padded terms add exact zeros.  First block: for k_gls_mfma (p <= 32) Lx
+// transposed (LT[j][m] = Lx[m][j] for m >= j, zero above); for k_gls_wide
+// (32 < p <= 96) M = Lx^-T, column i the back substitution Lx' m = e_i with
+// the zero-pivot rule (m_j = 0 where Lx[j][j] = 0), so beta = M h is the
+// kernels' basic solution.  Then the F quadratic form and diag (X'R^-1X)^-1.
 __global__ void __launch_bounds__(64) k_gls_epi_tables(PlanDev P, double* __restrict__ out) {
     const int k = blockIdx.x;
     if (k >= P.K) return;
-    const int p = P.p, PG = P.PG;
+    const int p = P.p, PG = P.PGA;
     const double* __restrict__ L = P.Lx + (size_t)k * p * p;
     const double* __restrict__ F = P.Fm + (size_t)k * p * p;
     const double* __restrict__ A = P.Ainv + (size_t)k * p * p;
@@ -265,10 +268,22 @@ __global__ void __launch_bounds__(64) k_gls_epi_tables(PlanDev P, double* __rest
     for (int i = threadIdx.x; i < PG * PG; i += 64) {
         const int r = i / PG, c = i % PG;
         const bool in = r < p && c < p;
-        o[i] = in && c >= r ? L[(size_t)c * p + r] : 0.0;  // Lx lower triangle only
+        if (P.PG) o[i] = in && c >= r ? L[(size_t)c * p + r] : 0.0;  // Lx lower triangle only
+        else if (!in) o[i] = 0.0;
         o[PG * PG + i] = in ? F[(size_t)r * p + c] : 0.0;
     }
     for (int j = threadIdx.x; j < PG; j += 64) o[2 * PG * PG + j] = j < p ? A[(size_t)j * p + j] : 0.0;
+    if (!P.PG) {
+        // column i of M, one lane per column, rows from the last up (each lane
+        // reads only the entries of its own column it has written)
+        for (int i = threadIdx.x; i < p; i += 64)
+            for (int jj = p - 1; jj >= 0; --jj) {
+                double acc = jj == i ? 1.0 : 0.0;
+                for (int m = jj + 1; m < p; ++m) acc -= L[(size_t)m * p + jj] * o[(size_t)m * PG + i];
+                const double ljj = L[(size_t)jj * p + jj];
+                o[(size_t)jj * PG + i] = ljj != 0.0 ? acc / ljj : 0.0;
+            }
+    }
 }
 
 template <int NT>
