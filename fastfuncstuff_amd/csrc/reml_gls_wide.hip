@@ -53,7 +53,7 @@ __device__ __forceinline__ void mat_vec(const double* __restrict__ M, int j, int
 }
 
 template <int NT>
-__global__ void __launch_bounds__(256) k_gls_wide(PlanDev P, const float* __restrict__ Y, int64_t ldY,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k_gls_wide(PlanDev P, const float* __restrict__ Y, int64_t ldY,
                                                   const float* __restrict__ scale, const int32_t* __restrict__ perm,
                                                   const int32_t* __restrict__ best, float* __restrict__ betas,
                                                   float* __restrict__ tstats, float* __restrict__ fstats,
@@ -129,14 +129,10 @@ __global__ void __launch_bounds__(256) k_gls_wide(PlanDev P, const float* __rest
     E *= inv2;
     sy *= inv;
     syy *= inv2;
-    // h (scaled) and beta = Lx^-T h in the distributed layout; columns >= p
-    // of h are zero and get beta 0
+    // h (scaled) in the distributed layout; its columns >= p are zero
     double bb[NS], hv[NS];
 #pragma unroll
-    for (int q = 0; q < NS; ++q) {
-        hv[q] = accG[q / 4][q % 4] * inv;
-        bb[q] = 0.0;
-    }
+    for (int q = 0; q < NS; ++q) hv[q] = accG[q / 4][q % 4] * inv;
     double hh = 0.0;
 #pragma unroll
     for (int q = 0; q < NS; ++q) hh = fma(hv[q], hv[q], hh);
@@ -145,15 +141,14 @@ __global__ void __launch_bounds__(256) k_gls_wide(PlanDev P, const float* __rest
     // substitution's zero-pivot rule included); then the quadratic forms
     const double* __restrict__ ep = P.gls_epi + (size_t)k * (2 * PG * PG + PG);
     mat_vec<NT>(ep, j, kq, hv, bb);
-    double ra[NS];
-    mat_vec<NT>(P.XtXp, j, kq, bb, ra);
     double bq = 0.0, bxb = 0.0;
 #pragma unroll
-    for (int q = 0; q < NS; ++q) {
-        bq = fma(bb[q], accX[q / 4][q % 4] * inv, bq);
-        bxb = fma(bb[q], ra[q], bxb);
-    }
+    for (int q = 0; q < NS; ++q) bq = fma(bb[q], accX[q / 4][q % 4] * inv, bq);  // q = X'y is dead after this
     bq = sum4(bq);
+    double ra[NS];
+    mat_vec<NT>(P.XtXp, j, kq, bb, ra);
+#pragma unroll
+    for (int q = 0; q < NS; ++q) bxb = fma(bb[q], ra[q], bxb);
     bxb = sum4(bxb);
     mat_vec<NT>(ep + PG * PG, j, kq, bb, ra);
     double quad = 0.0;

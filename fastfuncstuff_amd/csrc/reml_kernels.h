@@ -5,10 +5,10 @@
 
 namespace remlk {
 
-// Pairs handled by one wave of the energy kernel, in two halves; the pairs of
-// a half share the AR coefficient a (the pair list is a-major), so the
-// AR-filtered sample z_t = y_{t+1} - c_t y_t is computed once per time step
-// for each half.
+// Pairs handled by one wave of the full-grid energy kernel, in two halves;
+// the pairs of a half share the AR coefficient a (the pair list is a-major),
+// so the AR-filtered sample z_t = y_{t+1} - c_t y_t is computed once per time
+// step for each half.  Also the band sets' energy chunk (one pair per group).
 constexpr int kEnergyNB = 8;
 // Slots per half of an energy chunk: the pairs of a half share a (the two
 // halves of a chunk may differ), so the filtered sample is formed once per half.
