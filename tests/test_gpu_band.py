@@ -190,17 +190,20 @@ def test_learned_search_with_bands(dev):
     assert res1.n_fallback < res0.n_fallback
 
 
-@pytest.mark.parametrize("mode,use_coef", [("fp32", False), ("fp16x3", False), ("fp16x3", True)])
-def test_subset_search_matches_exhaustive(dev, mode, use_coef):
+@pytest.mark.parametrize("mode,use_coef,extra", [("fp32", False, 0), ("fp16x3", False, 0), ("fp16x3", True, 0),
+                                                ("bf16x3", False, 62)])
+def test_subset_search_matches_exhaustive(dev, mode, use_coef, extra):
     """Full-grid search of a device-counted subset: the first `cap` listed
     voxels get the exhaustive first minimum of the full surface (bitwise
     score); the others are untouched.  use_coef: the residualisation
-    coefficients and scales come from ffs_reml_prep_features instead."""
+    coefficients and scales come from ffs_reml_prep_features instead.
+    extra = 62: a p = 68 design (gathered columns through the f64-MFMA prep
+    for 32 < p <= 96 with a device-side column count)."""
     from fastfuncstuff_amd import _lib
 
-    print(f"[band] subset search mode={mode} coef={use_coef}", flush=True)
+    print(f"[band] subset search mode={mode} coef={use_coef} extra={extra}", flush=True)
     lib = _lib.load()
-    grid, Y = _grid(dev, 900)
+    grid, Y = _grid(dev, 900, extra=extra)
     grid.set_projection(mode)
     V = Y.shape[0]
     Yt, scale_ref, _ = grid.prep(Y)
