@@ -26,6 +26,7 @@
 #include "reml_f64.hip"
 #include "reml_group.hip"
 #include "reml_gls_wide.hip"
+#include "reml_seg.hip"
 
 using namespace remlk;
 
@@ -41,6 +42,17 @@ static int fail(int code, const std::string& msg) {
     do {                                                                                           \
         if ((P)->gls_only)                                                                         \
             return fail(-4, std::string(what) + ": pair-list plan (no lattice) supports GLS only"); \
+    } while (0)
+
+// Entry points built on the pair sets, the split 16-bit operands, the
+// per-lane FP64 search or the grouped engine hold a pair's columns in one
+// projection tile: designs of at most 96 columns.
+#define FFS_NEED_NARROW(P, what)                                                                          \
+    do {                                                                                                  \
+        if ((P)->d.nseg)                                                                                  \
+            return fail(-4, std::string(what) +                                                           \
+                                ": designs wider than 96 columns support the full-grid search, the selectors " \
+                                "and the GLS only");                                                      \
     } while (0)
 
 #define HIP_TRY(expr)                                                                          \
@@ -239,7 +251,8 @@ static int plan_alloc(ffs_reml_plan* P, int32_t T, int32_t p, const double* X, c
                       int32_t zero_a, int32_t zero_b, int32_t ltop, int32_t n_task, const int32_t* task_idx,
                       double ridge, hipStream_t s, TablesOut* to, KTablesArgs* ka, PlanPending* pend) {
     if (T < 2 || p < 1 || K < 1) return fail(-3, "plan_create: need T >= 2, p >= 1, K >= 1");
-    if (p > 96) return fail(-3, "plan_create: p > 96 is not supported");
+    if (p > kSegMaxP)
+        return fail(-3, "plan_create: p > " + std::to_string(kSegMaxP) + " is not supported");
     if (gap[T - 1] != 0) return fail(-3, "plan_create: gap[T-1] must be 0 (end of the last run)");
     if ((n_a <= 0) != (n_b <= 0)) return fail(-3, "plan_create: n_a and n_b must both be > 0 or both 0");
     const bool gls_only = n_a <= 0;
@@ -262,12 +275,14 @@ static int plan_alloc(ffs_reml_plan* P, int32_t T, int32_t p, const double* X, c
     d.ltop = ltop;
     d.zero_a = zero_a;
     d.zero_b = zero_b;
-    int npt = 0;
-    const int BN = pick_bn(p, &npt);
+    int npt = 1;
+    const bool seg = p > 96;  // pairs span segments of kSegBN columns
+    const int BN = seg ? kSegBN : pick_bn(p, &npt);
     if (BN < 0) return fail(-3, "plan_create: no projection tile fits p");
     d.BN = BN;
     d.npt = npt;
-    d.n_tiles = (K + npt - 1) / npt;
+    d.nseg = seg ? (p + kSegBN - 1) / kSegBN : 0;
+    d.n_tiles = seg ? K * d.nseg : (K + npt - 1) / npt;
     const int full_f = (n_task <= 0);
     std::vector<int32_t> tidx;
     if (full_f) {
@@ -350,7 +365,7 @@ static int plan_alloc(ffs_reml_plan* P, int32_t T, int32_t p, const double* X, c
     uint16_t *Ghf = nullptr, *Glf = nullptr, *Ghb = nullptr, *Glb = nullptr;
     float* g16s;
     if ((rc = P->alloc(&g16s, K))) return rc;
-    if (!gls_only) {
+    if (!gls_only && !seg) {  // split 16-bit operands: p <= 96
         const size_t n16 = (size_t)d.n_tiles * d.Tpad * BN;
         if ((rc = P->alloc(&Ghf, n16)) || (rc = P->alloc(&Glf, n16)) || (rc = P->alloc(&Ghb, n16)) ||
             (rc = P->alloc(&Glb, n16)))
@@ -384,9 +399,9 @@ static int plan_alloc(ffs_reml_plan* P, int32_t T, int32_t p, const double* X, c
     double *G64p = nullptr, *coefR = nullptr, *dXp = nullptr, *dXtX = nullptr, *dPp = nullptr;
     // f64 MFMA tables for every width (the GLS runs on them for p <= 96);
     // the MFMA prep and split operands keep p <= kGlsMfmaMaxP (d.PG)
-    d.PGA = 16 * ((p + 15) / 16);
+    d.PGA = seg ? 0 : 16 * ((p + 15) / 16);
     d.PG = p <= kGlsMfmaMaxP ? d.PGA : 0;
-    {
+    if (d.PGA) {
         const int PG = d.PGA;
         std::vector<double> Xp((size_t)d.Tpad * PG, 0.0), Pp((size_t)d.Tpad * PG, 0.0);
         for (int t = 0; t < T; ++t)
@@ -417,7 +432,8 @@ static int plan_alloc(ffs_reml_plan* P, int32_t T, int32_t p, const double* X, c
     HIP_TRY(hipMemsetAsync(coefR, 0, sizeof(double) * K * d.Tpad * 4, s));
     *to = TablesOut{dX,  soa, coef64, Xw64, G64, Lx,   Ainv,  Fm,  work, ld, ldR,
                     ldX, G32, Ghf,    Glf,  Ghb, Glb,  g16s,  G64p, coefR, ddep, dPi};
-    *ka = KTablesArgs{T, d.Tpad, p, K, dab, dwhiteflag, dgap, d.n_task, dtask, full_f, BN, npt, d.PGA, ridge};
+    *ka = KTablesArgs{T,  d.Tpad, p,   K,   dab, dwhiteflag, dgap,  d.n_task,
+                      dtask, full_f, BN, npt, d.PGA, ridge, d.nseg};
     d.gap = dgap;
     d.ab = dab;
     d.ld = ld;
@@ -511,7 +527,7 @@ static int plans_build(std::vector<PlanPending>& pend, const std::vector<TablesO
         if (status[g] == 2) return fail(-2, who + "task block of (X'R^-1X)^-1 is singular");
     }
     for (PlanPending& q : pend) {
-        if (!q.gls_only)
+        if (!q.gls_only && !q.P->d.nseg)  // window sets: set projection, p <= 96
             if (int rc = build_windows(q.P, s)) return rc;
         q.P->pinvT = q.dpinv;
         q.P->ridge = q.ridge;
@@ -638,6 +654,31 @@ static int prep_impl(const ffs_reml_plan* P, const float* Y, int64_t ldY, int64_
     if (ncp > 0x7fffffff || V > 0x7fffffff) return fail(-3, "prep: batch too large; split it");
     const int p = P->d.p;
     const int v = (int)V, nc = (int)ncols, ncpi = (int)ncp;
+    if (P->d.nseg) {
+        // p > 96: FP64 prep, kSegPrepVox columns per block
+        if (feats || coef_in || coef_out) return fail(-4, "prep: features and coefficients need p <= 96");
+        if (plans) return fail(-4, "prep: grouped layouts need p <= 96");
+        if (write_split && record_split) P->split_yt = nullptr;
+        const dim3 grid((unsigned)(ncp / kSegPrepVox)), block(256);
+        const size_t lds = seg_prep_lds(p);
+        if (p <= 256)
+            hipLaunchKernelGGL(k_prep_seg<1>, grid, block, lds, s, P->d.T, p, Y, ldY, v, rows, nc, ncpi, P->pinvT,
+                               P->d.X, normalize, Yt, ldYt, scale);
+        else if (p <= 512)
+            hipLaunchKernelGGL(k_prep_seg<2>, grid, block, lds, s, P->d.T, p, Y, ldY, v, rows, nc, ncpi, P->pinvT,
+                               P->d.X, normalize, Yt, ldYt, scale);
+        else
+            hipLaunchKernelGGL(k_prep_seg<3>, grid, block, lds, s, P->d.T, p, Y, ldY, v, rows, nc, ncpi, P->pinvT,
+                               P->d.X, normalize, Yt, ldYt, scale);
+        HIP_TRY(hipGetLastError());
+        if (Yt && P->d.Tpad > P->d.T) {
+            const int64_t n = (int64_t)(P->d.Tpad - P->d.T) * ncp;
+            hipLaunchKernelGGL(k_zero_rows, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, Yt, ldYt, P->d.T,
+                               P->d.Tpad, ncp);
+            HIP_TRY(hipGetLastError());
+        }
+        return 0;
+    }
     if (P->d.PG) {
         // f64 MFMA path: also writes the zero rows T..Tpad-1; in the split
         // projection modes also the hi / lo halves into rows Tpad..2 Tpad-1
@@ -729,6 +770,7 @@ extern "C" int ffs_reml_plan_coef_stride(const ffs_reml_plan* P, int32_t* stride
 extern "C" int ffs_reml_prep_features(const ffs_reml_plan* P, const float* Y, int64_t ldY, int64_t V, float* scale,
                                       int32_t normalize, float* feats, double* coef, void* stream) {
     if (!P) return fail(-3, "prep_features: null plan");
+    FFS_NEED_NARROW(P, "prep_features");
     if (!scale || !feats) return fail(-3, "prep_features: scale and feats are required");
     if (V <= 0) return 0;
     return prep_impl(P, Y, ldY, V, nullptr, V, nullptr, 0, scale, normalize, feats, (hipStream_t)stream, nullptr,
@@ -771,6 +813,15 @@ static int project_impl(const ffs_reml_plan* P, const float* Yt, int64_t ldYt, i
     if (nb > 0x7fffffff) return fail(-3, "project: grid too large; split the voxel batch");
     const dim3 g((unsigned)nb), b(256);
     const int mode = P->d.proj_mode;
+    if (P->d.nseg) {
+        // p > 96: fp32 MFMA over each pair's segments, items (voxel tile, pair)
+        if (plans) return fail(-4, "project: grouped layouts need p <= 96");
+        const int64_t nbs = nvt * 8 * ((P->d.K + 7) / 8);
+        if (nbs > 0x7fffffff) return fail(-3, "project: grid too large; split the voxel batch");
+        hipLaunchKernelGGL(k_project_seg, dim3((unsigned)nbs), b, 0, s, P->d, Yt, ldYt, (int)V, S, ldS, n_live);
+        HIP_TRY(hipGetLastError());
+        return 0;
+    }
     // split modes: 256-voxel blocks
     const int64_t nb16 = (V + kProj16BM - 1) / kProj16BM * grp8;
     const dim3 g16((unsigned)nb16);
@@ -962,7 +1013,7 @@ extern "C" int ffs_reml_gls(const ffs_reml_plan* P, const float* Y, int64_t ldY,
         return rc;
     if (V > 0 && P->d.Pi) {
         // rank-deficient design: minimum-norm beta and its statistics
-        hipLaunchKernelGGL(k_gls_rank, dim3((unsigned)V), dim3(64), 0, s, P->d, (int)V, scale, best, betas, tstats,
+        hipLaunchKernelGGL(k_gls_rank, dim3((unsigned)V), dim3(64), 2 * sizeof(double) * P->d.p, s, P->d, (int)V, scale, best, betas, tstats,
                            fstats, sigma2, nullptr, nullptr);
         HIP_TRY(hipGetLastError());
     }
@@ -984,6 +1035,24 @@ static int gls_impl(const ffs_reml_plan* P, const float* Y, int64_t ldY, const f
     hipLaunchKernelGGL(k_gls_invalid, dim3((unsigned)((V + 255) / 256)), dim3(256), 0, s, best, (int)V, K, p, betas,
                        tstats, fstats, sigma2, r2);
     const bool series_out = resid_w || resid || pred;
+    if (P->d.nseg) {
+        // p > 96: one wave per voxel, in pair order when a workspace is given
+        int32_t* perm = nullptr;
+        int64_t n = V;
+        if (work) {
+            int32_t* cnt = work;
+            int32_t* off = cnt + K;
+            int32_t* cursor = off + K + 1;
+            perm = cursor + K;
+            n = gls_perm_cap(V, K);
+            if (int rc = group_by_key(best, V, K, 1, cnt, off, cursor, perm, n, s)) return rc;
+        }
+        const dim3 g((unsigned)((n + kSegGlsWaves - 1) / kSegGlsWaves)), b(64 * kSegGlsWaves);
+        hipLaunchKernelGGL(k_gls_seg, g, b, seg_gls_lds(p), s, P->d, Y, ldY, scale, perm, best, (int)n, (int)V, betas,
+                           tstats, fstats, sigma2, r2, resid_w, ldR, resid, pred);
+        HIP_TRY(hipGetLastError());
+        return 0;
+    }
     if (work && P->d.PG && !series_out) {
         // pair-uniform waves: groups padded to 64 slots
         int32_t* cnt = work;
@@ -1106,6 +1175,7 @@ static void launch_surface_f64(const ffs_reml_plan* P, const TY* Y, int64_t ldY,
 extern "C" int ffs_reml_surface_f64(const ffs_reml_plan* P, const void* Y, int32_t y_is_f64, int64_t ldY, int64_t V,
                                     const float* scale, double* S, int64_t ldS, void* stream) {
     if (!P) return fail(-3, "surface_f64: null plan");
+    FFS_NEED_NARROW(P, "surface_f64");
     FFS_NEED_SEARCH(P, "surface_f64");
     if (V <= 0) return 0;
     if (!Y || !S) return fail(-3, "surface_f64: null argument");
@@ -1397,6 +1467,7 @@ int set_project(const ffs_reml_plan* P, const SetTable& ST, const SetLayout& L, 
 
 extern "C" int ffs_reml_local_workspace(const ffs_reml_plan* P, int64_t V, int64_t* bytes) {
     if (!P || !bytes) return fail(-3, "local_workspace: null argument");
+    FFS_NEED_NARROW(P, "local_workspace");
     *bytes = (int64_t)set_layout(P, P->d.win, std::max<int64_t>(V, 0)).total;
     return 0;
 }
@@ -1482,6 +1553,7 @@ extern "C" int ffs_reml_scatter_best(const int32_t* idx, int64_t n, const int32_
 // the set's slots.  Visit orders are those of k_select mode 1.
 extern "C" int ffs_reml_plan_build_hier(ffs_reml_plan* P, void* stream) {
     if (!P) return fail(-3, "plan_build_hier: null plan");
+    FFS_NEED_NARROW(P, "plan_build_hier");
     FFS_NEED_SEARCH(P, "plan_build_hier");
     if (!P->hier.empty()) return 0;
     hipStream_t s = (hipStream_t)stream;
@@ -1613,6 +1685,7 @@ HierLayout hier_layout(const ffs_reml_plan* P, int64_t V) {
 
 extern "C" int ffs_reml_hier_workspace(const ffs_reml_plan* P, int64_t V, int64_t* bytes) {
     if (!P || !bytes) return fail(-3, "hier_workspace: null argument");
+    FFS_NEED_NARROW(P, "hier_workspace");
     if (P->hier.empty()) return fail(-4, "hier_workspace: call ffs_reml_plan_build_hier first");
     *bytes = (int64_t)hier_layout(P, std::max<int64_t>(V, 0)).total;
     return 0;
@@ -1666,6 +1739,7 @@ extern "C" int ffs_reml_hier_search(const ffs_reml_plan* P, const float* Y, int6
 // ---------------------------------------------------------------------------
 extern "C" int ffs_reml_plan_set_bands(ffs_reml_plan* P, double half_width, double bin_width, void* stream) {
     if (!P) return fail(-3, "plan_set_bands: null plan");
+    FFS_NEED_NARROW(P, "plan_set_bands");
     FFS_NEED_SEARCH(P, "plan_set_bands");
     PlanDev& d = P->d;
     if (!(half_width > 0.0) || !(bin_width > 0.0)) {
@@ -1754,6 +1828,7 @@ extern "C" int ffs_reml_plan_window_info(const ffs_reml_plan* P, int32_t* n_wind
 
 extern "C" int ffs_reml_band_workspace(const ffs_reml_plan* P, int64_t V, int64_t* bytes) {
     if (!P || !bytes) return fail(-3, "band_workspace: null argument");
+    FFS_NEED_NARROW(P, "band_workspace");
     *bytes = (int64_t)set_layout(P, P->d.band, std::max<int64_t>(V, 0)).total;
     return 0;
 }
@@ -1827,6 +1902,7 @@ SubsetLayout subset_layout(const ffs_reml_plan* P, int64_t cap) {
 
 extern "C" int ffs_reml_subset_workspace(const ffs_reml_plan* P, int64_t cap, int64_t* bytes) {
     if (!P || !bytes) return fail(-3, "subset_workspace: null argument");
+    FFS_NEED_NARROW(P, "subset_workspace");
     *bytes = (int64_t)subset_layout(P, cap).total;
     return 0;
 }
@@ -1836,6 +1912,7 @@ extern "C" int ffs_reml_subset_search(const ffs_reml_plan* P, const float* Y, in
                                       float* best_score, float* scale, const double* coef, void* workspace,
                                       int64_t ws_bytes, void* stream) {
     if (!P) return fail(-3, "subset_search: null plan");
+    FFS_NEED_NARROW(P, "subset_search");
     FFS_NEED_SEARCH(P, "subset_search");
     if (V <= 0 || cap <= 0) return 0;
     if (!idx || !n_live) return fail(-3, "subset_search: idx and n_live are required");
@@ -1937,6 +2014,7 @@ extern "C" int ffs_reml_dsort_gls(const ffs_reml_plan* P, const int32_t* pair, i
 extern "C" int ffs_reml_plan_set_projection(ffs_reml_plan* P, int32_t mode) {
     if (!P) return fail(-3, "plan_set_projection: null plan");
     if (mode < 0 || mode > 2) return fail(-3, "plan_set_projection: mode must be 0 (fp32), 1 (bf16x3) or 2 (fp16x3)");
+    if (mode != 0) FFS_NEED_NARROW(P, "plan_set_projection (split modes)");
     P->d.proj_mode = mode;
     return 0;
 }
@@ -2009,6 +2087,7 @@ extern "C" int ffs_reml_group_create(ffs_reml_group** out, int32_t G, ffs_reml_p
         const ffs_reml_plan* P = plans[g];
         if (!P) return fail(-3, "group_create: null plan");
         if (P->gls_only || P->ridge > 0.0) return fail(-3, "group_create: plans must be lattice (search) plans");
+        FFS_NEED_NARROW(P, "group_create");
         if (P->d.T != P0->d.T || P->d.p != P0->d.p || P->d.K != P0->d.K || P->d.BN != P0->d.BN ||
             P->d.proj_mode != P0->d.proj_mode || P->d.n_task != P0->d.n_task || P->h_ab != P0->h_ab ||
             P->h_lut != P0->h_lut || P->device != P0->device)
@@ -2164,7 +2243,7 @@ static int group_enqueue(ffs_reml_group* R, const GroupIO& io, hipStream_t s) {
     }
     HIP_TRY(hipGetLastError());
     if (R->rank_deficient) {
-        hipLaunchKernelGGL(k_gls_rank, dim3((unsigned)R->V), dim3(64), 0, s, P0->d, (int)R->V, io.scale, io.best,
+        hipLaunchKernelGGL(k_gls_rank, dim3((unsigned)R->V), dim3(64), 2 * sizeof(double) * P0->d.p, s, P0->d, (int)R->V, io.scale, io.best,
                            io.betas, io.tstats, io.fstats, io.sigma2, R->dplans, R->vgroup);
         HIP_TRY(hipGetLastError());
     }

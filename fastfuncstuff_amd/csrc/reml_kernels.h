@@ -90,6 +90,7 @@ struct KTablesArgs {
     const int32_t* task_idx;   // [n_task]
     int full_f, BN, npt, PG;
     double ridge;
+    int nseg;  // > 0: p > 96, pair k's column j in tile k * nseg + j / BN (column j % BN)
 };
 
 // Output tables of k_tables for one design (device pointers; null where the
@@ -164,6 +165,9 @@ struct PlanDev {
     const double* Pi;
     int rank;
     int proj_mode;             // 0 fp32 MFMA, 1 bf16 x3, 2 fp16 x3
+    // p > 96: segments of BN = 64 columns per pair (pair k's column j in G32
+    // tile k * nseg + j / 64, column j % 64; n_tiles = K * nseg); 0 otherwise
+    int nseg;
     const float* ecoef;        // [n_chunks][T][2*NB] (alpha_t, 1/v_{t+1}) per chunk slot
     const float* ec;           // [n_chunks][T][2] c_t of the AR coefficient of each half of the chunk
     const float* es0;          // [n_chunks][NB] 1/v_0

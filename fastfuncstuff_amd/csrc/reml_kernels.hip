@@ -343,7 +343,12 @@ __global__ void __launch_bounds__(64) k_tables(KTablesArgs A, const TablesOut* _
     __syncthreads();
     // fp32 projection operand, tile-major: G32[tile][t][slot*p + j]
     const int tile = k / npt, slot = k % npt;
-    if (G32) {  // null for a pair-list (GLS-only) plan
+    if (G32 && A.nseg > 0) {  // p > 96: the pair's segments of BN columns
+        for (int idx = tid; idx < T * p; idx += 64) {
+            const int t = idx / p, j = idx % p;
+            G32[((size_t)k * A.nseg + j / BN) * Tpad * BN + (size_t)t * BN + j % BN] = (float)g[idx];
+        }
+    } else if (G32) {  // null for a pair-list (GLS-only) plan
         float* Gt = G32 + (size_t)tile * Tpad * BN + slot * p;
         for (int idx = tid; idx < T * p; idx += 64) {
             const int t = idx / p, j = idx % p;

@@ -456,7 +456,9 @@ __global__ void __launch_bounds__(64) k_gls_rank(PlanDev P, int V, const float* 
     const int k = best[v];
     if (k < 0 || k >= P.K) return;
     const int p = P.p, tid = threadIdx.x;
-    __shared__ double bb[96], bm[96];
+    extern __shared__ double rank_lds[];  // 2 p doubles
+    double* bb = rank_lds;
+    double* bm = rank_lds + p;
     const double s = scale ? (double)scale[v] : 1.0;
     for (int j = tid; j < p; j += 64) bb[j] = (double)betas[(size_t)v * p + j] / s;
     __syncthreads();

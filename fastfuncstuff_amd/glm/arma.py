@@ -35,6 +35,12 @@ VOXEL_SCATTER_ATTRS = [
     "ljung_box", "set_fstats", "set_sd", "set_coef",
 ]
 
+# Widest design of the pair-set searches (hierarchical descent on demand,
+# learned windows and bands), the split 16-bit projections, the FP64 search and
+# the grouped engine; wider designs (up to 768 columns) use the full-grid
+# surface with the fp32 projection over 64-column segments.
+WIDE_P = 96
+
 # Runs are offset by this much in the Ljung-Box pseudo-time (glm/arma.py:5969).
 _LJ_RUN_SEPARATION = 66666
 
@@ -1092,9 +1098,15 @@ def fit_glm_arma11(data, design, tr, a_grid=None, b_grid=None, estimate_per_voxe
             b_grid = db if b_grid is None else b_grid
         if grid is None:
             grid = RemlGrid(design, a_grid, b_grid, dev, run_starts=run_starts, tau=tau, task_indices=task_indices)
-        if grid.projection != projection:
-            grid.set_projection(projection)
+        # designs wider than 96 columns: fp32 projection over 64-column segments
+        proj = projection if p <= WIDE_P else "fp32"
+        if grid.projection != proj:
+            grid.set_projection(proj)
         fixed_best = None
+    if p > WIDE_P and use_double and fixed_best is None:
+        raise NotImplementedError(f"use_double search with p = {p} > {WIDE_P} design columns")
+    if p > WIDE_P and selector == "learned":
+        raise NotImplementedError(f"the learned selector needs p <= {WIDE_P} design columns (p = {p})")
 
     searcher = None
     if selector == "learned" and fixed_best is None:
@@ -1212,7 +1224,7 @@ def fit_glm_arma11(data, design, tr, a_grid=None, b_grid=None, estimate_per_voxe
     # on the full surface, when no surface is kept
     use_descent = (fixed_best is None and searcher is None and not use_double and surface_acc is None
                    and not force_exhaustive_search and selector in (None, "hierarchical")
-                   and hierarchical_search == "descent")
+                   and hierarchical_search == "descent" and p <= WIDE_P)
     with torch.cuda.device(dev):
         for b0 in range(0, V, nb):
             b1 = min(V, b0 + nb)
@@ -1910,7 +1922,7 @@ def fit_glm_arma11_grouped(data, designs_by_group: dict, group_indices, tr: floa
     # one launch per stage over all groups unless an option needs the per-group fit
     use_engine = kw.pop("grouped_engine", True)
     if (use_engine and pre is None and dsort_full is None and set(kw) <= _GROUPED_ENGINE_KW
-            and kw.get("estimate_per_voxel", True) and shape[1] <= 96 and n_voxels > 0
+            and kw.get("estimate_per_voxel", True) and shape[1] <= WIDE_P and n_voxels > 0
             and kw.get("selector") != "learned"):
         return _grouped_engine_fit(data_t, designs_by_group, groups, group_indices, dict(kw, tr=tr))
     kw.pop("autotune", None)

@@ -22,15 +22,18 @@ F64 = torch.float64
 
 def _case(name):
     rng = np.random.default_rng({"plain": 1, "censored": 2, "afni": 3, "ragged": 4, "wide": 5, "wide60": 6,
-                                 "wide68": 7, "wide90": 8}[name])
-    if name in ("plain", "ragged", "wide", "wide60", "wide68", "wide90"):
-        # design widths 40 / 60 / 68 / 90 cover the f64-MFMA prep's 3 to 6 column tiles
-        T = {"wide60": 140, "wide68": 150, "wide90": 180}.get(name, 100)
-        p = {"plain": 6, "ragged": 6, "wide": 40, "wide60": 60, "wide68": 68, "wide90": 90}[name]
-        V = {"plain": 256, "ragged": 333, "wide": 130}.get(name, 100)
+                                 "wide68": 7, "wide90": 8, "seg120": 9, "seg300": 10, "seg520": 11}[name])
+    if name in ("plain", "ragged", "wide", "wide60", "wide68", "wide90", "seg120", "seg300", "seg520"):
+        # design widths 40 / 60 / 68 / 90 cover the f64-MFMA prep's 3 to 6 column
+        # tiles; 120 / 300 / 520 the segmented path (2 / 5 / 9 segments of 64
+        # columns, one to three column groups of the prep)
+        T = {"wide60": 140, "wide68": 150, "wide90": 180, "seg120": 220, "seg300": 420, "seg520": 600}.get(name, 100)
+        p = {"plain": 6, "ragged": 6, "wide": 40, "wide60": 60, "wide68": 68, "wide90": 90, "seg120": 120,
+             "seg300": 300, "seg520": 520}[name]
+        V = {"plain": 256, "ragged": 333, "wide": 130, "seg520": 70}.get(name, 100)
         X = np.column_stack([np.ones(T), np.linspace(-1, 1, T), rng.standard_normal((T, p - 2))])
         run_starts, tau = None, None
-        ag = bg = torch.linspace(-0.8, 0.8, 33)
+        ag = bg = torch.linspace(-0.8, 0.8, {"seg300": 17, "seg520": 9}.get(name, 33))
     elif name == "censored":
         lens, p, V = [70, 80], 12, 200
         full = sum(lens)
@@ -70,7 +73,8 @@ def dev():
     return torch.device("cuda", 0)
 
 
-@pytest.mark.parametrize("name", ["plain", "censored", "afni", "ragged", "wide", "wide60", "wide68", "wide90"])
+@pytest.mark.parametrize("name", ["plain", "censored", "afni", "ragged", "wide", "wide60", "wide68", "wide90",
+                                  "seg120", "seg300", "seg520"])
 def test_surface_selection_and_gls_match_oracle(dev, name):
     from fastfuncstuff_amd.glm.arma import RemlGrid
 
