@@ -69,10 +69,11 @@ def parse():
     ap.add_argument("--save-network", default=None, help="write the trained proposal network to this path")
     ap.add_argument("--api-steps", type=int, default=3,
                     help="timed end-to-end fit_glm_arma11 calls (host input -> ARMA11Results) per path; 0: skip")
-    ap.add_argument("--workload", default="metric", choices=["metric", "learned_eval", "grouped"],
+    ap.add_argument("--workload", default="metric", choices=["metric", "learned_eval", "grouped", "im"],
                     help="metric: the headline line; learned_eval: learned path on a 100x100x50 volume with "
                          "spatially smooth ARMA parameters and 2%% t(3) voxels (T=600), its own JSON line; "
-                         "grouped: 20 uneven HRF groups with a 64-event single-trial design (p=68), full grid")
+                         "grouped: 20 uneven HRF groups with a 64-event single-trial design (p=68), full grid; "
+                         "im: 596-event single-trial design + Legendre 0-3 (p=600, T=1500, 50k voxels)")
     return ap.parse_args()
 
 
@@ -240,6 +241,84 @@ def grouped(args):
     print(json.dumps(line), flush=True)
 
 
+def im_design(args):
+    """IM (single-trial) design, the shape of the reference's IM REML benchmark
+    (about 600 regressors, benchmark/stages/glm_im_reml.py:17-22): 596 events +
+    Legendre 0-3 (p = 600), 5 runs x 300 volumes (T = 1500), 50k voxels unless
+    --voxels says otherwise, 33 x 33 grid.  Designs wider than 96 columns run
+    the segmented path: FP64 prep, recurrence energy, fp32 MFMA projection over
+    64-column segments, hierarchical selection on the surface, FP64 GLS with one
+    wave per voxel.  One step = one fit with the data resident in HBM (stages
+    timed with device events on the launch stream); the plan is built first and
+    timed apart, and the drop-in fit_glm_arma11 from host data is timed beside it."""
+    from fastfuncstuff_amd import _lib, synth
+    from fastfuncstuff_amd.glm.arma import RemlGrid, fit_glm_arma11
+
+    dev = _lib.require_device(None)
+    V = 50_000 if args.voxels == 200_000 else args.voxels
+    t0 = time.time()
+    Y, X, run_starts, ag, bg = synth.im_config(V=V)
+    gen_s = time.time() - t0
+    T, p = X.shape
+    print(f"[bench] IM workload generated in {gen_s:.1f} s (T={T}, p={p}, V={V})", file=sys.stderr, flush=True)
+    agt, bgt = torch.from_numpy(ag), torch.from_numpy(bg)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    grid = RemlGrid(torch.from_numpy(X), agt, bgt, dev, run_starts=run_starts)
+    torch.cuda.synchronize()
+    plan_s = time.perf_counter() - t0
+    print(f"[bench] plan built in {plan_s:.2f} s (K={grid.K})", file=sys.stderr, flush=True)
+    lib = _lib.load()
+    Yd = torch.from_numpy(Y).to(dev)
+    S = torch.empty(V, grid.K, dtype=torch.float32, device=dev)
+    sh = _lib.stream_handle(dev)
+
+    def step(tm):
+        if tm:
+            tm.mark("prep")
+        Yt, scale, _ = grid.prep(Yd)
+        if tm:
+            tm.mark("energy")
+        _lib.check(lib.ffs_reml_energy(grid.handle, _lib.ptr(Yt), Yt.stride(0), V, _lib.ptr(S), S.stride(0), sh),
+                   "energy")
+        if tm:
+            tm.mark("project")
+        _lib.check(lib.ffs_reml_project(grid.handle, _lib.ptr(Yt), Yt.stride(0), V, _lib.ptr(S), S.stride(0), sh),
+                   "project")
+        if tm:
+            tm.mark("select")
+        best, _ = grid.select(S, hierarchical=True)
+        if tm:
+            tm.mark("gls")
+        grid.gls(Yd, scale, best)
+        return best
+
+    ms, per, _, _ = timed(step, args.steps, args.warmup, dev)
+    t0 = time.perf_counter()
+    fit_glm_arma11(Y, X, 2.0, a_grid=agt, b_grid=bgt, device=dev, run_starts=run_starts, grid=grid)
+    torch.cuda.synchronize()
+    api_ms = (time.perf_counter() - t0) * 1e3
+    flop = 2.0 * V * T * grid.K * p
+    proj_ms = per.get("project", float("nan"))
+    line = {
+        "metric": f"REML voxels/s, IM design (p={p}, T={T}, V={V})",
+        "value": V / (ms / 1e3), "unit": "voxels/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
+        "data": "synthetic",
+        "config": {"workload": f"im596_T{T}_5runs_p{p}_grid33x33", "V": V, "T": T, "p": p, "K": grid.K,
+                   "path": "segmented (p > 96): full-grid surface, fp32 projection over 64-column segments, "
+                           "hierarchical selection, FP64 GLS"},
+        "kernels_ms": per,
+        "roofline": {"kernel": "k_project_seg", "bound": "mfma", "achieved": flop / (proj_ms / 1e3) / 1e12,
+                     "peak": 157.3, "unit": "TFLOP/s", "frac": flop / (proj_ms / 1e3) / 1e12 / 157.3,
+                     "algorithmic": f"2*V*T*K*p = {flop:.3e} flop per launch", "traffic": None},
+        "plan_build_s": plan_s, "api_end_to_end": {"ms": api_ms, "voxels_per_s": V / (api_ms / 1e3)},
+        "datagen_s": gen_s,
+    }
+    grid.close()
+    print(json.dumps(line), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -251,6 +330,9 @@ def main():
         return
     if args.workload == "grouped":
         grouped(args)
+        return
+    if args.workload == "im":
+        im_design(args)
         return
     from fastfuncstuff_amd import _lib, synth
     from fastfuncstuff_amd.glm.arma import RemlGrid

@@ -327,3 +327,32 @@ def grouped_config(V: int = 200_000, T: int = 400, tr: float = 2.0, n_groups: in
         Y[c0:c1] = (sig + noise).astype(np.float32)
     ag, bg = config_grid()
     return Y, designs, groups, ag, bg
+
+
+def im_config(V: int = 50_000, n_runs: int = 5, run_len: int = 300, tr: float = 2.0, n_events: int = 596,
+              seed: int = 6, chunk: int = 25_000):
+    """Single-trial (IM) design, the shape of the reference's IM REML
+    benchmark (about 600 regressors, benchmark/stages/glm_im_reml.py:17-22):
+    ``n_events`` 1 s events at random onsets over ``n_runs`` runs of
+    ``run_len`` volumes, one double-gamma regressor each, plus Legendre 0-3
+    over the series (p = n_events + 4); ARMA(1,1) noise per voxel as in the
+    plain config, restarted at each run.  Returns (Y (V, T) float32, X (T, p)
+    float64, run_starts, a_grid, b_grid)."""
+    rng = np.random.default_rng(seed)
+    T = n_runs * run_len
+    hrf = double_gamma_hrf(tr)
+    onsets = np.sort(rng.uniform(0.0, (T - 15) * tr, n_events))
+    cols = [boxcar_regressor(np.array([o]), 1.0, T, tr, hrf) for o in onsets]
+    X = np.concatenate([np.stack(cols, axis=1), legendre_baseline(T, 3)], axis=1)
+    run_starts = [r * run_len for r in range(n_runs)]
+    a = rng.uniform(0.1, 0.8, V)
+    b = rng.uniform(-0.5, 0.3, V)
+    Y = np.empty((V, T), dtype=np.float32)
+    p = X.shape[1]
+    for c0 in range(0, V, chunk):
+        c1 = min(V, c0 + chunk)
+        noise = np.concatenate([arma11_noise(rng, a[c0:c1], b[c0:c1], run_len) for _ in range(n_runs)], axis=1)
+        beta = rng.standard_normal((c1 - c0, p)) * 0.5
+        Y[c0:c1] = (beta @ X.T + 100.0 + noise).astype(np.float32)
+    ag, bg = config_grid()
+    return Y, X, run_starts, ag, bg
