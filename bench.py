@@ -265,6 +265,7 @@ def im_design(args):
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     grid = RemlGrid(torch.from_numpy(X), agt, bgt, dev, run_starts=run_starts)
+    grid.set_projection(args.projection)
     torch.cuda.synchronize()
     plan_s = time.perf_counter() - t0
     print(f"[bench] plan built in {plan_s:.2f} s (K={grid.K})", file=sys.stderr, flush=True)
@@ -295,23 +296,30 @@ def im_design(args):
 
     ms, per, _, _ = timed(step, args.steps, args.warmup, dev)
     t0 = time.perf_counter()
-    fit_glm_arma11(Y, X, 2.0, a_grid=agt, b_grid=bgt, device=dev, run_starts=run_starts, grid=grid)
+    fit_glm_arma11(Y, X, 2.0, a_grid=agt, b_grid=bgt, device=dev, run_starts=run_starts, grid=grid,
+                   projection=args.projection)
     torch.cuda.synchronize()
     api_ms = (time.perf_counter() - t0) * 1e3
-    flop = 2.0 * V * T * grid.K * p
+    # split modes: three 16-bit products per output (the dense fp16 / bf16
+    # peak); fp32: one fp32 MFMA product
+    split = args.projection != "fp32"
+    flop = (3.0 if split else 1.0) * 2.0 * V * T * grid.K * p
+    peak = 2500.0 if split else 157.3
     proj_ms = per.get("project", float("nan"))
     line = {
         "metric": f"REML voxels/s, IM design (p={p}, T={T}, V={V})",
         "value": V / (ms / 1e3), "unit": "voxels/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
-        "data": "synthetic",
+        "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": args.projection, "data": "synthetic",
         "config": {"workload": f"im596_T{T}_5runs_p{p}_grid33x33", "V": V, "T": T, "p": p, "K": grid.K,
-                   "path": "segmented (p > 96): full-grid surface, fp32 projection over 64-column segments, "
-                           "hierarchical selection, FP64 GLS"},
+                   "path": "segmented (p > 96): full-grid surface, projection over 64-column segments, "
+                           "hierarchical selection, FP64 GLS", "projection": args.projection},
         "kernels_ms": per,
-        "roofline": {"kernel": "k_project_seg", "bound": "mfma", "achieved": flop / (proj_ms / 1e3) / 1e12,
-                     "peak": 157.3, "unit": "TFLOP/s", "frac": flop / (proj_ms / 1e3) / 1e12 / 157.3,
-                     "algorithmic": f"2*V*T*K*p = {flop:.3e} flop per launch", "traffic": None},
+        "roofline": {"kernel": "k_project16_seg" if split else "k_project_seg", "bound": "mfma",
+                     "achieved": flop / (proj_ms / 1e3) / 1e12, "peak": peak, "unit": "TFLOP/s",
+                     "frac": flop / (proj_ms / 1e3) / 1e12 / peak,
+                     "algorithmic": f"{'3*' if split else ''}2*V*T*K*p = {flop:.3e} flop per launch",
+                     "traffic": None},
         "plan_build_s": plan_s, "api_end_to_end": {"ms": api_ms, "voxels_per_s": V / (api_ms / 1e3)},
         "datagen_s": gen_s,
     }

@@ -365,7 +365,7 @@ static int plan_alloc(ffs_reml_plan* P, int32_t T, int32_t p, const double* X, c
     uint16_t *Ghf = nullptr, *Glf = nullptr, *Ghb = nullptr, *Glb = nullptr;
     float* g16s;
     if ((rc = P->alloc(&g16s, K))) return rc;
-    if (!gls_only && !seg) {  // split 16-bit operands: p <= 96
+    if (!gls_only) {  // split 16-bit operands (per segment tile for p > 96)
         const size_t n16 = (size_t)d.n_tiles * d.Tpad * BN;
         if ((rc = P->alloc(&Ghf, n16)) || (rc = P->alloc(&Glf, n16)) || (rc = P->alloc(&Ghb, n16)) ||
             (rc = P->alloc(&Glb, n16)))
@@ -607,7 +607,7 @@ extern "C" int ffs_reml_plan_ainv(const ffs_reml_plan* P, double* ainv) {
 // reads: Tpad, or 2 Tpad when the projection runs on pre-split halves.
 // prep writes the split halves of the residual (rows Tpad..2 Tpad-1 of Yt) in
 // the split projection modes for every p <= 96 (f64 MFMA prep)
-static bool split_halves(const ffs_reml_plan* P) { return P->d.PGA != 0 && P->d.proj_mode != 0; }
+static bool split_halves(const ffs_reml_plan* P) { return (P->d.PGA != 0 || P->d.nseg) && P->d.proj_mode != 0; }
 static int yt_rows(const ffs_reml_plan* P) { return split_halves(P) ? 2 * P->d.Tpad : P->d.Tpad; }
 
 extern "C" int ffs_reml_plan_yt_rows(const ffs_reml_plan* P, int32_t* rows) {
@@ -655,28 +655,27 @@ static int prep_impl(const ffs_reml_plan* P, const float* Y, int64_t ldY, int64_
     const int p = P->d.p;
     const int v = (int)V, nc = (int)ncols, ncpi = (int)ncp;
     if (P->d.nseg) {
-        // p > 96: FP64 prep, kSegPrepVox columns per block
+        // p > 96: FP64 prep, kSegPrepVox columns per block; writes rows
+        // T..Tpad-1 as zeros, and the split halves in the split projection modes
         if (feats || coef_in || coef_out) return fail(-4, "prep: features and coefficients need p <= 96");
         if (plans) return fail(-4, "prep: grouped layouts need p <= 96");
-        if (write_split && record_split) P->split_yt = nullptr;
+        uint16_t *Ysh = nullptr, *Ysl = nullptr;
+        if (write_split && Yt && P->d.proj_mode != 0) {
+            Ysh = reinterpret_cast<uint16_t*>(Yt + (size_t)P->d.Tpad * ldYt);
+            Ysl = Ysh + (size_t)P->d.Tpad * ldYt;
+        }
+        if (write_split && record_split) P->split_yt = Ysh ? Yt : nullptr;
+        const int sbf = P->d.proj_mode == 1;
         const dim3 grid((unsigned)(ncp / kSegPrepVox)), block(256);
         const size_t lds = seg_prep_lds(p);
-        if (p <= 256)
-            hipLaunchKernelGGL(k_prep_seg<1>, grid, block, lds, s, P->d.T, p, Y, ldY, v, rows, nc, ncpi, P->pinvT,
-                               P->d.X, normalize, Yt, ldYt, scale);
-        else if (p <= 512)
-            hipLaunchKernelGGL(k_prep_seg<2>, grid, block, lds, s, P->d.T, p, Y, ldY, v, rows, nc, ncpi, P->pinvT,
-                               P->d.X, normalize, Yt, ldYt, scale);
-        else
-            hipLaunchKernelGGL(k_prep_seg<3>, grid, block, lds, s, P->d.T, p, Y, ldY, v, rows, nc, ncpi, P->pinvT,
-                               P->d.X, normalize, Yt, ldYt, scale);
+#define FFS_PREP_SEG(MJ_)                                                                                          \
+    hipLaunchKernelGGL(k_prep_seg<MJ_>, grid, block, lds, s, P->d.T, P->d.Tpad, p, Y, ldY, v, rows, nc, ncpi,      \
+                       P->pinvT, P->d.X, normalize, Yt, ldYt, scale, Ysh, Ysl, sbf)
+        if (p <= 256) FFS_PREP_SEG(1);
+        else if (p <= 512) FFS_PREP_SEG(2);
+        else FFS_PREP_SEG(3);
+#undef FFS_PREP_SEG
         HIP_TRY(hipGetLastError());
-        if (Yt && P->d.Tpad > P->d.T) {
-            const int64_t n = (int64_t)(P->d.Tpad - P->d.T) * ncp;
-            hipLaunchKernelGGL(k_zero_rows, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, Yt, ldYt, P->d.T,
-                               P->d.Tpad, ncp);
-            HIP_TRY(hipGetLastError());
-        }
         return 0;
     }
     if (P->d.PG) {
@@ -816,6 +815,22 @@ static int project_impl(const ffs_reml_plan* P, const float* Yt, int64_t ldYt, i
     if (P->d.nseg) {
         // p > 96: fp32 MFMA over each pair's segments, items (voxel tile, pair)
         if (plans) return fail(-4, "project: grouped layouts need p <= 96");
+        if (mode != 0) {
+            // split 16-bit modes: 256-voxel items on the halves prep wrote
+            if (!presplit)
+                return fail(-4, "project: the split modes for p > 96 read the halves that ffs_reml_prep wrote into "
+                                "this buffer in the same mode");
+            const int64_t nb16s = (V + kProj16BM - 1) / kProj16BM * 8 * ((P->d.K + 7) / 8);
+            if (nb16s > 0x7fffffff) return fail(-3, "project: grid too large; split the voxel batch");
+            if (mode == 1)
+                hipLaunchKernelGGL((k_project16_seg<true>), dim3((unsigned)nb16s), b, 0, s, P->d, Yt, ldYt, (int)V, S,
+                                   ldS, n_live);
+            else
+                hipLaunchKernelGGL((k_project16_seg<false>), dim3((unsigned)nb16s), b, 0, s, P->d, Yt, ldYt, (int)V, S,
+                                   ldS, n_live);
+            HIP_TRY(hipGetLastError());
+            return 0;
+        }
         const int64_t nbs = nvt * 8 * ((P->d.K + 7) / 8);
         if (nbs > 0x7fffffff) return fail(-3, "project: grid too large; split the voxel batch");
         hipLaunchKernelGGL(k_project_seg, dim3((unsigned)nbs), b, 0, s, P->d, Yt, ldYt, (int)V, S, ldS, n_live);
@@ -860,7 +875,7 @@ extern "C" int ffs_reml_project(const ffs_reml_plan* P, const float* Yt, int64_t
     if (V <= 0) return 0;
     // split modes read the halves that prep wrote into this buffer (f64 MFMA
     // prep, p <= 96); otherwise they split the residual in the kernel
-    const bool presplit = P->d.PGA != 0 && Yt == P->split_yt;
+    const bool presplit = split_halves(P) && Yt == P->split_yt;
     return project_impl(P, Yt, ldYt, V, S, ldS, nullptr, presplit, (hipStream_t)stream);
 }
 
@@ -1036,7 +1051,10 @@ static int gls_impl(const ffs_reml_plan* P, const float* Y, int64_t ldY, const f
                        tstats, fstats, sigma2, r2);
     const bool series_out = resid_w || resid || pred;
     if (P->d.nseg) {
-        // p > 96: one wave per voxel, in pair order when a workspace is given
+        // p > 96: nv voxels of one pair per wave (groups padded to nv slots;
+        // 4 for p <= 256, 2 above, so the per-voxel columns stay in registers)
+        // when a workspace is given, else one voxel per wave
+        const int nvg = p <= 256 ? kSegGlsVox : kSegGlsVox / 2;
         int32_t* perm = nullptr;
         int64_t n = V;
         if (work) {
@@ -1045,11 +1063,24 @@ static int gls_impl(const ffs_reml_plan* P, const float* Y, int64_t ldY, const f
             int32_t* cursor = off + K + 1;
             perm = cursor + K;
             n = gls_perm_cap(V, K);
-            if (int rc = group_by_key(best, V, K, 1, cnt, off, cursor, perm, n, s)) return rc;
+            if (int rc = group_by_key(best, V, K, nvg, cnt, off, cursor, perm, n, s)) return rc;
         }
-        const dim3 g((unsigned)((n + kSegGlsWaves - 1) / kSegGlsWaves)), b(64 * kSegGlsWaves);
-        hipLaunchKernelGGL(k_gls_seg, g, b, seg_gls_lds(p), s, P->d, Y, ldY, scale, perm, best, (int)n, (int)V, betas,
-                           tstats, fstats, sigma2, r2, resid_w, ldR, resid, pred);
+        const int nv = perm ? nvg : 1;
+        const int64_t waves = (n + nv - 1) / nv;
+        const dim3 g((unsigned)((waves + kSegGlsWaves - 1) / kSegGlsWaves)), b(64 * kSegGlsWaves);
+#define FFS_GLS_SEG(MJ_, NV_)                                                                                   \
+    hipLaunchKernelGGL((k_gls_seg<MJ_, NV_>), g, b, 0, s, P->d, Y, ldY, scale, perm, best, (int)n, (int)V, betas, \
+                       tstats, fstats, sigma2, r2, resid_w, ldR, resid, pred)
+        if (perm) {
+            if (p <= 256) FFS_GLS_SEG(4, kSegGlsVox);
+            else if (p <= 512) FFS_GLS_SEG(8, kSegGlsVox / 2);
+            else FFS_GLS_SEG(12, kSegGlsVox / 2);
+        } else {
+            if (p <= 256) FFS_GLS_SEG(4, 1);
+            else if (p <= 512) FFS_GLS_SEG(8, 1);
+            else FFS_GLS_SEG(12, 1);
+        }
+#undef FFS_GLS_SEG
         HIP_TRY(hipGetLastError());
         return 0;
     }
@@ -2014,7 +2045,6 @@ extern "C" int ffs_reml_dsort_gls(const ffs_reml_plan* P, const int32_t* pair, i
 extern "C" int ffs_reml_plan_set_projection(ffs_reml_plan* P, int32_t mode) {
     if (!P) return fail(-3, "plan_set_projection: null plan");
     if (mode < 0 || mode > 2) return fail(-3, "plan_set_projection: mode must be 0 (fp32), 1 (bf16x3) or 2 (fp16x3)");
-    if (mode != 0) FFS_NEED_NARROW(P, "plan_set_projection (split modes)");
     P->d.proj_mode = mode;
     return 0;
 }

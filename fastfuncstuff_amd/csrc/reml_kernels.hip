@@ -376,8 +376,10 @@ __global__ void __launch_bounds__(64) k_tables(KTablesArgs A, const TablesOut* _
         const size_t tbase = (size_t)tile * Tpad * BN;
         for (int idx = tid; idx < T * p; idx += 64) {
             const int t = idx / p, j = idx % p;
-            const int col = slot * p + j;
-            const size_t o = tbase + ((size_t)(t >> 3) * BN + col) * 8 + (t & 7);
+            // p > 96: column j % BN of the pair's segment tile k * nseg + j / BN
+            const size_t o = A.nseg > 0
+                                 ? ((size_t)k * A.nseg + j / BN) * Tpad * BN + ((size_t)(t >> 3) * BN + j % BN) * 8 + (t & 7)
+                                 : tbase + ((size_t)(t >> 3) * BN + slot * p + j) * 8 + (t & 7);
             const double gv = g[idx];
             const double gs = ldexp(gv, e);
             const _Float16 hf = (_Float16)gs;

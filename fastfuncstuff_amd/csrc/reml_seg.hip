@@ -12,17 +12,19 @@ constexpr int kSegBN = 64;          // columns per segment
 constexpr int kSegMaxP = 768;       // widest design of the segmented path
 constexpr int kSegPrepVox = 8;      // voxels per block of k_prep_seg
 constexpr int kSegPrepChunk = 64;   // samples staged per step of k_prep_seg
-constexpr int kSegGlsWaves = 4;     // voxels (one per wave) per block of k_gls_seg
+constexpr int kSegGlsWaves = 4;     // waves per block of k_gls_seg
+constexpr int kSegGlsVox = 4;       // voxels of one pair per wave of k_gls_seg (pair-grouped order)
 
-// LDS bytes of the two segmented kernels that size their shared memory at launch.
+// LDS bytes of k_prep_seg (sized at launch).
 inline size_t seg_prep_lds(int p) { return sizeof(double) * ((size_t)kSegPrepVox * (p + 1) + kSegPrepChunk * kSegPrepVox); }
-inline size_t seg_gls_lds(int p) { return sizeof(double) * 2 * (size_t)p * kSegGlsWaves; }
 
 // ---------------------------------------------------------------------------
 // Standardisation and residualisation (glm/arma.py:3947-3954, 1554-1571),
 // p > 96: per voxel c = pinv(X) y in FP64 (t ascending), s = the sample
 // standard deviation of y floored at 1e-8, r_t = y_t / s - sum_j X[t][j] c_j / s
-// (j ascending), written to the time-major buffer Yt[t][col].
+// (j ascending), written to the time-major buffer Yt[t][col] (rows T..Tpad-1
+// zero); in the split projection modes also its hi / lo 16-bit halves (Ysh,
+// Ysl: [t/8][col][8], the layout the split projection reads).
 // One block of 256 threads per kSegPrepVox consecutive output columns.
 // Pass 1: each thread owns columns j = tid + 256 m of c for all the block's
 // voxels; the samples are staged through LDS kSegPrepChunk at a time and the
@@ -31,11 +33,12 @@ inline size_t seg_gls_lds(int p) { return sizeof(double) * 2 * (size_t)p * kSegG
 // bit 0 of normalize: standardise; bit 1: no residualisation.
 // ---------------------------------------------------------------------------
 template <int MJ>
-__global__ void __launch_bounds__(256) k_prep_seg(int T, int p, const float* __restrict__ Y, int64_t ldY, int V,
-                                                  const int32_t* __restrict__ rows, int ncols, int ncols_pad,
+__global__ void __launch_bounds__(256) k_prep_seg(int T, int Tpad, int p, const float* __restrict__ Y, int64_t ldY,
+                                                  int V, const int32_t* __restrict__ rows, int ncols, int ncols_pad,
                                                   const double* __restrict__ pinvT, const double* __restrict__ X,
                                                   int normalize, float* __restrict__ Yt, int64_t ldYt,
-                                                  float* __restrict__ scale) {
+                                                  float* __restrict__ scale, uint16_t* __restrict__ Ysh,
+                                                  uint16_t* __restrict__ Ysl, int split_bf16) {
     constexpr int NV = kSegPrepVox, CH = kSegPrepChunk;
     extern __shared__ double seg_lds[];
     double* cs = seg_lds;                       // [NV][p + 1]  c / s
@@ -106,18 +109,29 @@ __global__ void __launch_bounds__(256) k_prep_seg(int T, int p, const float* __r
     }
     __syncthreads();
     if (!Yt) return;
-    for (int e = tid; e < T * NV; e += 256) {
+    for (int e = tid; e < Tpad * NV; e += 256) {
         const int t = e / NV, u = e % NV;
         const int v = st_v[u];
         float rf = 0.f;
-        if (v >= 0) {
+        if (v >= 0 && t < T) {
             double r = (double)Y[(size_t)v * ldY + t] * st_inv[u];
             const double* xr = X + (size_t)t * p;
             const double* cr = cs + u * (p + 1);
             for (int j = 0; j < p; ++j) r -= xr[j] * cr[j];
             rf = (float)r;
         }
-        Yt[(size_t)t * ldYt + col0 + u] = rf;
+        const int col = col0 + u;
+        Yt[(size_t)t * ldYt + col] = rf;
+        if (Ysh) {
+            uint16_t h, l;
+            if (split_bf16)
+                split1<true>(rf, h, l);
+            else
+                split1<false>(rf, h, l);
+            const size_t o = ((size_t)(t >> 3) * ldYt + col) * 8 + (t & 7);
+            Ysh[o] = h;
+            Ysl[o] = l;
+        }
     }
 }
 
@@ -214,6 +228,162 @@ __global__ void __launch_bounds__(256) k_project_seg(PlanDev P, const float* __r
     }
 }
 
+// ---------------------------------------------------------------------------
+// (ii') Split 16-bit projection for p > 96: C = Ah Bh + Ah Bl + Al Bh on
+// 32x32x16 MFMA (f16 or bf16 halves, fp32 accumulation, the product order and
+// operand layouts of k_project16b), item (256-voxel tile, pair k) over the
+// pair's segments in turn.  Each wave owns 64 voxel rows (two 32-row blocks);
+// after a segment, lanes 0..31 add the squares of their row's columns into
+// FP64 sums (one per row block, j ascending over the pair's p columns), then
+// P = sum * 2^-2e for fp16 and the score epilogue of k_project_seg.
+// ---------------------------------------------------------------------------
+template <bool BF16>
+__global__ void __launch_bounds__(256) k_project16_seg(PlanDev P, const float* __restrict__ Yt, int64_t ldY, int V,
+                                                       float* __restrict__ S, int64_t ldS,
+                                                       const int32_t* __restrict__ n_live) {
+    constexpr int BN = kSegBN;
+    using L = Proj16Lds<BN>;
+    constexpr int BM = kProj16BM, NBLK = BN / 32, NTB = L::NTB, RP = L::RP, C_LD = L::C_LD;
+    constexpr int B_N = 2 * NTB * BN;  // 16-byte B units per stage
+    constexpr int B_U = B_N / 256;
+    static_assert(B_N % 256 == 0, "B stage must split evenly over the block");
+    __shared__ __attribute__((aligned(16))) unsigned char lds_raw[L::BYTES];
+    uint16_t* Bh = reinterpret_cast<uint16_t*>(lds_raw);
+    uint16_t* Bl = Bh + L::B_H;
+
+    int vt, k;
+    if (!xcd_tile_map(blockIdx.x, gridDim.x, P.K, &vt, &k)) return;
+    if (n_live && vt * BM >= *n_live) return;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int v0 = vt * BM;
+    const int p = P.p, nseg = P.nseg;
+    const uint16_t* __restrict__ gh = BF16 ? P.Gh_bf16 : P.Gh_f16;
+    const uint16_t* __restrict__ gl = BF16 ? P.Gl_bf16 : P.Gl_f16;
+    const uint16_t* __restrict__ Ash = reinterpret_cast<const uint16_t*>(Yt + (size_t)P.Tpad * ldY);
+    const uint16_t* __restrict__ Asl = Ash + (size_t)P.Tpad * ldY;
+    const int a_col = v0 + 64 * w + (lane & 31);
+    const int a_tb = lane >> 5;
+    const int nk = P.Tpad / kProjBK;
+    double pp[2] = {0.0, 0.0};  // lanes 0..31: row (block r, lane)
+    for (int sg = 0; sg < nseg; ++sg) {
+        const size_t tile = (size_t)k * nseg + sg;
+        size_t ub[B_U];
+#pragma unroll
+        for (int u = 0; u < B_U; ++u) {
+            const int rr = (tid + u * 256) % (NTB * BN);
+            ub[u] = tile * P.Tpad * BN + ((size_t)(rr / BN) * BN + rr % BN) * 8;
+        }
+        u32x4 ah[2][2], al[2][2];
+        u32x4 rb[B_U];
+        auto load_a = [&](int kt) __attribute__((always_inline)) {
+#pragma unroll
+            for (int r = 0; r < 2; ++r)
+#pragma unroll
+                for (int ks = 0; ks < 2; ++ks) {
+                    const size_t o = ((size_t)(kt * NTB + 2 * ks + a_tb) * ldY + a_col + 32 * r) * 8;
+                    ah[r][ks] = *reinterpret_cast<const u32x4*>(Ash + o);
+                    al[r][ks] = *reinterpret_cast<const u32x4*>(Asl + o);
+                }
+        };
+        auto load_b = [&](int kt) __attribute__((always_inline)) {
+#pragma unroll
+            for (int u = 0; u < B_U; ++u) {
+                const int plane = (tid + u * 256) / (NTB * BN);
+                rb[u] = *reinterpret_cast<const u32x4*>((plane ? gl : gh) + ub[u] + (size_t)kt * NTB * BN * 8);
+            }
+        };
+        f32x16 acc[2][NBLK];
+#pragma unroll
+        for (int r = 0; r < 2; ++r)
+#pragma unroll
+            for (int i = 0; i < NBLK; ++i)
+#pragma unroll
+                for (int q = 0; q < 16; ++q) acc[r][i][q] = 0.f;
+        load_a(0);
+        load_b(0);
+        for (int kt = 0; kt < nk; ++kt) {
+#pragma unroll
+            for (int u = 0; u < B_U; ++u) {
+                const int q = tid + u * 256;
+                const int plane = q / (NTB * BN), rr = q % (NTB * BN);
+                const int tb = rr / BN, col = rr % BN;
+                *reinterpret_cast<u32x4*>((plane ? Bl : Bh) + col * RP + tb * 8) = rb[u];
+            }
+            u32x4 ahi[2][2], alo[2][2];
+#pragma unroll
+            for (int r = 0; r < 2; ++r)
+#pragma unroll
+                for (int ks = 0; ks < 2; ++ks) {
+                    ahi[r][ks] = ah[r][ks];
+                    alo[r][ks] = al[r][ks];
+                }
+            __syncthreads();
+            if (kt + 1 < nk) {
+                load_a(kt + 1);
+                load_b(kt + 1);
+            }
+#pragma unroll
+            for (int ks = 0; ks < 2; ++ks) {
+                const int tb = 2 * ks + (lane >> 5);
+                u32x4 bh[NBLK], bl[NBLK];
+#pragma unroll
+                for (int n = 0; n < NBLK; ++n) {
+                    const int col = n * 32 + (lane & 31);
+                    bh[n] = *reinterpret_cast<const u32x4*>(Bh + col * RP + tb * 8);
+                    bl[n] = *reinterpret_cast<const u32x4*>(Bl + col * RP + tb * 8);
+                }
+#pragma unroll
+                for (int n = 0; n < NBLK; ++n)
+#pragma unroll
+                    for (int r = 0; r < 2; ++r) acc[r][n] = mfma16u<BF16>(alo[r][ks], bh[n], acc[r][n]);
+#pragma unroll
+                for (int n = 0; n < NBLK; ++n)
+#pragma unroll
+                    for (int r = 0; r < 2; ++r) acc[r][n] = mfma16u<BF16>(ahi[r][ks], bl[n], acc[r][n]);
+#pragma unroll
+                for (int n = 0; n < NBLK; ++n)
+#pragma unroll
+                    for (int r = 0; r < 2; ++r) acc[r][n] = mfma16u<BF16>(ahi[r][ks], bh[n], acc[r][n]);
+            }
+            __syncthreads();
+        }
+        // squares of this segment's columns through a wave-private patch
+        float* Cw = reinterpret_cast<float*>(lds_raw) + w * 32 * C_LD;
+        const int nj = min(BN, p - sg * BN);
+#pragma unroll
+        for (int r = 0; r < 2; ++r) {
+#pragma unroll
+            for (int n = 0; n < NBLK; ++n)
+#pragma unroll
+                for (int q = 0; q < 16; ++q) {
+                    const int row = 8 * (q >> 2) + 4 * (lane >> 5) + (q & 3);
+                    Cw[row * C_LD + n * 32 + (lane & 31)] = acc[r][n][q];
+                }
+            __syncthreads();
+            if (lane < 32) {
+                const float* cr = Cw + lane * C_LD;
+                for (int j = 0; j < nj; ++j) {
+                    const double cv = (double)cr[j];
+                    pp[r] = fma(cv, cv, pp[r]);
+                }
+            }
+            __syncthreads();
+        }
+    }
+    if (lane < 32) {
+        const double tp = (double)(P.T - p);
+        const double un = BF16 ? 1.0 : (double)P.g16_unscale[k];
+#pragma unroll
+        for (int r = 0; r < 2; ++r) {
+            const int v = v0 + 64 * w + 32 * r + lane;
+            if (v >= V) continue;
+            float* sp = S + (size_t)v * ldS + k;
+            const double rss = (double)(*sp) - pp[r] * un;
+            *sp = (float)fma(tp, log(rss + 1e-10), P.ld[k]);
+        }
+    }
+}
+
 // Sum over the 64 lanes of a wave (every lane gets the total).
 __device__ __forceinline__ double wave_sum(double x) {
 #pragma unroll
@@ -256,17 +426,21 @@ __device__ __forceinline__ double seg_whiten_chunk(const double* __restrict__ cR
 
 // ---------------------------------------------------------------------------
 // (iv) GLS refit for p > 96 (glm/arma.py:4784-5601, 5827-5862): one wave per
-// voxel, in the given order (perm, -1 = empty slot) or voxel order.
+// NV voxels of one pair (the pair-grouped order padded to NV slots per group,
+// -1 = empty slot; NV = 1 for the voxel order), so each row of the pair's
+// tables is read once for the NV voxels.  Per voxel:
 //   yw = L^-1 y (the innovations recurrence, by chunk scans), q = X_w' yw,
 //   beta = (X'R^-1X)^-1 q  (the plan's Ainv; for a rank-deficient design
 //   its pseudo-inverse, which gives the minimum-norm solution),
 //   r = yw - X_w beta, sigma2 = r'r / max(1, T - p), t_j = beta_j / sqrt(sigma2 Ainv_jj),
 //   F = beta' Fm beta / (sigma2 n_task + 1e-10), R^2 = CoD(y, X beta),
 // with the outputs of k_gls (betas and sigma2 unscaled; optional series:
-// whitened residuals (FP64), residuals and fitted values).  Voxels whose pair
-// is not valid are left to k_gls_invalid.  q and beta of each wave live in
-// LDS; all waves reach the block barriers.
+// whitened residuals (FP64), residuals and fitted values).  Lane l holds
+// columns l + 64 m (m < MJ) of q, beta and the F form; a column's value
+// reaches every lane by a shuffle, so the kernel uses no LDS and no block
+// barrier.  Voxels whose pair is not valid are left to k_gls_invalid.
 // ---------------------------------------------------------------------------
+template <int MJ, int NV>
 __global__ void __launch_bounds__(64 * kSegGlsWaves) k_gls_seg(PlanDev P, const float* __restrict__ Y, int64_t ldY,
                                                                const float* __restrict__ scale,
                                                                const int32_t* __restrict__ perm,
@@ -276,158 +450,207 @@ __global__ void __launch_bounds__(64 * kSegGlsWaves) k_gls_seg(PlanDev P, const 
                                                                float* __restrict__ r2, double* __restrict__ resid_w,
                                                                int64_t ldR, float* __restrict__ resid,
                                                                float* __restrict__ pred_out) {
-    constexpr int MJ = (kSegMaxP + 63) / 64;
-    extern __shared__ double seg_lds[];
     const int lane = threadIdx.x & 63;
-    const int w = threadIdx.x >> 6;
+    const int wv = blockIdx.x * kSegGlsWaves + (threadIdx.x >> 6);
     const int T = P.T, p = P.p;
-    double* q = seg_lds + (size_t)w * 2 * p;
-    double* be = q + p;
-    const int i = blockIdx.x * kSegGlsWaves + w;
-    int v = i < n_slots ? (perm ? perm[i] : i) : -1;
-    if (v >= V) v = -1;
-    int k = v >= 0 ? best[v] : -1;
-    const bool active = v >= 0 && k >= 0 && k < P.K;  // wave-uniform
-    if (!active) k = 0;
-    const double s = (active && scale) ? (double)scale[v] : 1.0;
-    const double inv = 1.0 / s;
+    int vox[NV];
+    bool act[NV];
+    int k = -1;
+#pragma unroll
+    for (int u = 0; u < NV; ++u) {
+        const int i = wv * NV + u;
+        int v = i < n_slots ? (perm ? perm[i] : i) : -1;
+        if (v >= V) v = -1;
+        const int kv = v >= 0 ? best[v] : -1;
+        act[u] = v >= 0 && kv >= 0 && kv < P.K;
+        vox[u] = act[u] ? v : 0;
+        if (act[u] && k < 0) k = kv;  // the slots of a wave share one pair
+    }
+    if (k < 0) return;  // wave-uniform: no active slot
+    double sc[NV], inv[NV];
+    const float* rows[NV];
+#pragma unroll
+    for (int u = 0; u < NV; ++u) {
+        sc[u] = (act[u] && scale) ? (double)scale[vox[u]] : 1.0;
+        inv[u] = 1.0 / sc[u];
+        rows[u] = Y + (size_t)vox[u] * ldY;
+    }
     const double* __restrict__ Xw = P.Xw64 + (size_t)k * T * p;
     const double* __restrict__ cR = P.coefR + (size_t)k * P.Tpad * 4;
-    const float* __restrict__ row = Y + (size_t)(active ? v : 0) * ldY;
-    double sy = 0.0;
-    if (active) {
-        double qa[MJ];
+    const double* __restrict__ Ai = P.Ainv + (size_t)k * p * p;
+    const double* __restrict__ F = P.Fm + (size_t)k * p * p;
+
+    // pass 1: q = X_w' yw (t ascending) and the sums of y for the means
+    double acc[MJ][NV], sy[NV];
 #pragma unroll
-        for (int m = 0; m < MJ; ++m) qa[m] = 0.0;
-        double ec = 0.0, yc = 0.0;
+    for (int m = 0; m < MJ; ++m)
+#pragma unroll
+        for (int u = 0; u < NV; ++u) acc[m][u] = 0.0;
+    {
+        double ec[NV], yc[NV];
+#pragma unroll
+        for (int u = 0; u < NV; ++u) sy[u] = ec[u] = yc[u] = 0.0;
         for (int t0 = 0; t0 < T; t0 += 64) {
-            double yt;
-            const double yw = seg_whiten_chunk(cR, row, inv, T, t0, ec, yc, yt);
-            sy += yt;
+            double yw[NV];
+#pragma unroll
+            for (int u = 0; u < NV; ++u) {
+                double yt;
+                yw[u] = seg_whiten_chunk(cR, rows[u], inv[u], T, t0, ec[u], yc[u], yt);
+                sy[u] += yt;
+            }
             const int n = min(64, T - t0);
             for (int ii = 0; ii < n; ++ii) {
-                const double ywi = __shfl(yw, ii);
                 const double* xr = Xw + (size_t)(t0 + ii) * p;
+                double xv[MJ];
 #pragma unroll
                 for (int m = 0; m < MJ; ++m) {
                     const int j = lane + 64 * m;
-                    if (j < p) qa[m] = fma(xr[j], ywi, qa[m]);
+                    xv[m] = j < p ? xr[j] : 0.0;
+                }
+#pragma unroll
+                for (int u = 0; u < NV; ++u) {
+                    const double ywi = __shfl(yw[u], ii);
+#pragma unroll
+                    for (int m = 0; m < MJ; ++m) acc[m][u] = fma(xv[m], ywi, acc[m][u]);
                 }
             }
         }
-#pragma unroll
-        for (int m = 0; m < MJ; ++m) {
-            const int j = lane + 64 * m;
-            if (j < p) q[j] = qa[m];
-        }
     }
-    __syncthreads();
-    const double* __restrict__ Ai = P.Ainv + (size_t)k * p * p;
-    if (active) {
-        // beta_i = sum_j Ainv[i][j] q_j, read as row j of the symmetric Ainv (coalesced)
-        double ba[MJ];
+    // beta_c = sum_j Ainv[j][c] q_j (row j of the symmetric Ainv, coalesced)
+    double bl[MJ][NV];
 #pragma unroll
-        for (int m = 0; m < MJ; ++m) ba[m] = 0.0;
-        for (int j = 0; j < p; ++j) {
-            const double qj = q[j];
-            const double* ar = Ai + (size_t)j * p;
+    for (int m = 0; m < MJ; ++m)
+#pragma unroll
+        for (int u = 0; u < NV; ++u) bl[m][u] = 0.0;
+#pragma unroll
+    for (int m2 = 0; m2 < MJ; ++m2) {
+        const int nj = min(64, p - 64 * m2);
+        for (int ii = 0; ii < nj; ++ii) {
+            const double* ar = Ai + (size_t)(64 * m2 + ii) * p;
+            double av[MJ];
 #pragma unroll
             for (int m = 0; m < MJ; ++m) {
                 const int c = lane + 64 * m;
-                if (c < p) ba[m] = fma(ar[c], qj, ba[m]);
+                av[m] = c < p ? ar[c] : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < NV; ++u) {
+                const double qj = __shfl(acc[m2][u], ii);
+#pragma unroll
+                for (int m = 0; m < MJ; ++m) bl[m][u] = fma(av[m], qj, bl[m][u]);
             }
         }
-#pragma unroll
-        for (int m = 0; m < MJ; ++m) {
-            const int c = lane + 64 * m;
-            if (c < p) be[c] = ba[m];
-        }
     }
-    __syncthreads();
-    if (!active) return;  // no barrier below
-    sy = wave_sum(sy);
-    const double mean = sy / T;
-    double bl[MJ];
+    double mean[NV];
 #pragma unroll
-    for (int m = 0; m < MJ; ++m) {
-        const int c = lane + 64 * m;
-        bl[m] = c < p ? be[c] : 0.0;
-    }
-    // residual pass: per sample, the two dot products over j (lanes over j,
-    // then a wave sum); lane ii of the chunk keeps sample t0 + ii
-    double ssw = 0.0, ssres = 0.0, sstot = 0.0;
-    double ec = 0.0, yc = 0.0;
-    for (int t0 = 0; t0 < T; t0 += 64) {
-        double yt;
-        const double yw = seg_whiten_chunk(cR, row, inv, T, t0, ec, yc, yt);
-        const int n = min(64, T - t0);
-        double dw = 0.0, dx = 0.0;
-        for (int ii = 0; ii < n; ++ii) {
-            const double* xwr = Xw + (size_t)(t0 + ii) * p;
-            const double* xr = P.X + (size_t)(t0 + ii) * p;
-            double aw = 0.0, ax = 0.0;
+    for (int u = 0; u < NV; ++u) mean[u] = wave_sum(sy[u]) / T;
+    // residual pass: per sample, the dot products over j for each voxel (lanes
+    // over j, then wave sums); lane ii of the chunk keeps sample t0 + ii
+    double ssw[NV], ssres[NV], sstot[NV];
+    {
+        double ec[NV], yc[NV];
 #pragma unroll
-            for (int m = 0; m < MJ; ++m) {
-                const int j = lane + 64 * m;
-                if (j < p) {
-                    aw = fma(xwr[j], bl[m], aw);
-                    ax = fma(xr[j], bl[m], ax);
+        for (int u = 0; u < NV; ++u) ssw[u] = ssres[u] = sstot[u] = ec[u] = yc[u] = 0.0;
+        for (int t0 = 0; t0 < T; t0 += 64) {
+            double yw[NV], yt[NV], dw[NV], dx[NV];
+#pragma unroll
+            for (int u = 0; u < NV; ++u) {
+                yw[u] = seg_whiten_chunk(cR, rows[u], inv[u], T, t0, ec[u], yc[u], yt[u]);
+                dw[u] = dx[u] = 0.0;
+            }
+            const int n = min(64, T - t0);
+            for (int ii = 0; ii < n; ++ii) {
+                const double* xwr = Xw + (size_t)(t0 + ii) * p;
+                const double* xr = P.X + (size_t)(t0 + ii) * p;
+                double wv_[MJ], xv[MJ];
+#pragma unroll
+                for (int m = 0; m < MJ; ++m) {
+                    const int j = lane + 64 * m;
+                    wv_[m] = j < p ? xwr[j] : 0.0;
+                    xv[m] = j < p ? xr[j] : 0.0;
+                }
+#pragma unroll
+                for (int u = 0; u < NV; ++u) {
+                    double aw = 0.0, ax = 0.0;
+#pragma unroll
+                    for (int m = 0; m < MJ; ++m) {
+                        aw = fma(wv_[m], bl[m][u], aw);
+                        ax = fma(xv[m], bl[m][u], ax);
+                    }
+                    aw = wave_sum(aw);
+                    ax = wave_sum(ax);
+                    if (lane == ii) {
+                        dw[u] = aw;
+                        dx[u] = ax;
+                    }
                 }
             }
-            aw = wave_sum(aw);
-            ax = wave_sum(ax);
-            if (lane == ii) {
-                dw = aw;
-                dx = ax;
+            if (lane < n) {
+                const int t = t0 + lane;
+#pragma unroll
+                for (int u = 0; u < NV; ++u) {
+                    if (!act[u]) continue;
+                    const double r = yw[u] - dw[u];
+                    const double dr = yt[u] - dx[u], dm = yt[u] - mean[u];
+                    ssw[u] += r * r;
+                    ssres[u] += dr * dr;
+                    sstot[u] += dm * dm;
+                    const size_t o = (size_t)vox[u] * ldR + t;
+                    if (resid_w) resid_w[o] = r * sc[u];
+                    if (resid) resid[o] = (float)(dr * sc[u]);
+                    if (pred_out) pred_out[o] = (float)(dx[u] * sc[u]);
+                }
             }
         }
-        if (lane < n) {
-            const int t = t0 + lane;
-            const double r = yw - dw;
-            const double dr = yt - dx, dm = yt - mean;
-            ssw += r * r;
-            ssres += dr * dr;
-            sstot += dm * dm;
-            if (resid_w) resid_w[(size_t)v * ldR + t] = r * s;
-            if (resid) resid[(size_t)v * ldR + t] = (float)(dr * s);
-            if (pred_out) pred_out[(size_t)v * ldR + t] = (float)(dx * s);
+    }
+    // F form: u_c = sum_a beta_a Fm[a][c] (row a of Fm, coalesced)
+#pragma unroll
+    for (int m = 0; m < MJ; ++m)
+#pragma unroll
+        for (int u = 0; u < NV; ++u) acc[m][u] = 0.0;
+#pragma unroll
+    for (int m2 = 0; m2 < MJ; ++m2) {
+        const int na = min(64, p - 64 * m2);
+        for (int ii = 0; ii < na; ++ii) {
+            const double* fr = F + (size_t)(64 * m2 + ii) * p;
+            double fv[MJ];
+#pragma unroll
+            for (int m = 0; m < MJ; ++m) {
+                const int c = lane + 64 * m;
+                fv[m] = c < p ? fr[c] : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < NV; ++u) {
+                const double ba = __shfl(bl[m2][u], ii);
+#pragma unroll
+                for (int m = 0; m < MJ; ++m) acc[m][u] = fma(fv[m], ba, acc[m][u]);
+            }
         }
     }
-    ssw = wave_sum(ssw);
-    ssres = wave_sum(ssres);
-    sstot = wave_sum(sstot);
     const int dof = max(1, T - p);
-    const double sig = ssw / dof;
-    // t statistics and betas; u_c = sum_a beta_a Fm[a][c] for the F form
-    const double* __restrict__ F = P.Fm + (size_t)k * p * p;
-    double ua[MJ];
 #pragma unroll
-    for (int m = 0; m < MJ; ++m) ua[m] = 0.0;
-    for (int a = 0; a < p; ++a) {
-        const double ba = be[a];
-        const double* fr = F + (size_t)a * p;
+    for (int u = 0; u < NV; ++u) {
+        const double w2 = wave_sum(ssw[u]), rs = wave_sum(ssres[u]), tt = wave_sum(sstot[u]);
+        const double sig = w2 / dof;
+        double quad = 0.0;
 #pragma unroll
         for (int m = 0; m < MJ; ++m) {
             const int c = lane + 64 * m;
-            if (c < p) ua[m] = fma(fr[c], ba, ua[m]);
+            if (c < p) {
+                quad = fma(acc[m][u], bl[m][u], quad);
+                if (act[u]) {
+                    const double se = sqrt(sig * Ai[(size_t)c * p + c]);
+                    tstats[(size_t)vox[u] * p + c] = (float)(bl[m][u] / (se + 1e-10));
+                    betas[(size_t)vox[u] * p + c] = (float)(bl[m][u] * sc[u]);
+                }
+            }
         }
-    }
-    double quad = 0.0;
-#pragma unroll
-    for (int m = 0; m < MJ; ++m) {
-        const int c = lane + 64 * m;
-        if (c < p) {
-            quad = fma(ua[m], bl[m], quad);
-            const double se = sqrt(sig * Ai[(size_t)c * p + c]);
-            tstats[(size_t)v * p + c] = (float)(bl[m] / (se + 1e-10));
-            betas[(size_t)v * p + c] = (float)(bl[m] * s);
+        quad = wave_sum(quad);
+        if (lane == 0 && act[u]) {
+            fstats[vox[u]] = (float)(quad / (sig * P.n_task + 1e-10));
+            sigma2[vox[u]] = (float)(sig * sc[u] * sc[u]);
+            r2[vox[u]] = (float)(tt <= 1e-10 ? 0.0 : fmin(1.0 - rs / tt, 1.0));
         }
-    }
-    quad = wave_sum(quad);
-    if (lane == 0) {
-        fstats[v] = (float)(quad / (sig * P.n_task + 1e-10));
-        sigma2[v] = (float)(sig * s * s);
-        r2[v] = (float)(sstot <= 1e-10 ? 0.0 : fmin(1.0 - ssres / sstot, 1.0));
     }
 }
 

@@ -36,9 +36,9 @@ VOXEL_SCATTER_ATTRS = [
 ]
 
 # Widest design of the pair-set searches (hierarchical descent on demand,
-# learned windows and bands), the split 16-bit projections, the FP64 search and
-# the grouped engine; wider designs (up to 768 columns) use the full-grid
-# surface with the fp32 projection over 64-column segments.
+# learned windows and bands), the FP64 search and the grouped engine; wider
+# designs (up to 768 columns) use the full-grid surface, with the projection
+# over 64-column segments of each pair.
 WIDE_P = 96
 
 # Runs are offset by this much in the Ljung-Box pseudo-time (glm/arma.py:5969).
@@ -1098,10 +1098,8 @@ def fit_glm_arma11(data, design, tr, a_grid=None, b_grid=None, estimate_per_voxe
             b_grid = db if b_grid is None else b_grid
         if grid is None:
             grid = RemlGrid(design, a_grid, b_grid, dev, run_starts=run_starts, tau=tau, task_indices=task_indices)
-        # designs wider than 96 columns: fp32 projection over 64-column segments
-        proj = projection if p <= WIDE_P else "fp32"
-        if grid.projection != proj:
-            grid.set_projection(proj)
+        if grid.projection != projection:
+            grid.set_projection(projection)
         fixed_best = None
     if p > WIDE_P and use_double and fixed_best is None:
         raise NotImplementedError(f"use_double search with p = {p} > {WIDE_P} design columns")

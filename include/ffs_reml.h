@@ -45,10 +45,11 @@ const char* ffs_reml_last_error(void);
  *   T, p        retained time points and design columns (p <= 768).  Designs of
  *               more than 96 columns keep each pair's columns in segments of 64
  *               projection columns: they support prep, energy, project, surface,
- *               select and the GLS (fp32 projection only); the pair-set searches
- *               (windows, bands, hierarchical descent on demand, subset search),
- *               prep_features, the split 16-bit projection modes, surface_f64 and
- *               the grouped engine return -4 on them
+ *               select and the GLS (the split 16-bit projection modes read the
+ *               halves that prep wrote in the same mode, so ffs_reml_surface runs
+ *               them with n_chunks = 1); the pair-set searches (windows, bands,
+ *               hierarchical descent on demand, subset search), prep_features,
+ *               surface_f64 and the grouped engine return -4 on them
  *   X           host, T x p row-major float64 design
  *   pinvT       host, T x p row-major float64 transpose of the design pseudo-inverse
  *   gap         host, T int32: tau[t+1]-tau[t] inside a run, 0 at the last point of a run

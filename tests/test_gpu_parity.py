@@ -172,7 +172,7 @@ SPLIT_TOL = {"fp16x3": dict(atol=5e-4, rtol=2e-6, tie=2e-3), "bf16x3": dict(atol
 
 
 @pytest.mark.parametrize("mode", ["fp16x3", "bf16x3"])
-@pytest.mark.parametrize("name", ["plain", "censored", "afni", "wide"])
+@pytest.mark.parametrize("name", ["plain", "censored", "afni", "wide", "seg120", "seg300"])
 def test_split_projection_matches_oracle(dev, name, mode):
     from fastfuncstuff_amd.glm.arma import RemlGrid
 
