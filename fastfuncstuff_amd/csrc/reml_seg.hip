@@ -391,19 +391,17 @@ __device__ __forceinline__ double wave_sum(double x) {
     return x;
 }
 
-// Whitened samples yw_t = sqrt(1/v_t) e_t of one 64-sample chunk, lane i
-// holding t = t0 + i: e_t = (y_t - c_{t-1} y_{t-1}) + alpha_{t-1} e_{t-1}
+// Whitened samples yw_t = sqrt(1/v_t) e_t of one 64-sample chunk of a series
+// y (lane i holds y_t, t = t0 + i; 0 past T): e_t = (y_t - c_{t-1} y_{t-1}) + alpha_{t-1} e_{t-1}
 // (c = alpha = 0 at run starts: the shifted coefficient rows of coefR) as an
 // inclusive scan of the affine maps e -> alpha e + z over the chunk, started
 // from the previous chunk's last e and y (ecarry, ycarry; updated).  Lanes
 // past T give 0.
-__device__ __forceinline__ double seg_whiten_chunk(const double* __restrict__ cR, const float* __restrict__ row,
-                                                   double inv, int T, int t0, double& ecarry, double& ycarry,
-                                                   double& yt_out) {
+__device__ __forceinline__ double seg_whiten_vals(const double* __restrict__ cR, double yt, int T, int t0,
+                                                  double& ecarry, double& ycarry) {
     const int lane = threadIdx.x & 63;
     const int t = t0 + lane;
     const bool in = t < T;
-    const double yt = in ? (double)row[t] * inv : 0.0;
     double yprev = __shfl_up(yt, 1);
     if (lane == 0) yprev = ycarry;
     const double* c4 = cR + 4 * (size_t)(in ? t : 0);
@@ -420,8 +418,18 @@ __device__ __forceinline__ double seg_whiten_chunk(const double* __restrict__ cR
     const double e = fma(a, ecarry, z);
     ecarry = __shfl(e, 63);
     ycarry = __shfl(yt, 63);
-    yt_out = yt;
     return in ? e * c4[0] : 0.0;
+}
+
+// The same for the samples of a series row (float, scaled by inv); yt_out =
+// the lane's scaled sample (0 past T).
+__device__ __forceinline__ double seg_whiten_chunk(const double* __restrict__ cR, const float* __restrict__ row,
+                                                   double inv, int T, int t0, double& ecarry, double& ycarry,
+                                                   double& yt_out) {
+    const int t = t0 + (threadIdx.x & 63);
+    const double yt = t < T ? (double)row[t] * inv : 0.0;
+    yt_out = yt;
+    return seg_whiten_vals(cR, yt, T, t0, ecarry, ycarry);
 }
 
 // ---------------------------------------------------------------------------
@@ -432,7 +440,7 @@ __device__ __forceinline__ double seg_whiten_chunk(const double* __restrict__ cR
 //   yw = L^-1 y (the innovations recurrence, by chunk scans), q = X_w' yw,
 //   beta = (X'R^-1X)^-1 q  (the plan's Ainv; for a rank-deficient design
 //   its pseudo-inverse, which gives the minimum-norm solution),
-//   r = yw - X_w beta, sigma2 = r'r / max(1, T - p), t_j = beta_j / sqrt(sigma2 Ainv_jj),
+//   r = L^-1 (y - X beta) (= yw - X_w beta), sigma2 = r'r / max(1, T - p), t_j = beta_j / sqrt(sigma2 Ainv_jj),
 //   F = beta' Fm beta / (sigma2 n_task + 1e-10), R^2 = CoD(y, X beta),
 // with the outputs of k_gls (betas and sigma2 unscaled; optional series:
 // whitened residuals (FP64), residuals and fitted values).  Lane l holds
@@ -544,54 +552,47 @@ __global__ void __launch_bounds__(64 * kSegGlsWaves) k_gls_seg(PlanDev P, const 
     double mean[NV];
 #pragma unroll
     for (int u = 0; u < NV; ++u) mean[u] = wave_sum(sy[u]) / T;
-    // residual pass: per sample, the dot products over j for each voxel (lanes
-    // over j, then wave sums); lane ii of the chunk keeps sample t0 + ii
+    // residual pass: per sample, the fitted value x_t' beta of each voxel
+    // (lanes over j, then a wave sum; lane ii of the chunk keeps sample
+    // t0 + ii), then r = L^-1 (y - X beta) = yw - X_w beta by the chunk scan
+    // of the residual (the rows of the design X are shared by every voxel)
     double ssw[NV], ssres[NV], sstot[NV];
     {
-        double ec[NV], yc[NV];
+        double ec[NV], dc[NV];
 #pragma unroll
-        for (int u = 0; u < NV; ++u) ssw[u] = ssres[u] = sstot[u] = ec[u] = yc[u] = 0.0;
+        for (int u = 0; u < NV; ++u) ssw[u] = ssres[u] = sstot[u] = ec[u] = dc[u] = 0.0;
         for (int t0 = 0; t0 < T; t0 += 64) {
-            double yw[NV], yt[NV], dw[NV], dx[NV];
+            const int t = t0 + lane;
+            double yt[NV], dx[NV];
 #pragma unroll
             for (int u = 0; u < NV; ++u) {
-                yw[u] = seg_whiten_chunk(cR, rows[u], inv[u], T, t0, ec[u], yc[u], yt[u]);
-                dw[u] = dx[u] = 0.0;
+                yt[u] = t < T ? (double)rows[u][t] * inv[u] : 0.0;
+                dx[u] = 0.0;
             }
             const int n = min(64, T - t0);
             for (int ii = 0; ii < n; ++ii) {
-                const double* xwr = Xw + (size_t)(t0 + ii) * p;
                 const double* xr = P.X + (size_t)(t0 + ii) * p;
-                double wv_[MJ], xv[MJ];
+                double xv[MJ];
 #pragma unroll
                 for (int m = 0; m < MJ; ++m) {
                     const int j = lane + 64 * m;
-                    wv_[m] = j < p ? xwr[j] : 0.0;
                     xv[m] = j < p ? xr[j] : 0.0;
                 }
 #pragma unroll
                 for (int u = 0; u < NV; ++u) {
-                    double aw = 0.0, ax = 0.0;
+                    double ax = 0.0;
 #pragma unroll
-                    for (int m = 0; m < MJ; ++m) {
-                        aw = fma(wv_[m], bl[m][u], aw);
-                        ax = fma(xv[m], bl[m][u], ax);
-                    }
-                    aw = wave_sum(aw);
+                    for (int m = 0; m < MJ; ++m) ax = fma(xv[m], bl[m][u], ax);
                     ax = wave_sum(ax);
-                    if (lane == ii) {
-                        dw[u] = aw;
-                        dx[u] = ax;
-                    }
+                    if (lane == ii) dx[u] = ax;
                 }
             }
-            if (lane < n) {
-                const int t = t0 + lane;
 #pragma unroll
-                for (int u = 0; u < NV; ++u) {
-                    if (!act[u]) continue;
-                    const double r = yw[u] - dw[u];
-                    const double dr = yt[u] - dx[u], dm = yt[u] - mean[u];
+            for (int u = 0; u < NV; ++u) {
+                const double dr = t < T ? yt[u] - dx[u] : 0.0;
+                const double r = seg_whiten_vals(cR, dr, T, t0, ec[u], dc[u]);
+                if (t < T && act[u]) {
+                    const double dm = yt[u] - mean[u];
                     ssw[u] += r * r;
                     ssres[u] += dr * dr;
                     sstot[u] += dm * dm;
