@@ -312,6 +312,7 @@ def im_design(args):
         "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": args.projection, "data": "synthetic",
         "config": {"workload": f"im596_T{T}_5runs_p{p}_grid33x33", "V": V, "T": T, "p": p, "K": grid.K,
+                   "design_rank": grid.rank,
                    "path": "segmented (p > 96): full-grid surface, projection over 64-column segments, "
                            "hierarchical selection, FP64 GLS", "projection": args.projection},
         "kernels_ms": per,

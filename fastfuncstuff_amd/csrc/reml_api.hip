@@ -1026,8 +1026,9 @@ extern "C" int ffs_reml_gls(const ffs_reml_plan* P, const float* Y, int64_t ldY,
     if (int rc = gls_impl(P, Y, ldY, scale, best, V, betas, tstats, fstats, sigma2, r2, resid_w, ldR, resid, pred, work,
                           s))
         return rc;
-    if (V > 0 && P->d.Pi) {
-        // rank-deficient design: minimum-norm beta and its statistics
+    if (V > 0 && P->d.Pi && !P->d.nseg) {
+        // rank-deficient design: minimum-norm beta and its statistics (the
+        // segmented GLS computes beta with the pseudo-inverse directly)
         hipLaunchKernelGGL(k_gls_rank, dim3((unsigned)V), dim3(64), 2 * sizeof(double) * P->d.p, s, P->d, (int)V, scale, best, betas, tstats,
                            fstats, sigma2, nullptr, nullptr);
         HIP_TRY(hipGetLastError());
