@@ -501,7 +501,10 @@ static int plans_build(std::vector<PlanPending>& pend, const std::vector<TablesO
     if (int rc = P0->alloc(&dstatus, G)) return rc;
     HIP_TRY(hipMemcpyAsync(douts, touts.data(), sizeof(TablesOut) * G, hipMemcpyHostToDevice, s));
     HIP_TRY(hipMemsetAsync(dstatus, 0, sizeof(int32_t) * G, s));
-    hipLaunchKernelGGL(k_tables, dim3(ka.K, G), dim3(64), 0, s, ka, douts, dstatus);
+    // one block per (pair, design): 64 lanes, 256 for designs wider than 96
+    // columns (the O(T p^2) loops; every table element is computed by one lane
+    // in the same order either way)
+    hipLaunchKernelGGL(k_tables, dim3(ka.K, G), dim3(ka.p > 96 ? 256 : 64), 0, s, ka, douts, dstatus);
     HIP_TRY(hipGetLastError());
     for (PlanPending& q : pend) {
         if (q.gls_only) continue;

@@ -28,7 +28,7 @@ typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 //   work:    [K][2*p*p] Gauss-Jordan / Cholesky-inverse workspace
 // ---------------------------------------------------------------------------
 // Block-cooperative forms of cholesky_inplace / cholesky_inverse /
-// gauss_jordan_inverse (reml_math.h) for the 64-lane k_tables block.  Every
+// gauss_jordan_inverse (reml_math.h) for the k_tables block (64 or 256 lanes).  Every
 // element is computed by one lane with the serial routine's expression and
 // summation order; only independent rows / columns are spread over lanes, with
 // a barrier between dependent steps.  `flag` is block-shared and starts at 0;
@@ -57,7 +57,7 @@ __device__ int block_cholesky(int p, double* A, int* flag, const int32_t* dep = 
         __syncthreads();
         if (*flag) return -1;
         const double d = A[j * p + j];
-        for (int i = j + 1 + tid; i < p; i += 64) {
+        for (int i = j + 1 + tid; i < p; i += (int)blockDim.x) {
             if (skip) {
                 A[i * p + j] = 0.0;
                 continue;
@@ -66,7 +66,7 @@ __device__ int block_cholesky(int p, double* A, int* flag, const int32_t* dep = 
             for (int k = 0; k < j; ++k) s -= A[i * p + k] * A[j * p + k];
             A[i * p + j] = s / d;
         }
-        for (int k = j + 1 + tid; k < p; k += 64) A[j * p + k] = 0.0;
+        for (int k = j + 1 + tid; k < p; k += (int)blockDim.x) A[j * p + k] = 0.0;
         __syncthreads();
     }
     return 0;
@@ -78,7 +78,7 @@ __device__ int block_cholesky(int p, double* A, int* flag, const int32_t* dep = 
 // columns.
 __device__ void block_cholesky_inverse(int p, const double* L, double* Ainv, double* tmp) {
     const int tid = threadIdx.x;
-    for (int j = tid; j < p; j += 64) {
+    for (int j = tid; j < p; j += (int)blockDim.x) {
         for (int i = 0; i < p; ++i) tmp[i * p + j] = 0.0;
         if (L[j * p + j] == 0.0) continue;
         tmp[j * p + j] = 1.0 / L[j * p + j];
@@ -90,7 +90,7 @@ __device__ void block_cholesky_inverse(int p, const double* L, double* Ainv, dou
         }
     }
     __syncthreads();
-    for (int idx = tid; idx < p * p; idx += 64) {
+    for (int idx = tid; idx < p * p; idx += (int)blockDim.x) {
         const int i = idx / p, j = idx % p;
         if (j > i) continue;
         double s = 0.0;
@@ -104,14 +104,14 @@ __device__ void block_cholesky_inverse(int p, const double* L, double* Ainv, dou
 // A <- Pi A Pi' (p x p, A symmetric); W (p x p) is workspace.
 __device__ void block_sandwich(int p, const double* Pi, double* A, double* W) {
     const int tid = threadIdx.x;
-    for (int idx = tid; idx < p * p; idx += 64) {
+    for (int idx = tid; idx < p * p; idx += (int)blockDim.x) {
         const int i = idx / p, j = idx % p;
         double s = 0.0;
         for (int q = 0; q < p; ++q) s = fma(Pi[i * p + q], A[q * p + j], s);
         W[idx] = s;
     }
     __syncthreads();
-    for (int idx = tid; idx < p * p; idx += 64) {
+    for (int idx = tid; idx < p * p; idx += (int)blockDim.x) {
         const int i = idx / p, j = idx % p;
         if (j > i) continue;
         double s = 0.0;
@@ -126,7 +126,7 @@ __device__ void block_sandwich(int p, const double* Pi, double* A, double* W) {
 __device__ int block_gauss_jordan_inverse(int p, const double* A, double* inv, double* work, int* piv_sh, int* flag,
                                           int fail_code) {
     const int tid = threadIdx.x;
-    for (int i = tid; i < p * p; i += 64) {
+    for (int i = tid; i < p * p; i += (int)blockDim.x) {
         work[i] = A[i];
         inv[i] = (i / p == i % p) ? 1.0 : 0.0;
     }
@@ -147,7 +147,7 @@ __device__ int block_gauss_jordan_inverse(int p, const double* A, double* inv, d
         if (*flag) return -1;
         const int piv = *piv_sh;
         if (piv != col)
-            for (int j = tid; j < p; j += 64) {
+            for (int j = tid; j < p; j += (int)blockDim.x) {
                 double t = work[col * p + j];
                 work[col * p + j] = work[piv * p + j];
                 work[piv * p + j] = t;
@@ -158,12 +158,12 @@ __device__ int block_gauss_jordan_inverse(int p, const double* A, double* inv, d
         __syncthreads();
         const double d = work[col * p + col];
         __syncthreads();
-        for (int j = tid; j < p; j += 64) {
+        for (int j = tid; j < p; j += (int)blockDim.x) {
             work[col * p + j] /= d;
             inv[col * p + j] /= d;
         }
         __syncthreads();
-        for (int r = tid; r < p; r += 64) {
+        for (int r = tid; r < p; r += (int)blockDim.x) {
             if (r == col) continue;
             const double f = work[r * p + col];
             if (f == 0.0) continue;
@@ -177,7 +177,7 @@ __device__ int block_gauss_jordan_inverse(int p, const double* A, double* inv, d
     return 0;
 }
 
-__global__ void __launch_bounds__(64) k_tables(KTablesArgs A, const TablesOut* __restrict__ outs,
+__global__ void __launch_bounds__(256) k_tables(KTablesArgs A, const TablesOut* __restrict__ outs,
                                                int* __restrict__ status_all) {
     // one block per (pair, design): blockIdx.x = pair, blockIdx.y = design
     const TablesOut o = outs[blockIdx.y];
@@ -211,7 +211,7 @@ __global__ void __launch_bounds__(64) k_tables(KTablesArgs A, const TablesOut* _
     double* __restrict__ coefR = o.coefR;
     const int k = blockIdx.x;
     if (k >= K) return;
-    const int tid = threadIdx.x;
+    const int tid = threadIdx.x, nth = blockDim.x;  // 64, or 256 for wide designs
     const double a = ab[2 * k], b = ab[2 * k + 1];
     double* s2 = soa + (size_t)k * 3 * T;
     double* cc = s2 + T;
@@ -230,16 +230,16 @@ __global__ void __launch_bounds__(64) k_tables(KTablesArgs A, const TablesOut* _
     }
     __syncthreads();
     double* cf = coef64 + (size_t)k * T * 3;
-    for (int t = tid; t < T; t += 64) {
+    for (int t = tid; t < T; t += nth) {
         cf[3 * t + 0] = sqrt(s2[t]);
         cf[3 * t + 1] = cc[t];
         cf[3 * t + 2] = al[t];
     }
     // X_w = L^-1 X, one column per thread.
-    for (int j = tid; j < p; j += 64) whiten_forward(T, s2, cc, al, gap, X + j, p, xw + j, p);
+    for (int j = tid; j < p; j += nth) whiten_forward(T, s2, cc, al, gap, X + j, p, xw + j, p);
     __syncthreads();
     // A = X_w' X_w  (into L, full symmetric)
-    for (int idx = tid; idx < p * p; idx += 64) {
+    for (int idx = tid; idx < p * p; idx += nth) {
         const int i = idx / p, j = idx % p;
         if (j > i) continue;
         double s = 0.0;
@@ -253,12 +253,12 @@ __global__ void __launch_bounds__(64) k_tables(KTablesArgs A, const TablesOut* _
     // element gets the arithmetic of the one-lane reml_math.h routines in the
     // same order, so the tables are identical to a serial build.
     double* F = Fm + (size_t)k * p * p;
-    for (int i = tid; i < p * p; i += 64) F[i] = full_f ? L[i] : 0.0;
+    for (int i = tid; i < p * p; i += nth) F[i] = full_f ? L[i] : 0.0;
     __syncthreads();
     // ridge plans (glm/arma.py:1160-1176, 2724-2755): A + ridge I in the
     // log-determinant and in the solve for beta
     if (ridge > 0.0)
-        for (int j = tid; j < p; j += 64) L[j * p + j] += ridge;
+        for (int j = tid; j < p; j += nth) L[j * p + j] += ridge;
     __syncthreads();
     // rank-deficient designs (never ridge plans): the dependent columns get
     // zero pivots (glm/arma.py:1493-1500 falls back to QR, whose R has a
@@ -279,14 +279,14 @@ __global__ void __launch_bounds__(64) k_tables(KTablesArgs A, const TablesOut* _
             // |y_w - X_w beta|^2 = E - c' W c, c = X_w' y_w, M = (A + rI)^-1,
             // W = M (A + 2rI) M = M + r M M; L <- chol(W)
             double* Wm = wk;
-            for (int idx = tid; idx < p * p; idx += 64) {
+            for (int idx = tid; idx < p * p; idx += nth) {
                 const int i = idx / p, j = idx % p;
                 double mm = 0.0;
                 for (int q = 0; q < p; ++q) mm += Ai[i * p + q] * Ai[q * p + j];
                 Wm[idx] = Ai[idx] + ridge * mm;
             }
             __syncthreads();
-            for (int i = tid; i < p * p; i += 64) L[i] = Wm[i];
+            for (int i = tid; i < p * p; i += nth) L[i] = Wm[i];
             __syncthreads();
             block_cholesky(p, L, &sh_fail);
         }
@@ -296,17 +296,17 @@ __global__ void __launch_bounds__(64) k_tables(KTablesArgs A, const TablesOut* _
             // F zeroed and the block embedded at the task positions.
             double* At = wk;
             double* W2 = wk + p * p;
-            for (int idx = tid; idx < n_task * n_task; idx += 64) {
+            for (int idx = tid; idx < n_task * n_task; idx += nth) {
                 const int i = idx / n_task, j = idx % n_task;
                 At[idx] = Ai[task_idx[i] * p + task_idx[j]] + (i == j ? 1e-8 : 0.0);
             }
             __syncthreads();
             if (block_gauss_jordan_inverse(n_task, At, F, W2, &sh_piv, &sh_fail, 2) == 0) {
-                for (int q = tid; q < n_task * n_task; q += 64) W2[q] = F[q];
+                for (int q = tid; q < n_task * n_task; q += nth) W2[q] = F[q];
                 __syncthreads();
-                for (int q = tid; q < p * p; q += 64) F[q] = 0.0;
+                for (int q = tid; q < p * p; q += nth) F[q] = 0.0;
                 __syncthreads();
-                for (int idx = tid; idx < n_task * n_task; idx += 64) {
+                for (int idx = tid; idx < n_task * n_task; idx += nth) {
                     const int a2 = idx / n_task, b2 = idx % n_task;
                     F[task_idx[a2] * p + task_idx[b2]] = W2[idx];
                 }
@@ -320,7 +320,7 @@ __global__ void __launch_bounds__(64) k_tables(KTablesArgs A, const TablesOut* _
     }
     // Q rows: q_t = Lx^-1 xw_t (forward substitution), stored in g; ridge
     // plans: q_t = chol(W)' xw_t.
-    for (int t = tid; t < T; t += 64) {
+    for (int t = tid; t < T; t += nth) {
         const double* x = xw + (size_t)t * p;
         double* q = g + (size_t)t * p;
         if (ridge > 0.0) {
@@ -339,18 +339,18 @@ __global__ void __launch_bounds__(64) k_tables(KTablesArgs A, const TablesOut* _
     }
     __syncthreads();
     // G = L^-T Q, in place, one column per thread.
-    for (int j = tid; j < p; j += 64) whiten_adjoint(T, s2, cc, al, gap, g + j, p, g + j, p);
+    for (int j = tid; j < p; j += nth) whiten_adjoint(T, s2, cc, al, gap, g + j, p, g + j, p);
     __syncthreads();
     // fp32 projection operand, tile-major: G32[tile][t][slot*p + j]
     const int tile = k / npt, slot = k % npt;
     if (G32 && A.nseg > 0) {  // p > 96: the pair's segments of BN columns
-        for (int idx = tid; idx < T * p; idx += 64) {
+        for (int idx = tid; idx < T * p; idx += nth) {
             const int t = idx / p, j = idx % p;
             G32[((size_t)k * A.nseg + j / BN) * Tpad * BN + (size_t)t * BN + j % BN] = (float)g[idx];
         }
     } else if (G32) {  // null for a pair-list (GLS-only) plan
         float* Gt = G32 + (size_t)tile * Tpad * BN + slot * p;
-        for (int idx = tid; idx < T * p; idx += 64) {
+        for (int idx = tid; idx < T * p; idx += nth) {
             const int t = idx / p, j = idx % p;
             Gt[(size_t)t * BN + j] = (float)g[idx];
         }
@@ -360,12 +360,12 @@ __global__ void __launch_bounds__(64) k_tables(KTablesArgs A, const TablesOut* _
     // below the fp16 maximum and the low half stays out of the subnormal range.
     // The epilogue multiplies the sum of squares by 2^-2e (exact).
     if (Ghf) {
-        __shared__ double sh_max[64];
+        __shared__ double sh_max[256];
         double m = 0.0;
-        for (int idx = tid; idx < T * p; idx += 64) m = fmax(m, fabs(g[idx]));
+        for (int idx = tid; idx < T * p; idx += nth) m = fmax(m, fabs(g[idx]));
         sh_max[tid] = m;
         __syncthreads();
-        for (int off = 32; off > 0; off >>= 1) {
+        for (int off = nth >> 1; off > 0; off >>= 1) {
             if (tid < off) sh_max[tid] = fmax(sh_max[tid], sh_max[tid + off]);
             __syncthreads();
         }
@@ -374,7 +374,7 @@ __global__ void __launch_bounds__(64) k_tables(KTablesArgs A, const TablesOut* _
         const int e = min(max(14 - e2, -60), 60);
         if (tid == 0) g16_unscale[k] = (float)ldexp(1.0, -2 * e);
         const size_t tbase = (size_t)tile * Tpad * BN;
-        for (int idx = tid; idx < T * p; idx += 64) {
+        for (int idx = tid; idx < T * p; idx += nth) {
             const int t = idx / p, j = idx % p;
             // p > 96: column j % BN of the pair's segment tile k * nseg + j / BN
             const size_t o = A.nseg > 0
@@ -395,7 +395,7 @@ __global__ void __launch_bounds__(64) k_tables(KTablesArgs A, const TablesOut* _
     // f64 MFMA GLS operand: G padded to PG columns (p <= 32 only).
     if (G64p) {
         double* gp = G64p + (size_t)k * Tpad * PG;
-        for (int idx = tid; idx < T * p; idx += 64) {
+        for (int idx = tid; idx < T * p; idx += nth) {
             const int t = idx / p, j = idx % p;
             gp[(size_t)t * PG + j] = g[idx];
         }
@@ -404,7 +404,7 @@ __global__ void __launch_bounds__(64) k_tables(KTablesArgs A, const TablesOut* _
     // shifted so row t holds what step t needs (zeros past T and at run starts).
     {
         double* cr = coefR + (size_t)k * Tpad * 4;
-        for (int t = tid; t < T; t += 64) {
+        for (int t = tid; t < T; t += nth) {
             const bool start = (t == 0) || gap[t - 1] <= 0;
             cr[4 * t] = sqrt(s2[t]);
             cr[4 * t + 1] = start ? 0.0 : cc[t - 1];
