@@ -44,9 +44,8 @@ static int fail(int code, const std::string& msg) {
             return fail(-4, std::string(what) + ": pair-list plan (no lattice) supports GLS only"); \
     } while (0)
 
-// Entry points built on the pair sets, the split 16-bit operands, the
-// per-lane FP64 search or the grouped engine hold a pair's columns in one
-// projection tile: designs of at most 96 columns.
+// Entry points built on the pair sets or the grouped engine hold a pair's
+// columns in one projection tile: designs of at most 96 columns.
 #define FFS_NEED_NARROW(P, what)                                                                          \
     do {                                                                                                  \
         if ((P)->d.nseg)                                                                                  \
@@ -1197,7 +1196,9 @@ static void launch_surface_f64(const ffs_reml_plan* P, const TY* Y, int64_t ldY,
                                int64_t ldS, hipStream_t s) {
     const dim3 g((unsigned)((int64_t)((V + 63) / 64) * P->d.K)), b(64);
     const int p = P->d.p;
-    if (p <= 8)
+    if (P->d.nseg)
+        hipLaunchKernelGGL((k_surface_f64_seg<TY>), g, b, 0, s, P->d, Y, ldY, scale, V, S, ldS);
+    else if (p <= 8)
         hipLaunchKernelGGL((k_surface_f64<8, TY>), g, b, 0, s, P->d, Y, ldY, scale, V, S, ldS);
     else if (p <= 16)
         hipLaunchKernelGGL((k_surface_f64<16, TY>), g, b, 0, s, P->d, Y, ldY, scale, V, S, ldS);
@@ -1210,7 +1211,6 @@ static void launch_surface_f64(const ffs_reml_plan* P, const TY* Y, int64_t ldY,
 extern "C" int ffs_reml_surface_f64(const ffs_reml_plan* P, const void* Y, int32_t y_is_f64, int64_t ldY, int64_t V,
                                     const float* scale, double* S, int64_t ldS, void* stream) {
     if (!P) return fail(-3, "surface_f64: null plan");
-    FFS_NEED_NARROW(P, "surface_f64");
     FFS_NEED_SEARCH(P, "surface_f64");
     if (V <= 0) return 0;
     if (!Y || !S) return fail(-3, "surface_f64: null argument");

@@ -36,7 +36,7 @@ VOXEL_SCATTER_ATTRS = [
 ]
 
 # Widest design of the pair-set searches (hierarchical descent on demand,
-# learned windows and bands), the FP64 search and the grouped engine; wider
+# learned windows and bands) and the grouped engine; wider
 # designs (up to 768 columns) use the full-grid surface, with the projection
 # over 64-column segments of each pair.
 WIDE_P = 96
@@ -1101,8 +1101,6 @@ def fit_glm_arma11(data, design, tr, a_grid=None, b_grid=None, estimate_per_voxe
         if grid.projection != projection:
             grid.set_projection(projection)
         fixed_best = None
-    if p > WIDE_P and use_double and fixed_best is None:
-        raise NotImplementedError(f"use_double search with p = {p} > {WIDE_P} design columns")
     if p > WIDE_P and selector == "learned":
         raise NotImplementedError(f"the learned selector needs p <= {WIDE_P} design columns (p = {p})")
 

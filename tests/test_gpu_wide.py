@@ -48,7 +48,7 @@ def test_wide_design_fit_api_matches_oracle(dev):
     hierarchical descent, on the full surface for these widths) and every
     output against the FP64 oracle; a refit from the selected (a, b) as
     precomputed_arma_params (a pair-list plan) gives the same statistics; the
-    learned selector and the FP64 search refuse the width."""
+    learned selector refuses the width."""
     from fastfuncstuff_amd.glm.arma import fit_glm_arma11
 
     print("[wide] fit_glm_arma11 p=128", flush=True)
@@ -93,5 +93,29 @@ def test_wide_design_fit_api_matches_oracle(dev):
 
     with pytest.raises(NotImplementedError):
         fit_glm_arma11(Y[:8], X, 2.0, a_grid=ag, b_grid=bg, device=dev, selector="learned")
-    with pytest.raises(NotImplementedError):
-        fit_glm_arma11(Y[:8], X, 2.0, a_grid=ag, b_grid=bg, device=dev, use_double=True)
+
+
+def test_wide_design_use_double_is_the_float64_search(dev):
+    """use_double on a 128-column design: the float64 search on the raw
+    series (the FP64 surface over 32-column chunks of each pair's table)
+    against the FP64 oracle, with the bars of test_gpu_precision: scores within
+    1e-6 + 1e-10 |score|, hierarchical pair identical except near-ties <= 1e-6,
+    betas / t within 1e-5."""
+    from fastfuncstuff_amd.glm.arma import fit_glm_arma11
+
+    print("[wide] use_double p=128", flush=True)
+    X, Y = _case(seed=13, V=64)
+    V = Y.shape[0]
+    ag = bg = torch.linspace(-0.8, 0.8, 9)
+    res = fit_glm_arma11(Y, X, 2.0, a_grid=ag, b_grid=bg, device=dev, use_double=True, save_profile_likelihoods=True)
+    ref = O.fit_arma11(torch.from_numpy(Y).double(), torch.from_numpy(X), ag, bg, dtype=F64, selector="hierarchical")
+    err = (res.reml_lklhd_surface - ref.surface).abs()
+    assert bool((err <= 1e-6 + 1e-10 * ref.surface.abs()).all()), f"max score error {float(err.max()):.3e}"
+    diff = res.pair_index != ref.pair_index
+    gap = (ref.surface[torch.arange(V), res.pair_index] - ref.surface[torch.arange(V), ref.pair_index]).abs()
+    assert bool((gap[diff] <= 1e-6).all())
+    ok = ~diff
+    sc = torch.from_numpy(Y).double().std(dim=1)
+    db = (res.betas - ref.betas).abs() / torch.maximum(ref.betas.abs(), 1e-3 * sc[:, None])
+    assert float(db[ok].max()) <= 1e-5
+    assert float(((res.tstats - ref.tstats).abs() / ref.tstats.abs().clamp(min=1))[ok].max()) <= 1e-5
