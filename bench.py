@@ -248,9 +248,11 @@ def im_design(args):
     --voxels says otherwise, 33 x 33 grid.  Designs wider than 96 columns run
     the segmented path: FP64 prep, recurrence energy, fp32 MFMA projection over
     64-column segments, hierarchical selection on the surface, FP64 GLS with one
-    wave per voxel.  One step = one fit with the data resident in HBM (stages
-    timed with device events on the launch stream); the plan is built first and
-    timed apart, and the drop-in fit_glm_arma11 from host data is timed beside it."""
+    wave per voxel group.  One step = one fit with the data resident in HBM
+    (stages timed with device events on the launch stream); the plan is built
+    first and timed apart.  Timed beside it: the hierarchical descent on demand
+    (the same pairs from about 45 pairs per voxel) + GLS, and the drop-in
+    fit_glm_arma11 from host data."""
     from fastfuncstuff_amd import _lib, synth
     from fastfuncstuff_amd.glm.arma import RemlGrid, fit_glm_arma11
 
@@ -294,7 +296,19 @@ def im_design(args):
         grid.gls(Yd, scale, best)
         return best
 
-    ms, per, _, _ = timed(step, args.steps, args.warmup, dev)
+    ms, per, _, best_surface = timed(step, args.steps, args.warmup, dev)
+
+    def step_descent(tm):
+        if tm:
+            tm.mark("descent")
+        best, _, scale = grid.hier_search(Yd)
+        if tm:
+            tm.mark("gls")
+        grid.gls(Yd, scale, best)
+        return best
+
+    ms_d, per_d, _, best_d = timed(step_descent, args.steps, args.warmup, dev)
+    same_d = float((best_d == best_surface).double().mean())
     t0 = time.perf_counter()
     fit_glm_arma11(Y, X, 2.0, a_grid=agt, b_grid=bgt, device=dev, run_starts=run_starts, grid=grid,
                    projection=args.projection)
@@ -321,7 +335,14 @@ def im_design(args):
                      "frac": flop / (proj_ms / 1e3) / 1e12 / peak,
                      "algorithmic": f"{'3*' if split else ''}2*V*T*K*p = {flop:.3e} flop per launch",
                      "traffic": None},
-        "plan_build_s": plan_s, "api_end_to_end": {"ms": api_ms, "voxels_per_s": V / (api_ms / 1e3)},
+        "hierarchical_descent": {"value": V / (ms_d / 1e3), "ms_per_step": ms_d, "kernels_ms": per_d,
+                                 "same_pair_as_full_grid_hierarchical": same_d,
+                                 "note": "the reference's default selector on the lattice points its descent "
+                                         "visits (pair sets over the segmented operand; scores bitwise equal to "
+                                         "the surface) + GLS; the default of fit_glm_arma11"},
+        "plan_build_s": plan_s,
+        "api_end_to_end": {"ms": api_ms, "voxels_per_s": V / (api_ms / 1e3),
+                           "note": "fit_glm_arma11 from host data (default selector: the descent on demand)"},
         "datagen_s": gen_s,
     }
     grid.close()

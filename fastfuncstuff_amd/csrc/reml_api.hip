@@ -1475,16 +1475,20 @@ int set_scores(const ffs_reml_plan* P, const SetTable& ST, const SetLayout& L, c
     const dim3 g((unsigned)std::min<int64_t>(1024, std::max<int64_t>(8, (items + 7) / 8 * 8))), b(64 * kSetWaves);
     const int64_t ld = L.max_cols;
     const int mode = P->d.proj_mode;
-#define FFS_SET_PROJ(BN_, MODE_) \
-    hipLaunchKernelGGL((k_set_project<BN_, MODE_>), g, b, 0, s, P->d, ST, q.Yt, ld, q.tset, q.nt, q.S)
-    if (P->d.BN == 64) {
-        if (mode == 1) FFS_SET_PROJ(64, 1);
-        else if (mode == 2) FFS_SET_PROJ(64, 2);
-        else FFS_SET_PROJ(64, 0);
+#define FFS_SET_PROJ(BN_, MODE_, SEG_) \
+    hipLaunchKernelGGL((k_set_project<BN_, MODE_, SEG_>), g, b, 0, s, P->d, ST, q.Yt, ld, q.tset, q.nt, q.S)
+    if (P->d.nseg) {  // p > 96: pairs over 64-column segments
+        if (mode == 1) FFS_SET_PROJ(64, 1, true);
+        else if (mode == 2) FFS_SET_PROJ(64, 2, true);
+        else FFS_SET_PROJ(64, 0, true);
+    } else if (P->d.BN == 64) {
+        if (mode == 1) FFS_SET_PROJ(64, 1, false);
+        else if (mode == 2) FFS_SET_PROJ(64, 2, false);
+        else FFS_SET_PROJ(64, 0, false);
     } else {
-        if (mode == 1) FFS_SET_PROJ(96, 1);
-        else if (mode == 2) FFS_SET_PROJ(96, 2);
-        else FFS_SET_PROJ(96, 0);
+        if (mode == 1) FFS_SET_PROJ(96, 1, false);
+        else if (mode == 2) FFS_SET_PROJ(96, 2, false);
+        else FFS_SET_PROJ(96, 0, false);
     }
 #undef FFS_SET_PROJ
     HIP_TRY(hipGetLastError());
@@ -1588,7 +1592,6 @@ extern "C" int ffs_reml_scatter_best(const int32_t* idx, int64_t n, const int32_
 // the set's slots.  Visit orders are those of k_select mode 1.
 extern "C" int ffs_reml_plan_build_hier(ffs_reml_plan* P, void* stream) {
     if (!P) return fail(-3, "plan_build_hier: null plan");
-    FFS_NEED_NARROW(P, "plan_build_hier");
     FFS_NEED_SEARCH(P, "plan_build_hier");
     if (!P->hier.empty()) return 0;
     hipStream_t s = (hipStream_t)stream;
@@ -1720,7 +1723,6 @@ HierLayout hier_layout(const ffs_reml_plan* P, int64_t V) {
 
 extern "C" int ffs_reml_hier_workspace(const ffs_reml_plan* P, int64_t V, int64_t* bytes) {
     if (!P || !bytes) return fail(-3, "hier_workspace: null argument");
-    FFS_NEED_NARROW(P, "hier_workspace");
     if (P->hier.empty()) return fail(-4, "hier_workspace: call ffs_reml_plan_build_hier first");
     *bytes = (int64_t)hier_layout(P, std::max<int64_t>(V, 0)).total;
     return 0;

@@ -262,7 +262,9 @@ __global__ void __launch_bounds__(64 * kSetEnergyWaves) k_set_energy(PlanDev P, 
 // residual rows and the fp32 operand (k_project's k-order); MODE 1 / 2: split
 // bf16 / fp16 halves on 32x32x16 (k_project16b's order: per 16-sample k-step
 // al*bh, ah*bl, ah*bh).  Scores go over the energies in Sb[col][slot].
-template <int BN, int MODE>
+// SEG: p > 96, each pair's columns in P.nseg segments of BN (one pair per set
+// tile); a separate instantiation, so the narrow ones keep their schedule.
+template <int BN, int MODE, bool SEG>
 __global__ void __launch_bounds__(64 * kSetWaves) k_set_project(PlanDev P, SetTable ST, const float* __restrict__ Yt,
                                                                 int64_t ldY, const int32_t* __restrict__ tset,
                                                                 const int32_t* __restrict__ n_tiles,
@@ -284,12 +286,25 @@ __global__ void __launch_bounds__(64 * kSetWaves) k_set_project(PlanDev P, SetTa
         const int stile = quad * kSetWaves + w;
         const int32_t* __restrict__ kp = ST.pairs + (size_t)set * SB + stile * npt;
         const int col0 = tt * kSetTile;
+        // p > 96 (P.nseg segments of BN columns, one pair per set tile): the
+        // products of each segment in turn, each row's squares summed across
+        // them (lanes 0..31, one sum per row block), the score after the last
+        const int nsg = SEG ? P.nseg : 1;
+        double spp[2] = {0.0, 0.0};
+        for (int sg = 0; sg < nsg; ++sg) {
         // this lane's B column in n-block n: source tile and column of its pair
         int bst[NBLK], bcc[NBLK];
         bool bok[NBLK];
 #pragma unroll
         for (int n = 0; n < NBLK; ++n) {
             const int c = n * 32 + (lane & 31);
+            if (SEG) {
+                const int k = kp[0];
+                bok[n] = k >= 0 && sg * BN + c < p;
+                bst[n] = k >= 0 ? k * P.nseg + sg : 0;
+                bcc[n] = c;
+                continue;
+            }
             const int q = c / p, j = c - q * p;
             const int k = q < npt ? kp[q] : -1;
             bok[n] = k >= 0;
@@ -435,6 +450,17 @@ __global__ void __launch_bounds__(64 * kSetWaves) k_set_project(PlanDev P, SetTa
                     Cw[row * C_LD + n * 32 + (lane & 31)] = acc[r][n][q];
                 }
             __syncthreads();
+            if (SEG) {
+                if (lane < 32 && kp[0] >= 0) {
+                    const float* cr = Cw + lane * C_LD;
+                    const int nj = min(BN, p - sg * BN);
+                    for (int j = 0; j < nj; ++j) {
+                        const double cv = (double)cr[j];
+                        spp[r] = fma(cv, cv, spp[r]);
+                    }
+                }
+                continue;
+            }
             for (int i = lane; i < 32 * npt; i += 64) {
                 const int row = i / npt, qq = i % npt;
                 const int k = kp[qq];
@@ -452,6 +478,17 @@ __global__ void __launch_bounds__(64 * kSetWaves) k_set_project(PlanDev P, SetTa
             }
         }
         __syncthreads();
+        }  // segments
+        if (SEG && lane < 32 && kp[0] >= 0) {
+            const int k = kp[0];
+#pragma unroll
+            for (int r = 0; r < 2; ++r) {
+                const double pp = MODE == 2 ? spp[r] * (double)P.g16_unscale[k] : spp[r];
+                float* sp = Sb + (size_t)(col0 + 32 * r + lane) * SB + stile;
+                const double rss = (double)(*sp) - pp;
+                *sp = (float)fma(tp, log(rss + 1e-10), P.ld[k]);
+            }
+        }
     }
 }
 

@@ -35,8 +35,8 @@ VOXEL_SCATTER_ATTRS = [
     "ljung_box", "set_fstats", "set_sd", "set_coef",
 ]
 
-# Widest design of the pair-set searches (hierarchical descent on demand,
-# learned windows and bands) and the grouped engine; wider
+# Widest design of the learned searches (windows and lambda bands), the subset
+# search and the grouped engine; wider
 # designs (up to 768 columns) use the full-grid surface, with the projection
 # over 64-column segments of each pair.
 WIDE_P = 96
@@ -1220,7 +1220,7 @@ def fit_glm_arma11(data, design, tr, a_grid=None, b_grid=None, estimate_per_voxe
     # on the full surface, when no surface is kept
     use_descent = (fixed_best is None and searcher is None and not use_double and surface_acc is None
                    and not force_exhaustive_search and selector in (None, "hierarchical")
-                   and hierarchical_search == "descent" and p <= WIDE_P)
+                   and hierarchical_search == "descent")
     with torch.cuda.device(dev):
         for b0 in range(0, V, nb):
             b1 = min(V, b0 + nb)

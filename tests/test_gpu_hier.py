@@ -10,7 +10,8 @@ prep's.  Cases: each projection mode, the censored config-2 timeline, lattices
 whose zero point is not on the top level's points, a non-square lattice, a
 lattice small enough for a single pass (top level and the level below it), and
 designs wider than 32 columns (p = 40 in fp32 and fp16x3, p = 68 in bf16x3: no
-stored residualisation coefficients, every pass residualises from Y).
+stored residualisation coefficients, every pass residualises from Y) and a
+130-column design (pairs over 64-column segments) in fp32 and fp16x3.
 """
 
 import numpy as np
@@ -42,9 +43,10 @@ def _grid_case(name):
     elif name == "small":  # 5 x 6 lattice: top level 1, one pass
         ag = np.linspace(-0.6, 0.6, 5).astype(np.float32)
         bg = np.linspace(-0.5, 0.5, 6).astype(np.float32)
-    elif name in ("wide", "wide68"):  # p = 40 / 68 > 32
+    elif name in ("wide", "wide68", "seg130"):  # p = 40 / 68 > 32; 130 > 96 (segmented pairs)
         rng = np.random.default_rng(0)
-        X = np.concatenate([wl.X, rng.standard_normal((wl.X.shape[0], 34 if name == "wide" else 62))], 1)
+        extra = {"wide": 34, "wide68": 62, "seg130": 124}[name]
+        X = np.concatenate([wl.X, rng.standard_normal((wl.X.shape[0], extra))], 1)
         ag, bg = wl.a_grid, wl.b_grid
     else:
         ag, bg = wl.a_grid, wl.b_grid
@@ -53,7 +55,8 @@ def _grid_case(name):
 
 @pytest.mark.parametrize("name,mode", [("plain", "fp16x3"), ("plain", "bf16x3"), ("plain", "fp32"),
                                        ("config2", "fp16x3"), ("offcentre", "fp16x3"), ("small", "fp16x3"),
-                                       ("wide", "fp32"), ("wide", "fp16x3"), ("wide68", "bf16x3")])
+                                       ("wide", "fp32"), ("wide", "fp16x3"), ("wide68", "bf16x3"),
+                                       ("seg130", "fp32"), ("seg130", "fp16x3")])
 def test_hier_search_equals_surface_descent(dev, name, mode):
     from fastfuncstuff_amd.glm.arma import RemlGrid
 
