@@ -322,24 +322,24 @@ def im_design(args):
     proj_ms = per.get("project", float("nan"))
     line = {
         "metric": f"REML voxels/s, IM design (p={p}, T={T}, V={V})",
-        "value": V / (ms / 1e3), "unit": "voxels/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "value": V / (ms_d / 1e3), "unit": "voxels/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms_d, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": args.projection, "data": "synthetic",
         "config": {"workload": f"im596_T{T}_5runs_p{p}_grid33x33", "V": V, "T": T, "p": p, "K": grid.K,
                    "design_rank": grid.rank,
-                   "path": "segmented (p > 96): full-grid surface, projection over 64-column segments, "
-                           "hierarchical selection, FP64 GLS", "projection": args.projection},
-        "kernels_ms": per,
-        "roofline": {"kernel": "k_project16_seg" if split else "k_project_seg", "bound": "mfma",
-                     "achieved": flop / (proj_ms / 1e3) / 1e12, "peak": peak, "unit": "TFLOP/s",
-                     "frac": flop / (proj_ms / 1e3) / 1e12 / peak,
-                     "algorithmic": f"{'3*' if split else ''}2*V*T*K*p = {flop:.3e} flop per launch",
-                     "traffic": None},
-        "hierarchical_descent": {"value": V / (ms_d / 1e3), "ms_per_step": ms_d, "kernels_ms": per_d,
-                                 "same_pair_as_full_grid_hierarchical": same_d,
-                                 "note": "the reference's default selector on the lattice points its descent "
-                                         "visits (pair sets over the segmented operand; scores bitwise equal to "
-                                         "the surface) + GLS; the default of fit_glm_arma11"},
+                   "path": "segmented (p > 96): hierarchical descent on demand (the default of fit_glm_arma11: "
+                           "pair sets over the segmented operand, scores bitwise equal to the surface) + FP64 GLS",
+                   "projection": args.projection},
+        "kernels_ms": per_d,
+        "same_pair_as_full_grid_hierarchical": same_d,
+        "full_grid": {
+            "value": V / (ms / 1e3), "ms_per_step": ms, "kernels_ms": per,
+            "path": "full-grid surface, projection over 64-column segments, hierarchical selection, FP64 GLS",
+            "roofline": {"kernel": "k_project16_seg" if split else "k_project_seg", "bound": "mfma",
+                         "achieved": flop / (proj_ms / 1e3) / 1e12, "peak": peak, "unit": "TFLOP/s",
+                         "frac": flop / (proj_ms / 1e3) / 1e12 / peak,
+                         "algorithmic": f"{'3*' if split else ''}2*V*T*K*p = {flop:.3e} flop per launch",
+                         "traffic": None}},
         "plan_build_s": plan_s,
         "api_end_to_end": {"ms": api_ms, "voxels_per_s": V / (api_ms / 1e3),
                            "note": "fit_glm_arma11 from host data (default selector: the descent on demand)"},

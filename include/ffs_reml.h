@@ -47,9 +47,9 @@ const char* ffs_reml_last_error(void);
  *               projection columns: they support prep, energy, project, surface,
  *               select and the GLS (the split 16-bit projection modes read the
  *               halves that prep wrote in the same mode, so ffs_reml_surface runs
- *               them with n_chunks = 1); the pair-set searches (windows, bands,
- *               hierarchical descent on demand, subset search), prep_features
- *               and the grouped engine return -4 on them
+ *               them with n_chunks = 1), the hierarchical descent on demand and
+ *               surface_f64; the windows, bands, subset search, prep_features and
+ *               the grouped engine return -4 on them
  *   X           host, T x p row-major float64 design
  *   pinvT       host, T x p row-major float64 transpose of the design pseudo-inverse
  *   gap         host, T int32: tau[t+1]-tau[t] inside a run, 0 at the last point of a run
