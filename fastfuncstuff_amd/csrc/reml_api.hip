@@ -1451,6 +1451,20 @@ int set_group_prep(const ffs_reml_plan* P, const SetTable& ST, const float* Y, i
     return prep_impl(P, Y, ldY, V, q.rows, L.max_cols, q.Yt, L.max_cols, scale, 1, nullptr, s, q.fill + L.n, split,
                      false, coef, coef_out);
 }
+// the same layout gathered from a residual buffer prepared for all V voxels
+// (ldres columns, with the split halves in the split modes)
+int set_group_gather(const ffs_reml_plan* P, const SetTable& ST, const float* Yres, int64_t ldres, int64_t V,
+                     const SetLayout& L, const SetPtrs& q, hipStream_t s) {
+    if (int rc = group_by_key(q.key, V, (int)L.n, kSetTile, q.cnt, q.fill, q.cursor, q.rows, L.max_cols, s)) return rc;
+    hipLaunchKernelGGL(k_group_tiles, dim3((unsigned)((L.max_tiles + 255) / 256)), dim3(256), 0, s, q.fill, (int)L.n,
+                       (int)L.max_tiles, kSetTile, q.tset, q.tcol, q.nt);
+    const int64_t n = L.max_cols * (P->d.Tpad / 8);
+    if (n > 0)
+        hipLaunchKernelGGL(k_gather_cols, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, Yres, ldres, q.Yt,
+                           L.max_cols, q.rows, (int)L.max_cols, P->d.Tpad, (int)split_halves(P));
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
 int set_energy(const ffs_reml_plan* P, const SetTable& ST, const SetLayout& L, const SetPtrs& q, hipStream_t s) {
     // one block per (tile octet, chunk) of the largest tile count, 8 * ceil(items / 8)
     const int64_t items = (L.max_tiles + kSetEnergyWaves - 1) / kSetEnergyWaves * ST.mc;
@@ -1697,7 +1711,7 @@ extern "C" int ffs_reml_plan_build_hier(ffs_reml_plan* P, void* stream) {
 
 namespace {
 struct HierLayout {
-    size_t off_ba, off_bb, off_cur, off_coef, off_sets, total;
+    size_t off_ba, off_bb, off_cur, off_coef, off_res, off_sets, total;
     int stride;
 };
 HierLayout hier_layout(const ffs_reml_plan* P, int64_t V) {
@@ -1713,6 +1727,8 @@ HierLayout hier_layout(const ffs_reml_plan* P, int64_t V) {
     H.off_bb = take(4 * (size_t)V);
     H.off_cur = take(4 * (size_t)V);
     H.off_coef = take(8 * (size_t)V * H.stride);
+    // p > 96: the residual of every voxel, prepared once (passes gather from it)
+    H.off_res = take(P->d.nseg ? 4 * (size_t)yt_rows(P) * vpad_of(V) : 0);
     H.off_sets = o;
     size_t mx = 0;
     for (const HierPass& h : P->hier) mx = std::max(mx, set_layout(P, h.ST, V).total);
@@ -1746,6 +1762,14 @@ extern "C" int ffs_reml_hier_search(const ffs_reml_plan* P, const float* Y, int6
     double* coef = H.stride ? reinterpret_cast<double*>(ws + H.off_coef) : nullptr;
     hipStream_t s = (hipStream_t)stream;
     const int np = (int)P->hier.size();
+    // p > 96: one prep of every voxel (with the split halves in the split
+    // modes); each pass gathers its set layout's columns from it
+    float* Yres = P->d.nseg ? reinterpret_cast<float*>(ws + H.off_res) : nullptr;
+    const int64_t ldres = vpad_of(V);
+    if (Yres)
+        if (int rc = prep_impl(P, Y, ldY, V, nullptr, V, Yres, ldres, scale, 1, nullptr, s, nullptr, split_halves(P),
+                               false))
+            return rc;
     for (int i = 0; i < np; ++i) {
         const HierPass& h = P->hier[i];
         const SetLayout L = set_layout(P, h.ST, V);
@@ -1757,11 +1781,14 @@ extern "C" int ffs_reml_hier_search(const ffs_reml_plan* P, const float* Y, int6
             hipLaunchKernelGGL(k_hier_key, dim3((unsigned)((V + 255) / 256)), dim3(256), 0, s, P->d, h.center_set,
                                hba, hbb, (int)V, q.key);
         HIP_TRY(hipGetLastError());
-        // pass 0 residualises from Y and keeps each voxel's coefficients and
-        // scale; the later passes run the second half of the prep from them
-        if (int rc = set_group_prep(P, h.ST, Y, ldY, V, scale, i == 0 ? nullptr : coef, L, q, s,
-                                    i == 0 ? coef : nullptr))
+        if (Yres) {
+            if (int rc = set_group_gather(P, h.ST, Yres, ldres, V, L, q, s)) return rc;
+        } else if (int rc = set_group_prep(P, h.ST, Y, ldY, V, scale, i == 0 ? nullptr : coef, L, q, s,
+                                           i == 0 ? coef : nullptr)) {
+            // pass 0 residualises from Y and keeps each voxel's coefficients and
+            // scale; the later passes run the second half of the prep from them
             return rc;
+        }
         if (int rc = set_energy(P, h.ST, L, q, s)) return rc;
         if (int rc = set_scores(P, h.ST, L, q, s)) return rc;
         hipLaunchKernelGGL(k_hier_select, dim3((unsigned)(L.max_cols / 256)), dim3(256), 0, s, P->d, h, q.rows,

@@ -384,6 +384,37 @@ __global__ void __launch_bounds__(256) k_project16_seg(PlanDev P, const float* _
     }
 }
 
+// Columns of a prepared time-major residual buffer gathered into another
+// column order (dst column c = src column rows[c], zero where rows[c] < 0):
+// the Tpad fp32 rows, and with halves the split hi / lo planes that follow
+// them ([t/8][col][8] 16-bit units, 8 samples of a column per 16 bytes).  The
+// residual of a voxel does not depend on its column, so the gathered buffer
+// equals a prep of the voxels in the new order.  Threads: consecutive dst
+// columns, then 8-sample blocks.
+__global__ void k_gather_cols(const float* __restrict__ src, int64_t lds, float* __restrict__ dst, int64_t ldd,
+                              const int32_t* __restrict__ rows, int ncols, int Tpad, int halves) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = (int)(i % ncols);
+    const int tb = (int)(i / ncols);
+    if (tb >= Tpad / 8) return;
+    const int v = rows[c];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+        const int t = 8 * tb + e;
+        dst[(size_t)t * ldd + c] = v >= 0 ? src[(size_t)t * lds + v] : 0.f;
+    }
+    if (!halves) return;
+    const uint16_t* sh = reinterpret_cast<const uint16_t*>(src + (size_t)Tpad * lds);
+    uint16_t* dh = reinterpret_cast<uint16_t*>(dst + (size_t)Tpad * ldd);
+#pragma unroll
+    for (int plane = 0; plane < 2; ++plane) {
+        const uint16_t* sp = sh + (size_t)plane * Tpad * lds;
+        uint16_t* dp = dh + (size_t)plane * Tpad * ldd;
+        const u32x4 x = v >= 0 ? *reinterpret_cast<const u32x4*>(sp + ((size_t)tb * lds + v) * 8) : u32x4{0u, 0u, 0u, 0u};
+        *reinterpret_cast<u32x4*>(dp + ((size_t)tb * ldd + c) * 8) = x;
+    }
+}
+
 // Sum over the 64 lanes of a wave (every lane gets the total).
 __device__ __forceinline__ double wave_sum(double x) {
 #pragma unroll
