@@ -22,7 +22,8 @@ F64 = torch.float64
 
 def _case(name):
     rng = np.random.default_rng({"plain": 1, "censored": 2, "afni": 3, "ragged": 4, "wide": 5, "wide60": 6,
-                                 "wide68": 7, "wide90": 8, "seg120": 9, "seg300": 10, "seg520": 11}[name])
+                                 "wide68": 7, "wide90": 8, "seg120": 9, "seg300": 10, "seg520": 11,
+                                 "segcens": 12}[name])
     if name in ("plain", "ragged", "wide", "wide60", "wide68", "wide90", "seg120", "seg300", "seg520"):
         # design widths 40 / 60 / 68 / 90 cover the f64-MFMA prep's 3 to 6 column
         # tiles; 120 / 300 / 520 the segmented path (2 / 5 / 9 segments of 64
@@ -34,8 +35,9 @@ def _case(name):
         X = np.column_stack([np.ones(T), np.linspace(-1, 1, T), rng.standard_normal((T, p - 2))])
         run_starts, tau = None, None
         ag = bg = torch.linspace(-0.8, 0.8, {"seg300": 17, "seg520": 9}.get(name, 33))
-    elif name == "censored":
-        lens, p, V = [70, 80], 12, 200
+    elif name in ("censored", "segcens"):
+        # segcens: two runs with censored volumes at p = 110 (segmented path)
+        lens, p, V = ([70, 80], 12, 200) if name == "censored" else ([140, 150], 110, 120)
         full = sum(lens)
         good = np.sort(rng.choice(full, full - 8, replace=False))
         run_starts, tau = O.build_censor_run_info([0, lens[0]], full, good.tolist())
@@ -74,7 +76,7 @@ def dev():
 
 
 @pytest.mark.parametrize("name", ["plain", "censored", "afni", "ragged", "wide", "wide60", "wide68", "wide90",
-                                  "seg120", "seg300", "seg520"])
+                                  "seg120", "seg300", "seg520", "segcens"])
 def test_surface_selection_and_gls_match_oracle(dev, name):
     from fastfuncstuff_amd.glm.arma import RemlGrid
 
@@ -172,7 +174,7 @@ SPLIT_TOL = {"fp16x3": dict(atol=5e-4, rtol=2e-6, tie=2e-3), "bf16x3": dict(atol
 
 
 @pytest.mark.parametrize("mode", ["fp16x3", "bf16x3"])
-@pytest.mark.parametrize("name", ["plain", "censored", "afni", "wide", "seg120", "seg300"])
+@pytest.mark.parametrize("name", ["plain", "censored", "afni", "wide", "seg120", "seg300", "segcens"])
 def test_split_projection_matches_oracle(dev, name, mode):
     from fastfuncstuff_amd.glm.arma import RemlGrid
 
